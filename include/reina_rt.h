@@ -1,0 +1,269 @@
+/*
+ * reina_rt.h — C ABI of the MI355X path-tracing library (libreina_rt.so).
+ *
+ * The library replaces Reina's Vulkan ray-tracing pipeline: the driver-built
+ * BLAS/TLAS (src/graphics/Blas.cpp:8-124, src/tools/vktools.cpp:460-596), one
+ * vkCmdTraceRaysKHR over the raygen / closest-hit / miss shaders
+ * (src/Reina.cpp:425-470, shaders/raytrace/ *.glsl) and, as the first
+ * downstream consumer, the bloom + tone-map compute passes
+ * (src/Reina.cpp:472-577).
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = REINA_RT_OK, negative = error.
+ *     reina_rt_last_error() returns a thread-local message for the last error.
+ *     (The reference throws std::runtime_error everywhere, src/main.cpp:11-14.)
+ *   - Host pointers in descriptors are copied during the call; the caller may
+ *     free them afterwards.
+ *   - Image and ray/hit pointers passed to trace/intersect/postprocess are
+ *     DEVICE pointers (hipMalloc'd or torch tensors' data_ptr()).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). All
+ *     device work is enqueued on it; functions documented as "synchronous"
+ *     block until that work completes.
+ *   - A scene handle is not thread-safe; distinct handles may be used from
+ *     distinct threads.
+ */
+#ifndef REINA_RT_H
+#define REINA_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REINA_RT_ABI_VERSION 1u
+
+/* ---- status codes ------------------------------------------------------- */
+#define REINA_RT_OK                0
+#define REINA_RT_ERR_INVALID_ARG  -1  /* null pointer, zero size, bad index   */
+#define REINA_RT_ERR_ABI          -2  /* struct_size / ABI version mismatch   */
+#define REINA_RT_ERR_OUT_OF_MEMORY -3 /* device allocation failed             */
+#define REINA_RT_ERR_OVER_BUDGET  -4  /* request exceeds the HBM budget       */
+#define REINA_RT_ERR_DEVICE       -5  /* HIP runtime / kernel launch error    */
+#define REINA_RT_ERR_UNSUPPORTED  -6  /* feature not built (e.g. textures)    */
+#define REINA_RT_ERR_NO_EMISSIVE  -7  /* NEE requested on a scene w/o lights  */
+#define REINA_RT_ERR_BUSY         -8  /* server queue full (admission reject) */
+
+/* ---- byte-identical copies of polyglot/raytrace.h ----------------------- */
+
+/* polyglot/raytrace.h:18-41 (120 bytes, GLSL scalar layout).
+ * interpNormals is a C++ bool followed by 3 padding bytes in the reference;
+ * the device reads only that byte (padding is ignored). */
+typedef struct ReinaInstanceProperties {
+    uint32_t indicesOffset;        /*   0 */
+    float    albedo[3];            /*   4 */
+    float    emission[3];          /*  16 */
+    uint32_t tbnsIndicesOffset;    /*  28 */
+    uint32_t texIndicesOffset;     /*  32 (0xFFFFFFFF: model has no UVs) */
+    float    roughness;            /*  36 */
+    float    ior;                  /*  40 */
+    uint8_t  interpNormals;        /*  44 (C++ bool) */
+    uint8_t  _pad0[3];             /*  45 */
+    float    absorption;           /*  48 */
+    int32_t  textureID;            /*  52 (-1: none) */
+    int32_t  normalMapTexID;       /*  56 */
+    int32_t  bumpMapTexID;         /*  60 */
+    uint32_t cullBackface;         /*  64 */
+    float    anisotropic;          /*  68 */
+    float    subsurface;           /*  72 */
+    float    clearcoatGloss;       /*  76 */
+    float    sheenTint[3];         /*  80 */
+    float    specularTint[3];      /*  92 */
+    float    metallic;             /* 104 */
+    float    clearcoat;            /* 108 */
+    float    specularTransmission; /* 112 */
+    float    sheen;                /* 116 */
+} ReinaInstanceProperties;
+
+/* polyglot/raytrace.h:43-54 (160 bytes). Matrices are glm::mat4, column-major. */
+typedef struct RtPushConsts {
+    float    invView[16];          /*   0 */
+    float    invProjection[16];    /*  64 */
+    uint32_t sampleBatch;          /* 128 */
+    float    totalEmissiveWeight;  /* 132 */
+    float    focusDist;            /* 136 */
+    float    defocusMultiplier;    /* 140 (already divided by 100, Reina.cpp:150) */
+    float    directClamp;          /* 144 */
+    float    indirectClamp;        /* 148 (unused by the active path) */
+    uint32_t samplesPerPixel;      /* 152 */
+    uint32_t maxBounces;           /* 156 */
+} RtPushConsts;
+
+/* src/scene/Instances.h:15-27 (alignas(16), 112 bytes): emissive metadata for
+ * next-event estimation. Produced on the device from the scene description;
+ * exposed for inspection through reina_rt_scene_emissive(). */
+typedef struct ReinaInstanceData {
+    float    transform[16];        /*   0 */
+    uint32_t materialOffset;       /*  64 */
+    uint32_t cdfRangeStart;        /*  68 */
+    uint32_t cdfRangeEnd;          /*  72 */
+    uint32_t indexOffset;          /*  76 */
+    float    emission[3];          /*  80 */
+    float    weight;               /*  92 */
+    float    area;                 /*  96 */
+    uint8_t  cullBackface;         /* 100 (C++ bool) */
+    uint8_t  _pad0[3];             /* 101 */
+    float    padding[2];           /* 104 */
+} ReinaInstanceData;
+
+/* polyglot/bloom.h:4-8 and polyglot/tonemapping.h:14-16 */
+typedef struct BloomPushConsts { float radius, threshold, intensity; } BloomPushConsts;
+typedef struct TonemappingPushConsts { float exposure; } TonemappingPushConsts;
+
+/* ---- scene description (replaces Scene::build, src/scene/Scene.cpp:56-125) */
+
+/* src/scene/Models.h:12-21 */
+typedef struct ReinaModelRange {
+    uint32_t firstVertex;
+    uint32_t firstNormal;
+    uint32_t indexOffset;
+    uint32_t tbnsIndexOffset;
+    uint32_t texIndexOffset;
+    uint32_t indexCount;      /* triangles */
+    uint32_t tbnsIndexCount;  /* triangles */
+    uint32_t texIndexCount;   /* triangles */
+} ReinaModelRange;
+
+/* One TLAS instance (src/tools/vktools.cpp:460-488, Scene.cpp:99-111):
+ * object-to-world transform (glm::mat4, column-major; the bottom row is
+ * ignored exactly as VkTransformMatrixKHR drops it), the custom index
+ * (gl_InstanceCustomIndexEXT) and the hit group (SBT record offset). */
+typedef struct ReinaInstanceDesc {
+    float    transform[16];
+    uint32_t instancePropertiesID;
+    uint32_t materialIdx;      /* 0 lambertian, 1 metal, 2 dielectric, 3 disney (Reina.cpp:159-167) */
+    uint32_t modelID;
+    uint32_t _pad0;
+} ReinaInstanceDesc;
+
+/* BVH / build options */
+#define REINA_RT_BUILD_HOST        0x0u   /* binned-SAH build on the host CPU      */
+#define REINA_RT_BUILD_DEVICE      0x1u   /* LBVH+SAH-collapse build on the GPU    */
+#define REINA_RT_NODES_QUANTIZED   0x2u   /* 8-bit child boxes (80-byte nodes)      */
+
+/* The packed buffers exactly as Models::addModel produces them
+ * (src/scene/Models.cpp:24-92) plus the instance and properties lists. */
+typedef struct ReinaSceneDesc {
+    uint32_t struct_size;                    /* sizeof(ReinaSceneDesc)            */
+    uint32_t flags;                          /* REINA_RT_BUILD_* | REINA_RT_NODES_* */
+    const float*    vertices;     uint64_t num_vertices;     /* vec4 each (binding 2)        */
+    const uint32_t* indices;      uint64_t num_indices;      /* offset indices (binding 3)   */
+    const float*    tbns;         uint64_t num_tbns;         /* mat3 col-major (binding 5)   */
+    const uint32_t* tbn_indices;  uint64_t num_tbn_indices;  /* offset (binding 6)           */
+    const float*    tex_coords;   uint64_t num_tex_coords;   /* vec2 each (binding 10)       */
+    const uint32_t* tex_indices;  uint64_t num_tex_indices;  /* offset (binding 11)          */
+    const ReinaInstanceProperties* props;     uint32_t num_props;     /* binding 4 */
+    const ReinaModelRange*         models;    uint32_t num_models;
+    const ReinaInstanceDesc*       instances; uint32_t num_instances;
+    uint32_t device;                         /* HIP device ordinal               */
+} ReinaSceneDesc;
+
+typedef struct ReinaScene ReinaScene;
+
+typedef struct ReinaSceneInfo {
+    uint64_t num_triangles;   /* flattened world-space triangles          */
+    uint64_t num_nodes;       /* BVH8 nodes                               */
+    uint64_t node_bytes;      /* bytes per node as stored (256 or 80)     */
+    uint64_t device_bytes;    /* total HBM held by the scene              */
+    uint32_t max_depth;       /* BVH8 depth (root = 1)                    */
+    uint32_t quantized;       /* 1 if 8-bit child boxes                   */
+    float    build_ms;        /* BVH build wall time                      */
+    float    sah_cost;        /* SAH cost of the BVH8 (ci=1, ct=1)        */
+} ReinaSceneInfo;
+
+/* ---- rays and hits for the closest-hit query ---------------------------- */
+
+/* One ray: traceRayEXT(tlas, gl_RayFlagsOpaqueEXT, 0xFF, 0, 0, 0, o, tmin, d, tmax)
+ * (shaders/raytrace/raytrace.rgen.glsl:110-122). 32 bytes. */
+typedef struct ReinaRay {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} ReinaRay;
+
+/* Closest hit: t, flattened triangle id (instance-major order: all triangles
+ * of instance 0, then instance 1, ...), barycentrics (u, v) of vertices 1 and
+ * 2 as in gl hitAttributeEXT. prim = 0xFFFFFFFF and t = tmax on a miss. */
+typedef struct ReinaHit {
+    float    t;
+    uint32_t prim;
+    float    u, v;
+} ReinaHit;
+
+/* ---- API ---------------------------------------------------------------- */
+
+/* ABI version and the struct sizes the library was built with
+ * (120 / 160 / 112 bytes for the three polyglot structs). */
+int reina_rt_abi_version(uint32_t* version, uint32_t* sizeof_instance_properties,
+                         uint32_t* sizeof_push_consts, uint32_t* sizeof_instance_data);
+
+/* Thread-local message describing the last failed call ("" if none). */
+const char* reina_rt_last_error(void);
+
+/* Replaces Scene::build's BLAS + TLAS construction (src/scene/Scene.cpp:93-116,
+ * src/graphics/Blas.cpp:8-124, src/tools/vktools.cpp:460-596): flattens every
+ * instance to world-space triangles and builds a BVH8 in HBM. Synchronous. */
+int reina_rt_scene_create(const ReinaSceneDesc* desc, ReinaScene** out_scene);
+int reina_rt_scene_destroy(ReinaScene* scene);
+int reina_rt_scene_info(const ReinaScene* scene, ReinaSceneInfo* out_info);
+
+/* Copies the BVH8 and triangle arrays back to host memory (for the traversal
+ * oracle and for inspection). Pass NULL buffers to query the sizes first. */
+int reina_rt_scene_export_bvh(const ReinaScene* scene, void* nodes, uint64_t* nodes_bytes,
+                              void* tris, uint64_t* tris_bytes);
+
+/* Host-only BVH8 build of a world-space triangle soup (9 floats per triangle),
+ * the same builder reina_rt_scene_create uses; needs no GPU. Pass NULL
+ * buffers to query the sizes first. Layout of the outputs: see
+ * reina_rt_scene_export_bvh. */
+int reina_rt_bvh_build_host(const float* triangles, uint64_t num_triangles, void* nodes,
+                            uint64_t* nodes_bytes, void* tris, uint64_t* tris_bytes,
+                            uint32_t* max_depth);
+
+/* Replaces one vkCmdTraceRaysKHR(W, H, 1) of the raygen/closest-hit/miss
+ * pipeline (src/Reina.cpp:425-470): renders samplesPerPixel samples per pixel
+ * and writes the (running-mean) radiance into image_rgba32f (device pointer,
+ * float4[W*H], row-major, y = 0 at the top, w = 1). When pc->sampleBatch > 0
+ * the previous image content is blended as (prev*b + new)/(b+1). The caller
+ * increments sampleBatch after the call, as Reina::traceRays does. Enqueued
+ * on `stream`; returns after enqueueing all bounce iterations. */
+int reina_rt_trace(ReinaScene* scene, const RtPushConsts* pc, float* image_rgba32f,
+                   uint32_t width, uint32_t height, void* stream);
+
+/* Statistics of the last reina_rt_trace on this scene (synchronises it). */
+typedef struct ReinaTraceStats {
+    uint64_t extension_rays;  /* every traced extension ray (primary + secondary) */
+    uint64_t shadow_rays;     /* NEE shadow/connect rays (0 unless NEE enabled)   */
+    uint32_t iterations;      /* wavefront bounce iterations launched             */
+    uint32_t _pad0;
+    float    trace_ms;        /* GPU time of the whole trace (events)             */
+    float    extend_ms;       /* GPU time inside the extend kernel                */
+} ReinaTraceStats;
+int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
+
+/* Closest-hit query for a batch of rays (device pointers): the traceRayEXT
+ * of shaders/raytrace/raytrace.rgen.glsl:110-122 as a stand-alone kernel.
+ * If counters != NULL (device uint32[2*n]) the per-ray node visits and
+ * triangle tests are written there (counting build of the same kernel). */
+int reina_rt_intersect(ReinaScene* scene, const ReinaRay* rays, ReinaHit* hits,
+                       uint32_t n, uint32_t* counters, void* stream);
+
+/* Bloom (blurX with threshold, blurY, combine) then ACES-fitted tone map to
+ * RGBA8 UNORM: shaders/postprocessing/bloom/{blurX,blurY,combine}.comp.glsl, tonemap/tonemapping.comp.glsl,
+ * dispatch order src/Reina.cpp:472-577. in_rgba32f, scratch (2 * W*H float4)
+ * and out_rgba8 (W*H*4 bytes) are device pointers. */
+int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_t* out_rgba8,
+                         uint32_t width, uint32_t height, const BloomPushConsts* bloom,
+                         const TonemappingPushConsts* tonemap, void* stream);
+
+/* Diagnostic: evaluates the library's FP32 elementary functions on n inputs
+ * (device pointers) so that tests can compare them with the host definitions.
+ * fn: 0 sin, 1 cos, 2 log, 3 exp, 4 sqrt, 5 pow(x, y = in2). */
+int reina_rt_debug_math(int fn, const float* in, const float* in2, float* out,
+                        uint32_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REINA_RT_H */

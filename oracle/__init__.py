@@ -1,0 +1,196 @@
+"""ORACLE — ctypes binding of liboracle.so, the CPU restatement of Reina's
+ray-tracing hot path (see oracle.h). Test infrastructure only: used by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
+product path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    r = subprocess.run(["make", "-C", HERE, "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = ctypes.CDLL(LIB_PATH)
+    c_p, u32, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
+    L.or_scene_create.argtypes = [c_p]
+    L.or_scene_create.restype = c_p
+    L.or_scene_destroy.argtypes = [c_p]
+    L.or_scene_destroy.restype = None
+    L.or_scene_set_bvh.argtypes = [c_p, c_p, u64, c_p, u64]
+    L.or_intersect_brute32.argtypes = [c_p, c_p, c_p, u64, i32]
+    L.or_intersect_brute32.restype = None
+    L.or_intersect_bvh.argtypes = [c_p, c_p, c_p, c_p, u64, i32]
+    L.or_intersect_bvh.restype = None
+    L.or_intersect_fp64.argtypes = [c_p, c_p, u64, c_p, c_p, c_p, c_p, c_p, f64, i32]
+    L.or_intersect_fp64.restype = None
+    L.or_random.argtypes = [ctypes.POINTER(u32)]
+    L.or_random.restype = ctypes.c_float
+    L.or_pcg_stream.argtypes = [u32, u32, c_p, c_p]
+    L.or_pcg_stream.restype = None
+    L.or_offset_position.argtypes = [c_p, c_p, c_p, i32, c_p]
+    L.or_offset_position.restype = None
+    L.or_sky.argtypes = [c_p, c_p]
+    L.or_sky.restype = None
+    L.or_camera_ray.argtypes = [c_p, u32, u32, u32, u32, ctypes.POINTER(u32), c_p, c_p]
+    L.or_camera_ray.restype = None
+    L.or_diffuse_dir.argtypes = [c_p, ctypes.POINTER(u32), c_p]
+    L.or_diffuse_dir.restype = None
+    L.or_render.argtypes = [c_p, c_p, u32, u32, c_p, u32, u32, u32, u32, i32, i32, ctypes.POINTER(u64)]
+    L.or_render.restype = None
+    L.or_render_sum64.argtypes = [c_p, c_p, u32, u32, u32, u32, c_p, c_p, c_p, u32, u32, u32, u32, i32, i32]
+    L.or_render_sum64.restype = None
+    L.or_postprocess.argtypes = [c_p, u32, u32, c_p, c_p, c_p, c_p, c_p]
+    L.or_postprocess.restype = None
+    L.or_math_eval.argtypes = [i32, c_p, c_p, c_p, u64]
+    L.or_math_eval.restype = None
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class OracleScene:
+    """Flattened copy of a packed scene (+ optionally the library's BVH8)."""
+
+    def __init__(self, arrays, make_desc):
+        self.arrays = arrays
+        d = make_desc(arrays)
+        self.h = lib().or_scene_create(ctypes.byref(d))
+        self._bvh = None
+
+    def set_bvh(self, nodes, tris):
+        self._bvh = (np.ascontiguousarray(nodes), np.ascontiguousarray(tris))
+        lib().or_scene_set_bvh(self.h, _p(self._bvh[0]), self._bvh[0].nbytes, _p(self._bvh[1]), self._bvh[1].nbytes)
+
+    def brute32(self, rays, nthreads=8):
+        hits = np.zeros(len(rays), _hit_dtype())
+        lib().or_intersect_brute32(self.h, _p(rays), _p(hits), len(rays), nthreads)
+        return hits
+
+    def bvh(self, rays, nthreads=8):
+        hits = np.zeros(len(rays), _hit_dtype())
+        cnt = np.zeros((len(rays), 2), np.uint32)
+        lib().or_intersect_bvh(self.h, _p(rays), _p(hits), _p(cnt), len(rays), nthreads)
+        return hits, cnt
+
+    def fp64(self, rays, tie_rel=1e-5, nthreads=8):
+        n = len(rays)
+        t = np.zeros(n)
+        prim = np.zeros(n, np.uint32)
+        u = np.zeros(n)
+        v = np.zeros(n)
+        amb = np.zeros(n, np.uint8)
+        lib().or_intersect_fp64(self.h, _p(rays), n, _p(t), _p(prim), _p(u), _p(v), _p(amb), tie_rel, nthreads)
+        return t, prim, u, v, amb.astype(bool)
+
+    def render(self, pc, W, H, img=None, rect=None, use_bvh=True, nthreads=8):
+        if img is None:
+            img = np.zeros((H, W, 4), np.float32)
+        x0, y0, x1, y1 = rect if rect else (0, 0, W, H)
+        pcb = np.ascontiguousarray(np.asarray(pc).reshape(()))
+        rays = ctypes.c_uint64(0)
+        lib().or_render(self.h, pcb.ctypes.data, W, H, _p(img), x0, y0, x1, y1, int(use_bvh), nthreads,
+                        ctypes.byref(rays))
+        return img, rays.value
+
+    def render_sum64(self, pc, W, H, first_batch, nbatches, rect=None, use_bvh=True, nthreads=8):
+        s = np.zeros((H, W, 3))
+        sq = np.zeros((H, W, 3))
+        cnt = np.zeros((H, W), np.uint32)
+        x0, y0, x1, y1 = rect if rect else (0, 0, W, H)
+        pcb = np.ascontiguousarray(np.asarray(pc).reshape(()))
+        lib().or_render_sum64(self.h, pcb.ctypes.data, W, H, first_batch, nbatches, _p(s), _p(sq), _p(cnt),
+                              x0, y0, x1, y1, int(use_bvh), nthreads)
+        return s, sq, cnt
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().or_scene_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def _hit_dtype():
+    return np.dtype([("t", "<f4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
+
+
+def pcg_stream(seed, n):
+    st = np.zeros(n, np.uint32)
+    val = np.zeros(n, np.float32)
+    lib().or_pcg_stream(seed, n, _p(st), _p(val))
+    return st, val
+
+
+def offset_position(p, n, dielectric=False, d=None):
+    p = np.asarray(p, np.float32)
+    n = np.asarray(n, np.float32)
+    d = np.zeros(3, np.float32) if d is None else np.asarray(d, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().or_offset_position(_p(p), _p(n), _p(out), int(dielectric), _p(d))
+    return out
+
+
+def sky(d):
+    d = np.asarray(d, np.float32)
+    out = np.zeros(3, np.float32)
+    lib().or_sky(_p(d), _p(out))
+    return out
+
+
+def camera_ray(pc, px, py, W, H, rng):
+    pcb = np.ascontiguousarray(np.asarray(pc).reshape(()))
+    st = ctypes.c_uint32(rng)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().or_camera_ray(pcb.ctypes.data, px, py, W, H, ctypes.byref(st), _p(o), _p(d))
+    return o, d, st.value
+
+
+def diffuse_dir(n, rng):
+    n = np.asarray(n, np.float32)
+    st = ctypes.c_uint32(rng)
+    out = np.zeros(3, np.float32)
+    lib().or_diffuse_dir(_p(n), ctypes.byref(st), _p(out))
+    return out, st.value
+
+
+def math_eval(fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, np.float32)
+    out = np.zeros_like(x)
+    lib().or_math_eval(fn, _p(x), _p(y), _p(out), len(x))
+    return out
+
+
+def postprocess(img, radius=5.0, threshold=1.0, intensity=0.05, exposure=1.0):
+    H, W = img.shape[:2]
+    img = np.ascontiguousarray(img, np.float32)
+    ping = np.zeros_like(img)
+    pong = np.zeros_like(img)
+    out = np.zeros((H, W, 4), np.uint8)
+    b = np.array([radius, threshold, intensity], np.float32)
+    t = np.array([exposure], np.float32)
+    lib().or_postprocess(_p(img), W, H, _p(b), _p(t), _p(ping), _p(pong), _p(out))
+    return out, ping

@@ -1,0 +1,433 @@
+/*
+ * or_geom.c — ORACLE (test infrastructure only): scene flattening and the
+ * closest-hit query that the driver performs for traceRayEXT
+ * (shaders/raytrace/raytrace.rgen.glsl:110-122: opaque, no culling, mask 0xFF,
+ * tmin 0, tmax 10000).
+ *
+ *  - or_scene_create flattens every TLAS instance (src/tools/vktools.cpp:460-488:
+ *    3x4 object-to-world rows, customIndex = instancePropertiesID, SBT offset =
+ *    materialIdx) over its BLAS triangles (src/graphics/Blas.cpp:17-39: u32
+ *    non-offset indices + firstVertex; the offset index buffer of
+ *    src/scene/Models.cpp:78-81 addresses the same vertices).
+ *  - or_intersect_brute32: the watertight FP32 ray/triangle test (Woop, Benthin,
+ *    Wald, JCGT 2(1) 2013) over all triangles; closest hit = argmin over the
+ *    pair (t, flattened id), which makes the answer independent of visit order.
+ *  - or_intersect_bvh: the same test driven by a BVH8 traversal; must return the
+ *    brute-force answer bit for bit, and defines the node-visit / triangle-test
+ *    counts of the library's extend kernel.
+ *  - or_intersect_fp64: textbook Moller-Trumbore in double precision over all
+ *    triangles, with an ambiguity flag for edge/vertex/near-equal-t ties.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* world = ((c0*x + c1*y) + c2*z) + c3 : gl_ObjectToWorldEXT * vec4(p, 1) */
+static v3 xform_point(const OrInstance* in, v3 p) {
+    return vadd(vadd(vadd(vscale(in->c0, p.x), vscale(in->c1, p.y)), vscale(in->c2, p.z)), in->c3);
+}
+
+#define DUP(dst, src, count, type)                                             \
+    do {                                                                       \
+        (dst) = (type*)malloc((((count) != 0) ? (count) : 1) * sizeof(type));         \
+        if (((count) != 0) && (src)) memcpy((dst), (src), (count) * sizeof(type));    \
+    } while (0)
+
+OrScene* or_scene_create(const ReinaSceneDesc* d) {
+    OrScene* s = (OrScene*)calloc(1, sizeof(OrScene));
+    s->nv = d->num_vertices;     DUP(s->vertices, d->vertices, s->nv * 4, float);
+    s->ni = d->num_indices;      DUP(s->indices, d->indices, s->ni, uint32_t);
+    s->ntbn = d->num_tbns;       DUP(s->tbns, d->tbns, s->ntbn * 9, float);
+    s->ntbni = d->num_tbn_indices; DUP(s->tbn_indices, d->tbn_indices, s->ntbni, uint32_t);
+    s->ntc = d->num_tex_coords;  DUP(s->tex_coords, d->tex_coords, s->ntc * 2, float);
+    s->ntci = d->num_tex_indices; DUP(s->tex_indices, d->tex_indices, s->ntci, uint32_t);
+    s->nprops = d->num_props;    DUP(s->props, d->props, s->nprops, ReinaInstanceProperties);
+    s->nmodels = d->num_models;  DUP(s->models, d->models, s->nmodels, ReinaModelRange);
+    s->ninst = d->num_instances;
+    s->inst = (OrInstance*)calloc(s->ninst ? s->ninst : 1, sizeof(OrInstance));
+    uint64_t ntri = 0;
+    for (uint32_t i = 0; i < s->ninst; ++i) {
+        const ReinaInstanceDesc* id = &d->instances[i];
+        OrInstance* in = &s->inst[i];
+        const float* m = id->transform; /* column-major mat4 */
+        in->c0 = V3(m[0], m[1], m[2]);
+        in->c1 = V3(m[4], m[5], m[6]);
+        in->c2 = V3(m[8], m[9], m[10]);
+        in->c3 = V3(m[12], m[13], m[14]);
+        in->props_id = id->instancePropertiesID;
+        in->material = id->materialIdx;
+        in->model = id->modelID;
+        in->tri_base = (uint32_t)ntri;
+        ntri += s->models[id->modelID].indexCount;
+    }
+    s->ntri = ntri;
+    s->wtri = (float*)malloc((ntri ? ntri : 1) * 9 * sizeof(float));
+    s->tri_inst = (uint32_t*)malloc((ntri ? ntri : 1) * sizeof(uint32_t));
+    s->tri_prim = (uint32_t*)malloc((ntri ? ntri : 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i < s->ninst; ++i) {
+        const OrInstance* in = &s->inst[i];
+        const ReinaModelRange* mr = &s->models[in->model];
+        for (uint32_t p = 0; p < mr->indexCount; ++p) {
+            uint64_t g = (uint64_t)in->tri_base + p;
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = s->indices[3ull * p + mr->indexOffset + k];
+                v3 ov = V3(s->vertices[4ull * vi], s->vertices[4ull * vi + 1], s->vertices[4ull * vi + 2]);
+                v3 w = xform_point(in, ov);
+                s->wtri[9 * g + 3 * k + 0] = w.x;
+                s->wtri[9 * g + 3 * k + 1] = w.y;
+                s->wtri[9 * g + 3 * k + 2] = w.z;
+            }
+            s->tri_inst[g] = i;
+            s->tri_prim[g] = p;
+        }
+    }
+    return s;
+}
+
+void or_scene_destroy(OrScene* s) {
+    if (!s) return;
+    free(s->vertices); free(s->indices); free(s->tbns); free(s->tbn_indices);
+    free(s->tex_coords); free(s->tex_indices); free(s->props); free(s->models);
+    free(s->inst); free(s->wtri); free(s->tri_inst); free(s->tri_prim);
+    free(s);
+}
+
+int or_scene_set_bvh(OrScene* s, const void* nodes, uint64_t nodes_bytes, const void* tris,
+                     uint64_t tris_bytes) {
+    s->nodes = (const uint8_t*)nodes;
+    s->nnodes = nodes_bytes / 256;
+    s->btris = (const float*)tris;
+    s->nbtris = tris_bytes / 48;
+    return 0;
+}
+
+/* ---- watertight FP32 test ------------------------------------------------ */
+typedef struct {
+    v3 o, d;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float tmin, tmax;
+    int valid;   /* finite origin and direction, direction != 0; invalid rays miss */
+} WRay;
+
+static int or_finite(float x) { return (x == x) && (x - x == 0.0f); }
+
+static float comp(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+static void wray_init(WRay* r, const ReinaRay* in) {
+    r->o = V3(in->ox, in->oy, in->oz);
+    r->d = V3(in->dx, in->dy, in->dz);
+    r->tmin = in->tmin;
+    r->tmax = in->tmax;
+    r->valid = or_finite(r->o.x) && or_finite(r->o.y) && or_finite(r->o.z) && or_finite(r->d.x) && or_finite(r->d.y) &&
+               or_finite(r->d.z) && !(r->d.x == 0.0f && r->d.y == 0.0f && r->d.z == 0.0f);
+    float ax = fabsf(r->d.x), ay = fabsf(r->d.y), az = fabsf(r->d.z);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    if (comp(r->d, kz) < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r->kx = kx; r->ky = ky; r->kz = kz;
+    float dz = comp(r->d, kz);
+    r->Sx = comp(r->d, kx) / dz;
+    r->Sy = comp(r->d, ky) / dz;
+    r->Sz = 1.0f / dz;
+}
+
+/* returns 1 and writes t,u,v if the triangle is hit with t in [tmin, tbest] */
+static int wtest(const WRay* r, const float* v0p, const float* v1p, const float* v2p, float tbest,
+                 float* to, float* uo, float* vo) {
+    v3 A = vsub(V3(v0p[0], v0p[1], v0p[2]), r->o);
+    v3 B = vsub(V3(v1p[0], v1p[1], v1p[2]), r->o);
+    v3 C = vsub(V3(v2p[0], v2p[1], v2p[2]), r->o);
+    float Akz = comp(A, r->kz), Bkz = comp(B, r->kz), Ckz = comp(C, r->kz);
+    float Ax = comp(A, r->kx) - r->Sx * Akz;
+    float Ay = comp(A, r->ky) - r->Sy * Akz;
+    float Bx = comp(B, r->kx) - r->Sx * Bkz;
+    float By = comp(B, r->ky) - r->Sy * Bkz;
+    float Cx = comp(C, r->kx) - r->Sx * Ckz;
+    float Cy = comp(C, r->ky) - r->Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+    float det = (U + V) + W;
+    if (det == 0.0f) return 0;
+    float Az = r->Sz * Akz, Bz = r->Sz * Bkz, Cz = r->Sz * Ckz;
+    float T = (U * Az + V * Bz) + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r->tmin && t <= tbest)) return 0;
+    *to = t;
+    *uo = V * rcp;
+    *vo = W * rcp;
+    return 1;
+}
+
+/* closest = argmin (t, id) */
+static inline int better(float t, uint32_t id, float tb, uint32_t idb) {
+    return (t < tb) || (t == tb && id < idb);
+}
+
+static void brute32_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit) {
+    WRay r;
+    wray_init(&r, ray);
+    float tb = r.tmax;
+    uint32_t ib = 0xFFFFFFFFu;
+    float ub = 0.0f, vb = 0.0f;
+    for (uint64_t g = 0; r.valid && g < s->ntri; ++g) {
+        const float* v = &s->wtri[9 * g];
+        float t, u, w;
+        if (wtest(&r, v, v + 3, v + 6, tb, &t, &u, &w) && better(t, (uint32_t)g, tb, ib)) {
+            tb = t; ib = (uint32_t)g; ub = u; vb = w;
+        }
+    }
+    hit->t = tb; hit->prim = ib; hit->u = ub; hit->v = vb;
+}
+
+/* ---- BVH8 traversal restatement ------------------------------------------
+ * Node layout (256 bytes): float lo_x[8], hi_x[8], lo_y[8], hi_y[8], lo_z[8],
+ * hi_z[8]; u32 child_base; u32 tri_base; u8 meta[8]; 48 bytes padding.
+ * meta 0 = empty slot; 0x80|rank = internal child child_base+rank; otherwise a
+ * leaf of (meta>>5)&3 triangles starting at tri_base + (meta & 31).
+ * Triangles: float4 x3 (v0.xyz|id, v1.xyz|-, v2.xyz|-).
+ * Visit order: leaves of a node first, in slot order i ^ oct (oct = ray
+ * direction sign bits); then the node's internal hits become a group
+ * {child_base, internal-slot mask, ordered hit mask} on a LIFO stack, popped
+ * lowest ordered bit first. Box test: slabs with the near/far plane picked
+ * by the direction sign and the interval widened by 2^-16 relative
+ * (conservative: never rejects a box whose triangle passes). Rays with a
+ * non-finite component or a zero direction miss everything. */
+#define BOX_EPS 1.52587890625e-05f /* 2^-16 */
+
+typedef struct { uint32_t base, imask, om; } Group;
+
+static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32_t* cnt) {
+    WRay r;
+    wray_init(&r, ray);
+    float tb = r.tmax;
+    uint32_t ib = 0xFFFFFFFFu;
+    float ub = 0.0f, vb = 0.0f;
+    uint32_t nvis = 0, ntst = 0;
+    /* inverse direction with zero components replaced by +-1e-30 */
+    float dx = r.d.x, dy = r.d.y, dz = r.d.z;
+    if (fabsf(dx) < 1e-30f) dx = copysignf(1e-30f, dx);
+    if (fabsf(dy) < 1e-30f) dy = copysignf(1e-30f, dy);
+    if (fabsf(dz) < 1e-30f) dz = copysignf(1e-30f, dz);
+    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    uint32_t oct = (r.d.x < 0.0f ? 1u : 0u) | (r.d.y < 0.0f ? 2u : 0u) | (r.d.z < 0.0f ? 4u : 0u);
+    Group stack[64];
+    int sp = 0;
+    uint32_t cur = 0;
+    int have = (s->nnodes > 0) && r.valid;
+    while (have) {
+        const uint8_t* nb = s->nodes + (uint64_t)cur * 256;
+        const float* f = (const float*)nb;
+        uint32_t child_base, tri_base;
+        memcpy(&child_base, nb + 192, 4);
+        memcpy(&tri_base, nb + 196, 4);
+        const uint8_t* meta = nb + 200;
+        nvis++;
+        uint32_t hitm = 0;
+        /* near/far plane per axis by direction sign (lo <= hi, so this is min/max) */
+        int sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            float nx = sx ? f[0 + k] : f[8 + k], fx = sx ? f[8 + k] : f[0 + k];
+            float ny = sy ? f[16 + k] : f[24 + k], fy = sy ? f[24 + k] : f[16 + k];
+            float nz = sz ? f[32 + k] : f[40 + k], fz = sz ? f[40 + k] : f[32 + k];
+            float tn = fmaxf(fmaxf((nx - r.o.x) * ix, (ny - r.o.y) * iy), (nz - r.o.z) * iz);
+            float tf = fminf(fminf((fx - r.o.x) * ix, (fy - r.o.y) * iy), (fz - r.o.z) * iz);
+            tn = tn - fabsf(tn) * BOX_EPS;
+            tf = tf + fabsf(tf) * BOX_EPS;
+            tn = fmaxf(tn, r.tmin);
+            tf = fminf(tf, tb);
+            if (meta[k] != 0 && tn <= tf) hitm |= 1u << k;
+        }
+        uint32_t imask = 0, om = 0;
+        for (uint32_t i = 0; i < 8; ++i) {
+            uint32_t k = i ^ oct;
+            uint8_t m = meta[k];
+            if (m & 0x80) imask |= 1u << k;
+            if (!(hitm & (1u << k))) continue;
+            if (m & 0x80) { om |= 1u << i; continue; }
+            uint32_t cntl = (m >> 5) & 3u, off = m & 31u;
+            for (uint32_t j = 0; j < cntl; ++j) {
+                const float* tv = s->btris + 12ull * (tri_base + off + j);
+                uint32_t id;
+                memcpy(&id, &tv[3], 4);
+                float t, u, w;
+                ntst++;
+                if (wtest(&r, tv, tv + 4, tv + 8, tb, &t, &u, &w) && better(t, id, tb, ib)) {
+                    tb = t; ib = id; ub = u; vb = w;
+                }
+            }
+        }
+        if (om) { stack[sp].base = child_base; stack[sp].imask = imask; stack[sp].om = om; sp++; }
+        have = 0;
+        if (sp > 0) {
+            Group* g = &stack[sp - 1];
+            uint32_t i = (uint32_t)__builtin_ctz(g->om);
+            g->om &= g->om - 1;
+            uint32_t slot = i ^ oct;
+            cur = g->base + (uint32_t)__builtin_popcount(g->imask & ((1u << slot) - 1u));
+            if (g->om == 0) sp--;
+            have = 1;
+        }
+    }
+    hit->t = tb; hit->prim = ib; hit->u = ub; hit->v = vb;
+    if (cnt) { cnt[0] = nvis; cnt[1] = ntst; }
+}
+
+/* ---- FP64 Moller-Trumbore ------------------------------------------------ */
+typedef struct { double x, y, z; } d3;
+static d3 D3(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static d3 dsub(d3 a, d3 b) { return D3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static d3 dcross(d3 a, d3 b) { return D3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+static int mt64(d3 o, d3 d, const float* v, double tmin, double tmax, double* t, double* u, double* w) {
+    d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
+    d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
+    d3 p = dcross(d, e2);
+    double det = ddot(e1, p);
+    if (det == 0.0) return 0;
+    double inv = 1.0 / det;
+    d3 sv = dsub(o, v0);
+    double uu = ddot(sv, p) * inv;
+    if (uu < 0.0 || uu > 1.0) return 0;
+    d3 q = dcross(sv, e1);
+    double vv = ddot(d, q) * inv;
+    if (vv < 0.0 || uu + vv > 1.0) return 0;
+    double tt = ddot(e2, q) * inv;
+    if (tt < tmin || tt > tmax) return 0;
+    *t = tt; *u = uu; *w = vv;
+    return 1;
+}
+
+/* Signed-area margin used to flag near-edge hits / near misses. */
+static void fp64_one(const OrScene* s, const ReinaRay* ray, double* to, uint32_t* po, double* uo,
+                     double* vo, uint8_t* amb, double tie_rel) {
+    d3 o = D3(ray->ox, ray->oy, ray->oz), d = D3(ray->dx, ray->dy, ray->dz);
+    double tb = ray->tmax, t2 = INFINITY, ub = 0, vb = 0;
+    uint32_t ib = 0xFFFFFFFFu;
+    int near_miss = 0;
+    /* widen barycentric acceptance to find near misses */
+    for (uint64_t g = 0; g < s->ntri; ++g) {
+        double t, u, w;
+        const float* v = &s->wtri[9 * g];
+        if (mt64(o, d, v, ray->tmin, ray->tmax, &t, &u, &w)) {
+            if (t < tb || (t == tb && (uint32_t)g < ib)) {
+                if (ib != 0xFFFFFFFFu) t2 = tb < t2 ? tb : t2;
+                tb = t; ib = (uint32_t)g; ub = u; vb = w;
+            } else if (t < t2) {
+                t2 = t;
+            }
+        } else {
+            /* near miss: hit with a slightly enlarged triangle? */
+            d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
+            d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
+            d3 p = dcross(d, e2);
+            double det = ddot(e1, p);
+            if (det == 0.0) continue;
+            double inv = 1.0 / det;
+            d3 sv = dsub(o, v0);
+            double uu = ddot(sv, p) * inv;
+            d3 q = dcross(sv, e1);
+            double vv = ddot(d, q) * inv;
+            double tt = ddot(e2, q) * inv;
+            double m = tie_rel;
+            if (uu >= -m && vv >= -m && uu + vv <= 1.0 + m && tt >= ray->tmin - m && tt <= ray->tmax * (1 + m)) {
+                if (tt <= tb * (1 + m)) near_miss = 1;
+            }
+        }
+    }
+    *to = tb; *po = ib; *uo = ub; *vo = vb;
+    int a = near_miss;
+    if (ib != 0xFFFFFFFFu) {
+        double wb = 1.0 - ub - vb;
+        if (ub < tie_rel || vb < tie_rel || wb < tie_rel) a = 1;
+        if (t2 <= tb * (1.0 + tie_rel) + 1e-300) a = 1;
+        if (tb <= ray->tmin * (1.0 + tie_rel) + tie_rel) a = 1;
+    }
+    *amb = (uint8_t)a;
+}
+
+/* ---- threading ----------------------------------------------------------- */
+typedef struct {
+    int kind;
+    const OrScene* s;
+    const ReinaRay* rays;
+    ReinaHit* hits;
+    uint32_t* counters;
+    double *t, *u, *v;
+    uint32_t* prim;
+    uint8_t* amb;
+    double tie;
+    uint64_t begin, end;
+} IJob;
+
+static void* ijob_run(void* arg) {
+    IJob* j = (IJob*)arg;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        if (j->kind == 0) brute32_one(j->s, &j->rays[i], &j->hits[i]);
+        else if (j->kind == 1) bvh_one(j->s, &j->rays[i], &j->hits[i], j->counters ? &j->counters[2 * i] : NULL);
+        else fp64_one(j->s, &j->rays[i], &j->t[i], &j->prim[i], &j->u[i], &j->v[i], &j->amb[i], j->tie);
+    }
+    return NULL;
+}
+
+static void run_jobs(IJob* proto, uint64_t n, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > n) nthreads = n ? (int)n : 1;
+    pthread_t th[256];
+    IJob jobs[256];
+    if (nthreads > 256) nthreads = 256;
+    for (int k = 0; k < nthreads; ++k) {
+        jobs[k] = *proto;
+        jobs[k].begin = n * (uint64_t)k / (uint64_t)nthreads;
+        jobs[k].end = n * (uint64_t)(k + 1) / (uint64_t)nthreads;
+        pthread_create(&th[k], NULL, ijob_run, &jobs[k]);
+    }
+    for (int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+}
+
+void or_intersect_brute32(const OrScene* s, const ReinaRay* rays, ReinaHit* hits, uint64_t n, int nthreads) {
+    IJob j; memset(&j, 0, sizeof j);
+    j.kind = 0; j.s = s; j.rays = rays; j.hits = hits;
+    run_jobs(&j, n, nthreads);
+}
+
+void or_intersect_bvh(const OrScene* s, const ReinaRay* rays, ReinaHit* hits, uint32_t* counters,
+                      uint64_t n, int nthreads) {
+    IJob j; memset(&j, 0, sizeof j);
+    j.kind = 1; j.s = s; j.rays = rays; j.hits = hits; j.counters = counters;
+    run_jobs(&j, n, nthreads);
+}
+
+void or_intersect_fp64(const OrScene* s, const ReinaRay* rays, uint64_t n, double* t_out,
+                       uint32_t* prim_out, double* u_out, double* v_out, uint8_t* ambiguous,
+                       double tie_rel, int nthreads) {
+    IJob j; memset(&j, 0, sizeof j);
+    j.kind = 2; j.s = s; j.rays = rays; j.t = t_out; j.prim = prim_out; j.u = u_out; j.v = v_out;
+    j.amb = ambiguous; j.tie = tie_rel;
+    run_jobs(&j, n, nthreads);
+}
+
+/* used by or_shade.c */
+void or_closest_hit(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, int use_bvh) {
+    if (use_bvh && s->nnodes) bvh_one(s, ray, hit, NULL);
+    else brute32_one(s, ray, hit);
+}

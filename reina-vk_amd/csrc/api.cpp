@@ -1,0 +1,439 @@
+// api.cpp — the C ABI of libreina_rt (include/reina_rt.h).
+//
+// reina_rt_scene_create  replaces Scene::build's BLAS/TLAS construction
+//                        (src/scene/Scene.cpp:93-116): validate the packed
+//                        Models/Instances buffers, flatten every instance to
+//                        world space, build the BVH8, upload one HBM arena.
+// reina_rt_trace         replaces Reina::traceRays' vkCmdTraceRaysKHR
+//                        (src/Reina.cpp:425-470): init, then extend/shade
+//                        bounce iterations until every pixel has finished its
+//                        samples, then finish (running mean into the image).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bvh_host.h"
+#include "rt_layout.h"
+
+#include "wavefront.h"
+
+using namespace rt;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(REINA_RT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct ReinaScene {
+    int device = 0;
+    SceneView sv{};
+    void* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    ReinaSceneInfo info{};
+    std::vector<Bvh8Node> host_nodes;   // kept for export
+    std::vector<float> host_tris;
+    // trace buffers (grown on demand)
+    uint64_t cap_px = 0;
+    void* tbuf = nullptr;
+    PathState S{};
+    Queues Q{};
+    Ctrl* ctrl = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* fetch = nullptr;
+    uint32_t* h_count = nullptr;        // pinned
+    uint32_t grid_extend = 0, grid_shade = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool traced = false;
+    uint32_t last_iters = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+extern "C" {
+
+int reina_rt_abi_version(uint32_t* version, uint32_t* s_ip, uint32_t* s_pc, uint32_t* s_id) {
+    if (version) *version = REINA_RT_ABI_VERSION;
+    if (s_ip) *s_ip = (uint32_t)sizeof(ReinaInstanceProperties);
+    if (s_pc) *s_pc = (uint32_t)sizeof(RtPushConsts);
+    if (s_id) *s_id = (uint32_t)sizeof(ReinaInstanceData);
+    return REINA_RT_OK;
+}
+
+const char* reina_rt_last_error(void) { return g_err.c_str(); }
+
+static int validate(const ReinaSceneDesc* d) {
+    if (!d) return fail(REINA_RT_ERR_INVALID_ARG, "scene description is null");
+    if (d->struct_size != sizeof(ReinaSceneDesc))
+        return fail(REINA_RT_ERR_ABI, "ReinaSceneDesc.struct_size mismatch (expected " +
+                                          std::to_string(sizeof(ReinaSceneDesc)) + ")");
+    if (d->num_instances && (!d->instances || !d->models || !d->props))
+        return fail(REINA_RT_ERR_INVALID_ARG, "instances given without models/props");
+    for (uint32_t m = 0; m < d->num_models; ++m) {
+        const ReinaModelRange& r = d->models[m];
+        if ((uint64_t)r.indexOffset + 3ull * r.indexCount > d->num_indices)
+            return fail(REINA_RT_ERR_INVALID_ARG, "model " + std::to_string(m) + " index range exceeds indices");
+    }
+    for (uint64_t i = 0; i < d->num_indices; ++i)
+        if (d->indices[i] >= d->num_vertices)
+            return fail(REINA_RT_ERR_INVALID_ARG, "index " + std::to_string(i) + " out of vertex range");
+    for (uint64_t i = 0; i < d->num_tbn_indices; ++i)
+        if (d->tbn_indices[i] >= d->num_tbns)
+            return fail(REINA_RT_ERR_INVALID_ARG, "tbn index " + std::to_string(i) + " out of range");
+    for (uint64_t i = 0; i < d->num_tex_indices; ++i)
+        if (d->tex_indices[i] != 0xFFFFFFFFu && d->tex_indices[i] >= d->num_tex_coords)
+            return fail(REINA_RT_ERR_INVALID_ARG, "tex index " + std::to_string(i) + " out of range");
+    for (uint32_t i = 0; i < d->num_instances; ++i) {
+        const ReinaInstanceDesc& in = d->instances[i];
+        if (in.modelID >= d->num_models)
+            return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": bad modelID");
+        if (in.instancePropertiesID >= d->num_props)
+            return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": bad instancePropertiesID");
+        if (in.materialIdx > 3)
+            return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": materialIdx > 3");
+        const ReinaInstanceProperties& p = d->props[in.instancePropertiesID];
+        const ReinaModelRange& r = d->models[in.modelID];
+        if ((uint64_t)p.indicesOffset + 3ull * r.indexCount > d->num_indices)
+            return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": indicesOffset out of range");
+        if (p.texIndicesOffset != 0xFFFFFFFFu && (uint64_t)p.texIndicesOffset + 3ull * r.indexCount > d->num_tex_indices)
+            return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": texIndicesOffset out of range");
+        if (p.textureID >= 0 || p.normalMapTexID >= 0 || p.bumpMapTexID >= 0)
+            return fail(REINA_RT_ERR_UNSUPPORTED, "instance " + std::to_string(i) +
+                                                      ": texture, normal and bump maps are not supported");
+    }
+    return REINA_RT_OK;
+}
+
+static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vector<float>& wtri,
+                        const std::vector<uint32_t>& tri_inst, const std::vector<uint32_t>& tri_prim,
+                        const std::vector<InstanceRec>& inst) {
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    struct Part { const void* src; uint64_t bytes; uint64_t off; };
+    std::vector<Part> parts;
+    uint64_t off = 0;
+    auto add = [&](const void* src, uint64_t bytes) {
+        parts.push_back({src, bytes, off});
+        off += al(bytes ? bytes : 4);
+        return parts.size() - 1;
+    };
+    size_t p_nodes = add(s->host_nodes.data(), s->host_nodes.size() * sizeof(Bvh8Node));
+    size_t p_tris = add(s->host_tris.data(), s->host_tris.size() * sizeof(float));
+    size_t p_ti = add(tri_inst.data(), tri_inst.size() * 4);
+    size_t p_tp = add(tri_prim.data(), tri_prim.size() * 4);
+    size_t p_in = add(inst.data(), inst.size() * sizeof(InstanceRec));
+    size_t p_pr = add(d->props, (uint64_t)d->num_props * sizeof(ReinaInstanceProperties));
+    size_t p_v = add(d->vertices, d->num_vertices * 16);
+    size_t p_i = add(d->indices, d->num_indices * 4);
+    size_t p_t = add(d->tbns, d->num_tbns * 36);
+    size_t p_tix = add(d->tbn_indices, d->num_tbn_indices * 4);
+    size_t p_tc = add(d->tex_coords, d->num_tex_coords * 8);
+    size_t p_tci = add(d->tex_indices, d->num_tex_indices * 4);
+    (void)wtri;
+    HIP_TRY(hipSetDevice(s->device));
+    hipError_t e = hipMalloc(&s->arena, off);
+    if (e != hipSuccess) return fail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(off) + " bytes failed");
+    s->arena_bytes = off;
+    for (const Part& p : parts)
+        if (p.bytes && p.src) HIP_TRY(hipMemcpy((char*)s->arena + p.off, p.src, p.bytes, hipMemcpyHostToDevice));
+    char* a = (char*)s->arena;
+    s->sv.nodes = (const Bvh8Node*)(a + parts[p_nodes].off);
+    s->sv.tris = (const float4*)(a + parts[p_tris].off);
+    s->sv.tri_inst = (const uint32_t*)(a + parts[p_ti].off);
+    s->sv.tri_prim = (const uint32_t*)(a + parts[p_tp].off);
+    s->sv.inst = (const InstanceRec*)(a + parts[p_in].off);
+    s->sv.props = (const ReinaInstanceProperties*)(a + parts[p_pr].off);
+    s->sv.vertices = (const float4*)(a + parts[p_v].off);
+    s->sv.indices = (const uint32_t*)(a + parts[p_i].off);
+    s->sv.tbns = (const float*)(a + parts[p_t].off);
+    s->sv.tbn_indices = (const uint32_t*)(a + parts[p_tix].off);
+    s->sv.tex_coords = (const float*)(a + parts[p_tc].off);
+    s->sv.tex_indices = (const uint32_t*)(a + parts[p_tci].off);
+    s->sv.num_tbns = d->num_tbns;
+    s->sv.num_tbn_indices = d->num_tbn_indices;
+    s->sv.num_tex_coords = d->num_tex_coords;
+    s->sv.num_nodes = s->host_nodes.size();
+    s->sv.num_tris = tri_inst.size();
+    return REINA_RT_OK;
+}
+
+int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
+    if (!out) return fail(REINA_RT_ERR_INVALID_ARG, "out_scene is null");
+    *out = nullptr;
+    int rc = validate(d);
+    if (rc) return rc;
+    if (d->flags & REINA_RT_NODES_QUANTIZED)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "quantized nodes are not built");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
+    if ((int)d->device >= ndev) return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal out of range");
+    auto t0 = std::chrono::steady_clock::now();
+    // flatten: world = ((c0*x + c1*y) + c2*z) + c3, instance-major order
+    uint64_t ntri = 0;
+    std::vector<InstanceRec> inst(d->num_instances);
+    for (uint32_t i = 0; i < d->num_instances; ++i) {
+        const ReinaInstanceDesc& id = d->instances[i];
+        InstanceRec& r = inst[i];
+        const float* m = id.transform;
+        for (int k = 0; k < 3; ++k) { r.c0[k] = m[k]; r.c1[k] = m[4 + k]; r.c2[k] = m[8 + k]; r.c3[k] = m[12 + k]; }
+        r.props_id = id.instancePropertiesID;
+        r.material = id.materialIdx;
+        r.model = id.modelID;
+        r.tri_base = (uint32_t)ntri;
+        ntri += d->models[id.modelID].indexCount;
+    }
+    if (ntri >= (1ull << 27)) return fail(REINA_RT_ERR_INVALID_ARG, "more than 2^27 triangles");
+    std::vector<float> wtri(9 * ntri);
+    std::vector<uint32_t> tri_inst(ntri), tri_prim(ntri);
+    for (uint32_t i = 0; i < d->num_instances; ++i) {
+        const InstanceRec& r = inst[i];
+        const ReinaModelRange& mr = d->models[r.model];
+        for (uint32_t p = 0; p < mr.indexCount; ++p) {
+            uint64_t g = (uint64_t)r.tri_base + p;
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = d->indices[3ull * p + mr.indexOffset + k];
+                float x = d->vertices[4ull * vi], y = d->vertices[4ull * vi + 1], z = d->vertices[4ull * vi + 2];
+                for (int c = 0; c < 3; ++c)
+                    wtri[9 * g + 3 * k + c] = ((r.c0[c] * x + r.c1[c] * y) + r.c2[c] * z) + r.c3[c];
+            }
+            tri_inst[g] = i;
+            tri_prim[g] = p;
+        }
+    }
+    ReinaScene* s = new ReinaScene();
+    s->device = (int)d->device;
+    Bvh8Result bvh;
+    std::string err;
+    if (!build_bvh8_host(wtri.data(), ntri, bvh, err)) {
+        delete s;
+        return fail(REINA_RT_ERR_INVALID_ARG, err);
+    }
+    if (bvh.max_depth > kStackDepth) {
+        delete s;
+        return fail(REINA_RT_ERR_UNSUPPORTED, "BVH8 depth " + std::to_string(bvh.max_depth) +
+                                                  " exceeds the traversal stack (" + std::to_string(kStackDepth) + ")");
+    }
+    s->host_nodes = std::move(bvh.nodes);
+    s->host_tris = std::move(bvh.tris);
+    rc = scene_upload(s, d, wtri, tri_inst, tri_prim, inst);
+    if (rc) {
+        if (s->arena) hipFree(s->arena);
+        delete s;
+        return rc;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    s->info.num_triangles = ntri;
+    s->info.num_nodes = s->host_nodes.size();
+    s->info.node_bytes = sizeof(Bvh8Node);
+    s->info.device_bytes = s->arena_bytes;
+    s->info.max_depth = bvh.max_depth;
+    s->info.quantized = 0;
+    s->info.build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    s->info.sah_cost = bvh.sah_cost;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, s->device));
+    uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    s->grid_extend = cus * 4;
+    s->grid_shade = cus * 4;
+    *out = s;
+    return REINA_RT_OK;
+}
+
+static void free_trace_buffers(ReinaScene* s) {
+    if (s->tbuf) hipFree(s->tbuf);
+    s->tbuf = nullptr;
+    s->cap_px = 0;
+}
+
+int reina_rt_scene_destroy(ReinaScene* s) {
+    if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
+    hipSetDevice(s->device);
+    hipDeviceSynchronize();
+    free_trace_buffers(s);
+    if (s->ctrl) hipFree(s->ctrl);
+    if (s->h_count) hipHostFree(s->h_count);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->arena) hipFree(s->arena);
+    delete s;
+    return REINA_RT_OK;
+}
+
+int reina_rt_scene_info(const ReinaScene* s, ReinaSceneInfo* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    *o = s->info;
+    return REINA_RT_OK;
+}
+
+int reina_rt_scene_export_bvh(const ReinaScene* s, void* nodes, uint64_t* nodes_bytes, void* tris,
+                              uint64_t* tris_bytes) {
+    if (!s || !nodes_bytes || !tris_bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    uint64_t nb = s->host_nodes.size() * sizeof(Bvh8Node), tb = s->host_tris.size() * sizeof(float);
+    if (nodes) {
+        if (*nodes_bytes < nb) return fail(REINA_RT_ERR_INVALID_ARG, "nodes buffer too small");
+        std::memcpy(nodes, s->host_nodes.data(), nb);
+    }
+    if (tris) {
+        if (*tris_bytes < tb) return fail(REINA_RT_ERR_INVALID_ARG, "tris buffer too small");
+        std::memcpy(tris, s->host_tris.data(), tb);
+    }
+    *nodes_bytes = nb;
+    *tris_bytes = tb;
+    return REINA_RT_OK;
+}
+
+int reina_rt_bvh_build_host(const float* tri, uint64_t ntri, void* nodes, uint64_t* nodes_bytes, void* tris,
+                            uint64_t* tris_bytes, uint32_t* max_depth) {
+    if ((ntri && !tri) || !nodes_bytes || !tris_bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    Bvh8Result r;
+    std::string err;
+    if (!build_bvh8_host(tri, ntri, r, err)) return fail(REINA_RT_ERR_INVALID_ARG, err);
+    uint64_t nb = r.nodes.size() * sizeof(Bvh8Node), tb = r.tris.size() * sizeof(float);
+    if (nodes) {
+        if (*nodes_bytes < nb) return fail(REINA_RT_ERR_INVALID_ARG, "nodes buffer too small");
+        std::memcpy(nodes, r.nodes.data(), nb);
+    }
+    if (tris) {
+        if (*tris_bytes < tb) return fail(REINA_RT_ERR_INVALID_ARG, "tris buffer too small");
+        std::memcpy(tris, r.tris.data(), tb);
+    }
+    *nodes_bytes = nb;
+    *tris_bytes = tb;
+    if (max_depth) *max_depth = r.max_depth;
+    return REINA_RT_OK;
+}
+
+static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
+    if (!s->ctrl) {
+        HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 64));
+        s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
+        s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 16);
+        HIP_TRY(hipHostMalloc((void**)&s->h_count, 64, hipHostMallocDefault));
+        HIP_TRY(hipEventCreate(&s->ev0));
+        HIP_TRY(hipEventCreate(&s->ev1));
+    }
+    if (npx <= s->cap_px) return REINA_RT_OK;
+    free_trace_buffers(s);
+    uint64_t per = 64 /*state*/ + 64 /*2 ray queues*/ + 8 /*2 pix queues*/ + 16 /*hits*/;
+    uint64_t bytes = npx * per + 8 * 256;
+    hipError_t e = hipMalloc(&s->tbuf, bytes);
+    if (e != hipSuccess) return fail(REINA_RT_ERR_OUT_OF_MEMORY, "trace buffers: hipMalloc failed");
+    char* p = (char*)s->tbuf;
+    auto take = [&](uint64_t b) { char* r = p; p += (b + 255) & ~255ull; return r; };
+    s->S.st0 = (uint4*)take(npx * 16);
+    s->S.st1 = (float4*)take(npx * 16);
+    s->S.st2 = (float4*)take(npx * 16);
+    s->S.st3 = (float4*)take(npx * 16);
+    s->Q.ray[0] = (float4*)take(npx * 32);
+    s->Q.ray[1] = (float4*)take(npx * 32);
+    s->Q.pix[0] = (uint32_t*)take(npx * 4);
+    s->Q.pix[1] = (uint32_t*)take(npx * 4);
+    s->Q.hit = (float4*)take(npx * 16);
+    s->cap_px = npx;
+    return REINA_RT_OK;
+}
+
+int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t W, uint32_t H, void* stream) {
+    if (!s || !pc || !image) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    if (W == 0 || H == 0) return fail(REINA_RT_ERR_INVALID_ARG, "zero image size");
+    if ((uint64_t)W * H >= (1ull << 31)) return fail(REINA_RT_ERR_INVALID_ARG, "image too large");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(s->device));
+    int rc = ensure_trace_buffers(s, (uint64_t)W * H);
+    if (rc) return rc;
+    TraceParams P;
+    std::memcpy(P.cam.invView, pc->invView, 64);
+    std::memcpy(P.cam.invProj, pc->invProjection, 64);
+    P.cam.focusDist = pc->focusDist;
+    P.cam.defocus = pc->defocusMultiplier;
+    P.cam.W = (float)(int32_t)W;
+    P.cam.H = (float)(int32_t)H;
+    P.W = W;
+    P.H = H;
+    P.spp = pc->samplesPerPixel;
+    P.maxBounces = pc->maxBounces;
+    P.sampleBatch = pc->sampleBatch;
+    P.directClamp = pc->directClamp;
+    HIP_TRY(hipEventRecord(s->ev0, st));
+    HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, st));
+    uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
+    uint32_t it = 0;
+    const uint32_t K = 4;
+    while (it < max_iters) {
+        uint32_t slot = it % 3, in_q = it & 1u;
+        HIP_TRY(launch_extend(s->sv, s->Q.ray[in_q], s->Q.hit, s->ctrl, slot, s->stats, s->grid_extend, st));
+        HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->grid_shade, st));
+        ++it;
+        if (it % K == 0) {
+            HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count, 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (*s->h_count == 0) break;
+        }
+    }
+    HIP_TRY(launch_finish(P, s->S, (float4*)image, st));
+    HIP_TRY(hipEventRecord(s->ev1, st));
+    s->traced = true;
+    s->last_iters = it;
+    s->last_stream = st;
+    return REINA_RT_OK;
+}
+
+int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    std::memset(o, 0, sizeof *o);
+    if (!s->traced) return REINA_RT_OK;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    unsigned long long rays = 0;
+    HIP_TRY(hipMemcpy(&rays, s->stats, 8, hipMemcpyDeviceToHost));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    o->extension_rays = rays;
+    o->shadow_rays = 0;
+    o->iterations = s->last_iters;
+    o->trace_ms = ms;
+    o->extend_ms = 0.0f;
+    return REINA_RT_OK;
+}
+
+int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint32_t n, uint32_t* counters,
+                       void* stream) {
+    if (!s || (n && (!rays || !hits))) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(s->device));
+    int rc = ensure_trace_buffers(s, 0);
+    if (rc) return rc;
+    if (n == 0) return REINA_RT_OK;
+    HIP_TRY(hipMemsetAsync(s->fetch, 0, 4, st));
+    HIP_TRY(launch_intersect(s->sv, (const float4*)rays, (float4*)hits, n, counters, s->fetch, s->grid_extend, st));
+    return REINA_RT_OK;
+}
+
+int reina_rt_postprocess(const float* in, float* scratch, uint8_t* out, uint32_t W, uint32_t H,
+                         const BloomPushConsts* bloom, const TonemappingPushConsts* tm, void* stream) {
+    if (!in || !scratch || !out || !bloom || !tm) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    if (W == 0 || H == 0) return fail(REINA_RT_ERR_INVALID_ARG, "zero image size");
+    float4* ping = (float4*)scratch;
+    float4* pong = ping + (uint64_t)W * H;
+    HIP_TRY(launch_postprocess((const float4*)in, ping, pong, (uchar4*)out, W, H, *bloom, *tm, (hipStream_t)stream));
+    return REINA_RT_OK;
+}
+
+int reina_rt_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, void* stream) {
+    if (n && (!in || !out || (fn == 5 && !in2))) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(launch_debug_math(fn, in, in2, out, n, (hipStream_t)stream));
+    return REINA_RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,359 @@
+// bvh_host.cpp — BVH8 construction on the host CPU (replaces the driver's
+// BLAS/TLAS build, src/graphics/Blas.cpp:8-124 and src/tools/vktools.cpp:460-596,
+// for the flattened world-space triangle soup).
+//
+//   1. Binned SAH BVH2 (32 bins per axis, all three axes), leaves of at most
+//      3 triangles; nodes of one subtree are built in parallel on std::threads
+//      below the top levels.
+//   2. Collapse to BVH8: a node's children are grown from its two BVH2 children
+//      by repeatedly opening the internal child of largest surface area until
+//      there are 8 (Ylitie, Karras, Laine: "Efficient Incoherent Ray Traversal
+//      on GPUs Through Compressed Wide BVHs", HPG 2017).
+//   3. Children are placed in slots by octant: slot s stands for the direction
+//      (+-1,+-1,+-1) of its bits, and a greedy assignment gives each slot the
+//      child whose centroid lies furthest in that direction from the parent's
+//      centroid, so visiting slots i ^ oct approximates front-to-back order.
+//   4. Boxes are padded outward by one ulp; internal children are contiguous
+//      (breadth-first), and each node's leaf triangles are contiguous.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "bvh_host.h"
+
+namespace rt {
+
+namespace {
+
+struct Aabb {
+    float lo[3], hi[3];
+    void reset() {
+        lo[0] = lo[1] = lo[2] = INFINITY;
+        hi[0] = hi[1] = hi[2] = -INFINITY;
+    }
+    void grow(const Aabb& b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void grow(const float* p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0.0f)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct B2Node {
+    Aabb box;
+    int32_t left = -1, right = -1;   // children (internal)
+    uint32_t first = 0, count = 0;   // leaf range into ref[]
+};
+
+constexpr int kBins = 32;
+
+struct Builder {
+    const float* tri;          // 9 floats per triangle
+    std::vector<Aabb> tbox;
+    std::vector<float> cen;    // 3 per triangle
+    std::vector<uint32_t> ref;
+    std::vector<B2Node> nodes;
+    std::mutex* mtx = nullptr;
+
+    int32_t alloc_node() {
+        std::lock_guard<std::mutex> g(*mtx);
+        nodes.emplace_back();
+        return (int32_t)nodes.size() - 1;
+    }
+};
+
+// Build the subtree over ref[first, first+count); returns node index.
+void build_rec(Builder& b, int32_t ni, uint32_t first, uint32_t count, int depth, std::atomic<int>* threads,
+               std::vector<std::thread>* spawned, std::mutex* spawn_mtx);
+
+void split_node(Builder& b, int32_t ni, uint32_t first, uint32_t count, int depth, std::atomic<int>* threads,
+                std::vector<std::thread>* spawned, std::mutex* spawn_mtx) {
+    Aabb box, cb;
+    box.reset();
+    cb.reset();
+    for (uint32_t i = first; i < first + count; ++i) {
+        uint32_t t = b.ref[i];
+        box.grow(b.tbox[t]);
+        cb.grow(&b.cen[3ull * t]);
+    }
+    {
+        std::lock_guard<std::mutex> g(*b.mtx);
+        b.nodes[ni].box = box;
+    }
+    if (count <= 1) {
+        std::lock_guard<std::mutex> g(*b.mtx);
+        b.nodes[ni].first = first;
+        b.nodes[ni].count = count;
+        return;
+    }
+    // binned SAH over the three axes
+    float best_cost = INFINITY;
+    int best_axis = -1, best_split = -1;
+    for (int ax = 0; ax < 3; ++ax) {
+        float ext = cb.hi[ax] - cb.lo[ax];
+        if (!(ext > 0.0f)) continue;
+        Aabb bins[kBins];
+        uint32_t cnt[kBins] = {0};
+        for (int k = 0; k < kBins; ++k) bins[k].reset();
+        float sc = (float)kBins / ext;
+        for (uint32_t i = first; i < first + count; ++i) {
+            uint32_t t = b.ref[i];
+            int k = (int)((b.cen[3ull * t + ax] - cb.lo[ax]) * sc);
+            k = std::min(std::max(k, 0), kBins - 1);
+            cnt[k]++;
+            bins[k].grow(b.tbox[t]);
+        }
+        float rarea[kBins];
+        uint32_t rcnt[kBins];
+        Aabb acc;
+        acc.reset();
+        uint32_t c = 0;
+        for (int k = kBins - 1; k > 0; --k) {
+            acc.grow(bins[k]);
+            c += cnt[k];
+            rarea[k] = acc.area();
+            rcnt[k] = c;
+        }
+        acc.reset();
+        c = 0;
+        for (int k = 0; k < kBins - 1; ++k) {
+            acc.grow(bins[k]);
+            c += cnt[k];
+            if (c == 0 || rcnt[k + 1] == 0) continue;
+            float cost = acc.area() * (float)c + rarea[k + 1] * (float)rcnt[k + 1];
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_axis = ax;
+                best_split = k;
+            }
+        }
+    }
+    float parea = box.area();
+    float leaf_cost = (float)count;                                   // Ct = 1
+    float split_cost = 1.0f + (parea > 0.0f ? best_cost / parea : INFINITY);   // Ci = 1
+    if (count <= kMaxLeafTris && (best_axis < 0 || leaf_cost <= split_cost)) {
+        std::lock_guard<std::mutex> g(*b.mtx);
+        b.nodes[ni].first = first;
+        b.nodes[ni].count = count;
+        return;
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+        mid = first + count / 2;   // all centroids coincide: split by index
+    } else {
+        float ext = cb.hi[best_axis] - cb.lo[best_axis];
+        float sc = (float)kBins / ext;
+        auto it = std::partition(b.ref.begin() + first, b.ref.begin() + first + count, [&](uint32_t t) {
+            int k = (int)((b.cen[3ull * t + best_axis] - cb.lo[best_axis]) * sc);
+            k = std::min(std::max(k, 0), kBins - 1);
+            return k <= best_split;
+        });
+        mid = (uint32_t)(it - b.ref.begin());
+        if (mid == first || mid == first + count) mid = first + count / 2;
+    }
+    int32_t l = b.alloc_node(), r = b.alloc_node();
+    {
+        std::lock_guard<std::mutex> g(*b.mtx);
+        b.nodes[ni].left = l;
+        b.nodes[ni].right = r;
+    }
+    uint32_t lc = mid - first, rc = first + count - mid;
+    // hand large subtrees to another thread near the top
+    if (depth < 6 && lc > 65536 && threads->fetch_add(1) < (int)std::thread::hardware_concurrency()) {
+        std::lock_guard<std::mutex> g(*spawn_mtx);
+        spawned->emplace_back([&b, l, first, lc, depth, threads, spawned, spawn_mtx]() {
+            build_rec(b, l, first, lc, depth + 1, threads, spawned, spawn_mtx);
+        });
+    } else {
+        build_rec(b, l, first, lc, depth + 1, threads, spawned, spawn_mtx);
+    }
+    build_rec(b, r, mid, rc, depth + 1, threads, spawned, spawn_mtx);
+}
+
+void build_rec(Builder& b, int32_t ni, uint32_t first, uint32_t count, int depth, std::atomic<int>* threads,
+               std::vector<std::thread>* spawned, std::mutex* spawn_mtx) {
+    split_node(b, ni, first, count, depth, threads, spawned, spawn_mtx);
+}
+
+struct Child {
+    int32_t b2;       // BVH2 node
+    bool leaf;
+};
+
+uint8_t leaf_meta(uint32_t count, uint32_t off) { return (uint8_t)(((count & 3u) << 5) | (off & 31u)); }
+
+}  // namespace
+
+bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::string& err) {
+    out.nodes.clear();
+    out.tris.clear();
+    out.max_depth = 0;
+    if (ntri >= (1ull << 27)) {
+        err = "too many triangles for the BVH8 index encoding (max 2^27)";
+        return false;
+    }
+    if (ntri == 0) {
+        Bvh8Node root;
+        std::memset(&root, 0, sizeof root);
+        out.nodes.push_back(root);
+        out.max_depth = 1;
+        return true;
+    }
+    std::mutex mtx;
+    Builder b;
+    b.tri = tri;
+    b.mtx = &mtx;
+    b.tbox.resize(ntri);
+    b.cen.resize(3 * ntri);
+    b.ref.resize(ntri);
+    for (uint64_t t = 0; t < ntri; ++t) {
+        Aabb bx;
+        bx.reset();
+        bx.grow(&tri[9 * t]);
+        bx.grow(&tri[9 * t + 3]);
+        bx.grow(&tri[9 * t + 6]);
+        b.tbox[t] = bx;
+        for (int k = 0; k < 3; ++k) b.cen[3 * t + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
+        b.ref[t] = (uint32_t)t;
+    }
+    b.nodes.reserve(2 * ntri + 1);   // never reallocates: at most 2n-1 nodes
+    b.nodes.emplace_back();
+    std::atomic<int> threads{0};
+    std::vector<std::thread> spawned;
+    std::mutex spawn_mtx;
+    build_rec(b, 0, 0, (uint32_t)ntri, 0, &threads, &spawned, &spawn_mtx);
+    for (;;) {
+        std::thread th;
+        {
+            std::lock_guard<std::mutex> g(spawn_mtx);
+            if (spawned.empty()) break;
+            th = std::move(spawned.back());
+            spawned.pop_back();
+        }
+        th.join();
+    }
+
+    // ---- collapse to BVH8, breadth-first ---------------------------------------
+    struct Pending { int32_t b2; uint32_t node; uint32_t depth; };
+    std::vector<Pending> queue;
+    out.nodes.reserve(ntri / 2 + 8);
+    out.nodes.emplace_back();
+    queue.push_back({0, 0, 1});
+    size_t qh = 0;
+    double sah = 0.0;
+    float root_area = b.nodes[0].box.area();
+    while (qh < queue.size()) {
+        Pending pd = queue[qh++];
+        out.max_depth = std::max(out.max_depth, pd.depth);
+        const B2Node& src = b.nodes[pd.b2];
+        std::vector<Child> ch;
+        if (src.left < 0) ch.push_back({pd.b2, true});
+        else { ch.push_back({src.left, b.nodes[src.left].left < 0}); ch.push_back({src.right, b.nodes[src.right].left < 0}); }
+        while (ch.size() < 8) {
+            int best = -1;
+            float ba = -1.0f;
+            for (size_t k = 0; k < ch.size(); ++k) {
+                if (ch[k].leaf) continue;
+                float a = b.nodes[ch[k].b2].box.area();
+                if (a > ba) { ba = a; best = (int)k; }
+            }
+            if (best < 0) break;
+            const B2Node& o = b.nodes[ch[best].b2];
+            Child l{o.left, b.nodes[o.left].left < 0}, r{o.right, b.nodes[o.right].left < 0};
+            ch[best] = l;
+            ch.push_back(r);
+        }
+        // octant slot assignment
+        const Aabb& pb = src.box;
+        float pc[3] = {0.5f * (pb.lo[0] + pb.hi[0]), 0.5f * (pb.lo[1] + pb.hi[1]), 0.5f * (pb.lo[2] + pb.hi[2])};
+        int slot_of[8];
+        for (int k = 0; k < 8; ++k) slot_of[k] = -1;
+        {
+            struct Cand { float cost; int c, s; };
+            std::vector<Cand> cands;
+            for (size_t c = 0; c < ch.size(); ++c) {
+                const Aabb& cb2 = b.nodes[ch[c].b2].box;
+                float d[3];
+                for (int k = 0; k < 3; ++k) d[k] = 0.5f * (cb2.lo[k] + cb2.hi[k]) - pc[k];
+                for (int s = 0; s < 8; ++s) {
+                    float cost = ((s & 1) ? d[0] : -d[0]) + ((s & 2) ? d[1] : -d[1]) + ((s & 4) ? d[2] : -d[2]);
+                    cands.push_back({cost, (int)c, s});
+                }
+            }
+            std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& c2) { return a.cost > c2.cost; });
+            bool slot_used[8] = {false};
+            for (const Cand& cd : cands) {
+                if (slot_of[cd.c] >= 0 || slot_used[cd.s]) continue;
+                slot_of[cd.c] = cd.s;
+                slot_used[cd.s] = true;
+            }
+        }
+        int child_at[8];
+        for (int s = 0; s < 8; ++s) child_at[s] = -1;
+        for (size_t c = 0; c < ch.size(); ++c) child_at[slot_of[c]] = (int)c;
+        Bvh8Node nd;
+        std::memset(&nd, 0, sizeof nd);
+        nd.tri_base = (uint32_t)(out.tris.size() / 12);
+        uint32_t toff = 0, rank = 0;
+        uint32_t first_internal = (uint32_t)out.nodes.size();
+        uint32_t n_internal = 0;
+        for (int s = 0; s < 8; ++s)
+            if (child_at[s] >= 0 && !ch[child_at[s]].leaf) n_internal++;
+        nd.child_base = n_internal ? first_internal : 0;
+        for (int s = 0; s < 8; ++s) {
+            int c = child_at[s];
+            if (c < 0) {
+                nd.lo_x[s] = nd.lo_y[s] = nd.lo_z[s] = 0.0f;
+                nd.hi_x[s] = nd.hi_y[s] = nd.hi_z[s] = 0.0f;
+                nd.meta[s] = 0;
+                continue;
+            }
+            const B2Node& cn = b.nodes[ch[c].b2];
+            nd.lo_x[s] = std::nextafter(cn.box.lo[0], -INFINITY);
+            nd.lo_y[s] = std::nextafter(cn.box.lo[1], -INFINITY);
+            nd.lo_z[s] = std::nextafter(cn.box.lo[2], -INFINITY);
+            nd.hi_x[s] = std::nextafter(cn.box.hi[0], INFINITY);
+            nd.hi_y[s] = std::nextafter(cn.box.hi[1], INFINITY);
+            nd.hi_z[s] = std::nextafter(cn.box.hi[2], INFINITY);
+            float ca = cn.box.area();
+            if (ch[c].leaf) {
+                nd.meta[s] = leaf_meta(cn.count, toff);
+                for (uint32_t j = 0; j < cn.count; ++j) {
+                    uint32_t t = b.ref[cn.first + j];
+                    const float* v = &tri[9ull * t];
+                    float id;
+                    std::memcpy(&id, &t, 4);
+                    float rec[12] = {v[0], v[1], v[2], id, v[3], v[4], v[5], 0.0f, v[6], v[7], v[8], 0.0f};
+                    out.tris.insert(out.tris.end(), rec, rec + 12);
+                }
+                toff += cn.count;
+                if (root_area > 0.0f) sah += (double)ca / root_area * cn.count;
+            } else {
+                nd.meta[s] = (uint8_t)(0x80u | rank);
+                out.nodes.emplace_back();
+                queue.push_back({ch[c].b2, first_internal + rank, pd.depth + 1});
+                rank++;
+            }
+        }
+        if (root_area > 0.0f) sah += (double)pb.area() / root_area;
+        out.nodes[pd.node] = nd;
+    }
+    out.sah_cost = (float)sah;
+    return true;
+}
+
+}  // namespace rt

@@ -1,0 +1,22 @@
+// bvh_host.h — host BVH8 builder interface.
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rt_layout.h"
+
+namespace rt {
+
+struct Bvh8Result {
+    std::vector<Bvh8Node> nodes;
+    std::vector<float> tris;     // 12 floats per triangle (3 x float4), leaf order
+    uint32_t max_depth = 0;
+    float sah_cost = 0.0f;
+};
+
+// tri: 9 floats per triangle (world-space v0, v1, v2), flattened id = index.
+bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::string& err);
+
+}  // namespace rt

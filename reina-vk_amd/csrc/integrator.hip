@@ -1,0 +1,310 @@
+// integrator.hip — wavefront path tracer kernels.
+//
+// One vkCmdTraceRaysKHR of the reference (src/Reina.cpp:460-469) runs one
+// raygen invocation per pixel that loops over samplesPerPixel samples and
+// up to maxBounces segments each (shaders/raytrace/raytrace.rgen.glsl:97-184,
+// 250-285). Here each pixel owns a path slot with its state in HBM (SoA,
+// indexed by pixel) and the work proceeds in bounce iterations:
+//
+//   k_init    raygen of sample 0 for every pixel -> ray queue (pixel order)
+//   k_extend  closest hit for every queued ray (BVH8 traversal)
+//   k_shade   hit/miss shading + the raygen bounce logic; a path that ends
+//             finishes its sample and regenerates the pixel's next camera ray
+//             (the RNG state runs sequentially through the pixel's samples,
+//             exactly as in the reference); surviving rays are appended to the
+//             next queue with a wave64 ballot + one atomic per wave
+//   k_finish  mean over the non-NaN samples, running-mean blend, store
+//
+// Queue work is distributed dynamically: each wave claims 64 entries at a time
+// from a per-iteration counter, so long traversals do not stall a static
+// partition. Per-iteration control words are triple-buffered so that no host
+// round trip or memset sits between iterations.
+#include <hip/hip_runtime.h>
+
+#include "traverse.h"
+#include "wavefront.h"
+
+namespace rt {
+
+__device__ __forceinline__ uint32_t wave_claim(uint32_t* counter) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0);
+}
+
+__global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues Q, Ctrl* ctrl,
+                                              unsigned long long* stats) {
+    uint32_t npx = P.W * P.H;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        bool anyrays = P.spp > 0 && P.maxBounces > 0;
+        ctrl[0].count = anyrays ? npx : 0u;
+        ctrl[0].fetch_ext = 0; ctrl[0].fetch_shade = 0;
+        ctrl[1].count = 0; ctrl[1].fetch_ext = 0; ctrl[1].fetch_shade = 0;
+        ctrl[2].count = 0; ctrl[2].fetch_ext = 0; ctrl[2].fetch_shade = 0;
+        stats[0] = 0;
+    }
+    if (i >= npx) return;
+    uint32_t x = i % P.W, y = i / P.W;
+    uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
+    if (P.spp == 0 || P.maxBounces == 0) {
+        // no traced segment: every sample contributes clamp(0) = 0
+        uint32_t actual = P.spp;
+        S.st0[i] = make_uint4(rng, P.spp, 0u, 0u);
+        S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(actual));
+        return;
+    }
+    V3 o, d;
+    starting_ray(P.cam, (float)x, (float)y, rng, o, d);
+    S.st0[i] = make_uint4(rng, 0u, 0u, 0u);
+    S.st1[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);   // accumulatedDistance starts at 0 (Q8)
+    S.st2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    Q.ray[0][2 * i] = make_float4(o.x, o.y, o.z, 0.0f);
+    Q.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000.0f);
+    Q.pix[0][i] = i;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_extend(SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits,
+                                                Ctrl* ctrl, uint32_t slot, unsigned long long* stats,
+                                                uint32_t* counters) {
+    __shared__ uint2 stack[kStackDepth * 256];
+    uint32_t n = ctrl[slot].count;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && stats) atomicAdd(stats, (unsigned long long)n);
+    uint2* stk = stack + threadIdx.x;
+    for (;;) {
+        uint32_t base = wave_claim(&ctrl[slot].fetch_ext);
+        if (base >= n) break;
+        uint32_t i = base + __lane_id();
+        if (i < n) {
+            TRay r;
+            tray_init(r, rays[2 * i], rays[2 * i + 1]);
+            uint32_t nv, nt;
+            HitRec h = trace_closest<COUNT>(sv, r, stk, 256, &nv, &nt);
+            hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+            if (COUNT) { counters[2 * i] = nv; counters[2 * i + 1] = nt; }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, PathState S, Queues Q, uint32_t in_q,
+                                               Ctrl* ctrl, uint32_t slot) {
+    const uint32_t n = ctrl[slot].count;
+    const uint32_t nslot = (slot + 1) % 3, zslot = (slot + 2) % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctrl[zslot].count = 0; ctrl[zslot].fetch_ext = 0; ctrl[zslot].fetch_shade = 0;
+    }
+    const float4* rin = Q.ray[in_q];
+    const uint32_t* pin = Q.pix[in_q];
+    float4* rout = Q.ray[in_q ^ 1];
+    uint32_t* pout = Q.pix[in_q ^ 1];
+    for (;;) {
+        uint32_t base = wave_claim(&ctrl[slot].fetch_shade);
+        if (base >= n) break;
+        uint32_t i = base + __lane_id();
+        bool push = false;
+        uint32_t p = 0;
+        V3 no = v3(0.0f, 0.0f, 0.0f), nd = no;
+        if (i < n) {
+            p = pin[i];
+            float4 ra = rin[2 * i], rb = rin[2 * i + 1];
+            float4 hh = Q.hit[i];
+            uint4 s0 = S.st0[p];
+            float4 s1 = S.st1[p], s2 = S.st2[p], s3 = S.st3[p];
+            V3 O = v3(ra.x, ra.y, ra.z), D = v3(rb.x, rb.y, rb.z);
+            Payload pld;
+            pld.rng = s0.x;
+            pld.insideDielectric = s0.w != 0u;
+            pld.accumulatedDistance = s1.w;
+            pld.skip = false;
+            pld.rayHitSky = false;
+            pld.rayOrigin = O;
+            pld.rayDirection = D;
+            pld.emission = v3(0.0f, 0.0f, 0.0f);
+            pld.color = v3(0.0f, 0.0f, 0.0f);
+            uint32_t prim = __float_as_uint(hh.y);
+            if (prim == 0xFFFFFFFFu) {
+                pld.color = sky_color(D);            // raytrace.rmiss.glsl:10-19
+                pld.rayHitSky = true;
+                pld.skip = false;
+            } else {
+                const InstanceRec in = sv.inst[sv.tri_inst[prim]];
+                const ReinaInstanceProperties& pr = sv.props[in.props_id];
+                HitInfo hi = hit_info(sv, in, pr, sv.tri_prim[prim], hh.z, hh.w, D);
+                switch (in.material) {
+                    case 0: chs_lambert_metal(pld, hi, pr, D, false); break;
+                    case 1: chs_lambert_metal(pld, hi, pr, D, true); break;
+                    case 2: chs_dielectric(pld, hi, pr, O, D); break;
+                    default: chs_disney(pld, hi, pr, O, D); break;
+                }
+            }
+            // raytrace.rgen.glsl:124-180 (skipNEE = true: weightBRDF = 1, direct = 0)
+            V3 thr = v3(s1.x, s1.y, s1.z), L = v3(s2.x, s2.y, s2.z);
+            uint32_t seg = s0.z;
+            bool end = false;
+            if (!pld.skip) {
+                if (pld.rayHitSky) {
+                    L = add(L, mul(pld.color, thr));
+                    end = true;
+                } else if (!pld.insideDielectric) {
+                    V3 contrib = add(v3(0.0f, 0.0f, 0.0f), pld.emission);
+                    L = add(L, mul(contrib, thr));
+                    thr = mul(thr, pld.color);
+                }
+            }
+            seg += 1u;
+            if (seg >= P.maxBounces) end = true;
+            uint32_t sample = s0.y;
+            V3 sum = v3(s3.x, s3.y, s3.z);
+            uint32_t actual = __float_as_uint(s3.w);
+            if (end) {
+                // raytrace.rgen.glsl:265-273
+                V3 c = v3(clampf_(L.x, 0.0f, P.directClamp), clampf_(L.y, 0.0f, P.directClamp),
+                          clampf_(L.z, 0.0f, P.directClamp));
+                if (!anynan(c)) { actual += 1u; sum = add(sum, c); }
+                sample += 1u;
+                if (sample < P.spp) {
+                    uint32_t x = p % P.W, y = p / P.W;
+                    starting_ray(P.cam, (float)x, (float)y, pld.rng, no, nd);
+                    thr = v3(1.0f, 1.0f, 1.0f);
+                    L = v3(0.0f, 0.0f, 0.0f);
+                    seg = 0;
+                    pld.insideDielectric = false;   // raytrace.rgen.glsl:98
+                    push = true;
+                }
+            } else {
+                no = pld.rayOrigin;
+                nd = pld.rayDirection;
+                push = true;
+            }
+            S.st0[p] = make_uint4(pld.rng, sample, seg, pld.insideDielectric ? 1u : 0u);
+            S.st1[p] = make_float4(thr.x, thr.y, thr.z, pld.accumulatedDistance);
+            S.st2[p] = make_float4(L.x, L.y, L.z, 0.0f);
+            S.st3[p] = make_float4(sum.x, sum.y, sum.z, __uint_as_float(actual));
+        }
+        // wave64 ballot compaction into the next queue
+        uint64_t mask = __ballot(push ? 1 : 0);
+        if (mask) {
+            uint32_t cnt = (uint32_t)__popcll(mask);
+            uint32_t lane = __lane_id();
+            uint32_t before = (uint32_t)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+            uint32_t qb = 0;
+            if (lane == 0) qb = atomicAdd(&ctrl[nslot].count, cnt);
+            qb = __shfl(qb, 0);
+            if (push) {
+                uint32_t j = qb + before;
+                rout[2 * j] = make_float4(no.x, no.y, no.z, 0.0f);
+                rout[2 * j + 1] = make_float4(nd.x, nd.y, nd.z, 10000.0f);
+                pout[j] = p;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_finish(TraceParams P, PathState S, float4* img) {
+    uint32_t npx = P.W * P.H;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    float4 s3 = S.st3[i];
+    float fa = (float)__float_as_uint(s3.w);
+    V3 fin = v3(s3.x / fa, s3.y / fa, s3.z / fa);
+    if (P.sampleBatch > 0u) {                         // raytrace.rgen.glsl:277-282
+        float4 prev = img[i];
+        float b = (float)P.sampleBatch;
+        float b1 = (float)(P.sampleBatch + 1u);
+        fin = v3((prev.x * b + fin.x) / b1, (prev.y * b + fin.y) / b1, (prev.z * b + fin.z) / b1);
+    }
+    img[i] = make_float4(fin.x, fin.y, fin.z, 1.0f);
+}
+
+// Stand-alone closest-hit query over a ReinaRay batch.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_intersect(SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits,
+                                                   uint32_t n, uint32_t* counters, uint32_t* fetch) {
+    __shared__ uint2 stack[kStackDepth * 256];
+    uint2* stk = stack + threadIdx.x;
+    for (;;) {
+        uint32_t base = wave_claim(fetch);
+        if (base >= n) break;
+        uint32_t i = base + __lane_id();
+        if (i < n) {
+            TRay r;
+            tray_init(r, rays[2 * i], rays[2 * i + 1]);
+            uint32_t nv, nt;
+            HitRec h = trace_closest<COUNT>(sv, r, stk, 256, &nv, &nt);
+            hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+            if (COUNT) { counters[2 * i] = nv; counters[2 * i + 1] = nt; }
+        }
+    }
+}
+
+__global__ void k_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = in[i], r;
+    float s, c;
+    switch (fn) {
+        case 0: sincos_(x, &s, &c); r = s; break;
+        case 1: sincos_(x, &s, &c); r = c; break;
+        case 2: r = logf_(x); break;
+        case 3: r = expf_(x); break;
+        case 4: r = sqrtf_(x); break;
+        case 5: r = powf_(x, in2[i]); break;
+        case 6: r = exp2f_(x); break;
+        default: r = __builtin_nanf("");
+    }
+    out[i] = r;
+}
+
+}  // namespace rt
+
+// ---- launch wrappers (host) ------------------------------------------------------
+namespace rt {
+
+hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
+                       unsigned long long* stats, hipStream_t st) {
+    uint32_t npx = P.W * P.H;
+    uint32_t blocks = (npx + 255) / 256;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, st, P, S, Q, ctrl, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
+                         unsigned long long* stats, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats,
+                       (uint32_t*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
+                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, sv, P, S, Q, in_q, ctrl, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st) {
+    uint32_t npx = P.W * P.H;
+    uint32_t blocks = (npx + 255) / 256;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(256), 0, st, P, S, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_intersect(const SceneView& sv, const float4* rays, float4* hits, uint32_t n, uint32_t* counters,
+                            uint32_t* fetch, uint32_t grid, hipStream_t st) {
+    if (counters)
+        hipLaunchKernelGGL(k_intersect<true>, dim3(grid), dim3(256), 0, st, sv, rays, hits, n, counters, fetch);
+    else
+        hipLaunchKernelGGL(k_intersect<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, n, counters, fetch);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, st, fn, in, in2, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace rt

@@ -1,0 +1,606 @@
+// shade.h — device restatement of Reina's closest-hit / miss shaders and the
+// raygen camera model (shaders/raytrace/*.glsl), one function per shader
+// stage, operating on a per-path payload held in registers.
+#pragma once
+
+#include "rt_layout.h"
+#include "rt_math.h"
+
+namespace rt {
+
+constexpr float kPi = 3.14159265f;      // shaderCommon.h.glsl:47
+constexpr float kInvPi = 0.31830989f;   // shaderCommon.h.glsl:48
+
+// shaderCommon.h.glsl:39-45 (PCG-RXS-M-XS variant; 4294967295.0f rounds to 2^32)
+RT_D float rnd(uint32_t& s) {
+    s = s * 747796405u + 1u;
+    uint32_t w = ((s >> ((s >> 28) + 4u)) ^ s) * 277803737u;
+    w = (w >> 22) ^ w;
+    return (float)w / 4294967295.0f;
+}
+
+// The part of HitPayload (shaderCommon.h.glsl:18-36) that the active path reads.
+struct Payload {
+    V3 color, rayOrigin, rayDirection, emission;
+    uint32_t rng;
+    float accumulatedDistance;
+    bool rayHitSky, skip, insideDielectric;
+};
+
+struct HitInfo {
+    V3 worldPosition, worldNormal, worldNormalGeometry;
+    float uvx, uvy;
+    bool frontFace;
+    M3 tbn;
+};
+
+RT_D V3 ldv3(const float* p) { return v3(p[0], p[1], p[2]); }
+
+RT_D V3 tbn_col(const SceneView& sv, uint32_t pos, int col) {
+    if (pos >= sv.num_tbn_indices) return v3(0.0f, 0.0f, 0.0f);
+    uint32_t ti = sv.tbn_indices[pos];
+    if ((uint64_t)ti >= sv.num_tbns) return v3(0.0f, 0.0f, 0.0f);
+    return ldv3(sv.tbns + 9ull * ti + 3 * col);
+}
+
+// closestHitCommon.h.glsl:52-148
+RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaInstanceProperties& p,
+                      uint32_t prim, float bu, float bv, V3 D) {
+    HitInfo r;
+    uint32_t io = p.indicesOffset;
+    uint32_t i0 = sv.indices[3u * prim + io + 0];
+    uint32_t i1 = sv.indices[3u * prim + io + 1];
+    uint32_t i2 = sv.indices[3u * prim + io + 2];
+    float4 a = sv.vertices[i0], b = sv.vertices[i1], c = sv.vertices[i2];
+    V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+    float bx = (1.0f - bu) - bv, by = bu, bz = bv;
+    V3 op = add(add(scale(v0, bx), scale(v1, by)), scale(v2, bz));
+    V3 c0 = ldv3(in.c0), c1 = ldv3(in.c1), c2 = ldv3(in.c2), c3 = ldv3(in.c3);
+    r.worldPosition = add(add(add(scale(c0, op.x), scale(c1, op.y)), scale(c2, op.z)), c3);
+    V3 ong = normalize(cross(sub(v1, v0), sub(v2, v0)));
+    V3 on;
+    uint32_t to = p.tbnsIndicesOffset;
+    if (!p.interpNormals) {
+        on = ong;
+    } else {
+        V3 n0 = normalize(tbn_col(sv, 3u * prim + to + 0, 2));
+        V3 n1 = normalize(tbn_col(sv, 3u * prim + to + 1, 2));
+        V3 n2 = normalize(tbn_col(sv, 3u * prim + to + 2, 2));
+        on = normalize(add(add(scale(n0, bx), scale(n1, by)), scale(n2, bz)));
+    }
+    if (p.texIndicesOffset == 0xFFFFFFFFu) {
+        r.uvx = 0.0f; r.uvy = 0.0f;
+    } else {
+        uint32_t tio = p.texIndicesOffset;
+        float tu[3], tv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint32_t ti = sv.tex_indices[3u * prim + tio + k];
+            bool ok = (uint64_t)ti < sv.num_tex_coords;   // 0xFFFFFFFF entries read as (0, 0)
+            tu[k] = ok ? sv.tex_coords[2ull * ti] : 0.0f;
+            tv[k] = ok ? sv.tex_coords[2ull * ti + 1] : 0.0f;
+        }
+        r.uvx = (tu[0] * bx + tu[1] * by) + tu[2] * bz;
+        r.uvy = (tv[0] * bx + tv[1] * by) + tv[2] * bz;
+    }
+    M3 M;
+    M.c0 = c0; M.c1 = c1; M.c2 = c2;
+    r.worldNormal = normalize(mmul(M, on));
+    r.worldNormalGeometry = normalize(mmul(M, ong));
+    r.frontFace = dot(D, r.worldNormalGeometry) < 0.0f;
+    r.worldNormal = faceforward(r.worldNormal, D, r.worldNormalGeometry);
+    r.worldNormalGeometry = faceforward(r.worldNormalGeometry, D, r.worldNormalGeometry);
+    // TBN, read even when interpNormals is false (closestHitCommon.h.glsl:120-145)
+    V3 T = v3(0.0f, 0.0f, 0.0f), B = T, N = T;
+    const float w3[3] = {bx, by, bz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        V3 tk = tbn_col(sv, 3u * prim + to + k, 0);
+        V3 bk = tbn_col(sv, 3u * prim + to + k, 1);
+        V3 nk = tbn_col(sv, 3u * prim + to + k, 2);
+        if (k == 0) { T = scale(tk, w3[0]); B = scale(bk, w3[0]); N = scale(nk, w3[0]); }
+        else { T = add(T, scale(tk, w3[k])); B = add(B, scale(bk, w3[k])); N = add(N, scale(nk, w3[k])); }
+    }
+    V3 tangent = normalize(T), bitangent = normalize(B), normal = normalize(N);
+    float det = dot(M.c0, cross(M.c1, M.c2));
+    float invdet = 1.0f / det;
+    M3 Ni;
+    Ni.c0 = scale(cross(M.c1, M.c2), invdet);
+    Ni.c1 = scale(cross(M.c2, M.c0), invdet);
+    Ni.c2 = scale(cross(M.c0, M.c1), invdet);
+    V3 wT = normalize(mmul(M, tangent));
+    V3 wB = normalize(mmul(M, bitangent));
+    V3 wN = normalize(mmul(Ni, normal));
+    wT = normalize(sub(wT, scale(wN, dot(wN, wT))));
+    wB = normalize(sub(wB, scale(wN, dot(wN, wB))));
+    wB = scale(wB, -1.0f);
+    wN = scale(wN, r.frontFace ? 1.0f : -1.0f);
+    r.tbn.c0 = wT; r.tbn.c1 = wB; r.tbn.c2 = wN;
+    return r;
+}
+
+// closestHitCommon.h.glsl:156-177: integer ULP offset, float fallback near 0.
+RT_D float offc(float p, float n) {
+    int32_t of = f2i_(256.0f * n);
+    uint32_t nb = __float_as_uint(p) + (uint32_t)((p < 0.0f) ? -of : of);
+    float pi = __uint_as_float(nb);
+    return (fabsf(p) < (1.0f / 32.0f)) ? p + (1.0f / 65536.0f) * n : pi;
+}
+RT_D V3 offset_along(V3 p, V3 n) { return v3(offc(p.x, n.x), offc(p.y, n.y), offc(p.z, n.z)); }
+// dielectric.rchit.glsl:14-38 / disney.rchit.glsl:10-34
+RT_D V3 offset_dielectric(V3 p, V3 n, V3 dir) { return offset_along(p, (dot(n, dir) < 0.0f) ? neg(n) : n); }
+
+// closestHitCommon.h.glsl:195-205
+RT_D V3 random_unit_vec(uint32_t& rng) {
+    for (;;) {
+        float a = rnd(rng);
+        float b = rnd(rng);
+        float c = rnd(rng);
+        V3 v = sub(scale(v3(a, b, c), 2.0f), v3(1.0f, 1.0f, 1.0f));
+        float l2 = dot(v, v);
+        if (0.0001f < l2 && l2 < 1.0f) return normalize(v);
+    }
+}
+// closestHitCommon.h.glsl:207-222
+RT_D V3 fuzzy_reflection(V3 rayIn, V3 n, float fuzz, uint32_t& rng) {
+    return add(reflect(normalize(rayIn), normalize(n)), scale(random_unit_vec(rng), fuzz));
+}
+RT_D V3 diffuse_reflection(V3 n, uint32_t& rng) {
+    float theta = (2.0f * kPi) * rnd(rng);
+    float u = 2.0f * rnd(rng) - 1.0f;
+    float r = sqrtf_(1.0f - u * u);
+    float s, c;
+    sincos_(theta, &s, &c);
+    return normalize(add(n, v3(r * c, r * s, u)));
+}
+
+// raytrace.rmiss.glsl:10-19
+RT_D V3 sky_color(V3 dir) {
+    float t = 0.5f * (normalize(dir).y + 1.0f);
+    return scale(mix(v3(0.1f, 0.1f, 0.1f), v3(0.4f, 1.7f, 2.0f), t), 0.07f);
+}
+
+// ---- Disney BSDF (brdfDisney.h.glsl) ----------------------------------------
+RT_D V3 sample_ggx_vndf(V3 V, float ax, float ay, uint32_t& rng) {         // :9-40
+    bool flip = V.z < 0.0f;
+    if (flip) V.z = V.z * -1.0f;
+    float r1 = rnd(rng);
+    float r2 = rnd(rng);
+    V3 Vh = normalize(v3(ax * V.x, ay * V.y, V.z));
+    float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    V3 T1 = lensq > 0.0f ? scale(v3(-Vh.y, Vh.x, 0.0f), 1.0f / sqrtf_(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    V3 T2 = cross(Vh, T1);
+    float r = sqrtf_(r1);
+    float phi = (2.0f * kPi) * r2;
+    float sp, cp;
+    sincos_(phi, &sp, &cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float s = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - s) * sqrtf_(1.0f - t1 * t1) + s * t2;
+    V3 Nh = add(add(scale(T1, t1), scale(T2, t2)), scale(Vh, sqrtf_(maxf_(0.0f, (1.0f - t1 * t1) - t2 * t2))));
+    if (flip) Nh.z = Nh.z * -1.0f;
+    return normalize(v3(ax * Nh.x, ay * Nh.y, maxf_(0.0f, Nh.z)));
+}
+
+RT_D float d_ggx_aniso(V3 m, float ax, float ay) {                          // :42-48
+    float NoM = maxf_(m.z, 0.0f);
+    float inv = 1.0f / (((m.x / ax) * (m.x / ax) + (m.y / ay) * (m.y / ay)) + NoM * NoM);
+    return (inv * inv) / ((kPi * ax) * ay);
+}
+
+RT_D float pdf_ggx_reflection(V3 i, V3 o, float ax, float ay) {            // :55-73
+    V3 m = normalize(add(i, o));
+    float ndf = d_ggx_aniso(m, ax, ay);
+    float aix = ax * i.x, aiy = ay * i.y;
+    float len2 = aix * aix + aiy * aiy;
+    float t = sqrtf_(len2 + i.z * i.z);
+    if (i.z >= 0.0f) {
+        float a = clampf_(minf_(ax, ay), 0.0f, 1.0f);
+        float s = 1.0f + sqrtf_(i.x * i.x + i.y * i.y);
+        float a2 = a * a, s2 = s * s;
+        float k = ((1.0f - a2) * s2) / (s2 + (a2 * i.z) * i.z);
+        return ndf / (2.0f * (k * i.z + t));
+    }
+    return (ndf * (t - i.z)) / (2.0f * len2);
+}
+
+RT_D V3 sample_gtr1(float alpha, uint32_t& rng) {                           // :96-111
+    float r1 = rnd(rng);
+    float r2 = rnd(rng);
+    float a = maxf_(0.001f, alpha);
+    float c = sqrtf_((1.0f - expf_(r1 * logf_(a * a))) / (1.0f - a * a));
+    float sE = (c > 1.0f || c < -1.0f || c != c) ? __builtin_nanf("") : sqrtf_(1.0f - c * c);
+    float hAz = (2.0f * kPi) * r2;
+    float s, co;
+    sincos_(hAz, &s, &co);
+    return v3(sE * co, sE * s, c);
+}
+
+RT_D float fd_term(float fd90, V3 n, V3 w) { return 1.0f + (fd90 - 1.0f) * pow5_(1.0f - maxf_(dot(n, w), 0.0f)); }
+
+RT_D V3 eval_diffuse(float roughness, float subsurface, V3 bc, V3 n, V3 wi, V3 wo, V3 h) {   // :124-168
+    float hw = maxf_(dot(h, wo), 0.0f);
+    float fd90 = 0.5f + ((2.0f * roughness) * hw) * hw;
+    float FDin = fd_term(fd90, n, wi), FDout = fd_term(fd90, n, wo);
+    V3 base = scale(scale(v3(bc.x / kPi, bc.y / kPi, bc.z / kPi), FDin), FDout);
+    V3 k = scale(scale(bc, 1.25f), kInvPi);
+    float fss90 = (roughness * hw) * hw;
+    float fssIn = 1.0f + (fss90 - 1.0f) * pow5_(1.0f - maxf_(dot(n, wi), 0.0f));
+    float fssOut = 1.0f + (fss90 - 1.0f) * pow5_(1.0f - maxf_(dot(n, wo), 0.0f));
+    float third = 1.0f / (maxf_(dot(n, wi), 0.0f) + maxf_(dot(n, wo), 0.0f)) - 0.5f;
+    V3 subs = scale(k, (fssIn * fssOut) * third + 0.5f);
+    return mix(base, subs, subsurface);
+}
+
+RT_D float pdf_diffuse(V3 wo, V3 n) {                                       // :170-179
+    float c = dot(n, wo);
+    if (c <= 0.0f) return 0.0f;
+    return c * kInvPi;
+}
+
+RT_D V3 sample_diffuse(V3 n, uint32_t& rng) {                               // :181-201
+    float xi1 = rnd(rng);
+    float xi2 = rnd(rng);
+    float r = sqrtf_(xi1);
+    float phi = (2.0f * kPi) * xi2;
+    float s, c;
+    sincos_(phi, &s, &c);
+    float x = r * c, y = r * s;
+    float z = sqrtf_(maxf_(0.0f, 1.0f - xi1));
+    V3 t = fabsf(n.x) < 0.5f ? normalize(cross(n, v3(1.0f, 0.0f, 0.0f))) : normalize(cross(n, v3(0.0f, 1.0f, 0.0f)));
+    V3 b = cross(n, t);
+    return normalize(add(add(scale(t, x), scale(b, y)), scale(n, z)));
+}
+
+RT_D float eval_r0(float ior) { return ((ior - 1.0f) * (ior - 1.0f)) / ((ior + 1.0f) * (ior + 1.0f)); }
+RT_D float luminance(V3 c) { return dot(c, v3(0.2126f, 0.7152f, 0.0722f)); }
+
+RT_D V3 eval_fm(V3 bc, V3 h, V3 wo, float specular, V3 specularTint, float metallic, float eta) {   // :215-222
+    float lum = luminance(bc);
+    V3 ctint = lum > 0.0f ? divs(bc, lum) : v3(1.0f, 1.0f, 1.0f);
+    V3 ks = add(sub(v3(1.0f, 1.0f, 1.0f), ctint), mul(specularTint, ctint));
+    V3 c0 = add(scale(ks, (specular * eval_r0(eta)) * (1.0f - metallic)), scale(bc, metallic));
+    float p = pow5_(1.0f - fabsf(dot(h, wo)));
+    return add(c0, scale(sub(v3(1.0f, 1.0f, 1.0f), c0), p));
+}
+
+RT_D float eval_dm(V3 hl, float ax, float ay) {                             // :224-229
+    float constant = (kPi * ax) * ay;
+    float q = (pow2_(hl.x) / pow2_(ax) + pow2_(hl.y) / pow2_(ay)) + pow2_(hl.z);
+    return 1.0f / (constant * pow2_(q));
+}
+
+RT_D float smith_g(V3 wl, float ax, float ay) {                             // :231-239
+    float st = sqrtf_(1.0f + (pow2_(wl.x * ax) + pow2_(wl.y * ay)) / pow2_(wl.z));
+    return 1.0f / (1.0f + (st - 1.0f) / 2.0f);
+}
+
+RT_D void alphas(float anisotropic, float roughness, float& ax, float& ay) {
+    float aspect = sqrtf_(1.0f - 0.9f * anisotropic);
+    ax = maxf_(0.0001f, (roughness * roughness) / aspect);
+    ay = maxf_(0.0001f, (roughness * roughness) * aspect);
+}
+
+RT_D V3 eval_metal(const M3& tbn, V3 bc, float anisotropic, float roughness, V3 n, V3 wi, V3 wo, V3 h,
+                   float specular, V3 specularTint, float metallic, float eta) {   // :247-267
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    V3 fm = eval_fm(bc, h, wo, specular, specularTint, metallic, eta);
+    V3 wiT = normalize(tmul(tbn, wi)), woT = normalize(tmul(tbn, wo)), hT = normalize(tmul(tbn, h));
+    float dm = eval_dm(hT, ax, ay);
+    float gm = smith_g(wiT, ax, ay) * smith_g(woT, ax, ay);
+    float den = (4.0f * fabsf(dot(n, wi))) * fabsf(dot(n, wo));
+    return v3(((fm.x * dm) * gm) / den, ((fm.y * dm) * gm) / den, ((fm.z * dm) * gm) / den);
+}
+
+RT_D V3 sample_metal(const M3& tbn, float anisotropic, float roughness, V3 wi, uint32_t& rng) {   // :269-285
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    V3 h = sample_ggx_vndf(tmul(tbn, wi), ax, ay, rng);
+    h = normalize(mmul(tbn, h));
+    return reflect(neg(wi), h);
+}
+
+RT_D float pdf_metal(const M3& tbn, V3 wi, V3 wo, float anisotropic, float roughness) {           // :287-299
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    return pdf_ggx_reflection(tmul(tbn, wi), tmul(tbn, wo), ax, ay);
+}
+
+RT_D float g1c(V3 w, float a) {                                             // :313-319
+    float a2 = a * a;
+    return 2.0f / (1.0f + sqrtf_(a2 + ((1.0f - a2) * w.z) * w.z));
+}
+
+RT_D float eval_dc(float ag, V3 hl) {                                       // :328-333
+    float num = ag * ag - 1.0f;
+    float den = (kPi * logf_(ag * ag)) * (1.0f + (ag * ag - 1.0f) * (hl.z * hl.z));
+    return num / den;
+}
+
+RT_D V3 eval_clearcoat(const M3& tbn, V3 wi, V3 wo, float cg, V3 h) {      // :335-354
+    float ag = (1.0f - cg) * 0.1f + cg * 0.001f;
+    V3 hT = tmul(tbn, h), wiT = tmul(tbn, wi), woT = tmul(tbn, wo);
+    float r0 = eval_r0(1.5f);
+    float fc = r0 + (1.0f - r0) * pow5_(1.0f - dot(h, wo));
+    float gc = g1c(wiT, 0.25f) * g1c(woT, 0.25f);
+    float v = ((0.25f * fc) * gc) * eval_dc(ag, hT);
+    return v3(v, v, v);
+}
+
+RT_D V3 sample_clearcoat(const M3& tbn, float cg, V3 wi, uint32_t& rng) {  // :356-366
+    float ag = (1.0f - cg) * 0.1f + cg * 0.001f;
+    V3 h = normalize(sample_gtr1(ag, rng));
+    h = normalize(mmul(tbn, h));
+    return normalize(reflect(neg(wi), h));
+}
+
+RT_D float pdf_clearcoat(const M3& tbn, V3 wi, V3 wo, V3 h, float cg) {    // :368-383
+    float ag = (1.0f - cg) * 0.1f + cg * 0.001f;
+    V3 wiT = tmul(tbn, wi), woT = tmul(tbn, wo), hT = tmul(tbn, h);
+    if (wiT.z <= 0.0f || woT.z <= 0.0f) return 0.0f;
+    float dc = eval_dc(ag, hT) * hT.z;
+    return dc / (4.0f * fabsf(dot(woT, hT)));
+}
+
+// brdfDisney.h.glsl:389-394 and dielectric.rchit.glsl:7-12 (Schlick)
+RT_D float reflectance(float cosine, float ri) {
+    float r0 = (1.0f - ri) / (1.0f + ri);
+    r0 = r0 * r0;
+    return r0 + (1.0f - r0) * pow5_(1.0f - cosine);
+}
+
+RT_D V3 sample_glass(const M3& tbn, V3 wi, float roughness, float anisotropic, float eta, uint32_t& rng,
+                     bool& refracted) {                                     // :396-431
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    V3 wiT = tmul(tbn, wi);
+    V3 hT = sample_ggx_vndf(wiT, ax, ay, rng);
+    float cosTheta = dot(wiT, hT);
+    V3 hW = normalize(mmul(tbn, hT));
+    float refl = reflectance(cosTheta, eta);
+    float sinTheta = sqrtf_(1.0f - cosTheta * cosTheta);
+    bool cannot = eta * sinTheta > 1.0f;
+    if (cannot || refl > rnd(rng)) {
+        refracted = false;
+        return reflect(neg(wi), hW);
+    }
+    refracted = true;
+    return refract(neg(wi), hW, eta);
+}
+
+RT_D float smith_g_aniso(float NdV, float VdX, float VdY, float ax, float ay) {   // :433-439
+    float a = VdX * ax, b = VdY * ay, c = NdV;
+    return (2.0f * NdV) / (NdV + sqrtf_((a * a + b * b) + c * c));
+}
+
+RT_D V3 eval_microfacet_refraction(V3 bc, float anisotropic, float roughness, float eta, V3 V, V3 L, V3 H,
+                                   float& pdf) {                            // :441-467
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    pdf = 0.0f;
+    if (L.z >= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+    float LdH = dot(L, H), VdH = dot(V, H);
+    float D = eval_dm(H, ax, ay);
+    float G1 = smith_g_aniso(fabsf(V.z), V.x, V.y, ax, ay);
+    float G2 = G1 * smith_g_aniso(fabsf(L.z), L.x, L.y, ax, ay);
+    float denom = LdH + VdH * eta;
+    denom = denom * denom;
+    float jac = (fabsf(LdH) * (eta * eta)) / denom;
+    pdf = (((G1 * maxf_(0.0f, VdH)) * D) * jac) / V.z;
+    float F = reflectance(dot(V, H), eta);
+    V3 r = v3(powf_(bc.x, 0.5f), powf_(bc.y, 0.5f), powf_(bc.z, 0.5f));
+    r = scale(r, 1.0f - F);
+    r = scale(r, D);
+    r = scale(r, G2);
+    r = scale(r, fabsf(VdH));
+    r = scale(r, jac);
+    return divs(r, fabsf(L.z * V.z));
+}
+
+RT_D V3 eval_glass(const M3& tbn, V3 bc, float anisotropic, float roughness, float eta, V3 n, V3 wi, V3 wo,
+                   bool didRefract, float& pdf) {                           // :539-581
+    V3 h = didRefract ? normalize(add(wo, scale(wi, eta))) : normalize(add(wo, wi));
+    if (dot(h, n) < 0.0f) h = neg(h);
+    if (didRefract) {
+        float tp;
+        V3 f = eval_microfacet_refraction(bc, anisotropic, roughness, eta, tmul(tbn, wi), tmul(tbn, wo),
+                                          tmul(tbn, h), tp);
+        pdf = tp;
+        return f;
+    }
+    pdf = pdf_metal(tbn, wi, wo, anisotropic, roughness);
+    return eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, 0.0f, v3(1.0f, 1.0f, 1.0f), 1.0f, eta);
+}
+
+RT_D V3 eval_sheen(V3 bc, V3 wo, V3 h, V3 n, V3 sheenTint) {               // :587-594
+    float lum = luminance(bc);
+    V3 ctint = lum > 0.0f ? divs(bc, lum) : v3(1.0f, 1.0f, 1.0f);
+    V3 cs = v3(mixf(1.0f, ctint.x, sheenTint.x), mixf(1.0f, ctint.y, sheenTint.y), mixf(1.0f, ctint.z, sheenTint.z));
+    float p = pow5_(1.0f - maxf_(dot(h, wo), 0.0f));
+    return scale(scale(cs, p), maxf_(dot(n, wo), 0.0f));
+}
+
+RT_D V3 sample_disney(const M3& tbn, float anisotropic, float roughness, float cg, float eta, float metallic,
+                      float clearcoat, float st, V3 n, V3 wi, bool& didRefract, bool& choseGlass,
+                      uint32_t& rng) {                                      // :609-654
+    float dW = (1.0f - st) * (1.0f - metallic);
+    float mW = metallic;
+    float cW = 0.25f * clearcoat;
+    float gW = (1.0f - metallic) * st;
+    float cdf0 = dW, cdf1 = cdf0 + mW, cdf2 = cdf1 + cW, cdf3 = cdf2 + gW;
+    didRefract = false;
+    float r = rnd(rng) * cdf3;
+    choseGlass = false;
+    if (r < cdf0) return sample_diffuse(n, rng);
+    if (r < cdf1) return sample_metal(tbn, anisotropic, roughness, wi, rng);
+    if (r < cdf2) return sample_clearcoat(tbn, cg, wi, rng);
+    choseGlass = true;
+    return sample_glass(tbn, wi, roughness, anisotropic, eta, rng, didRefract);
+}
+
+RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float anisotropic, float roughness,
+                    float subsurface, float cg, float eta, float metallic, float clearcoat, float st, float sheen,
+                    bool didRefract, V3 n, V3 wi, V3 wo, V3 h, float& pdf) { // :656-711
+    float dW = (1.0f - st) * (1.0f - metallic);
+    float mW = metallic;
+    float cW = 0.25f * clearcoat;
+    float gW = (1.0f - metallic) * st;
+    float ws = (dW + mW) + gW;
+    V3 fd = eval_diffuse(roughness, subsurface, bc, n, wi, wo, h);
+    float dP = pdf_diffuse(wo, n);
+    V3 fs = eval_sheen(bc, wo, h, n, sheenTint);
+    V3 fm = eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, st, specularTint, metallic, eta);
+    float mP = pdf_metal(tbn, wi, wo, anisotropic, roughness);
+    V3 fc = eval_clearcoat(tbn, wi, wo, cg, h);
+    float cP = pdf_clearcoat(tbn, wi, wo, h, cg);
+    float gP;
+    V3 fg = eval_glass(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, gP);
+    pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cW) + (gP * gW) / ws;
+    V3 r = scale(add(fd, scale(fs, sheen)), dW / ws);
+    r = add(r, scale(fm, mW / ws));
+    r = add(r, scale(fc, cW));
+    r = add(r, scale(fg, gW / ws));
+    return r;
+}
+
+// ---- hit shaders ---------------------------------------------------------------
+RT_D void skip_hit(Payload& pld, const HitInfo& hi, V3 D) {                // closestHitCommon.h.glsl:185-193
+    pld.rayOrigin = offset_along(hi.worldPosition, neg(hi.worldNormal));
+    pld.rayDirection = D;
+    pld.rayHitSky = false;
+    pld.skip = true;
+}
+
+// lambertian.rchit.glsl:11-78 (metal = false) and metal.rchit.glsl:7-70 (metal = true)
+RT_D void chs_lambert_metal(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 D, bool metal) {
+    if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
+    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);
+    if (!metal && (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f)) { skip_hit(pld, hi, D); return; }
+    V3 wn = hi.worldNormal;
+    pld.color = ldv3(p.albedo);
+    pld.emission = ldv3(p.emission);
+    pld.rayOrigin = offset_along(hi.worldPosition, hi.worldNormalGeometry);
+    if (!metal) pld.rayDirection = diffuse_reflection(wn, pld.rng);
+    else pld.rayDirection = fuzzy_reflection(D, wn, p.roughness, pld.rng);
+    pld.rayHitSky = false;
+    pld.skip = false;
+    pld.insideDielectric = false;
+    pld.accumulatedDistance = 0.0f;
+}
+
+// dielectric.rchit.glsl:40-113
+RT_D void chs_dielectric(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O, V3 D) {
+    float ri = hi.frontFace ? 1.0f / p.ior : p.ior;
+    V3 unitDir = normalize(D);
+    float cosTheta = minf_(dot(neg(unitDir), hi.worldNormal), 1.0f);
+    float sinTheta = sqrtf_(1.0f - cosTheta * cosTheta);
+    bool cannot = ri * sinTheta > 1.0f;
+    float refl = reflectance(cosTheta, ri);
+    bool prevInside = pld.insideDielectric;
+    V3 albedo = ldv3(p.albedo);
+    V3 wn = hi.worldNormal;
+    if (cannot || refl > rnd(pld.rng)) {
+        pld.rayDirection = fuzzy_reflection(unitDir, wn, p.roughness, pld.rng);
+        pld.color = v3(1.0f, 1.0f, 1.0f);
+        pld.rayOrigin = offset_along(hi.worldPosition, hi.worldNormalGeometry);
+    } else {
+        pld.insideDielectric = hi.frontFace;
+        pld.rayDirection = add(refract(unitDir, wn, ri), scale(random_unit_vec(pld.rng), p.roughness));
+        pld.color = albedo;
+        pld.rayOrigin = offset_dielectric(hi.worldPosition, hi.worldNormalGeometry, unitDir);
+    }
+    pld.accumulatedDistance = pld.accumulatedDistance + (prevInside ? length(sub(hi.worldPosition, O)) : 0.0f);
+    if (prevInside && !pld.insideDielectric) {
+        float e = expf_(-p.absorption * pld.accumulatedDistance);
+        pld.color = mul(v3(e, e, e), albedo);
+        pld.accumulatedDistance = 0.0f;
+    }
+    pld.emission = ldv3(p.emission);
+    pld.rayHitSky = false;
+    pld.skip = false;
+}
+
+// disney.rchit.glsl:36-198
+RT_D void chs_disney(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O, V3 D) {
+    if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
+    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);
+    if (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f) { skip_hit(pld, hi, D); return; }
+    V3 wn = hi.worldNormal;
+    V3 albedo = ldv3(p.albedo);
+    float eta = hi.frontFace ? 1.0f / p.ior : p.ior;
+    bool didRefract = false, choseGlass = false;
+    V3 wi = neg(D);
+    V3 rayDir = sample_disney(hi.tbn, p.anisotropic, p.roughness, p.clearcoatGloss, eta, p.metallic, p.clearcoat,
+                              p.specularTransmission, wn, wi, didRefract, choseGlass, pld.rng);
+    V3 h = normalize(add(rayDir, wi));
+    float pdf;
+    V3 f = eval_disney(hi.tbn, albedo, ldv3(p.specularTint), ldv3(p.sheenTint), p.anisotropic, p.roughness,
+                       p.subsurface, p.clearcoatGloss, eta, p.metallic, p.clearcoat, p.specularTransmission, p.sheen,
+                       didRefract, wn, wi, rayDir, h, pdf);
+    float cosThetaI = maxf_(dot(wn, rayDir), 0.0f);
+    pld.color = divs(scale(f, cosThetaI), pdf);
+    pld.emission = ldv3(p.emission);
+    pld.rayOrigin = offset_dielectric(hi.worldPosition, hi.worldNormalGeometry, rayDir);
+    pld.rayDirection = rayDir;
+    pld.rayHitSky = false;
+    pld.skip = false;
+    pld.insideDielectric = choseGlass;
+    if (pld.insideDielectric)
+        pld.accumulatedDistance = pld.accumulatedDistance + length(sub(hi.worldPosition, O));
+    else
+        pld.accumulatedDistance = 0.0f;
+}
+
+// ---- raygen camera (raytrace.rgen.glsl:34-41, 194-247) ------------------------
+struct Camera {
+    float invView[16];
+    float invProj[16];
+    float focusDist, defocus;
+    float W, H;
+};
+
+RT_D void m4mul(const float* m, float x, float y, float z, float w, float* o) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * w;
+}
+
+RT_D void starting_ray(const Camera& cam, float px, float py, uint32_t& rng, V3& o, V3& d) {
+    float u1 = maxf_(1e-5f, rnd(rng));
+    float u2 = rnd(rng);
+    float r = sqrtf_(-2.0f * logf_(u1));
+    float theta = (2.0f * kPi) * u2;
+    float s, c;
+    sincos_(theta, &s, &c);
+    float gx = r * c, gy = r * s;
+    float cx = (px + 0.5f) + 0.375f * gx;
+    float cy = (py + 0.5f) + 0.375f * gy;
+    float ndcx = (cx / cam.W) * 2.0f - 1.0f;
+    float ndcy = -((cy / cam.H) * 2.0f - 1.0f);
+    float vp[4];
+    m4mul(cam.invProj, ndcx, ndcy, -1.0f, 1.0f, vp);
+    float w = vp[3];
+    V3 viewDir = normalize(v3(vp[0] / w, vp[1] / w, vp[2] / w));
+    float wd[4];
+    m4mul(cam.invView, viewDir.x, viewDir.y, viewDir.z, 0.0f, wd);
+    V3 rayDirection = normalize(v3(wd[0], wd[1], wd[2]));
+    V3 origin = v3(cam.invView[12], cam.invView[13], cam.invView[14]);
+    V3 focal = add(origin, scale(rayDirection, cam.focusDist));
+    // randomInUnitHexagon takes the state by value (Q3)
+    uint32_t hr = rng;
+    const float sqrt3 = 1.73205080757f;
+    float hx, hy;
+    do {
+        hx = 2.0f * rnd(hr) - 1.0f;
+        hy = (rnd(hr) - 0.5f) * sqrt3;
+    } while (fabsf(hy) > sqrt3 * 0.5f || (sqrt3 * fabsf(hx) + fabsf(hy)) > sqrt3);
+    float lx = hx * cam.defocus, ly = hy * cam.defocus;
+    V3 right = normalize(v3(cam.invView[0], cam.invView[1], cam.invView[2]));
+    V3 up = normalize(v3(cam.invView[4], cam.invView[5], cam.invView[6]));
+    V3 no = add(origin, add(scale(right, lx), scale(up, ly)));
+    o = no;
+    d = normalize(sub(focal, no));
+}
+
+}  // namespace rt

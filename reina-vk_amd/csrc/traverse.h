@@ -1,0 +1,228 @@
+// traverse.h — closest-hit query over the BVH8 (the software replacement of
+// traceRayEXT with gl_RayFlagsOpaqueEXT, shaders/raytrace/raytrace.rgen.glsl:110-122).
+//
+// One ray per lane. Per visited node the lane tests the 8 child boxes (slabs,
+// near/far plane chosen by the direction sign, interval widened by 2^-16
+// relative so no box holding a hit triangle is rejected), tests the
+// triangles of every hit leaf immediately (watertight test of Woop, Benthin
+// and Wald, JCGT 2013), and turns the hit internal children into one "group"
+// {child_base, internal-slot mask, ordered hit mask}. Children are visited in
+// slot order i ^ oct (oct = ray direction sign bits; the builder places each
+// child in the slot of its octant), so a group's lowest ordered bit is the
+// approximately nearest child. At most one group per tree level is live, so
+// the per-lane stack in LDS needs only max-depth entries.
+//
+// The closest hit is the argmin of (t, flattened triangle id) over triangles
+// with t in [tmin, tmax]; the result therefore does not depend on traversal
+// order and equals a brute-force scan of all triangles.
+#pragma once
+
+#include "rt_layout.h"
+#include "rt_math.h"
+
+namespace rt {
+
+constexpr float kBoxEps = 1.52587890625e-05f;  // 2^-16
+
+struct TRay {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+    float ix, iy, iz;
+    float Sx, Sy, Sz;
+    int kx, ky, kz;
+    uint32_t oct;
+    bool valid;
+};
+
+RT_D float comp3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+RT_D bool finite_(float x) { return (x == x) && (x - x == 0.0f); }
+
+RT_D void tray_init(TRay& r, float4 a, float4 b) {
+    r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
+    r.dx = b.x; r.dy = b.y; r.dz = b.z; r.tmax = b.w;
+    r.valid = finite_(r.ox) && finite_(r.oy) && finite_(r.oz) && finite_(r.dx) && finite_(r.dy) &&
+              finite_(r.dz) && !(r.dx == 0.0f && r.dy == 0.0f && r.dz == 0.0f);
+    float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = (kz == 2) ? 0 : kz + 1;
+    int ky = (kx == 2) ? 0 : kx + 1;
+    float dkz = comp3(r.dx, r.dy, r.dz, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp3(r.dx, r.dy, r.dz, kx) / dkz;
+    r.Sy = comp3(r.dx, r.dy, r.dz, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    float ddx = r.dx, ddy = r.dy, ddz = r.dz;
+    if (fabsf(ddx) < 1e-30f) ddx = copysignf(1e-30f, ddx);
+    if (fabsf(ddy) < 1e-30f) ddy = copysignf(1e-30f, ddy);
+    if (fabsf(ddz) < 1e-30f) ddz = copysignf(1e-30f, ddz);
+    r.ix = 1.0f / ddx; r.iy = 1.0f / ddy; r.iz = 1.0f / ddz;
+    r.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+}
+
+// Watertight ray/triangle test; true if hit with t in [tmin, tbest].
+RT_D bool wtest(const TRay& r, float4 p0, float4 p1, float4 p2, float tbest, float& to, float& uo,
+                float& vo) {
+    float Ax_ = p0.x - r.ox, Ay_ = p0.y - r.oy, Az_ = p0.z - r.oz;
+    float Bx_ = p1.x - r.ox, By_ = p1.y - r.oy, Bz_ = p1.z - r.oz;
+    float Cx_ = p2.x - r.ox, Cy_ = p2.y - r.oy, Cz_ = p2.z - r.oz;
+    float Akz = comp3(Ax_, Ay_, Az_, r.kz), Bkz = comp3(Bx_, By_, Bz_, r.kz), Ckz = comp3(Cx_, Cy_, Cz_, r.kz);
+    float Ax = comp3(Ax_, Ay_, Az_, r.kx) - r.Sx * Akz;
+    float Ay = comp3(Ax_, Ay_, Az_, r.ky) - r.Sy * Akz;
+    float Bx = comp3(Bx_, By_, Bz_, r.kx) - r.Sx * Bkz;
+    float By = comp3(Bx_, By_, Bz_, r.ky) - r.Sy * Bkz;
+    float Cx = comp3(Cx_, Cy_, Cz_, r.kx) - r.Sx * Ckz;
+    float Cy = comp3(Cx_, Cy_, Cz_, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = (U + V) + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = (U * Az + V * Bz) + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r.tmin && t <= tbest)) return false;
+    to = t;
+    uo = V * rcp;
+    vo = W * rcp;
+    return true;
+}
+
+struct HitRec {
+    float t;
+    uint32_t prim;
+    float u, v;
+};
+
+// Slab test of the 8 children; returns the hit mask in slot order.
+RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, uint32_t meta_lo, uint32_t meta_hi, float tb) {
+    const float* base = reinterpret_cast<const float*>(node);
+    // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
+    const float4* nx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 0 : 8));
+    const float4* fx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 8 : 0));
+    const float4* ny = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 16 : 24));
+    const float4* fy = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 24 : 16));
+    const float4* nz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 32 : 40));
+    const float4* fz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 40 : 32));
+    float NX[8], FX[8], NY[8], FY[8], NZ[8], FZ[8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float4 a = nx[h], b = fx[h], c = ny[h], d = fy[h], e = nz[h], f = fz[h];
+        NX[4 * h + 0] = a.x; NX[4 * h + 1] = a.y; NX[4 * h + 2] = a.z; NX[4 * h + 3] = a.w;
+        FX[4 * h + 0] = b.x; FX[4 * h + 1] = b.y; FX[4 * h + 2] = b.z; FX[4 * h + 3] = b.w;
+        NY[4 * h + 0] = c.x; NY[4 * h + 1] = c.y; NY[4 * h + 2] = c.z; NY[4 * h + 3] = c.w;
+        FY[4 * h + 0] = d.x; FY[4 * h + 1] = d.y; FY[4 * h + 2] = d.z; FY[4 * h + 3] = d.w;
+        NZ[4 * h + 0] = e.x; NZ[4 * h + 1] = e.y; NZ[4 * h + 2] = e.z; NZ[4 * h + 3] = e.w;
+        FZ[4 * h + 0] = f.x; FZ[4 * h + 1] = f.y; FZ[4 * h + 2] = f.z; FZ[4 * h + 3] = f.w;
+    }
+    uint32_t hitm = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+        float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
+        float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
+        tn = tn - fabsf(tn) * kBoxEps;
+        tf = tf + fabsf(tf) * kBoxEps;
+        tn = fmaxf(tn, r.tmin);
+        tf = fminf(tf, tb);
+        if (m != 0u && tn <= tf) hitm |= 1u << k;
+    }
+    return hitm;
+}
+
+// Closest hit. `stk` points at this lane's first LDS stack entry; entries are
+// `stride` uint2 apart. COUNT: also return node visits / triangle tests.
+template <bool COUNT>
+RT_D HitRec trace_closest(const SceneView& sv, const TRay& r, uint2* stk, uint32_t stride,
+                          uint32_t* nvis_out, uint32_t* ntst_out) {
+    HitRec h;
+    h.t = r.tmax;
+    h.prim = 0xFFFFFFFFu;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    uint32_t nvis = 0, ntst = 0;
+    if (!r.valid || sv.num_nodes == 0) {
+        if (COUNT) { *nvis_out = 0; *ntst_out = 0; }
+        return h;
+    }
+    uint32_t cur = 0;
+    int sp = 0;
+    for (;;) {
+        const Bvh8Node* node = sv.nodes + cur;
+        const uint4 hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
+        const uint32_t child_base = hdr.x, tri_base = hdr.y, meta_lo = hdr.z, meta_hi = hdr.w;
+        if (COUNT) nvis++;
+        uint32_t hitm = node_hits(r, node, meta_lo, meta_hi, h.t);
+        // internal-slot mask and ordered masks
+        uint32_t imask = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+            imask |= ((m >> 7) & 1u) << k;
+        }
+        uint32_t om = 0, lm = 0;
+        if (hitm) {
+#pragma unroll
+            for (uint32_t i = 0; i < 8; ++i) {
+                uint32_t k = i ^ r.oct;
+                uint32_t hb = (hitm >> k) & 1u;
+                uint32_t ib = (imask >> k) & 1u;
+                om |= (hb & ib) << i;
+                lm |= (hb & (ib ^ 1u)) << i;
+            }
+        }
+        // leaves of this node, ordered
+        while (lm) {
+            uint32_t i = __builtin_ctz(lm);
+            lm &= lm - 1u;
+            uint32_t k = i ^ r.oct;
+            uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+            uint32_t cnt = (m >> 5) & 3u, off = m & 31u;
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const float4* tv = sv.tris + 3ull * (tri_base + off + j);
+                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                uint32_t id = __float_as_uint(p0.w);
+                float t, u, v;
+                if (COUNT) ntst++;
+                if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
+                    h.t = t; h.prim = id; h.u = u; h.v = v;
+                }
+            }
+        }
+        // next node: first child of this node's group, else pop
+        if (om) {
+            uint32_t i = __builtin_ctz(om);
+            om &= om - 1u;
+            uint32_t k = i ^ r.oct;
+            if (om) { stk[sp * stride] = make_uint2(child_base, imask | (om << 8)); sp++; }
+            cur = child_base + __builtin_popcount(imask & ((1u << k) - 1u));
+            continue;
+        }
+        if (sp == 0) break;
+        uint2 g = stk[(sp - 1) * stride];
+        uint32_t gm = g.y >> 8, gi = g.y & 0xFFu;
+        uint32_t i = __builtin_ctz(gm);
+        gm &= gm - 1u;
+        uint32_t k = i ^ r.oct;
+        cur = g.x + __builtin_popcount(gi & ((1u << k) - 1u));
+        if (gm) stk[(sp - 1) * stride] = make_uint2(g.x, gi | (gm << 8));
+        else sp--;
+    }
+    if (COUNT) { *nvis_out = nvis; *ntst_out = ntst; }
+    return h;
+}
+
+}  // namespace rt

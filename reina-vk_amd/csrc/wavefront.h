@@ -1,0 +1,54 @@
+// wavefront.h — per-trace buffers of the wavefront integrator and the launch
+// wrappers shared by the host driver (api.cpp) and the kernels (integrator.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rt_layout.h"
+#include "shade.h"
+
+namespace rt {
+
+struct TraceParams {
+    Camera cam;
+    uint32_t W, H;
+    uint32_t spp, maxBounces;
+    uint32_t sampleBatch;
+    float directClamp;
+};
+
+// Control words of one bounce iteration: queue length, and the two dynamic
+// work-claim counters of the extend and shade kernels. Three slots rotate so
+// that iteration i's shade zeroes the slot of iteration i+2.
+struct Ctrl { uint32_t count, fetch_ext, fetch_shade, pad; };
+
+// Path state per pixel (SoA, 64 bytes per pixel).
+struct PathState {
+    uint4* st0;     // rng, sample index, segment index, insideDielectric
+    float4* st1;    // throughput.xyz, accumulatedDistance
+    float4* st2;    // radiance L.xyz, -
+    float4* st3;    // sum.xyz, actualSamples (uint bits)
+};
+
+// Double-buffered ray queues (ReinaRay layout) with the owning pixel, and the
+// hit records of the current queue.
+struct Queues {
+    float4* ray[2];
+    uint32_t* pix[2];
+    float4* hit;
+};
+
+hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
+                       unsigned long long* stats, hipStream_t st);
+hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
+                         unsigned long long* stats, uint32_t grid, hipStream_t st);
+hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
+                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
+hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);
+hipError_t launch_intersect(const SceneView& sv, const float4* rays, float4* hits, uint32_t n, uint32_t* counters,
+                            uint32_t* fetch, uint32_t grid, hipStream_t st);
+hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
+hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,
+                              const BloomPushConsts& b, const TonemappingPushConsts& t, hipStream_t st);
+
+}  // namespace rt

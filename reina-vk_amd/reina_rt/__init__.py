@@ -1,0 +1,150 @@
+"""Python bindings (ctypes) of libreina_rt — the MI355X path-tracing hot path.
+
+The product path is the HIP library libreina_rt.so built in this directory's
+parent by build_lib.py. Loading fails loudly if it is missing: there is no
+CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+from .scene import Material, ModelData, Scene, make_desc, push_consts
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libreina_rt.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "reina_rt.h")
+
+_lib = None
+
+
+class ReinaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("reina_rt error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libreina_rt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libreina_rt.so not built: run `python reina-vk_amd/build_lib.py` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    c_p, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "reina_rt_abi_version": [ctypes.POINTER(u32)] * 4,
+        "reina_rt_scene_create": [ctypes.POINTER(abi.ReinaSceneDesc), ctypes.POINTER(c_p)],
+        "reina_rt_scene_destroy": [c_p],
+        "reina_rt_scene_info": [c_p, ctypes.POINTER(abi.ReinaSceneInfo)],
+        "reina_rt_scene_export_bvh": [c_p, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64)],
+        "reina_rt_trace": [c_p, c_p, c_p, u32, u32, c_p],
+        "reina_rt_trace_stats": [c_p, ctypes.POINTER(abi.ReinaTraceStats)],
+        "reina_rt_intersect": [c_p, c_p, c_p, u32, c_p, c_p],
+        "reina_rt_postprocess": [c_p, c_p, c_p, u32, u32, c_p, c_p, c_p],
+        "reina_rt_debug_math": [i32, c_p, c_p, c_p, u32, c_p],
+        "reina_rt_bvh_build_host": [c_p, u64, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64),
+                                    ctypes.POINTER(u32)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = i32
+    L.reina_rt_last_error.argtypes = []
+    L.reina_rt_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise ReinaError(rc, lib().reina_rt_last_error().decode())
+    return rc
+
+
+def exported_symbols():
+    return ["reina_rt_abi_version", "reina_rt_last_error", "reina_rt_scene_create", "reina_rt_scene_destroy",
+            "reina_rt_scene_info", "reina_rt_scene_export_bvh", "reina_rt_trace", "reina_rt_trace_stats",
+            "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host"]
+
+
+class DeviceScene:
+    """A scene resident in HBM (reina_rt_scene_create / _destroy)."""
+
+    def __init__(self, scene_or_arrays, flags=0, device=0):
+        arrays = scene_or_arrays.build_arrays() if isinstance(scene_or_arrays, Scene) else scene_or_arrays
+        self.arrays = arrays
+        desc = make_desc(arrays, flags=flags, device=device)
+        h = ctypes.c_void_p()
+        check(lib().reina_rt_scene_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+
+    def info(self):
+        inf = abi.ReinaSceneInfo()
+        check(lib().reina_rt_scene_info(self.handle, ctypes.byref(inf)))
+        return inf
+
+    def export_bvh(self):
+        nb, tb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(lib().reina_rt_scene_export_bvh(self.handle, None, ctypes.byref(nb), None, ctypes.byref(tb)))
+        nodes = np.zeros(nb.value, np.uint8)
+        tris = np.zeros(tb.value // 4, np.float32)
+        check(lib().reina_rt_scene_export_bvh(self.handle, nodes.ctypes.data, ctypes.byref(nb),
+                                              tris.ctypes.data, ctypes.byref(tb)))
+        return nodes, tris
+
+    def trace(self, pc, image_ptr, width, height, stream=None):
+        pcb = np.ascontiguousarray(np.asarray(pc, dtype=abi.PUSH_CONSTS).reshape(()))
+        check(lib().reina_rt_trace(self.handle, pcb.ctypes.data, ctypes.c_void_p(image_ptr), width, height,
+                                   ctypes.c_void_p(stream) if stream else None))
+
+    def stats(self):
+        st = abi.ReinaTraceStats()
+        check(lib().reina_rt_trace_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def intersect(self, rays_ptr, hits_ptr, n, counters_ptr=None, stream=None):
+        check(lib().reina_rt_intersect(self.handle, ctypes.c_void_p(rays_ptr), ctypes.c_void_p(hits_ptr), n,
+                                       ctypes.c_void_p(counters_ptr) if counters_ptr else None,
+                                       ctypes.c_void_p(stream) if stream else None))
+
+    def close(self):
+        if self.handle:
+            check(lib().reina_rt_scene_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bvh_build_host(world_tris):
+    """Host-only BVH8 build of a (n, 9) float32 triangle array -> (nodes u8, tris f32, depth)."""
+    t = np.ascontiguousarray(world_tris, np.float32).reshape(-1, 9)
+    nb, tb, dep = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint32(0)
+    check(lib().reina_rt_bvh_build_host(t.ctypes.data if len(t) else None, len(t), None, ctypes.byref(nb), None,
+                                        ctypes.byref(tb), ctypes.byref(dep)))
+    nodes = np.zeros(nb.value, np.uint8)
+    tris = np.zeros(tb.value // 4, np.float32)
+    check(lib().reina_rt_bvh_build_host(t.ctypes.data if len(t) else None, len(t), nodes.ctypes.data,
+                                        ctypes.byref(nb), tris.ctypes.data, ctypes.byref(tb), ctypes.byref(dep)))
+    return nodes, tris, dep.value
+
+
+def postprocess(in_ptr, scratch_ptr, out_ptr, width, height, radius=5.0, threshold=1.0, intensity=0.05,
+                exposure=1.0, stream=None):
+    b = np.array([(radius, threshold, intensity)], abi.BLOOM)
+    t = np.array([(exposure,)], abi.TONEMAP)
+    check(lib().reina_rt_postprocess(ctypes.c_void_p(in_ptr), ctypes.c_void_p(scratch_ptr), ctypes.c_void_p(out_ptr),
+                                     width, height, b.ctypes.data, t.ctypes.data,
+                                     ctypes.c_void_p(stream) if stream else None))
+
+
+def debug_math(fn, in_ptr, in2_ptr, out_ptr, n, stream=None):
+    check(lib().reina_rt_debug_math(fn, ctypes.c_void_p(in_ptr), ctypes.c_void_p(in2_ptr) if in2_ptr else None,
+                                    ctypes.c_void_p(out_ptr), n, ctypes.c_void_p(stream) if stream else None))

@@ -1,0 +1,101 @@
+"""numpy / ctypes mirrors of the C ABI in include/reina_rt.h.
+
+The three polyglot structs keep the reference's byte layout:
+InstanceProperties 120 B (polyglot/raytrace.h:18-41), RtPushConsts 160 B
+(polyglot/raytrace.h:43-54), InstanceData 112 B (src/scene/Instances.h:15-27).
+"""
+import ctypes
+
+import numpy as np
+
+INSTANCE_PROPERTIES = np.dtype({
+    "names": ["indicesOffset", "albedo", "emission", "tbnsIndicesOffset", "texIndicesOffset",
+              "roughness", "ior", "interpNormals", "absorption", "textureID", "normalMapTexID",
+              "bumpMapTexID", "cullBackface", "anisotropic", "subsurface", "clearcoatGloss",
+              "sheenTint", "specularTint", "metallic", "clearcoat", "specularTransmission", "sheen"],
+    "formats": ["<u4", ("<f4", 3), ("<f4", 3), "<u4", "<u4", "<f4", "<f4", "u1", "<f4", "<i4", "<i4",
+                "<i4", "<u4", "<f4", "<f4", "<f4", ("<f4", 3), ("<f4", 3), "<f4", "<f4", "<f4", "<f4"],
+    "offsets": [0, 4, 16, 28, 32, 36, 40, 44, 48, 52, 56, 60, 64, 68, 72, 76, 80, 92, 104, 108, 112, 116],
+    "itemsize": 120,
+})
+
+PUSH_CONSTS = np.dtype({
+    "names": ["invView", "invProjection", "sampleBatch", "totalEmissiveWeight", "focusDist",
+              "defocusMultiplier", "directClamp", "indirectClamp", "samplesPerPixel", "maxBounces"],
+    "formats": [("<f4", 16), ("<f4", 16), "<u4", "<f4", "<f4", "<f4", "<f4", "<f4", "<u4", "<u4"],
+    "offsets": [0, 64, 128, 132, 136, 140, 144, 148, 152, 156],
+    "itemsize": 160,
+})
+
+INSTANCE_DATA = np.dtype({
+    "names": ["transform", "materialOffset", "cdfRangeStart", "cdfRangeEnd", "indexOffset", "emission",
+              "weight", "area", "cullBackface", "padding"],
+    "formats": [("<f4", 16), "<u4", "<u4", "<u4", "<u4", ("<f4", 3), "<f4", "<f4", "u1", ("<f4", 2)],
+    "offsets": [0, 64, 68, 72, 76, 80, 92, 96, 100, 104],
+    "itemsize": 112,
+})
+
+MODEL_RANGE = np.dtype([("firstVertex", "<u4"), ("firstNormal", "<u4"), ("indexOffset", "<u4"),
+                        ("tbnsIndexOffset", "<u4"), ("texIndexOffset", "<u4"), ("indexCount", "<u4"),
+                        ("tbnsIndexCount", "<u4"), ("texIndexCount", "<u4")])
+
+INSTANCE_DESC = np.dtype([("transform", "<f4", 16), ("instancePropertiesID", "<u4"), ("materialIdx", "<u4"),
+                          ("modelID", "<u4"), ("_pad0", "<u4")])
+
+RAY = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT = np.dtype([("t", "<f4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
+
+BLOOM = np.dtype([("radius", "<f4"), ("threshold", "<f4"), ("intensity", "<f4")])
+TONEMAP = np.dtype([("exposure", "<f4")])
+
+assert INSTANCE_PROPERTIES.itemsize == 120
+assert PUSH_CONSTS.itemsize == 160
+assert INSTANCE_DATA.itemsize == 112
+assert MODEL_RANGE.itemsize == 32
+assert INSTANCE_DESC.itemsize == 80
+
+
+class ReinaSceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+        ("vertices", ctypes.c_void_p), ("num_vertices", ctypes.c_uint64),
+        ("indices", ctypes.c_void_p), ("num_indices", ctypes.c_uint64),
+        ("tbns", ctypes.c_void_p), ("num_tbns", ctypes.c_uint64),
+        ("tbn_indices", ctypes.c_void_p), ("num_tbn_indices", ctypes.c_uint64),
+        ("tex_coords", ctypes.c_void_p), ("num_tex_coords", ctypes.c_uint64),
+        ("tex_indices", ctypes.c_void_p), ("num_tex_indices", ctypes.c_uint64),
+        ("props", ctypes.c_void_p), ("num_props", ctypes.c_uint32),
+        ("models", ctypes.c_void_p), ("num_models", ctypes.c_uint32),
+        ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_uint32),
+        ("device", ctypes.c_uint32),
+    ]
+
+
+class ReinaSceneInfo(ctypes.Structure):
+    _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
+                ("node_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
+                ("max_depth", ctypes.c_uint32), ("quantized", ctypes.c_uint32),
+                ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float)]
+
+
+class ReinaTraceStats(ctypes.Structure):
+    _fields_ = [("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint32), ("_pad0", ctypes.c_uint32),
+                ("trace_ms", ctypes.c_float), ("extend_ms", ctypes.c_float)]
+
+
+# status codes
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_ABI = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_OVER_BUDGET = -4
+ERR_DEVICE = -5
+ERR_UNSUPPORTED = -6
+ERR_NO_EMISSIVE = -7
+ERR_BUSY = -8
+
+BUILD_HOST = 0x0
+BUILD_DEVICE = 0x1
+NODES_QUANTIZED = 0x2

@@ -1,0 +1,245 @@
+"""Host-side scene assembly mirroring Reina's Scene / Models API.
+
+`Scene.define_object` / `add_instance` / `add_object` / `build_desc` pack the
+buffers exactly as src/scene/Models.cpp:24-92 (Models::addModel) and
+src/scene/Scene.cpp:27-54 (Scene::addInstance) do, and hand them to
+reina_rt_scene_create through a ReinaSceneDesc. Camera matrices follow
+src/graphics/Camera.cpp:10-11 (glm::perspective / glm::lookAt, OpenGL
+right-handed clip space with depth in [-1, 1]).
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class ModelData:
+    """src/scene/Models.h:23-30. vertices: (n, 4) float32 (w = 1); indices:
+    (m,) uint32 per-model; tbns: (k, 3, 3) columns (T, B, N); texCoords (j, 2)."""
+    vertices: np.ndarray
+    indices: np.ndarray
+    tbns: np.ndarray = None
+    tbnsIndices: np.ndarray = None
+    texCoords: np.ndarray = None
+    texIndices: np.ndarray = None
+
+
+@dataclass
+class Material:
+    """src/scene/Scene.h:21-46 (field order kept)."""
+    materialIdx: int = 0
+    textureID: int = -1
+    normalMapID: int = -1
+    bumpMapID: int = -1
+    albedo: tuple = (0.73, 0.73, 0.73)
+    emission: tuple = (0.0, 0.0, 0.0)
+    roughness: float = 0.0
+    ior: float = 1.5
+    interpNormals: bool = False
+    absorption: float = 0.0
+    cullBackface: bool = False
+    anisotropic: float = 0.0
+    subsurface: float = 0.0
+    clearcoatGloss: float = 0.0
+    sheenTint: tuple = (0.0, 0.0, 0.0)
+    specularTint: tuple = (1.0, 1.0, 1.0)
+    metallic: float = 0.0
+    clearcoat: float = 0.0
+    specularTransmission: float = 0.0
+    sheen: float = 0.0
+
+
+class Scene:
+    def __init__(self):
+        self._models = []          # ModelData
+        self._ranges = []          # ModelRange dicts
+        self._props = []
+        self._instances = []       # (propsID, materialIdx, objectID, transform)
+        self._vert_count = 0       # in floats (vec4 per vertex)
+        self._tbn_count = 0        # matrices
+        self._tex_count = 0        # floats (vec2 per coord)
+        self._idx_count = 0
+        self._tbni_count = 0
+        self._texi_count = 0
+        self._built = False
+
+    # Models::addModel (src/scene/Models.cpp:24-92)
+    def define_object(self, md: ModelData) -> int:
+        if self._built:
+            raise RuntimeError("Could not add model; buffers are already built")
+        v = np.ascontiguousarray(md.vertices, dtype=np.float32).reshape(-1, 4)
+        ind = np.ascontiguousarray(md.indices, dtype=np.uint32).reshape(-1)
+        tb = np.zeros((0, 3, 3), np.float32) if md.tbns is None else np.ascontiguousarray(md.tbns, np.float32).reshape(-1, 3, 3)
+        tbi = np.zeros(0, np.uint32) if md.tbnsIndices is None else np.ascontiguousarray(md.tbnsIndices, np.uint32).reshape(-1)
+        tc = np.zeros((0, 2), np.float32) if md.texCoords is None else np.ascontiguousarray(md.texCoords, np.float32).reshape(-1, 2)
+        ti = np.zeros(0, np.uint32) if md.texIndices is None else np.ascontiguousarray(md.texIndices, np.uint32).reshape(-1)
+        md2 = ModelData(v, ind, tb, tbi, tc, ti)
+        rng = dict(
+            firstVertex=self._vert_count // 4,
+            firstNormal=self._tbn_count,
+            indexOffset=self._idx_count,
+            tbnsIndexOffset=self._tbni_count,
+            texIndexOffset=0xFFFFFFFF if len(tc) == 0 else self._texi_count,
+            indexCount=len(ind) // 3,
+            tbnsIndexCount=len(ind) // 3,
+            texIndexCount=len(ti) // 3,
+        )
+        md2._offsets = (self._vert_count // 4, self._tbn_count, self._tex_count // 2)
+        self._ranges.append(rng)
+        self._models.append(md2)
+        self._vert_count += v.size
+        self._tbn_count += len(tb)
+        self._tex_count += tc.size
+        self._idx_count += len(ind)
+        self._tbni_count += len(tbi)
+        self._texi_count += len(ti)
+        return len(self._ranges) - 1
+
+    # Scene::addInstance (src/scene/Scene.cpp:27-54)
+    def add_instance(self, object_id: int, transform, mat: Material):
+        r = self._ranges[object_id]
+        p = np.zeros((), abi.INSTANCE_PROPERTIES)
+        p["indicesOffset"] = r["indexOffset"]
+        p["albedo"] = mat.albedo
+        p["emission"] = mat.emission
+        p["tbnsIndicesOffset"] = r["tbnsIndexOffset"]
+        p["texIndicesOffset"] = r["texIndexOffset"]
+        p["roughness"] = mat.roughness
+        p["ior"] = mat.ior
+        p["interpNormals"] = 1 if mat.interpNormals else 0
+        p["absorption"] = mat.absorption
+        p["textureID"] = mat.textureID
+        p["normalMapTexID"] = mat.normalMapID
+        p["bumpMapTexID"] = mat.bumpMapID
+        p["cullBackface"] = 1 if mat.cullBackface else 0
+        p["anisotropic"] = mat.anisotropic
+        p["subsurface"] = mat.subsurface
+        p["clearcoatGloss"] = mat.clearcoatGloss
+        p["sheenTint"] = mat.sheenTint
+        p["specularTint"] = mat.specularTint
+        p["metallic"] = mat.metallic
+        p["clearcoat"] = mat.clearcoat
+        p["specularTransmission"] = mat.specularTransmission
+        p["sheen"] = mat.sheen
+        self._props.append(p)
+        t = np.asarray(transform, dtype=np.float32).reshape(4, 4)
+        self._instances.append((len(self._props) - 1, int(mat.materialIdx), int(object_id), t))
+
+    def add_object(self, md: ModelData, transform, mat: Material) -> int:
+        oid = self.define_object(md)
+        self.add_instance(oid, transform, mat)
+        return oid
+
+    def build_arrays(self):
+        """Packed buffers (the same arrays Models::buildBuffers uploads)."""
+        nv = self._vert_count // 4
+        verts = np.zeros((nv, 4), np.float32)
+        tbns = np.zeros((self._tbn_count, 3, 3), np.float32)
+        texc = np.zeros((self._tex_count // 2, 2), np.float32)
+        idx = np.zeros(self._idx_count, np.uint32)
+        tbi = np.zeros(self._tbni_count, np.uint32)
+        tei = np.zeros(self._texi_count, np.uint32)
+        for md, r in zip(self._models, self._ranges):
+            vo, to, xo = md._offsets
+            verts[vo:vo + len(md.vertices)] = md.vertices
+            tbns[to:to + len(md.tbns)] = md.tbns
+            texc[xo:xo + len(md.texCoords)] = md.texCoords
+            io = r["indexOffset"]
+            idx[io:io + len(md.indices)] = md.indices + np.uint32(vo)                 # Models.cpp:78-81
+            ko = r["tbnsIndexOffset"]
+            tbi[ko:ko + len(md.tbnsIndices)] = md.tbnsIndices + np.uint32(to)         # Models.cpp:83-85
+            if len(md.texIndices):
+                so = r["texIndexOffset"]
+                ti = md.texIndices.copy()
+                m = ti != 0xFFFFFFFF
+                ti[m] = ti[m] + np.uint32(xo)                                         # Models.cpp:87-89
+                tei[so:so + len(ti)] = ti
+        models = np.zeros(len(self._ranges), abi.MODEL_RANGE)
+        for k, r in enumerate(self._ranges):
+            for name in abi.MODEL_RANGE.names:
+                models[k][name] = r[name]
+        props = np.array(self._props, dtype=abi.INSTANCE_PROPERTIES).reshape(-1)
+        inst = np.zeros(len(self._instances), abi.INSTANCE_DESC)
+        for k, (pid, mi, oid, t) in enumerate(self._instances):
+            # glm::mat4 is column-major: column c = t[:, c]
+            inst[k]["transform"] = t.T.reshape(-1)
+            inst[k]["instancePropertiesID"] = pid
+            inst[k]["materialIdx"] = mi
+            inst[k]["modelID"] = oid
+        return dict(vertices=verts, indices=idx, tbns=tbns, tbn_indices=tbi, tex_coords=texc,
+                    tex_indices=tei, props=props, models=models, instances=inst)
+
+
+def make_desc(arrays, flags=0, device=0):
+    """ReinaSceneDesc pointing into `arrays` (keep `arrays` alive while used)."""
+    d = abi.ReinaSceneDesc()
+    d.struct_size = ctypes.sizeof(abi.ReinaSceneDesc)
+    d.flags = flags
+    d.device = device
+
+    def ptr(a):
+        return a.ctypes.data if a.size else None
+
+    a = arrays
+    d.vertices, d.num_vertices = ptr(a["vertices"]), len(a["vertices"])
+    d.indices, d.num_indices = ptr(a["indices"]), len(a["indices"])
+    d.tbns, d.num_tbns = ptr(a["tbns"]), len(a["tbns"])
+    d.tbn_indices, d.num_tbn_indices = ptr(a["tbn_indices"]), len(a["tbn_indices"])
+    d.tex_coords, d.num_tex_coords = ptr(a["tex_coords"]), len(a["tex_coords"])
+    d.tex_indices, d.num_tex_indices = ptr(a["tex_indices"]), len(a["tex_indices"])
+    d.props, d.num_props = ptr(a["props"]), len(a["props"])
+    d.models, d.num_models = ptr(a["models"]), len(a["models"])
+    d.instances, d.num_instances = ptr(a["instances"]), len(a["instances"])
+    return d
+
+
+# ---- camera (src/graphics/Camera.cpp:10-11, Reina.cpp:138-156) --------------------
+def perspective(fovy, aspect, znear, zfar):
+    f = np.float32(1.0) / np.tan(np.float32(fovy) / np.float32(2.0), dtype=np.float32)
+    m = np.zeros((4, 4), np.float32)
+    m[0, 0] = f / np.float32(aspect)
+    m[1, 1] = f
+    m[2, 2] = -(np.float32(zfar) + np.float32(znear)) / (np.float32(zfar) - np.float32(znear))
+    m[3, 2] = np.float32(-1.0)
+    m[2, 3] = -(np.float32(2.0) * np.float32(zfar) * np.float32(znear)) / (np.float32(zfar) - np.float32(znear))
+    return m
+
+
+def look_at(eye, center, up):
+    eye = np.asarray(eye, np.float32)
+    f = np.asarray(center, np.float32) - eye
+    f = f / np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, np.float32))
+    s = s / np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4, dtype=np.float32)
+    m[0, :3] = s
+    m[1, :3] = u
+    m[2, :3] = -f
+    m[0, 3] = -np.dot(s, eye)
+    m[1, 3] = -np.dot(u, eye)
+    m[2, 3] = np.dot(f, eye)
+    return m.astype(np.float32)
+
+
+def push_consts(eye, center, fov_deg, width, height, spp, max_bounces, sample_batch=0,
+                focus_dist=2.2, defocus=0.0, direct_clamp=100.0, indirect_clamp=10.0, up=(0.0, 1.0, 0.0)):
+    """RtPushConsts as Reina fills them (Reina.cpp:144-155); matrices column-major."""
+    aspect = float(width) / float(height)
+    inv_proj = np.linalg.inv(perspective(np.radians(fov_deg), aspect, 0.1, 100.0).astype(np.float64)).astype(np.float32)
+    inv_view = np.linalg.inv(look_at(eye, center, up).astype(np.float64)).astype(np.float32)
+    pc = np.zeros((), abi.PUSH_CONSTS)
+    pc["invView"] = inv_view.T.reshape(-1)
+    pc["invProjection"] = inv_proj.T.reshape(-1)
+    pc["sampleBatch"] = sample_batch
+    pc["totalEmissiveWeight"] = 0.0
+    pc["focusDist"] = focus_dist
+    pc["defocusMultiplier"] = defocus
+    pc["directClamp"] = direct_clamp
+    pc["indirectClamp"] = indirect_clamp
+    pc["samplesPerPixel"] = spp
+    pc["maxBounces"] = max_bounces
+    return pc

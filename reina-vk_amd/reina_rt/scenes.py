@@ -1,0 +1,291 @@
+"""Seeded, deterministic procedural scenes for the benchmark configurations.
+
+cornell()      40 triangles: floor and ceiling as two quads each, back, red
+               left and green right walls, a ceiling quad light (emission 15,
+               at the height of models/cornell_light.obj), a tall box and a
+               short box rotated 15 degrees (unit-cube model instanced with
+               scale/rotate/translate). Open front; the box spans
+               [-1,1]x[0,2]x[-1,1] like models/cornell_box.obj.
+icosphere()    icosahedron subdivided 7 times (327,680 triangles), vertices
+               displaced radially by 4-octave value noise (amplitude 0.05);
+               one model + Disney instance per group of 256 triangles
+               (1,280 groups, roughness / metallic / base colour uniform per
+               group), a ground quad and two quad lights.
+instanced()    copies of a 2,048-triangle displaced UV sphere with random
+               scale, rotation and position in a 100^3 box; materials by
+               instance index mod 3 (dielectric, metal, Lambertian).
+"""
+import numpy as np
+
+from .scene import Material, ModelData, Scene, push_consts
+
+
+# ---- helpers ---------------------------------------------------------------------
+def _quad(c, u, v):
+    """Quad c +- u +- v with geometric normal along cross(u, v); flat TBN (u, v, n)."""
+    c, u, v = (np.asarray(a, np.float64) for a in (c, u, v))
+    p = np.array([c - u - v, c + u - v, c + u + v, c - u + v])
+    n = np.cross(u, v)
+    n /= np.linalg.norm(n)
+    t = u / np.linalg.norm(u)
+    b = v / np.linalg.norm(v)
+    return p, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), (t, b, n)
+
+
+def _model_from_quads(quads):
+    verts, idx, tbns = [], [], []
+    for p, f, (t, b, n) in quads:
+        base = len(verts)
+        for q in p:
+            verts.append([q[0], q[1], q[2], 1.0])
+            tbns.append(np.stack([t, b, n]))
+        idx.extend((f + base).reshape(-1).tolist())
+    verts = np.array(verts, np.float32)
+    idx = np.array(idx, np.uint32)
+    tbns = np.array(tbns, np.float32)
+    return ModelData(verts, idx, tbns, idx.copy())
+
+
+def _translate(t):
+    m = np.eye(4, dtype=np.float64)
+    m[:3, 3] = t
+    return m
+
+
+def _scale(s):
+    return np.diag([s[0], s[1], s[2], 1.0])
+
+
+def _rot_y(deg):
+    a = np.radians(deg)
+    m = np.eye(4)
+    m[0, 0], m[0, 2], m[2, 0], m[2, 2] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+    return m
+
+
+def _unit_cube():
+    h = 0.5
+    faces = [
+        ((h, 0, 0), (0, h, 0), (0, 0, h)), ((-h, 0, 0), (0, 0, h), (0, h, 0)),
+        ((0, h, 0), (0, 0, h), (h, 0, 0)), ((0, -h, 0), (h, 0, 0), (0, 0, h)),
+        ((0, 0, h), (h, 0, 0), (0, h, 0)), ((0, 0, -h), (0, h, 0), (h, 0, 0)),
+    ]
+    return _model_from_quads([_quad(*f) for f in faces])
+
+
+# ---- C1: Cornell box -------------------------------------------------------------
+RED = (0.63, 0.065, 0.05)
+GREEN = (0.14, 0.45, 0.091)
+GREY = (0.73, 0.73, 0.73)
+
+
+def cornell():
+    """Returns (Scene, camera dict). 40 triangles."""
+    s = Scene()
+    wall = lambda albedo: Material(materialIdx=0, albedo=albedo, cullBackface=True)
+    I = np.eye(4)
+    floor = [_quad((0, 0, z), (0, 0, 0.5), (1, 0, 0)) for z in (0.5, -0.5)]
+    ceil = [_quad((0, 2, z), (1, 0, 0), (0, 0, 0.5)) for z in (-0.5, 0.5)]
+    s.add_object(_model_from_quads(floor), I, wall(GREY))
+    s.add_object(_model_from_quads(ceil), I, wall(GREY))
+    s.add_object(_model_from_quads([_quad((0, 1, -1), (1, 0, 0), (0, 1, 0))]), I, wall(GREY))
+    s.add_object(_model_from_quads([_quad((-1, 1, 0), (0, 1, 0), (0, 0, 1))]), I, wall(RED))
+    s.add_object(_model_from_quads([_quad((1, 1, 0), (0, 0, 1), (0, 1, 0))]), I, wall(GREEN))
+    light = _model_from_quads([_quad((-0.005, 1.989, -0.03), (0.235, 0, 0), (0, 0, 0.19))])
+    s.add_object(light, I, Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(15.0, 15.0, 15.0),
+                                    cullBackface=True))
+    cube = s.define_object(_unit_cube())
+    tall = _translate((-0.35, 0.6, -0.35)) @ _scale((0.6, 1.2, 0.6))
+    short = _translate((0.35, 0.3, 0.3)) @ _rot_y(15.0) @ _scale((0.6, 0.6, 0.6))
+    s.add_instance(cube, tall, Material(materialIdx=0, albedo=GREY))
+    s.add_instance(cube, short, Material(materialIdx=0, albedo=GREY))
+    cam = dict(eye=(0.0, 1.0, 3.4), center=(0.0, 1.0, 0.0), fov=40.0)
+    return s, cam
+
+
+def cornell_push(width=256, height=256, spp=16, max_bounces=8, sample_batch=0):
+    _, cam = cornell()
+    return push_consts(cam["eye"], cam["center"], cam["fov"], width, height, spp, max_bounces,
+                       sample_batch=sample_batch, defocus=0.0, direct_clamp=100.0)
+
+
+# ---- value noise -----------------------------------------------------------------
+def _hash3(ix, iy, iz, seed):
+    h = (ix.astype(np.int64) * 73856093) ^ (iy.astype(np.int64) * 19349663) ^ (iz.astype(np.int64) * 83492791)
+    h = (h ^ (seed * 2654435761)) & 0xFFFFFFFF
+    h = (h * 747796405 + 2891336453) & 0xFFFFFFFF
+    h = ((h >> ((h >> 28) + 4)) ^ h) * 277803737 & 0xFFFFFFFF
+    h = (h >> 22) ^ h
+    return (h.astype(np.float64) / 4294967295.0) * 2.0 - 1.0
+
+
+def value_noise(p, seed, octaves=4, freq=2.0):
+    """Sum of `octaves` trilinear value-noise octaves (smoothstep weights),
+    amplitude halving and frequency doubling, normalised to [-1, 1]."""
+    total = np.zeros(len(p))
+    amp, norm = 1.0, 0.0
+    for o in range(octaves):
+        q = p * freq * (2.0 ** o)
+        i0 = np.floor(q)
+        f = q - i0
+        w = f * f * (3.0 - 2.0 * f)
+        i0 = i0.astype(np.int64)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    val = _hash3(i0[:, 0] + dx, i0[:, 1] + dy, i0[:, 2] + dz, seed + 1013 * o)
+                    wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                    wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                    wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                    acc += val * wx * wy * wz
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+# ---- C2: displaced icosphere -------------------------------------------------------
+def _icosahedron():
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    return v, f
+
+
+def subdivided_icosphere(levels):
+    """Unit icosphere; children of face i are faces 4i..4i+3 (depth-first order)."""
+    v, f = _icosahedron()
+    for _ in range(levels):
+        nv = len(v)
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        lo, hi = np.minimum(e[:, 0], e[:, 1]), np.maximum(e[:, 0], e[:, 1])
+        key = lo * nv + hi
+        uk, inv = np.unique(key, return_inverse=True)
+        a, b = uk // nv, uk % nv
+        m = v[a] + v[b]
+        m /= np.linalg.norm(m, axis=1, keepdims=True)
+        mid = nv + inv.reshape(3, -1).T          # (nf, 3): ab, bc, ca
+        v = np.concatenate([v, m])
+        ab, bc, ca = mid[:, 0], mid[:, 1], mid[:, 2]
+        a0, b0, c0 = f[:, 0], f[:, 1], f[:, 2]
+        nf = np.stack([np.stack([a0, ab, ca], 1), np.stack([ab, b0, bc], 1), np.stack([ca, bc, c0], 1),
+                       np.stack([ab, bc, ca], 1)], 1)
+        f = nf.reshape(-1, 3)
+    return v, f
+
+
+def _vertex_frames(v, f):
+    """Area-weighted vertex normals and a tangent frame (T, B, N)."""
+    fn = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    n = np.zeros_like(v)
+    for k in range(3):
+        np.add.at(n, f[:, k], fn)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    ref = np.where(np.abs(n[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(ref, n)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(n, t)
+    return np.stack([t, b, n], 1)
+
+
+def icosphere(levels=7, seed=7, amplitude=0.05, group=256):
+    """Returns (Scene, camera dict)."""
+    v, f = subdivided_icosphere(levels)
+    r = 1.0 + amplitude * value_noise(v, seed)
+    vd = v * r[:, None]
+    frames = _vertex_frames(vd, f)
+    rng = np.random.default_rng(seed)
+    ngroups = len(f) // group
+    params = rng.random((ngroups, 5))   # roughness, metallic, r, g, b
+    s = Scene()
+    I = np.eye(4)
+    v32 = vd.astype(np.float32)
+    fr32 = frames.astype(np.float32)
+    for g in range(ngroups):
+        ff = f[g * group:(g + 1) * group]
+        uv, loc = np.unique(ff.reshape(-1), return_inverse=True)
+        verts = np.concatenate([v32[uv], np.ones((len(uv), 1), np.float32)], 1)
+        idx = loc.astype(np.uint32)
+        md = ModelData(verts, idx, fr32[uv], idx.copy())
+        rough, metal, cr, cg, cb = params[g]
+        s.add_object(md, I, Material(materialIdx=3, albedo=(cr, cg, cb), roughness=float(rough),
+                                     metallic=float(metal), ior=1.5, interpNormals=True,
+                                     specularTint=(1.0, 1.0, 1.0)))
+    ground = _model_from_quads([_quad((0, -1.1, 0), (0, 0, 20), (20, 0, 0))])
+    s.add_object(ground, I, Material(materialIdx=0, albedo=(0.5, 0.5, 0.5)))
+    l1 = _model_from_quads([_quad((-2.0, 3.0, 1.0), (0.5, 0, 0), (0, 0, 0.5))])
+    l2 = _model_from_quads([_quad((2.5, 2.5, -1.0), (0.5, 0, 0), (0, 0, 0.5))])
+    s.add_object(l1, I, Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(10.0, 10.0, 10.0),
+                                 cullBackface=True))
+    s.add_object(l2, I, Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(6.0, 6.0, 8.0),
+                                 cullBackface=True))
+    cam = dict(eye=(0.0, 0.4, 3.2), center=(0.0, 0.0, 0.0), fov=45.0)
+    return s, cam
+
+
+# ---- C3: instanced displaced spheres ----------------------------------------------
+def uv_sphere(segments=32, rings=32, seed=3, amplitude=0.05):
+    """segments x rings quads -> 2*segments*rings triangles (pole triangles are
+    kept, degenerate), radially displaced by value noise."""
+    th = np.linspace(0.0, np.pi, rings + 1)
+    ph = np.linspace(0.0, 2.0 * np.pi, segments + 1)[:-1]
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    v = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+    r = 1.0 + amplitude * value_noise(v, seed)
+    v = v * r[:, None]
+    faces = []
+    for i in range(rings):
+        for j in range(segments):
+            a = i * segments + j
+            b = i * segments + (j + 1) % segments
+            c = (i + 1) * segments + j
+            d = (i + 1) * segments + (j + 1) % segments
+            faces.append([a, c, b])
+            faces.append([b, c, d])
+    f = np.array(faces, np.int64)
+    return v, f
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def instanced(copies=4096, seed=11, box=100.0):
+    """Returns (Scene, camera dict)."""
+    v, f = uv_sphere()
+    with np.errstate(invalid="ignore"):
+        frames = _vertex_frames(v, f)
+    frames = np.nan_to_num(frames)
+    md = ModelData(np.concatenate([v, np.ones((len(v), 1))], 1).astype(np.float32), f.reshape(-1).astype(np.uint32),
+                   frames.astype(np.float32), f.reshape(-1).astype(np.uint32))
+    s = Scene()
+    oid = s.define_object(md)
+    rng = np.random.default_rng(seed)
+    for k in range(copies):
+        sc = rng.uniform(0.2, 1.0)
+        R = _random_rotation(rng)
+        pos = rng.uniform(0.0, box, size=3)
+        M = np.eye(4)
+        M[:3, :3] = R * sc
+        M[:3, 3] = pos
+        kind = k % 3
+        if kind == 0:
+            mat = Material(materialIdx=2, albedo=(0.95, 0.95, 0.95), ior=1.5,
+                           absorption=float(rng.uniform(0.0, 2.0)))
+        elif kind == 1:
+            mat = Material(materialIdx=1, albedo=tuple(rng.uniform(0.3, 1.0, 3)), roughness=float(rng.uniform(0.0, 0.5)))
+        else:
+            mat = Material(materialIdx=0, albedo=tuple(rng.uniform(0.2, 0.9, 3)))
+        s.add_instance(oid, M, mat)
+    cam = dict(eye=(50.0, 50.0, -40.0), center=(50.0, 50.0, 50.0), fov=60.0)
+    return s, cam

@@ -1,0 +1,246 @@
+"""GPU parity: the HIP kernels against the oracle, through the C ABI.
+
+Bit-exact where both sides evaluate the same FP32 operation sequence (the
+elementary functions, the watertight intersection, the full integrator, the
+post-process); statistical / tolerance-based against the FP64 intersection
+oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import reina_rt
+from reina_rt import abi, scenes
+from reina_rt.scene import make_desc, push_consts
+
+from conftest import world_triangles
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def gpu_intersect(torch, dev, rays, counters=False):
+    r = torch.from_numpy(rays.view(np.float32).reshape(-1, 8).copy()).cuda()
+    h = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    c = torch.zeros((len(rays), 2), dtype=torch.int32, device="cuda") if counters else None
+    dev.intersect(r.data_ptr(), h.data_ptr(), len(rays), c.data_ptr() if counters else None,
+                  stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    hits = h.cpu().numpy().view(abi.HIT).reshape(-1)
+    return (hits, c.cpu().numpy().astype(np.uint32)) if counters else hits
+
+
+def gpu_render(torch, dev, pc, W, H, init=None):
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") if init is None else \
+        torch.from_numpy(init.copy()).cuda()
+    dev.trace(pc, img.data_ptr(), W, H, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return img.cpu().numpy()
+
+
+def random_rays(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, abi.RAY)
+    o = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    rays["ox"], rays["oy"], rays["oz"] = o.T
+    rays["dx"], rays["dy"], rays["dz"] = d.T
+    rays["tmax"] = 10000.0
+    return rays
+
+
+def camera_rays(pc, W, H, n, seed):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, abi.RAY)
+    for i in range(n):
+        o, d, _ = oracle.camera_ray(pc, int(rng.integers(0, W)), int(rng.integers(0, H)), W, H,
+                                    int(rng.integers(0, 2 ** 32)))
+        rays[i] = (o[0], o[1], o[2], 0.0, d[0], d[1], d[2], 10000.0)
+    return rays
+
+
+def secondary_rays(orc, rays, hits, seed):
+    """Rays leaving the hit points in random directions (origin offset by the
+    library's self-intersection rule)."""
+    rng = np.random.default_rng(seed)
+    m = hits["prim"] != 0xFFFFFFFF
+    r, h = rays[m], hits[m]
+    o = np.stack([r["ox"], r["oy"], r["oz"]], 1) + h["t"][:, None] * np.stack([r["dx"], r["dy"], r["dz"]], 1)
+    d = rng.normal(size=o.shape).astype(np.float32)
+    out = np.zeros(len(o), abi.RAY)
+    out["ox"], out["oy"], out["oz"] = o.astype(np.float32).T
+    out["dx"], out["dy"], out["dz"] = d.T
+    out["tmin"] = 1e-4
+    out["tmax"] = 10000.0
+    return out
+
+
+# ---- elementary functions ----------------------------------------------------------
+@pytest.mark.parametrize("fn,lo,hi", [(0, -20.0, 20.0), (1, -20.0, 20.0), (2, 1e-30, 1e4), (3, -100.0, 90.0),
+                                      (4, 0.0, 1e6), (5, 1e-6, 10.0), (6, -140.0, 130.0)])
+def test_math_bitexact(cuda, fn, lo, hi):
+    torch = cuda
+    rng = np.random.default_rng(fn)
+    x = rng.uniform(lo, hi, 200000).astype(np.float32)
+    if fn == 2:
+        x = np.exp(rng.uniform(np.log(1e-38), np.log(1e30), 200000)).astype(np.float32)
+    y = rng.uniform(-3.0, 3.0, len(x)).astype(np.float32)
+    specials = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 1e-40, 3.0e38], np.float32)
+    x = np.concatenate([x, specials])
+    y = np.concatenate([y, np.full(len(specials), 0.5, np.float32)])
+    ref = oracle.math_eval(fn, x, y)
+    tx, ty = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    out = torch.zeros_like(tx)
+    reina_rt.debug_math(fn, tx.data_ptr(), ty.data_ptr(), out.data_ptr(), len(x))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), "fn %d: %d mismatches, e.g. x=%r gpu=%r oracle=%r" % (
+        fn, (~same).sum(), x[~same][:3], got[~same][:3], ref[~same][:3])
+
+
+# ---- closest hit -------------------------------------------------------------------
+def test_intersect_bitexact_cornell(cuda, cornell_arrays):
+    torch = cuda
+    arrays, cam = cornell_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh())
+    pc = scenes.cornell_push(256, 256, 1, 8)
+    rays = np.concatenate([random_rays(100000, -1.5, 2.5, 1), camera_rays(pc, 256, 256, 4000, 2)])
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref_b = orc.brute32(rays)
+    ref_v, ref_c = orc.bvh(rays)
+    for f in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[f]), _bits(ref_b[f])), f
+    assert np.array_equal(hits["prim"], ref_b["prim"])
+    assert np.array_equal(cnt, ref_c), "node-visit / triangle-test counters differ from the traversal restatement"
+    dev.close()
+
+
+def test_intersect_bitexact_icosphere(cuda, icosphere_arrays):
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh())
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], 1920, 1080, 1, 8)
+    prim = camera_rays(pc, 1920, 1080, 3000, 3)
+    ph = orc.brute32(prim)
+    rays = np.concatenate([prim, secondary_rays(orc, prim, ph, 4), random_rays(2000, -1.5, 1.5, 5)])
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref = orc.brute32(rays)
+    ref_v, ref_c = orc.bvh(rays)
+    assert np.array_equal(hits["prim"], ref["prim"])
+    for f in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[f]), _bits(ref[f])), f
+    assert np.array_equal(cnt, ref_c)
+    dev.close()
+
+
+def test_intersect_vs_fp64(cuda, icosphere_arrays):
+    """Hit id agrees with the FP64 oracle on >= 99.99% of rays; every
+    disagreement is an FP64-detected edge/vertex/equal-t tie; t within 1e-6
+    relative on the agreeing primary rays."""
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], 1920, 1080, 1, 8)
+    prim = camera_rays(pc, 1920, 1080, 3000, 6)
+    hits = gpu_intersect(torch, dev, prim)
+    t64, p64, u64, v64, amb = orc.fp64(prim, tie_rel=1e-5)
+    agree = hits["prim"] == p64
+    assert agree.mean() >= 0.9999 or (~agree & ~amb).sum() == 0
+    assert (~agree & ~amb).sum() == 0, "non-tie disagreements: %d" % (~agree & ~amb).sum()
+    m = agree & (p64 != 0xFFFFFFFF)
+    rel = np.abs(hits["t"][m].astype(np.float64) - t64[m]) / t64[m]
+    assert rel.max() <= 1e-6, "max rel t error %g" % rel.max()
+    dev.close()
+
+
+# ---- integrator ------------------------------------------------------------------
+def test_render_cornell_bitexact(cuda, cornell_arrays):
+    torch = cuda
+    arrays, cam = cornell_arrays
+    W = H = 64
+    pc = scenes.cornell_push(W, H, spp=16, max_bounces=8)
+    dev = reina_rt.DeviceScene(arrays)
+    got = gpu_render(torch, dev, pc, W, H)
+    ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    assert np.array_equal(_bits(got), _bits(ref)), "pixels differing: %d" % (
+        (_bits(got) != _bits(ref)).any(-1).sum())
+    st = dev.stats()
+    assert st.extension_rays == nrays
+    dev.close()
+
+
+def test_render_icosphere_bitexact(cuda, icosphere_arrays):
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    W, H = 96, 54
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 8)
+    dev = reina_rt.DeviceScene(arrays)
+    got = gpu_render(torch, dev, pc, W, H)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh())
+    ref, nrays = orc.render(pc, W, H, use_bvh=True)
+    same = _bits(got) == _bits(ref)
+    same |= np.isnan(got) & np.isnan(ref)
+    assert same.all(), "pixels differing: %d" % (~same).any(-1).sum()
+    assert dev.stats().extension_rays == nrays
+    dev.close()
+
+
+def test_running_mean_bitexact(cuda, cornell_arrays):
+    """sampleBatch > 0 blends (prev * b + new) / (b + 1) (raytrace.rgen.glsl:277-282)."""
+    torch = cuda
+    arrays, _ = cornell_arrays
+    W, H = 40, 24
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    img_g, img_o = None, np.zeros((H, W, 4), np.float32)
+    for b in range(3):
+        pc = scenes.cornell_push(W, H, spp=2, max_bounces=4, sample_batch=b)
+        img_g = gpu_render(torch, dev, pc, W, H, init=img_g)
+        img_o, _ = orc.render(pc, W, H, img=img_o, use_bvh=False)
+        assert np.array_equal(_bits(img_g), _bits(img_o)), "batch %d" % b
+    dev.close()
+
+
+@pytest.mark.parametrize("spp,bounces", [(0, 8), (3, 0), (1, 1)])
+def test_render_edge_cases(cuda, cornell_arrays, spp, bounces):
+    torch = cuda
+    arrays, _ = cornell_arrays
+    W, H = 7, 5
+    pc = scenes.cornell_push(W, H, spp=spp, max_bounces=bounces)
+    dev = reina_rt.DeviceScene(arrays)
+    got = gpu_render(torch, dev, pc, W, H)
+    ref, _ = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all()
+    if spp == 0:
+        assert np.isnan(got[..., :3]).all()
+    if bounces == 0:
+        assert (got[..., :3] == 0).all()
+    dev.close()
+
+
+# ---- post-process ----------------------------------------------------------------
+def test_postprocess_bitexact(cuda):
+    torch = cuda
+    rng = np.random.default_rng(9)
+    H, W = 45, 70
+    img = (rng.random((H, W, 4)) ** 4 * 4.0).astype(np.float32)
+    img[..., 3] = 1.0
+    ref_out, ref_comb = oracle.postprocess(img, radius=5.0, threshold=1.0, intensity=0.05, exposure=1.0)
+    t_in = torch.from_numpy(img).cuda()
+    scratch = torch.zeros((2, H, W, 4), dtype=torch.float32, device="cuda")
+    out = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
+    reina_rt.postprocess(t_in.data_ptr(), scratch.data_ptr(), out.data_ptr(), W, H, 5.0, 1.0, 0.05, 1.0)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), ref_out)
+    assert np.array_equal(_bits(scratch[0].cpu().numpy()), _bits(ref_comb))
