@@ -1,0 +1,202 @@
+"""Benchmark: Mrays/s of the wavefront path tracer on one MI355X.
+
+Workload (one "step" = one reina_rt_trace of a full frame): displaced
+icosphere, 7 subdivision levels (327,680 triangles) + ground quad + 2 quad
+lights (327,686 triangles), 1,280 Disney material groups, 1920x1080,
+64 samples per pixel, max depth 8, FP32 BVH8. Frames accumulate
+progressively (sampleBatch = frame index), as Reina's render loop does.
+
+value = every traced extension ray (primary + secondary) / wall time of the
+K timed frames, scene already resident in HBM.
+
+roofline: the extend kernel (BVH8 traversal). Algorithmic bytes per ray =
+32 (ray in) + 16 (hit out) + 208 per visited node (useful bytes of the
+256-byte FP32 node: 8 x 24 B boxes + 8 B child bases + 8 B slot metadata)
++ 36 per tested triangle; visits come from the counting build of the same
+kernel on the same frame, kernel time from HIP events around every extend
+launch on the trace's stream.
+
+cpu_baseline: the oracle's C restatement of the reference shaders (BVH
+traversal over the same BVH8), multithreaded over pixels, on a bounded
+sample of the same frame (whole 1-spp passes for about --cpu-seconds).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "reina-vk_amd"))
+
+METRIC = "Mrays/s + % HBM roofline at 1080p; served images/s and p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 208
+TRI_BYTES = 36
+RAY_BYTES = 48
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path tracer has no multi-GPU path here); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import reina_rt
+    from reina_rt import abi, scenes
+    from reina_rt.scene import push_consts
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    W, H = a.width, a.height
+    t0 = time.perf_counter()
+    scene, cam = scenes.icosphere()
+    arrays = scene.build_arrays()
+    dev = reina_rt.DeviceScene(arrays)
+    info = dev.info()
+    setup_s = time.perf_counter() - t0
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda:0")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def frame(batch):
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, a.spp, a.depth, sample_batch=batch)
+        dev.trace(pc, img.data_ptr(), W, H, stream=sp)
+        return dev.stats()
+
+    batch = 0
+    for _ in range(a.warmup):
+        frame(batch)
+        batch += 1
+    torch.cuda.synchronize()
+    rays = 0
+    gpu_ms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        st = frame(batch)
+        rays += st.extension_rays
+        gpu_ms += st.trace_ms
+        batch += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    mrays = rays / elapsed / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(elapsed / max(a.steps, 1) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "displaced icosphere L7 (327,686 tris, 1,280 Disney groups) + ground + 2 quad lights, "
+                        "%dx%d, %d spp, depth %d, FP32 BVH8, progressive frames" % (W, H, a.spp, a.depth),
+            "triangles": int(info.num_triangles),
+            "bvh_nodes": int(info.num_nodes),
+            "bvh_depth": int(info.max_depth),
+            "rays_per_frame": int(rays // max(a.steps, 1)),
+            "gpu_trace_ms_per_frame": round(gpu_ms / max(a.steps, 1), 3),
+            "scene_setup_s": round(setup_s, 2),
+        },
+    }
+
+    if not a.no_roofline:
+        dev.set_trace_options(abi.TRACE_COUNT)
+        stc = frame(batch)
+        dev.set_trace_options(abi.TRACE_TIME_EXTEND)
+        stt = frame(batch)
+        dev.set_trace_options(0)
+        torch.cuda.synchronize()
+        assert stc.extension_rays == stt.extension_rays
+        nrays = stt.extension_rays
+        alg_bytes = nrays * RAY_BYTES + stc.node_visits * NODE_BYTES + stc.tri_tests * TRI_BYTES
+        launches = max(stt.extend_launches, 1)
+        avg_launch_ms = stt.extend_ms / launches
+        achieved = (alg_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
+        out["roofline"] = {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_extend (BVH8 traversal)",
+            "bytes_per_launch": int(alg_bytes // launches),
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "launches": int(launches),
+            "extend_ms_per_frame": round(stt.extend_ms, 3),
+            "extend_share_of_trace": round(stt.extend_ms / max(stt.trace_ms, 1e-9), 3),
+            "nodes_per_ray": round(stc.node_visits / max(nrays, 1), 3),
+            "tris_per_ray": round(stc.tri_tests / max(nrays, 1), 3),
+            "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
+        }
+
+    if a.calib_copy:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a, arrays, dev, cam, W, H)
+    dev.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(a, arrays, dev, cam, W, H):
+    """Oracle restatement timed on the host cores: whole 1-spp passes of the
+    same frame (sampleBatch = pass index, so every pass draws new samples)
+    until the target duration is reached."""
+    import numpy as np
+
+    import oracle
+    from reina_rt.scene import make_desc, push_consts
+
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh())
+    img = np.zeros((H, W, 4), np.float32)
+    passes, rays, secs = 0, 0, 0.0
+    while secs < a.cpu_seconds:
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 1, a.depth, sample_batch=passes)
+        t = time.perf_counter()
+        _, n = orc.render(pc, W, H, img=img, use_bvh=True, nthreads=cores)
+        secs += time.perf_counter() - t
+        rays += n
+        passes += 1
+    return {
+        "value": round(rays / secs / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "%d full %dx%d passes at 1 spp, depth %d (%d extension rays, %.1f s): oracle C restatement "
+                  "of the reference shaders with BVH8 traversal, pthreads over pixels" % (passes, W, H, a.depth,
+                                                                                        rays, secs),
+    }
+
+
+if __name__ == "__main__":
+    main()

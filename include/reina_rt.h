@@ -234,11 +234,20 @@ int reina_rt_trace(ReinaScene* scene, const RtPushConsts* pc, float* image_rgba3
 typedef struct ReinaTraceStats {
     uint64_t extension_rays;  /* every traced extension ray (primary + secondary) */
     uint64_t shadow_rays;     /* NEE shadow/connect rays (0 unless NEE enabled)   */
+    uint64_t node_visits;     /* BVH8 nodes visited (REINA_RT_TRACE_COUNT only)    */
+    uint64_t tri_tests;       /* triangles tested   (REINA_RT_TRACE_COUNT only)    */
     uint32_t iterations;      /* wavefront bounce iterations launched             */
-    uint32_t _pad0;
+    uint32_t extend_launches; /* extend kernel launches                           */
     float    trace_ms;        /* GPU time of the whole trace (events)             */
-    float    extend_ms;       /* GPU time inside the extend kernel                */
+    float    extend_ms;       /* summed extend-kernel time (REINA_RT_TRACE_TIME_EXTEND) */
 } ReinaTraceStats;
+
+/* Trace options (sticky per scene): record a HIP event pair around every
+ * extend launch on the trace's stream, and/or use the counting build of the
+ * extend kernel (per-ray node visits / triangle tests summed on the device). */
+#define REINA_RT_TRACE_TIME_EXTEND 0x1u
+#define REINA_RT_TRACE_COUNT       0x2u
+int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 
 /* Closest-hit query for a batch of rays (device pointers): the traceRayEXT

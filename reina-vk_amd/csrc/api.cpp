@@ -57,6 +57,9 @@ struct ReinaScene {
     uint32_t grid_extend = 0, grid_shade = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool traced = false;
+    uint32_t options = 0;
+    std::vector<hipEvent_t> xev;        // extend event pairs (REINA_RT_TRACE_TIME_EXTEND)
+    uint32_t xev_used = 0;
     uint32_t last_iters = 0;
     hipStream_t last_stream = nullptr;
 };
@@ -265,6 +268,7 @@ int reina_rt_scene_destroy(ReinaScene* s) {
     if (s->h_count) hipHostFree(s->h_count);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    for (hipEvent_t e : s->xev) hipEventDestroy(e);
     if (s->arena) hipFree(s->arena);
     delete s;
     return REINA_RT_OK;
@@ -318,7 +322,7 @@ static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     if (!s->ctrl) {
         HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 64));
         s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
-        s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 16);
+        s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 32);
         HIP_TRY(hipHostMalloc((void**)&s->h_count, 64, hipHostMallocDefault));
         HIP_TRY(hipEventCreate(&s->ev0));
         HIP_TRY(hipEventCreate(&s->ev1));
@@ -370,9 +374,23 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
     uint32_t it = 0;
     const uint32_t K = 4;
+    const bool timex = (s->options & REINA_RT_TRACE_TIME_EXTEND) != 0;
+    const bool count = (s->options & REINA_RT_TRACE_COUNT) != 0;
+    s->xev_used = 0;
     while (it < max_iters) {
         uint32_t slot = it % 3, in_q = it & 1u;
-        HIP_TRY(launch_extend(s->sv, s->Q.ray[in_q], s->Q.hit, s->ctrl, slot, s->stats, s->grid_extend, st));
+        if (timex) {
+            if (s->xev.size() < 2 * (size_t)(s->xev_used + 1)) {
+                hipEvent_t a, b;
+                HIP_TRY(hipEventCreate(&a));
+                HIP_TRY(hipEventCreate(&b));
+                s->xev.push_back(a);
+                s->xev.push_back(b);
+            }
+            HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used], st));
+        }
+        HIP_TRY(launch_extend(s->sv, s->Q.ray[in_q], s->Q.hit, s->ctrl, slot, s->stats, s->grid_extend, count, st));
+        if (timex) HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used++ + 1], st));
         HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->grid_shade, st));
         ++it;
         if (it % K == 0) {
@@ -395,15 +413,32 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
     if (!s->traced) return REINA_RT_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipEventSynchronize(s->ev1));
-    unsigned long long rays = 0;
-    HIP_TRY(hipMemcpy(&rays, s->stats, 8, hipMemcpyDeviceToHost));
+    unsigned long long st3[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpy(st3, s->stats, sizeof st3, hipMemcpyDeviceToHost));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    o->extension_rays = rays;
+    o->extension_rays = st3[0];
     o->shadow_rays = 0;
+    o->node_visits = st3[1];
+    o->tri_tests = st3[2];
     o->iterations = s->last_iters;
+    o->extend_launches = s->last_iters;
     o->trace_ms = ms;
-    o->extend_ms = 0.0f;
+    float xms = 0.0f;
+    for (uint32_t k = 0; k < s->xev_used; ++k) {
+        float e = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&e, s->xev[2 * k], s->xev[2 * k + 1]));
+        xms += e;
+    }
+    o->extend_ms = xms;
+    return REINA_RT_OK;
+}
+
+int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
+    if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
+    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT))
+        return fail(REINA_RT_ERR_INVALID_ARG, "unknown trace option bits");
+    s->options = options;
     return REINA_RT_OK;
 }
 

@@ -43,6 +43,8 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         ctrl[1].count = 0; ctrl[1].fetch_ext = 0; ctrl[1].fetch_shade = 0;
         ctrl[2].count = 0; ctrl[2].fetch_ext = 0; ctrl[2].fetch_shade = 0;
         stats[0] = 0;
+        stats[1] = 0;
+        stats[2] = 0;
     }
     if (i >= npx) return;
     uint32_t x = i % P.W, y = i / P.W;
@@ -67,12 +69,12 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
 
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_extend(SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits,
-                                                Ctrl* ctrl, uint32_t slot, unsigned long long* stats,
-                                                uint32_t* counters) {
+                                                Ctrl* ctrl, uint32_t slot, unsigned long long* stats) {
     __shared__ uint2 stack[kStackDepth * 256];
     uint32_t n = ctrl[slot].count;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && stats) atomicAdd(stats, (unsigned long long)n);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n);
     uint2* stk = stack + threadIdx.x;
+    uint32_t vis = 0, tst = 0;
     for (;;) {
         uint32_t base = wave_claim(&ctrl[slot].fetch_ext);
         if (base >= n) break;
@@ -80,10 +82,21 @@ __global__ __launch_bounds__(256) void k_extend(SceneView sv, const float4* __re
         if (i < n) {
             TRay r;
             tray_init(r, rays[2 * i], rays[2 * i + 1]);
-            uint32_t nv, nt;
+            uint32_t nv = 0, nt = 0;
             HitRec h = trace_closest<COUNT>(sv, r, stk, 256, &nv, &nt);
             hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
-            if (COUNT) { counters[2 * i] = nv; counters[2 * i + 1] = nt; }
+            if (COUNT) { vis += nv; tst += nt; }
+        }
+    }
+    if (COUNT) {
+        // wave reduction, one atomic per wave
+        for (int o = 32; o > 0; o >>= 1) {
+            vis += __shfl_xor(vis, o);
+            tst += __shfl_xor(tst, o);
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(stats + 1, (unsigned long long)vis);
+            atomicAdd(stats + 2, (unsigned long long)tst);
         }
     }
 }
@@ -272,9 +285,11 @@ hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q
 }
 
 hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
-                         unsigned long long* stats, uint32_t grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats,
-                       (uint32_t*)nullptr);
+                         unsigned long long* stats, uint32_t grid, bool count, hipStream_t st) {
+    if (count)
+        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats);
     return hipGetLastError();
 }
 

@@ -40,8 +40,9 @@ struct Queues {
 
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
                        unsigned long long* stats, hipStream_t st);
+// stats[0] += rays in the queue; when counting, stats[1] += node visits, stats[2] += triangle tests
 hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
-                         unsigned long long* stats, uint32_t grid, hipStream_t st);
+                         unsigned long long* stats, uint32_t grid, bool count, hipStream_t st);
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                         uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);

@@ -43,6 +43,7 @@ def lib():
         "reina_rt_scene_export_bvh": [c_p, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64)],
         "reina_rt_trace": [c_p, c_p, c_p, u32, u32, c_p],
         "reina_rt_trace_stats": [c_p, ctypes.POINTER(abi.ReinaTraceStats)],
+        "reina_rt_set_trace_options": [c_p, u32],
         "reina_rt_intersect": [c_p, c_p, c_p, u32, c_p, c_p],
         "reina_rt_postprocess": [c_p, c_p, c_p, u32, u32, c_p, c_p, c_p],
         "reina_rt_debug_math": [i32, c_p, c_p, c_p, u32, c_p],
@@ -68,7 +69,8 @@ def check(rc):
 def exported_symbols():
     return ["reina_rt_abi_version", "reina_rt_last_error", "reina_rt_scene_create", "reina_rt_scene_destroy",
             "reina_rt_scene_info", "reina_rt_scene_export_bvh", "reina_rt_trace", "reina_rt_trace_stats",
-            "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host"]
+            "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
+            "reina_rt_set_trace_options"]
 
 
 class DeviceScene:
@@ -100,6 +102,9 @@ class DeviceScene:
         pcb = np.ascontiguousarray(np.asarray(pc, dtype=abi.PUSH_CONSTS).reshape(()))
         check(lib().reina_rt_trace(self.handle, pcb.ctypes.data, ctypes.c_void_p(image_ptr), width, height,
                                    ctypes.c_void_p(stream) if stream else None))
+
+    def set_trace_options(self, options):
+        check(lib().reina_rt_set_trace_options(self.handle, options))
 
     def stats(self):
         st = abi.ReinaTraceStats()

@@ -81,8 +81,13 @@ class ReinaSceneInfo(ctypes.Structure):
 
 class ReinaTraceStats(ctypes.Structure):
     _fields_ = [("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
-                ("iterations", ctypes.c_uint32), ("_pad0", ctypes.c_uint32),
+                ("node_visits", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64),
+                ("iterations", ctypes.c_uint32), ("extend_launches", ctypes.c_uint32),
                 ("trace_ms", ctypes.c_float), ("extend_ms", ctypes.c_float)]
+
+
+TRACE_TIME_EXTEND = 0x1
+TRACE_COUNT = 0x2
 
 
 # status codes
