@@ -1,0 +1,163 @@
+"""Pins the oracle's closest-hit query: watertight FP32 test vs FP64
+Moller-Trumbore, BVH8 traversal restatement vs brute force (bit for bit),
+BVH8 structure invariants, watertightness on closed meshes."""
+import numpy as np
+import pytest
+
+import oracle
+import reina_rt
+from reina_rt import abi, scenes
+from reina_rt.scene import Material, ModelData, Scene, make_desc
+
+from conftest import world_triangles
+
+
+def soup_arrays(tris):
+    """A one-model scene from an (n, 9) triangle array (identity transform)."""
+    n = len(tris)
+    v = np.concatenate([tris.reshape(-1, 3), np.ones((3 * n, 1), np.float32)], 1).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.uint32)
+    s = Scene()
+    s.add_object(ModelData(v, idx, np.tile(np.eye(3, dtype=np.float32), (3 * n, 1, 1)), idx.copy()), np.eye(4),
+                 Material())
+    return s.build_arrays()
+
+
+def rays_from(o, d, tmin=0.0, tmax=10000.0):
+    r = np.zeros(len(o), abi.RAY)
+    r["ox"], r["oy"], r["oz"] = np.asarray(o, np.float32).T
+    r["dx"], r["dy"], r["dz"] = np.asarray(d, np.float32).T
+    r["tmin"], r["tmax"] = tmin, tmax
+    return r
+
+
+def check_bvh_equals_brute(arrays, rays):
+    orc = oracle.OracleScene(arrays, make_desc)
+    nodes, tris, depth = reina_rt.bvh_build_host(world_triangles(arrays))
+    orc.set_bvh(nodes, tris)
+    hb = orc.brute32(rays)
+    hv, cnt = orc.bvh(rays)
+    for f in ("t", "u", "v"):
+        assert np.array_equal(hb[f].view(np.uint32), hv[f].view(np.uint32)), f
+    assert np.array_equal(hb["prim"], hv["prim"])
+    return hb, cnt, depth
+
+
+def test_bvh_equals_brute_random_soup():
+    rng = np.random.default_rng(1)
+    c = rng.uniform(-1, 1, (3000, 1, 3))
+    tris = (c + 0.08 * rng.normal(size=(3000, 3, 3))).astype(np.float32).reshape(-1, 9)
+    # degenerate, duplicated and axis-aligned triangles
+    tris[:20, 3:6] = tris[:20, 0:3]
+    tris[20:40] = tris[40:60]
+    tris[60:80, 1::3] = 0.25
+    arrays = soup_arrays(tris)
+    rays = rays_from(rng.uniform(-1.5, 1.5, (60000, 3)), rng.normal(size=(60000, 3)))
+    hb, cnt, depth = check_bvh_equals_brute(arrays, rays)
+    assert (hb["prim"] != 0xFFFFFFFF).mean() > 0.3
+    assert depth <= 16
+    assert cnt[:, 0].min() >= 1
+
+
+def test_bvh_equals_brute_scenes(cornell_arrays, icosphere_arrays):
+    arrays, cam = cornell_arrays
+    rng = np.random.default_rng(2)
+    check_bvh_equals_brute(arrays, rays_from(rng.uniform(-1.2, 2.2, (50000, 3)), rng.normal(size=(50000, 3))))
+    arrays, cam = icosphere_arrays
+    o = np.tile(np.array(cam["eye"], np.float32), (1500, 1))
+    d = rng.normal(size=(1500, 3)) * 0.3 + (np.array(cam["center"]) - np.array(cam["eye"]))
+    check_bvh_equals_brute(arrays, rays_from(o, d))
+
+
+def test_bvh_structure():
+    """Every triangle appears once; child boxes contain their triangles and
+    their descendants' boxes; leaves hold 1..3 triangles."""
+    rng = np.random.default_rng(3)
+    tris = rng.uniform(-5, 5, (20000, 9)).astype(np.float32) * 0.1 + rng.uniform(-5, 5, (20000, 1)).astype(np.float32)
+    nodes, tri_out, depth = reina_rt.bvh_build_host(tris)
+    N = len(nodes) // 256
+    nb = nodes.reshape(N, 256)
+    f = nb[:, :192].copy().view(np.float32).reshape(N, 6, 8)   # lo_x hi_x lo_y hi_y lo_z hi_z
+    hdr = nb[:, 192:200].copy().view(np.uint32).reshape(N, 2)
+    meta = nb[:, 200:208]
+    t = tri_out.reshape(-1, 12)
+    ids = t[:, 3].copy().view(np.uint32)
+    assert sorted(ids.tolist()) == list(range(len(tris)))
+    seen_nodes = {0}
+    for n in range(N):
+        cb, tb = hdr[n]
+        for k in range(8):
+            m = int(meta[n, k])
+            if m == 0:
+                continue
+            lo = np.array([f[n, 0, k], f[n, 2, k], f[n, 4, k]])
+            hi = np.array([f[n, 1, k], f[n, 3, k], f[n, 5, k]])
+            if m & 0x80:
+                c = cb + (m & 7)
+                seen_nodes.add(int(c))
+                cf = f[c]
+                valid = meta[c] != 0
+                assert (cf[0][valid] >= lo[0]).all() and (cf[1][valid] <= hi[0]).all()
+                assert (cf[2][valid] >= lo[1]).all() and (cf[3][valid] <= hi[1]).all()
+                assert (cf[4][valid] >= lo[2]).all() and (cf[5][valid] <= hi[2]).all()
+            else:
+                cnt, off = (m >> 5) & 3, m & 31
+                assert 1 <= cnt <= 3
+                for j in range(cnt):
+                    tv = t[tb + off + j][[0, 1, 2, 4, 5, 6, 8, 9, 10]].reshape(3, 3)
+                    assert (tv >= lo - 1e-6).all() and (tv <= hi + 1e-6).all()
+    assert seen_nodes == set(range(N))
+
+
+def test_empty_and_tiny_scenes():
+    for n in (0, 1, 2, 3, 4, 9):
+        rng = np.random.default_rng(n)
+        tris = rng.uniform(-1, 1, (n, 9)).astype(np.float32)
+        nodes, t, depth = reina_rt.bvh_build_host(tris)
+        assert len(nodes) >= 256 and len(t) == 12 * n
+        if n:
+            arrays = soup_arrays(tris)
+            check_bvh_equals_brute(arrays, rays_from(rng.uniform(-2, 2, (2000, 3)), rng.normal(size=(2000, 3))))
+
+
+def test_watertight_closed_mesh():
+    """Rays from inside a closed icosphere (shared edges/vertices everywhere)
+    never escape: the watertight test leaves no crack."""
+    v, f = scenes.subdivided_icosphere(4)
+    tris = v[f].astype(np.float32).reshape(-1, 9)
+    arrays = soup_arrays(tris)
+    rng = np.random.default_rng(4)
+    n = 40000
+    o = rng.uniform(-0.3, 0.3, (n, 3))
+    d = rng.normal(size=(n, 3))
+    # aim a share of rays exactly at mesh vertices and edge midpoints
+    tgt = np.concatenate([v[f[:, 0]], 0.5 * (v[f[:, 0]] + v[f[:, 1]])])[: n // 2]
+    d[: len(tgt)] = tgt - o[: len(tgt)]
+    hb, cnt, _ = check_bvh_equals_brute(arrays, rays_from(o, d))
+    assert (hb["prim"] != 0xFFFFFFFF).all()
+
+
+def test_invalid_rays_miss(cornell_arrays):
+    arrays, _ = cornell_arrays
+    r = rays_from(np.array([[np.nan, 1, 0], [0, 1, 0], [np.inf, 1, 0], [0, 1, 0]]),
+                  np.array([[0, 0, -1], [0, 0, 0], [0, 0, -1], [np.nan, 0, -1]]))
+    orc = oracle.OracleScene(arrays, make_desc)
+    hb = orc.brute32(r)
+    assert (hb["prim"] == 0xFFFFFFFF).all()
+
+
+def test_fp32_vs_fp64_agreement(icosphere_arrays):
+    """On camera rays the FP32 watertight hit id equals the FP64 one except at
+    FP64-detected ties, and t agrees to 1e-6 relative."""
+    arrays, cam = icosphere_arrays
+    rng = np.random.default_rng(5)
+    orc = oracle.OracleScene(arrays, make_desc)
+    o = np.tile(np.array(cam["eye"], np.float32), (400, 1))
+    d = rng.normal(size=(400, 3)) * 0.25 + (np.array(cam["center"]) - np.array(cam["eye"]))
+    r = rays_from(o, d)
+    h = orc.brute32(r)
+    t64, p64, u64, v64, amb = orc.fp64(r)
+    dis = h["prim"] != p64
+    assert (dis & ~amb).sum() == 0
+    m = ~dis & (p64 != 0xFFFFFFFF)
+    assert (np.abs(h["t"][m] - t64[m]) / t64[m]).max() < 1e-6
