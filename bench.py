@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -81,6 +84,22 @@ def main():
         pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, a.spp, a.depth, sample_batch=batch)
         dev.trace(pc, img.data_ptr(), W, H, stream=sp)
         return dev.stats()
+
+    sec = sh = None
+    if not (a.no_secondary and a.no_autotune):
+        sec, sh = build_secondary(a, arrays, dev, cam, W, H, torch)
+    if not a.no_autotune:
+        from reina_rt.autotune import tune_traversal
+        d0 = dev.get_tuning()
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        win, table = tune_traversal(dev, torch, sec.data_ptr(), sh.data_ptr(), sec.shape[0], cus=cus)
+        dflt = next((r for r in table if r["stack_depth"] == d0["stack_depth"] and r["refill_min"] == d0["refill_min"]
+                     and r["blocks_per_cu"] * cus == d0["grid_blocks"]), None)
+        autotune = {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
+                    "secondary_batch_rays": int(sec.shape[0])}
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(os.path.join(ROOT, "gpurun_out", "autotune_table.json"), "w") as f:
+            json.dump(table, f)
 
     batch = 0
     for _ in range(a.warmup):
@@ -123,6 +142,9 @@ def main():
         },
     }
 
+    if not a.no_autotune:
+        out["config"]["autotune"] = autotune
+        out["config"]["tuning"] = dev.get_tuning()
     if not a.no_roofline:
         dev.set_trace_options(abi.TRACE_COUNT)
         stc = frame(batch)
@@ -154,6 +176,10 @@ def main():
             "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
         }
 
+    if not a.no_secondary:
+        out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch)
+        out["secondary_rays"]["tuning"] = dev.get_tuning()
+
     if a.calib_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
         dst = torch.empty_like(src)
@@ -164,6 +190,125 @@ def main():
         out["cpu_baseline"] = cpu_baseline(a, arrays, dev, cam, W, H)
     dev.close()
     print(json.dumps(out), flush=True)
+
+
+def build_secondary(a, arrays, dev, cam, W, H, torch):
+    """Incoherent secondary rays of the same frame: one pinhole primary ray per
+    pixel, closest hit on the GPU, then `--secondary-reps` cosine-distributed
+    directions about the geometric normal of every hit (origin offset 1e-4
+    along the normal). The batch is traced with reina_rt_intersect: HIP
+    events on the stream around the launch give the kernel time, the
+    counting build on the same batch gives node visits and triangle tests."""
+    import numpy as np
+
+    from reina_rt import abi
+    from reina_rt.scene import look_at, perspective
+
+    dev_ = torch.device("cuda:0")
+    sp = torch.cuda.current_stream().cuda_stream
+    inv_p = torch.tensor(np.linalg.inv(perspective(np.radians(cam["fov"]), W / H, 0.1, 100.0).astype(np.float64)),
+                         device=dev_)
+    inv_v = torch.tensor(np.linalg.inv(look_at(cam["eye"], cam["center"], (0, 1, 0)).astype(np.float64)), device=dev_)
+    ys, xs = torch.meshgrid(torch.arange(H, device=dev_, dtype=torch.float64),
+                            torch.arange(W, device=dev_, dtype=torch.float64), indexing="ij")
+    ndc = torch.stack([(xs + 0.5) / W * 2 - 1, -((ys + 0.5) / H * 2 - 1), -torch.ones_like(xs), torch.ones_like(xs)], -1)
+    vp = ndc.reshape(-1, 4) @ inv_p.T
+    vdir = vp[:, :3] / vp[:, 3:4]
+    wdir = torch.nn.functional.normalize(vdir @ inv_v[:3, :3].T, dim=1)
+    eye = torch.tensor(cam["eye"], device=dev_, dtype=torch.float64)
+    n1 = wdir.shape[0]
+    prim = torch.zeros((n1, 8), device=dev_, dtype=torch.float32)
+    prim[:, 0:3] = eye.float()
+    prim[:, 4:7] = wdir.float()
+    prim[:, 7] = 10000.0
+    hits = torch.zeros((n1, 4), device=dev_, dtype=torch.float32)
+    dev.intersect(prim.data_ptr(), hits.data_ptr(), n1, stream=sp)
+    torch.cuda.synchronize()
+    ids = hits[:, 1].contiguous().view(torch.int32)
+    m = ids != -1
+    wt = torch.tensor(world_triangles(arrays), device=dev_)
+    tri = wt[ids[m].long()].reshape(-1, 3, 3)
+    ng = torch.nn.functional.normalize(torch.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0], dim=1), dim=1)
+    d0 = prim[m, 4:7]
+    ng = torch.where(((ng * d0).sum(1) > 0)[:, None], -ng, ng)
+    p = prim[m, 0:3] + hits[m, 0:1] * d0
+    reps = a.secondary_reps
+    g = torch.Generator(device=dev_)
+    g.manual_seed(1234)
+    ng = ng.repeat(reps, 1)
+    p = p.repeat(reps, 1)
+    u1 = torch.rand(ng.shape[0], device=dev_, generator=g)
+    u2 = torch.rand(ng.shape[0], device=dev_, generator=g)
+    r = torch.sqrt(u1)
+    phi = 2 * np.pi * u2
+    helper = torch.where((ng[:, 0].abs() < 0.9)[:, None], torch.tensor([1.0, 0, 0], device=dev_),
+                         torch.tensor([0, 1.0, 0], device=dev_))
+    t1 = torch.nn.functional.normalize(torch.cross(ng, helper, dim=1), dim=1)
+    t2 = torch.cross(ng, t1, dim=1)
+    d = (t1 * (r * torch.cos(phi))[:, None] + t2 * (r * torch.sin(phi))[:, None] +
+         ng * torch.sqrt(torch.clamp(1 - u1, min=0))[:, None])
+    n2 = d.shape[0]
+    sec = torch.zeros((n2, 8), device=dev_, dtype=torch.float32)
+    sec[:, 0:3] = p + 1e-4 * ng
+    sec[:, 4:7] = d
+    sec[:, 7] = 10000.0
+    sh = torch.zeros((n2, 4), device=dev_, dtype=torch.float32)
+    return sec, sh
+
+
+def secondary_rays(a, dev, sec, sh, torch):
+    import numpy as np
+
+    n2 = sec.shape[0]
+    dev_ = sec.device
+    sp = torch.cuda.current_stream().cuda_stream
+    cnt = torch.zeros((n2, 2), device=dev_, dtype=torch.int32)
+    dev.intersect(sec.data_ptr(), sh.data_ptr(), n2, counters_ptr=cnt.data_ptr(), stream=sp)
+    torch.cuda.synchronize()
+    visits = int(cnt[:, 0].double().sum().item())
+    tests = int(cnt[:, 1].double().sum().item())
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps_t = 5
+    dev.intersect(sec.data_ptr(), sh.data_ptr(), n2, stream=sp)   # warm
+    times = []
+    for _ in range(reps_t):
+        e0.record(stream)
+        dev.intersect(sec.data_ptr(), sh.data_ptr(), n2, stream=sp)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    alg = n2 * RAY_BYTES + visits * NODE_BYTES + tests * TRI_BYTES
+    gbs = alg / (ms * 1e-3) / 1e9
+    hit_frac = float((sh[:, 1].contiguous().view(torch.int32) != -1).float().mean().item())
+    return {
+        "rays": int(n2),
+        "grays_per_s": round(n2 / (ms * 1e-3) / 1e9, 3),
+        "kernel_ms": round(ms, 3),
+        "nodes_per_ray": round(visits / n2, 3),
+        "tris_per_ray": round(tests / n2, 3),
+        "bytes_per_ray": round(alg / n2, 1),
+        "achieved_GBs": round(gbs, 1),
+        "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+        "hit_fraction": round(hit_frac, 3),
+        "note": "cosine-distributed bounce rays from the primary hits of the same 1080p frame, "
+                "traced by reina_rt_intersect (same traversal kernel as the frame's extend)",
+    }
+
+
+def world_triangles(arrays):
+    import numpy as np
+    out = []
+    for inst in arrays["instances"]:
+        T = inst["transform"].reshape(4, 4).T.astype(np.float32)
+        m = arrays["models"][inst["modelID"]]
+        idx = arrays["indices"][m["indexOffset"]:m["indexOffset"] + 3 * m["indexCount"]]
+        v = arrays["vertices"][idx][:, :3].astype(np.float32)
+        c0, c1, c2, c3 = T[:3, 0], T[:3, 1], T[:3, 2], T[:3, 3]
+        w = ((c0[None] * v[:, 0:1] + c1[None] * v[:, 1:2]) + c2[None] * v[:, 2:3]) + c3[None]
+        out.append(w.astype(np.float32).reshape(-1, 9))
+    return np.concatenate(out)
 
 
 def cpu_baseline(a, arrays, dev, cam, W, H):

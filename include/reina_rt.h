@@ -250,6 +250,18 @@ typedef struct ReinaTraceStats {
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 
+/* Launch parameters of the traversal / shading kernels. Defaults are set at
+ * scene creation (stack depth from the BVH depth); the autotuner in the
+ * Python package searches them per scene. */
+typedef struct ReinaTuning {
+    uint32_t stack_depth;   /* LDS stack entries per lane: 8, 16, 24 or 32 (>= BVH depth - 1) */
+    uint32_t refill_min;    /* idle lanes per wave that trigger a refill, 1..64                */
+    uint32_t grid_blocks;   /* persistent traversal blocks (256 threads)                       */
+    uint32_t shade_blocks;  /* shade blocks (256 threads)                                       */
+} ReinaTuning;
+int reina_rt_get_tuning(const ReinaScene* scene, ReinaTuning* out);
+int reina_rt_set_tuning(ReinaScene* scene, const ReinaTuning* tuning);
+
 /* Closest-hit query for a batch of rays (device pointers): the traceRayEXT
  * of shaders/raytrace/raytrace.rgen.glsl:110-122 as a stand-alone kernel.
  * If counters != NULL (device uint32[2*n]) the per-ray node visits and

@@ -54,7 +54,8 @@ struct ReinaScene {
     unsigned long long* stats = nullptr;
     uint32_t* fetch = nullptr;
     uint32_t* h_count = nullptr;        // pinned
-    uint32_t grid_extend = 0, grid_shade = 0;
+    Tuning tune{16, 16, 0, 0};
+    uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool traced = false;
     uint32_t options = 0;
@@ -222,10 +223,10 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
         delete s;
         return fail(REINA_RT_ERR_INVALID_ARG, err);
     }
-    if (bvh.max_depth > kStackDepth) {
+    if (bvh.max_depth > kMaxStackDepth + 1) {
         delete s;
         return fail(REINA_RT_ERR_UNSUPPORTED, "BVH8 depth " + std::to_string(bvh.max_depth) +
-                                                  " exceeds the traversal stack (" + std::to_string(kStackDepth) + ")");
+                                                  " exceeds the traversal stack (" + std::to_string(kMaxStackDepth) + ")");
     }
     s->host_nodes = std::move(bvh.nodes);
     s->host_tris = std::move(bvh.tris);
@@ -247,8 +248,12 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, s->device));
     uint32_t cus = (uint32_t)prop.multiProcessorCount;
-    s->grid_extend = cus * 4;
-    s->grid_shade = cus * 4;
+    uint32_t need = bvh.max_depth > 1 ? bvh.max_depth - 1 : 1;
+    s->tune.stack_depth = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
+    s->tune.refill_min = 16;
+    s->tune.grid = cus * (s->tune.stack_depth <= 8 ? 8 : 4);
+    s->tune.shade_grid = cus * 4;
+    s->min_stack = s->tune.stack_depth;
     *out = s;
     return REINA_RT_OK;
 }
@@ -389,9 +394,10 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
             }
             HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used], st));
         }
-        HIP_TRY(launch_extend(s->sv, s->Q.ray[in_q], s->Q.hit, s->ctrl, slot, s->stats, s->grid_extend, count, st));
+        HIP_TRY(launch_trace(s->sv, s->Q.ray[in_q], s->Q.hit, &s->ctrl[slot].count, 0, &s->ctrl[slot].fetch_ext,
+                             s->stats, nullptr, count, s->tune, st));
         if (timex) HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used++ + 1], st));
-        HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->grid_shade, st));
+        HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->tune.shade_grid, st));
         ++it;
         if (it % K == 0) {
             HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count, 4, hipMemcpyDeviceToHost, st));
@@ -434,6 +440,32 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
     return REINA_RT_OK;
 }
 
+int reina_rt_get_tuning(const ReinaScene* s, ReinaTuning* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    o->stack_depth = s->tune.stack_depth;
+    o->refill_min = s->tune.refill_min;
+    o->grid_blocks = s->tune.grid;
+    o->shade_blocks = s->tune.shade_grid;
+    return REINA_RT_OK;
+}
+
+int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
+    if (!s || !t) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    uint32_t sd = t->stack_depth;
+    if (sd != 8 && sd != 16 && sd != 24 && sd != 32)
+        return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");
+    if (sd < s->min_stack)
+        return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth below what this BVH needs (" + std::to_string(s->min_stack) + ")");
+    if (t->refill_min < 1 || t->refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be in 1..64");
+    if (t->grid_blocks < 1 || t->grid_blocks > 65536 || t->shade_blocks < 1 || t->shade_blocks > 65536)
+        return fail(REINA_RT_ERR_INVALID_ARG, "block counts must be in 1..65536");
+    s->tune.stack_depth = sd;
+    s->tune.refill_min = t->refill_min;
+    s->tune.grid = t->grid_blocks;
+    s->tune.shade_grid = t->shade_blocks;
+    return REINA_RT_OK;
+}
+
 int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
     if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
     if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT))
@@ -451,7 +483,8 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
     if (rc) return rc;
     if (n == 0) return REINA_RT_OK;
     HIP_TRY(hipMemsetAsync(s->fetch, 0, 4, st));
-    HIP_TRY(launch_intersect(s->sv, (const float4*)rays, (float4*)hits, n, counters, s->fetch, s->grid_extend, st));
+    HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, nullptr, n, s->fetch, nullptr, counters,
+                         counters != nullptr, s->tune, st));
     return REINA_RT_OK;
 }
 

@@ -67,36 +67,98 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
     Q.pix[0][i] = i;
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_extend(SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits,
-                                                Ctrl* ctrl, uint32_t slot, unsigned long long* stats) {
-    __shared__ uint2 stack[kStackDepth * 256];
-    uint32_t n = ctrl[slot].count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n);
+// Closest-hit kernel over a ray array, persistent with per-lane refill: a
+// lane whose ray is finished goes idle; once at least `refill_min` lanes of
+// the wave are idle they take the next rays from the wave's private chunk of
+// the queue (claimed 128 at a time with one global atomic), so waves stay
+// full while the queue lasts instead of waiting for their slowest ray.
+// Each loop iteration advances every active lane by one node visit.
+//   n = n_ptr ? *n_ptr : n_imm rays; fetch: the global claim counter.
+//   stats (optional): stats[0] += n; COUNT: stats[1..2] += node visits / tests.
+//   counters (optional, COUNT): per-ray {visits, tests}.
+template <int SD, bool COUNT>
+__global__ __launch_bounds__(256) void k_trace(SceneView sv, const float4* __restrict__ rays,
+                                               float4* __restrict__ hits, const uint32_t* n_ptr, uint32_t n_imm,
+                                               uint32_t* fetch, unsigned long long* stats, uint32_t* counters,
+                                               uint32_t refill_min) {
+    __shared__ uint2 stack[SD * 256];
+    constexpr uint32_t kChunk = 128;
+    const uint32_t n = n_ptr ? *n_ptr : n_imm;
+    if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n);
     uint2* stk = stack + threadIdx.x;
-    uint32_t vis = 0, tst = 0;
+    const uint32_t lane = __lane_id();
+    const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    bool active = false;
+    bool exhausted = false;           // wave-uniform
+    uint32_t pool_next = 0, pool_end = 0;   // wave-uniform
+    uint32_t idx = 0;
+    TRay r;
+    HitRec h;
+    TravState ts;
+    uint32_t nv = 0, nt = 0, wvis = 0, wtst = 0;
     for (;;) {
-        uint32_t base = wave_claim(&ctrl[slot].fetch_ext);
-        if (base >= n) break;
-        uint32_t i = base + __lane_id();
-        if (i < n) {
-            TRay r;
-            tray_init(r, rays[2 * i], rays[2 * i + 1]);
-            uint32_t nv = 0, nt = 0;
-            HitRec h = trace_closest<COUNT>(sv, r, stk, 256, &nv, &nt);
-            hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
-            if (COUNT) { vis += nv; tst += nt; }
+        uint64_t idle = __ballot(!active);
+        uint32_t nidle = (uint32_t)__popcll(idle);
+        if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
+            if (pool_next >= pool_end) {
+                uint32_t b = 0;
+                if (lane == 0) b = atomicAdd(fetch, kChunk);
+                b = __shfl(b, 0);
+                if (b >= n) {
+                    exhausted = true;
+                } else {
+                    pool_next = b;
+                    pool_end = min(b + kChunk, n);
+                }
+            }
+            if (!exhausted) {
+                uint32_t avail = pool_end - pool_next;
+                uint32_t take = min(nidle, avail);
+                if (!active) {
+                    uint32_t rank = (uint32_t)__popcll(idle & ltmask);
+                    if (rank < take) {
+                        idx = pool_next + rank;
+                        tray_init(r, rays[2 * idx], rays[2 * idx + 1]);
+                        h = miss_record(r);
+                        ts.cur = 0;
+                        ts.sp = 0;
+                        nv = 0;
+                        nt = 0;
+                        active = true;
+                        if (!r.valid || sv.num_nodes == 0) {
+                            hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+                            if (COUNT && counters) { counters[2 * idx] = 0; counters[2 * idx + 1] = 0; }
+                            active = false;
+                        }
+                    }
+                }
+                pool_next += take;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            if (!trav_step<COUNT>(sv, r, h, ts, stk, 256, nv, nt)) {
+                hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+                if (COUNT) {
+                    if (counters) { counters[2 * idx] = nv; counters[2 * idx + 1] = nt; }
+                    wvis += nv;
+                    wtst += nt;
+                }
+                active = false;
+            }
         }
     }
-    if (COUNT) {
-        // wave reduction, one atomic per wave
+    if (COUNT && stats) {
         for (int o = 32; o > 0; o >>= 1) {
-            vis += __shfl_xor(vis, o);
-            tst += __shfl_xor(tst, o);
+            wvis += __shfl_xor(wvis, o);
+            wtst += __shfl_xor(wtst, o);
         }
-        if (__lane_id() == 0) {
-            atomicAdd(stats + 1, (unsigned long long)vis);
-            atomicAdd(stats + 2, (unsigned long long)tst);
+        if (lane == 0) {
+            atomicAdd(stats + 1, (unsigned long long)wvis);
+            atomicAdd(stats + 2, (unsigned long long)wtst);
         }
     }
 }
@@ -231,27 +293,6 @@ __global__ __launch_bounds__(256) void k_finish(TraceParams P, PathState S, floa
     img[i] = make_float4(fin.x, fin.y, fin.z, 1.0f);
 }
 
-// Stand-alone closest-hit query over a ReinaRay batch.
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_intersect(SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits,
-                                                   uint32_t n, uint32_t* counters, uint32_t* fetch) {
-    __shared__ uint2 stack[kStackDepth * 256];
-    uint2* stk = stack + threadIdx.x;
-    for (;;) {
-        uint32_t base = wave_claim(fetch);
-        if (base >= n) break;
-        uint32_t i = base + __lane_id();
-        if (i < n) {
-            TRay r;
-            tray_init(r, rays[2 * i], rays[2 * i + 1]);
-            uint32_t nv, nt;
-            HitRec h = trace_closest<COUNT>(sv, r, stk, 256, &nv, &nt);
-            hits[i] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
-            if (COUNT) { counters[2 * i] = nv; counters[2 * i + 1] = nt; }
-        }
-    }
-}
-
 __global__ void k_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -284,12 +325,28 @@ hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q
     return hipGetLastError();
 }
 
-hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
-                         unsigned long long* stats, uint32_t grid, bool count, hipStream_t st) {
+template <int SD>
+static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr,
+                            uint32_t n_imm, uint32_t* fetch, unsigned long long* stats, uint32_t* counters,
+                            bool count, const Tuning& tu, hipStream_t st) {
+    dim3 g(tu.grid), b(256);
     if (count)
-        hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats);
+        hipLaunchKernelGGL((k_trace<SD, true>), g, b, 0, st, sv, rays, hits, n_ptr, n_imm, fetch, stats, counters,
+                           tu.refill_min);
     else
-        hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, ctrl, slot, stats);
+        hipLaunchKernelGGL((k_trace<SD, false>), g, b, 0, st, sv, rays, hits, n_ptr, n_imm, fetch, stats, counters,
+                           tu.refill_min);
+}
+
+hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr, uint32_t n_imm,
+                        uint32_t* fetch, unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                        hipStream_t st) {
+    switch (tu.stack_depth) {
+        case 8: launch_trace_sd<8>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
+        case 16: launch_trace_sd<16>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
+        case 24: launch_trace_sd<24>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
+        default: launch_trace_sd<32>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
+    }
     return hipGetLastError();
 }
 
@@ -304,15 +361,6 @@ hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, 
     uint32_t blocks = (npx + 255) / 256;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finish, dim3(blocks), dim3(256), 0, st, P, S, img);
-    return hipGetLastError();
-}
-
-hipError_t launch_intersect(const SceneView& sv, const float4* rays, float4* hits, uint32_t n, uint32_t* counters,
-                            uint32_t* fetch, uint32_t grid, hipStream_t st) {
-    if (counters)
-        hipLaunchKernelGGL(k_intersect<true>, dim3(grid), dim3(256), 0, st, sv, rays, hits, n, counters, fetch);
-    else
-        hipLaunchKernelGGL(k_intersect<false>, dim3(grid), dim3(256), 0, st, sv, rays, hits, n, counters, fetch);
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ static_assert(sizeof(Bvh8Node) == 256, "node size");
 constexpr uint32_t kNodeUsefulBytes = 208;
 constexpr uint32_t kTriUsefulBytes = 36;   // 3 x float3 per tested triangle
 constexpr uint32_t kMaxLeafTris = 3;
-constexpr uint32_t kStackDepth = 16;       // LDS group-stack entries per lane
+constexpr uint32_t kMaxStackDepth = 32;    // largest LDS group stack per lane (entries)
 
 // Triangles in leaf order: 3 x float4 = (v0.xyz, id bits), (v1.xyz, 0), (v2.xyz, 0).
 

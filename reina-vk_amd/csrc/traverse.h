@@ -143,85 +143,87 @@ RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, uint32_t meta_lo, u
     return hitm;
 }
 
-// Closest hit. `stk` points at this lane's first LDS stack entry; entries are
-// `stride` uint2 apart. COUNT: also return node visits / triangle tests.
+// Traversal state of one lane between node visits.
+struct TravState {
+    uint32_t cur;   // next node to visit
+    int sp;         // LDS stack depth
+};
+
+// One node visit: box tests of the 8 children, the triangles of every hit
+// leaf, then the next node (first child of this node's group, else the top
+// group of the stack). Returns false when the traversal is finished.
+// `stk` points at this lane's first LDS stack entry; entries are `stride`
+// uint2 apart.
 template <bool COUNT>
-RT_D HitRec trace_closest(const SceneView& sv, const TRay& r, uint2* stk, uint32_t stride,
-                          uint32_t* nvis_out, uint32_t* ntst_out) {
+RT_D bool trav_step(const SceneView& sv, const TRay& r, HitRec& h, TravState& ts, uint2* stk, uint32_t stride,
+                    uint32_t& nvis, uint32_t& ntst) {
+    const Bvh8Node* node = sv.nodes + ts.cur;
+    const uint4 hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
+    const uint32_t child_base = hdr.x, tri_base = hdr.y, meta_lo = hdr.z, meta_hi = hdr.w;
+    if (COUNT) nvis++;
+    uint32_t hitm = node_hits(r, node, meta_lo, meta_hi, h.t);
+    // internal-slot mask (bit 7 of each meta byte) and the ordered hit masks
+    uint32_t imask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+        imask |= ((m >> 7) & 1u) << k;
+    }
+    uint32_t om = 0, lm = 0;
+    if (hitm) {
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i) {
+            uint32_t k = i ^ r.oct;
+            uint32_t hb = (hitm >> k) & 1u;
+            uint32_t ib = (imask >> k) & 1u;
+            om |= (hb & ib) << i;
+            lm |= (hb & (ib ^ 1u)) << i;
+        }
+    }
+    // leaves of this node, in slot order i ^ oct
+    while (lm) {
+        uint32_t i = __builtin_ctz(lm);
+        lm &= lm - 1u;
+        uint32_t k = i ^ r.oct;
+        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
+        uint32_t cnt = (m >> 5) & 3u, off = m & 31u;
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const float4* tv = sv.tris + 3ull * (tri_base + off + j);
+            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+            uint32_t id = __float_as_uint(p0.w);
+            float t, u, v;
+            if (COUNT) ntst++;
+            if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
+                h.t = t; h.prim = id; h.u = u; h.v = v;
+            }
+        }
+    }
+    if (om) {
+        uint32_t i = __builtin_ctz(om);
+        om &= om - 1u;
+        uint32_t k = i ^ r.oct;
+        if (om) { stk[ts.sp * stride] = make_uint2(child_base, imask | (om << 8)); ts.sp++; }
+        ts.cur = child_base + __builtin_popcount(imask & ((1u << k) - 1u));
+        return true;
+    }
+    if (ts.sp == 0) return false;
+    uint2 g = stk[(ts.sp - 1) * stride];
+    uint32_t gm = g.y >> 8, gi = g.y & 0xFFu;
+    uint32_t i = __builtin_ctz(gm);
+    gm &= gm - 1u;
+    uint32_t k = i ^ r.oct;
+    ts.cur = g.x + __builtin_popcount(gi & ((1u << k) - 1u));
+    if (gm) stk[(ts.sp - 1) * stride] = make_uint2(g.x, gi | (gm << 8));
+    else ts.sp--;
+    return true;
+}
+
+RT_D HitRec miss_record(const TRay& r) {
     HitRec h;
     h.t = r.tmax;
     h.prim = 0xFFFFFFFFu;
     h.u = 0.0f;
     h.v = 0.0f;
-    uint32_t nvis = 0, ntst = 0;
-    if (!r.valid || sv.num_nodes == 0) {
-        if (COUNT) { *nvis_out = 0; *ntst_out = 0; }
-        return h;
-    }
-    uint32_t cur = 0;
-    int sp = 0;
-    for (;;) {
-        const Bvh8Node* node = sv.nodes + cur;
-        const uint4 hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
-        const uint32_t child_base = hdr.x, tri_base = hdr.y, meta_lo = hdr.z, meta_hi = hdr.w;
-        if (COUNT) nvis++;
-        uint32_t hitm = node_hits(r, node, meta_lo, meta_hi, h.t);
-        // internal-slot mask and ordered masks
-        uint32_t imask = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
-            imask |= ((m >> 7) & 1u) << k;
-        }
-        uint32_t om = 0, lm = 0;
-        if (hitm) {
-#pragma unroll
-            for (uint32_t i = 0; i < 8; ++i) {
-                uint32_t k = i ^ r.oct;
-                uint32_t hb = (hitm >> k) & 1u;
-                uint32_t ib = (imask >> k) & 1u;
-                om |= (hb & ib) << i;
-                lm |= (hb & (ib ^ 1u)) << i;
-            }
-        }
-        // leaves of this node, ordered
-        while (lm) {
-            uint32_t i = __builtin_ctz(lm);
-            lm &= lm - 1u;
-            uint32_t k = i ^ r.oct;
-            uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
-            uint32_t cnt = (m >> 5) & 3u, off = m & 31u;
-            for (uint32_t j = 0; j < cnt; ++j) {
-                const float4* tv = sv.tris + 3ull * (tri_base + off + j);
-                float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
-                uint32_t id = __float_as_uint(p0.w);
-                float t, u, v;
-                if (COUNT) ntst++;
-                if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
-                    h.t = t; h.prim = id; h.u = u; h.v = v;
-                }
-            }
-        }
-        // next node: first child of this node's group, else pop
-        if (om) {
-            uint32_t i = __builtin_ctz(om);
-            om &= om - 1u;
-            uint32_t k = i ^ r.oct;
-            if (om) { stk[sp * stride] = make_uint2(child_base, imask | (om << 8)); sp++; }
-            cur = child_base + __builtin_popcount(imask & ((1u << k) - 1u));
-            continue;
-        }
-        if (sp == 0) break;
-        uint2 g = stk[(sp - 1) * stride];
-        uint32_t gm = g.y >> 8, gi = g.y & 0xFFu;
-        uint32_t i = __builtin_ctz(gm);
-        gm &= gm - 1u;
-        uint32_t k = i ^ r.oct;
-        cur = g.x + __builtin_popcount(gi & ((1u << k) - 1u));
-        if (gm) stk[(sp - 1) * stride] = make_uint2(g.x, gi | (gm << 8));
-        else sp--;
-    }
-    if (COUNT) { *nvis_out = nvis; *ntst_out = ntst; }
     return h;
 }
 

@@ -40,14 +40,23 @@ struct Queues {
 
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
                        unsigned long long* stats, hipStream_t st);
-// stats[0] += rays in the queue; when counting, stats[1] += node visits, stats[2] += triangle tests
-hipError_t launch_extend(const SceneView& sv, const float4* rays, float4* hits, Ctrl* ctrl, uint32_t slot,
-                         unsigned long long* stats, uint32_t grid, bool count, hipStream_t st);
+// Launch parameters of the traversal kernel (chosen per scene; see the autotuner).
+struct Tuning {
+    uint32_t stack_depth;   // LDS group-stack entries per lane: 8, 16, 24 or 32 (>= BVH depth - 1)
+    uint32_t refill_min;    // idle lanes of a wave that trigger a refill (1..64)
+    uint32_t grid;          // persistent blocks of 256 threads
+    uint32_t shade_grid;    // blocks of the shade kernel
+};
+
+// Closest hit for n = n_ptr ? *n_ptr : n_imm rays. stats (optional):
+// stats[0] += n; when counting also stats[1] += node visits, stats[2] +=
+// triangle tests. counters (optional, counting): per-ray {visits, tests}.
+hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr, uint32_t n_imm,
+                        uint32_t* fetch, unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                        hipStream_t st);
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                         uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);
-hipError_t launch_intersect(const SceneView& sv, const float4* rays, float4* hits, uint32_t n, uint32_t* counters,
-                            uint32_t* fetch, uint32_t grid, hipStream_t st);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,
                               const BloomPushConsts& b, const TonemappingPushConsts& t, hipStream_t st);

@@ -44,6 +44,8 @@ def lib():
         "reina_rt_trace": [c_p, c_p, c_p, u32, u32, c_p],
         "reina_rt_trace_stats": [c_p, ctypes.POINTER(abi.ReinaTraceStats)],
         "reina_rt_set_trace_options": [c_p, u32],
+        "reina_rt_get_tuning": [c_p, ctypes.POINTER(abi.ReinaTuning)],
+        "reina_rt_set_tuning": [c_p, ctypes.POINTER(abi.ReinaTuning)],
         "reina_rt_intersect": [c_p, c_p, c_p, u32, c_p, c_p],
         "reina_rt_postprocess": [c_p, c_p, c_p, u32, u32, c_p, c_p, c_p],
         "reina_rt_debug_math": [i32, c_p, c_p, c_p, u32, c_p],
@@ -70,7 +72,7 @@ def exported_symbols():
     return ["reina_rt_abi_version", "reina_rt_last_error", "reina_rt_scene_create", "reina_rt_scene_destroy",
             "reina_rt_scene_info", "reina_rt_scene_export_bvh", "reina_rt_trace", "reina_rt_trace_stats",
             "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
-            "reina_rt_set_trace_options"]
+            "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning"]
 
 
 class DeviceScene:
@@ -102,6 +104,18 @@ class DeviceScene:
         pcb = np.ascontiguousarray(np.asarray(pc, dtype=abi.PUSH_CONSTS).reshape(()))
         check(lib().reina_rt_trace(self.handle, pcb.ctypes.data, ctypes.c_void_p(image_ptr), width, height,
                                    ctypes.c_void_p(stream) if stream else None))
+
+    def get_tuning(self):
+        t = abi.ReinaTuning()
+        check(lib().reina_rt_get_tuning(self.handle, ctypes.byref(t)))
+        return dict(stack_depth=t.stack_depth, refill_min=t.refill_min, grid_blocks=t.grid_blocks,
+                    shade_blocks=t.shade_blocks)
+
+    def set_tuning(self, **kw):
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = abi.ReinaTuning(cur["stack_depth"], cur["refill_min"], cur["grid_blocks"], cur["shade_blocks"])
+        check(lib().reina_rt_set_tuning(self.handle, ctypes.byref(t)))
 
     def set_trace_options(self, options):
         check(lib().reina_rt_set_trace_options(self.handle, options))

@@ -86,6 +86,11 @@ class ReinaTraceStats(ctypes.Structure):
                 ("trace_ms", ctypes.c_float), ("extend_ms", ctypes.c_float)]
 
 
+class ReinaTuning(ctypes.Structure):
+    _fields_ = [("stack_depth", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
+                ("grid_blocks", ctypes.c_uint32), ("shade_blocks", ctypes.c_uint32)]
+
+
 TRACE_TIME_EXTEND = 0x1
 TRACE_COUNT = 0x2
 
