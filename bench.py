@@ -94,7 +94,7 @@ def main():
         cus = torch.cuda.get_device_properties(0).multi_processor_count
         win, table = tune_traversal(dev, torch, sec.data_ptr(), sh.data_ptr(), sec.shape[0], cus=cus)
         dflt = next((r for r in table if r["stack_depth"] == d0["stack_depth"] and r["refill_min"] == d0["refill_min"]
-                     and r["blocks_per_cu"] * cus == d0["grid_blocks"]), None)
+                     and r["blocks_per_cu"] * cus == d0["grid_blocks"] and r["waves_per_simd"] == 0), None)
         autotune = {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
                     "secondary_batch_rays": int(sec.shape[0])}
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)

@@ -257,7 +257,8 @@ typedef struct ReinaTuning {
     uint32_t stack_depth;   /* LDS stack entries per lane: 8, 16, 24 or 32 (>= BVH depth - 1) */
     uint32_t refill_min;    /* idle lanes per wave that trigger a refill, 1..64                */
     uint32_t grid_blocks;   /* persistent traversal blocks (256 threads)                       */
-    uint32_t shade_blocks;  /* shade blocks (256 threads)                                       */
+    uint32_t shade_blocks;  /* shade blocks (256 threads), a multiple of 8                      */
+    uint32_t waves_per_simd;/* traversal register budget: 0 (compiler's choice), 5, 6 or 8      */
 } ReinaTuning;
 int reina_rt_get_tuning(const ReinaScene* scene, ReinaTuning* out);
 int reina_rt_set_tuning(ReinaScene* scene, const ReinaTuning* tuning);

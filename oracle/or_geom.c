@@ -200,7 +200,9 @@ static void brute32_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit) {
 
 /* ---- BVH8 traversal restatement ------------------------------------------
  * Node layout (256 bytes): float lo_x[8], hi_x[8], lo_y[8], hi_y[8], lo_z[8],
- * hi_z[8]; u32 child_base; u32 tri_base; u8 meta[8]; 48 bytes padding.
+ * hi_z[8]; u32 child_base | imask << 24; u32 tri_base | leafmask << 24;
+ * u8 meta[8]; 48 bytes padding (imask / leafmask: slots holding internal
+ * children / leaves).
  * meta 0 = empty slot; 0x80|rank = internal child child_base+rank; otherwise a
  * leaf of (meta>>5)&3 triangles starting at tri_base + (meta & 31).
  * Triangles: float4 x3 (v0.xyz|id, v1.xyz|-, v2.xyz|-).
@@ -236,13 +238,16 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
     while (have) {
         const uint8_t* nb = s->nodes + (uint64_t)cur * 256;
         const float* f = (const float*)nb;
-        uint32_t child_base, tri_base;
-        memcpy(&child_base, nb + 192, 4);
-        memcpy(&tri_base, nb + 196, 4);
+        uint32_t w48, w49;
+        memcpy(&w48, nb + 192, 4);
+        memcpy(&w49, nb + 196, 4);
+        uint32_t child_base = w48 & 0xFFFFFFu, tri_base = w49 & 0xFFFFFFu;
+        uint32_t imask = w48 >> 24, leafmask = w49 >> 24;
         const uint8_t* meta = nb + 200;
         nvis++;
         uint32_t hitm = 0;
-        /* near/far plane per axis by direction sign (lo <= hi, so this is min/max) */
+        /* near/far plane per axis by direction sign (lo <= hi, so this is min/max);
+         * the interval is widened by 2^-16 |t| with one fused multiply-add */
         int sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
         for (int k = 0; k < 8; ++k) {
             float nx = sx ? f[0 + k] : f[8 + k], fx = sx ? f[8 + k] : f[0 + k];
@@ -250,19 +255,17 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
             float nz = sz ? f[32 + k] : f[40 + k], fz = sz ? f[40 + k] : f[32 + k];
             float tn = fmaxf(fmaxf((nx - r.o.x) * ix, (ny - r.o.y) * iy), (nz - r.o.z) * iz);
             float tf = fminf(fminf((fx - r.o.x) * ix, (fy - r.o.y) * iy), (fz - r.o.z) * iz);
-            tn = tn - fabsf(tn) * BOX_EPS;
-            tf = tf + fabsf(tf) * BOX_EPS;
-            tn = fmaxf(tn, r.tmin);
-            tf = fminf(tf, tb);
-            if (meta[k] != 0 && tn <= tf) hitm |= 1u << k;
+            tn = fmaxf(fmaf(-fabsf(tn), BOX_EPS, tn), r.tmin);
+            tf = fminf(fmaf(fabsf(tf), BOX_EPS, tf), tb);
+            if (tn <= tf) hitm |= 1u << k;
         }
-        uint32_t imask = 0, om = 0;
+        uint32_t om = 0;
         for (uint32_t i = 0; i < 8; ++i) {
             uint32_t k = i ^ oct;
-            uint8_t m = meta[k];
-            if (m & 0x80) imask |= 1u << k;
             if (!(hitm & (1u << k))) continue;
-            if (m & 0x80) { om |= 1u << i; continue; }
+            if (imask & (1u << k)) { om |= 1u << i; continue; }
+            if (!(leafmask & (1u << k))) continue;
+            uint8_t m = meta[k];
             uint32_t cntl = (m >> 5) & 3u, off = m & 31u;
             for (uint32_t j = 0; j < cntl; ++j) {
                 const float* tv = s->btris + 12ull * (tri_base + off + j);

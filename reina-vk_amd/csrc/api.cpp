@@ -54,7 +54,7 @@ struct ReinaScene {
     unsigned long long* stats = nullptr;
     uint32_t* fetch = nullptr;
     uint32_t* h_count = nullptr;        // pinned
-    Tuning tune{16, 16, 0, 0};
+    Tuning tune{16, 16, 0, 0, 0};
     uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool traced = false;
@@ -197,7 +197,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
         r.tri_base = (uint32_t)ntri;
         ntri += d->models[id.modelID].indexCount;
     }
-    if (ntri >= (1ull << 27)) return fail(REINA_RT_ERR_INVALID_ARG, "more than 2^27 triangles");
+    if (ntri >= (1ull << 24)) return fail(REINA_RT_ERR_INVALID_ARG, "more than 2^24 triangles in one scene");
     std::vector<float> wtri(9 * ntri);
     std::vector<uint32_t> tri_inst(ntri), tri_prim(ntri);
     for (uint32_t i = 0; i < d->num_instances; ++i) {
@@ -325,17 +325,18 @@ int reina_rt_bvh_build_host(const float* tri, uint64_t ntri, void* nodes, uint64
 
 static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     if (!s->ctrl) {
-        HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 64));
+        // 3 iteration slots, the stats words, and the claim counters of reina_rt_intersect
+        HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 256 + kParts * kLine * 4));
         s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
-        s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 32);
-        HIP_TRY(hipHostMalloc((void**)&s->h_count, 64, hipHostMallocDefault));
+        s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 256);
+        HIP_TRY(hipHostMalloc((void**)&s->h_count, sizeof(Ctrl), hipHostMallocDefault));
         HIP_TRY(hipEventCreate(&s->ev0));
         HIP_TRY(hipEventCreate(&s->ev1));
     }
     if (npx <= s->cap_px) return REINA_RT_OK;
     free_trace_buffers(s);
-    uint64_t per = 64 /*state*/ + 64 /*2 ray queues*/ + 8 /*2 pix queues*/ + 16 /*hits*/;
-    uint64_t bytes = npx * per + 8 * 256;
+    uint64_t qn = (uint64_t)kParts * part_capacity((uint32_t)npx);   // queue entries (partitioned)
+    uint64_t bytes = npx * 64 /*state*/ + qn * (64 /*2 ray queues*/ + 8 /*2 pix queues*/ + 16 /*hits*/) + 16 * 256;
     hipError_t e = hipMalloc(&s->tbuf, bytes);
     if (e != hipSuccess) return fail(REINA_RT_ERR_OUT_OF_MEMORY, "trace buffers: hipMalloc failed");
     char* p = (char*)s->tbuf;
@@ -344,11 +345,11 @@ static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     s->S.st1 = (float4*)take(npx * 16);
     s->S.st2 = (float4*)take(npx * 16);
     s->S.st3 = (float4*)take(npx * 16);
-    s->Q.ray[0] = (float4*)take(npx * 32);
-    s->Q.ray[1] = (float4*)take(npx * 32);
-    s->Q.pix[0] = (uint32_t*)take(npx * 4);
-    s->Q.pix[1] = (uint32_t*)take(npx * 4);
-    s->Q.hit = (float4*)take(npx * 16);
+    s->Q.ray[0] = (float4*)take(qn * 32);
+    s->Q.ray[1] = (float4*)take(qn * 32);
+    s->Q.pix[0] = (uint32_t*)take(qn * 4);
+    s->Q.pix[1] = (uint32_t*)take(qn * 4);
+    s->Q.hit = (float4*)take(qn * 16);
     s->cap_px = npx;
     return REINA_RT_OK;
 }
@@ -375,6 +376,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     P.sampleBatch = pc->sampleBatch;
     P.directClamp = pc->directClamp;
     HIP_TRY(hipEventRecord(s->ev0, st));
+    s->Q.cap = part_capacity(W * H);
     HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, st));
     uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
     uint32_t it = 0;
@@ -394,15 +396,18 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
             }
             HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used], st));
         }
-        HIP_TRY(launch_trace(s->sv, s->Q.ray[in_q], s->Q.hit, &s->ctrl[slot].count, 0, &s->ctrl[slot].fetch_ext,
-                             s->stats, nullptr, count, s->tune, st));
+        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
+        HIP_TRY(launch_trace(s->sv, s->Q.ray[in_q], s->Q.hit, tin, s->stats, nullptr, count, s->tune, st));
         if (timex) HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used++ + 1], st));
         HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->tune.shade_grid, st));
         ++it;
         if (it % K == 0) {
-            HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count, 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count[0][0], sizeof(Ctrl::count),
+                                   hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
-            if (*s->h_count == 0) break;
+            uint64_t live = 0;
+            for (uint32_t p = 0; p < kParts; ++p) live += s->h_count[p * kLine];
+            if (live == 0) break;
         }
     }
     HIP_TRY(launch_finish(P, s->S, (float4*)image, st));
@@ -446,6 +451,7 @@ int reina_rt_get_tuning(const ReinaScene* s, ReinaTuning* o) {
     o->refill_min = s->tune.refill_min;
     o->grid_blocks = s->tune.grid;
     o->shade_blocks = s->tune.shade_grid;
+    o->waves_per_simd = s->tune.waves_per_simd;
     return REINA_RT_OK;
 }
 
@@ -457,12 +463,16 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
     if (sd < s->min_stack)
         return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth below what this BVH needs (" + std::to_string(s->min_stack) + ")");
     if (t->refill_min < 1 || t->refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be in 1..64");
-    if (t->grid_blocks < 1 || t->grid_blocks > 65536 || t->shade_blocks < 1 || t->shade_blocks > 65536)
-        return fail(REINA_RT_ERR_INVALID_ARG, "block counts must be in 1..65536");
+    if (t->grid_blocks < 1 || t->grid_blocks > 65536 || t->shade_blocks < kParts || t->shade_blocks > 65536 ||
+        t->shade_blocks % kParts)
+        return fail(REINA_RT_ERR_INVALID_ARG, "grid_blocks must be in 1..65536 and shade_blocks a multiple of 8");
     s->tune.stack_depth = sd;
     s->tune.refill_min = t->refill_min;
     s->tune.grid = t->grid_blocks;
     s->tune.shade_grid = t->shade_blocks;
+    if (t->waves_per_simd != 0 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
+        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    s->tune.waves_per_simd = t->waves_per_simd;
     return REINA_RT_OK;
 }
 
@@ -482,9 +492,10 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
     int rc = ensure_trace_buffers(s, 0);
     if (rc) return rc;
     if (n == 0) return REINA_RT_OK;
-    HIP_TRY(hipMemsetAsync(s->fetch, 0, 4, st));
-    HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, nullptr, n, s->fetch, nullptr, counters,
-                         counters != nullptr, s->tune, st));
+    HIP_TRY(hipMemsetAsync(s->fetch, 0, kParts * kLine * 4, st));
+    TraceInput tin{nullptr, 0, n, s->fetch};
+    HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters, counters != nullptr,
+                         s->tune, st));
     return REINA_RT_OK;
 }
 

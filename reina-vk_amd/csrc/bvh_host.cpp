@@ -202,8 +202,8 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
     out.nodes.clear();
     out.tris.clear();
     out.max_depth = 0;
-    if (ntri >= (1ull << 27)) {
-        err = "too many triangles for the BVH8 index encoding (max 2^27)";
+    if (ntri >= (1ull << 24)) {
+        err = "too many triangles for the BVH8 index encoding (max 2^24 per scene)";
         return false;
     }
     if (ntri == 0) {
@@ -350,9 +350,20 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
             }
         }
         if (root_area > 0.0f) sah += (double)pb.area() / root_area;
+        uint32_t imask = 0, lmask = 0;
+        for (int k = 0; k < 8; ++k) {
+            if (nd.meta[k] & 0x80) imask |= 1u << k;
+            else if (nd.meta[k]) lmask |= 1u << k;
+        }
+        nd.child_base |= imask << 24;
+        nd.tri_base |= lmask << 24;
         out.nodes[pd.node] = nd;
     }
     out.sah_cost = (float)sah;
+    if (out.nodes.size() >= (1ull << 24)) {
+        err = "too many BVH8 nodes for the index encoding (max 2^24)";
+        return false;
+    }
     return true;
 }
 

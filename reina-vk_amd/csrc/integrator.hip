@@ -6,19 +6,21 @@
 // 250-285). Here each pixel owns a path slot with its state in HBM (SoA,
 // indexed by pixel) and the work proceeds in bounce iterations:
 //
-//   k_init    raygen of sample 0 for every pixel -> ray queue (pixel order)
-//   k_extend  closest hit for every queued ray (BVH8 traversal)
+//   k_init    raygen of sample 0 for every pixel -> ray queue
+//   k_trace   closest hit for every queued ray (BVH8 traversal)
 //   k_shade   hit/miss shading + the raygen bounce logic; a path that ends
 //             finishes its sample and regenerates the pixel's next camera ray
 //             (the RNG state runs sequentially through the pixel's samples,
 //             exactly as in the reference); surviving rays are appended to the
-//             next queue with a wave64 ballot + one atomic per wave
+//             next queue (wave64 ballot + block scan, one atomic per 256 rays)
 //   k_finish  mean over the non-NaN samples, running-mean blend, store
 //
-// Queue work is distributed dynamically: each wave claims 64 entries at a time
-// from a per-iteration counter, so long traversals do not stall a static
-// partition. Per-iteration control words are triple-buffered so that no host
-// round trip or memset sits between iterations.
+// Queues are split into one partition per XCD (see wavefront.h). k_trace is
+// persistent: a wave claims chunks of its XCD's partition and steals from the
+// others when it runs dry; idle lanes are refilled mid-flight. k_shade is
+// static: block b works on partition b % 8 in tiles of 256. Per-iteration
+// control words are triple-buffered so no host round trip or memset sits
+// between iterations.
 #include <hip/hip_runtime.h>
 
 #include "traverse.h"
@@ -26,34 +28,30 @@
 
 namespace rt {
 
-__device__ __forceinline__ uint32_t wave_claim(uint32_t* counter) {
-    uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(counter, 64u);
-    return __shfl(base, 0);
-}
-
 __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues Q, Ctrl* ctrl,
                                               unsigned long long* stats) {
-    uint32_t npx = P.W * P.H;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        bool anyrays = P.spp > 0 && P.maxBounces > 0;
-        ctrl[0].count = anyrays ? npx : 0u;
-        ctrl[0].fetch_ext = 0; ctrl[0].fetch_shade = 0;
-        ctrl[1].count = 0; ctrl[1].fetch_ext = 0; ctrl[1].fetch_shade = 0;
-        ctrl[2].count = 0; ctrl[2].fetch_ext = 0; ctrl[2].fetch_shade = 0;
-        stats[0] = 0;
-        stats[1] = 0;
-        stats[2] = 0;
+    const uint32_t npx = P.W * P.H;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool anyrays = P.spp > 0 && P.maxBounces > 0;
+    if (blockIdx.x == 0 && threadIdx.x < 3 * kParts) {
+        uint32_t slot = threadIdx.x / kParts, p = threadIdx.x % kParts;
+        uint32_t c = 0;
+        if (slot == 0 && anyrays) {
+            // pixels of partition p: full tiles t = p, p + 8, ... plus a partial last tile
+            uint32_t tiles = (npx + 1023u) >> 10;
+            for (uint32_t t = p; t < tiles; t += kParts) c += min(1024u, npx - (t << 10));
+        }
+        ctrl[slot].count[p][0] = c;
+        ctrl[slot].fetch[p][0] = 0;
+        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; }
     }
     if (i >= npx) return;
     uint32_t x = i % P.W, y = i / P.W;
     uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
-    if (P.spp == 0 || P.maxBounces == 0) {
+    if (!anyrays) {
         // no traced segment: every sample contributes clamp(0) = 0
-        uint32_t actual = P.spp;
         S.st0[i] = make_uint4(rng, P.spp, 0u, 0u);
-        S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(actual));
+        S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(P.spp));
         return;
     }
     V3 o, d;
@@ -62,34 +60,51 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
     S.st1[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);   // accumulatedDistance starts at 0 (Q8)
     S.st2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-    Q.ray[0][2 * i] = make_float4(o.x, o.y, o.z, 0.0f);
-    Q.ray[0][2 * i + 1] = make_float4(d.x, d.y, d.z, 10000.0f);
-    Q.pix[0][i] = i;
+    uint32_t p = (i >> 10) & (kParts - 1u);
+    uint32_t j = p * Q.cap + (((i >> 13) << 10) | (i & 1023u));
+    Q.ray[0][2 * j] = make_float4(o.x, o.y, o.z, 0.0f);
+    Q.ray[0][2 * j + 1] = make_float4(d.x, d.y, d.z, 10000.0f);
+    Q.pix[0][j] = i;
 }
 
-// Closest-hit kernel over a ray array, persistent with per-lane refill: a
-// lane whose ray is finished goes idle; once at least `refill_min` lanes of
-// the wave are idle they take the next rays from the wave's private chunk of
-// the queue (claimed 128 at a time with one global atomic), so waves stay
-// full while the queue lasts instead of waiting for their slowest ray.
+__device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uint32_t& start, uint32_t& size) {
+    if (in.counts) {
+        start = p * in.cap;
+        size = in.counts[p * kLine];
+    } else {
+        uint32_t a = (uint32_t)(((uint64_t)in.n * p) / kParts);
+        uint32_t b = (uint32_t)(((uint64_t)in.n * (p + 1)) / kParts);
+        start = a;
+        size = b - a;
+    }
+}
+
+// Closest-hit kernel, persistent with per-lane refill: a lane whose ray is
+// finished goes idle; once at least `refill_min` lanes of the wave are idle
+// they take the next rays from the wave's current chunk (kChunk rays claimed
+// with one atomic on its partition's counter, own XCD's partition first).
 // Each loop iteration advances every active lane by one node visit.
-//   n = n_ptr ? *n_ptr : n_imm rays; fetch: the global claim counter.
-//   stats (optional): stats[0] += n; COUNT: stats[1..2] += node visits / tests.
-//   counters (optional, COUNT): per-ray {visits, tests}.
-template <int SD, bool COUNT>
-__global__ __launch_bounds__(256) void k_trace(SceneView sv, const float4* __restrict__ rays,
-                                               float4* __restrict__ hits, const uint32_t* n_ptr, uint32_t n_imm,
-                                               uint32_t* fetch, unsigned long long* stats, uint32_t* counters,
-                                               uint32_t refill_min) {
+template <int SD, int WAVES, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(SceneView sv, const float4* __restrict__ rays,
+                                               float4* __restrict__ hits, TraceInput in, unsigned long long* stats,
+                                               uint32_t* counters, uint32_t refill_min) {
     __shared__ uint2 stack[SD * 256];
     constexpr uint32_t kChunk = 128;
-    const uint32_t n = n_ptr ? *n_ptr : n_imm;
-    if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(stats, (unsigned long long)n);
+    if (stats && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t p = 0; p < kParts; ++p) {
+            uint32_t a, b;
+            part_range(in, p, a, b);
+            tot += b;
+        }
+        atomicAdd(stats, (unsigned long long)tot);
+    }
     uint2* stk = stack + threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t part = blockIdx.x % kParts;    // wave-uniform: partition to claim from next
     bool active = false;
-    bool exhausted = false;           // wave-uniform
+    bool exhausted = false;                 // wave-uniform
     uint32_t pool_next = 0, pool_end = 0;   // wave-uniform
     uint32_t idx = 0;
     TRay r;
@@ -101,19 +116,36 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sv, const float4* __res
         uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
             if (pool_next >= pool_end) {
-                uint32_t b = 0;
-                if (lane == 0) b = atomicAdd(fetch, kChunk);
-                b = __shfl(b, 0);
-                if (b >= n) {
+                // claim a chunk: own partition first, then the others; a plain
+                // read skips partitions that are already used up
+                uint32_t got_start = 0xFFFFFFFFu, got_end = 0, got_part = part;
+                if (lane == 0) {
+                    for (uint32_t k = 0; k < kParts; ++k) {
+                        uint32_t p = (part + k) % kParts;
+                        uint32_t a, sz;
+                        part_range(in, p, a, sz);
+                        uint32_t* f = in.fetch + p * kLine;
+                        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sz) continue;
+                        uint32_t b = atomicAdd(f, kChunk);
+                        if (b >= sz) continue;
+                        got_start = a + b;
+                        got_end = a + min(b + kChunk, sz);
+                        got_part = p;
+                        break;
+                    }
+                }
+                got_start = __shfl(got_start, 0);
+                got_end = __shfl(got_end, 0);
+                part = __shfl(got_part, 0);
+                if (got_start == 0xFFFFFFFFu) {
                     exhausted = true;
                 } else {
-                    pool_next = b;
-                    pool_end = min(b + kChunk, n);
+                    pool_next = got_start;
+                    pool_end = got_end;
                 }
             }
             if (!exhausted) {
-                uint32_t avail = pool_end - pool_next;
-                uint32_t take = min(nidle, avail);
+                uint32_t take = min(nidle, pool_end - pool_next);
                 if (!active) {
                     uint32_t rank = (uint32_t)__popcll(idle & ltmask);
                     if (rank < take) {
@@ -163,30 +195,41 @@ __global__ __launch_bounds__(256) void k_trace(SceneView sv, const float4* __res
     }
 }
 
+// Shade every ray of the current queue; block b serves partition b % 8 in
+// tiles of 256 rays, and the survivors of a tile are appended to the same
+// partition of the next queue with one atomic (wave ballot + LDS scan).
 __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, PathState S, Queues Q, uint32_t in_q,
                                                Ctrl* ctrl, uint32_t slot) {
-    const uint32_t n = ctrl[slot].count;
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
     const uint32_t nslot = (slot + 1) % 3, zslot = (slot + 2) % 3;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctrl[zslot].count = 0; ctrl[zslot].fetch_ext = 0; ctrl[zslot].fetch_shade = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kParts) {
+        ctrl[zslot].count[threadIdx.x][0] = 0;
+        ctrl[zslot].fetch[threadIdx.x][0] = 0;
     }
+    const uint32_t p = blockIdx.x % kParts;
+    const uint32_t nblk = gridDim.x / kParts;       // blocks serving this partition
+    const uint32_t bi = blockIdx.x / kParts;
+    const uint32_t n = ctrl[slot].count[p][0];
+    const uint32_t base_in = p * Q.cap;
     const float4* rin = Q.ray[in_q];
     const uint32_t* pin = Q.pix[in_q];
     float4* rout = Q.ray[in_q ^ 1];
     uint32_t* pout = Q.pix[in_q ^ 1];
-    for (;;) {
-        uint32_t base = wave_claim(&ctrl[slot].fetch_shade);
-        if (base >= n) break;
-        uint32_t i = base + __lane_id();
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t t0 = bi * 256u; t0 < n; t0 += nblk * 256u) {
+        const uint32_t k = t0 + threadIdx.x;
         bool push = false;
-        uint32_t p = 0;
+        uint32_t pxl = 0;
         V3 no = v3(0.0f, 0.0f, 0.0f), nd = no;
-        if (i < n) {
-            p = pin[i];
+        if (k < n) {
+            const uint32_t i = base_in + k;
+            pxl = pin[i];
+            const uint32_t pp = pxl;
             float4 ra = rin[2 * i], rb = rin[2 * i + 1];
             float4 hh = Q.hit[i];
-            uint4 s0 = S.st0[p];
-            float4 s1 = S.st1[p], s2 = S.st2[p], s3 = S.st3[p];
+            uint4 s0 = S.st0[pp];
+            float4 s1 = S.st1[pp], s2 = S.st2[pp], s3 = S.st3[pp];
             V3 O = v3(ra.x, ra.y, ra.z), D = v3(rb.x, rb.y, rb.z);
             Payload pld;
             pld.rng = s0.x;
@@ -240,7 +283,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, Path
                 if (!anynan(c)) { actual += 1u; sum = add(sum, c); }
                 sample += 1u;
                 if (sample < P.spp) {
-                    uint32_t x = p % P.W, y = p / P.W;
+                    uint32_t x = pp % P.W, y = pp / P.W;
                     starting_ray(P.cam, (float)x, (float)y, pld.rng, no, nd);
                     thr = v3(1.0f, 1.0f, 1.0f);
                     L = v3(0.0f, 0.0f, 0.0f);
@@ -248,32 +291,35 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, Path
                     pld.insideDielectric = false;   // raytrace.rgen.glsl:98
                     push = true;
                 }
+                S.st3[pp] = make_float4(sum.x, sum.y, sum.z, __uint_as_float(actual));
             } else {
                 no = pld.rayOrigin;
                 nd = pld.rayDirection;
                 push = true;
             }
-            S.st0[p] = make_uint4(pld.rng, sample, seg, pld.insideDielectric ? 1u : 0u);
-            S.st1[p] = make_float4(thr.x, thr.y, thr.z, pld.accumulatedDistance);
-            S.st2[p] = make_float4(L.x, L.y, L.z, 0.0f);
-            S.st3[p] = make_float4(sum.x, sum.y, sum.z, __uint_as_float(actual));
+            S.st0[pp] = make_uint4(pld.rng, sample, seg, pld.insideDielectric ? 1u : 0u);
+            S.st1[pp] = make_float4(thr.x, thr.y, thr.z, pld.accumulatedDistance);
+            S.st2[pp] = make_float4(L.x, L.y, L.z, 0.0f);
         }
-        // wave64 ballot compaction into the next queue
+        // compaction: wave ballot, block scan over the 4 waves, one atomic per tile
         uint64_t mask = __ballot(push ? 1 : 0);
-        if (mask) {
-            uint32_t cnt = (uint32_t)__popcll(mask);
-            uint32_t lane = __lane_id();
-            uint32_t before = (uint32_t)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-            uint32_t qb = 0;
-            if (lane == 0) qb = atomicAdd(&ctrl[nslot].count, cnt);
-            qb = __shfl(qb, 0);
-            if (push) {
-                uint32_t j = qb + before;
-                rout[2 * j] = make_float4(no.x, no.y, no.z, 0.0f);
-                rout[2 * j + 1] = make_float4(nd.x, nd.y, nd.z, 10000.0f);
-                pout[j] = p;
-            }
+        uint32_t before = (uint32_t)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+            s_base = tot ? atomicAdd(&ctrl[nslot].count[p][0], tot) : 0u;
         }
+        __syncthreads();
+        if (push) {
+            uint32_t off = s_base + before;
+            for (uint32_t w = 0; w < wave; ++w) off += s_wave[w];
+            uint32_t j = base_in + off;
+            rout[2 * j] = make_float4(no.x, no.y, no.z, 0.0f);
+            rout[2 * j + 1] = make_float4(nd.x, nd.y, nd.z, 10000.0f);
+            pout[j] = pxl;
+        }
+        __syncthreads();
     }
 }
 
@@ -325,33 +371,46 @@ hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q
     return hipGetLastError();
 }
 
-template <int SD>
-static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr,
-                            uint32_t n_imm, uint32_t* fetch, unsigned long long* stats, uint32_t* counters,
-                            bool count, const Tuning& tu, hipStream_t st) {
+template <int SD, int WAVES>
+static void launch_trace_sdw(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                             hipStream_t st) {
     dim3 g(tu.grid), b(256);
     if (count)
-        hipLaunchKernelGGL((k_trace<SD, true>), g, b, 0, st, sv, rays, hits, n_ptr, n_imm, fetch, stats, counters,
+        hipLaunchKernelGGL((k_trace<SD, WAVES, true>), g, b, 0, st, sv, rays, hits, in, stats, counters,
                            tu.refill_min);
     else
-        hipLaunchKernelGGL((k_trace<SD, false>), g, b, 0, st, sv, rays, hits, n_ptr, n_imm, fetch, stats, counters,
+        hipLaunchKernelGGL((k_trace<SD, WAVES, false>), g, b, 0, st, sv, rays, hits, in, stats, counters,
                            tu.refill_min);
 }
 
-hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr, uint32_t n_imm,
-                        uint32_t* fetch, unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                        hipStream_t st) {
+template <int SD>
+static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                            hipStream_t st) {
+    switch (tu.waves_per_simd) {
+        case 5: launch_trace_sdw<SD, 5>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        case 6: launch_trace_sdw<SD, 6>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        case 8: launch_trace_sdw<SD, 8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        default: launch_trace_sdw<SD, 1>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+    }
+}
+
+hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                        unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st) {
     switch (tu.stack_depth) {
-        case 8: launch_trace_sd<8>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
-        case 16: launch_trace_sd<16>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
-        case 24: launch_trace_sd<24>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
-        default: launch_trace_sd<32>(sv, rays, hits, n_ptr, n_imm, fetch, stats, counters, count, tu, st); break;
+        case 8: launch_trace_sd<8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        case 16: launch_trace_sd<16>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        case 24: launch_trace_sd<24>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        default: launch_trace_sd<32>(sv, rays, hits, in, stats, counters, count, tu, st); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                         uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st) {
+    // k_shade needs a whole number (>= 1) of blocks per partition
+    grid = grid < kParts ? kParts : grid - grid % kParts;
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, sv, P, S, Q, in_q, ctrl, slot);
     return hipGetLastError();
 }

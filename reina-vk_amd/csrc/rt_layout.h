@@ -11,17 +11,20 @@ namespace rt {
 // One BVH8 node, 256 bytes (two 128-byte cache lines). Child boxes are SoA
 // so each coordinate of all 8 children is one 32-byte run:
 //   lo_x[8] hi_x[8] lo_y[8] hi_y[8] lo_z[8] hi_z[8]  (192 B)
-//   child_base: index of the first internal child (internal children of a
-//               node are contiguous, in slot order)
-//   tri_base:   index of the first triangle of this node's leaves
-//   meta[8]:    0 = empty slot; 0x80 | rank = internal child child_base+rank;
-//               otherwise a leaf of (meta >> 5) & 3 (1..3) triangles starting
-//               at tri_base + (meta & 31)
+//   word 48: child_base (bits 0-23): index of the first internal child
+//            (internal children of a node are contiguous, in slot order);
+//            bits 24-31: imask, the slots holding internal children
+//   word 49: tri_base (bits 0-23): first triangle of this node's leaves;
+//            bits 24-31: leafmask, the slots holding leaves
+//   meta[8]: 0 = empty slot; 0x80 | rank = internal child child_base+rank;
+//            otherwise a leaf of (meta >> 5) & 3 (1..3) triangles starting
+//            at tri_base + (meta & 31)
+// At most 2^24 nodes and 2^24 triangles per scene.
 // Useful bytes per visit: 192 + 8 + 8 = 208 (the remaining 48 are padding).
 struct alignas(256) Bvh8Node {
     float lo_x[8], hi_x[8], lo_y[8], hi_y[8], lo_z[8], hi_z[8];
-    uint32_t child_base;
-    uint32_t tri_base;
+    uint32_t child_base;   // | imask << 24
+    uint32_t tri_base;     // | leafmask << 24
     uint8_t meta[8];
     uint32_t pad[12];
 };

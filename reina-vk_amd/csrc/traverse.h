@@ -107,8 +107,11 @@ struct HitRec {
     float u, v;
 };
 
-// Slab test of the 8 children; returns the hit mask in slot order.
-RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, uint32_t meta_lo, uint32_t meta_hi, float tb) {
+// Slab test of the 8 children; returns the hit mask in slot order (before
+// masking out empty slots). The interval is widened by 2^-16 of |t| with one
+// fused multiply-add each (conservative: never rejects a box whose triangle
+// passes the watertight test).
+RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
     const float* base = reinterpret_cast<const float*>(node);
     // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
     const float4* nx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 0 : 8));
@@ -131,16 +134,21 @@ RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, uint32_t meta_lo, u
     uint32_t hitm = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
         float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
         float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
-        tn = tn - fabsf(tn) * kBoxEps;
-        tf = tf + fabsf(tf) * kBoxEps;
-        tn = fmaxf(tn, r.tmin);
-        tf = fminf(tf, tb);
-        if (m != 0u && tn <= tf) hitm |= 1u << k;
+        tn = fmaxf(__builtin_fmaf(-fabsf(tn), kBoxEps, tn), r.tmin);
+        tf = fminf(__builtin_fmaf(fabsf(tf), kBoxEps, tf), tb);
+        hitm |= (tn <= tf ? 1u : 0u) << k;
     }
     return hitm;
+}
+
+// Bit i of the result is bit (i ^ oct) of the 8-bit mask m.
+RT_D uint32_t permute8(uint32_t m, uint32_t oct) {
+    m = (oct & 1u) ? (((m & 0x55u) << 1) | ((m >> 1) & 0x55u)) : m;
+    m = (oct & 2u) ? (((m & 0x33u) << 2) | ((m >> 2) & 0x33u)) : m;
+    m = (oct & 4u) ? (((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu)) : m;
+    return m;
 }
 
 // Traversal state of one lane between node visits.
@@ -159,27 +167,13 @@ RT_D bool trav_step(const SceneView& sv, const TRay& r, HitRec& h, TravState& ts
                     uint32_t& nvis, uint32_t& ntst) {
     const Bvh8Node* node = sv.nodes + ts.cur;
     const uint4 hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
-    const uint32_t child_base = hdr.x, tri_base = hdr.y, meta_lo = hdr.z, meta_hi = hdr.w;
+    const uint32_t child_base = hdr.x & 0xFFFFFFu, tri_base = hdr.y & 0xFFFFFFu;
+    const uint32_t imask = hdr.x >> 24, leafmask = hdr.y >> 24;
+    const uint32_t meta_lo = hdr.z, meta_hi = hdr.w;
     if (COUNT) nvis++;
-    uint32_t hitm = node_hits(r, node, meta_lo, meta_hi, h.t);
-    // internal-slot mask (bit 7 of each meta byte) and the ordered hit masks
-    uint32_t imask = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
-        imask |= ((m >> 7) & 1u) << k;
-    }
-    uint32_t om = 0, lm = 0;
-    if (hitm) {
-#pragma unroll
-        for (uint32_t i = 0; i < 8; ++i) {
-            uint32_t k = i ^ r.oct;
-            uint32_t hb = (hitm >> k) & 1u;
-            uint32_t ib = (imask >> k) & 1u;
-            om |= (hb & ib) << i;
-            lm |= (hb & (ib ^ 1u)) << i;
-        }
-    }
+    const uint32_t hitm = node_hits(r, node, h.t);
+    uint32_t om = permute8(hitm & imask, r.oct);
+    uint32_t lm = permute8(hitm & leafmask, r.oct);
     // leaves of this node, in slot order i ^ oct
     while (lm) {
         uint32_t i = __builtin_ctz(lm);

@@ -17,10 +17,29 @@ struct TraceParams {
     float directClamp;
 };
 
-// Control words of one bounce iteration: queue length, and the two dynamic
-// work-claim counters of the extend and shade kernels. Three slots rotate so
-// that iteration i's shade zeroes the slot of iteration i+2.
-struct Ctrl { uint32_t count, fetch_ext, fetch_shade, pad; };
+// The ray queues are split into kParts partitions, one per XCD: workgroup b
+// runs on XCD b % 8 (workgroups are dispatched to the 8 XCDs round-robin),
+// and each XCD's blocks work on their own partition, so the queue counters
+// that are hit by atomics are spread over 8 addresses (device-scope atomics
+// are resolved beyond the per-XCD L2 and serialise per address) and a
+// partition's rays stay close to the XCD that wrote them.
+constexpr uint32_t kParts = 8;
+constexpr uint32_t kLine = 32;   // uint32 words per 128-byte line
+
+// Control words of one bounce iteration: per partition the queue length and
+// the extend kernel's claim counter, each on its own 128-byte line. Three
+// slots rotate so that iteration i's shade zeroes the slot of iteration i+2.
+struct Ctrl {
+    uint32_t count[kParts][kLine];
+    uint32_t fetch[kParts][kLine];
+};
+
+// Pixel i starts in partition ((i >> 10) & 7) at position
+// ((i >> 13) << 10) | (i & 1023); `cap` entries per partition.
+__host__ __device__ inline uint32_t part_capacity(uint32_t npx) {
+    uint32_t tiles = (npx + 1023u) >> 10;
+    return ((tiles + kParts - 1u) / kParts) << 10;
+}
 
 // Path state per pixel (SoA, 64 bytes per pixel).
 struct PathState {
@@ -31,11 +50,12 @@ struct PathState {
 };
 
 // Double-buffered ray queues (ReinaRay layout) with the owning pixel, and the
-// hit records of the current queue.
+// hit records of the current queue; kParts partitions of `cap` entries each.
 struct Queues {
     float4* ray[2];
     uint32_t* pix[2];
     float4* hit;
+    uint32_t cap;
 };
 
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
@@ -46,14 +66,25 @@ struct Tuning {
     uint32_t refill_min;    // idle lanes of a wave that trigger a refill (1..64)
     uint32_t grid;          // persistent blocks of 256 threads
     uint32_t shade_grid;    // blocks of the shade kernel
+    uint32_t waves_per_simd;  // register budget of the traversal kernel: 0 (compiler's choice), 5, 6 or 8
 };
 
-// Closest hit for n = n_ptr ? *n_ptr : n_imm rays. stats (optional):
-// stats[0] += n; when counting also stats[1] += node visits, stats[2] +=
-// triangle tests. counters (optional, counting): per-ray {visits, tests}.
-hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const uint32_t* n_ptr, uint32_t n_imm,
-                        uint32_t* fetch, unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                        hipStream_t st);
+// Rays to trace: kParts partitions. Partitioned mode (counts != null):
+// partition p holds counts[p * kLine] rays starting at p * cap. Flat mode:
+// n rays split into kParts contiguous ranges. fetch: kParts claim counters
+// (stride kLine), zero before the launch.
+struct TraceInput {
+    const uint32_t* counts;
+    uint32_t cap;
+    uint32_t n;
+    uint32_t* fetch;
+};
+
+// Closest hit for every ray of `in`. stats (optional): stats[0] += rays;
+// when counting also stats[1] += node visits, stats[2] += triangle tests.
+// counters (optional, counting): per-ray {visits, tests}.
+hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                        unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st);
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                         uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);

@@ -88,7 +88,8 @@ class ReinaTraceStats(ctypes.Structure):
 
 class ReinaTuning(ctypes.Structure):
     _fields_ = [("stack_depth", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
-                ("grid_blocks", ctypes.c_uint32), ("shade_blocks", ctypes.c_uint32)]
+                ("grid_blocks", ctypes.c_uint32), ("shade_blocks", ctypes.c_uint32),
+                ("waves_per_simd", ctypes.c_uint32)]
 
 
 TRACE_TIME_EXTEND = 0x1

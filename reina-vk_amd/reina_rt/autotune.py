@@ -1,7 +1,8 @@
 """Autotuner for the traversal / shading launch parameters of one scene.
 
 It times reina_rt_intersect on a caller-supplied ray batch (device buffers)
-over a grid of (LDS stack depth, refill threshold, persistent blocks per CU)
+over a grid of (LDS stack depth, refill threshold, persistent blocks per CU,
+register budget in waves per SIMD)
 with HIP events on the given stream, and installs the fastest setting on the
 scene with reina_rt_set_tuning. The stack depth must cover the BVH depth, so
 candidates start at the scene's own default.
@@ -10,7 +11,7 @@ import itertools
 
 
 def tune_traversal(dev, torch, rays_ptr, hits_ptr, n, stream=None, cus=None, reps=3,
-                   refills=(1, 8, 16, 32, 64), blocks_per_cu=(2, 4, 8), depths=None):
+                   refills=(8, 16, 32, 64), blocks_per_cu=(4, 8), depths=None, waves=(0, 5, 6, 8)):
     stream = stream or torch.cuda.current_stream()
     sp = stream.cuda_stream
     if cus is None:
@@ -20,8 +21,8 @@ def tune_traversal(dev, torch, rays_ptr, hits_ptr, n, stream=None, cus=None, rep
         depths = [d for d in (8, 16, 24, 32) if d >= base["stack_depth"]][:2]
     table = []
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for sd, rf, bpc in itertools.product(depths, refills, blocks_per_cu):
-        dev.set_tuning(stack_depth=sd, refill_min=rf, grid_blocks=cus * bpc)
+    for sd, rf, bpc, wv in itertools.product(depths, refills, blocks_per_cu, waves):
+        dev.set_tuning(stack_depth=sd, refill_min=rf, grid_blocks=cus * bpc, waves_per_simd=wv)
         dev.intersect(rays_ptr, hits_ptr, n, stream=sp)   # warm
         best = None
         for _ in range(reps):
@@ -31,8 +32,9 @@ def tune_traversal(dev, torch, rays_ptr, hits_ptr, n, stream=None, cus=None, rep
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1)
             best = ms if best is None else min(best, ms)
-        table.append({"stack_depth": sd, "refill_min": rf, "blocks_per_cu": bpc, "ms": round(best, 4)})
+        table.append({"stack_depth": sd, "refill_min": rf, "blocks_per_cu": bpc, "waves_per_simd": wv,
+                      "ms": round(best, 4)})
     win = min(table, key=lambda r: r["ms"])
     dev.set_tuning(stack_depth=win["stack_depth"], refill_min=win["refill_min"],
-                   grid_blocks=cus * win["blocks_per_cu"])
+                   grid_blocks=cus * win["blocks_per_cu"], waves_per_simd=win["waves_per_simd"])
     return win, table

@@ -195,6 +195,21 @@ def test_render_icosphere_bitexact(cuda, icosphere_arrays):
     dev.close()
 
 
+def test_render_partitions_bitexact(cuda, cornell_arrays):
+    """150 x 100 pixels = 15 queue tiles of 1024 (two per XCD partition, the
+    last one partial): every partition is used and refilled."""
+    torch = cuda
+    arrays, _ = cornell_arrays
+    W, H = 150, 100
+    pc = scenes.cornell_push(W, H, spp=3, max_bounces=8)
+    dev = reina_rt.DeviceScene(arrays)
+    got = gpu_render(torch, dev, pc, W, H)
+    ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    assert np.array_equal(_bits(got), _bits(ref))
+    assert dev.stats().extension_rays == nrays
+    dev.close()
+
+
 def test_running_mean_bitexact(cuda, cornell_arrays):
     """sampleBatch > 0 blends (prev * b + new) / (b + 1) (raytrace.rgen.glsl:277-282)."""
     torch = cuda

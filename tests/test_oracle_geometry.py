@@ -85,7 +85,10 @@ def test_bvh_structure():
     assert sorted(ids.tolist()) == list(range(len(tris)))
     seen_nodes = {0}
     for n in range(N):
-        cb, tb = hdr[n]
+        cb, tb = hdr[n] & 0xFFFFFF
+        im, lmk = hdr[n] >> 24
+        assert im == sum(1 << k for k in range(8) if meta[n, k] & 0x80)
+        assert lmk == sum(1 << k for k in range(8) if meta[n, k] and not meta[n, k] & 0x80)
         for k in range(8):
             m = int(meta[n, k])
             if m == 0:
