@@ -16,6 +16,11 @@ roofline: the extend kernel (BVH8 traversal). Algorithmic bytes per ray =
 kernel on the same frame, kernel time from HIP events around every extend
 launch on the trace's stream.
 
+quantized_nodes: the same frame and secondary batch over 8-bit quantized
+BVH8 nodes (80 bytes per node, all useful), with a check that the frame is
+bit-identical to the FP32-node frame (the closest hit does not depend on the
+node format).
+
 cpu_baseline: the oracle's C restatement of the reference shaders (BVH
 traversal over the same BVH8), multithreaded over pixels, on a bounded
 sample of the same frame (whole 1-spp passes for about --cpu-seconds).
@@ -51,6 +56,7 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     a = ap.parse_args()
@@ -80,42 +86,22 @@ def main():
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 
-    def frame(batch):
+    def frame(batch, d=dev):
         pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, a.spp, a.depth, sample_batch=batch)
-        dev.trace(pc, img.data_ptr(), W, H, stream=sp)
-        return dev.stats()
+        d.trace(pc, img.data_ptr(), W, H, stream=sp)
+        return d.stats()
 
     sec = sh = None
-    if not (a.no_secondary and a.no_autotune):
+    if not (a.no_secondary and a.no_autotune and a.no_quantized):
         sec, sh = build_secondary(a, arrays, dev, cam, W, H, torch)
+    autotune = None
     if not a.no_autotune:
-        from reina_rt.autotune import tune_traversal
-        d0 = dev.get_tuning()
-        cus = torch.cuda.get_device_properties(0).multi_processor_count
-        win, table = tune_traversal(dev, torch, sec.data_ptr(), sh.data_ptr(), sec.shape[0], cus=cus)
-        dflt = next((r for r in table if r["stack_depth"] == d0["stack_depth"] and r["refill_min"] == d0["refill_min"]
-                     and r["blocks_per_cu"] * cus == d0["grid_blocks"] and r["waves_per_simd"] == 0), None)
-        autotune = {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
-                    "secondary_batch_rays": int(sec.shape[0])}
+        autotune = run_autotune(dev, sec, sh, torch)
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(os.path.join(ROOT, "gpurun_out", "autotune_table.json"), "w") as f:
-            json.dump(table, f)
+            json.dump(autotune.pop("table"), f)
 
-    batch = 0
-    for _ in range(a.warmup):
-        frame(batch)
-        batch += 1
-    torch.cuda.synchronize()
-    rays = 0
-    gpu_ms = 0.0
-    t_start = time.perf_counter()
-    for _ in range(a.steps):
-        st = frame(batch)
-        rays += st.extension_rays
-        gpu_ms += st.trace_ms
-        batch += 1
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
+    rays, elapsed, gpu_ms = time_frames(frame, a.warmup, a.steps, torch)
     mrays = rays / elapsed / 1e6
     out = {
         "metric": METRIC,
@@ -142,43 +128,18 @@ def main():
         },
     }
 
-    if not a.no_autotune:
+    if autotune is not None:
         out["config"]["autotune"] = autotune
         out["config"]["tuning"] = dev.get_tuning()
     if not a.no_roofline:
-        dev.set_trace_options(abi.TRACE_COUNT)
-        stc = frame(batch)
-        dev.set_trace_options(abi.TRACE_TIME_EXTEND)
-        stt = frame(batch)
-        dev.set_trace_options(0)
-        torch.cuda.synchronize()
-        assert stc.extension_rays == stt.extension_rays
-        nrays = stt.extension_rays
-        alg_bytes = nrays * RAY_BYTES + stc.node_visits * NODE_BYTES + stc.tri_tests * TRI_BYTES
-        launches = max(stt.extend_launches, 1)
-        avg_launch_ms = stt.extend_ms / launches
-        achieved = (alg_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
-        out["roofline"] = {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_extend (BVH8 traversal)",
-            "bytes_per_launch": int(alg_bytes // launches),
-            "avg_launch_ms": round(avg_launch_ms, 4),
-            "launches": int(launches),
-            "extend_ms_per_frame": round(stt.extend_ms, 3),
-            "extend_share_of_trace": round(stt.extend_ms / max(stt.trace_ms, 1e-9), 3),
-            "nodes_per_ray": round(stc.node_visits / max(nrays, 1), 3),
-            "tris_per_ray": round(stc.tri_tests / max(nrays, 1), 3),
-            "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
-        }
+        out["roofline"] = roofline(dev, frame, NODE_BYTES, torch)
 
     if not a.no_secondary:
-        out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch)
+        out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch, NODE_BYTES)
         out["secondary_rays"]["tuning"] = dev.get_tuning()
+
+    if not a.no_quantized:
+        out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
 
     if a.calib_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -191,6 +152,105 @@ def main():
     dev.close()
     print(json.dumps(out), flush=True)
 
+
+
+def run_autotune(dev, sec, sh, torch):
+    from reina_rt.autotune import tune_traversal
+    d0 = dev.get_tuning()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    win, table = tune_traversal(dev, torch, sec.data_ptr(), sh.data_ptr(), sec.shape[0], cus=cus)
+    dflt = next((r for r in table if r["stack_depth"] == d0["stack_depth"] and r["refill_min"] == d0["refill_min"]
+                 and r["blocks_per_cu"] * cus == d0["grid_blocks"] and r["waves_per_simd"] == 0), None)
+    return {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
+            "secondary_batch_rays": int(sec.shape[0]), "table": table}
+
+
+def time_frames(frame, warmup, steps, torch):
+    """`warmup` untimed then `steps` timed progressive frames (sampleBatch =
+    frame index); returns (extension rays, wall seconds, summed GPU ms)."""
+    batch = 0
+    for _ in range(warmup):
+        frame(batch)
+        batch += 1
+    torch.cuda.synchronize()
+    rays, gpu_ms = 0, 0.0
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        st = frame(batch)
+        rays += st.extension_rays
+        gpu_ms += st.trace_ms
+        batch += 1
+    torch.cuda.synchronize()
+    return rays, time.perf_counter() - t_start, gpu_ms
+
+
+def roofline(dev, frame, node_bytes, torch):
+    """Extend-kernel roofline of one frame: algorithmic bytes from the
+    counting build, average launch time from HIP events around every extend
+    launch on the trace's stream."""
+    from reina_rt import abi
+    dev.set_trace_options(abi.TRACE_COUNT)
+    stc = frame(0, dev)
+    dev.set_trace_options(abi.TRACE_TIME_EXTEND)
+    stt = frame(0, dev)
+    dev.set_trace_options(0)
+    torch.cuda.synchronize()
+    assert stc.extension_rays == stt.extension_rays
+    nrays = stt.extension_rays
+    alg_bytes = nrays * RAY_BYTES + stc.node_visits * node_bytes + stc.tri_tests * TRI_BYTES
+    launches = max(stt.extend_launches, 1)
+    avg_launch_ms = stt.extend_ms / launches
+    achieved = (alg_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "k_trace (BVH8 traversal, extend)",
+        "node_bytes": node_bytes,
+        "bytes_per_launch": int(alg_bytes // launches),
+        "avg_launch_ms": round(avg_launch_ms, 4),
+        "launches": int(launches),
+        "extend_ms_per_frame": round(stt.extend_ms, 3),
+        "extend_share_of_trace": round(stt.extend_ms / max(stt.trace_ms, 1e-9), 3),
+        "nodes_per_ray": round(stc.node_visits / max(nrays, 1), 3),
+        "tris_per_ray": round(stc.tri_tests / max(nrays, 1), 3),
+        "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
+    }
+
+
+def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
+    """The same frame and secondary batch over 8-bit quantized BVH8 nodes
+    (80 bytes per node). The closest hit does not depend on the node format,
+    so the frame must be bit-identical to the FP32-node frame."""
+    import reina_rt
+    from reina_rt import abi
+    t0 = time.perf_counter()
+    devq = reina_rt.DeviceScene(arrays, flags=abi.NODES_QUANTIZED)
+    setup_s = time.perf_counter() - t0
+    res = {"scene_setup_s": round(setup_s, 2), "node_bytes": int(devq.info().node_bytes),
+           "bvh_node_MB": round(devq.info().num_nodes * devq.info().node_bytes / 1e6, 2)}
+    if not a.no_autotune:
+        at = run_autotune(devq, sec, sh, torch)
+        at.pop("table")
+        res["autotune_best"] = at["best"]
+    # bit-identity of one frame against the FP32 nodes
+    frame(0, dev)
+    ref = img.clone()
+    frame(0, devq)
+    res["frame_bit_identical_to_fp32"] = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
+    rays, elapsed, gpu_ms = time_frames(lambda b, d=devq: frame(b, d), a.warmup, a.steps, torch)
+    res["mrays_per_s"] = round(rays / elapsed / 1e6, 2)
+    res["ms_per_step"] = round(elapsed / max(a.steps, 1) * 1e3, 3)
+    if not a.no_roofline:
+        res["roofline"] = roofline(devq, frame, 80, torch)
+    if not a.no_secondary:
+        res["secondary_rays"] = secondary_rays(a, devq, sec, sh, torch, 80)
+    res["tuning"] = devq.get_tuning()
+    devq.close()
+    return res
 
 def build_secondary(a, arrays, dev, cam, W, H, torch):
     """Incoherent secondary rays of the same frame: one pinhole primary ray per
@@ -256,7 +316,7 @@ def build_secondary(a, arrays, dev, cam, W, H, torch):
     return sec, sh
 
 
-def secondary_rays(a, dev, sec, sh, torch):
+def secondary_rays(a, dev, sec, sh, torch, node_bytes):
     import numpy as np
 
     n2 = sec.shape[0]
@@ -279,7 +339,7 @@ def secondary_rays(a, dev, sec, sh, torch):
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1))
     ms = float(np.median(times))
-    alg = n2 * RAY_BYTES + visits * NODE_BYTES + tests * TRI_BYTES
+    alg = n2 * RAY_BYTES + visits * node_bytes + tests * TRI_BYTES
     gbs = alg / (ms * 1e-3) / 1e9
     hit_frac = float((sh[:, 1].contiguous().view(torch.int32) != -1).float().mean().item())
     return {

@@ -220,6 +220,13 @@ int reina_rt_bvh_build_host(const float* triangles, uint64_t num_triangles, void
                             uint64_t* nodes_bytes, void* tris, uint64_t* tris_bytes,
                             uint32_t* max_depth);
 
+/* 8-bit quantized copy (80-byte nodes) of a BVH8 from reina_rt_bvh_build_host
+ * or reina_rt_scene_export_bvh: the node layout scenes created with
+ * REINA_RT_NODES_QUANTIZED hold. Every decoded child box contains the FP32
+ * child box. Host only; pass qnodes = NULL to query the size first. */
+int reina_rt_bvh_quantize_host(const void* nodes, uint64_t nodes_bytes, void* qnodes,
+                               uint64_t* qnodes_bytes);
+
 /* Replaces one vkCmdTraceRaysKHR(W, H, 1) of the raygen/closest-hit/miss
  * pipeline (src/Reina.cpp:425-470): renders samplesPerPixel samples per pixel
  * and writes the (running-mean) radiance into image_rgba32f (device pointer,

@@ -34,7 +34,8 @@ def lib():
     L.or_scene_create.restype = c_p
     L.or_scene_destroy.argtypes = [c_p]
     L.or_scene_destroy.restype = None
-    L.or_scene_set_bvh.argtypes = [c_p, c_p, u64, c_p, u64]
+    L.or_scene_set_bvh.argtypes = [c_p, c_p, u64, c_p, u64, u32]
+    L.or_scene_set_bvh.restype = i32
     L.or_intersect_brute32.argtypes = [c_p, c_p, c_p, u64, i32]
     L.or_intersect_brute32.restype = None
     L.or_intersect_bvh.argtypes = [c_p, c_p, c_p, c_p, u64, i32]
@@ -78,9 +79,13 @@ class OracleScene:
         self.h = lib().or_scene_create(ctypes.byref(d))
         self._bvh = None
 
-    def set_bvh(self, nodes, tris):
+    def set_bvh(self, nodes, tris, node_bytes=256):
+        """node_bytes: 256 for FP32 BVH8 nodes, 80 for 8-bit quantized nodes."""
         self._bvh = (np.ascontiguousarray(nodes), np.ascontiguousarray(tris))
-        lib().or_scene_set_bvh(self.h, _p(self._bvh[0]), self._bvh[0].nbytes, _p(self._bvh[1]), self._bvh[1].nbytes)
+        rc = lib().or_scene_set_bvh(self.h, _p(self._bvh[0]), self._bvh[0].nbytes, _p(self._bvh[1]),
+                                    self._bvh[1].nbytes, node_bytes)
+        if rc:
+            raise ValueError("node_bytes must be 256 or 80")
 
     def brute32(self, rays, nthreads=8):
         hits = np.zeros(len(rays), _hit_dtype())

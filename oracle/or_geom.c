@@ -96,9 +96,11 @@ void or_scene_destroy(OrScene* s) {
 }
 
 int or_scene_set_bvh(OrScene* s, const void* nodes, uint64_t nodes_bytes, const void* tris,
-                     uint64_t tris_bytes) {
+                     uint64_t tris_bytes, uint32_t node_bytes) {
+    if (node_bytes != 256 && node_bytes != 80) return -1;
     s->nodes = (const uint8_t*)nodes;
-    s->nnodes = nodes_bytes / 256;
+    s->node_bytes = node_bytes;
+    s->nnodes = nodes_bytes / node_bytes;
     s->btris = (const float*)tris;
     s->nbtris = tris_bytes / 48;
     return 0;
@@ -214,6 +216,7 @@ static void brute32_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit) {
  * (conservative: never rejects a box whose triangle passes). Rays with a
  * non-finite component or a zero direction miss everything. */
 #define BOX_EPS 1.52587890625e-05f /* 2^-16 */
+#define SLAB_BIG 3.0e38f
 
 typedef struct { uint32_t base, imask, om; } Group;
 
@@ -224,26 +227,48 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
     uint32_t ib = 0xFFFFFFFFu;
     float ub = 0.0f, vb = 0.0f;
     uint32_t nvis = 0, ntst = 0;
-    /* inverse direction with zero components replaced by +-1e-30 */
-    float dx = r.d.x, dy = r.d.y, dz = r.d.z;
-    if (fabsf(dx) < 1e-30f) dx = copysignf(1e-30f, dx);
-    if (fabsf(dy) < 1e-30f) dy = copysignf(1e-30f, dy);
-    if (fabsf(dz) < 1e-30f) dz = copysignf(1e-30f, dz);
-    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    /* inverse direction; a zero component gives +-inf */
+    float ix = 1.0f / r.d.x, iy = 1.0f / r.d.y, iz = 1.0f / r.d.z;
     uint32_t oct = (r.d.x < 0.0f ? 1u : 0u) | (r.d.y < 0.0f ? 2u : 0u) | (r.d.z < 0.0f ? 4u : 0u);
     Group stack[64];
     int sp = 0;
     uint32_t cur = 0;
     int have = (s->nnodes > 0) && r.valid;
     while (have) {
-        const uint8_t* nb = s->nodes + (uint64_t)cur * 256;
-        const float* f = (const float*)nb;
+        const uint8_t* nb = s->nodes + (uint64_t)cur * s->node_bytes;
+        float fq[48];
+        const float* f;
         uint32_t w48, w49;
-        memcpy(&w48, nb + 192, 4);
-        memcpy(&w49, nb + 196, 4);
+        const uint8_t* meta;
+        if (s->node_bytes == 256) {
+            f = (const float*)nb;
+            memcpy(&w48, nb + 192, 4);
+            memcpy(&w49, nb + 196, 4);
+            meta = nb + 200;
+        } else {
+            /* quantized node: p.xyz, exponent bytes, header, meta, then
+             * qlo_x qhi_x qlo_y qhi_y qlo_z qhi_z (8 bytes each); plane =
+             * fmaf(q, 2^(e-127), p) */
+            float p[3];
+            uint32_t ex;
+            memcpy(p, nb, 12);
+            memcpy(&ex, nb + 12, 4);
+            for (int a = 0; a < 3; ++a) {
+                uint32_t eb = ((ex >> (8 * a)) & 0xFFu) << 23;
+                float sc;
+                memcpy(&sc, &eb, 4);
+                for (int k = 0; k < 8; ++k) {
+                    fq[16 * a + k] = fmaf((float)nb[32 + 16 * a + k], sc, p[a]);
+                    fq[16 * a + 8 + k] = fmaf((float)nb[32 + 16 * a + 8 + k], sc, p[a]);
+                }
+            }
+            f = fq;
+            memcpy(&w48, nb + 16, 4);
+            memcpy(&w49, nb + 20, 4);
+            meta = nb + 24;
+        }
         uint32_t child_base = w48 & 0xFFFFFFu, tri_base = w49 & 0xFFFFFFu;
         uint32_t imask = w48 >> 24, leafmask = w49 >> 24;
-        const uint8_t* meta = nb + 200;
         nvis++;
         uint32_t hitm = 0;
         /* near/far plane per axis by direction sign (lo <= hi, so this is min/max);
@@ -255,6 +280,11 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
             float nz = sz ? f[32 + k] : f[40 + k], fz = sz ? f[40 + k] : f[32 + k];
             float tn = fmaxf(fmaxf((nx - r.o.x) * ix, (ny - r.o.y) * iy), (nz - r.o.z) * iz);
             float tf = fminf(fminf((fx - r.o.x) * ix, (fy - r.o.y) * iy), (fz - r.o.z) * iz);
+            /* d = 0 on an axis: its distances are +-inf, or NaN (0 * inf) for
+             * a ray inside a plane, which fmaxf/fminf ignore (closed slab);
+             * +inf / -inf interval ends are clamped before the widening */
+            tn = fminf(tn, SLAB_BIG);
+            tf = fmaxf(tf, -SLAB_BIG);
             tn = fmaxf(fmaf(-fabsf(tn), BOX_EPS, tn), r.tmin);
             tf = fminf(fmaf(fabsf(tf), BOX_EPS, tf), tb);
             if (tn <= tf) hitm |= 1u << k;

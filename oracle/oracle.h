@@ -48,13 +48,14 @@ typedef struct OrScene {
     uint32_t* tri_prim;      /* gl_PrimitiveID (local to the model)           */
     /* optional BVH8 (exported from the library) */
     const uint8_t* nodes; uint64_t nnodes;
+    uint32_t node_bytes;     /* 256: FP32 node, 80: 8-bit quantized node        */
     const float* btris; uint64_t nbtris;  /* float4 x3 per triangle            */
 } OrScene;
 
 OrScene* or_scene_create(const ReinaSceneDesc* desc);
 void or_scene_destroy(OrScene* s);
 int or_scene_set_bvh(OrScene* s, const void* nodes, uint64_t nodes_bytes,
-                     const void* tris, uint64_t tris_bytes);
+                     const void* tris, uint64_t tris_bytes, uint32_t node_bytes);
 
 /* ---- intersection -------------------------------------------------------- */
 void or_intersect_brute32(const OrScene* s, const ReinaRay* rays, ReinaHit* hits, uint64_t n,

@@ -44,6 +44,8 @@ struct ReinaScene {
     uint64_t arena_bytes = 0;
     ReinaSceneInfo info{};
     std::vector<Bvh8Node> host_nodes;   // kept for export
+    std::vector<QBvh8Node> host_qnodes; // quantized copy (REINA_RT_NODES_QUANTIZED)
+    bool quantized = false;
     std::vector<float> host_tris;
     // trace buffers (grown on demand)
     uint64_t cap_px = 0;
@@ -82,6 +84,10 @@ static int validate(const ReinaSceneDesc* d) {
     if (d->struct_size != sizeof(ReinaSceneDesc))
         return fail(REINA_RT_ERR_ABI, "ReinaSceneDesc.struct_size mismatch (expected " +
                                           std::to_string(sizeof(ReinaSceneDesc)) + ")");
+    if (d->flags & ~(REINA_RT_BUILD_DEVICE | REINA_RT_NODES_QUANTIZED))
+        return fail(REINA_RT_ERR_INVALID_ARG, "unknown ReinaSceneDesc.flags bits");
+    if (d->flags & REINA_RT_BUILD_DEVICE)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "the device BVH build is not available; use REINA_RT_BUILD_HOST");
     if (d->num_instances && (!d->instances || !d->models || !d->props))
         return fail(REINA_RT_ERR_INVALID_ARG, "instances given without models/props");
     for (uint32_t m = 0; m < d->num_models; ++m) {
@@ -131,7 +137,8 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
         off += al(bytes ? bytes : 4);
         return parts.size() - 1;
     };
-    size_t p_nodes = add(s->host_nodes.data(), s->host_nodes.size() * sizeof(Bvh8Node));
+    size_t p_nodes = s->quantized ? add(s->host_qnodes.data(), s->host_qnodes.size() * sizeof(QBvh8Node))
+                                  : add(s->host_nodes.data(), s->host_nodes.size() * sizeof(Bvh8Node));
     size_t p_tris = add(s->host_tris.data(), s->host_tris.size() * sizeof(float));
     size_t p_ti = add(tri_inst.data(), tri_inst.size() * 4);
     size_t p_tp = add(tri_prim.data(), tri_prim.size() * 4);
@@ -151,7 +158,8 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     for (const Part& p : parts)
         if (p.bytes && p.src) HIP_TRY(hipMemcpy((char*)s->arena + p.off, p.src, p.bytes, hipMemcpyHostToDevice));
     char* a = (char*)s->arena;
-    s->sv.nodes = (const Bvh8Node*)(a + parts[p_nodes].off);
+    s->sv.nodes = s->quantized ? nullptr : (const Bvh8Node*)(a + parts[p_nodes].off);
+    s->sv.qnodes = s->quantized ? (const QBvh8Node*)(a + parts[p_nodes].off) : nullptr;
     s->sv.tris = (const float4*)(a + parts[p_tris].off);
     s->sv.tri_inst = (const uint32_t*)(a + parts[p_ti].off);
     s->sv.tri_prim = (const uint32_t*)(a + parts[p_tp].off);
@@ -176,8 +184,6 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     *out = nullptr;
     int rc = validate(d);
     if (rc) return rc;
-    if (d->flags & REINA_RT_NODES_QUANTIZED)
-        return fail(REINA_RT_ERR_UNSUPPORTED, "quantized nodes are not built");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
@@ -230,6 +236,13 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     }
     s->host_nodes = std::move(bvh.nodes);
     s->host_tris = std::move(bvh.tris);
+    if (d->flags & REINA_RT_NODES_QUANTIZED) {
+        if (!quantize_bvh8(s->host_nodes, s->host_qnodes, err)) {
+            delete s;
+            return fail(REINA_RT_ERR_INVALID_ARG, err);
+        }
+        s->quantized = true;
+    }
     rc = scene_upload(s, d, wtri, tri_inst, tri_prim, inst);
     if (rc) {
         if (s->arena) hipFree(s->arena);
@@ -239,10 +252,10 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     auto t1 = std::chrono::steady_clock::now();
     s->info.num_triangles = ntri;
     s->info.num_nodes = s->host_nodes.size();
-    s->info.node_bytes = sizeof(Bvh8Node);
+    s->info.node_bytes = s->quantized ? sizeof(QBvh8Node) : sizeof(Bvh8Node);
     s->info.device_bytes = s->arena_bytes;
     s->info.max_depth = bvh.max_depth;
-    s->info.quantized = 0;
+    s->info.quantized = s->quantized ? 1u : 0u;
     s->info.build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     s->info.sah_cost = bvh.sah_cost;
     hipDeviceProp_t prop;
@@ -288,10 +301,12 @@ int reina_rt_scene_info(const ReinaScene* s, ReinaSceneInfo* o) {
 int reina_rt_scene_export_bvh(const ReinaScene* s, void* nodes, uint64_t* nodes_bytes, void* tris,
                               uint64_t* tris_bytes) {
     if (!s || !nodes_bytes || !tris_bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
-    uint64_t nb = s->host_nodes.size() * sizeof(Bvh8Node), tb = s->host_tris.size() * sizeof(float);
+    const void* src = s->quantized ? (const void*)s->host_qnodes.data() : (const void*)s->host_nodes.data();
+    uint64_t nb = s->quantized ? s->host_qnodes.size() * sizeof(QBvh8Node) : s->host_nodes.size() * sizeof(Bvh8Node);
+    uint64_t tb = s->host_tris.size() * sizeof(float);
     if (nodes) {
         if (*nodes_bytes < nb) return fail(REINA_RT_ERR_INVALID_ARG, "nodes buffer too small");
-        std::memcpy(nodes, s->host_nodes.data(), nb);
+        std::memcpy(nodes, src, nb);
     }
     if (tris) {
         if (*tris_bytes < tb) return fail(REINA_RT_ERR_INVALID_ARG, "tris buffer too small");
@@ -320,6 +335,23 @@ int reina_rt_bvh_build_host(const float* tri, uint64_t ntri, void* nodes, uint64
     *nodes_bytes = nb;
     *tris_bytes = tb;
     if (max_depth) *max_depth = r.max_depth;
+    return REINA_RT_OK;
+}
+
+int reina_rt_bvh_quantize_host(const void* nodes, uint64_t nodes_bytes, void* qnodes, uint64_t* qnodes_bytes) {
+    if ((nodes_bytes && !nodes) || !qnodes_bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    if (nodes_bytes % sizeof(Bvh8Node)) return fail(REINA_RT_ERR_INVALID_ARG, "nodes_bytes is not a multiple of 256");
+    std::vector<Bvh8Node> in(nodes_bytes / sizeof(Bvh8Node));
+    if (nodes_bytes) std::memcpy(in.data(), nodes, nodes_bytes);
+    std::vector<QBvh8Node> q;
+    std::string err;
+    if (!quantize_bvh8(in, q, err)) return fail(REINA_RT_ERR_INVALID_ARG, err);
+    uint64_t qb = q.size() * sizeof(QBvh8Node);
+    if (qnodes) {
+        if (*qnodes_bytes < qb) return fail(REINA_RT_ERR_INVALID_ARG, "qnodes buffer too small");
+        std::memcpy(qnodes, q.data(), qb);
+    }
+    *qnodes_bytes = qb;
     return REINA_RT_OK;
 }
 

@@ -367,4 +367,78 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
     return true;
 }
 
+// Per axis: origin p = smallest child lo, grid step s = 2^(e-127) with the
+// smallest e for which every child interval [lo, hi] can be covered by
+// decoded planes fmaf(q, s, p), q in 0..255, rounded outward.
+static bool quantize_axis(const float* lo, const float* hi, uint32_t valid, float& p, uint32_t& e,
+                          uint8_t* qlo, uint8_t* qhi) {
+    float pmin = INFINITY, pmax = -INFINITY;
+    for (int k = 0; k < 8; ++k) {
+        qlo[k] = 0;
+        qhi[k] = 0;
+        if (!(valid >> k & 1u)) continue;
+        if (!std::isfinite(lo[k]) || !std::isfinite(hi[k])) return false;
+        pmin = std::min(pmin, lo[k]);
+        pmax = std::max(pmax, hi[k]);
+    }
+    if (!valid) {
+        p = 0.0f;
+        e = 127;
+        return true;
+    }
+    p = pmin;
+    double ext = (double)pmax - (double)pmin;
+    int ex = -126;
+    if (ext > 0.0) std::frexp(ext / 255.0, &ex);   // ext / 255 <= 2^ex
+    int e0 = std::max(1, std::min(254, ex + 127));
+    for (int ee = e0; ee <= 254; ++ee) {
+        const float s = std::ldexp(1.0f, ee - 127);
+        bool ok = true;
+        for (int k = 0; k < 8 && ok; ++k) {
+            if (!(valid >> k & 1u)) continue;
+            double a = std::floor(((double)lo[k] - (double)p) / (double)s);
+            int ql = (int)std::max(0.0, std::min(255.0, a));
+            while (ql > 0 && std::fma((float)ql, s, p) > lo[k]) ql--;
+            double b = std::ceil(((double)hi[k] - (double)p) / (double)s);
+            int qh = (int)std::max(0.0, std::min(255.0, b));
+            while (qh < 255 && std::fma((float)qh, s, p) < hi[k]) qh++;
+            if (std::fma((float)ql, s, p) > lo[k] || std::fma((float)qh, s, p) < hi[k]) {
+                ok = false;
+                break;
+            }
+            qlo[k] = (uint8_t)ql;
+            qhi[k] = (uint8_t)qh;
+        }
+        if (ok) {
+            e = (uint32_t)ee;
+            return true;
+        }
+    }
+    return false;
+}
+
+bool quantize_bvh8(const std::vector<Bvh8Node>& nodes, std::vector<QBvh8Node>& out, std::string& err) {
+    out.resize(nodes.size());
+    for (size_t n = 0; n < nodes.size(); ++n) {
+        const Bvh8Node& a = nodes[n];
+        QBvh8Node& q = out[n];
+        std::memset(&q, 0, sizeof q);
+        uint32_t valid = 0;
+        for (int k = 0; k < 8; ++k)
+            if (a.meta[k]) valid |= 1u << k;
+        uint32_t ex = 0, ey = 0, ez = 0;
+        if (!quantize_axis(a.lo_x, a.hi_x, valid, q.px, ex, q.qlo_x, q.qhi_x) ||
+            !quantize_axis(a.lo_y, a.hi_y, valid, q.py, ey, q.qlo_y, q.qhi_y) ||
+            !quantize_axis(a.lo_z, a.hi_z, valid, q.pz, ez, q.qlo_z, q.qhi_z)) {
+            err = "BVH8 node " + std::to_string(n) + " has a box that 8-bit quantization cannot cover";
+            return false;
+        }
+        q.exps = ex | ey << 8 | ez << 16;
+        q.child_base = a.child_base;
+        q.tri_base = a.tri_base;
+        std::memcpy(q.meta, a.meta, 8);
+    }
+    return true;
+}
+
 }  // namespace rt

@@ -19,4 +19,9 @@ struct Bvh8Result {
 // tri: 9 floats per triangle (world-space v0, v1, v2), flattened id = index.
 bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::string& err);
 
+// 8-bit quantized copy of a BVH8 (same topology, header and meta). Every
+// decoded child box contains the FP32 child box. Fails only for non-finite
+// or astronomically large boxes (grid step above 2^127).
+bool quantize_bvh8(const std::vector<Bvh8Node>& nodes, std::vector<QBvh8Node>& out, std::string& err);
+
 }  // namespace rt

@@ -7,7 +7,7 @@
 // indexed by pixel) and the work proceeds in bounce iterations:
 //
 //   k_init    raygen of sample 0 for every pixel -> ray queue
-//   k_trace   closest hit for every queued ray (BVH8 traversal)
+//   k_trace   closest hit for every queued ray (BVH8 traversal, trace_kernel.h)
 //   k_shade   hit/miss shading + the raygen bounce logic; a path that ends
 //             finishes its sample and regenerates the pixel's next camera ray
 //             (the RNG state runs sequentially through the pixel's samples,
@@ -65,134 +65,6 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
     Q.ray[0][2 * j] = make_float4(o.x, o.y, o.z, 0.0f);
     Q.ray[0][2 * j + 1] = make_float4(d.x, d.y, d.z, 10000.0f);
     Q.pix[0][j] = i;
-}
-
-__device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uint32_t& start, uint32_t& size) {
-    if (in.counts) {
-        start = p * in.cap;
-        size = in.counts[p * kLine];
-    } else {
-        uint32_t a = (uint32_t)(((uint64_t)in.n * p) / kParts);
-        uint32_t b = (uint32_t)(((uint64_t)in.n * (p + 1)) / kParts);
-        start = a;
-        size = b - a;
-    }
-}
-
-// Closest-hit kernel, persistent with per-lane refill: a lane whose ray is
-// finished goes idle; once at least `refill_min` lanes of the wave are idle
-// they take the next rays from the wave's current chunk (kChunk rays claimed
-// with one atomic on its partition's counter, own XCD's partition first).
-// Each loop iteration advances every active lane by one node visit.
-template <int SD, int WAVES, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(SceneView sv, const float4* __restrict__ rays,
-                                               float4* __restrict__ hits, TraceInput in, unsigned long long* stats,
-                                               uint32_t* counters, uint32_t refill_min) {
-    __shared__ uint2 stack[SD * 256];
-    constexpr uint32_t kChunk = 128;
-    if (stats && blockIdx.x == 0 && threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (uint32_t p = 0; p < kParts; ++p) {
-            uint32_t a, b;
-            part_range(in, p, a, b);
-            tot += b;
-        }
-        atomicAdd(stats, (unsigned long long)tot);
-    }
-    uint2* stk = stack + threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t part = blockIdx.x % kParts;    // wave-uniform: partition to claim from next
-    bool active = false;
-    bool exhausted = false;                 // wave-uniform
-    uint32_t pool_next = 0, pool_end = 0;   // wave-uniform
-    uint32_t idx = 0;
-    TRay r;
-    HitRec h;
-    TravState ts;
-    uint32_t nv = 0, nt = 0, wvis = 0, wtst = 0;
-    for (;;) {
-        uint64_t idle = __ballot(!active);
-        uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && (nidle >= refill_min || nidle == 64u)) {
-            if (pool_next >= pool_end) {
-                // claim a chunk: own partition first, then the others; a plain
-                // read skips partitions that are already used up
-                uint32_t got_start = 0xFFFFFFFFu, got_end = 0, got_part = part;
-                if (lane == 0) {
-                    for (uint32_t k = 0; k < kParts; ++k) {
-                        uint32_t p = (part + k) % kParts;
-                        uint32_t a, sz;
-                        part_range(in, p, a, sz);
-                        uint32_t* f = in.fetch + p * kLine;
-                        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sz) continue;
-                        uint32_t b = atomicAdd(f, kChunk);
-                        if (b >= sz) continue;
-                        got_start = a + b;
-                        got_end = a + min(b + kChunk, sz);
-                        got_part = p;
-                        break;
-                    }
-                }
-                got_start = __shfl(got_start, 0);
-                got_end = __shfl(got_end, 0);
-                part = __shfl(got_part, 0);
-                if (got_start == 0xFFFFFFFFu) {
-                    exhausted = true;
-                } else {
-                    pool_next = got_start;
-                    pool_end = got_end;
-                }
-            }
-            if (!exhausted) {
-                uint32_t take = min(nidle, pool_end - pool_next);
-                if (!active) {
-                    uint32_t rank = (uint32_t)__popcll(idle & ltmask);
-                    if (rank < take) {
-                        idx = pool_next + rank;
-                        tray_init(r, rays[2 * idx], rays[2 * idx + 1]);
-                        h = miss_record(r);
-                        ts.cur = 0;
-                        ts.sp = 0;
-                        nv = 0;
-                        nt = 0;
-                        active = true;
-                        if (!r.valid || sv.num_nodes == 0) {
-                            hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
-                            if (COUNT && counters) { counters[2 * idx] = 0; counters[2 * idx + 1] = 0; }
-                            active = false;
-                        }
-                    }
-                }
-                pool_next += take;
-            }
-        }
-        if (__ballot(active) == 0ull) {
-            if (exhausted) break;
-            continue;
-        }
-        if (active) {
-            if (!trav_step<COUNT>(sv, r, h, ts, stk, 256, nv, nt)) {
-                hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
-                if (COUNT) {
-                    if (counters) { counters[2 * idx] = nv; counters[2 * idx + 1] = nt; }
-                    wvis += nv;
-                    wtst += nt;
-                }
-                active = false;
-            }
-        }
-    }
-    if (COUNT && stats) {
-        for (int o = 32; o > 0; o >>= 1) {
-            wvis += __shfl_xor(wvis, o);
-            wtst += __shfl_xor(wtst, o);
-        }
-        if (lane == 0) {
-            atomicAdd(stats + 1, (unsigned long long)wvis);
-            atomicAdd(stats + 2, (unsigned long long)wtst);
-        }
-    }
 }
 
 // Shade every ray of the current queue; block b serves partition b % 8 in
@@ -371,40 +243,10 @@ hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q
     return hipGetLastError();
 }
 
-template <int SD, int WAVES>
-static void launch_trace_sdw(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
-                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                             hipStream_t st) {
-    dim3 g(tu.grid), b(256);
-    if (count)
-        hipLaunchKernelGGL((k_trace<SD, WAVES, true>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min);
-    else
-        hipLaunchKernelGGL((k_trace<SD, WAVES, false>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min);
-}
-
-template <int SD>
-static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
-                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                            hipStream_t st) {
-    switch (tu.waves_per_simd) {
-        case 5: launch_trace_sdw<SD, 5>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 6: launch_trace_sdw<SD, 6>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 8: launch_trace_sdw<SD, 8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        default: launch_trace_sdw<SD, 1>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-    }
-}
-
 hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                         unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st) {
-    switch (tu.stack_depth) {
-        case 8: launch_trace_sd<8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 16: launch_trace_sd<16>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 24: launch_trace_sd<24>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        default: launch_trace_sd<32>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-    }
-    return hipGetLastError();
+    return sv.qnodes ? launch_trace_q8(sv, rays, hits, in, stats, counters, count, tu, st)
+                     : launch_trace_f32(sv, rays, hits, in, stats, counters, count, tu, st);
 }
 
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,

@@ -31,6 +31,30 @@ struct alignas(256) Bvh8Node {
 static_assert(sizeof(Bvh8Node) == 256, "node size");
 
 constexpr uint32_t kNodeUsefulBytes = 208;
+
+// Quantized BVH8 node, 80 bytes (five 16-byte loads). Same topology, header
+// and meta as Bvh8Node; each child box is stored as 8-bit offsets from the
+// node origin p on a power-of-two grid per axis:
+//   word 0-2: p.x, p.y, p.z (float)
+//   word 3:   ex | ey << 8 | ez << 16: grid step s = 2^(e - 127) per axis
+//             (the float with exponent field e and zero mantissa, 1 <= e <= 254)
+//   word 4:   child_base | imask << 24      word 5: tri_base | leafmask << 24
+//   word 6-7: meta[8]
+//   bytes 32-79: qlo_x[8] qhi_x[8] qlo_y[8] qhi_y[8] qlo_z[8] qhi_z[8]
+// Decoded plane = fmaf(float(q), s, p), one rounding (q * s is exact). The
+// builder picks q_lo / q_hi so that the decoded box contains the child's
+// FP32 box (checked with this exact decode), so traversal stays
+// conservative and the closest hit still equals brute force.
+struct alignas(16) QBvh8Node {
+    float px, py, pz;
+    uint32_t exps;
+    uint32_t child_base;   // | imask << 24
+    uint32_t tri_base;     // | leafmask << 24
+    uint8_t meta[8];
+    uint8_t qlo_x[8], qhi_x[8], qlo_y[8], qhi_y[8], qlo_z[8], qhi_z[8];
+};
+static_assert(sizeof(QBvh8Node) == 80, "quantized node size");
+constexpr uint32_t kQNodeUsefulBytes = 80;
 constexpr uint32_t kTriUsefulBytes = 36;   // 3 x float3 per tested triangle
 constexpr uint32_t kMaxLeafTris = 3;
 constexpr uint32_t kMaxStackDepth = 32;    // largest LDS group stack per lane (entries)
@@ -49,7 +73,8 @@ static_assert(sizeof(InstanceRec) == 64, "instance record size");
 
 // Device-resident scene (pointers into one HBM arena).
 struct SceneView {
-    const Bvh8Node* nodes;
+    const Bvh8Node* nodes;       // FP32 nodes (null when quantized)
+    const QBvh8Node* qnodes;     // quantized nodes (null when FP32)
     const float4* tris;          // 3 per triangle
     const uint32_t* tri_inst;    // flattened id -> instance
     const uint32_t* tri_prim;    // flattened id -> gl_PrimitiveID

@@ -55,11 +55,8 @@ RT_D void tray_init(TRay& r, float4 a, float4 b) {
     r.Sx = comp3(r.dx, r.dy, r.dz, kx) / dkz;
     r.Sy = comp3(r.dx, r.dy, r.dz, ky) / dkz;
     r.Sz = 1.0f / dkz;
-    float ddx = r.dx, ddy = r.dy, ddz = r.dz;
-    if (fabsf(ddx) < 1e-30f) ddx = copysignf(1e-30f, ddx);
-    if (fabsf(ddy) < 1e-30f) ddy = copysignf(1e-30f, ddy);
-    if (fabsf(ddz) < 1e-30f) ddz = copysignf(1e-30f, ddz);
-    r.ix = 1.0f / ddx; r.iy = 1.0f / ddy; r.iz = 1.0f / ddz;
+    // A zero component gives an infinite inverse (see slab_hit).
+    r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
     r.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
 }
 
@@ -107,6 +104,22 @@ struct HitRec {
     float u, v;
 };
 
+constexpr float kSlabBig = 3.0e38f;
+
+// Hit test of one widened slab interval. Per axis the plane distances are
+// (plane - o) * (1/d); with d = 0 they are +-inf, or NaN (0 * inf) for a ray
+// that runs inside a plane: fmaxf/fminf ignore a NaN operand, so such an
+// axis does not bound the interval (the slab is closed). An interval that
+// starts at +inf (or ends at -inf) is clamped to a finite value first so
+// that the widening cannot turn it into NaN, and it stays a miss.
+RT_D bool slab_hit(float tn, float tf, float tmin, float tb) {
+    tn = fminf(tn, kSlabBig);
+    tf = fmaxf(tf, -kSlabBig);
+    tn = fmaxf(__builtin_fmaf(-fabsf(tn), kBoxEps, tn), tmin);
+    tf = fminf(__builtin_fmaf(fabsf(tf), kBoxEps, tf), tb);
+    return tn <= tf;
+}
+
 // Slab test of the 8 children; returns the hit mask in slot order (before
 // masking out empty slots). The interval is widened by 2^-16 of |t| with one
 // fused multiply-add each (conservative: never rejects a box whose triangle
@@ -134,11 +147,40 @@ RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
     uint32_t hitm = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
-        float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
-        tn = fmaxf(__builtin_fmaf(-fabsf(tn), kBoxEps, tn), r.tmin);
-        tf = fminf(__builtin_fmaf(fabsf(tf), kBoxEps, tf), tb);
-        hitm |= (tn <= tf ? 1u : 0u) << k;
+        const float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
+        const float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
+        hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+    }
+    return hitm;
+}
+
+// Byte j of w as float times s plus p, one rounding (q * s is exact).
+RT_D float dq8(uint32_t w, int j, float s, float p) {
+    return __builtin_fmaf((float)((w >> (8 * j)) & 0xFFu), s, p);
+}
+
+// node_hits for a quantized node: q[0] = (p.xyz, exponents), q[2..4] =
+// (qlo[0..3], qlo[4..7], qhi[0..3], qhi[4..7]) for x, y, z. The decoded
+// planes go through the same slab test as the FP32 node.
+RT_D uint32_t node_hits_q8(const TRay& r, uint4 h0, uint4 qx, uint4 qy, uint4 qz, float tb) {
+    const float px = __uint_as_float(h0.x), py = __uint_as_float(h0.y), pz = __uint_as_float(h0.z);
+    const float sx = __uint_as_float((h0.w & 0xFFu) << 23);
+    const float sy = __uint_as_float(((h0.w >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((h0.w >> 16) & 0xFFu) << 23);
+    const bool bx = r.ix >= 0.0f, by = r.iy >= 0.0f, bz = r.iz >= 0.0f;
+    const uint32_t nx0 = bx ? qx.x : qx.z, nx1 = bx ? qx.y : qx.w, fx0 = bx ? qx.z : qx.x, fx1 = bx ? qx.w : qx.y;
+    const uint32_t ny0 = by ? qy.x : qy.z, ny1 = by ? qy.y : qy.w, fy0 = by ? qy.z : qy.x, fy1 = by ? qy.w : qy.y;
+    const uint32_t nz0 = bz ? qz.x : qz.z, nz1 = bz ? qz.y : qz.w, fz0 = bz ? qz.z : qz.x, fz1 = bz ? qz.w : qz.y;
+    uint32_t hitm = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int j = k & 3;
+        const float NX = dq8(k < 4 ? nx0 : nx1, j, sx, px), FX = dq8(k < 4 ? fx0 : fx1, j, sx, px);
+        const float NY = dq8(k < 4 ? ny0 : ny1, j, sy, py), FY = dq8(k < 4 ? fy0 : fy1, j, sy, py);
+        const float NZ = dq8(k < 4 ? nz0 : nz1, j, sz, pz), FZ = dq8(k < 4 ? fz0 : fz1, j, sz, pz);
+        const float tn = fmaxf(fmaxf((NX - r.ox) * r.ix, (NY - r.oy) * r.iy), (NZ - r.oz) * r.iz);
+        const float tf = fminf(fminf((FX - r.ox) * r.ix, (FY - r.oy) * r.iy), (FZ - r.oz) * r.iz);
+        hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
     }
     return hitm;
 }
@@ -162,16 +204,26 @@ struct TravState {
 // group of the stack). Returns false when the traversal is finished.
 // `stk` points at this lane's first LDS stack entry; entries are `stride`
 // uint2 apart.
-template <bool COUNT>
+// Q selects the node format: false = FP32 Bvh8Node, true = QBvh8Node.
+template <bool COUNT, bool Q>
 RT_D bool trav_step(const SceneView& sv, const TRay& r, HitRec& h, TravState& ts, uint2* stk, uint32_t stride,
                     uint32_t& nvis, uint32_t& ntst) {
-    const Bvh8Node* node = sv.nodes + ts.cur;
-    const uint4 hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
+    uint4 hdr;
+    uint32_t hitm;
+    if constexpr (Q) {
+        const uint4* qn = reinterpret_cast<const uint4*>(sv.qnodes + ts.cur);
+        const uint4 h0 = qn[0];
+        hdr = qn[1];
+        hitm = node_hits_q8(r, h0, qn[2], qn[3], qn[4], h.t);
+    } else {
+        const Bvh8Node* node = sv.nodes + ts.cur;
+        hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
+        hitm = node_hits(r, node, h.t);
+    }
     const uint32_t child_base = hdr.x & 0xFFFFFFu, tri_base = hdr.y & 0xFFFFFFu;
     const uint32_t imask = hdr.x >> 24, leafmask = hdr.y >> 24;
     const uint32_t meta_lo = hdr.z, meta_hi = hdr.w;
     if (COUNT) nvis++;
-    const uint32_t hitm = node_hits(r, node, h.t);
     uint32_t om = permute8(hitm & imask, r.oct);
     uint32_t lm = permute8(hitm & leafmask, r.oct);
     // leaves of this node, in slot order i ^ oct

@@ -85,6 +85,12 @@ struct TraceInput {
 // counters (optional, counting): per-ray {visits, tests}.
 hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                         unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st);
+hipError_t launch_trace_f32(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                            hipStream_t st);
+hipError_t launch_trace_q8(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                           unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                           hipStream_t st);
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                         uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);

@@ -51,6 +51,7 @@ def lib():
         "reina_rt_debug_math": [i32, c_p, c_p, c_p, u32, c_p],
         "reina_rt_bvh_build_host": [c_p, u64, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64),
                                     ctypes.POINTER(u32)],
+        "reina_rt_bvh_quantize_host": [c_p, u64, c_p, ctypes.POINTER(u64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -72,7 +73,8 @@ def exported_symbols():
     return ["reina_rt_abi_version", "reina_rt_last_error", "reina_rt_scene_create", "reina_rt_scene_destroy",
             "reina_rt_scene_info", "reina_rt_scene_export_bvh", "reina_rt_trace", "reina_rt_trace_stats",
             "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
-            "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning"]
+            "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning",
+            "reina_rt_bvh_quantize_host"]
 
 
 class DeviceScene:
@@ -154,6 +156,17 @@ def bvh_build_host(world_tris):
     check(lib().reina_rt_bvh_build_host(t.ctypes.data if len(t) else None, len(t), nodes.ctypes.data,
                                         ctypes.byref(nb), tris.ctypes.data, ctypes.byref(tb), ctypes.byref(dep)))
     return nodes, tris, dep.value
+
+
+def bvh_quantize_host(nodes):
+    """8-bit quantized copy (80-byte nodes, u8 array) of FP32 BVH8 nodes (u8 array)."""
+    n = np.ascontiguousarray(nodes, np.uint8)
+    qb = ctypes.c_uint64(0)
+    check(lib().reina_rt_bvh_quantize_host(n.ctypes.data if n.size else None, n.nbytes, None, ctypes.byref(qb)))
+    q = np.zeros(qb.value, np.uint8)
+    check(lib().reina_rt_bvh_quantize_host(n.ctypes.data if n.size else None, n.nbytes, q.ctypes.data,
+                                           ctypes.byref(qb)))
+    return q
 
 
 def postprocess(in_ptr, scratch_ptr, out_ptr, width, height, radius=5.0, threshold=1.0, intensity=0.05,

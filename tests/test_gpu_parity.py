@@ -141,6 +141,75 @@ def test_intersect_bitexact_icosphere(cuda, icosphere_arrays):
     dev.close()
 
 
+def test_intersect_quantized_bitexact(cuda, icosphere_arrays):
+    """8-bit quantized nodes: the GPU hit equals brute force bit for bit, and
+    the per-ray node visits / triangle tests equal the oracle's traversal of
+    the same quantized nodes."""
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    dev = reina_rt.DeviceScene(arrays, flags=abi.NODES_QUANTIZED)
+    info = dev.info()
+    assert info.quantized == 1 and info.node_bytes == 80
+    orc = oracle.OracleScene(arrays, make_desc)
+    nodes, tris = dev.export_bvh()
+    assert len(nodes) == 80 * info.num_nodes
+    orc.set_bvh(nodes, tris, 80)
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], 1920, 1080, 1, 8)
+    prim = camera_rays(pc, 1920, 1080, 3000, 13)
+    ph = orc.brute32(prim)
+    rays = np.concatenate([prim, secondary_rays(orc, prim, ph, 14), random_rays(2000, -1.5, 1.5, 15)])
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref = orc.brute32(rays)
+    ref_v, ref_c = orc.bvh(rays)
+    assert np.array_equal(hits["prim"], ref["prim"])
+    for f in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[f]), _bits(ref[f])), f
+    assert np.array_equal(cnt, ref_c)
+    dev.close()
+
+
+def test_render_quantized_equals_fp32(cuda, icosphere_arrays):
+    """The closest hit does not depend on the node format, so a frame traced
+    over quantized nodes is bit-identical to the FP32-node frame."""
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    W, H = 160, 90
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 8)
+    dev32 = reina_rt.DeviceScene(arrays)
+    devq = reina_rt.DeviceScene(arrays, flags=abi.NODES_QUANTIZED)
+    a = gpu_render(torch, dev32, pc, W, H)
+    b = gpu_render(torch, devq, pc, W, H)
+    same = (_bits(a) == _bits(b)) | (np.isnan(a) & np.isnan(b))
+    assert same.all(), "pixels differing: %d" % (~same).any(-1).sum()
+    assert dev32.stats().extension_rays == devq.stats().extension_rays
+    dev32.close()
+    devq.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.NODES_QUANTIZED])
+def test_intersect_zero_direction_components(cuda, flags):
+    """Rays inside the coordinate planes / along the axes (direction
+    components exactly 0) aimed at mesh vertices and edges in those planes:
+    GPU == brute force, counters == the oracle's traversal."""
+    from test_oracle_geometry import plane_rays, soup_arrays
+    torch = cuda
+    v, f = scenes.subdivided_icosphere(3)
+    v = v.astype(np.float32)
+    arrays = soup_arrays(v[f].reshape(-1, 9))
+    dev = reina_rt.DeviceScene(arrays, flags=flags)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh(), dev.info().node_bytes)
+    rays = plane_rays(v, 20000, 22)
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref = orc.brute32(rays)
+    ref_v, ref_c = orc.bvh(rays)
+    assert np.array_equal(hits["prim"], ref["prim"])
+    for fl in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[fl]), _bits(ref[fl])), fl
+    assert np.array_equal(cnt, ref_c)
+    dev.close()
+
+
 def test_intersect_vs_fp64(cuda, icosphere_arrays):
     """Hit id agrees with the FP64 oracle on >= 99.99% of rays; every
     disagreement is an FP64-detected edge/vertex/equal-t tie; t within 1e-6

@@ -32,15 +32,20 @@ def rays_from(o, d, tmin=0.0, tmax=10000.0):
 
 
 def check_bvh_equals_brute(arrays, rays):
+    """The BVH8 traversal, with FP32 nodes and with 8-bit quantized nodes,
+    returns the brute-force closest hit bit for bit."""
     orc = oracle.OracleScene(arrays, make_desc)
     nodes, tris, depth = reina_rt.bvh_build_host(world_triangles(arrays))
-    orc.set_bvh(nodes, tris)
     hb = orc.brute32(rays)
-    hv, cnt = orc.bvh(rays)
-    for f in ("t", "u", "v"):
-        assert np.array_equal(hb[f].view(np.uint32), hv[f].view(np.uint32)), f
-    assert np.array_equal(hb["prim"], hv["prim"])
-    return hb, cnt, depth
+    cnts = []
+    for nd, nbytes in ((nodes, 256), (reina_rt.bvh_quantize_host(nodes), 80)):
+        orc.set_bvh(nd, tris, nbytes)
+        hv, cnt = orc.bvh(rays)
+        for f in ("t", "u", "v"):
+            assert np.array_equal(hb[f].view(np.uint32), hv[f].view(np.uint32)), (f, nbytes)
+        assert np.array_equal(hb["prim"], hv["prim"]), nbytes
+        cnts.append(cnt)
+    return hb, cnts[0], depth
 
 
 def test_bvh_equals_brute_random_soup():
@@ -112,6 +117,61 @@ def test_bvh_structure():
     assert seen_nodes == set(range(N))
 
 
+def decode_quantized(q):
+    """(N, 6, 8) float32 decoded child planes of 80-byte quantized nodes, in
+    the FP32 node's order lo_x hi_x lo_y hi_y lo_z hi_z. q * s is exact in
+    float32 (8-bit q, power-of-two s), so the float32 product plus p rounds
+    once, exactly like fmaf(q, s, p)."""
+    N = len(q) // 80
+    qb = q.reshape(N, 80)
+    p = qb[:, :12].copy().view(np.float32)
+    ex = qb[:, 12:16].copy().view(np.uint32)[:, 0]
+    out = np.zeros((N, 6, 8), np.float32)
+    for a in range(3):
+        sc = ((((ex >> (8 * a)) & 0xFF).astype(np.uint32)) << 23).view(np.float32)
+        for h in range(2):
+            qq = qb[:, 32 + 16 * a + 8 * h: 40 + 16 * a + 8 * h].astype(np.float32)
+            out[:, 2 * a + h] = qq * sc[:, None] + p[:, a:a + 1]
+    return out
+
+
+@pytest.mark.parametrize("kind", ["soup", "icosphere", "flat"])
+def test_quantized_nodes_contain_fp32(kind, icosphere_arrays):
+    """Every decoded 8-bit child box contains the FP32 child box, the grid
+    step is the smallest power of two that can cover the node (so boxes grow
+    by at most a few grid steps), and topology, header and meta are the FP32
+    node's."""
+    rng = np.random.default_rng(11)
+    if kind == "soup":
+        tris = (rng.uniform(-50, 50, (30000, 1)) + rng.normal(size=(30000, 9))).astype(np.float32)
+    elif kind == "icosphere":
+        tris = world_triangles(icosphere_arrays[0])
+    else:   # thin and far from the origin: grid steps far below the origin's ulp
+        tris = rng.uniform(-1, 1, (5000, 9)).astype(np.float32)
+        tris[:, 1::3] = 1e-3 * tris[:, 1::3] + 1000.0
+    nodes, t, depth = reina_rt.bvh_build_host(tris)
+    q = reina_rt.bvh_quantize_host(nodes)
+    N = len(nodes) // 256
+    assert len(q) == 80 * N
+    nb, qb = nodes.reshape(N, 256), q.reshape(N, 80)
+    assert np.array_equal(qb[:, 16:32], nb[:, 192:208])
+    f = nb[:, :192].copy().view(np.float32).reshape(N, 6, 8)
+    d = decode_quantized(q)
+    valid = nb[:, 200:208] != 0
+    ex = qb[:, 12:16].copy().view(np.uint32)[:, 0]
+    for a in range(3):
+        lo, hi = f[:, 2 * a], f[:, 2 * a + 1]
+        assert (d[:, 2 * a][valid] <= lo[valid]).all()
+        assert (d[:, 2 * a + 1][valid] >= hi[valid]).all()
+        s = ((((ex >> (8 * a)) & 0xFF).astype(np.uint32)) << 23).view(np.float32).astype(np.float64)
+        ext = np.where(valid, hi, -np.inf).max(1).astype(np.float64) - np.where(valid, lo, np.inf).min(1)
+        # minimal step: half the step would not cover the node extent with 255 steps
+        big = s > 2.0 ** -125
+        assert (ext[big] >= 255 * s[big] / 2).all()
+        grow = (d[:, 2 * a + 1] - hi).astype(np.float64) + (lo - d[:, 2 * a])
+        assert (grow[valid] <= (3 * s[:, None] + 4 * np.spacing(np.abs(hi).astype(np.float32)))[valid]).all()
+
+
 def test_empty_and_tiny_scenes():
     for n in (0, 1, 2, 3, 4, 9):
         rng = np.random.default_rng(n)
@@ -138,6 +198,46 @@ def test_watertight_closed_mesh():
     d[: len(tgt)] = tgt - o[: len(tgt)]
     hb, cnt, _ = check_bvh_equals_brute(arrays, rays_from(o, d))
     assert (hb["prim"] != 0xFFFFFFFF).all()
+
+
+def plane_rays(v, n, seed):
+    """Rays that run inside the coordinate planes x=0, y=0, z=0 (direction
+    component exactly 0, origin component exactly 0) or along the axes, aimed
+    at mesh vertices in those planes: they hit shared edges and vertices
+    exactly (ties) and graze the boxes' faces at zero distance."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    for i in range(n):
+        a = i % 4
+        if a < 3:
+            o[i, a] = 0.0
+            d[i, a] = 0.0
+            on = v[v[:, a] == 0]
+            if len(on) and i % 8 < 4:
+                d[i] = on[rng.integers(len(on))] - o[i]
+                d[i, a] = 0.0
+        else:   # axis-parallel through a vertex's two coordinates
+            ax = rng.integers(3)
+            p = v[rng.integers(len(v))]
+            o[i] = p
+            o[i, ax] = 3.0
+            d[i] = 0.0
+            d[i, ax] = -1.0
+    return rays_from(o, d)
+
+
+def test_zero_direction_components():
+    """Zero direction components (infinite inverse direction) with origins
+    exactly on box planes: BVH traversal still equals brute force (a
+    regression: a tiny stand-in for 1/0 made the slab end right after the
+    origin when a box face sat one denormal away from it)."""
+    v, f = scenes.subdivided_icosphere(3)
+    v = v.astype(np.float32)
+    tris = v[f].reshape(-1, 9)
+    arrays = soup_arrays(tris)
+    hb, cnt, _ = check_bvh_equals_brute(arrays, plane_rays(v, 20000, 21))
+    assert (hb["prim"] != 0xFFFFFFFF).mean() > 0.3
 
 
 def test_invalid_rays_miss(cornell_arrays):
