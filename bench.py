@@ -35,6 +35,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "reina-vk_amd"))
 
+def log(*args):
+    """Progress to stderr (the JSON result line is the only stdout output)."""
+    print("[bench %.1fs]" % (time.perf_counter() - T0), *args, file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
 METRIC = "Mrays/s + % HBM roofline at 1080p; served images/s and p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 208
@@ -57,6 +63,9 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
+    ap.add_argument("--instanced", action="store_true",
+                    help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
+    ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     a = ap.parse_args()
@@ -82,6 +91,7 @@ def main():
     dev = reina_rt.DeviceScene(arrays)
     info = dev.info()
     setup_s = time.perf_counter() - t0
+    log("scene ready: %d triangles, %d nodes" % (info.num_triangles, info.num_nodes))
     img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
@@ -93,7 +103,7 @@ def main():
 
     sec = sh = None
     if not (a.no_secondary and a.no_autotune and a.no_quantized):
-        sec, sh = build_secondary(a, arrays, dev, cam, W, H, torch)
+        sec, sh = build_secondary(a.secondary_reps, arrays, dev, cam, W, H, torch)
     autotune = None
     if not a.no_autotune:
         autotune = run_autotune(dev, sec, sh, torch)
@@ -101,8 +111,10 @@ def main():
         with open(os.path.join(ROOT, "gpurun_out", "autotune_table.json"), "w") as f:
             json.dump(autotune.pop("table"), f)
 
+    log("timing %d frames" % a.steps)
     rays, elapsed, gpu_ms = time_frames(frame, a.warmup, a.steps, torch)
     mrays = rays / elapsed / 1e6
+    log("%.1f Mrays/s" % mrays)
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -139,7 +151,12 @@ def main():
         out["secondary_rays"]["tuning"] = dev.get_tuning()
 
     if not a.no_quantized:
+        log("quantized nodes")
         out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
+
+    if a.instanced:
+        log("instanced 8.4M-triangle scene")
+        out["instanced_8m"] = instanced_block(a, torch)
 
     if a.calib_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -148,6 +165,7 @@ def main():
         torch.cuda.synchronize()
 
     if not a.no_cpu_baseline:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a, arrays, dev, cam, W, H)
     dev.close()
     print(json.dumps(out), flush=True)
@@ -252,7 +270,116 @@ def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
     devq.close()
     return res
 
-def build_secondary(a, arrays, dev, cam, W, H, torch):
+
+def instanced_block(a, torch):
+    """4096 instances of a 2048-triangle sphere flattened to 8,388,608
+    triangles, 1920x1080, 32 spp, depth 12: frames over FP32 and over 8-bit
+    quantized BVH8 nodes, and the closest hit of a sampled ray set (camera
+    rays + cosine bounce rays) against the FP64 oracle."""
+    import numpy as np
+
+    import oracle
+    import reina_rt
+    from reina_rt import abi, scenes
+    from reina_rt.scene import make_desc, push_consts
+
+    W, H, spp, depth = 1920, 1080, 32, 12
+    scene, cam = scenes.instanced()
+    arrays = scene.build_arrays()
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda:0")
+    sp = torch.cuda.current_stream().cuda_stream
+    res = {"workload": "4096 instances x 2048-tri sphere = 8,388,608 tris, %dx%d, %d spp, depth %d" % (W, H, spp, depth)}
+    devs = {}
+    for name, flags in (("fp32", 0), ("quantized", abi.NODES_QUANTIZED)):
+        t0 = time.perf_counter()
+        d = reina_rt.DeviceScene(arrays, flags=flags)
+        inf = d.info()
+        devs[name] = d
+        log("instanced: %s scene built in %.1f s" % (name, time.perf_counter() - t0))
+        res[name] = {"scene_setup_s": round(time.perf_counter() - t0, 2), "node_bytes": int(inf.node_bytes),
+                     "bvh_nodes": int(inf.num_nodes), "bvh_depth": int(inf.max_depth),
+                     "scene_MB": round(inf.device_bytes / 1e6, 1),
+                     "node_MB": round(inf.num_nodes * inf.node_bytes / 1e6, 1)}
+
+    def frame(batch, d):
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, depth, sample_batch=batch)
+        d.trace(pc, img.data_ptr(), W, H, stream=sp)
+        return d.stats()
+
+    # sampled rays: camera rays + cosine bounce rays from their hits
+    prim, sec, sh = build_secondary(1, arrays, devs["fp32"], cam, W, H, torch, with_primary=True)
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(77)
+    n = a.fp64_rays // 2
+    rays = torch.cat([prim[torch.randperm(prim.shape[0], device="cuda:0", generator=g)[:n]],
+                      sec[torch.randperm(sec.shape[0], device="cuda:0", generator=g)[:n]]]).contiguous()
+    rays_np = rays.cpu().numpy().view(abi.RAY).reshape(-1)
+    hits = {}
+    for name, d in devs.items():
+        h = torch.zeros((rays.shape[0], 4), device="cuda:0", dtype=torch.float32)
+        d.intersect(rays.data_ptr(), h.data_ptr(), rays.shape[0], stream=sp)
+        torch.cuda.synchronize()
+        hits[name] = h.cpu().numpy().view(abi.HIT).reshape(-1)
+    orc = oracle.OracleScene(arrays, make_desc)
+    nodes, tris = devs["fp32"].export_bvh()
+    orc.set_bvh(nodes, tris, 256)
+    log("instanced: FP64 oracle on %d rays" % len(rays_np))
+    t0 = time.perf_counter()
+    t64, p64, u64, v64, amb = orc.fp64(rays_np, tie_rel=1e-5, use_bvh=True, nthreads=max(1, min(os.cpu_count() or 1, 16)))
+    oracle_s = time.perf_counter() - t0
+    wt = torch.tensor(world_triangles(arrays), device="cuda:0", dtype=torch.float64).reshape(-1, 3, 3)
+    o64 = rays[:, 0:3].double()
+    dn = rays[:, 4:7].double().norm(dim=1)
+    for name in devs:
+        hh = hits[name]
+        agree = hh["prim"] == p64
+        m = agree & (p64 != 0xFFFFFFFF)
+        err = np.abs(hh["t"].astype(np.float64) - t64)
+        idx = torch.from_numpy(np.nonzero(m)[0]).cuda()
+        D = ((wt[torch.from_numpy(p64[m].astype(np.int64)).cuda()] - o64[idx][:, None, :]).norm(dim=2).max(1).values
+             / dn[idx]).cpu().numpy()
+        scale = np.zeros(len(rays_np))
+        scale[m] = np.maximum(t64[m], D)
+        chk = {"rays": int(len(rays_np)), "hit_id_agreement": round(float(agree.mean()), 7),
+               "disagreements": int((~agree).sum()), "disagreements_not_at_ties": int((~agree & ~amb).sum()),
+               "hit_fraction": round(float((p64 != 0xFFFFFFFF).mean()), 4)}
+        for kind, sel in (("camera", np.arange(len(rays_np)) < len(rays_np) // 2),
+                          ("bounce", np.arange(len(rays_np)) >= len(rays_np) // 2)):
+            mm = m & sel
+            rel = err[mm] / t64[mm]
+            sc = err[mm] / scale[mm]
+            chk[kind] = {"hits": int(mm.sum()), "t_rel_err_max": float(rel.max()),
+                         "t_rel_err_p999": float(np.quantile(rel, 0.999)),
+                         "frac_t_rel_err_le_1e-6": round(float((rel <= 1e-6).mean()), 6),
+                         "t_err_over_vertex_scale_max": float(sc.max()),
+                         "frac_t_err_over_vertex_scale_le_1e-6": round(float((sc <= 1e-6).mean()), 6)}
+        res[name]["fp64_check"] = chk
+    del wt
+    res["fp64_oracle_s"] = round(oracle_s, 1)
+    res["hits_bit_identical_fp32_vs_quantized"] = bool(
+        np.array_equal(hits["fp32"].view(np.uint32), hits["quantized"].view(np.uint32)))
+    # frames
+    for name, d in devs.items():
+        if not a.no_autotune:
+            at = run_autotune(d, sec, torch.zeros((sec.shape[0], 4), device="cuda:0"), torch)
+            at.pop("table")
+            res[name]["autotune_best"] = at["best"]
+        log("instanced: %s frames" % name)
+        rays_f, elapsed, gpu_ms = time_frames(lambda b, d=d: frame(b, d), 1, 2, torch)
+        res[name]["mrays_per_s"] = round(rays_f / elapsed / 1e6, 2)
+        res[name]["ms_per_frame"] = round(elapsed / 2 * 1e3, 2)
+        res[name]["rays_per_frame"] = int(rays_f // 2)
+        if not a.no_roofline:
+            res[name]["roofline"] = roofline(d, frame, 208 if name == "fp32" else 80, torch)
+    frame(0, devs["fp32"])
+    ref = img.clone()
+    frame(0, devs["quantized"])
+    res["frame_bit_identical_fp32_vs_quantized"] = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
+    for d in devs.values():
+        d.close()
+    return res
+
+def build_secondary(reps, arrays, dev, cam, W, H, torch, with_primary=False):
     """Incoherent secondary rays of the same frame: one pinhole primary ray per
     pixel, closest hit on the GPU, then `--secondary-reps` cosine-distributed
     directions about the geometric normal of every hit (origin offset 1e-4
@@ -292,7 +419,6 @@ def build_secondary(a, arrays, dev, cam, W, H, torch):
     d0 = prim[m, 4:7]
     ng = torch.where(((ng * d0).sum(1) > 0)[:, None], -ng, ng)
     p = prim[m, 0:3] + hits[m, 0:1] * d0
-    reps = a.secondary_reps
     g = torch.Generator(device=dev_)
     g.manual_seed(1234)
     ng = ng.repeat(reps, 1)
@@ -313,6 +439,8 @@ def build_secondary(a, arrays, dev, cam, W, H, torch):
     sec[:, 4:7] = d
     sec[:, 7] = 10000.0
     sh = torch.zeros((n2, 4), device=dev_, dtype=torch.float32)
+    if with_primary:
+        return prim, sec, sh
     return sec, sh
 
 

@@ -40,7 +40,7 @@ def lib():
     L.or_intersect_brute32.restype = None
     L.or_intersect_bvh.argtypes = [c_p, c_p, c_p, c_p, u64, i32]
     L.or_intersect_bvh.restype = None
-    L.or_intersect_fp64.argtypes = [c_p, c_p, u64, c_p, c_p, c_p, c_p, c_p, f64, i32]
+    L.or_intersect_fp64.argtypes = [c_p, c_p, u64, c_p, c_p, c_p, c_p, c_p, f64, i32, i32]
     L.or_intersect_fp64.restype = None
     L.or_random.argtypes = [ctypes.POINTER(u32)]
     L.or_random.restype = ctypes.c_float
@@ -98,14 +98,18 @@ class OracleScene:
         lib().or_intersect_bvh(self.h, _p(rays), _p(hits), _p(cnt), len(rays), nthreads)
         return hits, cnt
 
-    def fp64(self, rays, tie_rel=1e-5, nthreads=8):
+    def fp64(self, rays, tie_rel=1e-5, use_bvh=False, nthreads=8):
+        """FP64 Moller-Trumbore closest hit (argmin of (t, id)) and an
+        ambiguity flag (edge/vertex/equal-t ties and near misses within
+        tie_rel). use_bvh: cull with the FP32 BVH8 given to set_bvh (boxes
+        padded by tie_rel; same result as the full scan)."""
         n = len(rays)
         t = np.zeros(n)
         prim = np.zeros(n, np.uint32)
         u = np.zeros(n)
         v = np.zeros(n)
         amb = np.zeros(n, np.uint8)
-        lib().or_intersect_fp64(self.h, _p(rays), n, _p(t), _p(prim), _p(u), _p(v), _p(amb), tie_rel, nthreads)
+        lib().or_intersect_fp64(self.h, _p(rays), n, _p(t), _p(prim), _p(u), _p(v), _p(amb), tie_rel, int(use_bvh), nthreads)
         return t, prim, u, v, amb.astype(bool)
 
     def render(self, pc, W, H, img=None, rect=None, use_bvh=True, nthreads=8):

@@ -350,52 +350,126 @@ static int mt64(d3 o, d3 d, const float* v, double tmin, double tmax, double* t,
     return 1;
 }
 
-/* Signed-area margin used to flag near-edge hits / near misses. */
+/* Per-ray FP64 closest-hit accumulator. Order independent: the best hit is
+ * the argmin of (t, id); t2 is the second smallest hit t; nm_t is the
+ * smallest t of a "near miss" (a triangle missed only by a barycentric
+ * margin m). */
+typedef struct {
+    double tb, t2, ub, vb, nm_t;
+    uint32_t ib;
+} F64Acc;
+
+static void f64_init(F64Acc* a, const ReinaRay* ray) {
+    a->tb = ray->tmax; a->t2 = INFINITY; a->ub = 0; a->vb = 0; a->nm_t = INFINITY;
+    a->ib = 0xFFFFFFFFu;
+}
+
+static void f64_tri(F64Acc* a, d3 o, d3 d, const ReinaRay* ray, const float* v, uint32_t g, double m) {
+    double t, u, w;
+    if (mt64(o, d, v, ray->tmin, ray->tmax, &t, &u, &w)) {
+        if (t < a->tb || (t == a->tb && g < a->ib)) {
+            if (a->ib != 0xFFFFFFFFu) a->t2 = a->tb < a->t2 ? a->tb : a->t2;
+            a->tb = t; a->ib = g; a->ub = u; a->vb = w;
+        } else if (t < a->t2) {
+            a->t2 = t;
+        }
+        return;
+    }
+    /* near miss: a hit of the triangle enlarged by the margin m? */
+    d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
+    d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
+    d3 p = dcross(d, e2);
+    double det = ddot(e1, p);
+    if (det == 0.0) return;
+    double inv = 1.0 / det;
+    d3 sv = dsub(o, v0);
+    double uu = ddot(sv, p) * inv;
+    d3 q = dcross(sv, e1);
+    double vv = ddot(d, q) * inv;
+    double tt = ddot(e2, q) * inv;
+    if (uu >= -m && vv >= -m && uu + vv <= 1.0 + m && tt >= ray->tmin - m && tt <= ray->tmax * (1 + m))
+        if (tt < a->nm_t) a->nm_t = tt;
+}
+
+static void f64_finish(const F64Acc* a, const ReinaRay* ray, double tie_rel, double* to, uint32_t* po,
+                       double* uo, double* vo, uint8_t* amb) {
+    *to = a->tb; *po = a->ib; *uo = a->ub; *vo = a->vb;
+    int am = a->nm_t <= a->tb * (1.0 + tie_rel);
+    if (a->ib != 0xFFFFFFFFu) {
+        double wb = 1.0 - a->ub - a->vb;
+        if (a->ub < tie_rel || a->vb < tie_rel || wb < tie_rel) am = 1;
+        if (a->t2 <= a->tb * (1.0 + tie_rel) + 1e-300) am = 1;
+        if (a->tb <= ray->tmin * (1.0 + tie_rel) + tie_rel) am = 1;
+    }
+    *amb = (uint8_t)am;
+}
+
 static void fp64_one(const OrScene* s, const ReinaRay* ray, double* to, uint32_t* po, double* uo,
                      double* vo, uint8_t* amb, double tie_rel) {
     d3 o = D3(ray->ox, ray->oy, ray->oz), d = D3(ray->dx, ray->dy, ray->dz);
-    double tb = ray->tmax, t2 = INFINITY, ub = 0, vb = 0;
-    uint32_t ib = 0xFFFFFFFFu;
-    int near_miss = 0;
-    /* widen barycentric acceptance to find near misses */
-    for (uint64_t g = 0; g < s->ntri; ++g) {
-        double t, u, w;
-        const float* v = &s->wtri[9 * g];
-        if (mt64(o, d, v, ray->tmin, ray->tmax, &t, &u, &w)) {
-            if (t < tb || (t == tb && (uint32_t)g < ib)) {
-                if (ib != 0xFFFFFFFFu) t2 = tb < t2 ? tb : t2;
-                tb = t; ib = (uint32_t)g; ub = u; vb = w;
-            } else if (t < t2) {
-                t2 = t;
+    F64Acc a;
+    f64_init(&a, ray);
+    for (uint64_t g = 0; g < s->ntri; ++g) f64_tri(&a, o, d, ray, &s->wtri[9 * g], (uint32_t)g, tie_rel);
+    f64_finish(&a, ray, tie_rel, to, po, uo, vo, amb);
+}
+
+/* The same FP64 query with the FP32 BVH8 (256-byte nodes) as a culling
+ * structure only: every child box is padded by tie_rel times its largest
+ * extent (which covers the margin-enlarged triangles of its subtree) and
+ * tested in double; boxes entered beyond tb * (1 + tie_rel) are skipped.
+ * Result identical to fp64_one (the accumulator is order independent). */
+static void fp64_bvh_one(const OrScene* s, const ReinaRay* ray, double* to, uint32_t* po, double* uo,
+                         double* vo, uint8_t* amb, double tie_rel) {
+    d3 o = D3(ray->ox, ray->oy, ray->oz), d = D3(ray->dx, ray->dy, ray->dz);
+    F64Acc a;
+    f64_init(&a, ray);
+    double dir[3] = {d.x, d.y, d.z}, org[3] = {o.x, o.y, o.z};
+    double inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    uint32_t stack[64 * 8];
+    int sp = 0;
+    if (s->nnodes && s->node_bytes == 256) stack[sp++] = 0;
+    while (sp > 0) {
+        const uint8_t* nb = s->nodes + (uint64_t)stack[--sp] * 256;
+        const float* f = (const float*)nb;
+        uint32_t w48, w49;
+        memcpy(&w48, nb + 192, 4);
+        memcpy(&w49, nb + 196, 4);
+        uint32_t child_base = w48 & 0xFFFFFFu, tri_base = w49 & 0xFFFFFFu;
+        const uint8_t* meta = nb + 200;
+        for (int k = 0; k < 8; ++k) {
+            uint8_t mk = meta[k];
+            if (!mk) continue;
+            double lo[3] = {f[0 + k], f[16 + k], f[32 + k]}, hi[3] = {f[8 + k], f[24 + k], f[40 + k]};
+            double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+            double pad = 2.0 * tie_rel * ext + 1e-12;
+            double tn = -INFINITY, tf = INFINITY;
+            for (int ax = 0; ax < 3; ++ax) {
+                double l = lo[ax] - pad, h = hi[ax] + pad;
+                if (dir[ax] == 0.0) {   /* runs parallel to the slab: inside or never */
+                    if (org[ax] < l || org[ax] > h) tn = INFINITY;
+                    continue;
+                }
+                double t0 = (l - org[ax]) * inv[ax], t1 = (h - org[ax]) * inv[ax];
+                tn = fmax(tn, fmin(t0, t1));
+                tf = fmin(tf, fmax(t0, t1));
             }
-        } else {
-            /* near miss: hit with a slightly enlarged triangle? */
-            d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
-            d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
-            d3 p = dcross(d, e2);
-            double det = ddot(e1, p);
-            if (det == 0.0) continue;
-            double inv = 1.0 / det;
-            d3 sv = dsub(o, v0);
-            double uu = ddot(sv, p) * inv;
-            d3 q = dcross(sv, e1);
-            double vv = ddot(d, q) * inv;
-            double tt = ddot(e2, q) * inv;
-            double m = tie_rel;
-            if (uu >= -m && vv >= -m && uu + vv <= 1.0 + m && tt >= ray->tmin - m && tt <= ray->tmax * (1 + m)) {
-                if (tt <= tb * (1 + m)) near_miss = 1;
+            tn = fmax(tn - 1e-9 * fabs(tn), ray->tmin - tie_rel);
+            tf = fmin(tf + 1e-9 * fabs(tf), a.tb * (1.0 + tie_rel) + 1e-300);
+            if (!(tn <= tf)) continue;
+            if (mk & 0x80) {
+                if (sp < 64 * 8) stack[sp++] = child_base + (mk & 7u);
+            } else {
+                uint32_t cntl = (mk >> 5) & 3u, off = mk & 31u;
+                for (uint32_t j = 0; j < cntl; ++j) {
+                    const float* tv = s->btris + 12ull * (tri_base + off + j);
+                    uint32_t id;
+                    memcpy(&id, &tv[3], 4);
+                    f64_tri(&a, o, d, ray, &s->wtri[9ull * id], id, tie_rel);
+                }
             }
         }
     }
-    *to = tb; *po = ib; *uo = ub; *vo = vb;
-    int a = near_miss;
-    if (ib != 0xFFFFFFFFu) {
-        double wb = 1.0 - ub - vb;
-        if (ub < tie_rel || vb < tie_rel || wb < tie_rel) a = 1;
-        if (t2 <= tb * (1.0 + tie_rel) + 1e-300) a = 1;
-        if (tb <= ray->tmin * (1.0 + tie_rel) + tie_rel) a = 1;
-    }
-    *amb = (uint8_t)a;
+    f64_finish(&a, ray, tie_rel, to, po, uo, vo, amb);
 }
 
 /* ---- threading ----------------------------------------------------------- */
@@ -417,7 +491,8 @@ static void* ijob_run(void* arg) {
     for (uint64_t i = j->begin; i < j->end; ++i) {
         if (j->kind == 0) brute32_one(j->s, &j->rays[i], &j->hits[i]);
         else if (j->kind == 1) bvh_one(j->s, &j->rays[i], &j->hits[i], j->counters ? &j->counters[2 * i] : NULL);
-        else fp64_one(j->s, &j->rays[i], &j->t[i], &j->prim[i], &j->u[i], &j->v[i], &j->amb[i], j->tie);
+        else if (j->kind == 2) fp64_one(j->s, &j->rays[i], &j->t[i], &j->prim[i], &j->u[i], &j->v[i], &j->amb[i], j->tie);
+        else fp64_bvh_one(j->s, &j->rays[i], &j->t[i], &j->prim[i], &j->u[i], &j->v[i], &j->amb[i], j->tie);
     }
     return NULL;
 }
@@ -452,9 +527,9 @@ void or_intersect_bvh(const OrScene* s, const ReinaRay* rays, ReinaHit* hits, ui
 
 void or_intersect_fp64(const OrScene* s, const ReinaRay* rays, uint64_t n, double* t_out,
                        uint32_t* prim_out, double* u_out, double* v_out, uint8_t* ambiguous,
-                       double tie_rel, int nthreads) {
+                       double tie_rel, int use_bvh, int nthreads) {
     IJob j; memset(&j, 0, sizeof j);
-    j.kind = 2; j.s = s; j.rays = rays; j.t = t_out; j.prim = prim_out; j.u = u_out; j.v = v_out;
+    j.kind = (use_bvh && s->nnodes && s->node_bytes == 256) ? 3 : 2; j.s = s; j.rays = rays; j.t = t_out; j.prim = prim_out; j.u = u_out; j.v = v_out;
     j.amb = ambiguous; j.tie = tie_rel;
     run_jobs(&j, n, nthreads);
 }

@@ -64,7 +64,7 @@ void or_intersect_bvh(const OrScene* s, const ReinaRay* rays, ReinaHit* hits,
                       uint32_t* counters, uint64_t n, int nthreads);
 void or_intersect_fp64(const OrScene* s, const ReinaRay* rays, uint64_t n, double* t_out,
                        uint32_t* prim_out, double* u_out, double* v_out, uint8_t* ambiguous,
-                       double tie_rel, int nthreads);
+                       double tie_rel, int use_bvh, int nthreads);
 
 /* ---- integrator ---------------------------------------------------------- */
 float or_random(uint32_t* state);

@@ -287,5 +287,6 @@ def instanced(copies=4096, seed=11, box=100.0):
         else:
             mat = Material(materialIdx=0, albedo=tuple(rng.uniform(0.2, 0.9, 3)))
         s.add_instance(oid, M, mat)
-    cam = dict(eye=(50.0, 50.0, -40.0), center=(50.0, 50.0, 50.0), fov=60.0)
+    h = 0.5 * box
+    cam = dict(eye=(h, h, -0.4 * box), center=(h, h, h), fov=60.0)
     return s, cam

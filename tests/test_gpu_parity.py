@@ -62,6 +62,18 @@ def camera_rays(pc, W, H, n, seed):
     return rays
 
 
+def fp32_t_error(arrays, rays, hits, t64, p64, m):
+    """|t32 - t64| on the rays in mask m, and the scale the FP32 watertight
+    test's error is proportional to: max(t, largest distance from the ray
+    origin to a vertex of the hit triangle, in units of |d|)."""
+    wt = world_triangles(arrays).reshape(-1, 3, 3).astype(np.float64)
+    o = np.stack([rays["ox"], rays["oy"], rays["oz"]], 1)[m].astype(np.float64)
+    dn = np.linalg.norm(np.stack([rays["dx"], rays["dy"], rays["dz"]], 1)[m].astype(np.float64), axis=1)
+    D = np.linalg.norm(wt[p64[m]] - o[:, None, :], axis=2).max(1) / dn
+    err = np.abs(hits["t"][m].astype(np.float64) - t64[m])
+    return err, np.maximum(t64[m], D)
+
+
 def secondary_rays(orc, rays, hits, seed):
     """Rays leaving the hit points in random directions (origin offset by the
     library's self-intersection rule)."""
@@ -228,6 +240,46 @@ def test_intersect_vs_fp64(cuda, icosphere_arrays):
     m = agree & (p64 != 0xFFFFFFFF)
     rel = np.abs(hits["t"][m].astype(np.float64) - t64[m]) / t64[m]
     assert rel.max() <= 1e-6, "max rel t error %g" % rel.max()
+    dev.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.NODES_QUANTIZED])
+def test_instanced_vs_fp64(cuda, flags):
+    """Instanced spheres (256 copies of the 2,048-triangle sphere; random
+    scale, rotation, position): camera rays and bounce rays against the
+    oracle's traversal (bit-exact) and the FP64 oracle (hit id on >= 99.99%,
+    every disagreement an FP64-detected tie, t within 1e-6 relative)."""
+    torch = cuda
+    scene, cam = scenes.instanced(copies=256, box=40.0)
+    arrays = scene.build_arrays()
+    dev = reina_rt.DeviceScene(arrays, flags=flags)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh(), dev.info().node_bytes)
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], 640, 360, 1, 12)
+    prim = camera_rays(pc, 640, 360, 10000, 41)
+    ph = orc.bvh(prim)[0]
+    rays = np.concatenate([prim, secondary_rays(orc, prim, ph, 42)])
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref_v, ref_c = orc.bvh(rays)
+    assert np.array_equal(hits["prim"], ref_v["prim"])
+    for f in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[f]), _bits(ref_v[f])), f
+    assert np.array_equal(cnt, ref_c)
+    nodes32, tris32, _ = reina_rt.bvh_build_host(world_triangles(arrays))   # FP32 boxes cull the FP64 scan
+    orc.set_bvh(nodes32, tris32, 256)
+    t64, p64, u64, v64, amb = orc.fp64(rays, tie_rel=1e-5, use_bvh=True)
+    agree = hits["prim"] == p64
+    assert agree.mean() >= 0.9999
+    assert (~agree & ~amb).sum() == 0
+    m = agree & (p64 != 0xFFFFFFFF)
+    assert m[:len(prim)].sum() > 500 and m[len(prim):].sum() > 300
+    err, t_err_scale = fp32_t_error(arrays, rays, hits, t64, p64, m)
+    rel = err / t64[m]
+    primary = np.arange(len(rays))[m] < len(prim)
+    # camera rays: 1e-6 relative on >= 99.8% (grazing hits are ill-conditioned)
+    assert (rel[primary] <= 1e-6).mean() >= 0.998 and rel[primary].max() <= 1e-5
+    # bounce rays: the FP32 error scales with the translated vertices, not t
+    assert (err / t_err_scale <= 1e-6).mean() >= 0.99 and (err / t_err_scale).max() <= 1e-5
     dev.close()
 
 

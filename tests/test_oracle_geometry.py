@@ -240,6 +240,32 @@ def test_zero_direction_components():
     assert (hb["prim"] != 0xFFFFFFFF).mean() > 0.3
 
 
+def test_fp64_bvh_culling_equals_full_scan(icosphere_arrays):
+    """The FP64 oracle with BVH culling returns exactly the full scan's hit,
+    t, barycentrics and ambiguity flags (camera rays, random rays, and
+    plane/axis rays that hit edges exactly)."""
+    v, f = scenes.subdivided_icosphere(3)
+    v = v.astype(np.float32)
+    cases = [(soup_arrays(v[f].reshape(-1, 9)), plane_rays(v, 3000, 31))]
+    arrays, cam = icosphere_arrays
+    rng = np.random.default_rng(32)
+    o = np.tile(np.array(cam["eye"], np.float32), (1500, 1))
+    d = rng.normal(size=(1500, 3)) * 0.3 + (np.array(cam["center"]) - np.array(cam["eye"]))
+    cases.append((arrays, np.concatenate([rays_from(o, d), rays_from(rng.uniform(-1.5, 1.5, (1500, 3)),
+                                                                     rng.normal(size=(1500, 3)))])))
+    ambiguous = []
+    for arr, rays in cases:
+        orc = oracle.OracleScene(arr, make_desc)
+        nodes, tris, _ = reina_rt.bvh_build_host(world_triangles(arr))
+        orc.set_bvh(nodes, tris)
+        full = orc.fp64(rays)
+        culled = orc.fp64(rays, use_bvh=True)
+        for a, b in zip(full, culled):
+            assert np.array_equal(a, b)
+        ambiguous.append(int(full[4].sum()))
+    assert ambiguous[0] > 100          # the plane rays hit shared edges: ties are exercised
+
+
 def test_invalid_rays_miss(cornell_arrays):
     arrays, _ = cornell_arrays
     r = rays_from(np.array([[np.nan, 1, 0], [0, 1, 0], [np.inf, 1, 0], [0, 1, 0]]),
