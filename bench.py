@@ -342,6 +342,7 @@ def instanced_block(a, torch):
         scale[m] = np.maximum(t64[m], D)
         chk = {"rays": int(len(rays_np)), "hit_id_agreement": round(float(agree.mean()), 7),
                "disagreements": int((~agree).sum()), "disagreements_not_at_ties": int((~agree & ~amb).sum()),
+               "ambiguous_fraction": round(float(amb.mean()), 5),
                "hit_fraction": round(float((p64 != 0xFFFFFFFF).mean()), 4)}
         for kind, sel in (("camera", np.arange(len(rays_np)) < len(rays_np) // 2),
                           ("bounce", np.arange(len(rays_np)) >= len(rays_np) // 2)):

@@ -331,51 +331,38 @@ static d3 dsub(d3 a, d3 b) { return D3(a.x - b.x, a.y - b.y, a.z - b.z); }
 static double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static d3 dcross(d3 a, d3 b) { return D3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
-static int mt64(d3 o, d3 d, const float* v, double tmin, double tmax, double* t, double* u, double* w) {
-    d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
-    d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
-    d3 p = dcross(d, e2);
-    double det = ddot(e1, p);
-    if (det == 0.0) return 0;
-    double inv = 1.0 / det;
-    d3 sv = dsub(o, v0);
-    double uu = ddot(sv, p) * inv;
-    if (uu < 0.0 || uu > 1.0) return 0;
-    d3 q = dcross(sv, e1);
-    double vv = ddot(d, q) * inv;
-    if (vv < 0.0 || uu + vv > 1.0) return 0;
-    double tt = ddot(e2, q) * inv;
-    if (tt < tmin || tt > tmax) return 0;
-    *t = tt; *u = uu; *w = vv;
-    return 1;
-}
-
 /* Per-ray FP64 closest-hit accumulator. Order independent: the best hit is
- * the argmin of (t, id); t2 is the second smallest hit t; nm_t is the
- * smallest t of a "near miss" (a triangle missed only by a barycentric
- * margin m). */
+ * the argmin of (t, id); t2 is the smallest lower t bound of the other hits;
+ * nm_t the smallest lower t bound of a near miss.
+ *
+ * Ties ("ambiguous" rays) are where FP32 rounding can decide the answer.
+ * The FP32 test translates the vertices to the ray origin and evaluates 2D
+ * edge functions, so its rounding moves an edge by up to about
+ * delta = TIE_ULPS * 2^-24 * max_i |v_i - o| in the plane perpendicular to
+ * the ray. For edge k that is a barycentric margin delta * |edge k| * |d| / |det|
+ * (|det| / |d| is twice the area projected along the ray), and a t margin
+ * delta / |d|; both are at least tie_rel (relative). A ray is ambiguous if
+ * the hit lies within its margin of an edge, if another triangle's hit lies
+ * within the t margins, if a triangle misses only by its margins at a t
+ * within the margins of the hit, or if the hit is within the margin of tmin. */
+#define F32_EPS 5.9604644775390625e-08 /* 2^-24 */
+#define TIE_ULPS 16.0
+
 typedef struct {
     double tb, t2, ub, vb, nm_t;
+    double mu, mv, mw, mt;   /* margins of the best hit */
     uint32_t ib;
 } F64Acc;
 
 static void f64_init(F64Acc* a, const ReinaRay* ray) {
     a->tb = ray->tmax; a->t2 = INFINITY; a->ub = 0; a->vb = 0; a->nm_t = INFINITY;
+    a->mu = a->mv = a->mw = 0.0; a->mt = 0.0;
     a->ib = 0xFFFFFFFFu;
 }
 
-static void f64_tri(F64Acc* a, d3 o, d3 d, const ReinaRay* ray, const float* v, uint32_t g, double m) {
-    double t, u, w;
-    if (mt64(o, d, v, ray->tmin, ray->tmax, &t, &u, &w)) {
-        if (t < a->tb || (t == a->tb && g < a->ib)) {
-            if (a->ib != 0xFFFFFFFFu) a->t2 = a->tb < a->t2 ? a->tb : a->t2;
-            a->tb = t; a->ib = g; a->ub = u; a->vb = w;
-        } else if (t < a->t2) {
-            a->t2 = t;
-        }
-        return;
-    }
-    /* near miss: a hit of the triangle enlarged by the margin m? */
+static double dlen(d3 a) { return sqrt(ddot(a, a)); }
+
+static void f64_tri(F64Acc* a, d3 o, d3 d, const ReinaRay* ray, const float* v, uint32_t g, double tie_rel) {
     d3 v0 = D3(v[0], v[1], v[2]), v1 = D3(v[3], v[4], v[5]), v2 = D3(v[6], v[7], v[8]);
     d3 e1 = dsub(v1, v0), e2 = dsub(v2, v0);
     d3 p = dcross(d, e2);
@@ -387,20 +374,40 @@ static void f64_tri(F64Acc* a, d3 o, d3 d, const ReinaRay* ray, const float* v, 
     d3 q = dcross(sv, e1);
     double vv = ddot(d, q) * inv;
     double tt = ddot(e2, q) * inv;
-    if (uu >= -m && vv >= -m && uu + vv <= 1.0 + m && tt >= ray->tmin - m && tt <= ray->tmax * (1 + m))
-        if (tt < a->nm_t) a->nm_t = tt;
+    double dn = dlen(d);
+    double dist = fmax(fmax(dlen(dsub(v0, o)), dlen(dsub(v1, o))), dlen(dsub(v2, o)));
+    double delta = TIE_ULPS * F32_EPS * dist;
+    double k = delta * dn / fabs(det);
+    double mu = fmax(tie_rel, k * dlen(e2)), mv = fmax(tie_rel, k * dlen(e1));
+    double mw = fmax(tie_rel, k * dlen(dsub(e2, e1)));
+    double mt = fmax(tie_rel * fabs(tt), delta / dn);
+    int inside = !(uu < 0.0 || uu > 1.0) && !(vv < 0.0 || uu + vv > 1.0);
+    if (inside && tt >= ray->tmin && tt <= ray->tmax) {
+        if (tt < a->tb || (tt == a->tb && g < a->ib)) {
+            if (a->ib != 0xFFFFFFFFu && a->tb - a->mt < a->t2) a->t2 = a->tb - a->mt;
+            a->tb = tt; a->ib = g; a->ub = uu; a->vb = vv;
+            a->mu = mu; a->mv = mv; a->mw = mw; a->mt = mt;
+        } else if (tt - mt < a->t2) {
+            a->t2 = tt - mt;
+        }
+        return;
+    }
+    /* near miss: inside the triangle enlarged by its margins? */
+    if (uu >= -mu && vv >= -mv && 1.0 - uu - vv >= -mw && tt >= ray->tmin - mt && tt <= ray->tmax + mt)
+        if (tt - mt < a->nm_t) a->nm_t = tt - mt;
 }
 
 static void f64_finish(const F64Acc* a, const ReinaRay* ray, double tie_rel, double* to, uint32_t* po,
                        double* uo, double* vo, uint8_t* amb) {
     *to = a->tb; *po = a->ib; *uo = a->ub; *vo = a->vb;
-    int am = a->nm_t <= a->tb * (1.0 + tie_rel);
+    int am = a->nm_t <= a->tb + a->mt;
     if (a->ib != 0xFFFFFFFFu) {
         double wb = 1.0 - a->ub - a->vb;
-        if (a->ub < tie_rel || a->vb < tie_rel || wb < tie_rel) am = 1;
-        if (a->t2 <= a->tb * (1.0 + tie_rel) + 1e-300) am = 1;
-        if (a->tb <= ray->tmin * (1.0 + tie_rel) + tie_rel) am = 1;
+        if (a->ub < a->mu || a->vb < a->mv || wb < a->mw) am = 1;
+        if (a->t2 <= a->tb + a->mt) am = 1;
+        if (a->tb <= ray->tmin + a->mt + tie_rel) am = 1;
     }
+    (void)tie_rel;
     *amb = (uint8_t)am;
 }
 
@@ -414,9 +421,9 @@ static void fp64_one(const OrScene* s, const ReinaRay* ray, double* to, uint32_t
 }
 
 /* The same FP64 query with the FP32 BVH8 (256-byte nodes) as a culling
- * structure only: every child box is padded by tie_rel times its largest
- * extent (which covers the margin-enlarged triangles of its subtree) and
- * tested in double; boxes entered beyond tb * (1 + tie_rel) are skipped.
+ * structure only: every child box is padded to cover the margin-enlarged
+ * triangles of its subtree and tested in double; boxes entered beyond the
+ * current best t plus the margins are skipped.
  * Result identical to fp64_one (the accumulator is order independent). */
 static void fp64_bvh_one(const OrScene* s, const ReinaRay* ray, double* to, uint32_t* po, double* uo,
                          double* vo, uint8_t* amb, double tie_rel) {
@@ -425,6 +432,7 @@ static void fp64_bvh_one(const OrScene* s, const ReinaRay* ray, double* to, uint
     f64_init(&a, ray);
     double dir[3] = {d.x, d.y, d.z}, org[3] = {o.x, o.y, o.z};
     double inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    double dnorm = dlen(d);
     uint32_t stack[64 * 8];
     int sp = 0;
     if (s->nnodes && s->node_bytes == 256) stack[sp++] = 0;
@@ -440,8 +448,13 @@ static void fp64_bvh_one(const OrScene* s, const ReinaRay* ray, double* to, uint
             uint8_t mk = meta[k];
             if (!mk) continue;
             double lo[3] = {f[0 + k], f[16 + k], f[32 + k]}, hi[3] = {f[8 + k], f[24 + k], f[40 + k]};
+            /* pad: the barycentric tie_rel margin, plus twice the largest
+             * FP32 edge shift of a triangle in this box (see F64Acc) */
             double ext = fmax(fmax(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
-            double pad = 2.0 * tie_rel * ext + 1e-12;
+            double far = 0.0;
+            for (int ax = 0; ax < 3; ++ax) far = fmax(far, fmax(fabs(lo[ax] - org[ax]), fabs(hi[ax] - org[ax])));
+            double shift = 2.0 * TIE_ULPS * F32_EPS * 1.7320508075688772 * far;
+            double pad = 2.0 * tie_rel * ext + shift + 1e-12;
             double tn = -INFINITY, tf = INFINITY;
             for (int ax = 0; ax < 3; ++ax) {
                 double l = lo[ax] - pad, h = hi[ax] + pad;
@@ -453,8 +466,9 @@ static void fp64_bvh_one(const OrScene* s, const ReinaRay* ray, double* to, uint
                 tn = fmax(tn, fmin(t0, t1));
                 tf = fmin(tf, fmax(t0, t1));
             }
-            tn = fmax(tn - 1e-9 * fabs(tn), ray->tmin - tie_rel);
-            tf = fmin(tf + 1e-9 * fabs(tf), a.tb * (1.0 + tie_rel) + 1e-300);
+            double tpad = shift / dnorm;
+            tn = fmax(tn - 1e-9 * fabs(tn), ray->tmin - tie_rel * fabs(ray->tmin) - tpad);
+            tf = fmin(tf + 1e-9 * fabs(tf), a.tb * (1.0 + tie_rel) + tpad + 1e-300);
             if (!(tn <= tf)) continue;
             if (mk & 0x80) {
                 if (sp < 64 * 8) stack[sp++] = child_base + (mk & 7u);
