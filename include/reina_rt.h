@@ -285,6 +285,78 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
                          uint32_t width, uint32_t height, const BloomPushConsts* bloom,
                          const TonemappingPushConsts* tonemap, void* stream);
 
+/* ---- batching and serving: many render requests in one wavefront --------
+ *
+ * No counterpart call in the reference, which renders one scene per
+ * vkCmdTraceRaysKHR (src/Reina.cpp:425-470); each request here is exactly
+ * such a call (scene, RtPushConsts, W x H image, same semantics as
+ * reina_rt_trace), and a request's image is bit-identical to its
+ * reina_rt_trace image whatever else runs beside it.
+ *
+ * A server owns a worker thread, a stream, and a pool of path-state slots
+ * sized by path_budget_bytes (REINA_RT_SERVER_BYTES_PER_PIXEL per pixel in
+ * flight). Submitted requests wait in a bounded FIFO; the worker admits them
+ * (in order) as soon as enough contiguous slots and a request index are
+ * free, generates their camera rays into the shared queues, and runs bounce
+ * iterations over every admitted request at once (captured in a HIP graph of
+ * 6 iterations). A request completes when all its pixels have finished all
+ * samples; its image is then written. Scenes must stay alive until their
+ * requests complete. submit / poll / wait may be called from any thread. */
+#define REINA_RT_SERVER_BYTES_PER_PIXEL 152u
+
+typedef struct ReinaServer ReinaServer;
+
+typedef struct ReinaServerDesc {
+    uint32_t struct_size;        /* sizeof(ReinaServerDesc)                              */
+    uint32_t device;             /* HIP device ordinal (scenes must live on it)          */
+    uint64_t path_budget_bytes;  /* HBM for in-flight pixels; at least 8 * 1024 pixels   */
+    uint32_t max_pending;        /* bounded FIFO; submit beyond it -> REINA_RT_ERR_BUSY  */
+    uint32_t quantized;          /* node format of every scene served: 0 FP32, 1 8-bit   */
+    ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
+                                    (stack_depth bounds the BVH depth of served scenes) */
+} ReinaServerDesc;
+
+#define REINA_RT_REQ_QUEUED    0u
+#define REINA_RT_REQ_RUNNING   1u
+#define REINA_RT_REQ_DONE      2u
+#define REINA_RT_REQ_FAILED    3u   /* a device error stopped the server (reina_rt_last_error) */
+
+typedef struct ReinaRequestStatus {
+    uint32_t state;          /* REINA_RT_REQ_*                                   */
+    uint32_t npx;            /* W * H                                            */
+    double   submit_s;       /* seconds since server creation                    */
+    double   admit_s;        /* rays generated (0 while queued)                  */
+    double   done_s;         /* image written (0 until done)                     */
+} ReinaRequestStatus;
+
+typedef struct ReinaServerStats {
+    uint64_t submitted, completed, rejected_busy, rejected_budget;
+    uint64_t extension_rays;     /* traced by the shared kernels                   */
+    uint64_t iterations;         /* bounce iterations run                          */
+    uint64_t graph_replays;      /* captured 6-iteration graphs launched           */
+    uint32_t pending, running;   /* now                                            */
+    uint32_t slot_tiles_total, slot_tiles_used;   /* 1024-pixel tiles              */
+    double   busy_s;             /* worker time with admitted requests             */
+} ReinaServerStats;
+
+int reina_rt_server_create(const ReinaServerDesc* desc, ReinaServer** out_server);
+/* Finishes every admitted and queued request, then stops the worker. */
+int reina_rt_server_destroy(ReinaServer* server);
+/* Enqueues one render request (image: device float4[W*H]; its previous
+ * content is blended when pc->sampleBatch > 0). Returns immediately:
+ * REINA_RT_ERR_BUSY when the FIFO is full, REINA_RT_ERR_OVER_BUDGET when the
+ * request alone needs more slots than the server has, REINA_RT_ERR_UNSUPPORTED
+ * when the scene's node format or BVH depth does not fit the server. */
+int reina_rt_submit(ReinaServer* server, ReinaScene* scene, const RtPushConsts* pc, float* image_rgba32f,
+                    uint32_t width, uint32_t height, uint64_t* ticket);
+/* Status of a request; the record is released by the first call that
+ * reports REINA_RT_REQ_DONE (later calls return REINA_RT_ERR_INVALID_ARG). */
+int reina_rt_poll(ReinaServer* server, uint64_t ticket, ReinaRequestStatus* out);
+/* Blocks until the request is done (or timeout_ms passes: REINA_RT_ERR_BUSY);
+ * fills *out like reina_rt_poll and releases the record when done. */
+int reina_rt_wait(ReinaServer* server, uint64_t ticket, uint32_t timeout_ms, ReinaRequestStatus* out);
+int reina_rt_server_stats(ReinaServer* server, ReinaServerStats* out);
+
 /* Diagnostic: evaluates the library's FP32 elementary functions on n inputs
  * (device pointers) so that tests can compare them with the host definitions.
  * fn: 0 sin, 1 cos, 2 log, 3 exp, 4 sqrt, 5 pow(x, y = in2). */

@@ -23,49 +23,14 @@
 
 using namespace rt;
 
+#include "api_internal.h"
+
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+int rt::fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(REINA_RT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct ReinaScene {
-    int device = 0;
-    SceneView sv{};
-    void* arena = nullptr;
-    uint64_t arena_bytes = 0;
-    ReinaSceneInfo info{};
-    std::vector<Bvh8Node> host_nodes;   // kept for export
-    std::vector<QBvh8Node> host_qnodes; // quantized copy (REINA_RT_NODES_QUANTIZED)
-    bool quantized = false;
-    std::vector<float> host_tris;
-    // trace buffers (grown on demand)
-    uint64_t cap_px = 0;
-    void* tbuf = nullptr;
-    PathState S{};
-    Queues Q{};
-    Ctrl* ctrl = nullptr;
-    unsigned long long* stats = nullptr;
-    uint32_t* fetch = nullptr;
-    uint32_t* h_count = nullptr;        // pinned
-    Tuning tune{16, 16, 0, 0, 0};
-    uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool traced = false;
-    uint32_t options = 0;
-    std::vector<hipEvent_t> xev;        // extend event pairs (REINA_RT_TRACE_TIME_EXTEND)
-    uint32_t xev_used = 0;
-    uint32_t last_iters = 0;
-    hipStream_t last_stream = nullptr;
-};
 
 extern "C" {
 
@@ -394,19 +359,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     HIP_TRY(hipSetDevice(s->device));
     int rc = ensure_trace_buffers(s, (uint64_t)W * H);
     if (rc) return rc;
-    TraceParams P;
-    std::memcpy(P.cam.invView, pc->invView, 64);
-    std::memcpy(P.cam.invProj, pc->invProjection, 64);
-    P.cam.focusDist = pc->focusDist;
-    P.cam.defocus = pc->defocusMultiplier;
-    P.cam.W = (float)(int32_t)W;
-    P.cam.H = (float)(int32_t)H;
-    P.W = W;
-    P.H = H;
-    P.spp = pc->samplesPerPixel;
-    P.maxBounces = pc->maxBounces;
-    P.sampleBatch = pc->sampleBatch;
-    P.directClamp = pc->directClamp;
+    const TraceParams P = make_trace_params(pc, W, H);
     HIP_TRY(hipEventRecord(s->ev0, st));
     s->Q.cap = part_capacity(W * H);
     HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, st));

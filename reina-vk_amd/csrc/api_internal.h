@@ -1,0 +1,78 @@
+// api_internal.h — host-side state shared by the C ABI translation units
+// (api.cpp: scenes and single-request tracing; server.cpp: the request loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_layout.h"
+#include "wavefront.h"
+
+namespace rt {
+
+// Records msg as the calling thread's last error (reina_rt_last_error) and
+// returns code.
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(REINA_RT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Kernel parameters of one vkCmdTraceRaysKHR-equivalent render.
+inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_t H) {
+    TraceParams P;
+    std::memcpy(P.cam.invView, pc->invView, 64);
+    std::memcpy(P.cam.invProj, pc->invProjection, 64);
+    P.cam.focusDist = pc->focusDist;
+    P.cam.defocus = pc->defocusMultiplier;
+    P.cam.W = (float)(int32_t)W;
+    P.cam.H = (float)(int32_t)H;
+    P.W = W;
+    P.H = H;
+    P.spp = pc->samplesPerPixel;
+    P.maxBounces = pc->maxBounces;
+    P.sampleBatch = pc->sampleBatch;
+    P.directClamp = pc->directClamp;
+    return P;
+}
+
+}  // namespace rt
+
+// A scene resident in HBM (reina_rt_scene_create).
+struct ReinaScene {
+    int device = 0;
+    rt::SceneView sv{};
+    void* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    ReinaSceneInfo info{};
+    std::vector<rt::Bvh8Node> host_nodes;   // kept for export
+    std::vector<rt::QBvh8Node> host_qnodes; // quantized copy (REINA_RT_NODES_QUANTIZED)
+    bool quantized = false;
+    std::vector<float> host_tris;
+    // trace buffers (grown on demand)
+    uint64_t cap_px = 0;
+    void* tbuf = nullptr;
+    rt::PathState S{};
+    rt::Queues Q{};
+    rt::Ctrl* ctrl = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* fetch = nullptr;
+    uint32_t* h_count = nullptr;        // pinned
+    rt::Tuning tune{16, 16, 0, 0, 0};
+    uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool traced = false;
+    uint32_t options = 0;
+    std::vector<hipEvent_t> xev;        // extend event pairs (REINA_RT_TRACE_TIME_EXTEND)
+    uint32_t xev_used = 0;
+    uint32_t last_iters = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+using rt::fail;

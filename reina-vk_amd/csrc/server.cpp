@@ -1,0 +1,422 @@
+// server.cpp — request loop of the batching / serving mode (reina_rt_server_*,
+// reina_rt_submit / poll / wait).
+//
+// The worker thread alternates, at 6-iteration boundaries:
+//   1. admission: FIFO requests take a request index (queue words carry it in
+//      their top 8 bits) and a contiguous run of 1024-pixel slot tiles
+//      (first fit); their JobRec is copied to the device table and
+//      k_init_job appends their camera rays to the current queue;
+//   2. one replay of the captured graph: 6 x (k_trace MULTI, k_shade_multi);
+//   3. a read-back of the per-request finished-pixel counters; complete
+//      requests get k_finish_job (image write) and release their slots.
+// Iteration i uses control slot i % 3 and queue i & 1; every boundary is a
+// multiple of 6, so admission always appends to control slot 0 / queue 0.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "api_internal.h"
+
+using namespace rt;
+
+namespace {
+
+constexpr uint32_t kGraphIters = 6;
+
+struct Request {
+    ReinaScene* scene = nullptr;
+    TraceParams P{};
+    float4* img = nullptr;
+    uint32_t npx = 0;
+    uint32_t state = REINA_RT_REQ_QUEUED;
+    double submit_s = 0, admit_s = 0, done_s = 0;
+    uint32_t job = 0, tile0 = 0, ntiles = 0;
+};
+
+}  // namespace
+
+struct ReinaServer {
+    int device = 0;
+    bool quantized = false;
+    Tuning tune{};
+    uint32_t tiles = 0;          // 1024-pixel slot tiles (multiple of kParts)
+    uint32_t max_pending = 0;
+    // device memory
+    void* dbuf = nullptr;
+    PathState S{};
+    Queues Q{};
+    Ctrl* ctrl = nullptr;
+    unsigned long long* stats = nullptr;
+    uint32_t* done_d = nullptr;
+    JobRec* jobs_d = nullptr;
+    // pinned host memory
+    JobRec* jobs_h = nullptr;
+    uint32_t* done_h = nullptr;
+    unsigned long long* stats_h = nullptr;
+    hipStream_t st = nullptr;
+    hipGraphExec_t graph = nullptr;
+    // bookkeeping (mu)
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, Request> reqs;
+    std::vector<uint8_t> tile_used;
+    std::vector<uint64_t> job_ticket;   // per request index: ticket, 0 = free
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    bool failed = false;
+    std::string error;
+    ReinaServerStats counts{};
+    std::chrono::steady_clock::time_point t0;
+    std::thread worker;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static int first_fit(const std::vector<uint8_t>& used, uint32_t n) {
+    uint32_t run = 0;
+    for (uint32_t t = 0; t < used.size(); ++t) {
+        run = used[t] ? 0 : run + 1;
+        if (run == n) return (int)(t + 1 - n);
+    }
+    return -1;
+}
+
+// Records the graph of kGraphIters bounce iterations over the shared queues.
+static hipError_t capture_graph(ReinaServer* s) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    for (uint32_t i = 0; i < kGraphIters; ++i) {
+        const uint32_t slot = i % 3, in_q = i & 1u;
+        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
+        launch_trace_multi(s->quantized, s->Q.ray[in_q], s->Q.hit, tin, s->stats, s->tune, s->Q.pix[in_q],
+                           s->jobs_d, s->st);
+        launch_shade_multi(s->jobs_d, s->S, s->Q, in_q, s->ctrl, slot, s->done_d, s->tune.shade_grid, s->st);
+    }
+    e = hipStreamEndCapture(s->st, &g);
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    return e;
+}
+
+#define W_TRY(expr)                                                                  \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            fail_worker(s, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            return;                                                                  \
+        }                                                                            \
+    } while (0)
+
+// A device error ends the server: every queued and running request fails.
+static void fail_worker(ReinaServer* s, const std::string& msg) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->failed = true;
+    s->error = msg;
+    for (auto& kv : s->reqs)
+        if (kv.second.state != REINA_RT_REQ_DONE) kv.second.state = REINA_RT_REQ_FAILED;
+    s->pending.clear();
+    s->cv_done.notify_all();
+}
+
+static void worker_main(ReinaServer* s) {
+    if (hipSetDevice(s->device) != hipSuccess) {
+        fail_worker(s, "hipSetDevice failed in the server thread");
+        return;
+    }
+    std::vector<uint64_t> running;   // tickets of admitted requests
+    auto busy_from = std::chrono::steady_clock::now();
+    for (;;) {
+        std::vector<Request> admitted;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (running.empty() && s->pending.empty()) {
+                if (s->stop) break;
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
+            }
+            if (running.empty()) busy_from = std::chrono::steady_clock::now();
+            while (!s->pending.empty()) {
+                const uint64_t tk = s->pending.front();
+                Request& r = s->reqs[tk];
+                int job = -1;
+                for (uint32_t j = 0; j < kMaxJobs; ++j)
+                    if (s->job_ticket[j] == 0) { job = (int)j; break; }
+                if (job < 0) break;
+                const int t0 = first_fit(s->tile_used, r.ntiles);
+                if (t0 < 0) break;
+                for (uint32_t t = 0; t < r.ntiles; ++t) s->tile_used[t0 + t] = 1;
+                s->counts.slot_tiles_used += r.ntiles;
+                r.job = (uint32_t)job;
+                r.tile0 = (uint32_t)t0;
+                r.state = REINA_RT_REQ_RUNNING;
+                r.admit_s = s->now();
+                s->job_ticket[job] = tk;
+                running.push_back(tk);
+                admitted.push_back(r);
+                s->pending.pop_front();
+            }
+        }
+        for (const Request& r : admitted) {
+            JobRec& jr = s->jobs_h[r.job];
+            jr.sv = r.scene->sv;
+            jr.P = r.P;
+            jr.img = r.img;
+            jr.px_base = r.tile0 << 10;
+            jr.npx = r.npx;
+            W_TRY(hipMemcpyAsync(s->jobs_d + r.job, &jr, sizeof(JobRec), hipMemcpyHostToDevice, s->st));
+            W_TRY(hipMemsetAsync(s->done_d + r.job, 0, sizeof(uint32_t), s->st));
+            W_TRY(launch_init_job(s->jobs_d, r.job, r.npx, s->S, s->Q, s->ctrl, 0, 0, s->done_d, s->st));
+        }
+        if (running.empty()) continue;
+        W_TRY(hipGraphLaunch(s->graph, s->st));
+        W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+        W_TRY(hipMemcpyAsync(s->stats_h, s->stats, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+        W_TRY(hipStreamSynchronize(s->st));
+        std::vector<uint64_t> finished;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->counts.iterations += kGraphIters;
+            s->counts.graph_replays += 1;
+            s->counts.extension_rays = s->stats_h[0];
+            for (uint64_t tk : running) {
+                const Request& r = s->reqs[tk];
+                if (s->done_h[r.job] >= r.npx) finished.push_back(tk);
+            }
+        }
+        if (finished.empty()) continue;
+        for (uint64_t tk : finished) {
+            Request r;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r = s->reqs[tk];
+            }
+            W_TRY(launch_finish_job(s->jobs_d, r.job, r.npx, s->S, s->st));
+        }
+        W_TRY(hipStreamSynchronize(s->st));
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            const double t = s->now();
+            for (uint64_t tk : finished) {
+                Request& r = s->reqs[tk];
+                r.state = REINA_RT_REQ_DONE;
+                r.done_s = t;
+                for (uint32_t k = 0; k < r.ntiles; ++k) s->tile_used[r.tile0 + k] = 0;
+                s->counts.slot_tiles_used -= r.ntiles;
+                s->job_ticket[r.job] = 0;
+                s->counts.completed += 1;
+                for (size_t k = 0; k < running.size(); ++k)
+                    if (running[k] == tk) { running.erase(running.begin() + k); break; }
+            }
+            if (running.empty())
+                s->counts.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy_from).count();
+            s->cv_done.notify_all();
+        }
+    }
+}
+
+static void free_server(ReinaServer* s) {
+    if (s->graph) hipGraphExecDestroy(s->graph);
+    if (s->st) hipStreamDestroy(s->st);
+    if (s->dbuf) hipFree(s->dbuf);
+    if (s->ctrl) hipFree(s->ctrl);
+    if (s->jobs_h) hipHostFree(s->jobs_h);
+    if (s->done_h) hipHostFree(s->done_h);
+    if (s->stats_h) hipHostFree(s->stats_h);
+    delete s;
+}
+
+extern "C" {
+
+int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
+    if (!d || !out) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (d->struct_size != sizeof(ReinaServerDesc))
+        return fail(REINA_RT_ERR_ABI, "ReinaServerDesc.struct_size mismatch (expected " +
+                                          std::to_string(sizeof(ReinaServerDesc)) + ")");
+    const uint64_t per_tile = (uint64_t)REINA_RT_SERVER_BYTES_PER_PIXEL * 1024;
+    uint64_t tiles = d->path_budget_bytes / per_tile;
+    tiles -= tiles % kParts;
+    if (tiles < kParts) return fail(REINA_RT_ERR_INVALID_ARG, "path_budget_bytes below 8 x 1024 pixels");
+    if (tiles * 1024 > (1ull << kSlotBits)) tiles = (1ull << kSlotBits) / 1024;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
+    if ((int)d->device >= ndev) return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal out of range");
+    Tuning tu{d->tuning.stack_depth, d->tuning.refill_min, d->tuning.grid_blocks, d->tuning.shade_blocks,
+              d->tuning.waves_per_simd};
+    if (tu.stack_depth == 0) tu.stack_depth = 16;
+    if (tu.stack_depth != 8 && tu.stack_depth != 16 && tu.stack_depth != 24 && tu.stack_depth != 32)
+        return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");
+    if (tu.refill_min == 0) tu.refill_min = 16;
+    if (tu.refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be 1..64");
+    if (tu.waves_per_simd != 0 && tu.waves_per_simd != 5 && tu.waves_per_simd != 6 && tu.waves_per_simd != 8)
+        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    ReinaServer* s = new ReinaServer();
+    s->device = (int)d->device;
+    s->quantized = d->quantized != 0;
+    s->tiles = (uint32_t)tiles;
+    s->max_pending = d->max_pending ? d->max_pending : 1024;
+    s->t0 = std::chrono::steady_clock::now();
+    auto bail = [&](int code, const std::string& msg) {
+        free_server(s);
+        return fail(code, msg);
+    };
+    if (hipSetDevice(s->device) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipSetDevice failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "device query failed");
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    if (tu.grid == 0) tu.grid = cus * (tu.stack_depth <= 8 ? 8 : 4);
+    if (tu.shade_grid == 0) tu.shade_grid = cus * 4;
+    s->tune = tu;
+    const uint64_t slots = tiles * 1024;
+    const uint64_t bytes = slots * REINA_RT_SERVER_BYTES_PER_PIXEL;
+    if (hipMalloc(&s->dbuf, bytes) != hipSuccess)
+        return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    char* p = (char*)s->dbuf;
+    s->S.st0 = (uint4*)p; p += 16 * slots;
+    s->S.st1 = (float4*)p; p += 16 * slots;
+    s->S.st2 = (float4*)p; p += 16 * slots;
+    s->S.st3 = (float4*)p; p += 16 * slots;
+    s->Q.ray[0] = (float4*)p; p += 32 * slots;
+    s->Q.ray[1] = (float4*)p; p += 32 * slots;
+    s->Q.hit = (float4*)p; p += 16 * slots;
+    s->Q.pix[0] = (uint32_t*)p; p += 4 * slots;
+    s->Q.pix[1] = (uint32_t*)p; p += 4 * slots;
+    s->Q.cap = (uint32_t)(slots / kParts);
+    const size_t cbytes = 3 * sizeof(Ctrl) + 256 + kMaxJobs * sizeof(uint32_t) + kMaxJobs * sizeof(JobRec);
+    if (hipMalloc(&s->ctrl, cbytes) != hipSuccess) return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
+    s->done_d = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 256);
+    s->jobs_d = (JobRec*)((char*)s->done_d + kMaxJobs * sizeof(uint32_t));
+    if (hipMemset(s->ctrl, 0, cbytes) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipMemset failed");
+    if (hipHostMalloc((void**)&s->jobs_h, kMaxJobs * sizeof(JobRec), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->done_h, kMaxJobs * sizeof(uint32_t), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->stats_h, sizeof(unsigned long long), 0) != hipSuccess)
+        return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
+    if (hipStreamCreate(&s->st) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipStreamCreate failed");
+    hipError_t e = capture_graph(s);
+    if (e != hipSuccess) return bail(REINA_RT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    s->tile_used.assign(s->tiles, 0);
+    s->job_ticket.assign(kMaxJobs, 0);
+    s->counts.slot_tiles_total = s->tiles;
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return REINA_RT_OK;
+}
+
+int reina_rt_server_destroy(ReinaServer* s) {
+    if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "server is null");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    hipSetDevice(s->device);
+    hipStreamSynchronize(s->st);
+    free_server(s);
+    return REINA_RT_OK;
+}
+
+int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, float* image, uint32_t W, uint32_t H,
+                    uint64_t* ticket) {
+    if (!s || !scene || !pc || !image || !ticket) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    if (W == 0 || H == 0) return fail(REINA_RT_ERR_INVALID_ARG, "zero image size");
+    if ((uint64_t)W * H >= (1ull << 31)) return fail(REINA_RT_ERR_INVALID_ARG, "image too large");
+    if (scene->device != s->device) return fail(REINA_RT_ERR_INVALID_ARG, "scene lives on another device");
+    if (scene->quantized != s->quantized)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "scene node format differs from the server's");
+    if (scene->info.max_depth > s->tune.stack_depth + 1)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "scene BVH depth " + std::to_string(scene->info.max_depth) +
+                                                  " exceeds the server's stack depth " +
+                                                  std::to_string(s->tune.stack_depth));
+    const uint32_t npx = W * H;
+    const uint32_t ntiles = (npx + 1023u) >> 10;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->failed) return fail(REINA_RT_ERR_DEVICE, "server stopped after a device error: " + s->error);
+    if (s->stop) return fail(REINA_RT_ERR_INVALID_ARG, "server is shutting down");
+    s->counts.submitted += 1;
+    if (ntiles > s->tiles) {
+        s->counts.rejected_budget += 1;
+        return fail(REINA_RT_ERR_OVER_BUDGET, "request needs " + std::to_string(ntiles) + " slot tiles, server has " +
+                                                  std::to_string(s->tiles));
+    }
+    if (s->pending.size() >= s->max_pending) {
+        s->counts.rejected_busy += 1;
+        return fail(REINA_RT_ERR_BUSY, "request queue full");
+    }
+    const uint64_t tk = s->next_ticket++;
+    Request r;
+    r.scene = scene;
+    r.P = make_trace_params(pc, W, H);
+    r.img = (float4*)image;
+    r.npx = npx;
+    r.ntiles = ntiles;
+    r.submit_s = s->now();
+    s->reqs[tk] = r;
+    s->pending.push_back(tk);
+    *ticket = tk;
+    s->cv_work.notify_one();
+    return REINA_RT_OK;
+}
+
+static void fill_status(const Request& r, ReinaRequestStatus* o) {
+    o->state = r.state;
+    o->npx = r.npx;
+    o->submit_s = r.submit_s;
+    o->admit_s = r.admit_s;
+    o->done_s = r.done_s;
+}
+
+int reina_rt_poll(ReinaServer* s, uint64_t ticket, ReinaRequestStatus* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(REINA_RT_ERR_INVALID_ARG, "unknown or released ticket");
+    fill_status(it->second, o);
+    if (it->second.state == REINA_RT_REQ_DONE || it->second.state == REINA_RT_REQ_FAILED) s->reqs.erase(it);
+    if (o->state == REINA_RT_REQ_FAILED) return fail(REINA_RT_ERR_DEVICE, s->error);
+    return REINA_RT_OK;
+}
+
+int reina_rt_wait(ReinaServer* s, uint64_t ticket, uint32_t timeout_ms, ReinaRequestStatus* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto done = [&]() {
+        auto it = s->reqs.find(ticket);
+        return it == s->reqs.end() || it->second.state == REINA_RT_REQ_DONE || it->second.state == REINA_RT_REQ_FAILED;
+    };
+    if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+        return fail(REINA_RT_ERR_BUSY, "timed out waiting for the request");
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(REINA_RT_ERR_INVALID_ARG, "unknown or released ticket");
+    fill_status(it->second, o);
+    s->reqs.erase(it);
+    if (o->state == REINA_RT_REQ_FAILED) return fail(REINA_RT_ERR_DEVICE, s->error);
+    return REINA_RT_OK;
+}
+
+int reina_rt_server_stats(ReinaServer* s, ReinaServerStats* o) {
+    if (!s || !o) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *o = s->counts;
+    o->pending = (uint32_t)s->pending.size();
+    uint32_t running = 0;
+    for (uint64_t t : s->job_ticket) running += t != 0;
+    o->running = running;
+    return REINA_RT_OK;
+}
+
+}  // extern "C"

@@ -1,4 +1,4 @@
-// trace_f32.hip — k_trace instantiations for FP32 BVH8 nodes (256 bytes).
+// trace_f32.hip — single-scene k_trace instantiations for FP32 BVH8 nodes (256 bytes).
 #include "trace_kernel.h"
 
 namespace rt {
@@ -6,7 +6,7 @@ namespace rt {
 hipError_t launch_trace_f32(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                          unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                          hipStream_t st) {
-    return launch_trace_fmt<false>(sv, rays, hits, in, stats, counters, count, tu, st);
+    return launch_trace_fmt<false, false>(sv, rays, hits, in, stats, counters, count, tu, nullptr, nullptr, st);
 }
 
 }  // namespace rt

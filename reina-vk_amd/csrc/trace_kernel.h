@@ -27,10 +27,14 @@ __device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uin
 // they take the next rays from the wave's current chunk (kChunk rays claimed
 // with one atomic on its partition's counter, own XCD's partition first).
 // Each loop iteration advances every active lane by one node visit.
-template <int SD, int WAVES, bool COUNT, bool Q>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(SceneView sv, const float4* __restrict__ rays,
-                                               float4* __restrict__ hits, TraceInput in, unsigned long long* stats,
-                                               uint32_t* counters, uint32_t refill_min) {
+// MULTI: the rays belong to several requests (possibly different scenes):
+// pix[idx] >> 24 names the request whose scene the ray is traced in; sv is
+// unused. Otherwise every ray is traced in sv.
+template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(
+    SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits, TraceInput in,
+    unsigned long long* stats, uint32_t* counters, uint32_t refill_min, const uint32_t* __restrict__ pix,
+    const JobRec* __restrict__ jobs) {
     __shared__ uint2 stack[SD * 256];
     constexpr uint32_t kChunk = 128;
     if (stats && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -54,6 +58,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     HitRec h;
     TravState ts;
     uint32_t nv = 0, nt = 0, wvis = 0, wtst = 0;
+    const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
+    const float4* tris = sv.tris;
+    uint64_t num_nodes = sv.num_nodes;
     for (;;) {
         uint64_t idle = __ballot(!active);
         uint32_t nidle = (uint32_t)__popcll(idle);
@@ -93,6 +100,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                     uint32_t rank = (uint32_t)__popcll(idle & ltmask);
                     if (rank < take) {
                         idx = pool_next + rank;
+                        if constexpr (MULTI) {
+                            const JobRec& J = jobs[pix[idx] >> kSlotBits];
+                            nodes = Q ? static_cast<const void*>(J.sv.qnodes) : static_cast<const void*>(J.sv.nodes);
+                            tris = J.sv.tris;
+                            num_nodes = J.sv.num_nodes;
+                        }
                         tray_init(r, rays[2 * idx], rays[2 * idx + 1]);
                         h = miss_record(r);
                         ts.cur = 0;
@@ -100,7 +113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                         nv = 0;
                         nt = 0;
                         active = true;
-                        if (!r.valid || sv.num_nodes == 0) {
+                        if (!r.valid || num_nodes == 0) {
                             hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
                             if (COUNT && counters) { counters[2 * idx] = 0; counters[2 * idx + 1] = 0; }
                             active = false;
@@ -115,7 +128,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             continue;
         }
         if (active) {
-            if (!trav_step<COUNT, Q>(sv, r, h, ts, stk, 256, nv, nt)) {
+            if (!trav_step<COUNT, Q>(nodes, tris, r, h, ts, stk, 256, nv, nt)) {
                 hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
                 if (COUNT) {
                     if (counters) { counters[2 * idx] = nv; counters[2 * idx + 1] = nt; }
@@ -138,39 +151,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     }
 }
 
-template <bool Q, int SD, int WAVES>
+template <bool Q, bool MULTI, int SD, int WAVES>
 static void launch_trace_sdw(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                              unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                             hipStream_t st) {
+                             const uint32_t* pix, const JobRec* jobs, hipStream_t st) {
     dim3 g(tu.grid), b(256);
-    if (count)
-        hipLaunchKernelGGL((k_trace<SD, WAVES, true, Q>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min);
-    else
-        hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min);
-}
-
-template <bool Q, int SD>
-static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
-                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
-                            hipStream_t st) {
-    switch (tu.waves_per_simd) {
-        case 5: launch_trace_sdw<Q, SD, 5>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 6: launch_trace_sdw<Q, SD, 6>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 8: launch_trace_sdw<Q, SD, 8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        default: launch_trace_sdw<Q, SD, 1>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+    if constexpr (MULTI) {
+        hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, true>), g, b, 0, st, sv, rays, hits, in, stats, counters,
+                           tu.refill_min, pix, jobs);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_trace<SD, WAVES, true, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
+                               counters, tu.refill_min, pix, jobs);
+        else
+            hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
+                               counters, tu.refill_min, pix, jobs);
     }
 }
 
-template <bool Q>
+template <bool Q, bool MULTI, int SD>
+static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                            const uint32_t* pix, const JobRec* jobs, hipStream_t st) {
+    switch (tu.waves_per_simd) {
+        case 5: launch_trace_sdw<Q, MULTI, SD, 5>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        case 6: launch_trace_sdw<Q, MULTI, SD, 6>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        case 8: launch_trace_sdw<Q, MULTI, SD, 8>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        default: launch_trace_sdw<Q, MULTI, SD, 1>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+    }
+}
+
+template <bool Q, bool MULTI>
 static hipError_t launch_trace_fmt(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
-                        unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st) {
+                                   unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                                   const uint32_t* pix, const JobRec* jobs, hipStream_t st) {
     switch (tu.stack_depth) {
-        case 8: launch_trace_sd<Q, 8>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 16: launch_trace_sd<Q, 16>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        case 24: launch_trace_sd<Q, 24>(sv, rays, hits, in, stats, counters, count, tu, st); break;
-        default: launch_trace_sd<Q, 32>(sv, rays, hits, in, stats, counters, count, tu, st); break;
+        case 8: launch_trace_sd<Q, MULTI, 8>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        case 16: launch_trace_sd<Q, MULTI, 16>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        case 24: launch_trace_sd<Q, MULTI, 24>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+        default: launch_trace_sd<Q, MULTI, 32>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
     }
     return hipGetLastError();
 }

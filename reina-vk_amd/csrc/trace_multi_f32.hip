@@ -1,0 +1,12 @@
+// trace_multi_f32.hip — multi-request k_trace instantiations for FP32 BVH8 nodes (256 bytes).
+#include "trace_kernel.h"
+
+namespace rt {
+
+hipError_t launch_trace_multi_f32(const float4* rays, float4* hits, const TraceInput& in, unsigned long long* stats,
+                               const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st) {
+    SceneView none{};
+    return launch_trace_fmt<false, true>(none, rays, hits, in, stats, nullptr, false, tu, pix, jobs, st);
+}
+
+}  // namespace rt

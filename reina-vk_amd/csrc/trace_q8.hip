@@ -1,4 +1,4 @@
-// trace_q8.hip — k_trace instantiations for 8-bit quantized BVH8 nodes (80 bytes).
+// trace_q8.hip — single-scene k_trace instantiations for 8-bit quantized BVH8 nodes (80 bytes).
 #include "trace_kernel.h"
 
 namespace rt {
@@ -6,7 +6,7 @@ namespace rt {
 hipError_t launch_trace_q8(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                          unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                          hipStream_t st) {
-    return launch_trace_fmt<true>(sv, rays, hits, in, stats, counters, count, tu, st);
+    return launch_trace_fmt<true, false>(sv, rays, hits, in, stats, counters, count, tu, nullptr, nullptr, st);
 }
 
 }  // namespace rt

@@ -205,18 +205,20 @@ struct TravState {
 // `stk` points at this lane's first LDS stack entry; entries are `stride`
 // uint2 apart.
 // Q selects the node format: false = FP32 Bvh8Node, true = QBvh8Node.
+// `nodes` points at the scene's Bvh8Node or QBvh8Node array, `tris` at its
+// leaf-ordered triangles.
 template <bool COUNT, bool Q>
-RT_D bool trav_step(const SceneView& sv, const TRay& r, HitRec& h, TravState& ts, uint2* stk, uint32_t stride,
-                    uint32_t& nvis, uint32_t& ntst) {
+RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec& h, TravState& ts, uint2* stk,
+                    uint32_t stride, uint32_t& nvis, uint32_t& ntst) {
     uint4 hdr;
     uint32_t hitm;
     if constexpr (Q) {
-        const uint4* qn = reinterpret_cast<const uint4*>(sv.qnodes + ts.cur);
+        const uint4* qn = reinterpret_cast<const uint4*>(static_cast<const QBvh8Node*>(nodes) + ts.cur);
         const uint4 h0 = qn[0];
         hdr = qn[1];
         hitm = node_hits_q8(r, h0, qn[2], qn[3], qn[4], h.t);
     } else {
-        const Bvh8Node* node = sv.nodes + ts.cur;
+        const Bvh8Node* node = static_cast<const Bvh8Node*>(nodes) + ts.cur;
         hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
         hitm = node_hits(r, node, h.t);
     }
@@ -234,7 +236,7 @@ RT_D bool trav_step(const SceneView& sv, const TRay& r, HitRec& h, TravState& ts
         uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
         uint32_t cnt = (m >> 5) & 3u, off = m & 31u;
         for (uint32_t j = 0; j < cnt; ++j) {
-            const float4* tv = sv.tris + 3ull * (tri_base + off + j);
+            const float4* tv = tris + 3ull * (tri_base + off + j);
             float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
             uint32_t id = __float_as_uint(p0.w);
             float t, u, v;

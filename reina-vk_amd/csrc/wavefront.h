@@ -58,6 +58,21 @@ struct Queues {
     uint32_t cap;
 };
 
+// ---- several render requests in one wavefront ----------------------------
+// A request owns a contiguous run of path-state slots starting at px_base (a
+// multiple of 1024, so each 1024-slot tile stays in one queue partition).
+// Queue entries carry slot | request << kSlotBits.
+constexpr uint32_t kSlotBits = 24;
+constexpr uint32_t kMaxJobs = 256;
+
+struct JobRec {
+    SceneView sv;
+    TraceParams P;
+    float4* img;
+    uint32_t px_base;   // first path-state slot
+    uint32_t npx;       // W * H
+};
+
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
                        unsigned long long* stats, hipStream_t st);
 // Launch parameters of the traversal kernel (chosen per scene; see the autotuner).
@@ -85,6 +100,20 @@ struct TraceInput {
 // counters (optional, counting): per-ray {visits, tests}.
 hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                         unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu, hipStream_t st);
+// Closest hit of partitioned queue entries that belong to several requests
+// (pix: the queue's slot | request words).
+hipError_t launch_trace_multi(bool quantized, const float4* rays, float4* hits, const TraceInput& in,
+                              unsigned long long* stats, const Tuning& tu, const uint32_t* pix, const JobRec* jobs,
+                              hipStream_t st);
+hipError_t launch_trace_multi_f32(const float4* rays, float4* hits, const TraceInput& in, unsigned long long* stats,
+                                  const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st);
+hipError_t launch_trace_multi_q8(const float4* rays, float4* hits, const TraceInput& in, unsigned long long* stats,
+                                 const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st);
+hipError_t launch_init_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, const Queues& Q,
+                           Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, hipStream_t st);
+hipError_t launch_shade_multi(const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q, Ctrl* ctrl,
+                              uint32_t slot, uint32_t* done, uint32_t grid, hipStream_t st);
+hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, hipStream_t st);
 hipError_t launch_trace_f32(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                             hipStream_t st);

@@ -52,6 +52,12 @@ def lib():
         "reina_rt_bvh_build_host": [c_p, u64, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64),
                                     ctypes.POINTER(u32)],
         "reina_rt_bvh_quantize_host": [c_p, u64, c_p, ctypes.POINTER(u64)],
+        "reina_rt_server_create": [c_p, c_p],
+        "reina_rt_server_destroy": [c_p],
+        "reina_rt_submit": [c_p, c_p, c_p, c_p, u32, u32, ctypes.POINTER(u64)],
+        "reina_rt_poll": [c_p, u64, c_p],
+        "reina_rt_wait": [c_p, u64, u32, c_p],
+        "reina_rt_server_stats": [c_p, c_p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -74,7 +80,8 @@ def exported_symbols():
             "reina_rt_scene_info", "reina_rt_scene_export_bvh", "reina_rt_trace", "reina_rt_trace_stats",
             "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
             "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning",
-            "reina_rt_bvh_quantize_host"]
+            "reina_rt_bvh_quantize_host", "reina_rt_server_create", "reina_rt_server_destroy", "reina_rt_submit",
+            "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats"]
 
 
 class DeviceScene:
@@ -137,6 +144,64 @@ class DeviceScene:
         if self.handle:
             check(lib().reina_rt_scene_destroy(self.handle))
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Batching / serving request loop (reina_rt_server_*): render requests
+    on resident DeviceScenes share one wavefront; submit() returns a ticket
+    at once, wait()/poll() report completion with timestamps."""
+
+    def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, **tuning):
+        d = abi.ReinaServerDesc()
+        d.struct_size = ctypes.sizeof(abi.ReinaServerDesc)
+        d.device = device
+        d.path_budget_bytes = int(path_budget_bytes)
+        d.max_pending = max_pending
+        d.quantized = int(bool(quantized))
+        for k, v in tuning.items():
+            setattr(d.tuning, k, v)
+        h = ctypes.c_void_p()
+        check(lib().reina_rt_server_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+        self._scenes = {}   # keep scenes alive while their requests run
+
+    def submit(self, dev, pc, image_ptr, width, height):
+        pcb = np.ascontiguousarray(np.asarray(pc, dtype=abi.PUSH_CONSTS).reshape(()))
+        t = ctypes.c_uint64(0)
+        check(lib().reina_rt_submit(self.handle, dev.handle, pcb.ctypes.data, ctypes.c_void_p(image_ptr), width, height,
+                                    ctypes.byref(t)))
+        self._scenes[t.value] = dev
+        return t.value
+
+    def wait(self, ticket, timeout_ms=600000):
+        st = abi.ReinaRequestStatus()
+        check(lib().reina_rt_wait(self.handle, ticket, timeout_ms, ctypes.byref(st)))
+        self._scenes.pop(ticket, None)
+        return st
+
+    def poll(self, ticket):
+        st = abi.ReinaRequestStatus()
+        check(lib().reina_rt_poll(self.handle, ticket, ctypes.byref(st)))
+        if st.state in (abi.REQ_DONE, abi.REQ_FAILED):
+            self._scenes.pop(ticket, None)
+        return st
+
+    def stats(self):
+        st = abi.ReinaServerStats()
+        check(lib().reina_rt_server_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.handle:
+            check(lib().reina_rt_server_destroy(self.handle))
+            self.handle = None
+            self._scenes.clear()
 
     def __del__(self):
         try:

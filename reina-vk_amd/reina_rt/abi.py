@@ -92,6 +92,29 @@ class ReinaTuning(ctypes.Structure):
                 ("waves_per_simd", ctypes.c_uint32)]
 
 
+class ReinaServerDesc(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_uint32),
+                ("path_budget_bytes", ctypes.c_uint64), ("max_pending", ctypes.c_uint32),
+                ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning)]
+
+
+class ReinaRequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint32), ("npx", ctypes.c_uint32), ("submit_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ReinaServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected_busy", ctypes.c_uint64), ("rejected_budget", ctypes.c_uint64),
+                ("extension_rays", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("graph_replays", ctypes.c_uint64), ("pending", ctypes.c_uint32), ("running", ctypes.c_uint32),
+                ("slot_tiles_total", ctypes.c_uint32), ("slot_tiles_used", ctypes.c_uint32),
+                ("busy_s", ctypes.c_double)]
+
+
+SERVER_BYTES_PER_PIXEL = 152
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
 TRACE_TIME_EXTEND = 0x1
 TRACE_COUNT = 0x2
 

@@ -54,6 +54,31 @@ def test_header_layout_with_gcc():
             assert int(got[st + "." + f]) == off, (st, f)
 
 
+def test_ctypes_structs_match_header():
+    """Every ctypes mirror of a header struct has the C layout (gcc)."""
+    structs = [abi.ReinaSceneDesc, abi.ReinaSceneInfo, abi.ReinaTraceStats, abi.ReinaTuning, abi.ReinaServerDesc,
+               abi.ReinaRequestStatus, abi.ReinaServerStats]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
+    for st in structs:
+        n = st.__name__
+        src.append('printf("%s %%zu\\n", sizeof(%s));' % (n, n))
+        for f, _ in st._fields_:
+            src.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (n, f, n, f))
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-std=c11", "-o", exe, c], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line)
+    for st in structs:
+        n = st.__name__
+        assert int(got[n]) == ctypes.sizeof(st), n
+        for f, _ in st._fields_:
+            assert int(got[n + "." + f]) == getattr(st, f).offset, (n, f)
+
+
 def test_numpy_dtypes_match():
     for st, dt in (("ReinaInstanceProperties", abi.INSTANCE_PROPERTIES), ("RtPushConsts", abi.PUSH_CONSTS),
                    ("ReinaInstanceData", abi.INSTANCE_DATA)):

@@ -1,0 +1,176 @@
+"""Batching / serving through the C ABI (reina_rt_server_*, reina_rt_submit,
+reina_rt_wait): each request's image must be bit-identical to the same
+request rendered by reina_rt_trace, alone or mixed with other scenes,
+resolutions, sample counts and accumulation states; admission respects the
+path budget and the bounded queue."""
+import numpy as np
+import pytest
+
+import oracle
+import reina_rt
+from reina_rt import abi, scenes
+from reina_rt.scene import make_desc, push_consts
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _same(a, b):
+    return bool(((_bits(a) == _bits(b)) | (np.isnan(a) & np.isnan(b))).all())
+
+
+def plain(torch, dev, pc, W, H, init=None):
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") if init is None else \
+        torch.from_numpy(init.copy()).cuda()
+    dev.trace(pc, img.data_ptr(), W, H, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    return img.cpu().numpy()
+
+
+def served(torch, srv, jobs):
+    """jobs: list of (dev, pc, W, H, init). Submits all, then waits for all."""
+    imgs, tickets = [], []
+    for dev, pc, W, H, init in jobs:
+        img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") if init is None else \
+            torch.from_numpy(init.copy()).cuda()
+        torch.cuda.synchronize()
+        imgs.append(img)
+        tickets.append(srv.submit(dev, pc, img.data_ptr(), W, H))
+    sts = [srv.wait(t) for t in tickets]
+    for st in sts:
+        assert st.state == abi.REQ_DONE and st.done_s >= st.admit_s >= st.submit_s
+    return [i.cpu().numpy() for i in imgs]
+
+
+@pytest.fixture(scope="module")
+def pool(cuda):
+    c_scene, c_cam = scenes.cornell()
+    i_scene, i_cam = scenes.icosphere(levels=4)
+    n_scene, n_cam = scenes.instanced(copies=64, box=20.0)
+    out = {}
+    for name, (sc, cam), flags in (("cornell", (c_scene, c_cam), 0), ("ico", (i_scene, i_cam), 0),
+                                   ("inst", (n_scene, n_cam), 0)):
+        arrays = sc.build_arrays()
+        out[name] = (arrays, cam, reina_rt.DeviceScene(arrays, flags=flags))
+    yield out
+    for _, _, d in out.values():
+        d.close()
+
+
+def _pc(cam, W, H, spp, depth, batch=0):
+    return push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, depth, sample_batch=batch)
+
+
+def test_server_request_equals_trace_and_oracle(cuda, pool):
+    torch = cuda
+    arrays, cam, dev = pool["cornell"]
+    W = H = 48
+    pc = scenes.cornell_push(W, H, spp=8, max_bounces=8)
+    srv = reina_rt.Server(path_budget_bytes=64 << 20)
+    got = served(torch, srv, [(dev, pc, W, H, None)])[0]
+    ref = plain(torch, dev, pc, W, H)
+    assert _same(got, ref)
+    orc, _ = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    assert _same(got, orc)
+    st = srv.stats()
+    assert st.completed == 1 and st.extension_rays > 0 and st.iterations % 6 == 0
+    srv.close()
+
+
+def test_server_mix_equals_alone(cuda, pool):
+    """Eight requests on three scenes (different resolutions including a
+    partial last tile, spp, depths, accumulation with sampleBatch > 0)
+    submitted together: each image equals its reina_rt_trace image and its
+    image when served alone."""
+    torch = cuda
+    rng = np.random.default_rng(3)
+    jobs = []
+    for k, (name, W, H, spp, depth, batch) in enumerate([
+            ("cornell", 64, 64, 4, 8, 0), ("ico", 150, 100, 2, 8, 0), ("inst", 96, 54, 3, 12, 0),
+            ("cornell", 33, 17, 5, 3, 2), ("ico", 64, 48, 1, 8, 5), ("inst", 128, 72, 2, 6, 0),
+            ("cornell", 200, 120, 1, 8, 0), ("ico", 40, 40, 6, 8, 1)]):
+        arrays, cam, dev = pool[name]
+        pc = scenes.cornell_push(W, H, spp=spp, max_bounces=depth, sample_batch=batch) if name == "cornell" \
+            else _pc(cam, W, H, spp, depth, batch)
+        init = rng.uniform(0, 1, (H, W, 4)).astype(np.float32) if batch else None
+        jobs.append((dev, pc, W, H, init))
+    srv = reina_rt.Server(path_budget_bytes=256 << 20)
+    mixed = served(torch, srv, jobs)
+    for (dev, pc, W, H, init), got in zip(jobs, mixed):
+        assert _same(got, plain(torch, dev, pc, W, H, init))
+    for j in (1, 3):
+        alone = served(torch, srv, [jobs[j]])[0]
+        assert _same(alone, mixed[j])
+    st = srv.stats()
+    assert st.completed == len(jobs) + 2 and st.pending == 0 and st.running == 0 and st.slot_tiles_used == 0
+    srv.close()
+
+
+def test_server_admission_by_budget(cuda, pool):
+    """A path budget of 8 tiles (8192 pixels): a request needing more is
+    rejected at submit; 6 requests of 4 tiles run at most two at a time
+    and all match reina_rt_trace."""
+    torch = cuda
+    arrays, cam, dev = pool["ico"]
+    srv = reina_rt.Server(path_budget_bytes=8 * 1024 * abi.SERVER_BYTES_PER_PIXEL)
+    big = torch.zeros((100, 100, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srv.submit(dev, _pc(cam, 100, 100, 1, 8), big.data_ptr(), 100, 100)
+    assert e.value.code == abi.ERR_OVER_BUDGET
+    jobs = [(dev, _pc(cam, 64, 64, 2, 8, b), 64, 64, None) for b in range(6)]
+    got = served(torch, srv, jobs)
+    for (d, pc, W, H, init), g in zip(jobs, got):
+        assert _same(g, plain(torch, d, pc, W, H))
+    st = srv.stats()
+    assert st.rejected_budget == 1 and st.completed == 6 and st.slot_tiles_total == 8
+    srv.close()
+
+
+def test_server_bounded_queue(cuda, pool):
+    """max_pending = 2 and a budget of one request at a time: while a long
+    request runs, further submits beyond two queued ones return BUSY."""
+    torch = cuda
+    arrays, cam, dev = pool["ico"]
+    srv = reina_rt.Server(path_budget_bytes=8 * 1024 * abi.SERVER_BYTES_PER_PIXEL, max_pending=2)
+    imgs = [torch.zeros((90, 90, 4), dtype=torch.float32, device="cuda") for _ in range(6)]
+    tickets, busy = [], 0
+    for k, img in enumerate(imgs):
+        try:
+            tickets.append(srv.submit(dev, _pc(cam, 90, 90, 64 if k == 0 else 1, 8), img.data_ptr(), 90, 90))
+        except reina_rt.ReinaError as e:
+            assert e.code == abi.ERR_BUSY
+            busy += 1
+    assert busy >= 2
+    for t in tickets:
+        assert srv.wait(t).state == abi.REQ_DONE
+    assert srv.stats().rejected_busy == busy
+    srv.close()
+
+
+def test_server_edge_requests(cuda, pool):
+    """spp = 0 and maxBounces = 0 requests complete (every sample is 0, as in
+    reina_rt_trace); a quantized scene is refused by an FP32-node server."""
+    torch = cuda
+    arrays, cam, dev = pool["cornell"]
+    srv = reina_rt.Server(path_budget_bytes=32 << 20)
+    jobs = [(dev, scenes.cornell_push(32, 32, spp=0, max_bounces=8), 32, 32, None),
+            (dev, scenes.cornell_push(32, 32, spp=3, max_bounces=0), 32, 32, None),
+            (dev, scenes.cornell_push(32, 32, spp=2, max_bounces=1), 32, 32, None)]
+    got = served(torch, srv, jobs)
+    for (d, pc, W, H, init), g in zip(jobs, got):
+        assert _same(g, plain(torch, d, pc, W, H))
+    q = reina_rt.DeviceScene(arrays, flags=abi.NODES_QUANTIZED)
+    img = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srv.submit(q, scenes.cornell_push(8, 8, spp=1, max_bounces=2), img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_UNSUPPORTED
+    srv.close()
+    srvq = reina_rt.Server(path_budget_bytes=32 << 20, quantized=True)
+    pc = scenes.cornell_push(40, 40, spp=4, max_bounces=8)
+    got = served(torch, srvq, [(q, pc, 40, 40, None)])[0]
+    assert _same(got, plain(torch, dev, pc, 40, 40))
+    srvq.close()
+    q.close()
