@@ -66,6 +66,12 @@ def main():
     ap.add_argument("--instanced", action="store_true",
                     help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
     ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched block")
+    ap.add_argument("--no-served", action="store_true", help="skip the served block")
+    ap.add_argument("--batched-full", action="store_true",
+                    help="batched block at full size: 64 scenes of 1e3-2e6 triangles, 64 requests")
+    ap.add_argument("--served-full", action="store_true",
+                    help="served block at full size: 32 scenes of 1e4-4e6 triangles, 50-400 requests/s")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     a = ap.parse_args()
@@ -153,6 +159,13 @@ def main():
     if not a.no_quantized:
         log("quantized nodes")
         out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
+
+    if not a.no_batched:
+        log("batched requests")
+        out["batched"] = batched_block(a, torch)
+    if not a.no_served:
+        log("served requests")
+        out["served"] = served_block(a, torch)
 
     if a.instanced:
         log("instanced 8.4M-triangle scene")
@@ -270,6 +283,69 @@ def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
     devq.close()
     return res
 
+
+
+SERVER_BUDGET = 152 * (1 << 24)   # the largest path pool: 2^24 pixels in flight (2.55 GB)
+
+
+def batched_block(a, torch):
+    """Many requests of uneven size at once: scenes of log-uniform triangle
+    count, resolutions from {128^2, 256^2, 512^2, 1024^2}, spp from
+    {4, 16, 64, 256}, depth 8, Disney / dielectric materials; one server run
+    vs the same requests through reina_rt_trace one after another."""
+    import numpy as np
+
+    from reina_rt import loadgen
+    rng = np.random.default_rng(64)
+    n_sc, lo, hi, n_req = (64, 1e3, 2e6, 64) if a.batched_full else (16, 1e3, 2e5, 32)
+    t0 = time.perf_counter()
+    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=4)
+    setup = time.perf_counter() - t0
+    log("batched: %d scenes built in %.1f s" % (n_sc, setup))
+    reqs = loadgen.make_requests(pool, n_req, rng, [(128, 128), (256, 256), (512, 512), (1024, 1024)],
+                                 spp_choices=(4, 16, 64, 256), depth=8, orbit=False)
+    res = loadgen.batched(torch, pool, reqs, dict(path_budget_bytes=SERVER_BUDGET))
+    res.update({"scenes": n_sc, "triangles_min": min(p["tris"] for p in pool),
+                "triangles_max": max(p["tris"] for p in pool),
+                "triangles_total": int(sum(p["tris"] for p in pool)), "scene_setup_s": round(setup, 1),
+                "note": "plain = reina_rt_trace per request, one after another; batched = all requests "
+                        "submitted to one server (shared queues, graph-replayed bounce iterations)"})
+    for p in pool:
+        p["dev"].close()
+    return res
+
+
+def served_block(a, torch):
+    """Open-loop Poisson arrivals over a resident scene pool; each request a
+    random orbit camera, resolution from {256^2, 512^2, 1280x720, 1920x1080},
+    spp U[16, 128], depth 8. Plain (one worker running reina_rt_trace) vs
+    server, both with the same bounded FIFO."""
+    import numpy as np
+
+    from reina_rt import loadgen
+    rng = np.random.default_rng(5)
+    if a.served_full:
+        n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 10.0), (100.0, 10.0), (200.0, 10.0), (400.0, 10.0), (4.0, 15.0)]
+    else:
+        n_sc, lo, hi, runs = 8, 1e4, 4e5, [(50.0, 4.0), (4.0, 6.0)]
+    t0 = time.perf_counter()
+    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=5)
+    setup = time.perf_counter() - t0
+    log("served: %d scenes built in %.1f s" % (n_sc, setup))
+    out = {"scenes": n_sc, "triangles_total": int(sum(p["tris"] for p in pool)),
+           "triangles_min": min(p["tris"] for p in pool), "triangles_max": max(p["tris"] for p in pool),
+           "scene_setup_s": round(setup, 1), "runs": []}
+    for rate, dur in runs:
+        reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng,
+                                     [(256, 256), (512, 512), (1280, 720), (1920, 1080)], spp_range=(16, 128))
+        r = loadgen.served(torch, pool, reqs, rate, dur, 16, dict(path_budget_bytes=SERVER_BUDGET),
+                           seed=int(rate))
+        log("served: rate %.0f/s plain %.2f img/s served %.2f img/s" % (rate, r["plain"]["images_per_s"],
+                                                                      r["served"]["images_per_s"]))
+        out["runs"].append(r)
+    for p in pool:
+        p["dev"].close()
+    return out
 
 def instanced_block(a, torch):
     """4096 instances of a 2048-triangle sphere flattened to 8,388,608

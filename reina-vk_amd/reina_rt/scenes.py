@@ -290,3 +290,71 @@ def instanced(copies=4096, seed=11, box=100.0):
     h = 0.5 * box
     cam = dict(eye=(h, h, -0.4 * box), center=(h, h, h), fov=60.0)
     return s, cam
+
+
+# ---- scene pools for batching and serving ------------------------------------------
+_SPHERE_CACHE = {}
+
+
+def _sphere_model(segments, rings):
+    key = (segments, rings)
+    if key not in _SPHERE_CACHE:
+        v, f = uv_sphere(segments, rings)
+        with np.errstate(invalid="ignore"):
+            frames = np.nan_to_num(_vertex_frames(v, f))
+        _SPHERE_CACHE[key] = ModelData(np.concatenate([v, np.ones((len(v), 1))], 1).astype(np.float32),
+                                       f.reshape(-1).astype(np.uint32), frames.astype(np.float32),
+                                       f.reshape(-1).astype(np.uint32))
+    return _SPHERE_CACHE[key]
+
+
+def pool_scene(ntris, seed):
+    """A seeded scene of about `ntris` triangles for the batching / serving
+    pools: copies of a displaced sphere (2,048 triangles each, or one finer
+    or coarser sphere below 8,192 triangles) at random scale, rotation and
+    position in a cube sized for constant density, materials alternating
+    Disney (random roughness / metallic / colour) and dielectric (IOR 1.5,
+    Beer absorption U[0, 1]), plus a ground quad and a quad light.
+    Returns (Scene, camera dict with eye, center, fov, radius)."""
+    rng = np.random.default_rng(seed)
+    if ntris <= 8192:
+        seg = max(4, int(round(np.sqrt(ntris / 2.0))))
+        model, copies = _sphere_model(seg, seg), 1
+    else:
+        model, copies = _sphere_model(32, 32), max(1, int(round(ntris / 2048.0)))
+    side = 4.0 * copies ** (1.0 / 3.0)
+    s = Scene()
+    oid = s.define_object(model)
+    for k in range(copies):
+        M = np.eye(4)
+        M[:3, :3] = _random_rotation(rng) * rng.uniform(0.3, 1.0)
+        M[:3, 3] = rng.uniform(0.0, side, size=3) if copies > 1 else 0.5 * side
+        if k % 2 == 0:
+            mat = Material(materialIdx=3, albedo=tuple(rng.uniform(0.2, 0.95, 3)), roughness=float(rng.uniform()),
+                           metallic=float(rng.uniform()), ior=1.5, interpNormals=True,
+                           specularTint=(1.0, 1.0, 1.0))
+        else:
+            mat = Material(materialIdx=2, albedo=(0.95, 0.95, 0.95), ior=1.5,
+                           absorption=float(rng.uniform(0.0, 1.0)))
+        s.add_instance(oid, M, mat)
+    c = 0.5 * side
+    ground = _model_from_quads([_quad((c, -1.0, c), (0, 0, 2.0 * side), (2.0 * side, 0, 0))])
+    s.add_object(ground, np.eye(4), Material(materialIdx=0, albedo=(0.5, 0.5, 0.5)))
+    light = _model_from_quads([_quad((c, side + 2.0, c), (0.25 * side, 0, 0), (0, 0, 0.25 * side))])
+    s.add_object(light, np.eye(4), Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(8.0, 8.0, 8.0)))
+    cam = dict(eye=(c, c + 0.3 * side, c - 1.4 * side - 2.0), center=(c, c, c), fov=50.0,
+               radius=1.5 * side + 2.0)
+    return s, cam
+
+
+def orbit_camera(cam, rng):
+    """A random camera on the sphere of radius cam['radius'] around the
+    scene centre (upper hemisphere), looking at the centre."""
+    c = np.asarray(cam["center"], np.float64)
+    while True:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        if d[1] > 0.05 and abs(d[1]) < 0.95:
+            break
+    eye = c + cam["radius"] * d
+    return dict(eye=tuple(eye), center=tuple(c), fov=cam["fov"])
