@@ -170,6 +170,8 @@ typedef struct ReinaSceneInfo {
     uint32_t quantized;       /* 1 if 8-bit child boxes                   */
     float    build_ms;        /* BVH build wall time                      */
     float    sah_cost;        /* SAH cost of the BVH8 (ci=1, ct=1)        */
+    uint32_t num_emissive;    /* instances sampled by next-event estimation */
+    uint32_t _pad0;
 } ReinaSceneInfo;
 
 /* ---- rays and hits for the closest-hit query ---------------------------- */
@@ -212,6 +214,14 @@ int reina_rt_scene_info(const ReinaScene* scene, ReinaSceneInfo* out_info);
 int reina_rt_scene_export_bvh(const ReinaScene* scene, void* nodes, uint64_t* nodes_bytes,
                               void* tris, uint64_t* tris_bytes);
 
+/* Copies the light-sampling tables back to host memory (for the oracle):
+ * uint32 {num_emissive, num_tri_cdf, num_instances, num_triangles}, then
+ * num_emissive 32-byte records {uint32 instance, ntri, cdf_base; float cdf,
+ * pdf_area; pad}, float tri_cdf[num_tri_cdf], int32 inst_emissive[num_instances]
+ * (-1: not sampled), uint8 tri_flags[num_triangles] (bit 0: back faces
+ * culled, bit 1: mirroring transform). Pass buf = NULL to query the size. */
+int reina_rt_scene_export_emissive(const ReinaScene* scene, void* buf, uint64_t* bytes);
+
 /* Host-only BVH8 build of a world-space triangle soup (9 floats per triangle),
  * the same builder reina_rt_scene_create uses; needs no GPU. Pass NULL
  * buffers to query the sizes first. Layout of the outputs: see
@@ -250,10 +260,18 @@ typedef struct ReinaTraceStats {
 } ReinaTraceStats;
 
 /* Trace options (sticky per scene): record a HIP event pair around every
- * extend launch on the trace's stream, and/or use the counting build of the
- * extend kernel (per-ray node visits / triangle tests summed on the device). */
+ * extend launch on the trace's stream, use the counting build of the
+ * extend kernel (per-ray node visits / triangle tests summed on the device),
+ * and/or next-event estimation. */
 #define REINA_RT_TRACE_TIME_EXTEND 0x1u
 #define REINA_RT_TRACE_COUNT       0x2u
+/* Next-event estimation with multiple importance sampling at Lambertian
+ * hits (one light sample and shadow ray per bounce, power heuristic). The
+ * reference compiles this but never runs it (skipNEE = true,
+ * shaders/raytrace/raytrace.rgen.glsl:146), so it is off by default; the
+ * default renders match the reference's active path. Needs an emissive
+ * instance (REINA_RT_ERR_NO_EMISSIVE otherwise). */
+#define REINA_RT_TRACE_NEE         0x4u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 

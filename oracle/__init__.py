@@ -34,6 +34,12 @@ def lib():
     L.or_scene_create.restype = c_p
     L.or_scene_destroy.argtypes = [c_p]
     L.or_scene_destroy.restype = None
+    L.or_export_emissive.argtypes = [c_p, c_p]
+    L.or_export_emissive.restype = u64
+    L.or_scene_set_nee.argtypes = [c_p, i32]
+    L.or_scene_set_nee.restype = None
+    L.or_occluded.argtypes = [c_p, c_p]
+    L.or_occluded.restype = i32
     L.or_scene_set_bvh.argtypes = [c_p, c_p, u64, c_p, u64, u32]
     L.or_scene_set_bvh.restype = i32
     L.or_intersect_brute32.argtypes = [c_p, c_p, c_p, u64, i32]
@@ -86,6 +92,22 @@ class OracleScene:
                                     self._bvh[1].nbytes, node_bytes)
         if rc:
             raise ValueError("node_bytes must be 256 or 80")
+
+    def set_nee(self, on):
+        """Next-event estimation with MIS in render / render_sum64 (the
+        library's REINA_RT_TRACE_NEE)."""
+        lib().or_scene_set_nee(self.h, int(bool(on)))
+
+    def export_emissive(self):
+        """Light tables in the byte layout of reina_rt_scene_export_emissive."""
+        n = lib().or_export_emissive(self.h, None)
+        buf = np.zeros(n, np.uint8)
+        lib().or_export_emissive(self.h, buf.ctypes.data)
+        return buf
+
+    def occluded(self, ray):
+        r = np.ascontiguousarray(np.asarray(ray).reshape(()))
+        return bool(lib().or_occluded(self.h, r.ctypes.data))
 
     def brute32(self, rays, nthreads=8):
         hits = np.zeros(len(rays), _hit_dtype())

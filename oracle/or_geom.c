@@ -36,6 +36,93 @@ static v3 xform_point(const OrInstance* in, v3 p) {
         if (((count) != 0) && (src)) memcpy((dst), (src), (count) * sizeof(type));    \
     } while (0)
 
+/* Light-sampling tables for next-event estimation (the library's
+ * build_emissive, api.cpp): an instance is emissive when |e|^2 > 1e-10
+ * (src/scene/Instance.cpp:75-77); weight = world-space area x luminance
+ * (Instance.cpp:10-11, 15-53, with the transform applied as a column-major
+ * matrix); instance CDF by weight, triangle CDF by area; pdf_area =
+ * (weight / total) / area. Per-triangle flags: 1 = back faces culled,
+ * 2 = mirroring transform (det < 0). */
+static void build_emissive(OrScene* s) {
+    s->inst_emi = (int32_t*)malloc((s->ninst ? s->ninst : 1) * sizeof(int32_t));
+    s->tri_flags = (uint8_t*)calloc(s->ntri ? s->ntri : 1, 1);
+    s->emi = (struct OrEmissive*)calloc(s->ninst ? s->ninst : 1, sizeof(struct OrEmissive));
+    s->tri_cdf = (float*)malloc((s->ntri ? s->ntri : 1) * sizeof(float));
+    double* weight = (double*)malloc((s->ninst ? s->ninst : 1) * sizeof(double));
+    double* cum = (double*)malloc((s->ntri ? s->ntri : 1) * sizeof(double));
+    double total = 0.0;
+    s->nemi = 0;
+    s->ntri_cdf = 0;
+    for (uint32_t i = 0; i < s->ninst; ++i) {
+        const OrInstance* r = &s->inst[i];
+        const ReinaInstanceProperties* p = &s->props[r->props_id];
+        const uint32_t ntri = s->models[r->model].indexCount;
+        s->inst_emi[i] = -1;
+        const double det = (double)r->c0.x * ((double)r->c1.y * r->c2.z - (double)r->c2.y * r->c1.z) -
+                           (double)r->c1.x * ((double)r->c0.y * r->c2.z - (double)r->c2.y * r->c0.z) +
+                           (double)r->c2.x * ((double)r->c0.y * r->c1.z - (double)r->c1.y * r->c0.z);
+        const uint8_t fl = (uint8_t)((p->cullBackface ? 1 : 0) | (det < 0.0 ? 2 : 0));
+        for (uint32_t t = 0; t < ntri; ++t) s->tri_flags[r->tri_base + t] = fl;
+        const float* e = p->emission;
+        if (!((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2] > 0.00001f * 0.00001f)) continue;
+        const double lum = 0.2126 * e[0] + 0.7152 * e[1] + 0.0722 * e[2];
+        double area = 0.0;
+        for (uint32_t t = 0; t < ntri; ++t) {
+            const float* w = &s->wtri[9ull * (r->tri_base + t)];
+            double ax = (double)w[3] - w[0], ay = (double)w[4] - w[1], az = (double)w[5] - w[2];
+            double bx = (double)w[6] - w[0], by = (double)w[7] - w[1], bz = (double)w[8] - w[2];
+            double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            area += 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            cum[t] = area;
+        }
+        if (!(area > 0.0) || !(lum > 0.0)) continue;
+        struct OrEmissive* ei = &s->emi[s->nemi];
+        ei->inst = i;
+        ei->ntri = ntri;
+        ei->cdf_base = s->ntri_cdf;
+        for (uint32_t t = 0; t < ntri; ++t) s->tri_cdf[s->ntri_cdf + t] = (float)(cum[t] / area);
+        s->ntri_cdf += ntri;
+        s->tri_cdf[s->ntri_cdf - 1] = 1.0f;
+        weight[s->nemi] = area * lum;
+        total += area * lum;
+        ei->pdf_area = (float)area;
+        s->inst_emi[i] = (int32_t)s->nemi;
+        s->nemi++;
+    }
+    double c = 0.0;
+    for (uint32_t k = 0; k < s->nemi; ++k) {
+        c += weight[k];
+        s->emi[k].cdf = (float)(c / total);
+        s->emi[k].pdf_area = (float)((weight[k] / total) / (double)s->emi[k].pdf_area);
+    }
+    if (s->nemi) s->emi[s->nemi - 1].cdf = 1.0f;
+    free(weight);
+    free(cum);
+}
+
+uint64_t or_export_emissive(const OrScene* s, void* buf) {
+    const uint64_t need = 16 + 32ull * s->nemi + 4ull * s->ntri_cdf + 4ull * s->ninst + s->ntri;
+    if (!buf) return need;
+    uint8_t* p = (uint8_t*)buf;
+    memset(p, 0, need);
+    uint32_t hdr[4] = {s->nemi, s->ntri_cdf, s->ninst, (uint32_t)s->ntri};
+    memcpy(p, hdr, 16); p += 16;
+    for (uint32_t k = 0; k < s->nemi; ++k) {
+        memcpy(p, &s->emi[k].inst, 4);
+        memcpy(p + 4, &s->emi[k].ntri, 4);
+        memcpy(p + 8, &s->emi[k].cdf_base, 4);
+        memcpy(p + 12, &s->emi[k].cdf, 4);
+        memcpy(p + 16, &s->emi[k].pdf_area, 4);
+        p += 32;
+    }
+    memcpy(p, s->tri_cdf, 4ull * s->ntri_cdf); p += 4ull * s->ntri_cdf;
+    memcpy(p, s->inst_emi, 4ull * s->ninst); p += 4ull * s->ninst;
+    memcpy(p, s->tri_flags, s->ntri);
+    return need;
+}
+
+void or_scene_set_nee(OrScene* s, int on) { s->nee = on; }
+
 OrScene* or_scene_create(const ReinaSceneDesc* d) {
     OrScene* s = (OrScene*)calloc(1, sizeof(OrScene));
     s->nv = d->num_vertices;     DUP(s->vertices, d->vertices, s->nv * 4, float);
@@ -84,6 +171,7 @@ OrScene* or_scene_create(const ReinaSceneDesc* d) {
             s->tri_prim[g] = p;
         }
     }
+    build_emissive(s);
     return s;
 }
 
@@ -92,6 +180,7 @@ void or_scene_destroy(OrScene* s) {
     free(s->vertices); free(s->indices); free(s->tbns); free(s->tbn_indices);
     free(s->tex_coords); free(s->tex_indices); free(s->props); free(s->models);
     free(s->inst); free(s->wtri); free(s->tri_inst); free(s->tri_prim);
+    free(s->emi); free(s->tri_cdf); free(s->inst_emi); free(s->tri_flags);
     free(s);
 }
 
@@ -549,6 +638,30 @@ void or_intersect_fp64(const OrScene* s, const ReinaRay* rays, uint64_t n, doubl
 }
 
 /* used by or_shade.c */
+/* Shadow ray (any hit): the library's k_trace ANY mode restated as a
+ * brute-force scan; back faces (world-space winding, flipped for mirroring
+ * transforms) of culling instances are passed through. */
+int or_occluded(const OrScene* s, const ReinaRay* ray) {
+    WRay r;
+    wray_init(&r, ray);
+    if (!r.valid) return 0;
+    for (uint64_t g = 0; g < s->ntri; ++g) {
+        const float* v = &s->wtri[9 * g];
+        float t, u, w;
+        if (!wtest(&r, v, v + 3, v + 6, r.tmax, &t, &u, &w)) continue;
+        const uint8_t fl = s->tri_flags[g];
+        if (fl & 1) {
+            const float ex = v[3] - v[0], ey = v[4] - v[1], ez = v[5] - v[2];
+            const float fx = v[6] - v[0], fy = v[7] - v[1], fz = v[8] - v[2];
+            float gg = (r.d.x * (ey * fz - ez * fy) + r.d.y * (ez * fx - ex * fz)) + r.d.z * (ex * fy - ey * fx);
+            if (fl & 2) gg = -gg;
+            if (!(gg < 0.0f)) continue;
+        }
+        return 1;
+    }
+    return 0;
+}
+
 void or_closest_hit(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, int use_bvh) {
     if (use_bvh && s->nnodes) bvh_one(s, ray, hit, NULL);
     else brute32_one(s, ray, hit);

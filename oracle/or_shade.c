@@ -651,6 +651,39 @@ void or_sky(const float* dir, float* out) {
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 
+/* ================== next-event estimation (the library's) ================ */
+/* The reference never runs its NEE (skipNEE = true, raytrace.rgen.glsl:146).
+ * This restates the library's shade_ray<NEE> (shade_step.h): a light sample
+ * at Lambertian hits whose continuation can still be traced, power-heuristic
+ * MIS against the cosine pdf of the same direction, and the matching weight
+ * on emitters reached by the continuation. */
+static float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
+
+static float light_pdf(float pdf_area, float d, v3 ng, v3 w, int cull) {
+    float cl = vdot(ng, vneg(w));
+    cl = cull ? or_maxf(cl, 0.0f) : fabsf(cl);
+    return cl > 0.0f ? (pdf_area * (d * d)) / cl : 0.0f;
+}
+
+static void world_triangle(const OrScene* s, const OrInstance* in, const ReinaInstanceProperties* p, uint32_t prim,
+                           v3* w) {
+    for (int k = 0; k < 3; ++k) {
+        const float* v = &s->vertices[4ull * s->indices[3u * prim + p->indicesOffset + k]];
+        w[k] = vadd(vadd(vadd(vscale(in->c0, v[0]), vscale(in->c1, v[1])), vscale(in->c2, v[2])), in->c3);
+    }
+}
+
+static uint32_t cdf_pick_emi(const OrScene* s, float u) {
+    uint32_t lo = 0, hi = s->nemi - 1u;
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2u; if (u <= s->emi[mid].cdf) hi = mid; else lo = mid + 1u; }
+    return lo;
+}
+
+static uint32_t cdf_pick_tri(const OrScene* s, uint32_t lo, uint32_t hi, float u) {
+    while (lo < hi) { uint32_t mid = (lo + hi) / 2u; if (u <= s->tri_cdf[mid]) hi = mid; else lo = mid + 1u; }
+    return lo;
+}
+
 /* ============================= raygen ===================================== */
 static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint32_t maxBounces,
                                int use_bvh, v3* Lout) {                      /* raytrace.rgen.glsl:97-184 */
@@ -658,7 +691,13 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
     pld->insideDielectric = 0;
     v3 acc = V3(1.0f, 1.0f, 1.0f);
     v3 L = V3(0.0f, 0.0f, 0.0f);
+    const int nee = s->nee;
+    v3 mis_x = V3(0.0f, 0.0f, 0.0f);   /* origin of the last Lambertian continuation */
+    float mis_pb = 0.0f;               /* its cosine pdf, 0: no MIS                   */
     for (uint32_t seg = 0; seg < maxBounces; ++seg) {
+        int lambert_hit = 0, emissive_mis = 0;
+        v3 nee_n = V3(0.0f, 0.0f, 0.0f), nee_albedo = nee_n;
+        float emis_w = 1.0f;
         ReinaRay ray = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, 10000.0f};
         ReinaHit hit;
         or_closest_hit(s, &ray, &hit, use_bvh);
@@ -683,6 +722,23 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
                 case 2: chs_dielectric(s, pld, &c); break;
                 default: chs_disney(s, pld, &c); break;
             }
+            if (nee && !pld->skip) {
+                HitInfo hi = getObjectHitInfo(s, &c);
+                lambert_hit = c.inst->material == 0u;
+                nee_n = hi.worldNormal;
+                nee_albedo = pv3(c.props->albedo);
+                const int32_t ek = s->inst_emi[s->tri_inst[hit.prim]];
+                if (mis_pb > 0.0f && ek >= 0) {
+                    v3 w[3];
+                    world_triangle(s, c.inst, c.props, c.prim, w);
+                    v3 ng = vnormalize(vcross(vsub(w[1], w[0]), vsub(w[2], w[0])));
+                    v3 dx = vsub(hi.worldPosition, mis_x);
+                    float pl = light_pdf(s->emi[ek].pdf_area, vlength(dx), ng, vnormalize(d),
+                                         c.props->cullBackface != 0u);
+                    emis_w = power_heuristic(mis_pb, pl);
+                    emissive_mis = 1;
+                }
+            }
         }
         o = pld->rayOrigin;
         d = pld->rayDirection;
@@ -692,9 +748,50 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
             break;
         }
         if (!pld->insideDielectric) {
-            v3 contrib = vadd(V3(0.0f, 0.0f, 0.0f), pld->emission);
+            v3 contrib = vadd(V3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? vscale(pld->emission, emis_w)
+                                                                        : pld->emission);
             L = vadd(L, vmul(contrib, acc));
+            int shadow = 0;
+            v3 scontrib = V3(0.0f, 0.0f, 0.0f);
+            ReinaRay sr;
+            if (nee && lambert_hit && seg + 1u < maxBounces && s->nemi > 0u) {
+                const float u1 = or_random(&pld->rngState), u2 = or_random(&pld->rngState);
+                const float u3 = or_random(&pld->rngState), u4 = or_random(&pld->rngState);
+                const struct OrEmissive* ei = &s->emi[cdf_pick_emi(s, u1)];
+                const uint32_t t = cdf_pick_tri(s, ei->cdf_base, ei->cdf_base + ei->ntri - 1u, u2) - ei->cdf_base;
+                const OrInstance* li = &s->inst[ei->inst];
+                const ReinaInstanceProperties* lp = &s->props[li->props_id];
+                v3 w[3];
+                world_triangle(s, li, lp, t, w);
+                const float beta = 1.0f - sqrtf(u3);
+                const float gamma = (1.0f - beta) * u4;
+                const float alpha = (1.0f - beta) - gamma;
+                v3 y = vadd(vadd(vscale(w[0], alpha), vscale(w[1], beta)), vscale(w[2], gamma));
+                v3 ng = vnormalize(vcross(vsub(w[1], w[0]), vsub(w[2], w[0])));
+                v3 dv = vsub(y, pld->rayOrigin);
+                float dist = vlength(dv);
+                v3 wdir = vnormalize(dv);
+                float pl = light_pdf(ei->pdf_area, dist, ng, wdir, lp->cullBackface != 0u);
+                float cx = or_maxf(vdot(nee_n, wdir), 0.0f);
+                if (pl > 0.0f && cx > 0.0f && dist > 0.001f) {
+                    float pb = cx / K_PI;
+                    float wn = power_heuristic(pl, pb);
+                    v3 f = V3(nee_albedo.x / K_PI, nee_albedo.y / K_PI, nee_albedo.z / K_PI);
+                    scontrib = vmul(vscale(vmul(pv3(lp->emission), f), (cx / pl) * wn), acc);
+                    shadow = 1;
+                    ReinaRay r0 = {pld->rayOrigin.x, pld->rayOrigin.y, pld->rayOrigin.z, 0.0f,
+                                   wdir.x, wdir.y, wdir.z, dist - 0.001f};
+                    sr = r0;
+                }
+                mis_x = pld->rayOrigin;
+                mis_pb = or_maxf(vdot(nee_n, pld->rayDirection), 0.0f) / K_PI;
+            } else if (nee) {
+                mis_pb = 0.0f;
+            }
             acc = vmul(acc, pld->color);
+            if (shadow && !or_occluded(s, &sr)) L = vadd(L, scontrib);
+        } else if (nee) {
+            mis_pb = 0.0f;
         }
     }
     *Lout = L;

@@ -46,6 +46,13 @@ typedef struct OrScene {
     float* wtri;             /* 9 floats per triangle: v0, v1, v2              */
     uint32_t* tri_inst;      /* instance index                                */
     uint32_t* tri_prim;      /* gl_PrimitiveID (local to the model)           */
+    /* next-event estimation (restatement of the library's light tables) */
+    struct OrEmissive { uint32_t inst, ntri, cdf_base; float cdf, pdf_area; } *emi;
+    uint32_t nemi;
+    float* tri_cdf; uint32_t ntri_cdf;
+    int32_t* inst_emi;
+    uint8_t* tri_flags;
+    int nee;                 /* next-event estimation on in or_render        */
     /* optional BVH8 (exported from the library) */
     const uint8_t* nodes; uint64_t nnodes;
     uint32_t node_bytes;     /* 256: FP32 node, 80: 8-bit quantized node        */
@@ -56,6 +63,14 @@ OrScene* or_scene_create(const ReinaSceneDesc* desc);
 void or_scene_destroy(OrScene* s);
 int or_scene_set_bvh(OrScene* s, const void* nodes, uint64_t nodes_bytes,
                      const void* tris, uint64_t tris_bytes, uint32_t node_bytes);
+
+/* Light tables in the layout of reina_rt_scene_export_emissive; returns the
+ * byte size (buf may be NULL). */
+uint64_t or_export_emissive(const OrScene* s, void* buf);
+void or_scene_set_nee(OrScene* s, int on);
+/* Shadow-ray visibility: 1 if a triangle other than a back face of a
+ * culling instance is hit with t in [tmin, tmax]. */
+int or_occluded(const OrScene* s, const ReinaRay* ray);
 
 /* ---- intersection -------------------------------------------------------- */
 void or_intersect_brute32(const OrScene* s, const ReinaRay* rays, ReinaHit* hits, uint64_t n,

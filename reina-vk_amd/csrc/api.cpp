@@ -90,6 +90,60 @@ static int validate(const ReinaSceneDesc* d) {
     return REINA_RT_OK;
 }
 
+// Light-sampling tables (EmissiveInst, rt_layout.h) from the flattened
+// world-space triangles, in double precision.
+static void build_emissive(ReinaScene* s, const ReinaSceneDesc* d, const std::vector<InstanceRec>& inst,
+                           const std::vector<float>& wtri) {
+    s->emi.clear();
+    s->tri_cdf.clear();
+    s->inst_emi.assign(inst.size(), -1);
+    s->tri_flags.assign(wtri.size() / 9, 0);
+    std::vector<double> weight;
+    double total = 0.0;
+    for (uint32_t i = 0; i < inst.size(); ++i) {
+        const InstanceRec& r = inst[i];
+        const ReinaInstanceProperties& p = d->props[r.props_id];
+        const uint32_t ntri = d->models[r.model].indexCount;
+        const double det = (double)r.c0[0] * ((double)r.c1[1] * r.c2[2] - (double)r.c2[1] * r.c1[2]) -
+                           (double)r.c1[0] * ((double)r.c0[1] * r.c2[2] - (double)r.c2[1] * r.c0[2]) +
+                           (double)r.c2[0] * ((double)r.c0[1] * r.c1[2] - (double)r.c1[1] * r.c0[2]);
+        const uint8_t fl = (uint8_t)((p.cullBackface ? kTriCull : 0) | (det < 0.0 ? kTriMirror : 0));
+        for (uint32_t t = 0; t < ntri; ++t) s->tri_flags[r.tri_base + t] = fl;
+        const float* e = p.emission;
+        if (!((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2] > 0.00001f * 0.00001f)) continue;
+        const double lum = 0.2126 * e[0] + 0.7152 * e[1] + 0.0722 * e[2];
+        std::vector<double> cum(ntri);
+        double area = 0.0;
+        for (uint32_t t = 0; t < ntri; ++t) {
+            const float* w = &wtri[9ull * (r.tri_base + t)];
+            double ax = (double)w[3] - w[0], ay = (double)w[4] - w[1], az = (double)w[5] - w[2];
+            double bx = (double)w[6] - w[0], by = (double)w[7] - w[1], bz = (double)w[8] - w[2];
+            double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            cum[t] = area;
+        }
+        if (!(area > 0.0) || !(lum > 0.0)) continue;   // not sampled; its emission is found by BSDF rays only
+        EmissiveInst ei{};
+        ei.inst = i;
+        ei.ntri = ntri;
+        ei.cdf_base = (uint32_t)s->tri_cdf.size();
+        for (uint32_t t = 0; t < ntri; ++t) s->tri_cdf.push_back((float)(cum[t] / area));
+        s->tri_cdf.back() = 1.0f;
+        weight.push_back(area * lum);
+        total += area * lum;
+        ei.pdf_area = (float)area;   // area for now; pdf below
+        s->inst_emi[i] = (int32_t)s->emi.size();
+        s->emi.push_back(ei);
+    }
+    double c = 0.0;
+    for (size_t k = 0; k < s->emi.size(); ++k) {
+        c += weight[k];
+        s->emi[k].cdf = (float)(c / total);
+        s->emi[k].pdf_area = (float)((weight[k] / total) / (double)s->emi[k].pdf_area);
+    }
+    if (!s->emi.empty()) s->emi.back().cdf = 1.0f;
+}
+
 static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vector<float>& wtri,
                         const std::vector<uint32_t>& tri_inst, const std::vector<uint32_t>& tri_prim,
                         const std::vector<InstanceRec>& inst) {
@@ -115,6 +169,10 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     size_t p_tix = add(d->tbn_indices, d->num_tbn_indices * 4);
     size_t p_tc = add(d->tex_coords, d->num_tex_coords * 8);
     size_t p_tci = add(d->tex_indices, d->num_tex_indices * 4);
+    size_t p_emi = add(s->emi.data(), s->emi.size() * sizeof(EmissiveInst));
+    size_t p_cdf = add(s->tri_cdf.data(), s->tri_cdf.size() * 4);
+    size_t p_ie = add(s->inst_emi.data(), s->inst_emi.size() * 4);
+    size_t p_tf = add(s->tri_flags.data(), s->tri_flags.size());
     (void)wtri;
     HIP_TRY(hipSetDevice(s->device));
     hipError_t e = hipMalloc(&s->arena, off);
@@ -141,6 +199,12 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     s->sv.num_tex_coords = d->num_tex_coords;
     s->sv.num_nodes = s->host_nodes.size();
     s->sv.num_tris = tri_inst.size();
+    s->sv.emi = (const EmissiveInst*)(a + parts[p_emi].off);
+    s->sv.tri_cdf = (const float*)(a + parts[p_cdf].off);
+    s->sv.inst_emi = (const int32_t*)(a + parts[p_ie].off);
+    s->sv.tri_flags = (const uint8_t*)(a + parts[p_tf].off);
+    s->sv.num_emi = (uint32_t)s->emi.size();
+    s->sv.num_inst = (uint32_t)inst.size();
     return REINA_RT_OK;
 }
 
@@ -208,6 +272,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
         }
         s->quantized = true;
     }
+    build_emissive(s, d, inst, wtri);
     rc = scene_upload(s, d, wtri, tri_inst, tri_prim, inst);
     if (rc) {
         if (s->arena) hipFree(s->arena);
@@ -221,6 +286,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     s->info.device_bytes = s->arena_bytes;
     s->info.max_depth = bvh.max_depth;
     s->info.quantized = s->quantized ? 1u : 0u;
+    s->info.num_emissive = (uint32_t)s->emi.size();
     s->info.build_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     s->info.sah_cost = bvh.sah_cost;
     hipDeviceProp_t prop;
@@ -282,6 +348,25 @@ int reina_rt_scene_export_bvh(const ReinaScene* s, void* nodes, uint64_t* nodes_
     return REINA_RT_OK;
 }
 
+int reina_rt_scene_export_emissive(const ReinaScene* s, void* buf, uint64_t* bytes) {
+    if (!s || !bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    const uint32_t hdr[4] = {(uint32_t)s->emi.size(), (uint32_t)s->tri_cdf.size(), (uint32_t)s->inst_emi.size(),
+                             (uint32_t)s->tri_flags.size()};
+    const uint64_t need = 16 + s->emi.size() * sizeof(EmissiveInst) + s->tri_cdf.size() * 4 + s->inst_emi.size() * 4 +
+                          s->tri_flags.size();
+    if (buf) {
+        if (*bytes < need) return fail(REINA_RT_ERR_INVALID_ARG, "buffer too small");
+        char* p = (char*)buf;
+        std::memcpy(p, hdr, 16); p += 16;
+        std::memcpy(p, s->emi.data(), s->emi.size() * sizeof(EmissiveInst)); p += s->emi.size() * sizeof(EmissiveInst);
+        std::memcpy(p, s->tri_cdf.data(), s->tri_cdf.size() * 4); p += s->tri_cdf.size() * 4;
+        std::memcpy(p, s->inst_emi.data(), s->inst_emi.size() * 4); p += s->inst_emi.size() * 4;
+        std::memcpy(p, s->tri_flags.data(), s->tri_flags.size());
+    }
+    *bytes = need;
+    return REINA_RT_OK;
+}
+
 int reina_rt_bvh_build_host(const float* tri, uint64_t ntri, void* nodes, uint64_t* nodes_bytes, void* tris,
                             uint64_t* tris_bytes, uint32_t* max_depth) {
     if ((ntri && !tri) || !nodes_bytes || !tris_bytes) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
@@ -333,7 +418,8 @@ static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     if (npx <= s->cap_px) return REINA_RT_OK;
     free_trace_buffers(s);
     uint64_t qn = (uint64_t)kParts * part_capacity((uint32_t)npx);   // queue entries (partitioned)
-    uint64_t bytes = npx * 64 /*state*/ + qn * (64 /*2 ray queues*/ + 8 /*2 pix queues*/ + 16 /*hits*/) + 16 * 256;
+    uint64_t bytes = npx * 80 /*state*/ + qn * (64 /*2 ray queues*/ + 8 /*2 pix queues*/ + 16 /*hits*/ +
+                                                48 /*shadow rays + contributions*/) + 16 * 256;
     hipError_t e = hipMalloc(&s->tbuf, bytes);
     if (e != hipSuccess) return fail(REINA_RT_ERR_OUT_OF_MEMORY, "trace buffers: hipMalloc failed");
     char* p = (char*)s->tbuf;
@@ -342,11 +428,14 @@ static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     s->S.st1 = (float4*)take(npx * 16);
     s->S.st2 = (float4*)take(npx * 16);
     s->S.st3 = (float4*)take(npx * 16);
+    s->S.st4 = (float4*)take(npx * 16);
     s->Q.ray[0] = (float4*)take(qn * 32);
     s->Q.ray[1] = (float4*)take(qn * 32);
     s->Q.pix[0] = (uint32_t*)take(qn * 4);
     s->Q.pix[1] = (uint32_t*)take(qn * 4);
     s->Q.hit = (float4*)take(qn * 16);
+    s->Q.sray = (float4*)take(qn * 32);
+    s->Q.scon = (float4*)take(qn * 16);
     s->cap_px = npx;
     return REINA_RT_OK;
 }
@@ -359,7 +448,11 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     HIP_TRY(hipSetDevice(s->device));
     int rc = ensure_trace_buffers(s, (uint64_t)W * H);
     if (rc) return rc;
-    const TraceParams P = make_trace_params(pc, W, H);
+    TraceParams P = make_trace_params(pc, W, H);
+    const bool nee = (s->options & REINA_RT_TRACE_NEE) != 0;
+    if (nee && s->emi.empty())
+        return fail(REINA_RT_ERR_NO_EMISSIVE, "next-event estimation needs at least one emissive instance");
+    P.nee = nee ? 1u : 0u;
     HIP_TRY(hipEventRecord(s->ev0, st));
     s->Q.cap = part_capacity(W * H);
     HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, st));
@@ -384,7 +477,8 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
         TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
         HIP_TRY(launch_trace(s->sv, s->Q.ray[in_q], s->Q.hit, tin, s->stats, nullptr, count, s->tune, st));
         if (timex) HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used++ + 1], st));
-        HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->tune.shade_grid, st));
+        HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->tune.shade_grid, nee, st));
+        if (nee) HIP_TRY(launch_connect(s->sv, s->Q, s->S, s->ctrl, slot, s->stats, s->tune, st));
         ++it;
         if (it % K == 0) {
             HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count[0][0], sizeof(Ctrl::count),
@@ -409,12 +503,12 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
     if (!s->traced) return REINA_RT_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipEventSynchronize(s->ev1));
-    unsigned long long st3[3] = {0, 0, 0};
+    unsigned long long st3[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpy(st3, s->stats, sizeof st3, hipMemcpyDeviceToHost));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     o->extension_rays = st3[0];
-    o->shadow_rays = 0;
+    o->shadow_rays = st3[3];
     o->node_visits = st3[1];
     o->tri_tests = st3[2];
     o->iterations = s->last_iters;
@@ -463,7 +557,7 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
 
 int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
     if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
-    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT))
+    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown trace option bits");
     s->options = options;
     return REINA_RT_OK;

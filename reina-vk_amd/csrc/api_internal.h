@@ -39,6 +39,7 @@ inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_
     P.maxBounces = pc->maxBounces;
     P.sampleBatch = pc->sampleBatch;
     P.directClamp = pc->directClamp;
+    P.nee = 0;
     return P;
 }
 
@@ -55,6 +56,11 @@ struct ReinaScene {
     std::vector<rt::QBvh8Node> host_qnodes; // quantized copy (REINA_RT_NODES_QUANTIZED)
     bool quantized = false;
     std::vector<float> host_tris;
+    // light-sampling tables (next-event estimation)
+    std::vector<rt::EmissiveInst> emi;
+    std::vector<float> tri_cdf;
+    std::vector<int32_t> inst_emi;
+    std::vector<uint8_t> tri_flags;
     // trace buffers (grown on demand)
     uint64_t cap_px = 0;
     void* tbuf = nullptr;

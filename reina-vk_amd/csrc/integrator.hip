@@ -44,7 +44,8 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         }
         ctrl[slot].count[p][0] = c;
         ctrl[slot].fetch[p][0] = 0;
-        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; }
+        ctrl[slot].sfetch[p][0] = 0;
+        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; }
     }
     if (i >= npx) return;
     uint32_t x = i % P.W, y = i / P.W;
@@ -61,6 +62,7 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
     S.st1[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);   // accumulatedDistance starts at 0 (Q8)
     S.st2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    if (P.nee) S.st4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     uint32_t p = (i >> 10) & (kParts - 1u);
     uint32_t j = p * Q.cap + (((i >> 13) << 10) | (i & 1023u));
     Q.ray[0][2 * j] = make_float4(o.x, o.y, o.z, 0.0f);
@@ -71,6 +73,7 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
 // Shade every ray of the current queue; block b serves partition b % 8 in
 // tiles of 256 rays, and the survivors of a tile are appended to the same
 // partition of the next queue with one atomic (wave ballot + LDS scan).
+template <bool NEE>
 __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, PathState S, Queues Q, uint32_t in_q,
                                                Ctrl* ctrl, uint32_t slot) {
     __shared__ uint32_t s_wave[4];
@@ -79,6 +82,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, Path
     if (blockIdx.x == 0 && threadIdx.x < kParts) {
         ctrl[zslot].count[threadIdx.x][0] = 0;
         ctrl[zslot].fetch[threadIdx.x][0] = 0;
+        ctrl[zslot].sfetch[threadIdx.x][0] = 0;
     }
     const uint32_t p = blockIdx.x % kParts;
     const uint32_t nblk = gridDim.x / kParts;       // blocks serving this partition
@@ -98,10 +102,19 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, Path
         if (k < n) {
             const uint32_t i = base_in + k;
             pxl = pin[i];
-            const ShadeResult res = shade_ray(sv, P, S, pxl, pxl, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
+            const ShadeResult res = shade_ray<NEE>(sv, P, S, pxl, pxl, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
             push = res.push;
             no = res.no;
             nd = res.nd;
+            if (NEE) {
+                if (res.shadow) {
+                    Q.sray[2 * i] = make_float4(res.so.x, res.so.y, res.so.z, 0.0f);
+                    Q.sray[2 * i + 1] = make_float4(res.sd.x, res.sd.y, res.sd.z, res.stmax);
+                    Q.scon[i] = make_float4(res.scontrib.x, res.scontrib.y, res.scontrib.z, __uint_as_float(pxl));
+                } else {
+                    Q.sray[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);   // no shadow ray
+                }
+            }
         }
         // compaction: wave ballot, block scan over the 4 waves, one atomic per tile
         uint64_t mask = __ballot(push ? 1 : 0);
@@ -180,11 +193,20 @@ hipError_t launch_trace(const SceneView& sv, const float4* rays, float4* hits, c
 }
 
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
-                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st) {
+                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, bool nee, hipStream_t st) {
     // k_shade needs a whole number (>= 1) of blocks per partition
     grid = grid < kParts ? kParts : grid - grid % kParts;
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, st, sv, P, S, Q, in_q, ctrl, slot);
+    if (nee)
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, st, sv, P, S, Q, in_q, ctrl, slot);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, st, sv, P, S, Q, in_q, ctrl, slot);
     return hipGetLastError();
+}
+
+hipError_t launch_connect(const SceneView& sv, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                          unsigned long long* stats, const Tuning& tu, hipStream_t st) {
+    return sv.qnodes ? launch_connect_q8(sv, Q, S, ctrl, slot, stats, tu, st)
+                     : launch_connect_f32(sv, Q, S, ctrl, slot, stats, tu, st);
 }
 
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st) {

@@ -71,6 +71,28 @@ struct alignas(16) InstanceRec {
 };
 static_assert(sizeof(InstanceRec) == 64, "instance record size");
 
+// Light sampling for next-event estimation: one record per emissive instance
+// (emission with |e|^2 > 1e-10, src/scene/Instance.cpp:75-77). Instances are
+// picked by the CDF of weight = sum of world-space triangle area x luminance
+// (0.2126 r + 0.7152 g + 0.0722 b), triangles within an instance by area,
+// points uniformly on the triangle, so the area density of a point of
+// instance k is pdf_area = (weight_k / total) / area_k.
+struct alignas(16) EmissiveInst {
+    uint32_t inst;       // instance index
+    uint32_t ntri;       // triangles of the instance
+    uint32_t cdf_base;   // first entry of the instance's triangle CDF in tri_cdf
+    float cdf;           // cumulative instance weight / total weight (last = 1)
+    float pdf_area;
+    float pad[3];
+};
+static_assert(sizeof(EmissiveInst) == 32, "emissive record size");
+
+// Per flattened triangle flags for shadow rays: bit 0: its instance culls
+// back faces (closest-hit shaders pass through them, shaders/raytrace/lambertian.rchit.glsl:15-19);
+// bit 1: the instance transform mirrors (det < 0), so the front face is the
+// clockwise one in world space.
+constexpr uint8_t kTriCull = 1, kTriMirror = 2;
+
 // Device-resident scene (pointers into one HBM arena).
 struct SceneView {
     const Bvh8Node* nodes;       // FP32 nodes (null when quantized)
@@ -91,6 +113,13 @@ struct SceneView {
     uint64_t num_tex_coords;
     uint64_t num_nodes;
     uint64_t num_tris;
+    // next-event estimation
+    const EmissiveInst* emi;
+    const float* tri_cdf;
+    const int32_t* inst_emi;     // instance -> emissive record, -1 if not emissive
+    const uint8_t* tri_flags;
+    uint32_t num_emi;
+    uint32_t num_inst;
 };
 
 }  // namespace rt

@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathSta
     if (blockIdx.x == 0 && threadIdx.x < kParts) {
         ctrl[zslot].count[threadIdx.x][0] = 0;
         ctrl[zslot].fetch[threadIdx.x][0] = 0;
+        ctrl[zslot].sfetch[threadIdx.x][0] = 0;
     }
     const uint32_t p = blockIdx.x % kParts;
     const uint32_t nblk = gridDim.x / kParts;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathSta
             pw = pin[i];
             const JobRec& J = jobs[pw >> kSlotBits];
             const uint32_t sl = pw & ((1u << kSlotBits) - 1u);
-            const ShadeResult res = shade_ray(J.sv, J.P, S, sl, sl - J.px_base, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
+            const ShadeResult res = shade_ray<false>(J.sv, J.P, S, sl, sl - J.px_base, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
             push = res.push;
             fin = res.finished;
             no = res.no;

@@ -30,14 +30,24 @@ __device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uin
 // MULTI: the rays belong to several requests (possibly different scenes):
 // pix[idx] >> 24 names the request whose scene the ray is traced in; sv is
 // unused. Otherwise every ray is traced in sv.
-template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI>
+// ANY: shadow rays (next-event estimation): any hit ends a ray; a ray that
+// reaches its tmax unoccluded adds ao.scon[idx].rgb to the radiance of pixel
+// ao.scon[idx].w; rays with a zero direction are empty entries. No hit
+// records are written; stats[3] += traced shadow rays.
+struct AnyHitOut {
+    float4* st2;
+    const float4* scon;
+};
+
+template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI, bool ANY = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(
     SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits, TraceInput in,
     unsigned long long* stats, uint32_t* counters, uint32_t refill_min, const uint32_t* __restrict__ pix,
-    const JobRec* __restrict__ jobs) {
+    const JobRec* __restrict__ jobs, AnyHitOut ao) {
     __shared__ uint2 stack[SD * 256];
     constexpr uint32_t kChunk = 128;
-    if (stats && blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t nshadow = 0;
+    if (!ANY && stats && blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t tot = 0;
         for (uint32_t p = 0; p < kParts; ++p) {
             uint32_t a, b;
@@ -113,7 +123,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                         nv = 0;
                         nt = 0;
                         active = true;
-                        if (!r.valid || num_nodes == 0) {
+                        if constexpr (ANY) {
+                            if (!r.valid) {
+                                active = false;
+                            } else {
+                                nshadow++;
+                                if (num_nodes == 0) {
+                                    const float4 c = ao.scon[idx];
+                                    float4& L = ao.st2[__float_as_uint(c.w)];
+                                    L = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+                                    active = false;
+                                }
+                            }
+                        } else if (!r.valid || num_nodes == 0) {
                             hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
                             if (COUNT && counters) { counters[2 * idx] = 0; counters[2 * idx + 1] = 0; }
                             active = false;
@@ -128,7 +150,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             continue;
         }
         if (active) {
-            if (!trav_step<COUNT, Q>(nodes, tris, r, h, ts, stk, 256, nv, nt)) {
+            if (!trav_step<COUNT, Q, ANY>(nodes, tris, r, h, ts, stk, 256, nv, nt, sv.tri_flags)) {
+                if constexpr (ANY) {
+                    if (h.prim == 0xFFFFFFFFu) {   // unoccluded
+                        const float4 c = ao.scon[idx];
+                        float4& L = ao.st2[__float_as_uint(c.w)];
+                        L = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+                    }
+                    active = false;
+                    continue;
+                }
                 hits[idx] = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
                 if (COUNT) {
                     if (counters) { counters[2 * idx] = nv; counters[2 * idx + 1] = nt; }
@@ -138,6 +169,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                 active = false;
             }
         }
+    }
+    if (ANY && stats) {
+        for (int o = 32; o > 0; o >>= 1) nshadow += __shfl_xor(nshadow, o);
+        if (lane == 0 && nshadow) atomicAdd(stats + 3, (unsigned long long)nshadow);
     }
     if (COUNT && stats) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -158,14 +193,14 @@ static void launch_trace_sdw(const SceneView& sv, const float4* rays, float4* hi
     dim3 g(tu.grid), b(256);
     if constexpr (MULTI) {
         hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, true>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min, pix, jobs);
+                           tu.refill_min, pix, jobs, AnyHitOut{});
     } else {
         if (count)
             hipLaunchKernelGGL((k_trace<SD, WAVES, true, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
-                               counters, tu.refill_min, pix, jobs);
+                               counters, tu.refill_min, pix, jobs, AnyHitOut{});
         else
             hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
-                               counters, tu.refill_min, pix, jobs);
+                               counters, tu.refill_min, pix, jobs, AnyHitOut{});
     }
 }
 
@@ -190,6 +225,26 @@ static hipError_t launch_trace_fmt(const SceneView& sv, const float4* rays, floa
         case 16: launch_trace_sd<Q, MULTI, 16>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
         case 24: launch_trace_sd<Q, MULTI, 24>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
         default: launch_trace_sd<Q, MULTI, 32>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool Q, int SD>
+static void launch_connect_sd(const SceneView& sv, const Queues& Q_, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                              unsigned long long* stats, const Tuning& tu, hipStream_t st) {
+    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
+    hipLaunchKernelGGL((k_trace<SD, 1, false, Q, false, true>), dim3(tu.grid), dim3(256), 0, st, sv, Q_.sray,
+                       nullptr, tin, stats, nullptr, tu.refill_min, nullptr, nullptr, AnyHitOut{S.st2, Q_.scon});
+}
+
+template <bool Q>
+static hipError_t launch_connect_fmt(const SceneView& sv, const Queues& Q_, const PathState& S, Ctrl* ctrl,
+                                     uint32_t slot, unsigned long long* stats, const Tuning& tu, hipStream_t st) {
+    switch (tu.stack_depth) {
+        case 8: launch_connect_sd<Q, 8>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
+        case 16: launch_connect_sd<Q, 16>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
+        case 24: launch_connect_sd<Q, 24>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
+        default: launch_connect_sd<Q, 32>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
     }
     return hipGetLastError();
 }

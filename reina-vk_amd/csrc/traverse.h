@@ -207,9 +207,12 @@ struct TravState {
 // Q selects the node format: false = FP32 Bvh8Node, true = QBvh8Node.
 // `nodes` points at the scene's Bvh8Node or QBvh8Node array, `tris` at its
 // leaf-ordered triangles.
-template <bool COUNT, bool Q>
+// ANY: shadow-ray mode: the first triangle hit with t in [tmin, tmax] ends
+// the traversal, except back faces of triangles whose flags (kTriCull,
+// kTriMirror) say their instance culls them.
+template <bool COUNT, bool Q, bool ANY = false>
 RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec& h, TravState& ts, uint2* stk,
-                    uint32_t stride, uint32_t& nvis, uint32_t& ntst) {
+                    uint32_t stride, uint32_t& nvis, uint32_t& ntst, const uint8_t* tflags = nullptr) {
     uint4 hdr;
     uint32_t hitm;
     if constexpr (Q) {
@@ -241,7 +244,23 @@ RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec
             uint32_t id = __float_as_uint(p0.w);
             float t, u, v;
             if (COUNT) ntst++;
-            if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
+            if constexpr (ANY) {
+                if (wtest(r, p0, p1, p2, h.t, t, u, v)) {
+                    const uint8_t fl = tflags[id];
+                    bool pass = false;
+                    if (fl & kTriCull) {
+                        const float ex = p1.x - p0.x, ey = p1.y - p0.y, ez = p1.z - p0.z;
+                        const float fx = p2.x - p0.x, fy = p2.y - p0.y, fz = p2.z - p0.z;
+                        float g = (r.dx * (ey * fz - ez * fy) + r.dy * (ez * fx - ex * fz)) + r.dz * (ex * fy - ey * fx);
+                        if (fl & kTriMirror) g = -g;
+                        pass = !(g < 0.0f);   // back face of a culling instance
+                    }
+                    if (!pass) {
+                        h.t = t; h.prim = id; h.u = u; h.v = v;
+                        return false;
+                    }
+                }
+            } else if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
                 h.t = t; h.prim = id; h.u = u; h.v = v;
             }
         }

@@ -15,6 +15,7 @@ struct TraceParams {
     uint32_t spp, maxBounces;
     uint32_t sampleBatch;
     float directClamp;
+    uint32_t nee;   // next-event estimation on (not a reference push constant)
 };
 
 // The ray queues are split into kParts partitions, one per XCD: workgroup b
@@ -32,6 +33,7 @@ constexpr uint32_t kLine = 32;   // uint32 words per 128-byte line
 struct Ctrl {
     uint32_t count[kParts][kLine];
     uint32_t fetch[kParts][kLine];
+    uint32_t sfetch[kParts][kLine];   // claim counters of the shadow-ray kernel
 };
 
 // Pixel i starts in partition ((i >> 10) & 7) at position
@@ -41,12 +43,13 @@ __host__ __device__ inline uint32_t part_capacity(uint32_t npx) {
     return ((tiles + kParts - 1u) / kParts) << 10;
 }
 
-// Path state per pixel (SoA, 64 bytes per pixel).
+// Path state per pixel (SoA, 64 bytes per pixel, 80 with next-event estimation).
 struct PathState {
     uint4* st0;     // rng, sample index, segment index, insideDielectric
     float4* st1;    // throughput.xyz, accumulatedDistance
     float4* st2;    // radiance L.xyz, -
     float4* st3;    // sum.xyz, actualSamples (uint bits)
+    float4* st4;    // NEE: origin of the last Lambertian continuation, its cosine pdf (0: no MIS)
 };
 
 // Double-buffered ray queues (ReinaRay layout) with the owning pixel, and the
@@ -55,6 +58,8 @@ struct Queues {
     float4* ray[2];
     uint32_t* pix[2];
     float4* hit;
+    float4* sray;   // NEE: shadow ray of queue entry i (ReinaRay layout; zero direction = none)
+    float4* scon;   // NEE: its unoccluded contribution rgb, pixel (uint bits)
     uint32_t cap;
 };
 
@@ -121,7 +126,16 @@ hipError_t launch_trace_q8(const SceneView& sv, const float4* rays, float4* hits
                            unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                            hipStream_t st);
 hipError_t launch_shade(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
-                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, hipStream_t st);
+                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, uint32_t grid, bool nee, hipStream_t st);
+// Shadow rays of the iteration's queue (any hit; back faces of culling
+// instances are passed through); an unoccluded ray adds its contribution to
+// its pixel's radiance. stats[3] += shadow rays.
+hipError_t launch_connect(const SceneView& sv, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                          unsigned long long* stats, const Tuning& tu, hipStream_t st);
+hipError_t launch_connect_f32(const SceneView& sv, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                              unsigned long long* stats, const Tuning& tu, hipStream_t st);
+hipError_t launch_connect_q8(const SceneView& sv, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                             unsigned long long* stats, const Tuning& tu, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,

@@ -52,6 +52,7 @@ def lib():
         "reina_rt_bvh_build_host": [c_p, u64, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64),
                                     ctypes.POINTER(u32)],
         "reina_rt_bvh_quantize_host": [c_p, u64, c_p, ctypes.POINTER(u64)],
+        "reina_rt_scene_export_emissive": [c_p, c_p, ctypes.POINTER(u64)],
         "reina_rt_server_create": [c_p, c_p],
         "reina_rt_server_destroy": [c_p],
         "reina_rt_submit": [c_p, c_p, c_p, c_p, u32, u32, ctypes.POINTER(u64)],
@@ -81,7 +82,7 @@ def exported_symbols():
             "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
             "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning",
             "reina_rt_bvh_quantize_host", "reina_rt_server_create", "reina_rt_server_destroy", "reina_rt_submit",
-            "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats"]
+            "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats", "reina_rt_scene_export_emissive"]
 
 
 class DeviceScene:
@@ -108,6 +109,14 @@ class DeviceScene:
         check(lib().reina_rt_scene_export_bvh(self.handle, nodes.ctypes.data, ctypes.byref(nb),
                                               tris.ctypes.data, ctypes.byref(tb)))
         return nodes, tris
+
+    def export_emissive(self):
+        """Light-sampling tables as a byte blob (see reina_rt_scene_export_emissive)."""
+        nb = ctypes.c_uint64(0)
+        check(lib().reina_rt_scene_export_emissive(self.handle, None, ctypes.byref(nb)))
+        buf = np.zeros(nb.value, np.uint8)
+        check(lib().reina_rt_scene_export_emissive(self.handle, buf.ctypes.data, ctypes.byref(nb)))
+        return buf
 
     def trace(self, pc, image_ptr, width, height, stream=None):
         pcb = np.ascontiguousarray(np.asarray(pc, dtype=abi.PUSH_CONSTS).reshape(()))

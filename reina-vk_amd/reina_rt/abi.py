@@ -76,7 +76,8 @@ class ReinaSceneInfo(ctypes.Structure):
     _fields_ = [("num_triangles", ctypes.c_uint64), ("num_nodes", ctypes.c_uint64),
                 ("node_bytes", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("max_depth", ctypes.c_uint32), ("quantized", ctypes.c_uint32),
-                ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float)]
+                ("build_ms", ctypes.c_float), ("sah_cost", ctypes.c_float),
+                ("num_emissive", ctypes.c_uint32), ("_pad0", ctypes.c_uint32)]
 
 
 class ReinaTraceStats(ctypes.Structure):
@@ -117,6 +118,7 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 TRACE_TIME_EXTEND = 0x1
 TRACE_COUNT = 0x2
+TRACE_NEE = 0x4
 
 
 # status codes

@@ -358,3 +358,24 @@ def orbit_camera(cam, rng):
             break
     eye = c + cam["radius"] * d
     return dict(eye=tuple(eye), center=tuple(c), fov=cam["fov"])
+
+
+def light_box(height=0.3, albedo=0.7, emission=2.0, half=1.0):
+    """A closed box for direct-lighting checks: Lambertian floor y = 0 of
+    albedo `albedo`, the whole ceiling y = height a quad light of radiance
+    `emission` facing down, black walls; every face culls its back side.
+    Camera inside, looking down at the floor. Returns (Scene, camera)."""
+    s = Scene()
+    I = np.eye(4)
+    h, a = float(height), float(half)
+    floor = _model_from_quads([_quad((0, 0, 0), (0, 0, a), (a, 0, 0))])            # normal +y
+    ceil = _model_from_quads([_quad((0, h, 0), (a, 0, 0), (0, 0, a))])             # normal -y
+    walls = _model_from_quads([_quad((0, 0.5 * h, -a), (a, 0, 0), (0, 0.5 * h, 0)),  # normal +z
+                               _quad((0, 0.5 * h, a), (0, 0.5 * h, 0), (a, 0, 0)),   # normal -z
+                               _quad((-a, 0.5 * h, 0), (0, 0.5 * h, 0), (0, 0, a)),  # normal +x
+                               _quad((a, 0.5 * h, 0), (0, 0, a), (0, 0.5 * h, 0))])  # normal -x
+    s.add_object(floor, I, Material(materialIdx=0, albedo=(albedo,) * 3, cullBackface=True))
+    s.add_object(ceil, I, Material(materialIdx=0, albedo=(0.0,) * 3, emission=(emission,) * 3, cullBackface=True))
+    s.add_object(walls, I, Material(materialIdx=0, albedo=(0.0,) * 3, cullBackface=True))
+    cam = dict(eye=(0.0, 0.9 * h, 0.9 * a), center=(0.0, 0.0, 0.0), fov=70.0)
+    return s, cam

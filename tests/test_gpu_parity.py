@@ -380,3 +380,62 @@ def test_postprocess_bitexact(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy(), ref_out)
     assert np.array_equal(_bits(scratch[0].cpu().numpy()), _bits(ref_comb))
+
+
+# ---- next-event estimation ---------------------------------------------------------
+def _nee_scenes():
+    c, ccam = scenes.cornell()
+    b, bcam = scenes.light_box()
+    i, icam = scenes.icosphere(levels=4)
+    return [("cornell", c.build_arrays(), ccam), ("light_box", b.build_arrays(), bcam),
+            ("icosphere", i.build_arrays(), icam)]
+
+
+def test_emissive_tables_match_oracle(cuda):
+    """The library's light tables equal the oracle's restatement byte for byte."""
+    for name, arrays, cam in _nee_scenes():
+        dev = reina_rt.DeviceScene(arrays)
+        orc = oracle.OracleScene(arrays, make_desc)
+        a, b = dev.export_emissive(), orc.export_emissive()
+        assert len(a) == len(b) and np.array_equal(a, b), name
+        assert dev.info().num_emissive > 0
+        dev.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.NODES_QUANTIZED])
+def test_render_nee_bitexact(cuda, flags):
+    """NEE + MIS frames (shade writes shadow rays, the any-hit kernel adds
+    the unoccluded ones) equal the oracle's sequential restatement bit for
+    bit; shadow rays are counted."""
+    torch = cuda
+    for name, arrays, cam in _nee_scenes():
+        W, H = (40, 30) if name != "icosphere" else (64, 36)
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 6)
+        dev = reina_rt.DeviceScene(arrays, flags=flags)
+        dev.set_trace_options(abi.TRACE_NEE)
+        got = gpu_render(torch, dev, pc, W, H)
+        st = dev.stats()
+        orc = oracle.OracleScene(arrays, make_desc)
+        orc.set_nee(True)
+        ref, nrays = orc.render(pc, W, H, use_bvh=False)
+        same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+        assert same.all(), "%s: pixels differing: %d" % (name, (~same).any(-1).sum())
+        assert st.extension_rays == nrays and st.shadow_rays > 0
+        dev.set_trace_options(0)
+        plain = gpu_render(torch, dev, pc, W, H)
+        assert not np.array_equal(_bits(plain), _bits(got))
+        dev.close()
+
+
+def test_nee_needs_an_emitter(cuda):
+    torch = cuda
+    v, f = scenes.subdivided_icosphere(2)
+    from test_oracle_geometry import soup_arrays
+    dev = reina_rt.DeviceScene(soup_arrays(v[f].astype(np.float32).reshape(-1, 9)))
+    assert dev.info().num_emissive == 0
+    dev.set_trace_options(abi.TRACE_NEE)
+    img = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(reina_rt.ReinaError) as e:
+        dev.trace(push_consts((0, 0, 3), (0, 0, 0), 45.0, 8, 8, 1, 4), img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_NO_EMISSIVE
+    dev.close()
