@@ -439,3 +439,47 @@ def test_nee_needs_an_emitter(cuda):
         dev.trace(push_consts((0, 0, 3), (0, 0, 0), 45.0, 8, 8, 1, 4), img.data_ptr(), 8, 8)
     assert e.value.code == abi.ERR_NO_EMISSIVE
     dev.close()
+
+
+# ---- statistics against the converged FP64 render ----------------------------------
+@pytest.mark.parametrize("nee", [False, True])
+def test_rmse_vs_converged(cuda, nee):
+    """Sixteen independent 64-spp GPU frames of the 24x24 Cornell box
+    (sampleBatch 200000+) against the 65,536-spp FP64 converged render in
+    tests/golden (made by tests/golden/make_converged.py). With no bias the
+    mean squared error of a frame equals the per-pixel variance of a frame
+    (plus the converged render's own, 1/1024 of a frame's): the measured MSE
+    matches the one predicted from the frames' spread within 2 sigma of its
+    standard error; the relative RMSE is printed. NEE frames are compared
+    with the same BSDF-only converged render (equal expectation)."""
+    import os
+    torch = cuda
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell_24x24_converged.npz"))
+    ref, var_ref = g["mean"], g["var_batch64"] / 1024.0
+    W = H = 24
+    arrays, cam = scenes.cornell()[0].build_arrays(), None
+    dev = reina_rt.DeviceScene(arrays)
+    if nee:
+        dev.set_trace_options(abi.TRACE_NEE)
+    frames = []
+    for f in range(16):
+        pc = scenes.cornell_push(W, H, spp=64, max_bounces=8)
+        pc = np.asarray(pc).copy()
+        pc["sampleBatch"] = 200000 + f
+        img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        dev.trace(pc, img.data_ptr(), W, H, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        frames.append(img.cpu().numpy()[..., :3].astype(np.float64))
+    dev.close()
+    # sampleBatch > 0 blends with the (zero) previous image: undo the running mean
+    fr = np.array([x * (200001 + k) for k, x in enumerate(frames)])
+    v = fr.var(0, ddof=1)
+    mse = ((fr - ref) ** 2).mean(axis=(1, 2, 3))
+    pred = (v + var_ref).mean()
+    se = mse.std(ddof=1) / np.sqrt(len(mse))
+    rrmse = np.sqrt(((fr - ref) ** 2 / (ref ** 2 + 1e-3)).mean())
+    print("nee=%s mse=%.4g pred=%.4g se=%.3g rel_rmse=%.4f" % (nee, mse.mean(), pred, se, rrmse))
+    assert abs(mse.mean() - pred) < 2.0 * se + 0.1 * pred
+    if not nee:
+        # the frames' variance agrees with the converged render's own batch variance
+        assert 0.7 < v.mean() / g["var_batch64"].mean() < 1.4
