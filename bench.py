@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
+    ap.add_argument("--no-device-build", action="store_true", help="skip the device-built BVH comparison")
     ap.add_argument("--instanced", action="store_true",
                     help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
     ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
@@ -160,6 +161,10 @@ def main():
         log("quantized nodes")
         out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
 
+    if not a.no_device_build:
+        log("device-built BVH")
+        out["device_build"] = device_build_block(a, arrays, dev, frame, img, torch)
+
     if not a.no_batched:
         log("batched requests")
         out["batched"] = batched_block(a, torch)
@@ -251,6 +256,33 @@ def roofline(dev, frame, node_bytes, torch):
         "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
     }
 
+
+
+def device_build_block(a, arrays, dev, frame, img, torch):
+    """The same frame over a BVH8 built on the GPU (REINA_RT_BUILD_DEVICE):
+    build time, SAH cost and depth against the host build, frame rate, and a
+    check that the frame is bit-identical (the closest hit does not depend on
+    the tree)."""
+    import reina_rt
+    from reina_rt import abi
+    t0 = time.perf_counter()
+    dd = reina_rt.DeviceScene(arrays, flags=abi.BUILD_DEVICE)
+    setup = time.perf_counter() - t0
+    hi, di = dev.info(), dd.info()
+    t = dev.get_tuning()
+    dd.set_tuning(refill_min=t["refill_min"], grid_blocks=t["grid_blocks"], waves_per_simd=t["waves_per_simd"])
+    frame(0, dev)
+    ref = img.clone()
+    frame(0, dd)
+    same = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
+    rays, elapsed, _ = time_frames(lambda b, d=dd: frame(b, d), a.warmup, a.steps, torch)
+    res = {"scene_setup_s": round(setup, 3), "build_ms": round(float(di.build_ms), 1),
+           "host_build_ms": round(float(hi.build_ms), 1), "sah_cost": round(float(di.sah_cost), 3),
+           "host_sah_cost": round(float(hi.sah_cost), 3), "bvh_depth": int(di.max_depth),
+           "host_bvh_depth": int(hi.max_depth), "bvh_nodes": int(di.num_nodes),
+           "mrays_per_s": round(rays / elapsed / 1e6, 2), "frame_bit_identical_to_host_build": same}
+    dd.close()
+    return res
 
 def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
     """The same frame and secondary batch over 8-bit quantized BVH8 nodes
@@ -366,7 +398,7 @@ def instanced_block(a, torch):
     sp = torch.cuda.current_stream().cuda_stream
     res = {"workload": "4096 instances x 2048-tri sphere = 8,388,608 tris, %dx%d, %d spp, depth %d" % (W, H, spp, depth)}
     devs = {}
-    for name, flags in (("fp32", 0), ("quantized", abi.NODES_QUANTIZED)):
+    for name, flags in (("fp32", 0), ("quantized", abi.NODES_QUANTIZED), ("fp32_device_build", abi.BUILD_DEVICE)):
         t0 = time.perf_counter()
         d = reina_rt.DeviceScene(arrays, flags=flags)
         inf = d.info()
@@ -374,6 +406,7 @@ def instanced_block(a, torch):
         log("instanced: %s scene built in %.1f s" % (name, time.perf_counter() - t0))
         res[name] = {"scene_setup_s": round(time.perf_counter() - t0, 2), "node_bytes": int(inf.node_bytes),
                      "bvh_nodes": int(inf.num_nodes), "bvh_depth": int(inf.max_depth),
+                     "sah_cost": round(float(inf.sah_cost), 3),
                      "scene_MB": round(inf.device_bytes / 1e6, 1),
                      "node_MB": round(inf.num_nodes * inf.node_bytes / 1e6, 1)}
 
@@ -447,7 +480,7 @@ def instanced_block(a, torch):
         res[name]["ms_per_frame"] = round(elapsed / 2 * 1e3, 2)
         res[name]["rays_per_frame"] = int(rays_f // 2)
         if not a.no_roofline:
-            res[name]["roofline"] = roofline(d, frame, 208 if name == "fp32" else 80, torch)
+            res[name]["roofline"] = roofline(d, frame, 80 if name == "quantized" else 208, torch)
     frame(0, devs["fp32"])
     ref = img.clone()
     frame(0, devs["quantized"])

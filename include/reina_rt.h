@@ -139,7 +139,7 @@ typedef struct ReinaInstanceDesc {
 
 /* BVH / build options */
 #define REINA_RT_BUILD_HOST        0x0u   /* binned-SAH build on the host CPU      */
-#define REINA_RT_BUILD_DEVICE      0x1u   /* LBVH+SAH-collapse build on the GPU    */
+#define REINA_RT_BUILD_DEVICE      0x1u   /* binned-SAH build on the GPU (Morton order) */
 #define REINA_RT_NODES_QUANTIZED   0x2u   /* 8-bit child boxes (80-byte nodes)      */
 
 /* The packed buffers exactly as Models::addModel produces them

@@ -51,8 +51,6 @@ static int validate(const ReinaSceneDesc* d) {
                                           std::to_string(sizeof(ReinaSceneDesc)) + ")");
     if (d->flags & ~(REINA_RT_BUILD_DEVICE | REINA_RT_NODES_QUANTIZED))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown ReinaSceneDesc.flags bits");
-    if (d->flags & REINA_RT_BUILD_DEVICE)
-        return fail(REINA_RT_ERR_UNSUPPORTED, "the device BVH build is not available; use REINA_RT_BUILD_HOST");
     if (d->num_instances && (!d->instances || !d->models || !d->props))
         return fail(REINA_RT_ERR_INVALID_ARG, "instances given without models/props");
     for (uint32_t m = 0; m < d->num_models; ++m) {
@@ -254,7 +252,16 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     s->device = (int)d->device;
     Bvh8Result bvh;
     std::string err;
-    if (!build_bvh8_host(wtri.data(), ntri, bvh, err)) {
+    if (d->flags & REINA_RT_BUILD_DEVICE) {
+        if (hipSetDevice(s->device) != hipSuccess) {
+            delete s;
+            return fail(REINA_RT_ERR_DEVICE, "hipSetDevice failed");
+        }
+        if (!build_bvh8_device(wtri.data(), ntri, bvh, err, &s->device_build_ms)) {
+            delete s;
+            return fail(REINA_RT_ERR_DEVICE, "device BVH build: " + err);
+        }
+    } else if (!build_bvh8_host(wtri.data(), ntri, bvh, err)) {
         delete s;
         return fail(REINA_RT_ERR_INVALID_ARG, err);
     }

@@ -55,6 +55,7 @@ struct ReinaScene {
     std::vector<rt::Bvh8Node> host_nodes;   // kept for export
     std::vector<rt::QBvh8Node> host_qnodes; // quantized copy (REINA_RT_NODES_QUANTIZED)
     bool quantized = false;
+    float device_build_ms = 0.0f;       // GPU time of a REINA_RT_BUILD_DEVICE build
     std::vector<float> host_tris;
     // light-sampling tables (next-event estimation)
     std::vector<rt::EmissiveInst> emi;

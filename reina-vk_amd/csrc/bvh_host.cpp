@@ -359,7 +359,8 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
         nd.tri_base |= lmask << 24;
         out.nodes[pd.node] = nd;
     }
-    out.sah_cost = (float)sah;
+    (void)sah;
+    out.sah_cost = bvh8_sah_cost(out.nodes);
     if (out.nodes.size() >= (1ull << 24)) {
         err = "too many BVH8 nodes for the index encoding (max 2^24)";
         return false;
@@ -417,28 +418,70 @@ static bool quantize_axis(const float* lo, const float* hi, uint32_t valid, floa
     return false;
 }
 
+// One node: per axis origin and grid step, then the child codes.
+static bool quantize_node(const Bvh8Node& a, QBvh8Node& q) {
+    std::memset(&q, 0, sizeof q);
+    uint32_t valid = 0;
+    for (int k = 0; k < 8; ++k)
+        if (a.meta[k]) valid |= 1u << k;
+    uint32_t ex = 0, ey = 0, ez = 0;
+    if (!quantize_axis(a.lo_x, a.hi_x, valid, q.px, ex, q.qlo_x, q.qhi_x) ||
+        !quantize_axis(a.lo_y, a.hi_y, valid, q.py, ey, q.qlo_y, q.qhi_y) ||
+        !quantize_axis(a.lo_z, a.hi_z, valid, q.pz, ez, q.qlo_z, q.qhi_z))
+        return false;
+    q.exps = ex | ey << 8 | ez << 16;
+    q.child_base = a.child_base;
+    q.tri_base = a.tri_base;
+    std::memcpy(q.meta, a.meta, 8);
+    return true;
+}
+
 bool quantize_bvh8(const std::vector<Bvh8Node>& nodes, std::vector<QBvh8Node>& out, std::string& err) {
     out.resize(nodes.size());
-    for (size_t n = 0; n < nodes.size(); ++n) {
-        const Bvh8Node& a = nodes[n];
-        QBvh8Node& q = out[n];
-        std::memset(&q, 0, sizeof q);
-        uint32_t valid = 0;
-        for (int k = 0; k < 8; ++k)
-            if (a.meta[k]) valid |= 1u << k;
-        uint32_t ex = 0, ey = 0, ez = 0;
-        if (!quantize_axis(a.lo_x, a.hi_x, valid, q.px, ex, q.qlo_x, q.qhi_x) ||
-            !quantize_axis(a.lo_y, a.hi_y, valid, q.py, ey, q.qlo_y, q.qhi_y) ||
-            !quantize_axis(a.lo_z, a.hi_z, valid, q.pz, ez, q.qlo_z, q.qhi_z)) {
-            err = "BVH8 node " + std::to_string(n) + " has a box that 8-bit quantization cannot cover";
-            return false;
-        }
-        q.exps = ex | ey << 8 | ez << 16;
-        q.child_base = a.child_base;
-        q.tri_base = a.tri_base;
-        std::memcpy(q.meta, a.meta, 8);
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    std::atomic<int64_t> bad{-1};
+    for (unsigned t = 0; t < nth; ++t)
+        pool.emplace_back([&, t]() {
+            const size_t a = nodes.size() * t / nth, b = nodes.size() * (t + 1) / nth;
+            for (size_t n = a; n < b; ++n)
+                if (!quantize_node(nodes[n], out[n])) {
+                    int64_t expect = -1;
+                    bad.compare_exchange_strong(expect, (int64_t)n);
+                }
+        });
+    for (auto& th : pool) th.join();
+    if (bad.load() >= 0) {
+        err = "BVH8 node " + std::to_string(bad.load()) + " has a box that 8-bit quantization cannot cover";
+        return false;
     }
     return true;
+}
+
+float bvh8_sah_cost(const std::vector<Bvh8Node>& nodes) {
+    if (nodes.empty()) return 0.0f;
+    auto area = [](double lx, double ly, double lz, double hx, double hy, double hz) {
+        const double dx = hx - lx, dy = hy - ly, dz = hz - lz;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    };
+    double rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; ++k) {
+        if (!nodes[0].meta[k]) continue;
+        rl[0] = std::min(rl[0], (double)nodes[0].lo_x[k]); rh[0] = std::max(rh[0], (double)nodes[0].hi_x[k]);
+        rl[1] = std::min(rl[1], (double)nodes[0].lo_y[k]); rh[1] = std::max(rh[1], (double)nodes[0].hi_y[k]);
+        rl[2] = std::min(rl[2], (double)nodes[0].lo_z[k]); rh[2] = std::max(rh[2], (double)nodes[0].hi_z[k]);
+    }
+    const double ra = area(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]);
+    if (!(ra > 0.0)) return 0.0f;
+    double sah = 1.0;   // the root node itself
+    for (const Bvh8Node& n : nodes)
+        for (int k = 0; k < 8; ++k) {
+            const uint8_t m = n.meta[k];
+            if (!m) continue;
+            const double a = area(n.lo_x[k], n.lo_y[k], n.lo_z[k], n.hi_x[k], n.hi_y[k], n.hi_z[k]) / ra;
+            sah += (m & 0x80) ? a : a * ((m >> 5) & 3);
+        }
+    return (float)sah;
 }
 
 }  // namespace rt

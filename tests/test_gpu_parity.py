@@ -483,3 +483,64 @@ def test_rmse_vs_converged(cuda, nee):
     if not nee:
         # the frames' variance agrees with the converged render's own batch variance
         assert 0.7 < v.mean() / g["var_batch64"].mean() < 1.4
+
+
+# ---- BVH8 built on the device --------------------------------------------------------
+def _device_build_cases():
+    from test_oracle_geometry import soup_arrays
+    rng = np.random.default_rng(51)
+    cases = []
+    for n in (1, 2, 3, 4, 9, 100):
+        cases.append(("soup%d" % n, soup_arrays(rng.uniform(-1, 1, (n, 9)).astype(np.float32))))
+    same = np.tile(rng.uniform(-1, 1, (1, 9)).astype(np.float32), (50, 1))   # 50 identical triangles
+    cases.append(("identical", soup_arrays(same)))
+    c = rng.uniform(-1, 1, (20000, 1, 3))
+    cases.append(("soup20k", soup_arrays((c + 0.05 * rng.normal(size=(20000, 3, 3))).astype(np.float32).reshape(-1, 9))))
+    cases.append(("cornell", scenes.cornell()[0].build_arrays()))
+    cases.append(("instanced", scenes.instanced(copies=128, box=20.0)[0].build_arrays()))
+    return cases
+
+
+def test_device_bvh_build(cuda):
+    """REINA_RT_BUILD_DEVICE: the BVH8 built on the GPU satisfies the node
+    invariants, its traversal equals brute force (oracle) and the GPU
+    traversal equals the oracle's restatement, hits and counters."""
+    from test_oracle_geometry import check_bvh8_structure
+    torch = cuda
+    for name, arrays in _device_build_cases():
+        dev = reina_rt.DeviceScene(arrays, flags=abi.BUILD_DEVICE)
+        nodes, tris = dev.export_bvh()
+        ntri = int(dev.info().num_triangles)
+        check_bvh8_structure(nodes, tris, ntri)
+        orc = oracle.OracleScene(arrays, make_desc)
+        orc.set_bvh(nodes, tris)
+        rays = random_rays(4000, -1.5, 1.5, 52) if name != "instanced" else random_rays(4000, -2.0, 22.0, 52)
+        hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+        ref = orc.brute32(rays)
+        ref_v, ref_c = orc.bvh(rays)
+        assert np.array_equal(hits["prim"], ref["prim"]), name
+        for f in ("t", "u", "v"):
+            assert np.array_equal(_bits(hits[f]), _bits(ref[f])), (name, f)
+        assert np.array_equal(cnt, ref_c), name
+        dev.close()
+
+
+@pytest.mark.parametrize("flags", [abi.BUILD_DEVICE, abi.BUILD_DEVICE | abi.NODES_QUANTIZED])
+def test_device_bvh_render_equals_host(cuda, flags, icosphere_arrays):
+    """A frame over the device-built BVH is bit-identical to the host-built
+    one (the closest hit does not depend on the tree); SAH cost within 1.5x."""
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    W, H = 96, 54
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 2, 8)
+    dh = reina_rt.DeviceScene(arrays)
+    dd = reina_rt.DeviceScene(arrays, flags=flags)
+    a = gpu_render(torch, dh, pc, W, H)
+    b = gpu_render(torch, dd, pc, W, H)
+    same = (_bits(a) == _bits(b)) | (np.isnan(a) & np.isnan(b))
+    assert same.all()
+    print("sah host %.2f device %.2f depth %d/%d" % (dh.info().sah_cost, dd.info().sah_cost, dh.info().max_depth,
+                                                     dd.info().max_depth))
+    assert dd.info().sah_cost < 1.5 * dh.info().sah_cost
+    dh.close()
+    dd.close()

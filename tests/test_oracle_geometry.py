@@ -80,6 +80,11 @@ def test_bvh_structure():
     rng = np.random.default_rng(3)
     tris = rng.uniform(-5, 5, (20000, 9)).astype(np.float32) * 0.1 + rng.uniform(-5, 5, (20000, 1)).astype(np.float32)
     nodes, tri_out, depth = reina_rt.bvh_build_host(tris)
+    check_bvh8_structure(nodes, tri_out, len(tris))
+
+
+def check_bvh8_structure(nodes, tri_out, ntri):
+    """Invariants of a BVH8 in the library's FP32 node format."""
     N = len(nodes) // 256
     nb = nodes.reshape(N, 256)
     f = nb[:, :192].copy().view(np.float32).reshape(N, 6, 8)   # lo_x hi_x lo_y hi_y lo_z hi_z
@@ -87,7 +92,7 @@ def test_bvh_structure():
     meta = nb[:, 200:208]
     t = tri_out.reshape(-1, 12)
     ids = t[:, 3].copy().view(np.uint32)
-    assert sorted(ids.tolist()) == list(range(len(tris)))
+    assert sorted(ids.tolist()) == list(range(ntri))
     seen_nodes = {0}
     for n in range(N):
         cb, tb = hdr[n] & 0xFFFFFF
