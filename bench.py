@@ -9,7 +9,10 @@ progressively (sampleBatch = frame index), as Reina's render loop does.
 value = every traced extension ray (primary + secondary) / wall time of the
 K timed frames, scene already resident in HBM.
 
-roofline: the extend kernel (BVH8 traversal). Algorithmic bytes per ray =
+roofline: the extend kernel (BVH8 traversal). traffic: HBM bytes per launch of
+that kernel on this workload from the committed rocprofv3 PMC summary
+(profiles/k_trace_c2_1080p_64spp.md: FETCH_SIZE x 2 + WRITE_SIZE, scaled by a
+256 MiB copy in the same runs), or null for other sizes. Algorithmic bytes per ray =
 32 (ray in) + 16 (hit out) + 208 per visited node (useful bytes of the
 256-byte FP32 node: 8 x 24 B boxes + 8 B child bases + 8 B slot metadata)
 + 36 per tested triangle; visits come from the counting build of the same
@@ -152,6 +155,8 @@ def main():
         out["config"]["tuning"] = dev.get_tuning()
     if not a.no_roofline:
         out["roofline"] = roofline(dev, frame, NODE_BYTES, torch)
+        if (W, H, a.spp, a.depth) == (1920, 1080, 64, 8):
+            out["roofline"]["traffic"] = profiled_traffic("k_trace_c2_1080p_64spp.md")
 
     if not a.no_secondary:
         out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch, NODE_BYTES)
@@ -189,6 +194,18 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+
+
+def profiled_traffic(name):
+    """traffic_bytes_per_launch from a committed rocprofv3 summary, or None."""
+    p = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(p):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 def run_autotune(dev, sec, sh, torch):
     from reina_rt.autotune import tune_traversal
