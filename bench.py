@@ -9,7 +9,10 @@ progressively (sampleBatch = frame index), as Reina's render loop does.
 value = every traced extension ray (primary + secondary) / wall time of the
 K timed frames, scene already resident in HBM.
 
-roofline: the extend kernel (BVH8 traversal). traffic: HBM bytes per launch of
+roofline: the extend kernel (BVH8 traversal) over the frame's bounce
+iterations; the last few live paths (ReinaTuning.tail_paths) finish in one
+tail launch that is timed separately (tail_ms) and not part of the roofline.
+traffic: HBM bytes per launch of
 that kernel on this workload from the committed rocprofv3 PMC summary
 (profiles/k_trace_c2_1080p_64spp.md: FETCH_SIZE x 2 + WRITE_SIZE, scaled by a
 256 MiB copy in the same runs), or null for other sizes. Algorithmic bytes per ray =
@@ -248,8 +251,8 @@ def roofline(dev, frame, node_bytes, torch):
     stt = frame(0, dev)
     dev.set_trace_options(0)
     torch.cuda.synchronize()
-    assert stc.extension_rays == stt.extension_rays
-    nrays = stt.extension_rays
+    assert stc.extension_rays == stt.extension_rays and stc.tail_rays == stt.tail_rays
+    nrays = stt.extension_rays - stt.tail_rays   # rays traced by k_trace launches (the tail kernel is not counted)
     alg_bytes = nrays * RAY_BYTES + stc.node_visits * node_bytes + stc.tri_tests * TRI_BYTES
     launches = max(stt.extend_launches, 1)
     avg_launch_ms = stt.extend_ms / launches
@@ -271,6 +274,8 @@ def roofline(dev, frame, node_bytes, torch):
         "nodes_per_ray": round(stc.node_visits / max(nrays, 1), 3),
         "tris_per_ray": round(stc.tri_tests / max(nrays, 1), 3),
         "extend_grays_per_s": round(nrays / (stt.extend_ms * 1e-3) / 1e9, 3),
+        "tail_rays": int(stt.tail_rays),
+        "tail_ms": round(float(stt.tail_ms), 3),
     }
 
 

@@ -257,6 +257,9 @@ typedef struct ReinaTraceStats {
     uint32_t extend_launches; /* extend kernel launches                           */
     float    trace_ms;        /* GPU time of the whole trace (events)             */
     float    extend_ms;       /* summed extend-kernel time (REINA_RT_TRACE_TIME_EXTEND) */
+    uint64_t tail_rays;       /* extension rays traced by the tail kernel (in extension_rays) */
+    float    tail_ms;         /* GPU time of the tail kernel (0 if not used)      */
+    uint32_t tail_launches;   /* 1 if the frame ended in the tail kernel          */
 } ReinaTraceStats;
 
 /* Trace options (sticky per scene): record a HIP event pair around every
@@ -284,6 +287,8 @@ typedef struct ReinaTuning {
     uint32_t grid_blocks;   /* persistent traversal blocks (256 threads)                       */
     uint32_t shade_blocks;  /* shade blocks (256 threads), a multiple of 8                      */
     uint32_t waves_per_simd;/* traversal register budget: 0 (compiler's choice), 5, 6 or 8      */
+    uint32_t tail_paths;    /* when at most this many paths are live, the frame ends in one
+                               tail launch (per-wave extend/shade steps); 0 = never            */
 } ReinaTuning;
 int reina_rt_get_tuning(const ReinaScene* scene, ReinaTuning* out);
 int reina_rt_set_tuning(ReinaScene* scene, const ReinaTuning* tuning);

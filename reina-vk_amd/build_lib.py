@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libreina_rt.so")
 
-SOURCES = ["api.cpp", "bvh_host.cpp", "bvh_device.hip", "integrator.hip", "trace_f32.hip", "trace_q8.hip", "trace_multi_f32.hip", "trace_multi_q8.hip", "server.hip", "server.cpp", "post.hip"]
+SOURCES = ["api.cpp", "bvh_host.cpp", "bvh_device.hip", "integrator.hip", "trace_f32.hip", "trace_q8.hip", "trace_multi_f32.hip", "trace_multi_q8.hip", "server.hip", "server.cpp", "tail.hip", "post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

@@ -304,6 +304,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     s->tune.refill_min = 16;
     s->tune.grid = cus * (s->tune.stack_depth <= 8 ? 8 : 4);
     s->tune.shade_grid = cus * 4;
+    s->tune.tail_paths = kDefaultTailPaths;
     s->min_stack = s->tune.stack_depth;
     *out = s;
     return REINA_RT_OK;
@@ -324,6 +325,8 @@ int reina_rt_scene_destroy(ReinaScene* s) {
     if (s->h_count) hipHostFree(s->h_count);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->tev0) hipEventDestroy(s->tev0);
+    if (s->tev1) hipEventDestroy(s->tev1);
     for (hipEvent_t e : s->xev) hipEventDestroy(e);
     if (s->arena) hipFree(s->arena);
     delete s;
@@ -421,6 +424,8 @@ static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
         HIP_TRY(hipHostMalloc((void**)&s->h_count, sizeof(Ctrl), hipHostMallocDefault));
         HIP_TRY(hipEventCreate(&s->ev0));
         HIP_TRY(hipEventCreate(&s->ev1));
+        HIP_TRY(hipEventCreate(&s->tev0));
+        HIP_TRY(hipEventCreate(&s->tev1));
     }
     if (npx <= s->cap_px) return REINA_RT_OK;
     free_trace_buffers(s);
@@ -466,11 +471,21 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
     uint32_t it = 0;
     const uint32_t K = 4;
+    uint64_t live = max_iters ? (uint64_t)W * H : 0;
+    s->tail_used = false;
     const bool timex = (s->options & REINA_RT_TRACE_TIME_EXTEND) != 0;
     const bool count = (s->options & REINA_RT_TRACE_COUNT) != 0;
     s->xev_used = 0;
     while (it < max_iters) {
         uint32_t slot = it % 3, in_q = it & 1u;
+        if (live <= s->tune.tail_paths) {
+            // few paths left: finish them in one launch
+            HIP_TRY(hipEventRecord(s->tev0, st));
+            HIP_TRY(launch_tail(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->stats, s->tune, (uint32_t)live, st));
+            HIP_TRY(hipEventRecord(s->tev1, st));
+            s->tail_used = true;
+            break;
+        }
         if (timex) {
             if (s->xev.size() < 2 * (size_t)(s->xev_used + 1)) {
                 hipEvent_t a, b;
@@ -491,7 +506,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
             HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count[0][0], sizeof(Ctrl::count),
                                    hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
-            uint64_t live = 0;
+            live = 0;
             for (uint32_t p = 0; p < kParts; ++p) live += s->h_count[p * kLine];
             if (live == 0) break;
         }
@@ -510,7 +525,7 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
     if (!s->traced) return REINA_RT_OK;
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipEventSynchronize(s->ev1));
-    unsigned long long st3[4] = {0, 0, 0, 0};
+    unsigned long long st3[5] = {0, 0, 0, 0, 0};
     HIP_TRY(hipMemcpy(st3, s->stats, sizeof st3, hipMemcpyDeviceToHost));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
@@ -528,6 +543,9 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
         xms += e;
     }
     o->extend_ms = xms;
+    o->tail_rays = st3[4];
+    o->tail_launches = s->tail_used ? 1u : 0u;
+    if (s->tail_used) HIP_TRY(hipEventElapsedTime(&o->tail_ms, s->tev0, s->tev1));
     return REINA_RT_OK;
 }
 
@@ -538,6 +556,7 @@ int reina_rt_get_tuning(const ReinaScene* s, ReinaTuning* o) {
     o->grid_blocks = s->tune.grid;
     o->shade_blocks = s->tune.shade_grid;
     o->waves_per_simd = s->tune.waves_per_simd;
+    o->tail_paths = s->tune.tail_paths;
     return REINA_RT_OK;
 }
 
@@ -552,13 +571,14 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
     if (t->grid_blocks < 1 || t->grid_blocks > 65536 || t->shade_blocks < kParts || t->shade_blocks > 65536 ||
         t->shade_blocks % kParts)
         return fail(REINA_RT_ERR_INVALID_ARG, "grid_blocks must be in 1..65536 and shade_blocks a multiple of 8");
+    if (t->waves_per_simd != 0 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
+        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
     s->tune.stack_depth = sd;
     s->tune.refill_min = t->refill_min;
     s->tune.grid = t->grid_blocks;
     s->tune.shade_grid = t->shade_blocks;
-    if (t->waves_per_simd != 0 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
-        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
     s->tune.waves_per_simd = t->waves_per_simd;
+    s->tune.tail_paths = t->tail_paths;
     return REINA_RT_OK;
 }
 

@@ -24,6 +24,10 @@ int fail(int code, const std::string& msg);
             return fail(REINA_RT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Live paths at or below which reina_rt_trace ends the frame in one tail
+// launch (ReinaTuning.tail_paths default).
+constexpr uint32_t kDefaultTailPaths = 1u << 18;
+
 // Kernel parameters of one vkCmdTraceRaysKHR-equivalent render.
 inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_t H) {
     TraceParams P;
@@ -71,7 +75,9 @@ struct ReinaScene {
     unsigned long long* stats = nullptr;
     uint32_t* fetch = nullptr;
     uint32_t* h_count = nullptr;        // pinned
-    rt::Tuning tune{16, 16, 0, 0, 0};
+    rt::Tuning tune{16, 16, 0, 0, 0, 0};
+    bool tail_used = false;
+    hipEvent_t tev0 = nullptr, tev1 = nullptr;   // around the tail launch
     uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool traced = false;

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         ctrl[slot].count[p][0] = c;
         ctrl[slot].fetch[p][0] = 0;
         ctrl[slot].sfetch[p][0] = 0;
-        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; }
+        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; stats[4] = 0; }
     }
     if (i >= npx) return;
     uint32_t x = i % P.W, y = i / P.W;

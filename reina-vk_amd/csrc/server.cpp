@@ -255,7 +255,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
     if ((int)d->device >= ndev) return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal out of range");
     Tuning tu{d->tuning.stack_depth, d->tuning.refill_min, d->tuning.grid_blocks, d->tuning.shade_blocks,
-              d->tuning.waves_per_simd};
+              d->tuning.waves_per_simd, 0};
     if (tu.stack_depth == 0) tu.stack_depth = 16;
     if (tu.stack_depth != 8 && tu.stack_depth != 16 && tu.stack_depth != 24 && tu.stack_depth != 32)
         return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");

@@ -1,0 +1,146 @@
+// tail.hip — the end of a frame in one launch.
+//
+// A frame's bounce iterations shrink as pixels finish their samples, but a
+// pixel's samples are sequential (one RNG state per pixel, Q2 of the
+// reference: shaders/raytrace/raytrace.rgen.glsl:259-264), so the last few
+// thousand pixels with long paths need hundreds more iterations, each paying
+// a launch and the latency of its slowest traversal. Once few paths remain,
+// k_tail takes them over: each wave claims queue entries (one pixel each) and
+// runs extend → shade (→ shadow) steps for its 64 lanes in lockstep, a lane
+// taking the next entry when its pixel has finished all samples. Per pixel the
+// arithmetic is exactly the wavefront kernels' (trav_step, shade_ray), so the
+// image is bit-identical.
+#include <hip/hip_runtime.h>
+
+#include "shade_step.h"
+#include "traverse.h"
+#include "wavefront.h"
+
+namespace rt {
+
+template <int SD, bool Q, bool NEE>
+__global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathState S, Queues Q_, uint32_t in_q,
+                                              Ctrl* ctrl, uint32_t slot, unsigned long long* stats) {
+    __shared__ uint2 stack[SD * 256];
+    uint2* stk = stack + threadIdx.x;
+    const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
+    const float4* rin = Q_.ray[in_q];
+    const uint32_t* pin = Q_.pix[in_q];
+    uint32_t part = blockIdx.x % kParts;
+    bool have = false, drained = false;
+    uint32_t pp = 0;
+    float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+    uint32_t nrays = 0, nshadow = 0, dummy_v = 0, dummy_t = 0;
+    for (;;) {
+        // lanes without a pixel take the next queue entry (own XCD's partition first)
+        if (!have && !drained) {
+            for (uint32_t k = 0; k < kParts && !have; ++k) {
+                const uint32_t p = (part + k) % kParts;
+                const uint32_t n = ctrl[slot].count[p][0];
+                uint32_t* f = &ctrl[slot].fetch[p][0];
+                if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
+                const uint32_t j = atomicAdd(f, 1u);
+                if (j >= n) continue;
+                const uint32_t i = p * Q_.cap + j;
+                pp = pin[i];
+                ra = rin[2 * i];
+                rb = rin[2 * i + 1];
+                have = true;
+                part = p;
+            }
+            if (!have) drained = true;
+        }
+        if (__ballot(have) == 0ull) break;
+        // extend
+        TRay r;
+        HitRec h;
+        TravState ts;
+        bool tracing = false;
+        if (have) {
+            tray_init(r, ra, rb);
+            h = miss_record(r);
+            ts.cur = 0;
+            ts.sp = 0;
+            tracing = r.valid && sv.num_nodes != 0;
+            nrays++;
+        }
+        while (__ballot(tracing)) {
+            if (tracing) tracing = trav_step<false, Q>(nodes, sv.tris, r, h, ts, stk, 256, dummy_v, dummy_t);
+        }
+        // shade
+        ShadeResult res;
+        res.push = false;
+        res.shadow = false;
+        if (have) {
+            res = shade_ray<NEE>(sv, P, S, pp, pp, ra, rb, make_float4(h.t, __uint_as_float(h.prim), h.u, h.v));
+        }
+        // shadow ray (next-event estimation)
+        if (NEE) {
+            bool sh = have && res.shadow;
+            HitRec hs;
+            if (sh) {
+                tray_init(r, make_float4(res.so.x, res.so.y, res.so.z, 0.0f),
+                          make_float4(res.sd.x, res.sd.y, res.sd.z, res.stmax));
+                hs = miss_record(r);
+                ts.cur = 0;
+                ts.sp = 0;
+                nshadow++;
+                if (!r.valid || sv.num_nodes == 0) sh = false;
+            }
+            while (__ballot(sh)) {
+                if (sh) sh = trav_step<false, Q, true>(nodes, sv.tris, r, hs, ts, stk, 256, dummy_v, dummy_t, sv.tri_flags);
+            }
+            if (have && res.shadow && hs.prim == 0xFFFFFFFFu) {
+                float4& L = S.st2[pp];
+                L = make_float4(L.x + res.scontrib.x, L.y + res.scontrib.y, L.z + res.scontrib.z, L.w);
+            }
+        }
+        if (have) {
+            if (res.push) {
+                ra = make_float4(res.no.x, res.no.y, res.no.z, 0.0f);
+                rb = make_float4(res.nd.x, res.nd.y, res.nd.z, 10000.0f);
+            } else {
+                have = false;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        nrays += __shfl_xor(nrays, o);
+        nshadow += __shfl_xor(nshadow, o);
+    }
+    if (__lane_id() == 0) {
+        if (nrays) { atomicAdd(stats, (unsigned long long)nrays); atomicAdd(stats + 4, (unsigned long long)nrays); }
+        if (nshadow) atomicAdd(stats + 3, (unsigned long long)nshadow);
+    }
+}
+
+template <bool Q, bool NEE>
+static void launch_tail_qn(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q_,
+                           uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
+                           uint32_t blocks, hipStream_t st) {
+    switch (tu.stack_depth) {
+        case 8: hipLaunchKernelGGL((k_tail<8, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
+        case 16: hipLaunchKernelGGL((k_tail<16, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
+        case 24: hipLaunchKernelGGL((k_tail<24, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
+        default: hipLaunchKernelGGL((k_tail<32, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
+    }
+}
+
+hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
+                       uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
+                       uint32_t live, hipStream_t st) {
+    // one lane per live path, at least one wave per partition
+    uint32_t blocks = (live + 255u) / 256u;
+    blocks = blocks < kParts ? kParts : blocks;
+    const bool q = sv.qnodes != nullptr;
+    if (q) {
+        if (P.nee) launch_tail_qn<true, true>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+        else launch_tail_qn<true, false>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+    } else {
+        if (P.nee) launch_tail_qn<false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+        else launch_tail_qn<false, false>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rt

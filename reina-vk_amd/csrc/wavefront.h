@@ -87,6 +87,7 @@ struct Tuning {
     uint32_t grid;          // persistent blocks of 256 threads
     uint32_t shade_grid;    // blocks of the shade kernel
     uint32_t waves_per_simd;  // register budget of the traversal kernel: 0 (compiler's choice), 5, 6 or 8
+    uint32_t tail_paths;      // live paths at which the frame ends in one tail launch (0: never)
 };
 
 // Rays to trace: kParts partitions. Partitioned mode (counts != null):
@@ -137,6 +138,11 @@ hipError_t launch_connect_f32(const SceneView& sv, const Queues& Q, const PathSt
 hipError_t launch_connect_q8(const SceneView& sv, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
                              unsigned long long* stats, const Tuning& tu, hipStream_t st);
 hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, hipStream_t st);
+// Finishes every live path of the current queue in one launch (tail.hip);
+// stats[0] and stats[4] += its extension rays, stats[3] += its shadow rays.
+hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
+                       uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
+                       uint32_t live, hipStream_t st);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,
                               const BloomPushConsts& b, const TonemappingPushConsts& t, hipStream_t st);

@@ -127,13 +127,13 @@ class DeviceScene:
         t = abi.ReinaTuning()
         check(lib().reina_rt_get_tuning(self.handle, ctypes.byref(t)))
         return dict(stack_depth=t.stack_depth, refill_min=t.refill_min, grid_blocks=t.grid_blocks,
-                    shade_blocks=t.shade_blocks, waves_per_simd=t.waves_per_simd)
+                    shade_blocks=t.shade_blocks, waves_per_simd=t.waves_per_simd, tail_paths=t.tail_paths)
 
     def set_tuning(self, **kw):
         cur = self.get_tuning()
         cur.update(kw)
         t = abi.ReinaTuning(cur["stack_depth"], cur["refill_min"], cur["grid_blocks"], cur["shade_blocks"],
-                            cur["waves_per_simd"])
+                            cur["waves_per_simd"], cur["tail_paths"])
         check(lib().reina_rt_set_tuning(self.handle, ctypes.byref(t)))
 
     def set_trace_options(self, options):

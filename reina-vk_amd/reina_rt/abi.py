@@ -84,13 +84,14 @@ class ReinaTraceStats(ctypes.Structure):
     _fields_ = [("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("node_visits", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64),
                 ("iterations", ctypes.c_uint32), ("extend_launches", ctypes.c_uint32),
-                ("trace_ms", ctypes.c_float), ("extend_ms", ctypes.c_float)]
+                ("trace_ms", ctypes.c_float), ("extend_ms", ctypes.c_float),
+                ("tail_rays", ctypes.c_uint64), ("tail_ms", ctypes.c_float), ("tail_launches", ctypes.c_uint32)]
 
 
 class ReinaTuning(ctypes.Structure):
     _fields_ = [("stack_depth", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
                 ("grid_blocks", ctypes.c_uint32), ("shade_blocks", ctypes.c_uint32),
-                ("waves_per_simd", ctypes.c_uint32)]
+                ("waves_per_simd", ctypes.c_uint32), ("tail_paths", ctypes.c_uint32)]
 
 
 class ReinaServerDesc(ctypes.Structure):

@@ -284,27 +284,41 @@ def test_instanced_vs_fp64(cuda, flags):
 
 
 # ---- integrator ------------------------------------------------------------------
-def test_render_cornell_bitexact(cuda, cornell_arrays):
+TAILS = [0, 1500, 1 << 30]   # wavefront only; wavefront then tail kernel; tail kernel only
+
+
+@pytest.mark.parametrize("tail", TAILS)
+def test_render_cornell_bitexact(cuda, cornell_arrays, tail):
     torch = cuda
     arrays, cam = cornell_arrays
     W = H = 64
     pc = scenes.cornell_push(W, H, spp=16, max_bounces=8)
     dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
     got = gpu_render(torch, dev, pc, W, H)
     ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
     assert np.array_equal(_bits(got), _bits(ref)), "pixels differing: %d" % (
         (_bits(got) != _bits(ref)).any(-1).sum())
     st = dev.stats()
     assert st.extension_rays == nrays
+    assert st.tail_launches == (1 if tail else 0)
+    if tail == 0:
+        assert st.tail_rays == 0
+    elif tail == 1500:
+        assert 0 < st.tail_rays < st.extension_rays
+    else:
+        assert st.tail_rays == st.extension_rays
     dev.close()
 
 
-def test_render_icosphere_bitexact(cuda, icosphere_arrays):
+@pytest.mark.parametrize("tail", TAILS)
+def test_render_icosphere_bitexact(cuda, icosphere_arrays, tail):
     torch = cuda
     arrays, cam = icosphere_arrays
     W, H = 96, 54
     pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 8)
     dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
     got = gpu_render(torch, dev, pc, W, H)
     orc = oracle.OracleScene(arrays, make_desc)
     orc.set_bvh(*dev.export_bvh())
@@ -316,14 +330,17 @@ def test_render_icosphere_bitexact(cuda, icosphere_arrays):
     dev.close()
 
 
-def test_render_partitions_bitexact(cuda, cornell_arrays):
+@pytest.mark.parametrize("tail", [0, 1 << 30])
+def test_render_partitions_bitexact(cuda, cornell_arrays, tail):
     """150 x 100 pixels = 15 queue tiles of 1024 (two per XCD partition, the
-    last one partial): every partition is used and refilled."""
+    last one partial): every partition is used and refilled (by the
+    wavefront kernels, or claimed by the tail kernel)."""
     torch = cuda
     arrays, _ = cornell_arrays
     W, H = 150, 100
     pc = scenes.cornell_push(W, H, spp=3, max_bounces=8)
     dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
     got = gpu_render(torch, dev, pc, W, H)
     ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
     assert np.array_equal(_bits(got), _bits(ref))
@@ -402,8 +419,8 @@ def test_emissive_tables_match_oracle(cuda):
         dev.close()
 
 
-@pytest.mark.parametrize("flags", [0, abi.NODES_QUANTIZED])
-def test_render_nee_bitexact(cuda, flags):
+@pytest.mark.parametrize("flags,tail", [(0, 0), (0, 1 << 30), (abi.NODES_QUANTIZED, 700)])
+def test_render_nee_bitexact(cuda, flags, tail):
     """NEE + MIS frames (shade writes shadow rays, the any-hit kernel adds
     the unoccluded ones) equal the oracle's sequential restatement bit for
     bit; shadow rays are counted."""
@@ -412,6 +429,7 @@ def test_render_nee_bitexact(cuda, flags):
         W, H = (40, 30) if name != "icosphere" else (64, 36)
         pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 6)
         dev = reina_rt.DeviceScene(arrays, flags=flags)
+        dev.set_tuning(tail_paths=tail)
         dev.set_trace_options(abi.TRACE_NEE)
         got = gpu_render(torch, dev, pc, W, H)
         st = dev.stats()
