@@ -49,8 +49,17 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # the headless C++ host (links the library; rpath next to it)
+    host_src = os.path.join(HERE, "host", "reina_headless.cpp")
+    host_bin = os.path.join(HERE, "reina_headless")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", host_src, "-o", host_bin + ".tmp", "-L" + HERE, "-lreina_rt",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(host_bin + ".tmp", host_bin)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", host_bin)
     return LIB
 
 

@@ -1,0 +1,351 @@
+// reina_headless.cpp — a headless Reina host driving libreina_rt through its
+// C ABI: the render loop of src/Reina.cpp:296-392 without a window.
+//
+//   reina_headless <config.toml> <scene.rscn> <output-prefix> [push_consts.bin]
+//
+// config.toml uses the reference's keys (config/config.toml): [camera.dof]
+// focus_dist, defocus_multiplier; [sampling] samples_per_pixel, max_bounces,
+// direct_clamp, indirect_clamp; [saving] save_on_samples;
+// [postprocessing.bloom] radius, threshold, intensity;
+// [postprocessing.tonemap] exposure; plus a [headless] section for what the
+// reference takes from its window and camera: width, height, eye, center,
+// fov, total_samples, nee. Each frame is one reina_rt_trace (samples_per_pixel
+// samples, sampleBatch = frame index, running mean as Reina::traceRays); when
+// the accumulated sample count reaches an entry of save_on_samples the image
+// is post-processed (bloom + ACES tone map, reina_rt_postprocess) and written
+// as <prefix>_<samples>.png, as Reina's save() writes its RGBA8 swapchain
+// image (src/Reina.cpp:19-47). The scene file is the packed ReinaSceneDesc
+// arrays (reina_rt.scene.save_scene); push_consts.bin, if given, replaces the
+// camera and sampling push constants by a host's own 160-byte RtPushConsts.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/reina_rt.h"
+
+namespace {
+
+// ---- minimal TOML: [section.sub] headers, key = number | [n, n, ...], # comments
+struct Config {
+    std::map<std::string, std::vector<double>> v;
+    double num(const std::string& k, double dflt) const {
+        auto it = v.find(k);
+        return (it == v.end() || it->second.empty()) ? dflt : it->second[0];
+    }
+    std::vector<double> list(const std::string& k, std::vector<double> dflt) const {
+        auto it = v.find(k);
+        return it == v.end() ? dflt : it->second;
+    }
+};
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+Config parse_toml(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    Config c;
+    std::string line, section;
+    while (std::getline(f, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line = line.substr(0, hash);
+        line = trim(line);
+        if (line.empty()) continue;
+        if (line.front() == '[') {
+            section = trim(line.substr(1, line.find(']') - 1));
+            continue;
+        }
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string key = trim(line.substr(0, eq)), val = trim(line.substr(eq + 1));
+        std::vector<double> nums;
+        for (char& ch : val)
+            if (ch == '[' || ch == ']' || ch == ',') ch = ' ';
+        std::istringstream is(val);
+        std::string tok;
+        while (is >> tok) {
+            if (tok == "true") nums.push_back(1.0);
+            else if (tok == "false") nums.push_back(0.0);
+            else nums.push_back(std::stod(tok));
+        }
+        c.v[section.empty() ? key : section + "." + key] = nums;
+    }
+    return c;
+}
+
+// ---- packed scene file ------------------------------------------------------
+struct SceneFile {
+    std::vector<float> vertices, tbns, tex_coords;
+    std::vector<uint32_t> indices, tbn_indices, tex_indices;
+    std::vector<ReinaInstanceProperties> props;
+    std::vector<ReinaModelRange> models;
+    std::vector<ReinaInstanceDesc> instances;
+};
+
+template <typename T>
+void read_array(std::ifstream& f, std::vector<T>& out, size_t elem_bytes) {
+    uint64_t n = 0;
+    f.read(reinterpret_cast<char*>(&n), 8);
+    if (!f || n > (1ull << 34) / elem_bytes) throw std::runtime_error("scene file: bad array size");
+    out.resize(n * elem_bytes / sizeof(T));
+    f.read(reinterpret_cast<char*>(out.data()), (std::streamsize)(n * elem_bytes));
+    if (!f) throw std::runtime_error("scene file: truncated");
+}
+
+SceneFile load_scene(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    char magic[4];
+    uint32_t version = 0;
+    f.read(magic, 4);
+    f.read(reinterpret_cast<char*>(&version), 4);
+    if (!f || std::memcmp(magic, "RSCN", 4) != 0 || version != 1) throw std::runtime_error("not a scene file");
+    SceneFile s;
+    read_array(f, s.vertices, 16);
+    read_array(f, s.indices, 4);
+    read_array(f, s.tbns, 36);
+    read_array(f, s.tbn_indices, 4);
+    read_array(f, s.tex_coords, 8);
+    read_array(f, s.tex_indices, 4);
+    read_array(f, s.props, sizeof(ReinaInstanceProperties));
+    read_array(f, s.models, sizeof(ReinaModelRange));
+    read_array(f, s.instances, sizeof(ReinaInstanceDesc));
+    return s;
+}
+
+// ---- camera: glm::perspective / lookAt (src/graphics/Camera.cpp:10-11), inverted
+void mat_inverse(const double* m, double* inv) {   // 4x4 column-major, Gauss-Jordan
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) { a[r][c] = m[c * 4 + r]; a[r][c + 4] = r == c ? 1.0 : 0.0; }
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        for (int k = 0; k < 8; ++k) std::swap(a[c][k], a[p][k]);
+        const double d = a[c][c];
+        for (int k = 0; k < 8; ++k) a[c][k] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double f = a[r][c];
+            for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[c * 4 + r] = a[r][c + 4];
+}
+
+void camera_matrices(const double* eye, const double* center, double fov_deg, double aspect, float* inv_view,
+                     float* inv_proj) {
+    double f[3] = {center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]};
+    double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& x : f) x /= fl;
+    const double up[3] = {0.0, 1.0, 0.0};
+    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    double sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (double& x : s) x /= sl;
+    const double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    double view[16] = {s[0], u[0], -f[0], 0, s[1], u[1], -f[1], 0, s[2], u[2], -f[2], 0,
+                       -(s[0] * eye[0] + s[1] * eye[1] + s[2] * eye[2]),
+                       -(u[0] * eye[0] + u[1] * eye[1] + u[2] * eye[2]),
+                       (f[0] * eye[0] + f[1] * eye[1] + f[2] * eye[2]), 1};
+    const double zn = 0.1, zf = 100.0, t = std::tan(fov_deg * M_PI / 360.0);
+    double proj[16] = {0};
+    proj[0] = 1.0 / (aspect * t);
+    proj[5] = 1.0 / t;
+    proj[10] = -(zf + zn) / (zf - zn);
+    proj[11] = -1.0;
+    proj[14] = -(2.0 * zf * zn) / (zf - zn);
+    double iv[16], ip[16];
+    mat_inverse(view, iv);
+    mat_inverse(proj, ip);
+    for (int k = 0; k < 16; ++k) { inv_view[k] = (float)iv[k]; inv_proj[k] = (float)ip[k]; }
+}
+
+// ---- PNG (RGBA8, stored deflate blocks) -----------------------------------
+uint32_t crc32(const uint8_t* p, size_t n, uint32_t c = 0) {
+    static uint32_t table[256];
+    static bool init = false;
+    if (!init) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t v = i;
+            for (int k = 0; k < 8; ++k) v = (v & 1) ? 0xEDB88320u ^ (v >> 1) : v >> 1;
+            table[i] = v;
+        }
+        init = true;
+    }
+    c = ~c;
+    for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return ~c;
+}
+
+void put_be32(std::vector<uint8_t>& o, uint32_t v) {
+    for (int s = 24; s >= 0; s -= 8) o.push_back((uint8_t)(v >> s));
+}
+
+void chunk(std::vector<uint8_t>& png, const char* type, const std::vector<uint8_t>& data) {
+    put_be32(png, (uint32_t)data.size());
+    std::vector<uint8_t> td(type, type + 4);
+    td.insert(td.end(), data.begin(), data.end());
+    png.insert(png.end(), td.begin(), td.end());
+    put_be32(png, crc32(td.data(), td.size()));
+}
+
+void write_png(const std::string& path, const uint8_t* rgba, uint32_t W, uint32_t H) {
+    std::vector<uint8_t> raw;
+    raw.reserve((size_t)H * (4 * W + 1));
+    for (uint32_t y = 0; y < H; ++y) {
+        raw.push_back(0);   // filter: none
+        raw.insert(raw.end(), rgba + (size_t)y * W * 4, rgba + (size_t)(y + 1) * W * 4);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};   // zlib header, no compression
+    for (size_t off = 0; off < raw.size() || off == 0; off += 65535) {
+        const size_t n = std::min<size_t>(65535, raw.size() - off);
+        z.push_back(off + n >= raw.size() ? 1 : 0);
+        z.push_back((uint8_t)(n & 0xFF));
+        z.push_back((uint8_t)(n >> 8));
+        z.push_back((uint8_t)(~n & 0xFF));
+        z.push_back((uint8_t)((~n >> 8) & 0xFF));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + n);
+        if (raw.empty()) break;
+    }
+    uint32_t a = 1, b = 0;   // Adler-32
+    for (uint8_t x : raw) { a = (a + x) % 65521; b = (b + a) % 65521; }
+    put_be32(z, (b << 16) | a);
+    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, W);
+    put_be32(ihdr, H);
+    ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});   // 8-bit RGBA
+    chunk(png, "IHDR", ihdr);
+    chunk(png, "IDAT", z);
+    chunk(png, "IEND", {});
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(png.data()), (std::streamsize)png.size());
+    if (!f) throw std::runtime_error("could not write " + path);
+}
+
+void check(int rc, const char* what) {
+    if (rc != REINA_RT_OK) throw std::runtime_error(std::string(what) + ": " + reina_rt_last_error());
+}
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int run(int argc, char** argv) {
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: %s <config.toml> <scene.rscn> <output-prefix> [push_consts.bin]\n", argv[0]);
+        return 2;
+    }
+    const Config cfg = parse_toml(argv[1]);
+    SceneFile sf = load_scene(argv[2]);
+    const std::string prefix = argv[3];
+    const uint32_t W = (uint32_t)cfg.num("headless.width", 256), H = (uint32_t)cfg.num("headless.height", 256);
+    const std::vector<double> eye = cfg.list("headless.eye", {0.0, 1.0, 3.4});
+    const std::vector<double> center = cfg.list("headless.center", {0.0, 1.0, 0.0});
+    if (eye.size() != 3 || center.size() != 3) throw std::runtime_error("eye / center need 3 numbers");
+    const double fov = cfg.num("headless.fov", 40.0);
+    const uint32_t spp = (uint32_t)cfg.num("sampling.samples_per_pixel", 8);
+    const uint64_t total = (uint64_t)cfg.num("headless.total_samples", 64);
+    std::vector<double> saves = cfg.list("saving.save_on_samples", {(double)total});
+
+    ReinaSceneDesc d{};
+    d.struct_size = sizeof d;
+    d.vertices = sf.vertices.data(); d.num_vertices = sf.vertices.size() / 4;
+    d.indices = sf.indices.data(); d.num_indices = sf.indices.size();
+    d.tbns = sf.tbns.data(); d.num_tbns = sf.tbns.size() / 9;
+    d.tbn_indices = sf.tbn_indices.data(); d.num_tbn_indices = sf.tbn_indices.size();
+    d.tex_coords = sf.tex_coords.data(); d.num_tex_coords = sf.tex_coords.size() / 2;
+    d.tex_indices = sf.tex_indices.data(); d.num_tex_indices = sf.tex_indices.size();
+    d.props = sf.props.data(); d.num_props = (uint32_t)sf.props.size();
+    d.models = sf.models.data(); d.num_models = (uint32_t)sf.models.size();
+    d.instances = sf.instances.data(); d.num_instances = (uint32_t)sf.instances.size();
+    ReinaScene* scene = nullptr;
+    check(reina_rt_scene_create(&d, &scene), "reina_rt_scene_create");
+    if (cfg.num("headless.nee", 0) != 0) check(reina_rt_set_trace_options(scene, REINA_RT_TRACE_NEE), "options");
+
+    RtPushConsts pc{};
+    camera_matrices(eye.data(), center.data(), fov, (double)W / H, pc.invView, pc.invProjection);
+
+    pc.sampleBatch = 0;
+    pc.focusDist = (float)cfg.num("camera.dof.focus_dist", 2.2);
+    pc.defocusMultiplier = (float)(cfg.num("camera.dof.defocus_multiplier", 0.0) / 100.0);   // Reina.cpp:149
+    pc.directClamp = (float)cfg.num("sampling.direct_clamp", 100.0);
+    pc.indirectClamp = (float)cfg.num("sampling.indirect_clamp", 10.0);
+    pc.samplesPerPixel = spp;
+    pc.maxBounces = (uint32_t)cfg.num("sampling.max_bounces", 8);
+    if (argc == 5) {
+        // the host application's own RtPushConsts (160 bytes, polyglot/raytrace.h layout)
+        std::ifstream pf(argv[4], std::ios::binary);
+        RtPushConsts given{};
+        pf.read(reinterpret_cast<char*>(&given), sizeof given);
+        if (!pf) throw std::runtime_error(std::string("cannot read 160 bytes from ") + argv[4]);
+        pc = given;
+        pc.sampleBatch = 0;
+    }
+    BloomPushConsts bloom{};
+    bloom.radius = (float)cfg.num("postprocessing.bloom.radius", 5.0);
+    bloom.threshold = (float)cfg.num("postprocessing.bloom.threshold", 1.0);
+    bloom.intensity = (float)cfg.num("postprocessing.bloom.intensity", 0.05);
+    TonemappingPushConsts tone{};
+    tone.exposure = (float)cfg.num("postprocessing.tonemap.exposure", 1.0);
+
+    hipStream_t st;
+    hip_check(hipStreamCreate(&st), "hipStreamCreate");
+    float *image = nullptr, *scratch = nullptr;
+    uint8_t* ldr = nullptr;
+    hip_check(hipMalloc(&image, (size_t)W * H * 16), "hipMalloc");
+    hip_check(hipMalloc(&scratch, (size_t)W * H * 32), "hipMalloc");
+    hip_check(hipMalloc(&ldr, (size_t)W * H * 4), "hipMalloc");
+    std::vector<uint8_t> host((size_t)W * H * 4);
+    uint64_t samples = 0;
+    size_t next_save = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (samples < total) {
+        check(reina_rt_trace(scene, &pc, image, W, H, st), "reina_rt_trace");
+        pc.sampleBatch++;   // after recording, as Reina::traceRays (src/Reina.cpp:431-432)
+        samples += spp;
+        while (next_save < saves.size() && samples >= (uint64_t)saves[next_save]) {
+            check(reina_rt_postprocess(image, scratch, ldr, W, H, &bloom, &tone, st), "reina_rt_postprocess");
+            hip_check(hipMemcpyAsync(host.data(), ldr, host.size(), hipMemcpyDeviceToHost, st), "copy");
+            hip_check(hipStreamSynchronize(st), "sync");
+            const std::string name = prefix + "_" + std::to_string((uint64_t)saves[next_save]) + ".png";
+            write_png(name, host.data(), W, H);
+            std::printf("saved %s after %llu samples per pixel\n", name.c_str(), (unsigned long long)samples);
+            next_save++;
+        }
+    }
+    hip_check(hipStreamSynchronize(st), "sync");
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("rendered %llu samples per pixel at %ux%u in %.3f s (%llu frames)\n", (unsigned long long)samples, W,
+                H, secs, (unsigned long long)pc.sampleBatch);
+    (void)hipFree(image);
+    (void)hipFree(scratch);
+    (void)hipFree(ldr);
+    (void)hipStreamDestroy(st);
+    check(reina_rt_scene_destroy(scene), "reina_rt_scene_destroy");
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        return run(argc, argv);
+    } catch (const std::exception& e) {   // same error behaviour as the reference (src/main.cpp:11-14)
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+}

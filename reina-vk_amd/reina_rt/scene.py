@@ -243,3 +243,24 @@ def push_consts(eye, center, fov_deg, width, height, spp, max_bounces, sample_ba
     pc["samplesPerPixel"] = spp
     pc["maxBounces"] = max_bounces
     return pc
+
+
+# ---- packed scene file (input of the C++ headless host) ----------------------------
+SCENE_FILE_MAGIC = b"RSCN"
+SCENE_FILE_ARRAYS = ("vertices", "indices", "tbns", "tbn_indices", "tex_coords", "tex_indices", "props", "models",
+                     "instances")
+
+
+def save_scene(arrays, path):
+    """Writes the packed buffers of build_arrays() (the ReinaSceneDesc
+    arrays, in its field order) as: "RSCN", uint32 version 1, then per array
+    uint64 element count and the raw little-endian elements (vertices 16 B,
+    indices 4 B, tbns 36 B, tbn / tex indices 4 B, tex coords 8 B, props 120 B,
+    model ranges 32 B, instances 80 B)."""
+    import struct
+    with open(path, "wb") as f:
+        f.write(SCENE_FILE_MAGIC + struct.pack("<I", 1))
+        for name in SCENE_FILE_ARRAYS:
+            a = np.ascontiguousarray(arrays[name])
+            f.write(struct.pack("<Q", len(a)))
+            f.write(a.tobytes())
