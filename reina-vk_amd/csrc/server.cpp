@@ -278,7 +278,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "device query failed");
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     if (tu.grid == 0) tu.grid = cus * (tu.stack_depth <= 8 ? 8 : 4);
-    if (tu.shade_grid == 0) tu.shade_grid = cus * 4;
+    if (tu.shade_grid == 0) tu.shade_grid = cus * 16;
     s->tune = tu;
     const uint64_t slots = tiles * 1024;
     const uint64_t bytes = slots * REINA_RT_SERVER_BYTES_PER_PIXEL;
