@@ -303,7 +303,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     s->tune.stack_depth = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
     s->tune.refill_min = 16;
     s->tune.grid = cus * (s->tune.stack_depth <= 8 ? 8 : 4);
-    s->tune.shade_grid = cus * 16;   // 16 waves per CU in flight hide the shading loads
+    s->tune.shade_grid = cus * 16;   // fills every CU to its register-limited occupancy (latency hiding)
     s->tune.tail_paths = kDefaultTailPaths;
     s->min_stack = s->tune.stack_depth;
     *out = s;
