@@ -322,7 +322,10 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  * (in order) as soon as enough contiguous slots and a request index are
  * free, generates their camera rays into the shared queues, and runs bounce
  * iterations over every admitted request at once (captured in a HIP graph of
- * 6 iterations). A request completes when all its pixels have finished all
+ * 6 iterations). When at most tuning.tail_paths paths are live after a graph,
+ * one tail launch (per-wave extend/shade steps, as reina_rt_trace's) finishes
+ * every running request; requests arriving meanwhile wait for it. A request
+ * completes when all its pixels have finished all
  * samples; its image is then written. Scenes must stay alive until their
  * requests complete. submit / poll / wait may be called from any thread. */
 #define REINA_RT_SERVER_BYTES_PER_PIXEL 152u
@@ -336,8 +339,12 @@ typedef struct ReinaServerDesc {
     uint32_t max_pending;        /* bounded FIFO; submit beyond it -> REINA_RT_ERR_BUSY  */
     uint32_t quantized;          /* node format of every scene served: 0 FP32, 1 8-bit   */
     ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
-                                    (stack_depth bounds the BVH depth of served scenes) */
+                                    (stack_depth bounds the BVH depth of served scenes;
+                                    tail_paths 0 = 2^18, REINA_RT_TAIL_OFF = never)      */
 } ReinaServerDesc;
+
+/* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */
+#define REINA_RT_TAIL_OFF 0xFFFFFFFFu
 
 #define REINA_RT_REQ_QUEUED    0u
 #define REINA_RT_REQ_RUNNING   1u
@@ -360,6 +367,8 @@ typedef struct ReinaServerStats {
     uint32_t pending, running;   /* now                                            */
     uint32_t slot_tiles_total, slot_tiles_used;   /* 1024-pixel tiles              */
     double   busy_s;             /* worker time with admitted requests             */
+    uint64_t tail_launches;      /* tail launches that finished every running request */
+    uint64_t tail_rays;          /* extension rays traced by those launches        */
 } ReinaServerStats;
 
 int reina_rt_server_create(const ReinaServerDesc* desc, ReinaServer** out_server);

@@ -61,6 +61,7 @@ struct ReinaServer {
     JobRec* jobs_h = nullptr;
     uint32_t* done_h = nullptr;
     unsigned long long* stats_h = nullptr;
+    uint32_t* count_h = nullptr;         // ctrl[0].count after a graph: the live paths per partition
     hipStream_t st = nullptr;
     hipGraphExec_t graph = nullptr;
     // bookkeeping (mu)
@@ -183,8 +184,25 @@ static void worker_main(ReinaServer* s) {
         if (running.empty()) continue;
         W_TRY(hipGraphLaunch(s->graph, s->st));
         W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
-        W_TRY(hipMemcpyAsync(s->stats_h, s->stats, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+        W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+        W_TRY(hipMemcpyAsync(s->count_h, s->ctrl[0].count, sizeof(Ctrl::count), hipMemcpyDeviceToHost, s->st));
         W_TRY(hipStreamSynchronize(s->st));
+        // Few paths left: one tail launch finishes every running request (the
+        // graph leaves the next iteration's rays in queue 0, control slot 0).
+        uint64_t live = 0;
+        for (uint32_t p = 0; p < kParts; ++p) live += s->count_h[p * kLine];
+        if (s->tune.tail_paths != 0 && live != 0 && live <= s->tune.tail_paths) {
+            const unsigned long long before = s->stats_h[0];
+            W_TRY(launch_tail_multi(s->quantized, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats, s->done_d, s->tune,
+                                    (uint32_t)live, s->st));
+            W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
+            W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+            W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+            W_TRY(hipStreamSynchronize(s->st));
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->counts.tail_launches += 1;
+            s->counts.tail_rays += s->stats_h[0] - before;
+        }
         std::vector<uint64_t> finished;
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -235,6 +253,7 @@ static void free_server(ReinaServer* s) {
     if (s->jobs_h) hipHostFree(s->jobs_h);
     if (s->done_h) hipHostFree(s->done_h);
     if (s->stats_h) hipHostFree(s->stats_h);
+    if (s->count_h) hipHostFree(s->count_h);
     delete s;
 }
 
@@ -255,7 +274,9 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
     if ((int)d->device >= ndev) return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal out of range");
     Tuning tu{d->tuning.stack_depth, d->tuning.refill_min, d->tuning.grid_blocks, d->tuning.shade_blocks,
-              d->tuning.waves_per_simd, 0};
+              d->tuning.waves_per_simd, d->tuning.tail_paths};
+    if (tu.tail_paths == 0) tu.tail_paths = kDefaultTailPaths;
+    else if (tu.tail_paths == REINA_RT_TAIL_OFF) tu.tail_paths = 0;   // internally 0 = never
     if (tu.stack_depth == 0) tu.stack_depth = 16;
     if (tu.stack_depth != 8 && tu.stack_depth != 16 && tu.stack_depth != 24 && tu.stack_depth != 32)
         return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");
@@ -303,7 +324,8 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (hipMemset(s->ctrl, 0, cbytes) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipMemset failed");
     if (hipHostMalloc((void**)&s->jobs_h, kMaxJobs * sizeof(JobRec), 0) != hipSuccess ||
         hipHostMalloc((void**)&s->done_h, kMaxJobs * sizeof(uint32_t), 0) != hipSuccess ||
-        hipHostMalloc((void**)&s->stats_h, sizeof(unsigned long long), 0) != hipSuccess)
+        hipHostMalloc((void**)&s->stats_h, 8 * sizeof(unsigned long long), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->count_h, sizeof(Ctrl::count), 0) != hipSuccess)
         return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
     if (hipStreamCreate(&s->st) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipStreamCreate failed");
     hipError_t e = capture_graph(s);

@@ -10,6 +10,11 @@
 // taking the next entry when its pixel has finished all samples. Per pixel the
 // arithmetic is exactly the wavefront kernels' (trav_step, shade_ray), so the
 // image is bit-identical.
+//
+// The server's version (MULTI) does the same over the shared queues of many
+// requests: each entry carries its request index, the lane reads that
+// request's scene and parameters, and pixels that finish their last sample
+// are counted per request as in k_shade_multi.
 #include <hip/hip_runtime.h>
 
 #include "shade_step.h"
@@ -18,17 +23,21 @@
 
 namespace rt {
 
-template <int SD, bool Q, bool NEE>
+template <int SD, bool Q, bool NEE, bool MULTI>
 __global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathState S, Queues Q_, uint32_t in_q,
-                                              Ctrl* ctrl, uint32_t slot, unsigned long long* stats) {
+                                              Ctrl* ctrl, uint32_t slot, unsigned long long* stats,
+                                              const JobRec* __restrict__ jobs, uint32_t* done) {
     __shared__ uint2 stack[SD * 256];
     uint2* stk = stack + threadIdx.x;
-    const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
     const float4* rin = Q_.ray[in_q];
     const uint32_t* pin = Q_.pix[in_q];
     uint32_t part = blockIdx.x % kParts;
     bool have = false, drained = false;
-    uint32_t pp = 0;
+    uint32_t pw = 0, pp = 0, px = 0;
+    const JobRec* J = nullptr;
+    const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
+    const float4* tris = sv.tris;
+    uint64_t num_nodes = sv.num_nodes;
     float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
     uint32_t nrays = 0, nshadow = 0, dummy_v = 0, dummy_t = 0;
     for (;;) {
@@ -42,13 +51,24 @@ __global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathS
                 const uint32_t j = atomicAdd(f, 1u);
                 if (j >= n) continue;
                 const uint32_t i = p * Q_.cap + j;
-                pp = pin[i];
+                pw = pin[i];
                 ra = rin[2 * i];
                 rb = rin[2 * i + 1];
                 have = true;
                 part = p;
             }
-            if (!have) drained = true;
+            if (!have) {
+                drained = true;
+            } else if (MULTI) {
+                J = jobs + (pw >> kSlotBits);
+                pp = pw & ((1u << kSlotBits) - 1u);
+                px = pp - J->px_base;
+                nodes = Q ? static_cast<const void*>(J->sv.qnodes) : static_cast<const void*>(J->sv.nodes);
+                tris = J->sv.tris;
+                num_nodes = J->sv.num_nodes;
+            } else {
+                pp = px = pw;
+            }
         }
         if (__ballot(have) == 0ull) break;
         // extend
@@ -61,18 +81,21 @@ __global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathS
             h = miss_record(r);
             ts.cur = 0;
             ts.sp = 0;
-            tracing = r.valid && sv.num_nodes != 0;
+            tracing = r.valid && num_nodes != 0;
             nrays++;
         }
         while (__ballot(tracing)) {
-            if (tracing) tracing = trav_step<false, Q>(nodes, sv.tris, r, h, ts, stk, 256, dummy_v, dummy_t);
+            if (tracing) tracing = trav_step<false, Q>(nodes, tris, r, h, ts, stk, 256, dummy_v, dummy_t);
         }
         // shade
         ShadeResult res;
         res.push = false;
         res.shadow = false;
+        res.finished = false;
         if (have) {
-            res = shade_ray<NEE>(sv, P, S, pp, pp, ra, rb, make_float4(h.t, __uint_as_float(h.prim), h.u, h.v));
+            const float4 hh = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+            res = MULTI ? shade_ray<NEE>(J->sv, J->P, S, pp, px, ra, rb, hh)
+                        : shade_ray<NEE>(sv, P, S, pp, px, ra, rb, hh);
         }
         // shadow ray (next-event estimation)
         if (NEE) {
@@ -85,14 +108,27 @@ __global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathS
                 ts.cur = 0;
                 ts.sp = 0;
                 nshadow++;
-                if (!r.valid || sv.num_nodes == 0) sh = false;
+                if (!r.valid || num_nodes == 0) sh = false;
             }
+            const uint8_t* tflags = MULTI ? (J ? J->sv.tri_flags : nullptr) : sv.tri_flags;
             while (__ballot(sh)) {
-                if (sh) sh = trav_step<false, Q, true>(nodes, sv.tris, r, hs, ts, stk, 256, dummy_v, dummy_t, sv.tri_flags);
+                if (sh) sh = trav_step<false, Q, true>(nodes, tris, r, hs, ts, stk, 256, dummy_v, dummy_t, tflags);
             }
             if (have && res.shadow && hs.prim == 0xFFFFFFFFu) {
                 float4& L = S.st2[pp];
                 L = make_float4(L.x + res.scontrib.x, L.y + res.scontrib.y, L.z + res.scontrib.z, L.w);
+            }
+        }
+        if (MULTI) {
+            // pixels that finished their last sample: one atomic per request among the wave's finishers
+            const bool fin = have && res.finished;
+            uint64_t fm = __ballot(fin ? 1 : 0);
+            while (fm) {
+                const uint32_t lead = (uint32_t)__builtin_ctzll(fm);
+                const uint32_t jl = __shfl(pw >> kSlotBits, (int)lead);
+                const uint64_t same = __ballot((fin && (pw >> kSlotBits) == jl) ? 1 : 0);
+                if (__lane_id() == lead) atomicAdd(done + jl, (uint32_t)__popcll(same));
+                fm &= ~same;
             }
         }
         if (have) {
@@ -114,32 +150,46 @@ __global__ __launch_bounds__(256) void k_tail(SceneView sv, TraceParams P, PathS
     }
 }
 
-template <bool Q, bool NEE>
+template <bool Q, bool NEE, bool MULTI>
 static void launch_tail_qn(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q_,
-                           uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
-                           uint32_t blocks, hipStream_t st) {
+                           uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const JobRec* jobs,
+                           uint32_t* done, const Tuning& tu, uint32_t blocks, hipStream_t st) {
     switch (tu.stack_depth) {
-        case 8: hipLaunchKernelGGL((k_tail<8, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
-        case 16: hipLaunchKernelGGL((k_tail<16, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
-        case 24: hipLaunchKernelGGL((k_tail<24, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
-        default: hipLaunchKernelGGL((k_tail<32, Q, NEE>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats); break;
+        case 8: hipLaunchKernelGGL((k_tail<8, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
+        case 16: hipLaunchKernelGGL((k_tail<16, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
+        case 24: hipLaunchKernelGGL((k_tail<24, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
+        default: hipLaunchKernelGGL((k_tail<32, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
     }
+}
+
+// one lane per live path, at least one wave per partition
+static uint32_t tail_blocks(uint32_t live) {
+    const uint32_t blocks = (live + 255u) / 256u;
+    return blocks < kParts ? kParts : blocks;
 }
 
 hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
                        uint32_t live, hipStream_t st) {
-    // one lane per live path, at least one wave per partition
-    uint32_t blocks = (live + 255u) / 256u;
-    blocks = blocks < kParts ? kParts : blocks;
-    const bool q = sv.qnodes != nullptr;
-    if (q) {
-        if (P.nee) launch_tail_qn<true, true>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
-        else launch_tail_qn<true, false>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+    const uint32_t blocks = tail_blocks(live);
+    if (sv.qnodes != nullptr) {
+        if (P.nee) launch_tail_qn<true, true, false>(sv, P, S, Q, in_q, ctrl, slot, stats, nullptr, nullptr, tu, blocks, st);
+        else launch_tail_qn<true, false, false>(sv, P, S, Q, in_q, ctrl, slot, stats, nullptr, nullptr, tu, blocks, st);
     } else {
-        if (P.nee) launch_tail_qn<false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
-        else launch_tail_qn<false, false>(sv, P, S, Q, in_q, ctrl, slot, stats, tu, blocks, st);
+        if (P.nee) launch_tail_qn<false, true, false>(sv, P, S, Q, in_q, ctrl, slot, stats, nullptr, nullptr, tu, blocks, st);
+        else launch_tail_qn<false, false, false>(sv, P, S, Q, in_q, ctrl, slot, stats, nullptr, nullptr, tu, blocks, st);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_tail_multi(bool quantized, const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q,
+                             Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
+                             const Tuning& tu, uint32_t live, hipStream_t st) {
+    const uint32_t blocks = tail_blocks(live);
+    const SceneView sv{};
+    const TraceParams P{};
+    if (quantized) launch_tail_qn<true, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+    else launch_tail_qn<false, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
     return hipGetLastError();
 }
 

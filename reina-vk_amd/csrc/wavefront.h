@@ -143,6 +143,11 @@ hipError_t launch_finish(const TraceParams& P, const PathState& S, float4* img, 
 hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathState& S, const Queues& Q,
                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
                        uint32_t live, hipStream_t st);
+// Server version: finishes every request in the shared queues (pixels that
+// finish their last sample are counted in done[request]).
+hipError_t launch_tail_multi(bool quantized, const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q,
+                             Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
+                             const Tuning& tu, uint32_t live, hipStream_t st);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,
                               const BloomPushConsts& b, const TonemappingPushConsts& t, hipStream_t st);

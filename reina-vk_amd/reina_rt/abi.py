@@ -111,10 +111,11 @@ class ReinaServerStats(ctypes.Structure):
                 ("extension_rays", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
                 ("graph_replays", ctypes.c_uint64), ("pending", ctypes.c_uint32), ("running", ctypes.c_uint32),
                 ("slot_tiles_total", ctypes.c_uint32), ("slot_tiles_used", ctypes.c_uint32),
-                ("busy_s", ctypes.c_double)]
+                ("busy_s", ctypes.c_double), ("tail_launches", ctypes.c_uint64), ("tail_rays", ctypes.c_uint64)]
 
 
 SERVER_BYTES_PER_PIXEL = 152
+TAIL_OFF = 0xFFFFFFFF
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 TRACE_TIME_EXTEND = 0x1

@@ -98,7 +98,9 @@ def batched(torch, pool, reqs, server_kw, check=True):
                   "mrays_per_s": round(rays_p / t_plain / 1e6, 1), "extension_rays": int(rays_p)},
         "batched": {"seconds": round(t_batch, 3), "images_per_s": round(len(reqs) / t_batch, 3),
                     "mrays_per_s": round(rays_b / t_batch / 1e6, 1), "extension_rays": int(rays_b),
-                    "iterations": int(st1.iterations - st0.iterations), "latency": _pcts(lat)},
+                    "iterations": int(st1.iterations - st0.iterations),
+                    "tail_launches": int(st1.tail_launches - st0.tail_launches),
+                    "tail_rays": int(st1.tail_rays - st0.tail_rays), "latency": _pcts(lat)},
         "speedup_images_per_s": round(t_plain / t_batch, 3),
         "images_bit_identical": same,
     }

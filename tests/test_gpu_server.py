@@ -80,11 +80,14 @@ def test_server_request_equals_trace_and_oracle(cuda, pool):
     srv.close()
 
 
-def test_server_mix_equals_alone(cuda, pool):
+@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF, 3000], ids=["tail_default", "tail_off", "tail_3000"])
+def test_server_mix_equals_alone(cuda, pool, tail):
     """Eight requests on three scenes (different resolutions including a
     partial last tile, spp, depths, accumulation with sampleBatch > 0)
     submitted together: each image equals its reina_rt_trace image and its
-    image when served alone."""
+    image when served alone, whether the shared queues end with the tail
+    launch at once (default threshold 2^18 paths), never, or after some
+    graph iterations (3000 paths)."""
     torch = cuda
     rng = np.random.default_rng(3)
     jobs = []
@@ -97,8 +100,9 @@ def test_server_mix_equals_alone(cuda, pool):
             else _pc(cam, W, H, spp, depth, batch)
         init = rng.uniform(0, 1, (H, W, 4)).astype(np.float32) if batch else None
         jobs.append((dev, pc, W, H, init))
-    srv = reina_rt.Server(path_budget_bytes=256 << 20)
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, tail_paths=tail)
     mixed = served(torch, srv, jobs)
+    st_mix = srv.stats()
     for (dev, pc, W, H, init), got in zip(jobs, mixed):
         assert _same(got, plain(torch, dev, pc, W, H, init))
     for j in (1, 3):
@@ -106,6 +110,10 @@ def test_server_mix_equals_alone(cuda, pool):
         assert _same(alone, mixed[j])
     st = srv.stats()
     assert st.completed == len(jobs) + 2 and st.pending == 0 and st.running == 0 and st.slot_tiles_used == 0
+    if tail == abi.TAIL_OFF:
+        assert st.tail_launches == 0 and st.tail_rays == 0
+    else:
+        assert st_mix.tail_launches >= 1 and 0 < st_mix.tail_rays < st_mix.extension_rays
     srv.close()
 
 
