@@ -5,16 +5,22 @@
 //   1. triangle boxes and centroids; centroid bounds
 //   2. 63-bit Morton codes of the centroids, radix-sorted with the triangle
 //      index as payload (ties keep index order)
-//   3. a binary BVH, top-down and level-synchronous: each node covers a
-//      contiguous range of the sorted order and is split at the best of up to
-//      32 equal-count bins along that order by the surface area heuristic
-//      (bin boxes by per-chunk reduction + ordered-integer atomics; left and
-//      right prefix areas and costs summed in FP64); ranges of <= 3
-//      triangles are leaves; from level 40 on, splits are at the median
-//   4. collapse to BVH8 level by level: a node opens its largest-area
-//      internal child until it has 8 children, children take octant slots as
-//      in the host builder, and node / triangle offsets come from exclusive
-//      scans, so the result is deterministic.
+//   3. a binary BVH, bottom-up, by parallel locally-ordered clustering
+//      (PLOC: Meister and Bittner, "Parallel Locally-Ordered Clustering for
+//      Bounding Volume Hierarchy Construction", IEEE TVCG 2018): the clusters
+//      (at first one per triangle, in Morton order) each find the neighbour
+//      within kRadius positions whose union box has the smallest surface
+//      area; mutual nearest neighbours merge into a new node, which takes the
+//      lower position; the array is compacted by a scan, and this repeats
+//      until one cluster is left. Ties are broken by the distance in the
+//      array, then by the parity of the lower position, then by the index,
+//      so every round merges at least one pair and the tree is deterministic
+//      (equal boxes pair up as (0,1), (2,3), ...).
+//   4. collapse to BVH8 level by level: a subtree of <= 3 triangles is a
+//      leaf unless splitting it is cheaper by the SAH; a node opens its largest-area internal child until it has 8
+//      children, children take octant slots as in the host builder, and node
+//      / triangle offsets come from exclusive scans, so the result is
+//      deterministic.
 // The output has the host builder's format (Bvh8Node, leaf-ordered triangles,
 // boxes padded outward by one ulp), so the traversal, the quantizer and the
 // oracle's restatement apply unchanged.
@@ -32,10 +38,8 @@
 namespace rt {
 namespace {
 
-constexpr uint32_t kBins = 32;
 constexpr uint32_t kLeafMax = 3;
-constexpr uint32_t kMedianLevel = 40;
-constexpr uint32_t kChunk = 32;
+constexpr int kRadius = 32;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t ord_key(float f) {
@@ -111,153 +115,104 @@ __global__ void k_morton(const float4* __restrict__ cent, uint32_t n, const Box6
     ids[i] = i;
 }
 
-// ---- binary build ----------------------------------------------------------
+// ---- binary build (PLOC) -------------------------------------------------------
 struct BNode {
-    uint32_t first, count;
-    uint32_t left;      // first of two consecutive children, kNone for a leaf
-    uint32_t pad;
+    uint32_t left, right;   // children; left = kNone for a triangle
+    uint32_t count;         // triangles below
+    uint32_t tri;           // triangle index (leaves)
     float lo[3], hi[3];
 };
 
-__device__ __forceinline__ uint32_t nbins(uint32_t count) { return count < kBins ? count : kBins; }
-
-__device__ __forceinline__ uint32_t bin_of(uint32_t off, uint32_t count) {
-    return (uint32_t)(((uint64_t)off * nbins(count)) / count);
+__global__ void k_leaves(const float* __restrict__ wtri, const uint32_t* __restrict__ ids, uint32_t n,
+                         BNode* __restrict__ nodes, uint32_t* __restrict__ cl) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    BNode b;
+    b.left = b.right = kNone;
+    b.count = 1;
+    b.tri = ids[k];
+    tri_box(wtri + 9ull * b.tri, b.lo, b.hi);
+    nodes[k] = b;
+    cl[k] = k;
 }
 
-__global__ void k_bins_reset(Box6* bins, uint32_t nact) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nact * kBins) return;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { bins[i].lo[a] = 0xFFFFFFFFu; bins[i].hi[a] = 0u; }
-}
-
-// Each thread folds kChunk consecutive sorted positions into running boxes,
-// flushing with ordered-integer atomics when the (node, bin) changes.
-__global__ void k_bins_accum(const float* __restrict__ wtri, const uint32_t* __restrict__ ids,
-                             const uint32_t* __restrict__ pos_node, const uint32_t* __restrict__ node_slot,
-                             const BNode* __restrict__ nodes, uint32_t n, Box6* bins) {
-    const uint32_t c0 = (blockIdx.x * blockDim.x + threadIdx.x) * kChunk;
-    if (c0 >= n) return;
-    const uint32_t c1 = min(c0 + kChunk, n);
-    uint32_t cur = kNone;
-    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    auto flush = [&]() {
-        if (cur == kNone) return;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&bins[cur].lo[a], ord_key(lo[a]));
-            atomicMax(&bins[cur].hi[a], ord_key(hi[a]));
-        }
-    };
-    for (uint32_t p = c0; p < c1; ++p) {
-        const uint32_t nd = pos_node[p];
-        if (nd == kNone) continue;
-        const BNode& b = nodes[nd];
-        const uint32_t key = node_slot[nd] * kBins + bin_of(p - b.first, b.count);
-        float tl[3], th[3];
-        tri_box(wtri + 9ull * ids[p], tl, th);
-        if (key != cur) {
-            flush();
-            cur = key;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) { lo[a] = tl[a]; hi[a] = th[a]; }
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) { lo[a] = fminf(lo[a], tl[a]); hi[a] = fmaxf(hi[a], th[a]); }
-        }
-    }
-    flush();
-}
-
-__device__ __forceinline__ double half_area(const float* lo, const float* hi) {
-    const double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+__device__ __forceinline__ float half_area_f(float x0, float y0, float z0, float x1, float y1, float z1) {
+    const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
     return dx * dy + dy * dz + dz * dx;
 }
 
-// One thread per active node: node box, leaf or split position (SAH over the
-// bins, prefix / suffix areas in FP64), and whether it has children.
-__global__ void k_split(BNode* nodes, const uint32_t* __restrict__ act, uint32_t nact, const Box6* __restrict__ bins,
-                        uint32_t level, uint32_t* __restrict__ split_pos, uint32_t* __restrict__ has_kids) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nact) return;
-    BNode& b = nodes[act[s]];
-    const uint32_t K = nbins(b.count);
-    float blo[kBins][3], bhi[kBins][3];
-    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t k = 0; k < K; ++k) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            blo[k][a] = key_ord(bins[s * kBins + k].lo[a]);
-            bhi[k][a] = key_ord(bins[s * kBins + k].hi[a]);
-            nlo[a] = fminf(nlo[a], blo[k][a]);
-            nhi[a] = fmaxf(nhi[a], bhi[k][a]);
+// Nearest neighbour of every cluster within kRadius array positions: the
+// smallest key (union half-area bits, |i - j|, parity of min(i, j), j). The
+// block stages the boxes of its 256 clusters and kRadius on each side in LDS.
+__global__ __launch_bounds__(256) void k_nn(const BNode* __restrict__ nodes, const uint32_t* __restrict__ cl,
+                                            uint32_t N, uint32_t* __restrict__ nn) {
+    __shared__ float sb[6][256 + 2 * kRadius];
+    const int b0 = (int)(blockIdx.x * 256u) - kRadius;
+    for (int t = threadIdx.x; t < 256 + 2 * kRadius; t += 256) {
+        const int j = b0 + t;
+        if (j >= 0 && j < (int)N) {
+            const BNode& x = nodes[cl[j]];
+            sb[0][t] = x.lo[0]; sb[1][t] = x.lo[1]; sb[2][t] = x.lo[2];
+            sb[3][t] = x.hi[0]; sb[4][t] = x.hi[1]; sb[5][t] = x.hi[2];
         }
     }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) { b.lo[a] = nlo[a]; b.hi[a] = nhi[a]; }
-    if (b.count <= kLeafMax) {
-        b.left = kNone;
-        has_kids[s] = 0;
-        split_pos[s] = 0;
-        return;
+    __syncthreads();
+    const int i = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (i >= (int)N) return;
+    const int ti = i - b0;
+    const float lx = sb[0][ti], ly = sb[1][ti], lz = sb[2][ti], hx = sb[3][ti], hy = sb[4][ti], hz = sb[5][ti];
+    uint64_t best = ~0ull;
+    const int j0 = max(0, i - kRadius), j1 = min((int)N - 1, i + kRadius);
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const int tj = j - b0;
+        const float a = half_area_f(fminf(lx, sb[0][tj]), fminf(ly, sb[1][tj]), fminf(lz, sb[2][tj]),
+                                    fmaxf(hx, sb[3][tj]), fmaxf(hy, sb[4][tj]), fmaxf(hz, sb[5][tj]));
+        const uint32_t d = (uint32_t)abs(i - j);
+        const uint32_t par = (uint32_t)min(i, j) & 1u;
+        const uint64_t key = ((uint64_t)__float_as_uint(a) << 32) | ((uint64_t)d << 25) | ((uint64_t)par << 24) |
+                             (uint64_t)j;
+        best = key < best ? key : best;
     }
-    uint32_t nl_best = b.count / 2;
-    if (level < kMedianLevel) {
-        // suffix areas of bins k..K-1
-        double rarea[kBins];
-        float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int k = (int)K - 1; k >= 1; --k) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) { rlo[a] = fminf(rlo[a], blo[k][a]); rhi[a] = fmaxf(rhi[a], bhi[k][a]); }
-            rarea[k] = half_area(rlo, rhi);
-        }
-        float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        double best = INFINITY;
-        for (uint32_t k = 0; k + 1 < K; ++k) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) { llo[a] = fminf(llo[a], blo[k][a]); lhi[a] = fmaxf(lhi[a], bhi[k][a]); }
-            // positions in bins 0..k (bin_of: floor(off * K / count) <= k)
-            const uint32_t nl = (uint32_t)(((uint64_t)(k + 1) * b.count + K - 1) / K);
-            const double cost = half_area(llo, lhi) * (double)nl + rarea[k + 1] * (double)(b.count - nl);
-            if (cost < best) { best = cost; nl_best = nl; }
-        }
-    }
-    split_pos[s] = b.first + nl_best;
-    has_kids[s] = 1;
+    nn[i] = (uint32_t)(best & 0xFFFFFFu);
 }
 
-__global__ void k_make_kids(BNode* nodes, const uint32_t* __restrict__ act, uint32_t nact,
-                            const uint32_t* __restrict__ split_pos, const uint32_t* __restrict__ has_kids,
-                            const uint32_t* __restrict__ scan, uint32_t node_base, uint32_t* __restrict__ next_act,
-                            uint32_t* __restrict__ node_slot) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nact || !has_kids[s]) return;
-    BNode& b = nodes[act[s]];
-    const uint32_t l = node_base + 2 * scan[s];
-    b.left = l;
-    BNode L{}, R{};
-    L.first = b.first;
-    L.count = split_pos[s] - b.first;
-    R.first = split_pos[s];
-    R.count = b.first + b.count - split_pos[s];
-    L.left = R.left = kNone;
-    nodes[l] = L;
-    nodes[l + 1] = R;
-    next_act[2 * scan[s]] = l;
-    next_act[2 * scan[s] + 1] = l + 1;
-    node_slot[l] = 2 * scan[s];
-    node_slot[l + 1] = 2 * scan[s] + 1;
+// flags[i] = (leads a merge) << 32 | (keeps a position)
+__global__ void k_flags(const uint32_t* __restrict__ nn, uint32_t N, uint64_t* __restrict__ flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N) return;
+    if (i == N) { flags[N] = 0; return; }
+    const uint32_t j = nn[i];
+    const bool mutual = nn[j] == i;
+    const uint64_t lead = (mutual && i < j) ? 1ull : 0ull;
+    const uint64_t keep = (mutual && i > j) ? 0ull : 1ull;
+    flags[i] = (lead << 32) | keep;
 }
 
-__global__ void k_assign(const BNode* __restrict__ nodes, uint32_t* __restrict__ pos_node, uint32_t n) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t nd = pos_node[p];
-    if (nd == kNone) return;
-    const BNode& b = nodes[nd];
-    if (b.left == kNone) { pos_node[p] = kNone; return; }
-    pos_node[p] = (p < nodes[b.left + 1].first) ? b.left : b.left + 1;
+__global__ void k_emit(BNode* __restrict__ nodes, const uint32_t* __restrict__ cl, const uint32_t* __restrict__ nn,
+                       const uint64_t* __restrict__ flags, const uint64_t* __restrict__ scan, uint32_t N,
+                       uint32_t node_base, uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint64_t f = flags[i];
+    if (!(f & 1ull)) return;
+    const uint32_t pos = (uint32_t)(scan[i] & 0xFFFFFFFFull);
+    if (f >> 32) {
+        const uint32_t id = node_base + (uint32_t)(scan[i] >> 32);
+        const BNode& l = nodes[cl[i]];
+        const BNode& r = nodes[cl[nn[i]]];
+        BNode b;
+        b.left = cl[i];
+        b.right = cl[nn[i]];
+        b.count = l.count + r.count;
+        b.tri = kNone;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { b.lo[a] = fminf(l.lo[a], r.lo[a]); b.hi[a] = fmaxf(l.hi[a], r.hi[a]); }
+        nodes[id] = b;
+        out[pos] = id;
+    } else {
+        out[pos] = cl[i];
+    }
 }
 
 // ---- collapse to BVH8 ---------------------------------------------------------
@@ -270,27 +225,41 @@ __device__ __forceinline__ float area_f(const BNode& b) {
     return dx * dy + dy * dz + dz * dx;
 }
 
+// A subtree of at most kLeafMax triangles becomes one leaf slot when that is
+// no dearer than splitting it by the surface area heuristic (triangle test
+// and node visit cost 1 each), as in the host builder.
+__device__ __forceinline__ bool inner(const BNode* nodes, const BNode& x) {
+    if (x.left == kNone) return false;
+    if (x.count > kLeafMax) return true;
+    const float pa = area_f(x);
+    if (!(pa > 0.0f)) return false;
+    const BNode& l = nodes[x.left];
+    const BNode& r = nodes[x.right];
+    const float split = 1.0f + (area_f(l) * (float)l.count + area_f(r) * (float)r.count) / pa;
+    return (float)x.count > split;
+}
+
 // Children of one BVH8 node in slot order (kNone = empty slot).
 __device__ void collapse_children(const BNode* nodes, uint32_t bn, uint32_t* slot_child) {
     uint32_t ch[8];
     int nc = 0;
     const BNode& b = nodes[bn];
-    if (b.left == kNone) {
+    if (!inner(nodes, b)) {
         ch[nc++] = bn;   // a leaf root
     } else {
         ch[nc++] = b.left;
-        ch[nc++] = b.left + 1;
+        ch[nc++] = b.right;
         while (nc < 8) {
             int bi = -1;
             float ba = -1.0f;
             for (int c = 0; c < nc; ++c) {
                 const BNode& x = nodes[ch[c]];
-                if (x.left != kNone && area_f(x) > ba) { ba = area_f(x); bi = c; }
+                if (inner(nodes, x) && area_f(x) > ba) { ba = area_f(x); bi = c; }
             }
             if (bi < 0) break;
-            const uint32_t l = nodes[ch[bi]].left;
-            ch[bi] = l;
-            ch[nc++] = l + 1;
+            const BNode& o = nodes[ch[bi]];
+            ch[bi] = o.left;
+            ch[nc++] = o.right;
         }
     }
     // octant slots: greedy on the signed centroid offsets from the node centre
@@ -335,7 +304,7 @@ __global__ void k_collapse_count(const BNode* __restrict__ nodes, const Item* __
         kids[8 * i + s] = sc[s];
         if (sc[s] == kNone) continue;
         const BNode& x = nodes[sc[s]];
-        if (x.left != kNone) ni++;
+        if (inner(nodes, x)) ni++;
         else nt += x.count;
     }
     n_int[i] = ni;
@@ -345,7 +314,7 @@ __global__ void k_collapse_count(const BNode* __restrict__ nodes, const Item* __
 __global__ void k_collapse_write(const BNode* __restrict__ nodes, const Item* __restrict__ items, uint32_t nitems,
                                  const uint32_t* __restrict__ kids, const uint32_t* __restrict__ scan_int,
                                  const uint32_t* __restrict__ scan_tri, uint32_t node_base, uint32_t tri_base0,
-                                 const float* __restrict__ wtri, const uint32_t* __restrict__ ids,
+                                 const float* __restrict__ wtri,
                                  Bvh8Node* __restrict__ out, float4* __restrict__ out_tris, Item* __restrict__ next) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nitems) return;
@@ -369,20 +338,32 @@ __global__ void k_collapse_write(const BNode* __restrict__ nodes, const Item* __
         nd.hi_x[s] = nextafterf(x.hi[0], INFINITY);
         nd.hi_y[s] = nextafterf(x.hi[1], INFINITY);
         nd.hi_z[s] = nextafterf(x.hi[2], INFINITY);
-        if (x.left != kNone) {
+        if (inner(nodes, x)) {
             meta[s] = (uint8_t)(0x80u | rank);
             next[scan_int[i] + rank] = Item{c, child_base + rank};
             imask |= 1u << s;
             rank++;
         } else {
             meta[s] = (uint8_t)((x.count << 5) | toff);
-            for (uint32_t j = 0; j < x.count; ++j) {
-                const uint32_t t = ids[x.first + j];
+            // the subtree's triangles, left first (at most kLeafMax leaves, depth <= kLeafMax - 1)
+            uint32_t stk[kLeafMax + 1];
+            int sp = 0;
+            stk[sp++] = c;
+            uint32_t j = 0;
+            while (sp > 0) {
+                const BNode& y = nodes[stk[--sp]];
+                if (y.left != kNone) {
+                    stk[sp++] = y.right;
+                    stk[sp++] = y.left;
+                    continue;
+                }
+                const uint32_t t = y.tri;
                 const float* w = wtri + 9ull * t;
                 float4* o = out_tris + 3ull * (tri_base + toff + j);
                 o[0] = make_float4(w[0], w[1], w[2], __uint_as_float(t));
                 o[1] = make_float4(w[3], w[4], w[5], 0.0f);
                 o[2] = make_float4(w[6], w[7], w[8], 0.0f);
+                j++;
             }
             lmask |= 1u << s;
             toff += x.count;
@@ -460,65 +441,49 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
     DBuf<uint8_t> d_tmp;
     size_t scan_bytes = 0;
     {
-        uint32_t* a = nullptr;
-        D_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, a, a, (int)n + 2, st));
+        uint64_t* a = nullptr;
+        D_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, a, a, (int)n + 1, st));
     }
     D_TRY(d_tmp.alloc(std::max(tmp_bytes, scan_bytes)));
     D_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp.p, tmp_bytes, d_k0.p, d_k1.p, d_i0.p, d_i1.p, (int)n, 0, 63, st));
     const uint32_t* ids = d_i1.p;
 
-    // binary build
-    const uint32_t cap = 2 * n + 1;
+    // binary build: clustering rounds until one cluster is left
     DBuf<BNode> d_nodes;
-    DBuf<uint32_t> d_pos, d_slot, d_act0, d_act1, d_split, d_kids, d_scan;
-    DBuf<Box6> d_bins;
-    D_TRY(d_nodes.alloc(cap)); D_TRY(d_pos.alloc(n)); D_TRY(d_slot.alloc(cap));
-    D_TRY(d_act0.alloc(n + 2)); D_TRY(d_act1.alloc(n + 2)); D_TRY(d_split.alloc(n + 2)); D_TRY(d_kids.alloc(n + 2));
-    D_TRY(d_scan.alloc(n + 2));
-    {
-        BNode root{};
-        root.first = 0; root.count = n; root.left = kNone;
-        D_TRY(hipMemcpyAsync(d_nodes.p, &root, sizeof root, hipMemcpyHostToDevice, st));
-        D_TRY(hipMemsetAsync(d_pos.p, 0, 4ull * n, st));
-        D_TRY(hipMemsetAsync(d_slot.p, 0, 4, st));
-        D_TRY(hipMemsetAsync(d_act0.p, 0, 4, st));
-    }
-    uint32_t nact = 1, node_count = 1, level = 0;
-    uint32_t* act = d_act0.p;
-    uint32_t* nxt = d_act1.p;
-    size_t bins_cap = 0;
-    while (nact > 0) {
-        if ((size_t)nact * kBins > bins_cap) {
-            bins_cap = std::max((size_t)nact * kBins, (size_t)2 * bins_cap);
-            if (d_bins.p) { (void)hipStreamSynchronize(st); (void)hipFree(d_bins.p); d_bins.p = nullptr; }
-            D_TRY(d_bins.alloc(bins_cap));
-        }
-        hipLaunchKernelGGL(k_bins_reset, dim3(blocks((uint64_t)nact * kBins)), dim3(256), 0, st, d_bins.p, nact);
-        hipLaunchKernelGGL(k_bins_accum, dim3(blocks((n + kChunk - 1) / kChunk)), dim3(256), 0, st, d_w.p, ids,
-                           d_pos.p, d_slot.p, d_nodes.p, n, d_bins.p);
-        hipLaunchKernelGGL(k_split, dim3(blocks(nact, 64)), dim3(64), 0, st, d_nodes.p, act, nact, d_bins.p, level,
-                           d_split.p, d_kids.p);
+    DBuf<uint32_t> d_cl0, d_cl1, d_nn;
+    DBuf<uint64_t> d_flags, d_fscan;
+    D_TRY(d_nodes.alloc(2ull * n)); D_TRY(d_cl0.alloc(n)); D_TRY(d_cl1.alloc(n)); D_TRY(d_nn.alloc(n));
+    D_TRY(d_flags.alloc(n + 1ull)); D_TRY(d_fscan.alloc(n + 1ull));
+    hipLaunchKernelGGL(k_leaves, dim3(blocks(n)), dim3(256), 0, st, d_w.p, ids, n, d_nodes.p, d_cl0.p);
+    D_TRY(hipGetLastError());
+    uint32_t N = n, node_count = n, rounds = 0;
+    uint32_t* cl = d_cl0.p;
+    uint32_t* cl_out = d_cl1.p;
+    while (N > 1) {
+        hipLaunchKernelGGL(k_nn, dim3(blocks(N)), dim3(256), 0, st, d_nodes.p, cl, N, d_nn.p);
+        hipLaunchKernelGGL(k_flags, dim3(blocks(N + 1ull)), dim3(256), 0, st, d_nn.p, N, d_flags.p);
         D_TRY(hipGetLastError());
-        D_TRY(hipMemsetAsync(d_kids.p + nact, 0, 4, st));
-        D_TRY(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, scan_bytes, d_kids.p, d_scan.p, (int)nact + 1, st));
-        uint32_t tot = 0;
-        D_TRY(hipMemcpyAsync(&tot, d_scan.p + nact, 4, hipMemcpyDeviceToHost, st));
-        hipLaunchKernelGGL(k_make_kids, dim3(blocks(nact)), dim3(256), 0, st, d_nodes.p, act, nact, d_split.p,
-                           d_kids.p, d_scan.p, node_count, nxt, d_slot.p);
-        hipLaunchKernelGGL(k_assign, dim3(blocks(n)), dim3(256), 0, st, d_nodes.p, d_pos.p, n);
-        D_TRY(hipGetLastError());
+        D_TRY(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, scan_bytes, d_flags.p, d_fscan.p, (int)N + 1, st));
+        uint64_t tot = 0;
+        D_TRY(hipMemcpyAsync(&tot, d_fscan.p + N, 8, hipMemcpyDeviceToHost, st));
         D_TRY(hipStreamSynchronize(st));
-        node_count += 2 * tot;
-        nact = 2 * tot;
-        std::swap(act, nxt);
-        level++;
-        if (level > 4096 || node_count > cap) {
+        const uint32_t merges = (uint32_t)(tot >> 32), keep = (uint32_t)tot;
+        if (merges == 0 || keep + merges != N || node_count + merges > 2 * n - 1 || ++rounds > 1000000) {
             err = "device BVH build did not converge";
             return false;
         }
+        hipLaunchKernelGGL(k_emit, dim3(blocks(N)), dim3(256), 0, st, d_nodes.p, cl, d_nn.p, d_flags.p, d_fscan.p, N,
+                           node_count, cl_out);
+        D_TRY(hipGetLastError());
+        node_count += merges;
+        N = keep;
+        std::swap(cl, cl_out);
     }
+    uint32_t root = 0;
+    D_TRY(hipMemcpyAsync(&root, cl, 4, hipMemcpyDeviceToHost, st));
+    D_TRY(hipStreamSynchronize(st));
     // collapse
-    const uint32_t n_internal = (node_count - 1) / 2;
+    const uint32_t n_internal = n - 1;
     std::vector<Bvh8Node> nodes_h;
     DBuf<Bvh8Node> d_out;
     DBuf<float4> d_otri;
@@ -530,8 +495,8 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
     D_TRY(d_ck.alloc(8ull * out_cap)); D_TRY(d_ni.alloc(out_cap + 1)); D_TRY(d_nt.alloc(out_cap + 1));
     D_TRY(d_si.alloc(out_cap + 1)); D_TRY(d_stri.alloc(out_cap + 1));
     {
-        Item root{0, 0};
-        D_TRY(hipMemcpyAsync(d_it0.p, &root, sizeof root, hipMemcpyHostToDevice, st));
+        Item r0{root, 0};
+        D_TRY(hipMemcpyAsync(d_it0.p, &r0, sizeof r0, hipMemcpyHostToDevice, st));
     }
     uint32_t nitems = 1, out_nodes = 1, out_tris = 0, depth = 0;
     Item* items = d_it0.p;
@@ -561,7 +526,7 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
             return false;
         }
         hipLaunchKernelGGL(k_collapse_write, dim3(blocks(nitems, 64)), dim3(64), 0, st, d_nodes.p, items, nitems,
-                           d_ck.p, d_si.p, d_stri.p, out_nodes, out_tris, d_w.p, ids, d_out.p, d_otri.p, nitems_buf);
+                           d_ck.p, d_si.p, d_stri.p, out_nodes, out_tris, d_w.p, d_out.p, d_otri.p, nitems_buf);
         D_TRY(hipGetLastError());
         out_nodes += tot[0];
         out_tris += tot[1];
