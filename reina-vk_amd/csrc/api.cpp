@@ -10,6 +10,7 @@
 //                        samples, then finish (running mean into the image).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -171,6 +172,7 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     size_t p_cdf = add(s->tri_cdf.data(), s->tri_cdf.size() * 4);
     size_t p_ie = add(s->inst_emi.data(), s->inst_emi.size() * 4);
     size_t p_tf = add(s->tri_flags.data(), s->tri_flags.size());
+    size_t p_rec = add(nullptr, tri_inst.size() * (uint64_t)kShadeRecF4 * 16);   // filled on the device
     (void)wtri;
     HIP_TRY(hipSetDevice(s->device));
     hipError_t e = hipMalloc(&s->arena, off);
@@ -203,6 +205,10 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     s->sv.tri_flags = (const uint8_t*)(a + parts[p_tf].off);
     s->sv.num_emi = (uint32_t)s->emi.size();
     s->sv.num_inst = (uint32_t)inst.size();
+    float4* rec = (float4*)(a + parts[p_rec].off);
+    HIP_TRY(launch_shade_records(s->sv, rec, (uint32_t)tri_inst.size(), nullptr));
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    s->sv.srec = rec;
     return REINA_RT_OK;
 }
 
@@ -327,6 +333,8 @@ int reina_rt_scene_destroy(ReinaScene* s) {
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->tev0) hipEventDestroy(s->tev0);
     if (s->tev1) hipEventDestroy(s->tev1);
+    for (int r = 0; r < 2; ++r)
+        if (s->cev[r]) hipEventDestroy(s->cev[r]);
     for (hipEvent_t e : s->xev) hipEventDestroy(e);
     if (s->arena) hipFree(s->arena);
     delete s;
@@ -416,16 +424,19 @@ int reina_rt_bvh_quantize_host(const void* nodes, uint64_t nodes_bytes, void* qn
 }
 
 static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
-    if (!s->ctrl) {
+    if (!s->trace_ready) {
         // 3 iteration slots, the stats words, and the claim counters of reina_rt_intersect
-        HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 256 + kParts * kLine * 4));
+        if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 256 + kParts * kLine * 4));
         s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
         s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 256);
-        HIP_TRY(hipHostMalloc((void**)&s->h_count, sizeof(Ctrl), hipHostMallocDefault));
-        HIP_TRY(hipEventCreate(&s->ev0));
-        HIP_TRY(hipEventCreate(&s->ev1));
-        HIP_TRY(hipEventCreate(&s->tev0));
-        HIP_TRY(hipEventCreate(&s->tev1));
+        if (!s->h_count) HIP_TRY(hipHostMalloc((void**)&s->h_count, 2 * sizeof(Ctrl::count), hipHostMallocDefault));
+        if (!s->ev0) HIP_TRY(hipEventCreate(&s->ev0));
+        if (!s->ev1) HIP_TRY(hipEventCreate(&s->ev1));
+        if (!s->tev0) HIP_TRY(hipEventCreate(&s->tev0));
+        if (!s->tev1) HIP_TRY(hipEventCreate(&s->tev1));
+        for (int r = 0; r < 2; ++r)
+            if (!s->cev[r]) HIP_TRY(hipEventCreateWithFlags(&s->cev[r], hipEventDisableTiming));
+        s->trace_ready = true;
     }
     if (npx <= s->cap_px) return REINA_RT_OK;
     free_trace_buffers(s);
@@ -465,34 +476,36 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     if (nee && s->emi.empty())
         return fail(REINA_RT_ERR_NO_EMISSIVE, "next-event estimation needs at least one emissive instance");
     P.nee = nee ? 1u : 0u;
-    HIP_TRY(hipEventRecord(s->ev0, st));
-    s->Q.cap = part_capacity(W * H);
-    HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, st));
-    uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
-    uint32_t it = 0;
-    const uint32_t K = 4;
-    uint64_t live = max_iters ? (uint64_t)W * H : 0;
-    s->tail_used = false;
     const bool timex = (s->options & REINA_RT_TRACE_TIME_EXTEND) != 0;
     const bool count = (s->options & REINA_RT_TRACE_COUNT) != 0;
+    const uint32_t npx = W * H;
+    s->Q.cap = part_capacity(npx);
+    HIP_TRY(hipEventRecord(s->ev0, st));
+    HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, 0, npx, st));
+    const uint64_t max_iters = (uint64_t)P.spp * P.maxBounces;
+    uint64_t live = max_iters ? npx : 0;
+    bool pending[2] = {false, false};
+    uint32_t it = 0, rb = 0;
+    const uint32_t K = 4;   // iterations between queue-length readbacks
+    s->tail_count = 0;
     s->xev_used = 0;
-    while (it < max_iters) {
-        uint32_t slot = it % 3, in_q = it & 1u;
+    while (it < max_iters && live != 0) {
+        const uint32_t slot = it % 3, in_q = it & 1u;
         if (live <= s->tune.tail_paths) {
             // few paths left: finish them in one launch
             HIP_TRY(hipEventRecord(s->tev0, st));
             HIP_TRY(launch_tail(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->stats, s->tune, (uint32_t)live, st));
             HIP_TRY(hipEventRecord(s->tev1, st));
-            s->tail_used = true;
+            s->tail_count = 1;
             break;
         }
         if (timex) {
             if (s->xev.size() < 2 * (size_t)(s->xev_used + 1)) {
-                hipEvent_t a, b;
-                HIP_TRY(hipEventCreate(&a));
-                HIP_TRY(hipEventCreate(&b));
-                s->xev.push_back(a);
-                s->xev.push_back(b);
+                hipEvent_t e0, e1;
+                HIP_TRY(hipEventCreate(&e0));
+                HIP_TRY(hipEventCreate(&e1));
+                s->xev.push_back(e0);
+                s->xev.push_back(e1);
             }
             HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used], st));
         }
@@ -503,12 +516,23 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
         if (nee) HIP_TRY(launch_connect(s->sv, s->Q, s->S, s->ctrl, slot, s->stats, s->tune, st));
         ++it;
         if (it % K == 0) {
-            HIP_TRY(hipMemcpyAsync(s->h_count, &s->ctrl[it % 3].count[0][0], sizeof(Ctrl::count),
-                                   hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            live = 0;
-            for (uint32_t p = 0; p < kParts; ++p) live += s->h_count[p * kLine];
-            if (live == 0) break;
+            // The queue length after this iteration is copied back in stream
+            // order; the host reads the previous readback, so it waits only on
+            // work enqueued K iterations ago and the GPU keeps K iterations
+            // queued (no idle gap per readback). Decisions lag by K iterations.
+            const uint32_t r = rb & 1u;
+            HIP_TRY(hipMemcpyAsync(s->h_count + r * kParts * kLine, &s->ctrl[it % 3].count[0][0],
+                                   sizeof(Ctrl::count), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(s->cev[r], st));
+            pending[r] = true;
+            if (pending[r ^ 1u]) {
+                pending[r ^ 1u] = false;
+                HIP_TRY(hipEventSynchronize(s->cev[r ^ 1u]));
+                const uint32_t* h = s->h_count + (r ^ 1u) * kParts * kLine;
+                live = 0;
+                for (uint32_t p = 0; p < kParts; ++p) live += h[p * kLine];
+            }
+            rb++;
         }
     }
     HIP_TRY(launch_finish(P, s->S, (float4*)image, st));
@@ -544,8 +568,8 @@ int reina_rt_trace_stats(ReinaScene* s, ReinaTraceStats* o) {
     }
     o->extend_ms = xms;
     o->tail_rays = st3[4];
-    o->tail_launches = s->tail_used ? 1u : 0u;
-    if (s->tail_used) HIP_TRY(hipEventElapsedTime(&o->tail_ms, s->tev0, s->tev1));
+    o->tail_launches = s->tail_count;
+    if (s->tail_count) HIP_TRY(hipEventElapsedTime(&o->tail_ms, s->tev0, s->tev1));
     return REINA_RT_OK;
 }
 

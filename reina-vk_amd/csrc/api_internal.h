@@ -74,9 +74,10 @@ struct ReinaScene {
     rt::Ctrl* ctrl = nullptr;
     unsigned long long* stats = nullptr;
     uint32_t* fetch = nullptr;
-    uint32_t* h_count = nullptr;        // pinned
+    uint32_t* h_count = nullptr;        // pinned: [2 readbacks][partition * kLine] queue lengths
     rt::Tuning tune{16, 16, 0, 0, 0, 0};
-    bool tail_used = false;
+    hipEvent_t cev[2] = {};             // queue-length readbacks
+    uint32_t tail_count = 0;            // 1 if the last trace ended with a tail launch
     hipEvent_t tev0 = nullptr, tev1 = nullptr;   // around the tail launch
     uint32_t min_stack = 8;             // smallest legal stack depth for this BVH
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -85,6 +86,7 @@ struct ReinaScene {
     std::vector<hipEvent_t> xev;        // extend event pairs (REINA_RT_TRACE_TIME_EXTEND)
     uint32_t xev_used = 0;
     uint32_t last_iters = 0;
+    bool trace_ready = false;           // control words and events created
     hipStream_t last_stream = nullptr;
 };
 

@@ -30,9 +30,8 @@
 namespace rt {
 
 __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues Q, Ctrl* ctrl,
-                                              unsigned long long* stats) {
-    const uint32_t npx = P.W * P.H;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                              unsigned long long* stats, uint32_t base, uint32_t npx) {
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;   // index in the group
     const bool anyrays = P.spp > 0 && P.maxBounces > 0;
     if (blockIdx.x == 0 && threadIdx.x < 3 * kParts) {
         uint32_t slot = threadIdx.x / kParts, p = threadIdx.x % kParts;
@@ -45,9 +44,10 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         ctrl[slot].count[p][0] = c;
         ctrl[slot].fetch[p][0] = 0;
         ctrl[slot].sfetch[p][0] = 0;
-        if (threadIdx.x == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; stats[4] = 0; }
+        if (threadIdx.x == 0 && base == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; stats[4] = 0; }
     }
-    if (i >= npx) return;
+    if (l >= npx) return;
+    const uint32_t i = base + l;
     uint32_t x = i % P.W, y = i / P.W;
     uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
     if (!anyrays) {
@@ -63,8 +63,8 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
     S.st2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     if (P.nee) S.st4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t p = (i >> 10) & (kParts - 1u);
-    uint32_t j = p * Q.cap + (((i >> 13) << 10) | (i & 1023u));
+    uint32_t p = (l >> 10) & (kParts - 1u);
+    uint32_t j = p * Q.cap + (((l >> 13) << 10) | (l & 1023u));
     Q.ray[0][2 * j] = make_float4(o.x, o.y, o.z, 0.0f);
     Q.ray[0][2 * j + 1] = make_float4(d.x, d.y, d.z, 10000.0f);
     Q.pix[0][j] = i;
@@ -177,12 +177,26 @@ __global__ void k_debug_math(int fn, const float* in, const float* in2, float* o
 // ---- launch wrappers (host) ------------------------------------------------------
 namespace rt {
 
+__global__ __launch_bounds__(256) void k_shade_records(SceneView sv, float4* __restrict__ out, uint32_t n) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    float4 r[kShadeRecF4];
+    shade_record(sv, g, r);
+#pragma unroll
+    for (uint32_t k = 0; k < kShadeRecF4; ++k) out[(size_t)kShadeRecF4 * g + k] = r[k];
+}
+
+hipError_t launch_shade_records(const SceneView& sv, float4* out, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_records, dim3((n + 255) / 256), dim3(256), 0, st, sv, out, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
-                       unsigned long long* stats, hipStream_t st) {
-    uint32_t npx = P.W * P.H;
-    uint32_t blocks = (npx + 255) / 256;
+                       unsigned long long* stats, uint32_t base, uint32_t n, hipStream_t st) {
+    uint32_t blocks = (n + 255) / 256;
     if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, st, P, S, Q, ctrl, stats);
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, st, P, S, Q, ctrl, stats, base, n);
     return hipGetLastError();
 }
 

@@ -120,6 +120,12 @@ struct SceneView {
     const uint8_t* tri_flags;
     uint32_t num_emi;
     uint32_t num_inst;
+    // shading record of flattened triangle g, kShadeRecF4 float4 at srec + kShadeRecF4 * g:
+    // (v0.xyz, instance), (v1.xyz, primitive), (v2.xyz, properties index), then T, B, N of
+    // vertex 0, 1, 2 (27 floats, 1 pad): the object-space values hit_info reads, gathered
+    // once per scene
+    const float4* srec;
 };
+constexpr uint32_t kShadeRecF4 = 10;   // 160 bytes
 
 }  // namespace rt

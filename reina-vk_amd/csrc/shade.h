@@ -43,29 +43,56 @@ RT_D V3 tbn_col(const SceneView& sv, uint32_t pos, int col) {
     return ldv3(sv.tbns + 9ull * ti + 3 * col);
 }
 
-// closestHitCommon.h.glsl:52-148
+// The shading record of flattened triangle g (SceneView::srec): the same
+// loads as closestHitCommon.h.glsl:52-148 makes through the instance's index
+// and TBN offsets, done once per scene.
+RT_D void shade_record(const SceneView& sv, uint32_t g, float4* r) {
+    const uint32_t ii = sv.tri_inst[g], prim = sv.tri_prim[g];
+    const InstanceRec in = sv.inst[ii];
+    const ReinaInstanceProperties& p = sv.props[in.props_id];
+    const uint32_t io = p.indicesOffset, to = p.tbnsIndicesOffset;
+    float f[28];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float4 v = sv.vertices[sv.indices[3u * prim + io + k]];
+        r[k] = make_float4(v.x, v.y, v.z, __uint_as_float(k == 0 ? ii : (k == 1 ? prim : in.props_id)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const V3 col = tbn_col(sv, 3u * prim + to + k, c);
+            f[9 * k + 3 * c + 0] = col.x;
+            f[9 * k + 3 * c + 1] = col.y;
+            f[9 * k + 3 * c + 2] = col.z;
+        }
+    }
+    f[27] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) r[3 + q] = make_float4(f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3]);
+}
+
+// closestHitCommon.h.glsl:52-148, from the triangle's shading record
 RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaInstanceProperties& p,
-                      uint32_t prim, float bu, float bv, V3 D) {
+                      const float4* rec, uint32_t prim, float bu, float bv, V3 D) {
     HitInfo r;
-    uint32_t io = p.indicesOffset;
-    uint32_t i0 = sv.indices[3u * prim + io + 0];
-    uint32_t i1 = sv.indices[3u * prim + io + 1];
-    uint32_t i2 = sv.indices[3u * prim + io + 2];
-    float4 a = sv.vertices[i0], b = sv.vertices[i1], c = sv.vertices[i2];
-    V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+    float4 q[kShadeRecF4];
+#pragma unroll
+    for (int k = 0; k < (int)kShadeRecF4; ++k) q[k] = rec[k];
+    float tb[28];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { tb[4 * k] = q[3 + k].x; tb[4 * k + 1] = q[3 + k].y; tb[4 * k + 2] = q[3 + k].z; tb[4 * k + 3] = q[3 + k].w; }
+    // T, B, N of vertex k: tb[9k + 0..2], tb[9k + 3..5], tb[9k + 6..8]
+    V3 v0 = v3(q[0].x, q[0].y, q[0].z), v1 = v3(q[1].x, q[1].y, q[1].z), v2 = v3(q[2].x, q[2].y, q[2].z);
     float bx = (1.0f - bu) - bv, by = bu, bz = bv;
     V3 op = add(add(scale(v0, bx), scale(v1, by)), scale(v2, bz));
     V3 c0 = ldv3(in.c0), c1 = ldv3(in.c1), c2 = ldv3(in.c2), c3 = ldv3(in.c3);
     r.worldPosition = add(add(add(scale(c0, op.x), scale(c1, op.y)), scale(c2, op.z)), c3);
     V3 ong = normalize(cross(sub(v1, v0), sub(v2, v0)));
     V3 on;
-    uint32_t to = p.tbnsIndicesOffset;
     if (!p.interpNormals) {
         on = ong;
     } else {
-        V3 n0 = normalize(tbn_col(sv, 3u * prim + to + 0, 2));
-        V3 n1 = normalize(tbn_col(sv, 3u * prim + to + 1, 2));
-        V3 n2 = normalize(tbn_col(sv, 3u * prim + to + 2, 2));
+        V3 n0 = normalize(v3(tb[6], tb[7], tb[8]));
+        V3 n1 = normalize(v3(tb[15], tb[16], tb[17]));
+        V3 n2 = normalize(v3(tb[24], tb[25], tb[26]));
         on = normalize(add(add(scale(n0, bx), scale(n1, by)), scale(n2, bz)));
     }
     if (p.texIndicesOffset == 0xFFFFFFFFu) {
@@ -95,9 +122,9 @@ RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaIns
     const float w3[3] = {bx, by, bz};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        V3 tk = tbn_col(sv, 3u * prim + to + k, 0);
-        V3 bk = tbn_col(sv, 3u * prim + to + k, 1);
-        V3 nk = tbn_col(sv, 3u * prim + to + k, 2);
+        V3 tk = v3(tb[9 * k], tb[9 * k + 1], tb[9 * k + 2]);
+        V3 bk = v3(tb[9 * k + 3], tb[9 * k + 4], tb[9 * k + 5]);
+        V3 nk = v3(tb[9 * k + 6], tb[9 * k + 7], tb[9 * k + 8]);
         if (k == 0) { T = scale(tk, w3[0]); B = scale(bk, w3[0]); N = scale(nk, w3[0]); }
         else { T = add(T, scale(tk, w3[k])); B = add(B, scale(bk, w3[k])); N = add(N, scale(nk, w3[k])); }
     }

@@ -112,9 +112,11 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         pld.rayHitSky = true;
         pld.skip = false;
     } else {
-        const InstanceRec in = sv.inst[sv.tri_inst[prim]];
-        const ReinaInstanceProperties& pr = sv.props[in.props_id];
-        HitInfo hi = hit_info(sv, in, pr, sv.tri_prim[prim], hh.z, hh.w, D);
+        const float4* rec = sv.srec + (size_t)kShadeRecF4 * prim;
+        const uint32_t ii = __float_as_uint(rec[0].w), pm = __float_as_uint(rec[1].w);
+        const InstanceRec in = sv.inst[ii];
+        const ReinaInstanceProperties& pr = sv.props[__float_as_uint(rec[2].w)];
+        HitInfo hi = hit_info(sv, in, pr, rec, pm, hh.z, hh.w, D);
         switch (in.material) {
             case 0: chs_lambert_metal(pld, hi, pr, D, false); break;
             case 1: chs_lambert_metal(pld, hi, pr, D, true); break;
@@ -126,10 +128,10 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
             nee_n = hi.worldNormal;
             nee_albedo = ldv3(pr.albedo);
             // an emitter reached by the continuation of a Lambertian bounce: MIS weight
-            const int32_t ek = sv.inst_emi[sv.tri_inst[prim]];
+            const int32_t ek = sv.inst_emi[ii];
             if (s4.w > 0.0f && ek >= 0) {
                 V3 w0, w1, w2;
-                world_triangle(sv, in, pr, sv.tri_prim[prim], w0, w1, w2);
+                world_triangle(sv, in, pr, pm, w0, w1, w2);
                 const V3 ng = normalize(cross(sub(w1, w0), sub(w2, w0)));
                 const V3 dx = sub(hi.worldPosition, v3(s4.x, s4.y, s4.z));
                 const float pl = light_pdf(sv.emi[ek].pdf_area, length(dx), ng, normalize(D), pr.cullBackface != 0u);

@@ -78,8 +78,12 @@ struct JobRec {
     uint32_t npx;       // W * H
 };
 
+// Fills the shading records (SceneView::srec layout) of flattened triangles 0..n-1.
+hipError_t launch_shade_records(const SceneView& sv, float4* out, uint32_t n, hipStream_t st);
+// Pixels [base, base + n) start in queue Q (partition (l >> 10) & 7 for
+// local index l = pixel - base).
 hipError_t launch_init(const TraceParams& P, const PathState& S, const Queues& Q, Ctrl* ctrl,
-                       unsigned long long* stats, hipStream_t st);
+                       unsigned long long* stats, uint32_t base, uint32_t n, hipStream_t st);
 // Launch parameters of the traversal kernel (chosen per scene; see the autotuner).
 struct Tuning {
     uint32_t stack_depth;   // LDS group-stack entries per lane: 8, 16, 24 or 32 (>= BVH depth - 1)

@@ -348,6 +348,28 @@ def test_render_partitions_bitexact(cuda, cornell_arrays, tail):
     dev.close()
 
 
+@pytest.mark.parametrize("tail", [0, 20000])
+def test_render_readbacks_bitexact(cuda, cornell_arrays, tail):
+    """320 x 256 pixels (10 queue tiles per partition), 4 spp, depth 8: up to
+    32 bounce iterations, so the host's pipelined queue-length readbacks (every
+    4 iterations, read one round late) decide the end of the frame and the
+    switch to the tail launch; the frame equals the oracle's bit for bit."""
+    torch = cuda
+    arrays, _ = cornell_arrays
+    W, H = 320, 256
+    pc = scenes.cornell_push(W, H, spp=4, max_bounces=8)
+    dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
+    got = gpu_render(torch, dev, pc, W, H)
+    ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    assert np.array_equal(_bits(got), _bits(ref)), "pixels differing: %d" % (
+        (_bits(got) != _bits(ref)).any(-1).sum())
+    st = dev.stats()
+    assert st.extension_rays == nrays
+    assert st.tail_launches == (1 if tail else 0)
+    dev.close()
+
+
 def test_running_mean_bitexact(cuda, cornell_arrays):
     """sampleBatch > 0 blends (prev * b + new) / (b + 1) (raytrace.rgen.glsl:277-282)."""
     torch = cuda
