@@ -14,7 +14,7 @@ iterations; the last few live paths (ReinaTuning.tail_paths) finish in one
 tail launch that is timed separately (tail_ms) and not part of the roofline.
 traffic: HBM bytes per launch of
 that kernel on this workload from the committed rocprofv3 PMC summary
-(profiles/k_trace_c2_1080p_64spp.md: FETCH_SIZE x 2 + WRITE_SIZE, scaled by a
+(profiles/k_trace_icosphere_1080p_64spp.md: FETCH_SIZE x 2 + WRITE_SIZE, scaled by a
 256 MiB copy in the same runs), or null for other sizes. Algorithmic bytes per ray =
 32 (ray in) + 16 (hit out) + 208 per visited node (useful bytes of the
 256-byte FP32 node: 8 x 24 B boxes + 8 B child bases + 8 B slot metadata)
@@ -159,7 +159,7 @@ def main():
     if not a.no_roofline:
         out["roofline"] = roofline(dev, frame, NODE_BYTES, torch)
         if (W, H, a.spp, a.depth) == (1920, 1080, 64, 8):
-            out["roofline"]["traffic"] = profiled_traffic("k_trace_c2_1080p_64spp.md")
+            out["roofline"]["traffic"] = profiled_traffic("k_trace_icosphere_1080p_64spp.md")
 
     if not a.no_secondary:
         out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch, NODE_BYTES)
