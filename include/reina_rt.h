@@ -340,7 +340,7 @@ typedef struct ReinaServerDesc {
     uint32_t quantized;          /* node format of every scene served: 0 FP32, 1 8-bit   */
     ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
                                     (stack_depth bounds the BVH depth of served scenes;
-                                    tail_paths 0 = 2^18, REINA_RT_TAIL_OFF = never)      */
+                                    tail_paths 0 = 2^19, REINA_RT_TAIL_OFF = never)      */
 } ReinaServerDesc;
 
 /* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */

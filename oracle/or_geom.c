@@ -301,11 +301,17 @@ static void brute32_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit) {
  * direction sign bits); then the node's internal hits become a group
  * {child_base, internal-slot mask, ordered hit mask} on a LIFO stack, popped
  * lowest ordered bit first. Box test: slabs with the near/far plane picked
- * by the direction sign and the interval widened by 2^-16 relative
- * (conservative: never rejects a box whose triangle passes). Rays with a
- * non-finite component or a zero direction miss everything. */
+ * by the direction sign; a plane distance is fmaf(plane, 1/d, -o/d -+ E)
+ * with E = 2^-16 |o/d| when every |1/d| <= 2^40 and o/d is finite, else
+ * (plane - o) * (1/d) with +-inf interval ends clamped to +-3e38; then the
+ * interval is widened by 2^-16 |t| (conservative: never rejects a box whose
+ * triangle passes). Rays with a non-finite component or a zero direction
+ * miss everything. */
 #define BOX_EPS 1.52587890625e-05f /* 2^-16 */
 #define SLAB_BIG 3.0e38f
+#define FMA_INV_MAX 1099511627776.0f /* 2^40 */
+
+static int finitef_(float x) { return (x == x) && (x - x == 0.0f); }
 
 typedef struct { uint32_t base, imask, om; } Group;
 
@@ -318,6 +324,13 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
     uint32_t nvis = 0, ntst = 0;
     /* inverse direction; a zero component gives +-inf */
     float ix = 1.0f / r.d.x, iy = 1.0f / r.d.y, iz = 1.0f / r.d.z;
+    /* slab constants -o/d -+ E, E = 2^-16 |o/d| */
+    float oix = -(r.o.x * ix), oiy = -(r.o.y * iy), oiz = -(r.o.z * iz);
+    float ex = fabsf(oix) * BOX_EPS, ey = fabsf(oiy) * BOX_EPS, ez = fabsf(oiz) * BOX_EPS;
+    float cnx = oix - ex, cny = oiy - ey, cnz = oiz - ez;
+    float cfx = oix + ex, cfy = oiy + ey, cfz = oiz + ez;
+    int fma_slabs = finitef_(oix) && finitef_(oiy) && finitef_(oiz) && fabsf(ix) <= FMA_INV_MAX &&
+                    fabsf(iy) <= FMA_INV_MAX && fabsf(iz) <= FMA_INV_MAX;
     uint32_t oct = (r.d.x < 0.0f ? 1u : 0u) | (r.d.y < 0.0f ? 2u : 0u) | (r.d.z < 0.0f ? 4u : 0u);
     Group stack[64];
     int sp = 0;
@@ -367,13 +380,20 @@ static void bvh_one(const OrScene* s, const ReinaRay* ray, ReinaHit* hit, uint32
             float nx = sx ? f[0 + k] : f[8 + k], fx = sx ? f[8 + k] : f[0 + k];
             float ny = sy ? f[16 + k] : f[24 + k], fy = sy ? f[24 + k] : f[16 + k];
             float nz = sz ? f[32 + k] : f[40 + k], fz = sz ? f[40 + k] : f[32 + k];
-            float tn = fmaxf(fmaxf((nx - r.o.x) * ix, (ny - r.o.y) * iy), (nz - r.o.z) * iz);
-            float tf = fminf(fminf((fx - r.o.x) * ix, (fy - r.o.y) * iy), (fz - r.o.z) * iz);
-            /* d = 0 on an axis: its distances are +-inf, or NaN (0 * inf) for
-             * a ray inside a plane, which fmaxf/fminf ignore (closed slab);
-             * +inf / -inf interval ends are clamped before the widening */
-            tn = fminf(tn, SLAB_BIG);
-            tf = fmaxf(tf, -SLAB_BIG);
+            float tn, tf;
+            if (fma_slabs) {
+                tn = fmaxf(fmaxf(fmaf(nx, ix, cnx), fmaf(ny, iy, cny)), fmaf(nz, iz, cnz));
+                tf = fminf(fminf(fmaf(fx, ix, cfx), fmaf(fy, iy, cfy)), fmaf(fz, iz, cfz));
+            } else {
+                /* d = 0 on an axis: its distances are +-inf, or NaN (0 * inf)
+                 * for a ray inside a plane, which fmaxf/fminf ignore (closed
+                 * slab); +inf / -inf interval ends are clamped before the
+                 * widening */
+                tn = fmaxf(fmaxf((nx - r.o.x) * ix, (ny - r.o.y) * iy), (nz - r.o.z) * iz);
+                tf = fminf(fminf((fx - r.o.x) * ix, (fy - r.o.y) * iy), (fz - r.o.z) * iz);
+                tn = fminf(tn, SLAB_BIG);
+                tf = fmaxf(tf, -SLAB_BIG);
+            }
             tn = fmaxf(fmaf(-fabsf(tn), BOX_EPS, tn), r.tmin);
             tf = fminf(fmaf(fabsf(tf), BOX_EPS, tf), tb);
             if (tn <= tf) hitm |= 1u << k;

@@ -2,8 +2,8 @@
 // traceRayEXT with gl_RayFlagsOpaqueEXT, shaders/raytrace/raytrace.rgen.glsl:110-122).
 //
 // One ray per lane. Per visited node the lane tests the 8 child boxes (slabs,
-// near/far plane chosen by the direction sign, interval widened by 2^-16
-// relative so no box holding a hit triangle is rejected), tests the
+// near/far plane chosen by the direction sign, interval widened by 2^-16 of
+// |t| and of |o/d| so no box holding a hit triangle is rejected), tests the
 // triangles of every hit leaf immediately (watertight test of Woop, Benthin
 // and Wald, JCGT 2013), and turns the hit internal children into one "group"
 // {child_base, internal-slot mask, ordered hit mask}. Children are visited in
@@ -23,10 +23,16 @@
 namespace rt {
 
 constexpr float kBoxEps = 1.52587890625e-05f;  // 2^-16
+constexpr float kFmaInvMax = 1099511627776.0f;  // 2^40
+constexpr float kSlabBig = 3.0e38f;
 
 struct TRay {
     float ox, oy, oz, dx, dy, dz, tmin, tmax;
     float ix, iy, iz;
+    // slab constants: plane distance = fma(plane, 1/d, c), c = -o/d - E (near
+    // planes) or -o/d + E (far planes), E = 2^-16 |o/d|
+    float cnx, cny, cnz, cfx, cfy, cfz;
+    bool fma_slabs;   // every |1/d| <= 2^40 and o/d finite; else the exact (plane - o) * (1/d) form
     float Sx, Sy, Sz;
     int kx, ky, kz;
     uint32_t oct;
@@ -55,8 +61,14 @@ RT_D void tray_init(TRay& r, float4 a, float4 b) {
     r.Sx = comp3(r.dx, r.dy, r.dz, kx) / dkz;
     r.Sy = comp3(r.dx, r.dy, r.dz, ky) / dkz;
     r.Sz = 1.0f / dkz;
-    // A zero component gives an infinite inverse (see slab_hit).
+    // A zero component gives an infinite inverse (see node_hits).
     r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
+    const float oix = -(r.ox * r.ix), oiy = -(r.oy * r.iy), oiz = -(r.oz * r.iz);
+    const float ex = fabsf(oix) * kBoxEps, ey = fabsf(oiy) * kBoxEps, ez = fabsf(oiz) * kBoxEps;
+    r.cnx = oix - ex; r.cny = oiy - ey; r.cnz = oiz - ez;
+    r.cfx = oix + ex; r.cfy = oiy + ey; r.cfz = oiz + ez;
+    r.fma_slabs = finite_(oix) && finite_(oiy) && finite_(oiz) && fabsf(r.ix) <= kFmaInvMax &&
+                  fabsf(r.iy) <= kFmaInvMax && fabsf(r.iz) <= kFmaInvMax;
     r.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
 }
 
@@ -104,26 +116,54 @@ struct HitRec {
     float u, v;
 };
 
-constexpr float kSlabBig = 3.0e38f;
-
-// Hit test of one widened slab interval. Per axis the plane distances are
-// (plane - o) * (1/d); with d = 0 they are +-inf, or NaN (0 * inf) for a ray
-// that runs inside a plane: fmaxf/fminf ignore a NaN operand, so such an
-// axis does not bound the interval (the slab is closed). An interval that
-// starts at +inf (or ends at -inf) is clamped to a finite value first so
-// that the widening cannot turn it into NaN, and it stays a miss.
+// Hit test of one slab interval [tn, tf]: widened by 2^-16 of |t| and
+// clipped to [tmin, tb].
+//
+// Why the box test never rejects a box whose triangle passes the watertight
+// test. FMA form (node_hits with fma_slabs): a plane distance is
+// fma(plane, 1/d, -o/d -+ E), whose error is at most 2^-24 |t| plus the
+// roundings of -o/d and of the E term, together under 2^-23 |o/d|; E =
+// 2^-16 |o/d| covers the latter, the 2^-16 |t| widening the former and the
+// relative error of the triangle test's t. Exact form (rays with a direction
+// component below 2^-40, or o/d not finite): (plane - o) * (1/d) has error
+// 2^-23 |t|; with d = 0 on an axis the distances are +-inf, or NaN (0 * inf)
+// for a ray inside a plane, which fmaxf/fminf ignore (closed slab), and an
+// interval end at +-inf is clamped to a finite value so the widening keeps it
+// a miss.
 RT_D bool slab_hit(float tn, float tf, float tmin, float tb) {
-    tn = fminf(tn, kSlabBig);
-    tf = fmaxf(tf, -kSlabBig);
     tn = fmaxf(__builtin_fmaf(-fabsf(tn), kBoxEps, tn), tmin);
     tf = fminf(__builtin_fmaf(fabsf(tf), kBoxEps, tf), tb);
     return tn <= tf;
 }
 
+// The 8 children's intervals from their near / far planes, both forms.
+RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float* NY, const float* FY,
+                    const float* NZ, const float* FZ, float tb) {
+    uint32_t hitm = 0;
+    if (r.fma_slabs) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float tn = fmaxf(fmaxf(__builtin_fmaf(NX[k], r.ix, r.cnx), __builtin_fmaf(NY[k], r.iy, r.cny)),
+                                   __builtin_fmaf(NZ[k], r.iz, r.cnz));
+            const float tf = fminf(fminf(__builtin_fmaf(FX[k], r.ix, r.cfx), __builtin_fmaf(FY[k], r.iy, r.cfy)),
+                                   __builtin_fmaf(FZ[k], r.iz, r.cfz));
+            hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
+            float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
+            tn = fminf(tn, kSlabBig);
+            tf = fmaxf(tf, -kSlabBig);
+            hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+        }
+    }
+    return hitm;
+}
+
 // Slab test of the 8 children; returns the hit mask in slot order (before
-// masking out empty slots). The interval is widened by 2^-16 of |t| with one
-// fused multiply-add each (conservative: never rejects a box whose triangle
-// passes the watertight test).
+// masking out empty slots); see slab_hit for the two forms.
 RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
     const float* base = reinterpret_cast<const float*>(node);
     // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
@@ -144,14 +184,7 @@ RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
         NZ[4 * h + 0] = e.x; NZ[4 * h + 1] = e.y; NZ[4 * h + 2] = e.z; NZ[4 * h + 3] = e.w;
         FZ[4 * h + 0] = f.x; FZ[4 * h + 1] = f.y; FZ[4 * h + 2] = f.z; FZ[4 * h + 3] = f.w;
     }
-    uint32_t hitm = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const float tn = fmaxf(fmaxf((NX[k] - r.ox) * r.ix, (NY[k] - r.oy) * r.iy), (NZ[k] - r.oz) * r.iz);
-        const float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
-        hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
-    }
-    return hitm;
+    return slab8(r, NX, FX, NY, FY, NZ, FZ, tb);
 }
 
 // Byte j of w as float times s plus p, one rounding (q * s is exact).
@@ -171,18 +204,15 @@ RT_D uint32_t node_hits_q8(const TRay& r, uint4 h0, uint4 qx, uint4 qy, uint4 qz
     const uint32_t nx0 = bx ? qx.x : qx.z, nx1 = bx ? qx.y : qx.w, fx0 = bx ? qx.z : qx.x, fx1 = bx ? qx.w : qx.y;
     const uint32_t ny0 = by ? qy.x : qy.z, ny1 = by ? qy.y : qy.w, fy0 = by ? qy.z : qy.x, fy1 = by ? qy.w : qy.y;
     const uint32_t nz0 = bz ? qz.x : qz.z, nz1 = bz ? qz.y : qz.w, fz0 = bz ? qz.z : qz.x, fz1 = bz ? qz.w : qz.y;
-    uint32_t hitm = 0;
+    float NX[8], FX[8], NY[8], FY[8], NZ[8], FZ[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int j = k & 3;
-        const float NX = dq8(k < 4 ? nx0 : nx1, j, sx, px), FX = dq8(k < 4 ? fx0 : fx1, j, sx, px);
-        const float NY = dq8(k < 4 ? ny0 : ny1, j, sy, py), FY = dq8(k < 4 ? fy0 : fy1, j, sy, py);
-        const float NZ = dq8(k < 4 ? nz0 : nz1, j, sz, pz), FZ = dq8(k < 4 ? fz0 : fz1, j, sz, pz);
-        const float tn = fmaxf(fmaxf((NX - r.ox) * r.ix, (NY - r.oy) * r.iy), (NZ - r.oz) * r.iz);
-        const float tf = fminf(fminf((FX - r.ox) * r.ix, (FY - r.oy) * r.iy), (FZ - r.oz) * r.iz);
-        hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+        NX[k] = dq8(k < 4 ? nx0 : nx1, j, sx, px); FX[k] = dq8(k < 4 ? fx0 : fx1, j, sx, px);
+        NY[k] = dq8(k < 4 ? ny0 : ny1, j, sy, py); FY[k] = dq8(k < 4 ? fy0 : fy1, j, sy, py);
+        NZ[k] = dq8(k < 4 ? nz0 : nz1, j, sz, pz); FZ[k] = dq8(k < 4 ? fz0 : fz1, j, sz, pz);
     }
-    return hitm;
+    return slab8(r, NX, FX, NY, FY, NZ, FZ, tb);
 }
 
 // Bit i of the result is bit (i ^ oct) of the 8-bit mask m.

@@ -86,7 +86,7 @@ def test_server_mix_equals_alone(cuda, pool, tail):
     partial last tile, spp, depths, accumulation with sampleBatch > 0)
     submitted together: each image equals its reina_rt_trace image and its
     image when served alone, whether the shared queues end with the tail
-    launch at once (default threshold 2^18 paths), never, or after some
+    launch at once (default threshold 2^19 paths), never, or after some
     graph iterations (3000 paths)."""
     torch = cuda
     rng = np.random.default_rng(3)
