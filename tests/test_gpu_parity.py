@@ -222,6 +222,29 @@ def test_intersect_zero_direction_components(cuda, flags):
     dev.close()
 
 
+@pytest.mark.parametrize("flags", [0, abi.NODES_QUANTIZED])
+@pytest.mark.parametrize("shift", [0.0, 3000.0])
+def test_intersect_near_axis_far_origin(cuda, flags, shift):
+    """Tiny direction components on either side of the switch between the
+    two slab forms (2^-41, 2^-39, 1e-9, 1e-6) and a mesh 3000 units from the
+    coordinate origin: GPU == brute force, counters == the oracle's traversal."""
+    from test_oracle_geometry import near_axis_rays, shifted_icosphere
+    torch = cuda
+    v, arrays = shifted_icosphere(shift)
+    dev = reina_rt.DeviceScene(arrays, flags=flags)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_bvh(*dev.export_bvh(), dev.info().node_bytes)
+    rays = near_axis_rays(v, 20000, 32)
+    hits, cnt = gpu_intersect(torch, dev, rays, counters=True)
+    ref = orc.brute32(rays)
+    ref_v, ref_c = orc.bvh(rays)
+    assert np.array_equal(hits["prim"], ref["prim"])
+    for fl in ("t", "u", "v"):
+        assert np.array_equal(_bits(hits[fl]), _bits(ref[fl])), fl
+    assert np.array_equal(cnt, ref_c)
+    dev.close()
+
+
 def test_intersect_vs_fp64(cuda, icosphere_arrays):
     """Hit id agrees with the FP64 oracle on >= 99.99% of rays; every
     disagreement is an FP64-detected edge/vertex/equal-t tie; t within 1e-6

@@ -245,6 +245,39 @@ def test_zero_direction_components():
     assert (hb["prim"] != 0xFFFFFFFF).mean() > 0.3
 
 
+def near_axis_rays(v, n, seed):
+    """Rays for both slab forms of the box test: origins near the mesh
+    vertices v (placed far from the coordinate origin, |o/d| is much larger
+    than the hit distance: the case the 2^-16 |o/d| term of the one-FMA form
+    covers), aimed at vertices; every other ray gets one direction component
+    of magnitude 2^-41 (below the 2^-40 switch: two-rounding form), 2^-39
+    (one-FMA form with a large inverse), 1e-9 or 1e-6."""
+    rng = np.random.default_rng(seed)
+    tiny = [2.0 ** -41, 2.0 ** -39, 1e-9, 1e-6]
+    vs = v.astype(np.float64)
+    o = vs[rng.integers(len(v), size=n)] + rng.normal(scale=0.3, size=(n, 3))
+    d = vs[rng.integers(len(v), size=n)] - o
+    for i in range(0, n, 2):
+        d[i, (i // 2) % 3] = tiny[(i // 6) % 4] * (1.0 if i % 4 == 0 else -1.0)
+    return rays_from(o.astype(np.float32), d.astype(np.float32))
+
+
+def shifted_icosphere(shift):
+    v, f = scenes.subdivided_icosphere(3)
+    v = (v.astype(np.float64) + shift).astype(np.float32)
+    return v, soup_arrays(v[f].reshape(-1, 9))
+
+
+@pytest.mark.parametrize("shift", [0.0, 3000.0])
+def test_near_axis_and_far_origin_rays(shift):
+    """Tiny direction components on either side of the 2^-40 switch between
+    the two slab forms, and a mesh 3000 units from the coordinate origin:
+    BVH traversal equals brute force bit for bit."""
+    v, arrays = shifted_icosphere(shift)
+    hb, cnt, _ = check_bvh_equals_brute(arrays, near_axis_rays(v, 20000, 31))
+    assert (hb["prim"] != 0xFFFFFFFF).mean() > 0.3
+
+
 def test_fp64_bvh_culling_equals_full_scan(icosphere_arrays):
     """The FP64 oracle with BVH culling returns exactly the full scan's hit,
     t, barycentrics and ambiguity flags (camera rays, random rays, and
