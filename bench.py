@@ -119,10 +119,10 @@ def main():
         sec, sh = build_secondary(a.secondary_reps, arrays, dev, cam, W, H, torch)
     autotune = None
     if not a.no_autotune:
-        autotune = run_autotune(dev, sec, sh, torch)
+        autotune = run_autotune(dev, sec, sh, torch, frame=lambda: frame(0))
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(os.path.join(ROOT, "gpurun_out", "autotune_table.json"), "w") as f:
-            json.dump(autotune.pop("table"), f)
+            json.dump({"intersect": autotune.pop("table"), "frame": autotune.pop("frame_table", None)}, f)
 
     log("timing %d frames" % a.steps)
     rays, elapsed, gpu_ms = time_frames(frame, a.warmup, a.steps, torch)
@@ -210,15 +210,39 @@ def profiled_traffic(name):
         pass
     return None
 
-def run_autotune(dev, sec, sh, torch):
-    from reina_rt.autotune import tune_traversal
+def run_autotune(dev, sec, sh, torch, frame=None):
+    """Stage 1 on the secondary batch; stage 2 (when `frame` is given) on
+    whole frames: the 3 fastest stage-1 settings and the defaults, each with
+    tail thresholds 2^18 / 2^19 / 2^20."""
+    from reina_rt.autotune import tune_frame, tune_traversal
     d0 = dev.get_tuning()
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     win, table = tune_traversal(dev, torch, sec.data_ptr(), sh.data_ptr(), sec.shape[0], cus=cus)
     dflt = next((r for r in table if r["stack_depth"] == d0["stack_depth"] and r["refill_min"] == d0["refill_min"]
                  and r["blocks_per_cu"] * cus == d0["grid_blocks"] and r["waves_per_simd"] == 0), None)
-    return {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
-            "secondary_batch_rays": int(sec.shape[0]), "table": table}
+    out = {"best": win, "default": dflt, "candidates": len(table), "worst_ms": max(r["ms"] for r in table),
+           "secondary_batch_rays": int(sec.shape[0]), "table": table}
+    if frame is not None:
+        keys = ("stack_depth", "refill_min", "waves_per_simd")
+        top = []
+        for r in sorted(table, key=lambda r: r["ms"]):
+            c = {k: r[k] for k in keys}
+            c["grid_blocks"] = r["blocks_per_cu"] * cus
+            if c not in top:
+                top.append(c)
+            if len(top) == 3:
+                break
+        base = {k: d0[k] for k in keys}
+        base["grid_blocks"] = d0["grid_blocks"]
+        if base not in top:
+            top.append(base)
+        cands = [dict(c, tail_paths=tp) for c in top for tp in (1 << 18, 1 << 19, 1 << 20)]
+        fwin, ftable = tune_frame(dev, torch, frame, cands)
+        out["frame_best"] = fwin
+        out["frame_candidates"] = len(ftable)
+        out["frame_worst_ms"] = max(r["ms"] for r in ftable)
+        out["frame_table"] = ftable
+    return out
 
 
 def time_frames(frame, warmup, steps, torch):
@@ -318,9 +342,11 @@ def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
     res = {"scene_setup_s": round(setup_s, 2), "node_bytes": int(devq.info().node_bytes),
            "bvh_node_MB": round(devq.info().num_nodes * devq.info().node_bytes / 1e6, 2)}
     if not a.no_autotune:
-        at = run_autotune(devq, sec, sh, torch)
+        at = run_autotune(devq, sec, sh, torch, frame=lambda: frame(0, devq))
         at.pop("table")
+        at.pop("frame_table", None)
         res["autotune_best"] = at["best"]
+        res["autotune_frame_best"] = at.get("frame_best")
     # bit-identity of one frame against the FP32 nodes
     frame(0, dev)
     ref = img.clone()

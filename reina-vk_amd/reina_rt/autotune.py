@@ -1,12 +1,19 @@
 """Autotuner for the traversal / shading launch parameters of one scene.
 
-It times reina_rt_intersect on a caller-supplied ray batch (device buffers)
-over a grid of (LDS stack depth, refill threshold, persistent blocks per CU,
-register budget in waves per SIMD)
-with HIP events on the given stream, and installs the fastest setting on the
-scene with reina_rt_set_tuning. The stack depth must cover the BVH depth, so
-candidates start at the scene's own default.
+Stage 1 (tune_traversal) times reina_rt_intersect on a caller-supplied ray
+batch (device buffers) over a grid of (LDS stack depth, refill threshold,
+persistent blocks per CU, register budget in waves per SIMD) with HIP events
+on the given stream, and installs the fastest setting on the scene with
+reina_rt_set_tuning. The stack depth must cover the BVH depth, so candidates
+start at the scene's own default.
+
+Stage 2 (tune_frame) times whole frames through reina_rt_trace for a few
+candidate settings (the best of stage 1, the defaults, each with several
+tail-launch thresholds) and installs the fastest: a frame's bounce launches
+shrink as pixels finish, so the best setting for one large batch is not
+always the best for a frame.
 """
+import time
 import itertools
 
 
@@ -37,4 +44,27 @@ def tune_traversal(dev, torch, rays_ptr, hits_ptr, n, stream=None, cus=None, rep
     win = min(table, key=lambda r: r["ms"])
     dev.set_tuning(stack_depth=win["stack_depth"], refill_min=win["refill_min"],
                    grid_blocks=cus * win["blocks_per_cu"], waves_per_simd=win["waves_per_simd"])
+    return win, table
+
+
+def tune_frame(dev, torch, frame, candidates, reps=2):
+    """frame() renders one frame on dev's current tuning (enqueued on the
+    current stream). Each candidate is a dict of reina_rt_set_tuning fields;
+    the fastest (min over reps of the synchronized wall time) is installed.
+    Returns (winner, table)."""
+    table = []
+    for c in candidates:
+        dev.set_tuning(**c)
+        frame()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            frame()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            best = ms if best is None else min(best, ms)
+        table.append(dict(c, ms=round(best, 3)))
+    win = min(table, key=lambda r: r["ms"])
+    dev.set_tuning(**{k: v for k, v in win.items() if k != "ms"})
     return win, table
