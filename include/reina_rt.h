@@ -329,6 +329,14 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  * samples; its image is then written. Scenes must stay alive until their
  * requests complete. submit / poll / wait may be called from any thread. */
 #define REINA_RT_SERVER_BYTES_PER_PIXEL 152u
+/* with REINA_RT_SERVER_NEE: + 16 B of light-sampling state and a 48 B shadow
+ * ray per pixel in flight */
+#define REINA_RT_SERVER_BYTES_PER_PIXEL_NEE 216u
+/* ReinaServerDesc.flags: hold next-event-estimation state, so requests on
+ * scenes whose trace options include REINA_RT_TRACE_NEE render with NEE + MIS
+ * (as reina_rt_trace does on that scene); without it such requests are
+ * refused with REINA_RT_ERR_UNSUPPORTED. */
+#define REINA_RT_SERVER_NEE 0x1u
 
 typedef struct ReinaServer ReinaServer;
 
@@ -341,6 +349,8 @@ typedef struct ReinaServerDesc {
     ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
                                     (stack_depth bounds the BVH depth of served scenes;
                                     tail_paths 0 = 2^19, REINA_RT_TAIL_OFF = never)      */
+    uint32_t flags;              /* REINA_RT_SERVER_NEE or 0                             */
+    uint32_t _pad0;
 } ReinaServerDesc;
 
 /* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */
@@ -378,7 +388,10 @@ int reina_rt_server_destroy(ReinaServer* server);
  * content is blended when pc->sampleBatch > 0). Returns immediately:
  * REINA_RT_ERR_BUSY when the FIFO is full, REINA_RT_ERR_OVER_BUDGET when the
  * request alone needs more slots than the server has, REINA_RT_ERR_UNSUPPORTED
- * when the scene's node format or BVH depth does not fit the server. */
+ * when the scene's node format or BVH depth does not fit the server or the
+ * scene's trace options ask for NEE on a server created without
+ * REINA_RT_SERVER_NEE, REINA_RT_ERR_NO_EMISSIVE when they ask for NEE and the
+ * scene has no emissive instance. */
 int reina_rt_submit(ReinaServer* server, ReinaScene* scene, const RtPushConsts* pc, float* image_rgba32f,
                     uint32_t width, uint32_t height, uint64_t* ticket);
 /* Status of a request; the record is released by the first call that

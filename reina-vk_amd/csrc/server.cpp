@@ -46,6 +46,7 @@ struct Request {
 struct ReinaServer {
     int device = 0;
     bool quantized = false;
+    bool nee = false;            // holds NEE state and shadow queues (REINA_RT_SERVER_NEE)
     Tuning tune{};
     uint32_t tiles = 0;          // 1024-pixel slot tiles (multiple of kParts)
     uint32_t max_pending = 0;
@@ -103,7 +104,10 @@ static hipError_t capture_graph(ReinaServer* s) {
         TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
         launch_trace_multi(s->quantized, s->Q.ray[in_q], s->Q.hit, tin, s->stats, s->tune, s->Q.pix[in_q],
                            s->jobs_d, s->st);
-        launch_shade_multi(s->jobs_d, s->S, s->Q, in_q, s->ctrl, slot, s->done_d, s->tune.shade_grid, s->st);
+        launch_shade_multi(s->jobs_d, s->S, s->Q, in_q, s->ctrl, slot, s->done_d, s->tune.shade_grid, s->nee, s->st);
+        if (s->nee)
+            launch_connect_multi(s->quantized, s->jobs_d, s->Q, s->S, s->ctrl, slot, s->Q.pix[in_q], s->stats, s->tune,
+                                 s->st);
     }
     e = hipStreamEndCapture(s->st, &g);
     if (e != hipSuccess) return e;
@@ -193,8 +197,8 @@ static void worker_main(ReinaServer* s) {
         for (uint32_t p = 0; p < kParts; ++p) live += s->count_h[p * kLine];
         if (s->tune.tail_paths != 0 && live != 0 && live <= s->tune.tail_paths) {
             const unsigned long long before = s->stats_h[0];
-            W_TRY(launch_tail_multi(s->quantized, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats, s->done_d, s->tune,
-                                    (uint32_t)live, s->st));
+            W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats, s->done_d,
+                                    s->tune, (uint32_t)live, s->st));
             W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
             W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
             W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
@@ -265,7 +269,10 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (d->struct_size != sizeof(ReinaServerDesc))
         return fail(REINA_RT_ERR_ABI, "ReinaServerDesc.struct_size mismatch (expected " +
                                           std::to_string(sizeof(ReinaServerDesc)) + ")");
-    const uint64_t per_tile = (uint64_t)REINA_RT_SERVER_BYTES_PER_PIXEL * 1024;
+    if (d->flags & ~REINA_RT_SERVER_NEE) return fail(REINA_RT_ERR_INVALID_ARG, "unknown server flag bits");
+    const bool nee = (d->flags & REINA_RT_SERVER_NEE) != 0;
+    const uint64_t bpp = nee ? REINA_RT_SERVER_BYTES_PER_PIXEL_NEE : REINA_RT_SERVER_BYTES_PER_PIXEL;
+    const uint64_t per_tile = bpp * 1024;
     uint64_t tiles = d->path_budget_bytes / per_tile;
     tiles -= tiles % kParts;
     if (tiles < kParts) return fail(REINA_RT_ERR_INVALID_ARG, "path_budget_bytes below 8 x 1024 pixels");
@@ -287,6 +294,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     ReinaServer* s = new ReinaServer();
     s->device = (int)d->device;
     s->quantized = d->quantized != 0;
+    s->nee = nee;
     s->tiles = (uint32_t)tiles;
     s->max_pending = d->max_pending ? d->max_pending : 1024;
     s->t0 = std::chrono::steady_clock::now();
@@ -302,7 +310,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (tu.shade_grid == 0) tu.shade_grid = cus * 16;
     s->tune = tu;
     const uint64_t slots = tiles * 1024;
-    const uint64_t bytes = slots * REINA_RT_SERVER_BYTES_PER_PIXEL;
+    const uint64_t bytes = slots * bpp;
     if (hipMalloc(&s->dbuf, bytes) != hipSuccess)
         return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     char* p = (char*)s->dbuf;
@@ -315,6 +323,11 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     s->Q.hit = (float4*)p; p += 16 * slots;
     s->Q.pix[0] = (uint32_t*)p; p += 4 * slots;
     s->Q.pix[1] = (uint32_t*)p; p += 4 * slots;
+    if (nee) {
+        s->S.st4 = (float4*)p; p += 16 * slots;
+        s->Q.sray = (float4*)p; p += 32 * slots;
+        s->Q.scon = (float4*)p; p += 16 * slots;
+    }
     s->Q.cap = (uint32_t)(slots / kParts);
     const size_t cbytes = 3 * sizeof(Ctrl) + 256 + kMaxJobs * sizeof(uint32_t) + kMaxJobs * sizeof(JobRec);
     if (hipMalloc(&s->ctrl, cbytes) != hipSuccess) return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
@@ -360,6 +373,12 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     if (scene->device != s->device) return fail(REINA_RT_ERR_INVALID_ARG, "scene lives on another device");
     if (scene->quantized != s->quantized)
         return fail(REINA_RT_ERR_UNSUPPORTED, "scene node format differs from the server's");
+    const bool nee = (scene->options & REINA_RT_TRACE_NEE) != 0;
+    if (nee && !s->nee)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "the scene asks for next-event estimation; the server was created "
+                                              "without REINA_RT_SERVER_NEE");
+    if (nee && scene->emi.empty())
+        return fail(REINA_RT_ERR_NO_EMISSIVE, "next-event estimation needs at least one emissive instance");
     if (scene->info.max_depth > s->tune.stack_depth + 1)
         return fail(REINA_RT_ERR_UNSUPPORTED, "scene BVH depth " + std::to_string(scene->info.max_depth) +
                                                   " exceeds the server's stack depth " +
@@ -383,6 +402,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     Request r;
     r.scene = scene;
     r.P = make_trace_params(pc, W, H);
+    r.P.nee = nee ? 1u : 0u;
     r.img = (float4*)image;
     r.npx = npx;
     r.ntiles = ntiles;

@@ -36,6 +36,7 @@ __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j
     const uint32_t slot = J.px_base + k;
     V3 o = v3(0.0f, 0.0f, 0.0f), d = o;
     if (valid) {
+        if (P.nee) S.st4[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t x = k % P.W, y = k / P.W;
         uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
         if (!anyrays) {
@@ -77,7 +78,9 @@ __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j
 
 // k_shade of integrator.hip with per-ray request lookup; the pixels that
 // finish their last sample are counted per request (one atomic per request
-// present in the wave).
+// present in the wave). NEE: the server holds shadow queues; requests with
+// TraceParams.nee write their shadow rays, every other entry an empty one.
+template <bool NEE>
 __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathState S, Queues Q, uint32_t in_q,
                                                      Ctrl* ctrl, uint32_t slot, uint32_t* done) {
     __shared__ uint32_t s_wave[4];
@@ -109,11 +112,20 @@ __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathSta
             pw = pin[i];
             const JobRec& J = jobs[pw >> kSlotBits];
             const uint32_t sl = pw & ((1u << kSlotBits) - 1u);
-            const ShadeResult res = shade_ray<false>(J.sv, J.P, S, sl, sl - J.px_base, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
+            const ShadeResult res = shade_ray<NEE>(J.sv, J.P, S, sl, sl - J.px_base, rin[2 * i], rin[2 * i + 1], Q.hit[i]);
             push = res.push;
             fin = res.finished;
             no = res.no;
             nd = res.nd;
+            if (NEE) {
+                if (res.shadow) {
+                    Q.sray[2 * i] = make_float4(res.so.x, res.so.y, res.so.z, 0.0f);
+                    Q.sray[2 * i + 1] = make_float4(res.sd.x, res.sd.y, res.sd.z, res.stmax);
+                    Q.scon[i] = make_float4(res.scontrib.x, res.scontrib.y, res.scontrib.z, __uint_as_float(sl));
+                } else {
+                    Q.sray[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);   // no shadow ray
+                }
+            }
         }
         // finished pixels: one atomic per request present among the wave's finishers
         uint64_t fm = __ballot(fin ? 1 : 0);
@@ -170,10 +182,18 @@ hipError_t launch_init_job(const JobRec* jobs, uint32_t j, uint32_t npx, const P
 }
 
 hipError_t launch_shade_multi(const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q, Ctrl* ctrl,
-                              uint32_t slot, uint32_t* done, uint32_t grid, hipStream_t st) {
+                              uint32_t slot, uint32_t* done, uint32_t grid, bool nee, hipStream_t st) {
     grid = grid < kParts ? kParts : grid - grid % kParts;
-    hipLaunchKernelGGL(k_shade_multi, dim3(grid), dim3(256), 0, st, jobs, S, Q, in_q, ctrl, slot, done);
+    if (nee) hipLaunchKernelGGL(k_shade_multi<true>, dim3(grid), dim3(256), 0, st, jobs, S, Q, in_q, ctrl, slot, done);
+    else hipLaunchKernelGGL(k_shade_multi<false>, dim3(grid), dim3(256), 0, st, jobs, S, Q, in_q, ctrl, slot, done);
     return hipGetLastError();
+}
+
+hipError_t launch_connect_multi(bool quantized, const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl,
+                                uint32_t slot, const uint32_t* pix, unsigned long long* stats, const Tuning& tu,
+                                hipStream_t st) {
+    return quantized ? launch_connect_multi_q8(jobs, Q, S, ctrl, slot, pix, stats, tu, st)
+                     : launch_connect_multi_f32(jobs, Q, S, ctrl, slot, pix, stats, tu, st);
 }
 
 hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, hipStream_t st) {

@@ -88,8 +88,10 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
     r.sd = r.no;
     r.scontrib = r.no;
     r.stmax = 0.0f;
+    // NEE code is compiled in with NEE and runs for requests that ask for it
+    const bool nee = NEE && P.nee != 0u;
     float4 s4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (NEE) s4 = S.st4[pp];
+    if (nee) s4 = S.st4[pp];
     bool lambert_hit = false, emissive_mis = false;
     V3 nee_n = r.no, nee_albedo = r.no;
     float emis_w = 1.0f;
@@ -123,7 +125,7 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
             case 2: chs_dielectric(pld, hi, pr, O, D); break;
             default: chs_disney(pld, hi, pr, O, D); break;
         }
-        if (NEE && !pld.skip) {
+        if (nee && !pld.skip) {
             lambert_hit = in.material == 0u;
             nee_n = hi.worldNormal;
             nee_albedo = ldv3(pr.albedo);
@@ -149,9 +151,9 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
             L = add(L, mul(pld.color, thr));
             end = true;
         } else if (!pld.insideDielectric) {
-            V3 contrib = add(v3(0.0f, 0.0f, 0.0f), (NEE && emissive_mis) ? scale(pld.emission, emis_w) : pld.emission);
+            V3 contrib = add(v3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? scale(pld.emission, emis_w) : pld.emission);
             L = add(L, mul(contrib, thr));
-            if (NEE && lambert_hit && seg + 1u < P.maxBounces && sv.num_emi > 0u) {
+            if (nee && lambert_hit && seg + 1u < P.maxBounces && sv.num_emi > 0u) {
                 // light sample (4 draws after the BSDF's), connected by the shadow kernel
                 const float u1 = rnd(pld.rng), u2 = rnd(pld.rng), u3 = rnd(pld.rng), u4 = rnd(pld.rng);
                 const uint32_t k = cdf_pick(0u, sv.num_emi - 1u, u1, [&](uint32_t i) { return sv.emi[i].cdf; });
@@ -184,11 +186,11 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
                 }
                 s4 = make_float4(pld.rayOrigin.x, pld.rayOrigin.y, pld.rayOrigin.z,
                                  maxf_(dot(nee_n, pld.rayDirection), 0.0f) / kPi);
-            } else if (NEE) {
+            } else if (nee) {
                 s4.w = 0.0f;
             }
             thr = mul(thr, pld.color);
-        } else if (NEE) {
+        } else if (nee) {
             s4.w = 0.0f;
         }
     }
@@ -222,7 +224,7 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
     S.st0[pp] = make_uint4(pld.rng, sample, seg, pld.insideDielectric ? 1u : 0u);
     S.st1[pp] = make_float4(thr.x, thr.y, thr.z, pld.accumulatedDistance);
     S.st2[pp] = make_float4(L.x, L.y, L.z, 0.0f);
-    if (NEE) S.st4[pp] = s4;
+    if (nee) S.st4[pp] = s4;
     r.finished = !r.push;
     return r;
 }

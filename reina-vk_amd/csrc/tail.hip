@@ -182,14 +182,19 @@ hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathStat
     return hipGetLastError();
 }
 
-hipError_t launch_tail_multi(bool quantized, const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q,
-                             Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
+hipError_t launch_tail_multi(bool quantized, bool nee, const JobRec* jobs, const PathState& S, const Queues& Q,
+                             uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
                              const Tuning& tu, uint32_t live, hipStream_t st) {
     const uint32_t blocks = tail_blocks(live);
     const SceneView sv{};
     const TraceParams P{};
-    if (quantized) launch_tail_qn<true, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
-    else launch_tail_qn<false, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+    if (quantized) {
+        if (nee) launch_tail_qn<true, true, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+        else launch_tail_qn<true, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+    } else {
+        if (nee) launch_tail_qn<false, true, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+        else launch_tail_qn<false, false, true>(sv, P, S, Q, in_q, ctrl, slot, stats, jobs, done, tu, blocks, st);
+    }
     return hipGetLastError();
 }
 

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
     const float4* tris = sv.tris;
     uint64_t num_nodes = sv.num_nodes;
+    const uint8_t* tflags = sv.tri_flags;
     for (;;) {
         uint64_t idle = __ballot(!active);
         uint32_t nidle = (uint32_t)__popcll(idle);
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                             nodes = Q ? static_cast<const void*>(J.sv.qnodes) : static_cast<const void*>(J.sv.nodes);
                             tris = J.sv.tris;
                             num_nodes = J.sv.num_nodes;
+                            tflags = J.sv.tri_flags;
                         }
                         tray_init(r, rays[2 * idx], rays[2 * idx + 1]);
                         h = miss_record(r);
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             continue;
         }
         if (active) {
-            if (!trav_step<COUNT, Q, ANY>(nodes, tris, r, h, ts, stk, 256, nv, nt, sv.tri_flags)) {
+            if (!trav_step<COUNT, Q, ANY>(nodes, tris, r, h, ts, stk, 256, nv, nt, tflags)) {
                 if constexpr (ANY) {
                     if (h.prim == 0xFFFFFFFFu) {   // unoccluded
                         const float4 c = ao.scon[idx];
@@ -245,6 +247,30 @@ static hipError_t launch_connect_fmt(const SceneView& sv, const Queues& Q_, cons
         case 16: launch_connect_sd<Q, 16>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
         case 24: launch_connect_sd<Q, 24>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
         default: launch_connect_sd<Q, 32>(sv, Q_, S, ctrl, slot, stats, tu, st); break;
+    }
+    return hipGetLastError();
+}
+
+// Shadow rays of the shared queues (server): entry i's ray is sray[i], its
+// request is pix[i] >> kSlotBits.
+template <bool Q, int SD>
+static void launch_connect_multi_sd(const JobRec* jobs, const Queues& Q_, const PathState& S, Ctrl* ctrl,
+                                    uint32_t slot, const uint32_t* pix, unsigned long long* stats, const Tuning& tu,
+                                    hipStream_t st) {
+    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
+    hipLaunchKernelGGL((k_trace<SD, 1, false, Q, true, true>), dim3(tu.grid), dim3(256), 0, st, SceneView{}, Q_.sray,
+                       nullptr, tin, stats, nullptr, tu.refill_min, pix, jobs, AnyHitOut{S.st2, Q_.scon});
+}
+
+template <bool Q>
+static hipError_t launch_connect_multi_fmt(const JobRec* jobs, const Queues& Q_, const PathState& S, Ctrl* ctrl,
+                                           uint32_t slot, const uint32_t* pix, unsigned long long* stats,
+                                           const Tuning& tu, hipStream_t st) {
+    switch (tu.stack_depth) {
+        case 8: launch_connect_multi_sd<Q, 8>(jobs, Q_, S, ctrl, slot, pix, stats, tu, st); break;
+        case 16: launch_connect_multi_sd<Q, 16>(jobs, Q_, S, ctrl, slot, pix, stats, tu, st); break;
+        case 24: launch_connect_multi_sd<Q, 24>(jobs, Q_, S, ctrl, slot, pix, stats, tu, st); break;
+        default: launch_connect_multi_sd<Q, 32>(jobs, Q_, S, ctrl, slot, pix, stats, tu, st); break;
     }
     return hipGetLastError();
 }

@@ -9,4 +9,9 @@ hipError_t launch_trace_multi_f32(const float4* rays, float4* hits, const TraceI
     return launch_trace_fmt<false, true>(none, rays, hits, in, stats, nullptr, false, tu, pix, jobs, st);
 }
 
+hipError_t launch_connect_multi_f32(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                                    const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st) {
+    return launch_connect_multi_fmt<false>(jobs, Q, S, ctrl, slot, pix, stats, tu, st);
+}
+
 }  // namespace rt

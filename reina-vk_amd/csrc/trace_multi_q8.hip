@@ -9,4 +9,9 @@ hipError_t launch_trace_multi_q8(const float4* rays, float4* hits, const TraceIn
     return launch_trace_fmt<true, true>(none, rays, hits, in, stats, nullptr, false, tu, pix, jobs, st);
 }
 
+hipError_t launch_connect_multi_q8(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                                    const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st) {
+    return launch_connect_multi_fmt<true>(jobs, Q, S, ctrl, slot, pix, stats, tu, st);
+}
+
 }  // namespace rt

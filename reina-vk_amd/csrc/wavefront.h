@@ -121,8 +121,19 @@ hipError_t launch_trace_multi_q8(const float4* rays, float4* hits, const TraceIn
                                  const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st);
 hipError_t launch_init_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, const Queues& Q,
                            Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, hipStream_t st);
+// nee: the server holds next-event-estimation state (S.st4, Q.sray, Q.scon);
+// each request's TraceParams.nee then selects NEE for its rays, and every
+// queue entry gets a shadow ray or an empty one.
 hipError_t launch_shade_multi(const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q, Ctrl* ctrl,
-                              uint32_t slot, uint32_t* done, uint32_t grid, hipStream_t st);
+                              uint32_t slot, uint32_t* done, uint32_t grid, bool nee, hipStream_t st);
+// Shadow rays of the shared queues (pix: the iteration's slot | request words).
+hipError_t launch_connect_multi(bool quantized, const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl,
+                                uint32_t slot, const uint32_t* pix, unsigned long long* stats, const Tuning& tu,
+                                hipStream_t st);
+hipError_t launch_connect_multi_f32(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                                    const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st);
+hipError_t launch_connect_multi_q8(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
+                                   const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st);
 hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, hipStream_t st);
 hipError_t launch_trace_f32(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
@@ -149,8 +160,8 @@ hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathStat
                        uint32_t live, hipStream_t st);
 // Server version: finishes every request in the shared queues (pixels that
 // finish their last sample are counted in done[request]).
-hipError_t launch_tail_multi(bool quantized, const JobRec* jobs, const PathState& S, const Queues& Q, uint32_t in_q,
-                             Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
+hipError_t launch_tail_multi(bool quantized, bool nee, const JobRec* jobs, const PathState& S, const Queues& Q,
+                             uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
                              const Tuning& tu, uint32_t live, hipStream_t st);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,

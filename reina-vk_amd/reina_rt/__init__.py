@@ -166,13 +166,14 @@ class Server:
     on resident DeviceScenes share one wavefront; submit() returns a ticket
     at once, wait()/poll() report completion with timestamps."""
 
-    def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, **tuning):
+    def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, nee=False, **tuning):
         d = abi.ReinaServerDesc()
         d.struct_size = ctypes.sizeof(abi.ReinaServerDesc)
         d.device = device
         d.path_budget_bytes = int(path_budget_bytes)
         d.max_pending = max_pending
         d.quantized = int(bool(quantized))
+        d.flags = abi.SERVER_NEE if nee else 0
         for k, v in tuning.items():
             setattr(d.tuning, k, v)
         h = ctypes.c_void_p()

@@ -97,7 +97,8 @@ class ReinaTuning(ctypes.Structure):
 class ReinaServerDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("path_budget_bytes", ctypes.c_uint64), ("max_pending", ctypes.c_uint32),
-                ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning)]
+                ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning), ("flags", ctypes.c_uint32),
+                ("_pad0", ctypes.c_uint32)]
 
 
 class ReinaRequestStatus(ctypes.Structure):
@@ -115,6 +116,8 @@ class ReinaServerStats(ctypes.Structure):
 
 
 SERVER_BYTES_PER_PIXEL = 152
+SERVER_BYTES_PER_PIXEL_NEE = 216
+SERVER_NEE = 0x1
 TAIL_OFF = 0xFFFFFFFF
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 

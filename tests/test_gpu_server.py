@@ -182,3 +182,67 @@ def test_server_edge_requests(cuda, pool):
     assert _same(got, plain(torch, dev, pc, 40, 40))
     srvq.close()
     q.close()
+
+
+@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF], ids=["tail_default", "tail_off"])
+def test_server_nee_mix(cuda, pool, tail):
+    """A server created with REINA_RT_SERVER_NEE: requests on scenes whose
+    trace options include REINA_RT_TRACE_NEE render with NEE + MIS, the
+    others without; every image equals reina_rt_trace's on the same scene
+    and options, mixed in one wavefront, with and without the tail launch."""
+    torch = cuda
+    rng = np.random.default_rng(11)
+    c_arrays, c_cam, c_dev = pool["cornell"]
+    i_arrays, i_cam, i_dev = pool["ico"]
+    c_nee = reina_rt.DeviceScene(c_arrays)
+    c_nee.set_trace_options(abi.TRACE_NEE)
+    i_nee = reina_rt.DeviceScene(i_arrays)
+    i_nee.set_trace_options(abi.TRACE_NEE)
+    jobs = []
+    for dev, W, H, spp, depth, batch, cornell in (
+            (c_nee, 64, 64, 4, 8, 0, True), (c_dev, 48, 40, 3, 8, 0, True), (i_nee, 96, 54, 2, 8, 0, False),
+            (i_dev, 64, 48, 2, 6, 0, False), (c_nee, 33, 17, 5, 3, 2, True), (i_nee, 40, 40, 3, 8, 1, False)):
+        pc = scenes.cornell_push(W, H, spp=spp, max_bounces=depth, sample_batch=batch) if cornell \
+            else _pc(i_cam, W, H, spp, depth, batch)
+        init = rng.uniform(0, 1, (H, W, 4)).astype(np.float32) if batch else None
+        jobs.append((dev, pc, W, H, init))
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, nee=True, tail_paths=tail)
+    mixed = served(torch, srv, jobs)
+    for (dev, pc, W, H, init), got in zip(jobs, mixed):
+        assert _same(got, plain(torch, dev, pc, W, H, init))
+    # NEE changes the image (the light sample draws extra random numbers)
+    assert not _same(mixed[0], plain(torch, c_dev, jobs[0][1], 64, 64))
+    st = srv.stats()
+    assert st.completed == len(jobs) and st.slot_tiles_used == 0
+    if tail == abi.TAIL_OFF:
+        assert st.tail_launches == 0
+    srv.close()
+    c_nee.close()
+    i_nee.close()
+
+
+def test_server_nee_refusals(cuda, pool):
+    """A NEE request on a server without REINA_RT_SERVER_NEE is refused
+    (REINA_RT_ERR_UNSUPPORTED), and one on a scene without emissive
+    instances too (REINA_RT_ERR_NO_EMISSIVE), as reina_rt_trace refuses it."""
+    from test_oracle_geometry import soup_arrays
+    torch = cuda
+    c_arrays, _, _ = pool["cornell"]
+    c_nee = reina_rt.DeviceScene(c_arrays)
+    c_nee.set_trace_options(abi.TRACE_NEE)
+    img = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
+    pc = scenes.cornell_push(8, 8, spp=1, max_bounces=2)
+    srv = reina_rt.Server(path_budget_bytes=32 << 20)
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srv.submit(c_nee, pc, img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_UNSUPPORTED
+    srv.close()
+    dark = reina_rt.DeviceScene(soup_arrays(np.random.default_rng(2).uniform(-1, 1, (16, 9)).astype(np.float32)))
+    dark.set_trace_options(abi.TRACE_NEE)
+    srvn = reina_rt.Server(path_budget_bytes=32 << 20, nee=True)
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srvn.submit(dark, pc, img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_NO_EMISSIVE
+    srvn.close()
+    dark.close()
+    c_nee.close()
