@@ -315,8 +315,8 @@ def device_build_block(a, arrays, dev, frame, img, torch):
     dd = reina_rt.DeviceScene(arrays, flags=abi.BUILD_DEVICE)
     setup = time.perf_counter() - t0
     hi, di = dev.info(), dd.info()
-    t = dev.get_tuning()
-    dd.set_tuning(refill_min=t["refill_min"], grid_blocks=t["grid_blocks"], waves_per_simd=t["waves_per_simd"])
+    # the device-built tree keeps its own default tuning (its depth, and so
+    # its stack depth, can differ from the host-built tree's)
     frame(0, dev)
     ref = img.clone()
     frame(0, dd)
