@@ -519,9 +519,12 @@ def instanced_block(a, torch):
     # frames
     for name, d in devs.items():
         if not a.no_autotune:
-            at = run_autotune(d, sec, torch.zeros((sec.shape[0], 4), device="cuda:0"), torch)
+            at = run_autotune(d, sec, torch.zeros((sec.shape[0], 4), device="cuda:0"), torch,
+                              frame=lambda d=d: frame(0, d))
             at.pop("table")
+            at.pop("frame_table", None)
             res[name]["autotune_best"] = at["best"]
+            res[name]["autotune_frame_best"] = at.get("frame_best")
         log("instanced: %s frames" % name)
         rays_f, elapsed, gpu_ms = time_frames(lambda b, d=d: frame(b, d), 1, 2, torch)
         res[name]["mrays_per_s"] = round(rays_f / elapsed / 1e6, 2)
