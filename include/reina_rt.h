@@ -144,6 +144,14 @@ typedef struct ReinaInstanceDesc {
 
 /* The packed buffers exactly as Models::addModel produces them
  * (src/scene/Models.cpp:24-92) plus the instance and properties lists. */
+/* One texture of the hit shaders' bindless `sampler2D textures[]`
+ * (src/graphics/Image.cpp:10-86: RGBA8 UNORM, one mip level): texel (x, y)
+ * at rgba8[4 * (y * width + x)], y = 0 the first row (texture v = 0). */
+typedef struct ReinaTextureDesc {
+    uint32_t width, height;
+    const uint8_t* rgba8;
+} ReinaTextureDesc;
+
 typedef struct ReinaSceneDesc {
     uint32_t struct_size;                    /* sizeof(ReinaSceneDesc)            */
     uint32_t flags;                          /* REINA_RT_BUILD_* | REINA_RT_NODES_* */
@@ -157,6 +165,10 @@ typedef struct ReinaSceneDesc {
     const ReinaModelRange*         models;    uint32_t num_models;
     const ReinaInstanceDesc*       instances; uint32_t num_instances;
     uint32_t device;                         /* HIP device ordinal               */
+    /* InstanceProperties.textureID / normalMapTexID / bumpMapTexID index this
+     * array (-1: none); copied at scene creation */
+    const ReinaTextureDesc*        textures;  uint32_t num_textures;
+    uint32_t _pad0;
 } ReinaSceneDesc;
 
 typedef struct ReinaScene ReinaScene;

@@ -68,6 +68,10 @@ def lib():
     L.or_postprocess.restype = None
     L.or_math_eval.argtypes = [i32, c_p, c_p, c_p, u64]
     L.or_math_eval.restype = None
+    L.or_tex_sample_at.argtypes = [c_p, i32, ctypes.c_float, ctypes.c_float, c_p]
+    L.or_tex_sample_at.restype = None
+    L.or_bump_at.argtypes = [c_p, i32, ctypes.c_float, ctypes.c_float, c_p, c_p, c_p]
+    L.or_bump_at.restype = None
     _lib = L
     return L
 
@@ -104,6 +108,22 @@ class OracleScene:
         buf = np.zeros(n, np.uint8)
         lib().or_export_emissive(self.h, buf.ctypes.data)
         return buf
+
+    def tex_sample(self, tex_id, u, v):
+        """texture(textures[tex_id], (u, v)) as the hit shaders sample it
+        (RGBA, float32)."""
+        out = np.zeros(4, np.float32)
+        lib().or_tex_sample_at(self.h, tex_id, float(u), float(v), out.ctypes.data)
+        return out
+
+    def bump(self, tex_id, u, v, ray_dir, tbn):
+        """bumpMapping(uv, rayIn, T, textures[tex_id]) (texutils.h.glsl:4-41);
+        tbn: 3x3 with columns T, B, N."""
+        d = np.ascontiguousarray(ray_dir, np.float32).reshape(3)
+        t = np.ascontiguousarray(np.asarray(tbn, np.float32).T.reshape(9))   # column-major
+        out = np.zeros(2, np.float32)
+        lib().or_bump_at(self.h, tex_id, float(u), float(v), d.ctypes.data, t.ctypes.data, out.ctypes.data)
+        return out
 
     def occluded(self, ray):
         r = np.ascontiguousarray(np.asarray(ray).reshape(()))

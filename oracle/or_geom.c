@@ -133,6 +133,13 @@ OrScene* or_scene_create(const ReinaSceneDesc* d) {
     s->ntci = d->num_tex_indices; DUP(s->tex_indices, d->tex_indices, s->ntci, uint32_t);
     s->nprops = d->num_props;    DUP(s->props, d->props, s->nprops, ReinaInstanceProperties);
     s->nmodels = d->num_models;  DUP(s->models, d->models, s->nmodels, ReinaModelRange);
+    s->ntex = d->num_textures;
+    s->tex = (struct OrTexture*)calloc(s->ntex ? s->ntex : 1, sizeof(struct OrTexture));
+    for (uint32_t t = 0; t < s->ntex; ++t) {
+        s->tex[t].w = d->textures[t].width;
+        s->tex[t].h = d->textures[t].height;
+        DUP(s->tex[t].rgba8, d->textures[t].rgba8, 4ull * s->tex[t].w * s->tex[t].h, uint8_t);
+    }
     s->ninst = d->num_instances;
     s->inst = (OrInstance*)calloc(s->ninst ? s->ninst : 1, sizeof(OrInstance));
     uint64_t ntri = 0;
@@ -181,6 +188,8 @@ void or_scene_destroy(OrScene* s) {
     free(s->tex_coords); free(s->tex_indices); free(s->props); free(s->models);
     free(s->inst); free(s->wtri); free(s->tri_inst); free(s->tri_prim);
     free(s->emi); free(s->tri_cdf); free(s->inst_emi); free(s->tri_flags);
+    for (uint32_t t = 0; t < s->ntex; ++t) free(s->tex[t].rgba8);
+    free(s->tex);
     free(s);
 }
 

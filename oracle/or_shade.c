@@ -529,6 +529,96 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
     return r;
 }
 
+/* ============================ textures ==================================== */
+/* texture(textures[id], uv) of the hit shaders: RGBA8 UNORM, bilinear, repeat,
+ * base level (a ray-tracing stage has no derivatives; Image.cpp:56,
+ * vktools.cpp:765-788): two linear interpolations along x, one along y. */
+typedef struct { float x, y, z, w; } f4;
+
+static f4 unorm8(const uint8_t* p) {
+    f4 r = {(float)p[0] / 255.0f, (float)p[1] / 255.0f, (float)p[2] / 255.0f, (float)p[3] / 255.0f};
+    return r;
+}
+
+/* two linear interpolations along x, then one along y */
+static float lerp2(float a, float b, float p00, float p10, float p01, float p11) {
+    float top = p00 + a * (p10 - p00), bottom = p01 + a * (p11 - p01);
+    return top + b * (bottom - top);
+}
+
+static uint32_t repeat_index(float fx, uint32_t n) {
+    int32_t m = (int32_t)fx % (int32_t)n;
+    return (uint32_t)(m < 0 ? m + (int32_t)n : m);
+}
+
+static f4 tex_sample(const OrScene* s, int32_t id, float u, float v) {
+    const struct OrTexture* t = &s->tex[id];
+    float x = u * (float)t->w - 0.5f, y = v * (float)t->h - 0.5f;
+    x = (x == x) ? or_clampf(x, -1073741824.0f, 1073741824.0f) : 0.0f;
+    y = (y == y) ? or_clampf(y, -1073741824.0f, 1073741824.0f) : 0.0f;
+    float fx = floorf(x), fy = floorf(y);
+    float a = x - fx, b = y - fy;
+    uint32_t i0 = repeat_index(fx, t->w), i1 = repeat_index(fx + 1.0f, t->w);
+    uint32_t j0 = repeat_index(fy, t->h), j1 = repeat_index(fy + 1.0f, t->h);
+    f4 t00 = unorm8(t->rgba8 + 4ull * ((uint64_t)j0 * t->w + i0)), t10 = unorm8(t->rgba8 + 4ull * ((uint64_t)j0 * t->w + i1));
+    f4 t01 = unorm8(t->rgba8 + 4ull * ((uint64_t)j1 * t->w + i0)), t11 = unorm8(t->rgba8 + 4ull * ((uint64_t)j1 * t->w + i1));
+    f4 r = {lerp2(a, b, t00.x, t10.x, t01.x, t11.x), lerp2(a, b, t00.y, t10.y, t01.y, t11.y),
+            lerp2(a, b, t00.z, t10.z, t01.z, t11.z), lerp2(a, b, t00.w, t10.w, t01.w, t11.w)};
+    return r;
+}
+
+/* texutils.h.glsl:4-41 */
+static void bumpMapping(const OrScene* s, int32_t id, float* u, float* v, v3 rayIn, m3 T) {
+    const float heightScale = 0.2f, minLayers = 64.0f, maxLayers = 512.0f;
+    v3 V = vnormalize(m3tmul(T, vneg(rayIn)));
+    if (V.z <= 0.0f) return;
+    float numLayers = mixf(maxLayers, minLayers, or_clampf(V.z, 0.0f, 1.0f));
+    float layerDepth = 1.0f / numLayers;
+    float Px = V.x / V.z * heightScale, Py = V.y / V.z * heightScale;
+    float du = Px / numLayers, dv = Py / numLayers;
+    float cu = *u, cv = *v, depthSum = 0.0f;
+    float h = tex_sample(s, id, cu, cv).x * heightScale;
+    for (int k = 0; k < 1024 && depthSum < h; ++k) {   /* ends within 104 steps for finite input */
+        cu = cu - du;
+        cv = cv - dv;
+        depthSum = depthSum + layerDepth;
+        h = tex_sample(s, id, cu, cv).x * heightScale;
+    }
+    float pu = cu + du, pv = cv + dv;
+    float hPrev = tex_sample(s, id, pu, pv).x * heightScale;
+    float sumPrev = depthSum - layerDepth;
+    float after = h - depthSum;
+    float before = hPrev - sumPrev;
+    float weight = after / (after - before);
+    *u = mixf(pu, cu, weight);
+    *v = mixf(pv, cv, weight);
+}
+
+/* lambertian.rchit.glsl:37-41: texture(normalMap).rgb * 2 - 1, y flipped, tbn * n */
+static v3 normalMap(const OrScene* s, int32_t id, float u, float v, m3 tbn) {
+    f4 t = tex_sample(s, id, u, v);
+    v3 tn = V3(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f);
+    tn.y = -tn.y;
+    return vnormalize(m3mul(tbn, tn));
+}
+
+/* entry points for the tests: the sampler and the parallax search alone */
+void or_tex_sample_at(const OrScene* s, int32_t id, float u, float v, float* out4) {
+    f4 r = tex_sample(s, id, u, v);
+    out4[0] = r.x; out4[1] = r.y; out4[2] = r.z; out4[3] = r.w;
+}
+
+void or_bump_at(const OrScene* s, int32_t id, float u, float v, const float* dir3, const float* tbn9,
+                float* out2) {
+    m3 T;
+    T.c0 = V3(tbn9[0], tbn9[1], tbn9[2]);
+    T.c1 = V3(tbn9[3], tbn9[4], tbn9[5]);
+    T.c2 = V3(tbn9[6], tbn9[7], tbn9[8]);
+    bumpMapping(s, id, &u, &v, V3(dir3[0], dir3[1], dir3[2]), T);
+    out2[0] = u;
+    out2[1] = v;
+}
+
 /* ============================ hit shaders ================================= */
 /* lambertian.rchit.glsl:11-78 and metal.rchit.glsl:7-70 */
 static void chs_lambert_metal(const OrScene* s, Payload* pld, const HitCtx* c, int metal) {
@@ -536,9 +626,16 @@ static void chs_lambert_metal(const OrScene* s, Payload* pld, const HitCtx* c, i
     const ReinaInstanceProperties* p = c->props;
     if (p->cullBackface != 0u && !hi.frontFace) { skip_hit(pld, &hi, c); return; }
     float ux = fmodf1(hi.uvx), uy = fmodf1(hi.uvy);
+    if (p->bumpMapTexID >= 0) bumpMapping(s, p->bumpMapTexID, &ux, &uy, vnormalize(c->worldRayDirection), hi.tbn);
     if (!metal && (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f)) { skip_hit(pld, &hi, c); return; }
     v3 wn = hi.worldNormal;
+    if (p->normalMapTexID >= 0) wn = normalMap(s, p->normalMapTexID, ux, uy, hi.tbn);
     pld->color = pv3(p->albedo);
+    if (p->textureID >= 0) {
+        f4 tc = tex_sample(s, p->textureID, ux, uy);
+        if (tc.w < 0.999f && or_random(&pld->rngState) > tc.w) { skip_hit(pld, &hi, c); return; }
+        pld->color = vmul(pld->color, V3(tc.x, tc.y, tc.z));
+    }
     pld->albedo = pld->color;
     pld->emission = pv3(p->emission);
     pld->rayOrigin = offsetPositionAlongNormal(hi.worldPosition, hi.worldNormalGeometry);
@@ -567,8 +664,15 @@ static void chs_dielectric(const OrScene* s, Payload* pld, const HitCtx* c) {
     int cannot = ri * sinTheta > 1.0f;
     float refl = reflectance(cosTheta, ri);
     int prevInside = pld->insideDielectric;
+    float ux = fmodf1(hi.uvx), uy = fmodf1(hi.uvy);
+    if (p->bumpMapTexID >= 0) bumpMapping(s, p->bumpMapTexID, &ux, &uy, vnormalize(c->worldRayDirection), hi.tbn);
     v3 albedo = pv3(p->albedo);
+    if (p->textureID >= 0) {
+        f4 tc = tex_sample(s, p->textureID, ux, uy);
+        albedo = vmul(albedo, V3(tc.x, tc.y, tc.z));
+    }
     v3 wn = hi.worldNormal;
+    if (p->normalMapTexID >= 0) wn = normalMap(s, p->normalMapTexID, ux, uy, hi.tbn);
     if (cannot || refl > or_random(&pld->rngState)) {
         pld->rayDirection = fuzzyReflection(unitDir, wn, p->roughness, &pld->rngState);
         pld->color = V3(1.0f, 1.0f, 1.0f);
@@ -604,9 +708,16 @@ static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c) {
     const ReinaInstanceProperties* p = c->props;
     if (p->cullBackface != 0u && !hi.frontFace) { skip_hit(pld, &hi, c); return; }
     float ux = fmodf1(hi.uvx), uy = fmodf1(hi.uvy);
+    if (p->bumpMapTexID >= 0) bumpMapping(s, p->bumpMapTexID, &ux, &uy, vnormalize(c->worldRayDirection), hi.tbn);
     if (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f) { skip_hit(pld, &hi, c); return; }
     v3 wn = hi.worldNormal;
+    if (p->normalMapTexID >= 0) wn = normalMap(s, p->normalMapTexID, ux, uy, hi.tbn);
     v3 albedo = pv3(p->albedo);
+    if (p->textureID >= 0) {
+        f4 tc = tex_sample(s, p->textureID, ux, uy);
+        if (tc.w < 0.999f && or_random(&pld->rngState) > tc.w) { skip_hit(pld, &hi, c); return; }
+        albedo = vmul(albedo, V3(tc.x, tc.y, tc.z));
+    }
     float eta = hi.frontFace ? 1.0f / p->ior : p->ior;
     int didRefract = 0, choseGlass = 0;
     v3 wi = vneg(c->worldRayDirection);
@@ -725,8 +836,8 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
             if (nee && !pld->skip) {
                 HitInfo hi = getObjectHitInfo(s, &c);
                 lambert_hit = c.inst->material == 0u;
-                nee_n = hi.worldNormal;
-                nee_albedo = pv3(c.props->albedo);
+                nee_n = pld->surfaceNormal;   /* normal-mapped shading normal */
+                nee_albedo = pld->albedo;     /* textured colour */
                 const int32_t ek = s->inst_emi[s->tri_inst[hit.prim]];
                 if (mis_pb > 0.0f && ek >= 0) {
                     v3 w[3];

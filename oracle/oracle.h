@@ -41,6 +41,9 @@ typedef struct OrScene {
     ReinaInstanceProperties* props; uint32_t nprops;
     ReinaModelRange* models; uint32_t nmodels;
     OrInstance* inst; uint32_t ninst;
+    /* textures (RGBA8, 4 bytes per texel, row-major; copied) */
+    struct OrTexture { uint32_t w, h; uint8_t* rgba8; } *tex;
+    uint32_t ntex;
     /* flattened world-space triangles, instance-major */
     uint64_t ntri;
     float* wtri;             /* 9 floats per triangle: v0, v1, v2              */
@@ -60,6 +63,9 @@ typedef struct OrScene {
 } OrScene;
 
 OrScene* or_scene_create(const ReinaSceneDesc* desc);
+/* texture(textures[id], (u, v)) and bumpMapping of the hit shaders (tests) */
+void or_tex_sample_at(const OrScene* s, int32_t id, float u, float v, float* out4);
+void or_bump_at(const OrScene* s, int32_t id, float u, float v, const float* dir3, const float* tbn9, float* out2);
 void or_scene_destroy(OrScene* s);
 int or_scene_set_bvh(OrScene* s, const void* nodes, uint64_t nodes_bytes,
                      const void* tris, uint64_t tris_bytes, uint32_t node_bytes);

@@ -82,9 +82,18 @@ static int validate(const ReinaSceneDesc* d) {
             return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": indicesOffset out of range");
         if (p.texIndicesOffset != 0xFFFFFFFFu && (uint64_t)p.texIndicesOffset + 3ull * r.indexCount > d->num_tex_indices)
             return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": texIndicesOffset out of range");
-        if (p.textureID >= 0 || p.normalMapTexID >= 0 || p.bumpMapTexID >= 0)
-            return fail(REINA_RT_ERR_UNSUPPORTED, "instance " + std::to_string(i) +
-                                                      ": texture, normal and bump maps are not supported");
+        const int32_t ids[3] = {p.textureID, p.normalMapTexID, p.bumpMapTexID};
+        for (int32_t id : ids)
+            if (id >= 0 && (uint32_t)id >= d->num_textures)
+                return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": texture index " +
+                                                          std::to_string(id) + " out of range");
+    }
+    if (d->num_textures && !d->textures) return fail(REINA_RT_ERR_INVALID_ARG, "num_textures without textures");
+    for (uint32_t t = 0; t < d->num_textures; ++t) {
+        const ReinaTextureDesc& x = d->textures[t];
+        if (x.width == 0 || x.height == 0 || x.width > (1u << 16) || x.height > (1u << 16) || !x.rgba8)
+            return fail(REINA_RT_ERR_INVALID_ARG, "texture " + std::to_string(t) +
+                                                      ": width and height must be 1..65536 with data");
     }
     return REINA_RT_OK;
 }
@@ -173,6 +182,15 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     size_t p_ie = add(s->inst_emi.data(), s->inst_emi.size() * 4);
     size_t p_tf = add(s->tri_flags.data(), s->tri_flags.size());
     size_t p_rec = add(nullptr, tri_inst.size() * (uint64_t)kShadeRecF4 * 16);   // filled on the device
+    // textures: a TexInfo table, then every texture's texels (RGBA8 packed in 32 bits)
+    std::vector<TexInfo> tinfo(d->num_textures);
+    uint64_t ntexel = 0;
+    for (uint32_t t = 0; t < d->num_textures; ++t) {
+        tinfo[t] = TexInfo{d->textures[t].width, d->textures[t].height, ntexel};
+        ntexel += (uint64_t)d->textures[t].width * d->textures[t].height;
+    }
+    size_t p_tinfo = add(tinfo.data(), tinfo.size() * sizeof(TexInfo));
+    size_t p_texel = add(nullptr, ntexel * 4);
     (void)wtri;
     HIP_TRY(hipSetDevice(s->device));
     hipError_t e = hipMalloc(&s->arena, off);
@@ -181,6 +199,9 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     for (const Part& p : parts)
         if (p.bytes && p.src) HIP_TRY(hipMemcpy((char*)s->arena + p.off, p.src, p.bytes, hipMemcpyHostToDevice));
     char* a = (char*)s->arena;
+    for (uint32_t t = 0; t < d->num_textures; ++t)   // byte order r, g, b, a = the packed word's low to high byte
+        HIP_TRY(hipMemcpy(a + parts[p_texel].off + 4 * tinfo[t].base, d->textures[t].rgba8,
+                          4ull * tinfo[t].w * tinfo[t].h, hipMemcpyHostToDevice));
     s->sv.nodes = s->quantized ? nullptr : (const Bvh8Node*)(a + parts[p_nodes].off);
     s->sv.qnodes = s->quantized ? (const QBvh8Node*)(a + parts[p_nodes].off) : nullptr;
     s->sv.tris = (const float4*)(a + parts[p_tris].off);
@@ -205,6 +226,9 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     s->sv.tri_flags = (const uint8_t*)(a + parts[p_tf].off);
     s->sv.num_emi = (uint32_t)s->emi.size();
     s->sv.num_inst = (uint32_t)inst.size();
+    s->sv.tex = (const TexInfo*)(a + parts[p_tinfo].off);
+    s->sv.texels = (const uint32_t*)(a + parts[p_texel].off);
+    s->sv.num_tex = d->num_textures;
     float4* rec = (float4*)(a + parts[p_rec].off);
     HIP_TRY(launch_shade_records(s->sv, rec, (uint32_t)tri_inst.size(), nullptr));
     HIP_TRY(hipStreamSynchronize(nullptr));

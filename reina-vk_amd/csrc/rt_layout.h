@@ -94,6 +94,13 @@ static_assert(sizeof(EmissiveInst) == 32, "emissive record size");
 constexpr uint8_t kTriCull = 1, kTriMirror = 2;
 
 // Device-resident scene (pointers into one HBM arena).
+// A texture in the scene arena: w x h RGBA8 texels (packed r | g << 8 |
+// b << 16 | a << 24) starting at texel index `base`.
+struct TexInfo {
+    uint32_t w, h;
+    uint64_t base;
+};
+
 struct SceneView {
     const Bvh8Node* nodes;       // FP32 nodes (null when quantized)
     const QBvh8Node* qnodes;     // quantized nodes (null when FP32)
@@ -125,6 +132,10 @@ struct SceneView {
     // vertex 0, 1, 2 (27 floats, 1 pad): the object-space values hit_info reads, gathered
     // once per scene
     const float4* srec;
+    // textures of InstanceProperties.textureID / normalMapTexID / bumpMapTexID
+    const TexInfo* tex;
+    const uint32_t* texels;
+    uint32_t num_tex;
 };
 constexpr uint32_t kShadeRecF4 = 10;   // 160 bytes
 

@@ -22,6 +22,7 @@ RT_D float rnd(uint32_t& s) {
 // The part of HitPayload (shaderCommon.h.glsl:18-36) that the active path reads.
 struct Payload {
     V3 color, rayOrigin, rayDirection, emission;
+    V3 albedo, surfaceNormal;   // the BSDF's colour and shading normal (read by next-event estimation)
     uint32_t rng;
     float accumulatedDistance;
     bool rayHitSky, skip, insideDielectric;
@@ -493,6 +494,88 @@ RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float a
 }
 
 // ---- hit shaders ---------------------------------------------------------------
+// ---- textures -------------------------------------------------------------
+// The hit shaders' texture(textures[id], uv): RGBA8 UNORM, linear filter,
+// repeat addressing (sampler of src/tools/vktools.cpp:765-788, one mip level,
+// src/graphics/Image.cpp:56). A ray-tracing stage has no derivatives, so
+// texture() samples the base level and the sampler's anisotropy has no
+// effect: the result is the bilinear filter of the Vulkan specification
+// (texel coordinates u * w - 0.5, v * h - 0.5), evaluated here in FP32 as two
+// linear interpolations along x and one along y. Hardware filtering rounds its weights to a few
+// fraction bits, so texel-edge values of the reference are implementation
+// defined; this order is the library's definition.
+RT_D float4 unorm8(uint32_t t) {
+    return make_float4((float)(t & 0xFFu) / 255.0f, (float)((t >> 8) & 0xFFu) / 255.0f,
+                       (float)((t >> 16) & 0xFFu) / 255.0f, (float)(t >> 24) / 255.0f);
+}
+
+// floor(x) mod n for a floored, finite x with |x| <= 2^30
+RT_D uint32_t repeat_index(float fx, uint32_t n) {
+    const int32_t m = (int32_t)fx % (int32_t)n;
+    return (uint32_t)(m < 0 ? m + (int32_t)n : m);
+}
+
+RT_D float4 tex_sample(const SceneView& sv, int32_t id, float u, float v) {
+    const TexInfo t = sv.tex[id];
+    float x = u * (float)t.w - 0.5f, y = v * (float)t.h - 0.5f;
+    // non-finite coordinates (a NaN uv) read texel 0; huge ones are clamped so
+    // the integer conversion stays defined
+    x = (x == x) ? clampf_(x, -1073741824.0f, 1073741824.0f) : 0.0f;
+    y = (y == y) ? clampf_(y, -1073741824.0f, 1073741824.0f) : 0.0f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const uint32_t i0 = repeat_index(fx, t.w), i1 = repeat_index(fx + 1.0f, t.w);
+    const uint32_t j0 = repeat_index(fy, t.h), j1 = repeat_index(fy + 1.0f, t.h);
+    const uint32_t* row0 = sv.texels + t.base + (uint64_t)j0 * t.w;
+    const uint32_t* row1 = sv.texels + t.base + (uint64_t)j1 * t.w;
+    const float4 t00 = unorm8(row0[i0]), t10 = unorm8(row0[i1]), t01 = unorm8(row1[i0]), t11 = unorm8(row1[i1]);
+    // two linear interpolations along x, then one along y: exact for equal texels
+    const auto lerp2 = [a, b](float p00, float p10, float p01, float p11) {
+        const float top = p00 + a * (p10 - p00), bottom = p01 + a * (p11 - p01);
+        return top + b * (bottom - top);
+    };
+    return make_float4(lerp2(t00.x, t10.x, t01.x, t11.x), lerp2(t00.y, t10.y, t01.y, t11.y),
+                       lerp2(t00.z, t10.z, t01.z, t11.z), lerp2(t00.w, t10.w, t01.w, t11.w));
+}
+
+// texutils.h.glsl:4-41 (parallax occlusion search). The layer loop ends
+// within 0.2 * 512 + 1 steps for finite input; the cap only guards NaN.
+constexpr int kBumpMaxSteps = 1024;
+RT_D void bump_mapping(const SceneView& sv, int32_t id, float& u, float& v, V3 rayIn, const M3& T) {
+    const float heightScale = 0.2f, minLayers = 64.0f, maxLayers = 512.0f;
+    const V3 V = normalize(tmul(T, neg(rayIn)));
+    if (V.z <= 0.0f) return;
+    const float numLayers = mixf(maxLayers, minLayers, clampf_(V.z, 0.0f, 1.0f));
+    const float layerDepth = 1.0f / numLayers;
+    const float Px = V.x / V.z * heightScale, Py = V.y / V.z * heightScale;
+    const float du = Px / numLayers, dv = Py / numLayers;
+    float cu = u, cv = v, depthSum = 0.0f;
+    float h = tex_sample(sv, id, cu, cv).x * heightScale;
+    for (int k = 0; k < kBumpMaxSteps && depthSum < h; ++k) {
+        cu = cu - du;
+        cv = cv - dv;
+        depthSum = depthSum + layerDepth;
+        h = tex_sample(sv, id, cu, cv).x * heightScale;
+    }
+    const float pu = cu + du, pv = cv + dv;
+    const float hPrev = tex_sample(sv, id, pu, pv).x * heightScale;
+    const float sumPrev = depthSum - layerDepth;
+    const float after = h - depthSum;
+    const float before = hPrev - sumPrev;
+    const float weight = after / (after - before);
+    u = mixf(pu, cu, weight);
+    v = mixf(pv, cv, weight);
+}
+
+// tangentNormal = texture(normalMap, uv).rgb * 2 - 1; y flipped;
+// normalize(tbn * tangentNormal) (lambertian.rchit.glsl:37-41)
+RT_D V3 normal_map(const SceneView& sv, int32_t id, float u, float v, const M3& tbn) {
+    const float4 t = tex_sample(sv, id, u, v);
+    V3 tn = v3(t.x * 2.0f - 1.0f, t.y * 2.0f - 1.0f, t.z * 2.0f - 1.0f);
+    tn.y = -tn.y;
+    return normalize(mmul(tbn, tn));
+}
+
 RT_D void skip_hit(Payload& pld, const HitInfo& hi, V3 D) {                // closestHitCommon.h.glsl:185-193
     pld.rayOrigin = offset_along(hi.worldPosition, neg(hi.worldNormal));
     pld.rayDirection = D;
@@ -501,12 +584,22 @@ RT_D void skip_hit(Payload& pld, const HitInfo& hi, V3 D) {                // cl
 }
 
 // lambertian.rchit.glsl:11-78 (metal = false) and metal.rchit.glsl:7-70 (metal = true)
-RT_D void chs_lambert_metal(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 D, bool metal) {
+RT_D void chs_lambert_metal(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p,
+                            V3 D, bool metal) {
     if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
-    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);
+    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
+    if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
     if (!metal && (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f)) { skip_hit(pld, hi, D); return; }
     V3 wn = hi.worldNormal;
+    if (p.normalMapTexID >= 0) wn = normal_map(sv, p.normalMapTexID, ux, uy, hi.tbn);
     pld.color = ldv3(p.albedo);
+    if (p.textureID >= 0) {
+        const float4 tc = tex_sample(sv, p.textureID, ux, uy);
+        if (tc.w < 0.999f && rnd(pld.rng) > tc.w) { skip_hit(pld, hi, D); return; }   // transparency
+        pld.color = mul(pld.color, v3(tc.x, tc.y, tc.z));
+    }
+    pld.albedo = pld.color;
+    pld.surfaceNormal = wn;
     pld.emission = ldv3(p.emission);
     pld.rayOrigin = offset_along(hi.worldPosition, hi.worldNormalGeometry);
     if (!metal) pld.rayDirection = diffuse_reflection(wn, pld.rng);
@@ -518,7 +611,8 @@ RT_D void chs_lambert_metal(Payload& pld, const HitInfo& hi, const ReinaInstance
 }
 
 // dielectric.rchit.glsl:40-113
-RT_D void chs_dielectric(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O, V3 D) {
+RT_D void chs_dielectric(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O,
+                         V3 D) {
     float ri = hi.frontFace ? 1.0f / p.ior : p.ior;
     V3 unitDir = normalize(D);
     float cosTheta = minf_(dot(neg(unitDir), hi.worldNormal), 1.0f);
@@ -526,8 +620,15 @@ RT_D void chs_dielectric(Payload& pld, const HitInfo& hi, const ReinaInstancePro
     bool cannot = ri * sinTheta > 1.0f;
     float refl = reflectance(cosTheta, ri);
     bool prevInside = pld.insideDielectric;
+    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
+    if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
     V3 albedo = ldv3(p.albedo);
+    if (p.textureID >= 0) {
+        const float4 tc = tex_sample(sv, p.textureID, ux, uy);
+        albedo = mul(albedo, v3(tc.x, tc.y, tc.z));
+    }
     V3 wn = hi.worldNormal;
+    if (p.normalMapTexID >= 0) wn = normal_map(sv, p.normalMapTexID, ux, uy, hi.tbn);
     if (cannot || refl > rnd(pld.rng)) {
         pld.rayDirection = fuzzy_reflection(unitDir, wn, p.roughness, pld.rng);
         pld.color = v3(1.0f, 1.0f, 1.0f);
@@ -544,18 +645,28 @@ RT_D void chs_dielectric(Payload& pld, const HitInfo& hi, const ReinaInstancePro
         pld.color = mul(v3(e, e, e), albedo);
         pld.accumulatedDistance = 0.0f;
     }
+    pld.albedo = pld.color;
+    pld.surfaceNormal = wn;
     pld.emission = ldv3(p.emission);
     pld.rayHitSky = false;
     pld.skip = false;
 }
 
 // disney.rchit.glsl:36-198
-RT_D void chs_disney(Payload& pld, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O, V3 D) {
+RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O,
+                     V3 D) {
     if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
-    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);
+    float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
+    if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
     if (ux < 0.0f || ux > 1.0f || uy < 0.0f || uy > 1.0f) { skip_hit(pld, hi, D); return; }
     V3 wn = hi.worldNormal;
+    if (p.normalMapTexID >= 0) wn = normal_map(sv, p.normalMapTexID, ux, uy, hi.tbn);
     V3 albedo = ldv3(p.albedo);
+    if (p.textureID >= 0) {
+        const float4 tc = tex_sample(sv, p.textureID, ux, uy);
+        if (tc.w < 0.999f && rnd(pld.rng) > tc.w) { skip_hit(pld, hi, D); return; }   // transparency
+        albedo = mul(albedo, v3(tc.x, tc.y, tc.z));
+    }
     float eta = hi.frontFace ? 1.0f / p.ior : p.ior;
     bool didRefract = false, choseGlass = false;
     V3 wi = neg(D);
@@ -568,6 +679,8 @@ RT_D void chs_disney(Payload& pld, const HitInfo& hi, const ReinaInstancePropert
                        didRefract, wn, wi, rayDir, h, pdf);
     float cosThetaI = maxf_(dot(wn, rayDir), 0.0f);
     pld.color = divs(scale(f, cosThetaI), pdf);
+    pld.albedo = albedo;
+    pld.surfaceNormal = wn;
     pld.emission = ldv3(p.emission);
     pld.rayOrigin = offset_dielectric(hi.worldPosition, hi.worldNormalGeometry, rayDir);
     pld.rayDirection = rayDir;

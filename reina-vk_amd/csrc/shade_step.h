@@ -120,15 +120,15 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         const ReinaInstanceProperties& pr = sv.props[__float_as_uint(rec[2].w)];
         HitInfo hi = hit_info(sv, in, pr, rec, pm, hh.z, hh.w, D);
         switch (in.material) {
-            case 0: chs_lambert_metal(pld, hi, pr, D, false); break;
-            case 1: chs_lambert_metal(pld, hi, pr, D, true); break;
-            case 2: chs_dielectric(pld, hi, pr, O, D); break;
-            default: chs_disney(pld, hi, pr, O, D); break;
+            case 0: chs_lambert_metal(pld, sv, hi, pr, D, false); break;
+            case 1: chs_lambert_metal(pld, sv, hi, pr, D, true); break;
+            case 2: chs_dielectric(pld, sv, hi, pr, O, D); break;
+            default: chs_disney(pld, sv, hi, pr, O, D); break;
         }
         if (nee && !pld.skip) {
             lambert_hit = in.material == 0u;
-            nee_n = hi.worldNormal;
-            nee_albedo = ldv3(pr.albedo);
+            nee_n = pld.surfaceNormal;     // the shading normal (normal-mapped) and
+            nee_albedo = pld.albedo;       // textured colour of the Lambertian BSDF
             // an emitter reached by the continuation of a Lambertian bounce: MIS weight
             const int32_t ek = sv.inst_emi[ii];
             if (s4.w > 0.0f && ek >= 0) {

@@ -56,6 +56,10 @@ assert MODEL_RANGE.itemsize == 32
 assert INSTANCE_DESC.itemsize == 80
 
 
+class ReinaTextureDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba8", ctypes.c_void_p)]
+
+
 class ReinaSceneDesc(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32), ("flags", ctypes.c_uint32),
@@ -69,6 +73,7 @@ class ReinaSceneDesc(ctypes.Structure):
         ("models", ctypes.c_void_p), ("num_models", ctypes.c_uint32),
         ("instances", ctypes.c_void_p), ("num_instances", ctypes.c_uint32),
         ("device", ctypes.c_uint32),
+        ("textures", ctypes.c_void_p), ("num_textures", ctypes.c_uint32), ("_pad0", ctypes.c_uint32),
     ]
 
 

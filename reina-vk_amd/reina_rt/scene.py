@@ -64,7 +64,18 @@ class Scene:
         self._idx_count = 0
         self._tbni_count = 0
         self._texi_count = 0
+        self._textures = []        # (H, W, 4) uint8 RGBA images (Scene::textures)
         self._built = False
+
+    def add_texture(self, rgba8) -> int:
+        """An RGBA8 image, row 0 = texture v = 0 (the order stb loads a file
+        in, src/graphics/Image.cpp:10-86); returns the texture index that
+        Material.textureID / normalMapID / bumpMapID refer to."""
+        img = np.ascontiguousarray(rgba8, dtype=np.uint8)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise ValueError("texture must be (height, width, 4) uint8")
+        self._textures.append(img)
+        return len(self._textures) - 1
 
     # Models::addModel (src/scene/Models.cpp:24-92)
     def define_object(self, md: ModelData) -> int:
@@ -170,7 +181,7 @@ class Scene:
             inst[k]["materialIdx"] = mi
             inst[k]["modelID"] = oid
         return dict(vertices=verts, indices=idx, tbns=tbns, tbn_indices=tbi, tex_coords=texc,
-                    tex_indices=tei, props=props, models=models, instances=inst)
+                    tex_indices=tei, props=props, models=models, instances=inst, textures=list(self._textures))
 
 
 def make_desc(arrays, flags=0, device=0):
@@ -193,6 +204,13 @@ def make_desc(arrays, flags=0, device=0):
     d.props, d.num_props = ptr(a["props"]), len(a["props"])
     d.models, d.num_models = ptr(a["models"]), len(a["models"])
     d.instances, d.num_instances = ptr(a["instances"]), len(a["instances"])
+    tex = a.get("textures") or []
+    if tex:
+        td = (abi.ReinaTextureDesc * len(tex))()
+        for k, img in enumerate(tex):
+            td[k].width, td[k].height, td[k].rgba8 = img.shape[1], img.shape[0], img.ctypes.data
+        d._texture_descs = td         # kept alive with the descriptor
+        d.textures, d.num_textures = ctypes.addressof(td), len(tex)
     return d
 
 

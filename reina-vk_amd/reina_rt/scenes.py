@@ -32,18 +32,24 @@ def _quad(c, u, v):
     return p, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), (t, b, n)
 
 
-def _model_from_quads(quads):
-    verts, idx, tbns = [], [], []
+def _model_from_quads(quads, uv_scale=None):
+    """uv_scale: give every quad texture coordinates (0,0), (s,0), (s,s), (0,s)
+    at its corners (s > 1 repeats the texture)."""
+    verts, idx, tbns, uvs = [], [], [], []
     for p, f, (t, b, n) in quads:
         base = len(verts)
-        for q in p:
+        for k, q in enumerate(p):
             verts.append([q[0], q[1], q[2], 1.0])
             tbns.append(np.stack([t, b, n]))
+            if uv_scale is not None:
+                uvs.append([uv_scale * (k in (1, 2)), uv_scale * (k in (2, 3))])
         idx.extend((f + base).reshape(-1).tolist())
     verts = np.array(verts, np.float32)
     idx = np.array(idx, np.uint32)
     tbns = np.array(tbns, np.float32)
-    return ModelData(verts, idx, tbns, idx.copy())
+    if uv_scale is None:
+        return ModelData(verts, idx, tbns, idx.copy())
+    return ModelData(verts, idx, tbns, idx.copy(), np.array(uvs, np.float32), idx.copy())
 
 
 def _translate(t):
@@ -107,6 +113,99 @@ def cornell_push(width=256, height=256, spp=16, max_bounces=8, sample_batch=0):
     _, cam = cornell()
     return push_consts(cam["eye"], cam["center"], cam["fov"], width, height, spp, max_bounces,
                        sample_batch=sample_batch, defocus=0.0, direct_clamp=100.0)
+
+
+# ---- textured Cornell box (the hit shaders' texture, normal-map and bump-map paths) -
+def checker_texture(w=64, h=48, cells=8, seed=1):
+    """RGBA8 checkerboard; some cells fully transparent, some half."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    c = ((x * cells // w) + (y * cells // h)) % 2
+    img = np.zeros((h, w, 4), np.uint8)
+    img[..., 0] = np.where(c, 230, 40)
+    img[..., 1] = np.where(c, 200, 90)
+    img[..., 2] = np.where(c, 60, 200)
+    alpha = np.full((cells, cells), 255, np.uint8)
+    alpha[rng.integers(cells, size=4), rng.integers(cells, size=4)] = 0
+    alpha[rng.integers(cells, size=4), rng.integers(cells, size=4)] = 128
+    img[..., 3] = alpha[y * cells // h, x * cells // w]
+    return img
+
+
+def wave_normal_map(w=32, h=32, amp=0.35):
+    """Tangent-space normal map of a wavy surface, encoded rgb = n * 0.5 + 0.5."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    n = np.stack([amp * np.sin(2 * np.pi * x / w), amp * np.cos(2 * np.pi * y / h), np.ones_like(x)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    img = np.zeros((h, w, 4), np.uint8)
+    img[..., :3] = np.clip(np.round((n * 0.5 + 0.5) * 255), 0, 255)
+    img[..., 3] = 255
+    return img
+
+
+def height_texture(w=32, h=32, seed=5):
+    """Height map in the red channel (value noise in [0, 1])."""
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    p = np.stack([x / w * 4, y / h * 4, np.zeros_like(x)], -1)
+    v = value_noise(p.reshape(-1, 3), seed, octaves=3).reshape(h, w)
+    v = (v - v.min()) / max(v.max() - v.min(), 1e-9)
+    img = np.zeros((h, w, 4), np.uint8)
+    img[..., 0] = np.round(v * 255)
+    img[..., 3] = 255
+    return img
+
+
+def textured_cornell(white_opaque=False):
+    """The Cornell box with texture coordinates on every wall and box.
+    Default: the back wall has a checker albedo texture with transparent
+    cells, repeated twice (Lambertian); the floor a height map (bump
+    mapping) and the checker; the left wall and the ceiling a normal map;
+    the right wall is a normal-mapped metal, the tall box a textured,
+    bump-mapped dielectric, the short box a textured, normal-mapped Disney
+    surface. white_opaque: cornell()'s materials, each with an opaque white
+    (255, 255, 255, 255) albedo texture and no normal or bump map: it renders
+    exactly like cornell()."""
+    s = Scene()
+    I = np.eye(4)
+    if white_opaque:
+        tex = s.add_texture(np.full((4, 4, 4), 255, np.uint8))
+        wall = lambda albedo: Material(materialIdx=0, albedo=albedo, cullBackface=True, textureID=tex)
+        floor_m = ceil_m = back_m = wall(GREY)
+        left_m, right_m = wall(RED), wall(GREEN)
+        tall_m = short_m = Material(materialIdx=0, albedo=GREY, textureID=tex)
+    else:
+        tex = s.add_texture(checker_texture())
+        nrm = s.add_texture(wave_normal_map())
+        bump = s.add_texture(height_texture())
+        floor_m = Material(materialIdx=0, albedo=GREY, cullBackface=True, textureID=tex, bumpMapID=bump)
+        ceil_m = Material(materialIdx=0, albedo=GREY, cullBackface=True, normalMapID=nrm)
+        back_m = Material(materialIdx=0, albedo=GREY, cullBackface=True, textureID=tex)
+        left_m = Material(materialIdx=0, albedo=RED, cullBackface=True, textureID=tex, normalMapID=nrm)
+        right_m = Material(materialIdx=1, albedo=GREEN, roughness=0.2, normalMapID=nrm)
+        tall_m = Material(materialIdx=2, albedo=(0.9, 0.95, 0.9), ior=1.5, textureID=tex, bumpMapID=bump)
+        short_m = Material(materialIdx=3, albedo=GREY, roughness=0.4, metallic=0.3, textureID=tex, normalMapID=nrm,
+                           interpNormals=True)
+    floor = [_quad((0, 0, z), (0, 0, 0.5), (1, 0, 0)) for z in (0.5, -0.5)]
+    ceil = [_quad((0, 2, z), (1, 0, 0), (0, 0, 0.5)) for z in (-0.5, 0.5)]
+    s.add_object(_model_from_quads(floor, 1.0), I, floor_m)
+    s.add_object(_model_from_quads(ceil, 1.0), I, ceil_m)
+    s.add_object(_model_from_quads([_quad((0, 1, -1), (1, 0, 0), (0, 1, 0))], 2.0), I, back_m)
+    s.add_object(_model_from_quads([_quad((-1, 1, 0), (0, 1, 0), (0, 0, 1))], 1.0), I, left_m)
+    s.add_object(_model_from_quads([_quad((1, 1, 0), (0, 0, 1), (0, 1, 0))], 1.0), I, right_m)
+    light = _model_from_quads([_quad((-0.005, 1.989, -0.03), (0.235, 0, 0), (0, 0, 0.19))])
+    s.add_object(light, I, Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(15.0, 15.0, 15.0),
+                                    cullBackface=True))
+    h = 0.5
+    cube_faces = [((h, 0, 0), (0, h, 0), (0, 0, h)), ((-h, 0, 0), (0, 0, h), (0, h, 0)),
+                  ((0, h, 0), (0, 0, h), (h, 0, 0)), ((0, -h, 0), (h, 0, 0), (0, 0, h)),
+                  ((0, 0, h), (h, 0, 0), (0, h, 0)), ((0, 0, -h), (0, h, 0), (h, 0, 0))]
+    cube = s.define_object(_model_from_quads([_quad(*f) for f in cube_faces], 1.0))
+    tall = _translate((-0.35, 0.6, -0.35)) @ _scale((0.6, 1.2, 0.6))
+    short = _translate((0.35, 0.3, 0.3)) @ _rot_y(15.0) @ _scale((0.6, 0.6, 0.6))
+    s.add_instance(cube, tall, tall_m)
+    s.add_instance(cube, short, short_m)
+    cam = dict(eye=(0.0, 1.0, 3.4), center=(0.0, 1.0, 0.0), fov=40.0)
+    return s, cam
 
 
 # ---- value noise -----------------------------------------------------------------

@@ -57,7 +57,7 @@ def test_header_layout_with_gcc():
 def test_ctypes_structs_match_header():
     """Every ctypes mirror of a header struct has the C layout (gcc)."""
     structs = [abi.ReinaSceneDesc, abi.ReinaSceneInfo, abi.ReinaTraceStats, abi.ReinaTuning, abi.ReinaServerDesc,
-               abi.ReinaRequestStatus, abi.ReinaServerStats]
+               abi.ReinaRequestStatus, abi.ReinaServerStats, abi.ReinaTextureDesc]
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for st in structs:
         n = st.__name__
@@ -135,9 +135,9 @@ def test_create_rejects_bad_descriptions(cornell_arrays):
     bad["indices"][5] = len(bad["vertices"]) + 3
     assert _create(bad)[0] == abi.ERR_INVALID_ARG
     bad = {k: v.copy() for k, v in arrays.items()}
-    bad["props"][2]["textureID"] = 0
+    bad["props"][2]["textureID"] = 0        # the scene has no textures
     rc, msg = _create(bad)
-    assert rc == abi.ERR_UNSUPPORTED and "texture" in msg
+    assert rc == abi.ERR_INVALID_ARG and "texture index" in msg
     bad = {k: v.copy() for k, v in arrays.items()}
     bad["instances"][0]["instancePropertiesID"] = 1000
     assert _create(bad)[0] == abi.ERR_INVALID_ARG
