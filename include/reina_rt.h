@@ -414,6 +414,49 @@ int reina_rt_poll(ReinaServer* server, uint64_t ticket, ReinaRequestStatus* out)
 int reina_rt_wait(ReinaServer* server, uint64_t ticket, uint32_t timeout_ms, ReinaRequestStatus* out);
 int reina_rt_server_stats(ReinaServer* server, ReinaServerStats* out);
 
+/* ---- scene ingest (host only: no device, no stream) ---------------------- */
+
+/* One model's buffers as src/scene/Models.h:23-30 (ModelData) holds them and
+ * Models::addModel (src/scene/Models.cpp:24-92) consumes them: the arrays of
+ * one ReinaSceneDesc model before the per-model offsets are added. The index
+ * arrays are model-local. num_indices need not be a multiple of 3 (line and
+ * point records of an OBJ file add 2- and 1-index faces, as in the
+ * reference); a model has num_indices / 3 triangles. */
+typedef struct ReinaModelData {
+    float*    vertices;     uint64_t num_vertices;     /* x, y, z, 1 per vertex            */
+    uint32_t* indices;      uint64_t num_indices;
+    float*    tbns;         uint64_t num_tbns;         /* 9 floats each: columns T, B, N  */
+    uint32_t* tbn_indices;  uint64_t num_tbn_indices;
+    float*    tex_coords;   uint64_t num_tex_coords;   /* u, v per coordinate (0 = none) */
+    uint32_t* tex_indices;  uint64_t num_tex_indices;
+} ReinaModelData;
+
+/* Models::getObjData (src/scene/Models.cpp:117-175): the first mesh of a
+ * Wavefront OBJ file as Assimp imports it with aiProcess_Triangulate |
+ * GenSmoothNormals | JoinIdenticalVertices | CalcTangentSpace | FlipUVs.
+ * `text` holds the file's bytes (need not be NUL-terminated). The result is
+ * released with reina_rt_model_free. REINA_RT_ERR_INVALID_ARG on a malformed
+ * file or one without faces (the reference throws there). */
+int reina_rt_model_load_obj(const char* path, ReinaModelData** out);
+int reina_rt_model_load_obj_memory(const char* text, uint64_t len, ReinaModelData** out);
+void reina_rt_model_free(ReinaModelData* model);
+
+/* An RGBA8 image as stb_image returns it with 4 channels requested
+ * (src/graphics/Image.cpp:10-41). From a file, rows are flipped vertically
+ * (stbi_set_flip_vertically_on_load(true), Image.cpp:14: row 0 = the file's
+ * last row); from memory (glTF-embedded images, Image.cpp:24-41) they are
+ * not. PNG only (all bit depths, colour types, tRNS and Adam7 interlacing);
+ * other formats return REINA_RT_ERR_UNSUPPORTED. Released with
+ * reina_rt_image_free. */
+typedef struct ReinaImage {
+    uint32_t width, height;
+    uint8_t* rgba8;              /* width * height * 4 bytes, row-major */
+} ReinaImage;
+
+int reina_rt_image_load(const char* path, ReinaImage** out);
+int reina_rt_image_load_memory(const uint8_t* bytes, uint64_t len, ReinaImage** out);
+void reina_rt_image_free(ReinaImage* image);
+
 /* Diagnostic: evaluates the library's FP32 elementary functions on n inputs
  * (device pointers) so that tests can compare them with the host definitions.
  * fn: 0 sin, 1 cos, 2 log, 3 exp, 4 sqrt, 5 pow(x, y = in2). */

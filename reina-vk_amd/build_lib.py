@@ -17,9 +17,12 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libreina_rt.so")
 
+# host-only translation units (no device code): g++ with the same flags
+HOST_SOURCES = ["ingest_obj.cpp", "ingest_png.cpp"]
 SOURCES = ["api.cpp", "bvh_host.cpp", "bvh_device.hip", "integrator.hip", "trace_f32.hip", "trace_q8.hip", "trace_multi_f32.hip", "trace_multi_q8.hip", "server.hip", "server.cpp", "tail.hip", "post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
@@ -32,7 +35,10 @@ DEVICE = [
 
 def compile_one(src):
     out = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", out]
+    if src in HOST_SOURCES:
+        cmd = [CXX] + COMMON + ["-c", os.path.join(CSRC, src), "-o", out]
+    else:
+        cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
@@ -42,9 +48,9 @@ def compile_one(src):
 def build(jobs=4, verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+        objs = list(ex.map(compile_one, SOURCES + HOST_SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs + ["-lpthread"]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp] + objs + ["-lpthread", "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))

@@ -65,9 +65,10 @@ static int validate(const ReinaSceneDesc* d) {
     for (uint64_t i = 0; i < d->num_tbn_indices; ++i)
         if (d->tbn_indices[i] >= d->num_tbns)
             return fail(REINA_RT_ERR_INVALID_ARG, "tbn index " + std::to_string(i) + " out of range");
-    for (uint64_t i = 0; i < d->num_tex_indices; ++i)
-        if (d->tex_indices[i] != 0xFFFFFFFFu && d->tex_indices[i] >= d->num_tex_coords)
-            return fail(REINA_RT_ERR_INVALID_ARG, "tex index " + std::to_string(i) + " out of range");
+    // texture-coordinate indices are read only through an instance whose
+    // texIndicesOffset is set: a model without UVs (texIndexOffset = -1,
+    // Models.cpp:50) keeps index values with no coordinates behind them
+    std::vector<uint8_t> tex_reached(d->num_tex_indices, 0);
     for (uint32_t i = 0; i < d->num_instances; ++i) {
         const ReinaInstanceDesc& in = d->instances[i];
         if (in.modelID >= d->num_models)
@@ -82,12 +83,17 @@ static int validate(const ReinaSceneDesc* d) {
             return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": indicesOffset out of range");
         if (p.texIndicesOffset != 0xFFFFFFFFu && (uint64_t)p.texIndicesOffset + 3ull * r.indexCount > d->num_tex_indices)
             return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": texIndicesOffset out of range");
+        if (p.texIndicesOffset != 0xFFFFFFFFu && r.indexCount)
+            std::memset(tex_reached.data() + p.texIndicesOffset, 1, 3ull * r.indexCount);
         const int32_t ids[3] = {p.textureID, p.normalMapTexID, p.bumpMapTexID};
         for (int32_t id : ids)
             if (id >= 0 && (uint32_t)id >= d->num_textures)
                 return fail(REINA_RT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": texture index " +
                                                           std::to_string(id) + " out of range");
     }
+    for (uint64_t i = 0; i < d->num_tex_indices; ++i)
+        if (tex_reached[i] && d->tex_indices[i] != 0xFFFFFFFFu && d->tex_indices[i] >= d->num_tex_coords)
+            return fail(REINA_RT_ERR_INVALID_ARG, "tex index " + std::to_string(i) + " out of range");
     if (d->num_textures && !d->textures) return fail(REINA_RT_ERR_INVALID_ARG, "num_textures without textures");
     for (uint32_t t = 0; t < d->num_textures; ++t) {
         const ReinaTextureDesc& x = d->textures[t];

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import abi
-from .scene import Material, ModelData, Scene, make_desc, push_consts
+from .scene import Material, ModelData, Scene, load_image, load_obj, make_desc, push_consts
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libreina_rt.so")
@@ -59,11 +59,19 @@ def lib():
         "reina_rt_poll": [c_p, u64, c_p],
         "reina_rt_wait": [c_p, u64, u32, c_p],
         "reina_rt_server_stats": [c_p, c_p],
+        "reina_rt_model_load_obj": [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(abi.ReinaModelData))],
+        "reina_rt_model_load_obj_memory": [ctypes.c_char_p, u64, ctypes.POINTER(ctypes.POINTER(abi.ReinaModelData))],
+        "reina_rt_image_load": [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(abi.ReinaImage))],
+        "reina_rt_image_load_memory": [ctypes.c_char_p, u64, ctypes.POINTER(ctypes.POINTER(abi.ReinaImage))],
     }
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = i32
+    L.reina_rt_model_free.argtypes = [ctypes.POINTER(abi.ReinaModelData)]
+    L.reina_rt_model_free.restype = None
+    L.reina_rt_image_free.argtypes = [ctypes.POINTER(abi.ReinaImage)]
+    L.reina_rt_image_free.restype = None
     L.reina_rt_last_error.argtypes = []
     L.reina_rt_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -82,7 +90,9 @@ def exported_symbols():
             "reina_rt_intersect", "reina_rt_postprocess", "reina_rt_debug_math", "reina_rt_bvh_build_host",
             "reina_rt_set_trace_options", "reina_rt_get_tuning", "reina_rt_set_tuning",
             "reina_rt_bvh_quantize_host", "reina_rt_server_create", "reina_rt_server_destroy", "reina_rt_submit",
-            "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats", "reina_rt_scene_export_emissive"]
+            "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats", "reina_rt_scene_export_emissive",
+            "reina_rt_model_load_obj", "reina_rt_model_load_obj_memory", "reina_rt_model_free",
+            "reina_rt_image_load", "reina_rt_image_load_memory", "reina_rt_image_free"]
 
 
 class DeviceScene:

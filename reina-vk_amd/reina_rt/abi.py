@@ -60,6 +60,21 @@ class ReinaTextureDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba8", ctypes.c_void_p)]
 
 
+class ReinaModelData(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.POINTER(ctypes.c_float)), ("num_vertices", ctypes.c_uint64),
+        ("indices", ctypes.POINTER(ctypes.c_uint32)), ("num_indices", ctypes.c_uint64),
+        ("tbns", ctypes.POINTER(ctypes.c_float)), ("num_tbns", ctypes.c_uint64),
+        ("tbn_indices", ctypes.POINTER(ctypes.c_uint32)), ("num_tbn_indices", ctypes.c_uint64),
+        ("tex_coords", ctypes.POINTER(ctypes.c_float)), ("num_tex_coords", ctypes.c_uint64),
+        ("tex_indices", ctypes.POINTER(ctypes.c_uint32)), ("num_tex_indices", ctypes.c_uint64),
+    ]
+
+
+class ReinaImage(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba8", ctypes.POINTER(ctypes.c_uint8))]
+
+
 class ReinaSceneDesc(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32), ("flags", ctypes.c_uint32),

@@ -27,6 +27,59 @@ class ModelData:
     texIndices: np.ndarray = None
 
 
+def _copy(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def load_obj(path_or_text):
+    """Models::getObjData (src/scene/Models.cpp:117-175) through the library's
+    native OBJ loader (reina_rt_model_load_obj): the file's first mesh,
+    triangulated, with smooth normals where the file has none, tangent space,
+    joined vertices and flipped v. `path_or_text` is a path, or the file's
+    bytes. Returns ModelData (texCoords None when the mesh has none)."""
+    from . import check, lib
+    L = lib()
+    out = ctypes.POINTER(abi.ReinaModelData)()
+    if isinstance(path_or_text, (bytes, bytearray)):
+        data = bytes(path_or_text)
+        check(L.reina_rt_model_load_obj_memory(data, len(data), ctypes.byref(out)))
+    else:
+        check(L.reina_rt_model_load_obj(str(path_or_text).encode(), ctypes.byref(out)))
+    try:
+        m = out.contents
+        v = _copy(m.vertices, 4 * m.num_vertices, np.float32).reshape(-1, 4)
+        idx = _copy(m.indices, m.num_indices, np.uint32)
+        tb = _copy(m.tbns, 9 * m.num_tbns, np.float32).reshape(-1, 3, 3)
+        tbi = _copy(m.tbn_indices, m.num_tbn_indices, np.uint32)
+        tc = _copy(m.tex_coords, 2 * m.num_tex_coords, np.float32).reshape(-1, 2) if m.num_tex_coords else None
+        ti = _copy(m.tex_indices, m.num_tex_indices, np.uint32)
+    finally:
+        L.reina_rt_model_free(out)
+    return ModelData(v, idx, tb, tbi, tc, ti)
+
+
+def load_image(path_or_bytes):
+    """A PNG as the RGBA8 (height, width, 4) array stb_image returns with 4
+    channels (src/graphics/Image.cpp:10-41): a path loads with rows flipped
+    (row 0 = the file's last row), bytes load unflipped."""
+    from . import check, lib
+    L = lib()
+    out = ctypes.POINTER(abi.ReinaImage)()
+    if isinstance(path_or_bytes, (bytes, bytearray)):
+        data = bytes(path_or_bytes)
+        check(L.reina_rt_image_load_memory(data, len(data), ctypes.byref(out)))
+    else:
+        check(L.reina_rt_image_load(str(path_or_bytes).encode(), ctypes.byref(out)))
+    try:
+        im = out.contents
+        img = _copy(im.rgba8, 4 * im.width * im.height, np.uint8).reshape(im.height, im.width, 4)
+    finally:
+        L.reina_rt_image_free(out)
+    return img
+
+
 @dataclass
 class Material:
     """src/scene/Scene.h:21-46 (field order kept)."""
@@ -77,8 +130,16 @@ class Scene:
         self._textures.append(img)
         return len(self._textures) - 1
 
-    # Models::addModel (src/scene/Models.cpp:24-92)
-    def define_object(self, md: ModelData) -> int:
+    def define_texture(self, path_or_bytes) -> int:
+        """Scene::defineTexture (src/scene/Scene.cpp:15-25): a PNG file
+        (rows flipped as stbi_load with flip-on-load) or PNG bytes (unflipped)."""
+        return self.add_texture(load_image(path_or_bytes))
+
+    # Models::addModel (src/scene/Models.cpp:24-92); a path goes through
+    # Models::addModel(filepath) = getObjData first
+    def define_object(self, md) -> int:
+        if not isinstance(md, ModelData):
+            md = load_obj(md)
         if self._built:
             raise RuntimeError("Could not add model; buffers are already built")
         v = np.ascontiguousarray(md.vertices, dtype=np.float32).reshape(-1, 4)
@@ -139,7 +200,7 @@ class Scene:
         t = np.asarray(transform, dtype=np.float32).reshape(4, 4)
         self._instances.append((len(self._props) - 1, int(mat.materialIdx), int(object_id), t))
 
-    def add_object(self, md: ModelData, transform, mat: Material) -> int:
+    def add_object(self, md, transform, mat: Material) -> int:
         oid = self.define_object(md)
         self.add_instance(oid, transform, mat)
         return oid

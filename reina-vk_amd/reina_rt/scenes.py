@@ -208,6 +208,32 @@ def textured_cornell(white_opaque=False):
     return s, cam
 
 
+def obj_cornell(models_dir, textures_dir=None, subject="uv_sphere_highres.obj"):
+    """The OBJ scene of src/Reina.cpp:103-115, loaded through the library's
+    OBJ and PNG loaders: models/cornell_box.obj with textures/cornell_texture.png
+    (Lambertian, back faces culled), models/cornell_light.obj (emission 16) and
+    a rough Disney subject scaled by 0.25 after a (0, 2, 0) translation
+    (glm::translate(glm::scale(I, 0.25), (0, 2, 0))). textures_dir defaults to
+    models_dir."""
+    import os
+    textures_dir = textures_dir or models_dir
+    s = Scene()
+    wall_tex = s.define_texture(os.path.join(textures_dir, "cornell_texture.png"))
+    wall = Material(materialIdx=0, textureID=wall_tex, albedo=(0.9, 0.9, 0.9), roughness=0.0, ior=0.0,
+                    cullBackface=True, sheenTint=(0.0, 0.0, 0.0))
+    subject_m = Material(materialIdx=3, albedo=(1.0, 1.0, 1.0), roughness=0.5, ior=1.5, interpNormals=True,
+                         sheenTint=(1.0, 1.0, 1.0))
+    light_m = Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(16.0, 16.0, 16.0), roughness=0.0, ior=0.0,
+                       cullBackface=True, sheenTint=(0.0, 0.0, 0.0))
+    I = np.eye(4)
+    s.add_object(os.path.join(models_dir, "cornell_box.obj"), I, wall)
+    s.add_object(os.path.join(models_dir, "cornell_light.obj"), I, light_m)
+    s.add_object(os.path.join(models_dir, subject), _scale((0.25, 0.25, 0.25)) @ _translate((0.0, 2.0, 0.0)),
+                 subject_m)
+    cam = dict(eye=(0.0, 1.0, 3.4), center=(0.0, 1.0, 0.0), fov=40.0)
+    return s, cam
+
+
 # ---- value noise -----------------------------------------------------------------
 def _hash3(ix, iy, iz, seed):
     h = (ix.astype(np.int64) * 73856093) ^ (iy.astype(np.int64) * 19349663) ^ (iz.astype(np.int64) * 83492791)

@@ -57,7 +57,8 @@ def test_header_layout_with_gcc():
 def test_ctypes_structs_match_header():
     """Every ctypes mirror of a header struct has the C layout (gcc)."""
     structs = [abi.ReinaSceneDesc, abi.ReinaSceneInfo, abi.ReinaTraceStats, abi.ReinaTuning, abi.ReinaServerDesc,
-               abi.ReinaRequestStatus, abi.ReinaServerStats, abi.ReinaTextureDesc]
+               abi.ReinaRequestStatus, abi.ReinaServerStats, abi.ReinaTextureDesc, abi.ReinaModelData,
+               abi.ReinaImage]
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for st in structs:
         n = st.__name__
@@ -96,7 +97,7 @@ def test_library_sizes_and_version():
 
 def test_every_declared_symbol_is_exported():
     text = open(HEADER).read()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(reina_rt_\w+)\s*\(", text, re.M))
+    declared = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(reina_rt_\w+)\s*\(", text, re.M))
     assert len(declared) >= 10
     L = reina_rt.lib()
     for name in declared:
