@@ -81,6 +81,7 @@ def main():
                     help="served block at full size: 32 scenes of 1e4-4e6 triangles, 50-400 requests/s")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the path tracer has no multi-GPU path here); "
@@ -183,6 +184,10 @@ def main():
     if a.instanced:
         log("instanced 8.4M-triangle scene")
         out["instanced_8m"] = instanced_block(a, torch)
+
+    if not a.no_ingest:
+        log("scene ingest")
+        out["ingest"] = ingest_block()
 
     if a.calib_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
@@ -539,6 +544,70 @@ def instanced_block(a, torch):
     for d in devs.values():
         d.close()
     return res
+
+def ingest_block(reps=3):
+    """Host-side loaders (reina_rt_model_load_obj_memory, reina_rt_image_load_memory)
+    on synthetic files held in memory: the icosphere L7 mesh as OBJ text with
+    positions, spherical texture coordinates and normals (v/vt/vn faces), and
+    a 2048x2048 RGBA PNG (gradient + noise, Paeth-filtered rows, zlib level 6).
+    Best of `reps` runs; Assimp and stb are not in this image, so no
+    reference-side timing is given."""
+    import zlib
+
+    import numpy as np
+
+    from reina_rt import scenes
+    from reina_rt.scene import load_image, load_obj
+
+    v, f = scenes.subdivided_icosphere(7)
+    uv = np.stack([0.5 + np.arctan2(v[:, 2], v[:, 0]) / (2 * np.pi), 0.5 - np.arcsin(v[:, 1]) / np.pi], axis=1)
+    lines = ["v %.6f %.6f %.6f" % tuple(p) for p in v]
+    lines += ["vt %.6f %.6f" % tuple(t) for t in uv]
+    lines += ["vn %.6f %.6f %.6f" % tuple(p) for p in v]
+    lines += ["f %d/%d/%d %d/%d/%d %d/%d/%d" % (a + 1, a + 1, a + 1, b + 1, b + 1, b + 1, c + 1, c + 1, c + 1)
+              for a, b, c in f]
+    text = ("\n".join(lines) + "\n").encode()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        md = load_obj(text)
+        best = min(best, time.perf_counter() - t0)
+    ntri = len(md.indices) // 3
+    res = {"obj": {"file": "icosphere L7 as OBJ (v/vt/vn)", "bytes": len(text), "triangles": ntri,
+                   "vertices_out": int(len(md.vertices)), "ms": round(best * 1e3, 1),
+                   "mtris_per_s": round(ntri / best / 1e6, 2), "mb_per_s": round(len(text) / best / 1e6, 1)}}
+
+    rng = np.random.default_rng(3)
+    Wt = Ht = 2048
+    yy, xx = np.mgrid[0:Ht, 0:Wt]
+    img = np.stack([xx % 256, yy % 256, (xx + yy) % 256, np.full_like(xx, 255)], axis=-1)
+    img = (img + rng.integers(0, 8, img.shape)).clip(0, 255).astype(np.uint8)
+    raw = img.reshape(Ht, Wt * 4).astype(np.int16)
+    up = np.vstack([np.zeros((1, Wt * 4), np.int16), raw[:-1]])
+    left = np.hstack([np.zeros((Ht, 4), np.int16), raw[:, :-4]]) if Wt > 1 else np.zeros_like(raw)
+    ul = np.hstack([np.zeros((Ht, 4), np.int16), up[:, :-4]])
+    p = left + up - ul
+    pa, pb, pc = np.abs(p - left), np.abs(p - up), np.abs(p - ul)
+    pred = np.where((pa <= pb) & (pa <= pc), left, np.where(pb <= pc, up, ul))
+    filt = ((raw - pred) & 0xFF).astype(np.uint8)
+    rows = np.hstack([np.full((Ht, 1), 4, np.uint8), filt])
+    import struct
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+
+    png = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", Wt, Ht, 8, 6, 0, 0, 0))
+           + chunk(b"IDAT", zlib.compress(rows.tobytes(), 6)) + chunk(b"IEND", b""))
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        dec = load_image(png)
+        best = min(best, time.perf_counter() - t0)
+    assert np.array_equal(dec, img), "PNG round trip differs"
+    res["png"] = {"file": "2048x2048 RGBA8, Paeth rows", "bytes": len(png), "ms": round(best * 1e3, 1),
+                  "mpixels_per_s": round(Wt * Ht / best / 1e6, 1)}
+    return res
+
 
 def build_secondary(reps, arrays, dev, cam, W, H, torch, with_primary=False):
     """Incoherent secondary rays of the same frame: one pinhole primary ray per

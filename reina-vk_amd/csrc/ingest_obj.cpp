@@ -138,15 +138,19 @@ float fast_atof(const char* c) {
 }
 
 // ---- parsed model (ObjFile::Model) ------------------------------------------
-enum class Prim { Polygon, Line, Point };
+enum class Prim : uint8_t { Polygon, Line, Point };
 
+// a face record: its corners' v / vt / vn indices sit in the mesh's flat
+// index arrays at [first, first + count) (vt / vn may hold fewer)
 struct Face {
     Prim type;
-    std::vector<uint32_t> v, vt, vn;
+    uint32_t nv, nvt, nvn;
+    uint64_t v0, vt0, vn0;
 };
 
 struct Mesh {
     std::vector<Face> faces;
+    std::vector<uint32_t> v, vt, vn;
     int material = -1;
     bool has_normals = false;
     uint64_t uv_refs = 0;
@@ -159,7 +163,6 @@ struct Object {
 
 struct Model {
     std::vector<V3> pos, colors, normals, uvs;
-    unsigned uv_dim = 0;
     std::vector<Object> objects;
     std::vector<Mesh> meshes;
     std::vector<std::string> materials;
@@ -187,19 +190,28 @@ struct Model {
 
 bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\f' || c == '\v'; }
 
-// whitespace-separated tokens of a line, stopping at a comment
-std::vector<std::string> tokens(const std::string& line, size_t from) {
-    std::vector<std::string> out;
+// a word of the current line (the line buffer is NUL-terminated, so number
+// parsing that starts in a word stops at its end at the latest)
+struct Word {
+    const char* p;
+    size_t n;
+    bool is(const char* s) const { return std::strlen(s) == n && std::memcmp(p, s, n) == 0; }
+    std::string str() const { return std::string(p, n); }
+};
+
+// the whitespace-separated words of line[from..], stopping at a comment
+void words(const std::string& line, size_t from, std::vector<Word>& out) {
+    out.clear();
     size_t i = from;
-    while (i < line.size()) {
-        while (i < line.size() && is_space(line[i])) ++i;
-        if (i >= line.size() || line[i] == '#') break;
+    const size_t e = line.size();
+    while (i < e) {
+        while (i < e && is_space(line[i])) ++i;
+        if (i >= e || line[i] == '#') break;
         size_t j = i;
-        while (j < line.size() && !is_space(line[j])) ++j;
-        out.push_back(line.substr(i, j - i));
+        while (j < e && !is_space(line[j])) ++j;
+        out.push_back(Word{line.data() + i, j - i});
         i = j;
     }
-    return out;
 }
 
 std::string rest_of_line(const std::string& line, size_t from) {
@@ -209,57 +221,73 @@ std::string rest_of_line(const std::string& line, size_t from) {
     return line.substr(a, b - a);
 }
 
-bool numeric_token(const std::string& t) {
-    const char c = t[0];
-    return (c >= '0' && c <= '9') || c == '-' || c == '+' || strncasecmp(t.c_str(), "nan", 3) == 0 ||
-           strncasecmp(t.c_str(), "inf", 3) == 0;
+bool numeric_word(const Word& w) {
+    const char c = w.p[0];
+    return (c >= '0' && c <= '9') || c == '-' || c == '+' || (w.n >= 3 && (strncasecmp(w.p, "nan", 3) == 0 ||
+                                                                            strncasecmp(w.p, "inf", 3) == 0));
 }
 
-std::vector<float> numbers(const std::vector<std::string>& tok) {
-    std::vector<float> out;
-    for (const std::string& t : tok)
-        if (numeric_token(t)) out.push_back(fast_atof(t.c_str()));
-    return out;
+// the numeric words of a record (Assimp counts components this way)
+size_t numbers(const std::vector<Word>& ws, float* out, size_t cap) {
+    size_t n = 0;
+    for (const Word& w : ws)
+        if (numeric_word(w)) {
+            const float f = fast_atof(w.p);
+            if (n < cap) out[n] = f;
+            ++n;
+        }
+    return n;
 }
 
 // ObjFileParser::getFace: "v", "v/vt", "v//vn", "v/vt/vn"; negative = relative
-void parse_face(Model& m, const std::vector<std::string>& tok, Prim type) {
-    Face f;
-    f.type = type;
+void parse_face(Model& m, const std::vector<Word>& ws, Prim type) {
     const bool have_vt = !m.uvs.empty(), have_vn = !m.normals.empty();
+    if (m.cur_object < 0) m.create_object("defaultobject");
+    if (m.cur_mesh < 0) m.create_mesh();
+    Mesh& mesh = m.meshes[m.cur_mesh];
+    Face f{type, 0, 0, 0, mesh.v.size(), mesh.vt.size(), mesh.vn.size()};
     bool has_normal = false;
-    for (const std::string& t : tok) {
+    for (const Word& w : ws) {
         int slot = 0;
         size_t i = 0;
-        while (i < t.size()) {
-            if (t[i] == '/') {
+        while (i < w.n) {
+            if (w.p[i] == '/') {
                 ++slot;
                 ++i;
                 continue;
             }
-            char* end = nullptr;
-            const long val = std::strtol(t.c_str() + i, &end, 10);
-            if (end == t.c_str() + i) throw Error{"OBJ: invalid face index"};
-            i = (size_t)(end - t.c_str());
+            // a signed decimal integer
+            size_t k = i;
+            const bool neg = w.p[k] == '-';
+            if (neg || w.p[k] == '+') ++k;
+            if (k >= w.n || w.p[k] < '0' || w.p[k] > '9') throw Error{"OBJ: invalid face index"};
+            long val = 0;
+            for (; k < w.n && w.p[k] >= '0' && w.p[k] <= '9'; ++k) {
+                val = val * 10 + (w.p[k] - '0');
+                if (val > (1l << 40)) throw Error{"OBJ: face index out of range"};
+            }
+            if (neg) val = -val;
+            i = k;
             int s = slot;
             if (s == 1 && !have_vt && have_vn) s = 2;   // "v/vn" in files without texture coordinates
             if (val == 0) throw Error{"OBJ: invalid face index"};
             const size_t count = s == 0 ? m.pos.size() : s == 1 ? m.uvs.size() : m.normals.size();
             const long idx = val > 0 ? val - 1 : (long)count + val;
             if (idx < 0) throw Error{"OBJ: relative face index before the first element"};
-            if (s == 0) f.v.push_back((uint32_t)idx);
-            else if (s == 1) f.vt.push_back((uint32_t)idx);
-            else if (s == 2) { f.vn.push_back((uint32_t)idx); has_normal = true; }
+            if (s == 0) { mesh.v.push_back((uint32_t)idx); ++f.nv; }
+            else if (s == 1) { mesh.vt.push_back((uint32_t)idx); ++f.nvt; }
+            else if (s == 2) { mesh.vn.push_back((uint32_t)idx); ++f.nvn; has_normal = true; }
             else throw Error{"OBJ: too many indices in a face element"};
         }
     }
-    if (f.v.empty()) return;   // "Obj: Ignoring empty face"
-    if (m.cur_object < 0) m.create_object("defaultobject");
-    if (m.cur_mesh < 0) m.create_mesh();
-    Mesh& mesh = m.meshes[m.cur_mesh];
-    mesh.uv_refs += f.vt.size();
+    if (f.nv == 0) {   // "Obj: Ignoring empty face"
+        mesh.vt.resize(f.vt0);
+        mesh.vn.resize(f.vn0);
+        return;
+    }
+    mesh.uv_refs += f.nvt;
     if (has_normal) mesh.has_normals = true;
-    mesh.faces.push_back(std::move(f));
+    mesh.faces.push_back(f);
 }
 
 // ObjFileParser::getMaterialDesc: a new mesh when the current one has faces
@@ -283,9 +311,11 @@ Model parse(const char* text, uint64_t len) {
     Model m;
     uint64_t pos = 0;
     std::string line;
+    std::vector<Word> ws;
+    float n[8];
     while (pos < len) {
-        uint64_t e = pos;
-        while (e < len && text[e] != '\n') ++e;
+        const char* nl = static_cast<const char*>(std::memchr(text + pos, '\n', len - pos));
+        const uint64_t e = nl ? (uint64_t)(nl - text) : len;
         line.assign(text + pos, text + e);
         pos = e + 1;
         size_t i = 0;
@@ -293,59 +323,58 @@ Model parse(const char* text, uint64_t len) {
         if (i >= line.size()) continue;
         size_t j = i;
         while (j < line.size() && !is_space(line[j])) ++j;
-        const std::string key = line.substr(i, j - i);
-        if (key == "v") {
-            const std::vector<float> n = numbers(tokens(line, j));
-            if (n.size() == 3) {
+        const Word key{line.data() + i, j - i};
+        if (key.is("v")) {
+            words(line, j, ws);
+            const size_t c = numbers(ws, n, 8);
+            if (c == 3) {
                 m.pos.push_back(v3(n[0], n[1], n[2]));
-            } else if (n.size() == 4) {
+            } else if (c == 4) {
                 if (n[3] == 0.0f) throw Error{"OBJ: invalid component in homogeneous vector (division by zero)"};
                 m.pos.push_back(v3(n[0] / n[3], n[1] / n[3], n[2] / n[3]));
-            } else if (n.size() == 6) {
+            } else if (c == 6) {
                 if (m.colors.size() < m.pos.size()) m.colors.resize(m.pos.size());   // earlier vertices: black
                 m.pos.push_back(v3(n[0], n[1], n[2]));
                 m.colors.push_back(v3(n[3], n[4], n[5]));
             }
             // other component counts add no vertex (as Assimp's parser)
             if (!m.colors.empty() && m.colors.size() < m.pos.size()) m.colors.resize(m.pos.size());
-        } else if (key == "vt") {
-            const std::vector<float> n = numbers(tokens(line, j));
-            if (n.size() != 2 && n.size() != 3) throw Error{"OBJ: invalid number of components in a texture coordinate"};
-            float x = n[0], y = n[1], z = n.size() == 3 ? n[2] : 0.0f;
+        } else if (key.is("vt")) {
+            words(line, j, ws);
+            const size_t c = numbers(ws, n, 8);
+            if (c != 2 && c != 3) throw Error{"OBJ: invalid number of components in a texture coordinate"};
+            float x = n[0], y = n[1], z = c == 3 ? n[2] : 0.0f;
             if (!std::isfinite(x)) x = 0.0f;   // nan and inf coerced to 0
             if (!std::isfinite(y)) y = 0.0f;
             if (!std::isfinite(z)) z = 0.0f;
             m.uvs.push_back(v3(x, y, z));
-            m.uv_dim = std::max(m.uv_dim, (unsigned)n.size());
-        } else if (key == "vn") {
-            const std::vector<std::string> t = tokens(line, j);
-            if (t.size() < 3) throw Error{"OBJ: normal with fewer than 3 components"};
-            m.normals.push_back(v3(fast_atof(t[0].c_str()), fast_atof(t[1].c_str()), fast_atof(t[2].c_str())));
-        } else if (key == "f") {
-            parse_face(m, tokens(line, j), Prim::Polygon);
-        } else if (key == "l") {
-            parse_face(m, tokens(line, j), Prim::Line);
-        } else if (key == "p") {
-            parse_face(m, tokens(line, j), Prim::Point);
-        } else if (key == "o") {
+        } else if (key.is("vn")) {
+            words(line, j, ws);
+            if (ws.size() < 3) throw Error{"OBJ: normal with fewer than 3 components"};
+            m.normals.push_back(v3(fast_atof(ws[0].p), fast_atof(ws[1].p), fast_atof(ws[2].p)));
+        } else if (key.is("f") || key.is("l") || key.is("p")) {
+            words(line, j, ws);
+            parse_face(m, ws, key.is("f") ? Prim::Polygon : key.is("l") ? Prim::Line : Prim::Point);
+        } else if (key.is("o")) {
             // the first word names the object; an existing name makes that
             // object current again
-            const std::vector<std::string> t = tokens(line, j);
-            if (!t.empty()) {
+            words(line, j, ws);
+            if (!ws.empty()) {
+                const std::string name = ws[0].str();
                 int found = -1;
                 for (size_t k = 0; k < m.objects.size() && found < 0; ++k)
-                    if (m.objects[k].name == t[0]) found = (int)k;
+                    if (m.objects[k].name == name) found = (int)k;
                 if (found >= 0) m.cur_object = found;
-                else m.create_object(t[0]);
+                else m.create_object(name);
             }
-        } else if (key == "g") {
+        } else if (key.is("g")) {
             // groups map to objects; the name is the rest of the line
             const std::string name = rest_of_line(line, j);
             if (!name.empty() && name != m.active_group) {
                 m.create_object(name);
                 m.active_group = name;
             }
-        } else if (key == "usemtl") {
+        } else if (key.is("usemtl")) {
             use_material(m, rest_of_line(line, j));
         }
         // mtllib, s, comments and unknown records do not change the geometry
@@ -358,16 +387,24 @@ struct Verbose {
     std::vector<V3> pos, nrm, uv, col, tan, bit;
     std::vector<float> col_a;   // colour alpha: 1 where copied from the file
     bool has_n = false, has_uv = false, has_col = false, has_tan = false;
-    std::vector<std::vector<uint32_t>> faces;
+    // faces: face k's corners are idx[start[k] .. start[k+1])
+    std::vector<uint32_t> idx;
+    std::vector<uint64_t> start{0};
     bool has_polys = false;   // triangles or larger polygons
+    size_t num_faces() const { return start.size() - 1; }
+    size_t face_size(size_t k) const { return (size_t)(start[k + 1] - start[k]); }
+    const uint32_t* face(size_t k) const { return idx.data() + start[k]; }
+    void add_face(std::initializer_list<uint32_t> c) {
+        idx.insert(idx.end(), c);
+        start.push_back(idx.size());
+    }
 };
 
 // ObjFileImporter::createVertexArray for one mesh
 Verbose build_mesh(const Model& m, const Mesh& mesh) {
     Verbose o;
     uint64_t nv = 0;
-    for (const Face& f : mesh.faces)
-        nv += f.type == Prim::Line ? 2 * (f.v.size() - 1) : f.v.size();
+    for (const Face& f : mesh.faces) nv += f.type == Prim::Line ? 2ull * (f.nv - 1) : f.nv;
     o.has_n = !m.normals.empty() && mesh.has_normals;
     o.has_uv = !m.uvs.empty() && mesh.uv_refs != 0;
     o.has_col = !m.colors.empty();
@@ -378,58 +415,63 @@ Verbose build_mesh(const Model& m, const Mesh& mesh) {
         o.col.resize(nv);
         o.col_a.resize(nv, 0.0f);
     }
+    o.idx.reserve(nv);
+    o.start.reserve(mesh.faces.size() + 1);
     bool normals_ok = true, uv_ok = true;
     uint32_t next = 0;
     for (const Face& f : mesh.faces) {
-        const size_t k = f.v.size();
+        const size_t k = f.nv;
         if (f.type == Prim::Polygon && k >= 3) o.has_polys = true;
-        if (f.type == Prim::Polygon) o.faces.emplace_back();
+        const uint64_t line_first = o.start.size() - 1;   // first segment face of a polyline
         for (size_t c = 0; c < k; ++c) {
-            const uint32_t vi = f.v[c];
+            const uint32_t vi = mesh.v[f.v0 + c];
             if (vi >= m.pos.size()) throw Error{"OBJ: vertex index out of range"};
             if (!std::isfinite(m.pos[vi].x) || !std::isfinite(m.pos[vi].y) || !std::isfinite(m.pos[vi].z))
                 throw Error{"OBJ: non-finite vertex position"};
             o.pos[next] = m.pos[vi];
-            if (o.has_n && normals_ok && c < f.vn.size()) {
-                if (f.vn[c] >= m.normals.size()) normals_ok = false;
-                else o.nrm[next] = m.normals[f.vn[c]];
+            if (o.has_n && normals_ok && c < f.nvn) {
+                const uint32_t ni = mesh.vn[f.vn0 + c];
+                if (ni >= m.normals.size()) normals_ok = false;
+                else o.nrm[next] = m.normals[ni];
             }
             if (o.has_col) {
                 o.col[next] = m.colors[vi];
                 o.col_a[next] = 1.0f;
             }
-            if (o.has_uv && uv_ok && c < f.vt.size()) {
-                if (f.vt[c] >= m.uvs.size()) uv_ok = false;
-                else o.uv[next] = m.uvs[f.vt[c]];
+            if (o.has_uv && uv_ok && c < f.nvt) {
+                const uint32_t ti = mesh.vt[f.vt0 + c];
+                if (ti >= m.uvs.size()) uv_ok = false;
+                else o.uv[next] = m.uvs[ti];
             }
             if (f.type == Prim::Polygon) {
-                o.faces.back().push_back(next);
+                o.idx.push_back(next);
             } else if (f.type == Prim::Point) {
-                o.faces.push_back({next});
+                o.add_face({next});
             } else {
                 // a polyline: segment c starts at this corner; inner corners
                 // are duplicated so each segment owns its two vertices
                 const bool last = c + 1 == k;
-                if (!last) o.faces.push_back({next, 0u});
+                if (!last) o.add_face({next, 0u});
                 if (c > 0) {
                     if (!last) {
                         o.pos[next + 1] = o.pos[next];
-                        if (o.has_n && !f.vn.empty()) o.nrm[next + 1] = o.nrm[next];
+                        if (o.has_n && f.nvn) o.nrm[next + 1] = o.nrm[next];
                         if (o.has_uv) o.uv[next + 1] = o.uv[next];
                         ++next;
                     }
-                    o.faces[o.faces.size() - (last ? 1 : 2)][1] = next;
+                    const uint64_t seg = line_first + c - 1;   // the segment ending here
+                    o.idx[o.start[seg] + 1] = next;
                 }
             }
             ++next;
         }
+        if (f.type == Prim::Polygon) o.start.push_back(o.idx.size());
     }
     return o;
 }
 
 // TriangulateProcess for n > 4: ear clipping in the polygon's best-fit plane
-void clip_ears(const Verbose& o, const std::vector<uint32_t>& poly, std::vector<std::vector<uint32_t>>& out) {
-    const size_t n = poly.size();
+void clip_ears(const Verbose& o, const uint32_t* poly, size_t n, Verbose& out) {
     V3 nrm;   // Newell normal
     for (size_t i = 0; i < n; ++i) {
         const V3 a = o.pos[poly[i]], b = o.pos[poly[(i + 1) % n]];
@@ -465,7 +507,7 @@ void clip_ears(const Verbose& o, const std::vector<uint32_t>& poly, std::vector<
             if (area2(a, b, p) >= 0.0f && area2(b, c, p) >= 0.0f && area2(c, a, p) >= 0.0f) ear = false;
         }
         if (ear || guard > m) {   // no ear in a full cycle: degenerate polygon, clip anyway
-            out.push_back({poly[a], poly[b], poly[c]});
+            out.add_face({poly[a], poly[b], poly[c]});
             ring.erase(ring.begin() + (long)(i % m));
             guard = 0;
             if (i >= ring.size()) i = 0;
@@ -474,34 +516,39 @@ void clip_ears(const Verbose& o, const std::vector<uint32_t>& poly, std::vector<
             ++guard;
         }
     }
-    out.push_back({poly[ring[0]], poly[ring[1]], poly[ring[2]]});
+    out.add_face({poly[ring[0]], poly[ring[1]], poly[ring[2]]});
 }
 
 void triangulate(Verbose& o) {
-    std::vector<std::vector<uint32_t>> out;
-    out.reserve(o.faces.size() * 2);
-    for (const std::vector<uint32_t>& f : o.faces) {
-        if (f.size() <= 3) {
-            out.push_back(f);
-        } else if (f.size() == 4) {
-            unsigned start = 0;
+    Verbose out;   // only its face arrays are used
+    out.idx.reserve(o.idx.size() * 3 / 2);
+    out.start.reserve(o.num_faces() * 2 + 1);
+    for (size_t k = 0; k < o.num_faces(); ++k) {
+        const uint32_t* f = o.face(k);
+        const size_t n = o.face_size(k);
+        if (n <= 3) {
+            out.idx.insert(out.idx.end(), f, f + n);
+            out.start.push_back(out.idx.size());
+        } else if (n == 4) {
+            unsigned s = 0;
             for (unsigned i = 0; i < 4; ++i) {
                 const V3 p0 = o.pos[f[(i + 3) % 4]], p1 = o.pos[f[(i + 2) % 4]], p2 = o.pos[f[(i + 1) % 4]];
                 const V3 p = o.pos[f[i]];
                 const V3 left = normalize(p0 - p), diag = normalize(p1 - p), right = normalize(p2 - p);
                 const float angle = std::acos(dot(left, diag)) + std::acos(dot(right, diag));
                 if (angle > 3.1415926538f) {   // the concave corner
-                    start = i;
+                    s = i;
                     break;
                 }
             }
-            out.push_back({f[start], f[(start + 1) % 4], f[(start + 2) % 4]});
-            out.push_back({f[start], f[(start + 2) % 4], f[(start + 3) % 4]});
+            out.add_face({f[s], f[(s + 1) % 4], f[(s + 2) % 4]});
+            out.add_face({f[s], f[(s + 2) % 4], f[(s + 3) % 4]});
         } else {
-            clip_ears(o, f, out);
+            clip_ears(o, f, n, out);
         }
     }
-    o.faces.swap(out);
+    o.idx.swap(out.idx);
+    o.start.swap(out.start);
 }
 
 // ComputePositionEpsilon: 1e-4 * |max - min| over all positions
@@ -514,39 +561,101 @@ float position_epsilon(const Verbose& o) {
     return length(hi - lo) * 1e-4f;
 }
 
-// all corners strictly within r of a position, ascending (a uniform grid of
-// cell size r: a neighbour lies in one of the 27 cells around the query)
+// open-addressing table from a 3-part key to the head of an intrusive list
+// (next[] links the entries; a new entry becomes the head)
+template <class Key>
+struct ListTable {
+    struct Slot {
+        Key key;
+        uint32_t head;
+        bool used;
+    };
+    std::vector<Slot> slots;
+    uint64_t mask;
+    explicit ListTable(size_t n) {
+        size_t cap = 16;
+        while (cap < 2 * n + 16) cap <<= 1;
+        slots.assign(cap, Slot{Key{}, 0u, false});
+        mask = cap - 1;
+    }
+    static uint64_t hash(const Key& k) {
+        uint64_t h = 0x9E3779B97F4A7C15ull;
+        for (int i = 0; i < 3; ++i) {
+            h ^= (uint64_t)k.c[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+            h *= 0xBF58476D1CE4E5B9ull;
+        }
+        return h ^ (h >> 31);
+    }
+    // the slot of key (inserting it with head = none when absent)
+    Slot& at(const Key& k) {
+        for (uint64_t i = hash(k) & mask;; i = (i + 1) & mask) {
+            Slot& s = slots[i];
+            if (!s.used) {
+                s.used = true;
+                s.key = k;
+                s.head = 0xFFFFFFFFu;
+                return s;
+            }
+            if (s.key == k) return s;
+        }
+    }
+    const Slot* find(const Key& k) const {
+        for (uint64_t i = hash(k) & mask;; i = (i + 1) & mask) {
+            const Slot& s = slots[i];
+            if (!s.used) return nullptr;
+            if (s.key == k) return &s;
+        }
+    }
+};
+
+struct CellKey {
+    int64_t c[3];
+    bool operator==(const CellKey& o) const { return c[0] == o.c[0] && c[1] == o.c[1] && c[2] == o.c[2]; }
+};
+
+// all corners strictly within r of a position, ascending: a uniform grid of
+// cell size 16 r, so a query ball touches 1 to 8 cells
 struct Neighbours {
     float r, r2;
-    std::unordered_map<uint64_t, std::vector<uint32_t>> cells;
+    double cell;
     const std::vector<V3>& pos;
-    static int64_t cell_of(float x, float r) {
-        const double c = std::floor((double)x / (double)r);
+    ListTable<CellKey> table;
+    std::vector<uint32_t> next;
+    static int64_t coord(double x, double cell) {
+        const double c = std::floor(x / cell);
         return (int64_t)std::max(-4.0e18, std::min(4.0e18, c));
     }
-    static uint64_t key(int64_t x, int64_t y, int64_t z) {
-        return ((uint64_t)(x & 0x1FFFFF) << 42) | ((uint64_t)(y & 0x1FFFFF) << 21) | (uint64_t)(z & 0x1FFFFF);
-    }
-    Neighbours(const std::vector<V3>& p, float radius) : r(radius), r2(radius * radius), pos(p) {
-        if (!(r > 0.0f) || !std::isfinite(r)) r = 1.0f;   // a single point (or no extent): one cell
-        for (uint32_t i = 0; i < (uint32_t)pos.size(); ++i)
-            cells[key(cell_of(pos[i].x, r), cell_of(pos[i].y, r), cell_of(pos[i].z, r))].push_back(i);
+    Neighbours(const std::vector<V3>& p, float radius)
+        : r(radius), r2(radius * radius), pos(p), table(p.size()), next(p.size()) {
+        cell = (r > 0.0f && std::isfinite(r)) ? 16.0 * (double)r : 1.0;
+        for (uint32_t i = 0; i < (uint32_t)pos.size(); ++i) {
+            auto& s = table.at(CellKey{{coord(pos[i].x, cell), coord(pos[i].y, cell), coord(pos[i].z, cell)}});
+            next[i] = s.head;
+            s.head = i;
+        }
     }
     void find(V3 q, std::vector<uint32_t>& out) const {
         out.clear();
-        const int64_t cx = cell_of(q.x, r), cy = cell_of(q.y, r), cz = cell_of(q.z, r);
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    auto it = cells.find(key(cx + dx, cy + dy, cz + dz));
-                    if (it == cells.end()) continue;
-                    for (uint32_t j : it->second) {
+        // a neighbour differs from q by less than r (slightly more after the
+        // float rounding of its squared distance) in every coordinate
+        const double m = 1.01 * (double)r;
+        int64_t lo[3], hi[3];
+        const double qc[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = coord(qc[a] - m, cell);
+            hi[a] = coord(qc[a] + m, cell);
+        }
+        for (int64_t x = lo[0]; x <= hi[0]; ++x)
+            for (int64_t y = lo[1]; y <= hi[1]; ++y)
+                for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+                    const auto* s = table.find(CellKey{{x, y, z}});
+                    if (!s) continue;
+                    for (uint32_t j = s->head; j != 0xFFFFFFFFu; j = next[j]) {
                         const V3 d = pos[j] - q;
                         if (d.x * d.x + d.y * d.y + d.z * d.z < r2) out.push_back(j);
                     }
                 }
         std::sort(out.begin(), out.end());
-        out.erase(std::unique(out.begin(), out.end()), out.end());
     }
 };
 
@@ -555,14 +664,16 @@ void gen_smooth_normals(Verbose& o) {
     if (o.has_n || !o.has_polys) return;
     const float qnan = std::numeric_limits<float>::quiet_NaN();
     std::vector<V3> face_n(o.pos.size());
-    for (const std::vector<uint32_t>& f : o.faces) {
-        if (f.size() < 3) {
-            for (uint32_t i : f) face_n[i] = v3(qnan, qnan, qnan);
+    for (size_t k = 0; k < o.num_faces(); ++k) {
+        const uint32_t* f = o.face(k);
+        const size_t n = o.face_size(k);
+        if (n < 3) {
+            for (size_t c = 0; c < n; ++c) face_n[f[c]] = v3(qnan, qnan, qnan);
             continue;
         }
-        const V3 p1 = o.pos[f[0]], p2 = o.pos[f[1]], p3 = o.pos[f[f.size() - 1]];
-        const V3 n = normalize_safe(cross(p2 - p1, p3 - p1));
-        for (uint32_t i : f) face_n[i] = n;
+        const V3 p1 = o.pos[f[0]], p2 = o.pos[f[1]], p3 = o.pos[f[n - 1]];
+        const V3 nv = normalize_safe(cross(p2 - p1, p3 - p1));
+        for (size_t c = 0; c < n; ++c) face_n[f[c]] = nv;
     }
     const float eps = position_epsilon(o);
     Neighbours nb(o.pos, eps);
@@ -592,12 +703,14 @@ void calc_tangents(Verbose& o) {
     o.tan.assign(nv, V3{});
     o.bit.assign(nv, V3{});
     std::vector<bool> done(nv, false);
-    for (const std::vector<uint32_t>& f : o.faces) {
-        if (f.size() < 3) {
-            for (uint32_t i : f) {
-                done[i] = true;
-                o.tan[i] = v3(qnan, qnan, qnan);
-                o.bit[i] = v3(qnan, qnan, qnan);
+    for (size_t k = 0; k < o.num_faces(); ++k) {
+        const uint32_t* f = o.face(k);
+        const size_t n = o.face_size(k);
+        if (n < 3) {
+            for (size_t c = 0; c < n; ++c) {
+                done[f[c]] = true;
+                o.tan[f[c]] = v3(qnan, qnan, qnan);
+                o.bit[f[c]] = v3(qnan, qnan, qnan);
             }
             continue;
         }
@@ -612,17 +725,18 @@ void calc_tangents(Verbose& o) {
         }
         const V3 tangent = v3((w.x * sy - v.x * ty) * dir, (w.y * sy - v.y * ty) * dir, (w.z * sy - v.z * ty) * dir);
         const V3 bitangent = v3((w.x * sx - v.x * tx) * dir, (w.y * sx - v.y * tx) * dir, (w.z * sx - v.z * tx) * dir);
-        for (uint32_t p : f) {
-            const V3 n = o.nrm[p];
-            V3 lt = tangent - n * dot(tangent, n);
-            V3 lb = (bitangent - n * dot(bitangent, n)) - lt * dot(bitangent, lt);
+        for (size_t c = 0; c < n; ++c) {
+            const uint32_t p = f[c];
+            const V3 nn = o.nrm[p];
+            V3 lt = tangent - nn * dot(tangent, nn);
+            V3 lb = (bitangent - nn * dot(bitangent, nn)) - lt * dot(bitangent, lt);
             lt = normalize_safe(lt);
             lb = normalize_safe(lb);
             const bool bad_t = special(lt.x) || special(lt.y) || special(lt.z);
             const bool bad_b = special(lb.x) || special(lb.y) || special(lb.z);
             if (bad_t != bad_b) {
-                if (bad_t) lt = normalize_safe(cross(n, lb));
-                else lb = normalize_safe(cross(lt, n));
+                if (bad_t) lt = normalize_safe(cross(nn, lb));
+                else lb = normalize_safe(cross(lt, nn));
             }
             o.tan[p] = lt;
             o.bit[p] = lb;
@@ -662,6 +776,11 @@ void calc_tangents(Verbose& o) {
     o.has_tan = true;
 }
 
+struct PosKey {
+    uint32_t c[3];
+    bool operator==(const PosKey& o) const { return c[0] == o.c[0] && c[1] == o.c[1] && c[2] == o.c[2]; }
+};
+
 // JoinVerticesProcess
 void join_vertices(Verbose& o) {
     const size_t nv = o.pos.size();
@@ -682,36 +801,34 @@ void join_vertices(Verbose& o) {
         }
         return true;
     };
-    // candidates share the position's hash (+0 and -0 hash alike); the most
-    // recently added unique vertex is tried first
+    // candidates share the position (+0 and -0 alike); the most recently
+    // added unique vertex is tried first
     const auto bits = [](float f) {
         uint32_t u;
         std::memcpy(&u, &f, 4);
         return f == 0.0f ? 0u : u;
     };
-    std::unordered_map<uint64_t, std::vector<uint32_t>> by_pos;
-    std::vector<uint32_t> unique, remap(nv);
+    ListTable<PosKey> table(nv);
+    std::vector<uint32_t> unique, remap(nv), next;
     unique.reserve(nv);
+    next.reserve(nv);
     for (uint32_t a = 0; a < (uint32_t)nv; ++a) {
         const V3 p = o.pos[a];
-        const uint64_t k = ((uint64_t)bits(p.x) * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)bits(p.y) << 32 | bits(p.z));
-        std::vector<uint32_t>& cand = by_pos[k];
-        int64_t match = -1;
-        for (size_t c = cand.size(); c-- > 0;) {
-            const uint32_t u = unique[cand[c]];
-            if (bits(o.pos[u].x) != bits(p.x) || bits(o.pos[u].y) != bits(p.y) || bits(o.pos[u].z) != bits(p.z))
-                continue;
-            if (equal(a, u)) {
-                match = cand[c];
+        auto& s = table.at(PosKey{{bits(p.x), bits(p.y), bits(p.z)}});
+        uint32_t match = 0xFFFFFFFFu;
+        for (uint32_t u = s.head; u != 0xFFFFFFFFu; u = next[u])
+            if (equal(a, unique[u])) {
+                match = u;
                 break;
             }
-        }
-        if (match < 0) {
-            remap[a] = (uint32_t)unique.size();
-            cand.push_back((uint32_t)unique.size());
+        if (match == 0xFFFFFFFFu) {
+            const uint32_t u = (uint32_t)unique.size();
+            remap[a] = u;
             unique.push_back(a);
+            next.push_back(s.head);
+            s.head = u;
         } else {
-            remap[a] = (uint32_t)match;
+            remap[a] = match;
         }
     }
     const auto gather = [&](std::vector<V3>& arr) {
@@ -731,8 +848,7 @@ void join_vertices(Verbose& o) {
         for (size_t i = 0; i < unique.size(); ++i) out[i] = o.col_a[unique[i]];
         o.col_a.swap(out);
     }
-    for (std::vector<uint32_t>& f : o.faces)
-        for (uint32_t& i : f) i = remap[i];
+    for (uint32_t& i : o.idx) i = remap[i];
 }
 
 template <class T>
@@ -778,7 +894,7 @@ ReinaModelData* load(const char* text, uint64_t len) {
         }
         if (o.has_uv) uvs.insert(uvs.end(), {o.uv[i].x, o.uv[i].y});
     }
-    for (const std::vector<uint32_t>& f : o.faces) idx.insert(idx.end(), f.begin(), f.end());
+    idx = o.idx;
 
     ReinaModelData* r = static_cast<ReinaModelData*>(std::calloc(1, sizeof(ReinaModelData)));
     if (!r) throw std::bad_alloc();

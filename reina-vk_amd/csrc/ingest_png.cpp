@@ -94,26 +94,35 @@ int paeth(int a, int b, int c) {
 // their filter bytes
 std::vector<uint8_t> unfilter(const uint8_t* data, uint64_t rows, uint64_t rb, unsigned bpp) {
     std::vector<uint8_t> out(rows * rb);
+    const std::vector<uint8_t> zero(rb, 0);
     for (uint64_t y = 0; y < rows; ++y) {
-        const uint8_t* src = data + y * (rb + 1);
-        const unsigned f = src[0];
-        ++src;
+        const uint8_t* src = data + y * (rb + 1) + 1;
+        const unsigned f = src[-1];
         uint8_t* cur = out.data() + y * rb;
-        const uint8_t* prev = y ? out.data() + (y - 1) * rb : nullptr;
-        for (uint64_t i = 0; i < rb; ++i) {
-            const int a = i >= bpp ? cur[i - bpp] : 0;
-            const int b = prev ? prev[i] : 0;
-            const int c = (prev && i >= bpp) ? prev[i - bpp] : 0;
-            int v;
-            switch (f) {
-                case 0: v = src[i]; break;
-                case 1: v = src[i] + a; break;
-                case 2: v = src[i] + b; break;
-                case 3: v = src[i] + ((a + b) >> 1); break;
-                case 4: v = src[i] + paeth(a, b, c); break;
-                default: throw Error{REINA_RT_ERR_INVALID_ARG, "PNG: invalid filter type"};
-            }
-            cur[i] = (uint8_t)v;
+        const uint8_t* prev = y ? out.data() + (y - 1) * rb : zero.data();
+        const uint64_t head = std::min<uint64_t>(bpp, rb);
+        switch (f) {
+            case 0:
+                std::memcpy(cur, src, rb);
+                break;
+            case 1:
+                std::memcpy(cur, src, head);
+                for (uint64_t i = bpp; i < rb; ++i) cur[i] = (uint8_t)(src[i] + cur[i - bpp]);
+                break;
+            case 2:
+                for (uint64_t i = 0; i < rb; ++i) cur[i] = (uint8_t)(src[i] + prev[i]);
+                break;
+            case 3:
+                for (uint64_t i = 0; i < head; ++i) cur[i] = (uint8_t)(src[i] + (prev[i] >> 1));
+                for (uint64_t i = bpp; i < rb; ++i) cur[i] = (uint8_t)(src[i] + ((cur[i - bpp] + prev[i]) >> 1));
+                break;
+            case 4:
+                for (uint64_t i = 0; i < head; ++i) cur[i] = (uint8_t)(src[i] + prev[i]);   // paeth(0, b, 0) = b
+                for (uint64_t i = bpp; i < rb; ++i)
+                    cur[i] = (uint8_t)(src[i] + paeth(cur[i - bpp], prev[i], prev[i - bpp]));
+                break;
+            default:
+                throw Error{REINA_RT_ERR_INVALID_ARG, "PNG: invalid filter type"};
         }
     }
     return out;
@@ -239,6 +248,20 @@ ReinaImage* decode(const uint8_t* p, uint64_t n, bool flip) {
         for (uint64_t y = 0; y < ph; ++y) {
             const uint8_t* row = rows.data() + y * rb;
             const uint64_t oy = ps.y0 + y * ps.dy;
+            if (hd.depth == 8 && !have_key && hd.color != 3 && !hd.interlace) {
+                // 8-bit samples without a colour key: direct expansion
+                uint8_t* o = rgba.data() + oy * hd.w * 4;
+                if (hd.color == 6) {
+                    std::memcpy(o, row, (size_t)pw * 4);
+                } else {
+                    for (uint64_t x = 0; x < pw; ++x, o += 4) {
+                        if (hd.color == 2) { o[0] = row[3 * x]; o[1] = row[3 * x + 1]; o[2] = row[3 * x + 2]; o[3] = 255; }
+                        else if (hd.color == 0) { o[0] = o[1] = o[2] = row[x]; o[3] = 255; }
+                        else { o[0] = o[1] = o[2] = row[2 * x]; o[3] = row[2 * x + 1]; }
+                    }
+                }
+                continue;
+            }
             for (uint64_t x = 0; x < pw; ++x) {
                 const uint64_t ox = ps.x0 + x * ps.dx;
                 uint8_t* o = rgba.data() + (oy * hd.w + ox) * 4;

@@ -91,6 +91,8 @@ struct SceneFile {
     std::vector<ReinaInstanceProperties> props;
     std::vector<ReinaModelRange> models;
     std::vector<ReinaInstanceDesc> instances;
+    std::vector<std::vector<uint8_t>> texels;   // RGBA8 per texture
+    std::vector<ReinaTextureDesc> textures;
 };
 
 template <typename T>
@@ -110,7 +112,8 @@ SceneFile load_scene(const std::string& path) {
     uint32_t version = 0;
     f.read(magic, 4);
     f.read(reinterpret_cast<char*>(&version), 4);
-    if (!f || std::memcmp(magic, "RSCN", 4) != 0 || version != 1) throw std::runtime_error("not a scene file");
+    if (!f || std::memcmp(magic, "RSCN", 4) != 0 || (version != 1 && version != 2))
+        throw std::runtime_error("not a scene file");
     SceneFile s;
     read_array(f, s.vertices, 16);
     read_array(f, s.indices, 4);
@@ -121,6 +124,23 @@ SceneFile load_scene(const std::string& path) {
     read_array(f, s.props, sizeof(ReinaInstanceProperties));
     read_array(f, s.models, sizeof(ReinaModelRange));
     read_array(f, s.instances, sizeof(ReinaInstanceDesc));
+    if (version >= 2) {   // textures: count, then width, height, RGBA8 texels each
+        uint64_t n = 0;
+        f.read(reinterpret_cast<char*>(&n), 8);
+        if (!f || n > (1u << 20)) throw std::runtime_error("scene file: bad texture count");
+        s.texels.resize(n);
+        s.textures.resize(n);
+        for (uint64_t t = 0; t < n; ++t) {
+            uint32_t wh[2] = {0, 0};
+            f.read(reinterpret_cast<char*>(wh), 8);
+            if (!f || wh[0] == 0 || wh[1] == 0 || wh[0] > (1u << 16) || wh[1] > (1u << 16))
+                throw std::runtime_error("scene file: bad texture size");
+            s.texels[t].resize((size_t)wh[0] * wh[1] * 4);
+            f.read(reinterpret_cast<char*>(s.texels[t].data()), (std::streamsize)s.texels[t].size());
+            if (!f) throw std::runtime_error("scene file: truncated");
+            s.textures[t] = ReinaTextureDesc{wh[0], wh[1], s.texels[t].data()};
+        }
+    }
     return s;
 }
 
@@ -272,6 +292,8 @@ int run(int argc, char** argv) {
     d.props = sf.props.data(); d.num_props = (uint32_t)sf.props.size();
     d.models = sf.models.data(); d.num_models = (uint32_t)sf.models.size();
     d.instances = sf.instances.data(); d.num_instances = (uint32_t)sf.instances.size();
+    d.textures = sf.textures.empty() ? nullptr : sf.textures.data();
+    d.num_textures = (uint32_t)sf.textures.size();
     ReinaScene* scene = nullptr;
     check(reina_rt_scene_create(&d, &scene), "reina_rt_scene_create");
     if (cfg.num("headless.nee", 0) != 0) check(reina_rt_set_trace_options(scene, REINA_RT_TRACE_NEE), "options");

@@ -332,14 +332,22 @@ SCENE_FILE_ARRAYS = ("vertices", "indices", "tbns", "tbn_indices", "tex_coords",
 
 def save_scene(arrays, path):
     """Writes the packed buffers of build_arrays() (the ReinaSceneDesc
-    arrays, in its field order) as: "RSCN", uint32 version 1, then per array
+    arrays, in its field order) as: "RSCN", uint32 version 2, then per array
     uint64 element count and the raw little-endian elements (vertices 16 B,
     indices 4 B, tbns 36 B, tbn / tex indices 4 B, tex coords 8 B, props 120 B,
-    model ranges 32 B, instances 80 B)."""
+    model ranges 32 B, instances 80 B), then uint64 texture count and per
+    texture uint32 width, uint32 height and width * height RGBA8 texels (row 0
+    first). Version 1 files (no texture section) are still read."""
     import struct
     with open(path, "wb") as f:
-        f.write(SCENE_FILE_MAGIC + struct.pack("<I", 1))
+        f.write(SCENE_FILE_MAGIC + struct.pack("<I", 2))
         for name in SCENE_FILE_ARRAYS:
             a = np.ascontiguousarray(arrays[name])
             f.write(struct.pack("<Q", len(a)))
             f.write(a.tobytes())
+        tex = arrays.get("textures") or []
+        f.write(struct.pack("<Q", len(tex)))
+        for t in tex:
+            t = np.ascontiguousarray(t, np.uint8)
+            f.write(struct.pack("<II", t.shape[1], t.shape[0]))
+            f.write(t.tobytes())

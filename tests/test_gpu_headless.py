@@ -48,10 +48,20 @@ total_samples = 12
 """
 
 
-def test_headless_host_pngs_match_binding(cuda, tmp_path):
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
+
+
+@pytest.mark.parametrize("which", ["cornell", "obj_textured"])
+def test_headless_host_pngs_match_binding(cuda, tmp_path, which):
+    """The scene file carries the packed arrays and (version 2) the textures:
+    the OBJ scene with its PNG wall texture renders through the host as
+    through the binding."""
     from PIL import Image
     torch = cuda
-    sc, cam = scenes.cornell()
+    if which == "cornell":
+        sc, cam = scenes.cornell()
+    else:
+        sc, cam = scenes.obj_cornell(ASSETS, subject="uv_sphere.obj")
     arrays = sc.build_arrays()
     save_scene(arrays, str(tmp_path / "cornell.rscn"))
     (tmp_path / "config.toml").write_text(CONFIG)
