@@ -457,6 +457,49 @@ int reina_rt_image_load(const char* path, ReinaImage** out);
 int reina_rt_image_load_memory(const uint8_t* bytes, uint64_t len, ReinaImage** out);
 void reina_rt_image_free(ReinaImage* image);
 
+/* src/scene/Scene.h:21-46 (Material), field for field in that order; the bools
+ * are uint32 (0 / 1). What Scene::addInstance turns into
+ * ReinaInstanceProperties. */
+typedef struct ReinaMaterial {
+    uint32_t materialIdx;             /* 0 Lambertian, 1 metal, 2 dielectric, 3 Disney */
+    int32_t  textureID, normalMapID, bumpMapID;
+    float    albedo[3], emission[3];
+    float    roughness, ior;
+    uint32_t interpNormals;
+    float    absorption;
+    uint32_t cullBackface;
+    float    anisotropic, subsurface, clearcoatGloss;
+    float    sheenTint[3], specularTint[3];
+    float    metallic, clearcoat, specularTransmission, sheen;
+} ReinaMaterial;
+
+typedef struct ReinaGltfInstance {
+    uint32_t model;                   /* index into ReinaGltfScene.models */
+    float transform[16];              /* column-major object-to-world (glm::make_mat4 order) */
+    ReinaMaterial material;
+} ReinaGltfInstance;
+
+typedef struct ReinaGltfScene {
+    ReinaModelData* models;       uint32_t num_models;      /* one per mesh primitive */
+    ReinaImage* images;           uint32_t num_images;      /* Material texture IDs index these */
+    ReinaGltfInstance* instances; uint32_t num_instances;   /* one per node and primitive */
+} ReinaGltfScene;
+
+/* Images the PNG decoder cannot read (JPEG) become texture ID -1 instead of
+ * failing the load (the reference's fallback for an unmapped texture). */
+#define REINA_RT_GLTF_SKIP_UNDECODABLE_IMAGES 0x1u
+
+/* reina::scene::gltf::loadScene (src/scene/gltf/gltfloader.cpp:432-456): the
+ * default scene of a .gltf (external buffers and images next to it, or data:
+ * URIs) or .glb, as models, decoded images and per-node instances with the
+ * reference's material mapping. The memory form takes a .glb or a
+ * self-contained .gltf. REINA_RT_ERR_INVALID_ARG on a malformed asset,
+ * REINA_RT_ERR_UNSUPPORTED for a required extension outside the reference's
+ * list or an undecodable image. Released with reina_rt_gltf_free. */
+int reina_rt_gltf_load(const char* path, uint32_t flags, ReinaGltfScene** out);
+int reina_rt_gltf_load_memory(const uint8_t* bytes, uint64_t len, uint32_t flags, ReinaGltfScene** out);
+void reina_rt_gltf_free(ReinaGltfScene* scene);
+
 /* Diagnostic: evaluates the library's FP32 elementary functions on n inputs
  * (device pointers) so that tests can compare them with the host definitions.
  * fn: 0 sin, 1 cos, 2 log, 3 exp, 4 sqrt, 5 pow(x, y = in2). */

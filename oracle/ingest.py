@@ -1,7 +1,7 @@
 """ORACLE — CPU restatement of Reina's scene ingest, in Python + numpy.
 Test infrastructure only: used by tests/ as the checker of the library's
-native loaders (reina_rt_model_load_obj, reina_rt_image_load), never by the
-product path.
+native loaders (reina_rt_model_load_obj, reina_rt_image_load,
+reina_rt_gltf_load), never by the product path.
 
 obj_load(text) restates Models::getObjData (src/scene/Models.cpp:117-175):
 Assimp's OBJ importer (a vcpkg dependency of the reference, CMakeLists.txt:8,
@@ -15,6 +15,10 @@ ascending vertex order.
 
 png_expected_rgba(...) states stb_image's 4-channel expansion of decoded
 PNG samples (src/graphics/Image.cpp:10-41 requests 4 channels).
+
+gltf_load(data) restates reina::scene::gltf::loadScene
+(src/scene/gltf/gltfloader.cpp) over fastgltf's accessor conversions and the
+glTF 2.0 node transform rules; images are decoded by Pillow.
 
 Pins (tests/test_ingest.py): closed forms (quad and cube tangent frames,
 triangulation of convex and concave quads, smooth normals of a cube without
@@ -597,3 +601,249 @@ def png_expected_rgba(samples, color, depth, palette=None, trns=None):
             hit = np.all(to8[..., :nk] == keys, axis=-1)
         out[..., 3][hit] = 0
     return out
+
+
+# ---- glTF ----------------------------------------------------------------------------------
+_GLTF_SUPPORTED = {"KHR_mesh_quantization", "KHR_texture_transform", "KHR_materials_variants",
+                   "KHR_materials_transmission", "KHR_materials_clearcoat", "KHR_materials_emissive_strength"}
+_COMP = {5120: ("<i1", 127.0), 5121: ("<u1", 255.0), 5122: ("<i2", 32767.0), 5123: ("<u2", 65535.0),
+         5125: ("<u4", 4294967295.0), 5126: ("<f4", None)}
+_NCOMP = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
+
+
+class GltfError(ValueError):
+    pass
+
+
+def _gltf_uri(uri, base_dir):
+    import base64
+    import os
+    import urllib.parse
+    if uri.startswith("data:"):
+        head, _, data = uri.partition(",")
+        if ";base64" not in head:
+            raise GltfError("only base64 data URIs")
+        return base64.b64decode(data)
+    if base_dir is None:
+        raise GltfError("external file in an in-memory asset")
+    return open(os.path.join(base_dir, urllib.parse.unquote(uri)), "rb").read()
+
+
+def _gltf_doc(data, base_dir):
+    import json
+    import struct
+    bin_chunk = None
+    if data[:4] == b"glTF":
+        _, version, length = struct.unpack_from("<III", data, 0)
+        if version != 2:
+            raise GltfError("GLB version")
+        off, doc = 12, None
+        while off + 8 <= length:
+            n, kind = struct.unpack_from("<II", data, off)
+            chunk = data[off + 8: off + 8 + n]
+            if kind == 0x4E4F534A and doc is None:
+                doc = json.loads(chunk.decode("utf-8"))
+            elif kind == 0x004E4942 and bin_chunk is None:
+                bin_chunk = chunk
+            off += 8 + n
+    else:
+        doc = json.loads(data.decode("utf-8"))
+    for e in doc.get("extensionsRequired", []):
+        if e not in _GLTF_SUPPORTED:
+            raise GltfError("unsupported required extension " + e)
+    buffers = []
+    for b in doc.get("buffers", []):
+        buffers.append(_gltf_uri(b["uri"], base_dir) if "uri" in b else bin_chunk)
+    return doc, buffers
+
+
+def _gltf_view(doc, buffers, view, offset, count, elem):
+    v = doc["bufferViews"][view]
+    stride = v.get("byteStride", 0) or elem
+    base = v.get("byteOffset", 0) + offset
+    buf = buffers[v["buffer"]]
+    if count and offset + stride * (count - 1) + elem > v["byteLength"]:
+        raise GltfError("accessor outside its view")
+    return buf, base, stride
+
+
+def _gltf_accessor(doc, buffers, idx, as_index=False):
+    a = doc["accessors"][idx]
+    dt, mx = _COMP[a["componentType"]]
+    nc = _NCOMP[a["type"]]
+    count = a["count"]
+    norm = a.get("normalized", False)
+    size = np.dtype(dt).itemsize
+
+    def conv(raw):
+        if as_index:
+            return raw.astype(np.uint32)
+        if dt == "<f4":
+            return raw.astype(F)
+        f = raw.astype(F)
+        if norm:
+            return np.maximum((f / F(mx)).astype(F), F(-1))
+        return f
+
+    out = np.zeros((count, nc), np.uint32 if as_index else F)
+    if "bufferView" in a:
+        buf, base, stride = _gltf_view(doc, buffers, a["bufferView"], a.get("byteOffset", 0), count, size * nc)
+        rows = [np.frombuffer(buf, dt, nc, base + i * stride) for i in range(count)]
+        if rows:
+            out[:] = conv(np.stack(rows))
+    if "sparse" in a:
+        sp = a["sparse"]
+        n = sp["count"]
+        idt, _ = _COMP[sp["indices"]["componentType"]]
+        ib, ibase, _ = _gltf_view(doc, buffers, sp["indices"]["bufferView"], sp["indices"].get("byteOffset", 0), n,
+                                  np.dtype(idt).itemsize)
+        vb, vbase, _ = _gltf_view(doc, buffers, sp["values"]["bufferView"], sp["values"].get("byteOffset", 0), n,
+                                  size * nc)
+        ids = np.frombuffer(ib, idt, n, ibase).astype(np.int64)
+        vals = np.frombuffer(vb, dt, n * nc, vbase).reshape(n, nc)
+        out[ids] = conv(vals)
+    return out[:, 0] if as_index else out
+
+
+def _mat_mul(a, b):
+    """column-major float32 4x4 product, terms summed k = 0..3 left to right"""
+    A = a.reshape(4, 4)   # A[c, r]
+    B = b.reshape(4, 4)
+    r = np.zeros((4, 4), F)
+    for c in range(4):
+        for row in range(4):
+            r[c, row] = F(F(F(A[0, row] * B[c, 0]) + F(A[1, row] * B[c, 1])) + F(A[2, row] * B[c, 2])) \
+                + F(A[3, row] * B[c, 3])
+    return r.reshape(16)
+
+
+def _node_matrix(node, parent):
+    if "matrix" in node:
+        return _mat_mul(parent, np.array(node["matrix"], np.float64).astype(F))
+    t = np.array(node.get("translation", [0, 0, 0]), np.float64).astype(F)
+    x, y, z, w = np.array(node.get("rotation", [0, 0, 0, 1]), np.float64).astype(F)
+    s = np.array(node.get("scale", [1, 1, 1]), np.float64).astype(F)
+    P = parent.reshape(4, 4)
+    m = P.copy()
+    m[3] = F(F(F(P[0] * t[0]) + F(P[1] * t[1])) + F(P[2] * t[2])) + P[3]
+    xx, yy, zz, xy, xz, yz, wx, wy, wz = x * x, y * y, z * z, x * y, x * z, y * z, w * x, w * y, w * z
+    one, two = F(1), F(2)
+    R = np.eye(4, dtype=F)
+    R[0, :3] = [one - two * (yy + zz), two * (xy + wz), two * (xz - wy)]
+    R[1, :3] = [two * (xy - wz), one - two * (xx + zz), two * (yz + wx)]
+    R[2, :3] = [two * (xz + wy), two * (yz - wx), one - two * (xx + yy)]
+    m = _mat_mul(m.reshape(16), R.reshape(16)).reshape(4, 4)
+    m[0] = (m[0] * s[0]).astype(F)
+    m[1] = (m[1] * s[1]).astype(F)
+    m[2] = (m[2] * s[2]).astype(F)
+    return m.reshape(16)
+
+
+def gltf_load(data, base_dir=None):
+    """Restates reina::scene::gltf::loadScene (src/scene/gltf/gltfloader.cpp)
+    with fastgltf's accessor conversions: returns dict(models=[dict(vertices,
+    indices, tbns (rows T, B, N), texCoords)], images=[RGBA8 via Pillow],
+    instances=[(model, column-major float32[16], material dict)])."""
+    import io
+
+    from PIL import Image
+
+    doc, buffers = _gltf_doc(data, base_dir)
+    scenes = doc.get("scenes", [])
+    if not scenes:
+        raise GltfError("no scenes")
+    visits = []
+
+    def walk(ni, parent):
+        node = doc["nodes"][ni]
+        m = _node_matrix(node, parent)
+        visits.append((ni, m))
+        for c in node.get("children", []):
+            walk(c, m)
+
+    ident = np.eye(4, dtype=F).reshape(16)
+    for r in scenes[doc.get("scene", 0)].get("nodes", []):
+        walk(r, ident)
+
+    images = []
+    for im in doc.get("images", []):
+        if "uri" in im:
+            raw = _gltf_uri(im["uri"], base_dir)
+        else:
+            v = doc["bufferViews"][im["bufferView"]]
+            buf = buffers[v["buffer"]]
+            raw = buf[v.get("byteOffset", 0): v.get("byteOffset", 0) + v["byteLength"]]
+        images.append(np.asarray(Image.open(io.BytesIO(raw)).convert("RGBA")))
+
+    used = sorted({doc["nodes"][ni]["mesh"] for ni, _ in visits if "mesh" in doc["nodes"][ni]})
+    models, mesh_models = [], {}
+    for mi in used:
+        mesh_models[mi] = []
+        for p in doc["meshes"][mi]["primitives"]:
+            at = p["attributes"]
+            P = _gltf_accessor(doc, buffers, at["POSITION"])
+            if "NORMAL" not in at:
+                raise GltfError("no normals")
+            N = _gltf_accessor(doc, buffers, at["NORMAL"])
+            n = len(P)
+            T = np.zeros((n, 3), F)
+            B = np.zeros((n, 3), F)
+            if "TANGENT" in at:
+                Ta = _gltf_accessor(doc, buffers, at["TANGENT"])
+                T[:] = Ta[:, :3]
+                B[:] = _cross(N, T)
+                if Ta.shape[1] == 4:
+                    B[:] = (B * Ta[:, 3:4]).astype(F)
+            else:
+                k = n // 3 * 3
+                T[:k, 0] = 1
+                B[:k, 1] = 1
+            uv = _gltf_accessor(doc, buffers, at["TEXCOORD_0"]) if "TEXCOORD_0" in at else np.zeros((n, 2), F)
+            idx = _gltf_accessor(doc, buffers, p["indices"], True) if "indices" in p else np.arange(n, dtype=np.uint32)
+            mesh_models[mi].append(len(models))
+            models.append(dict(vertices=np.concatenate([P, np.ones((n, 1), F)], 1).astype(F),
+                               indices=idx.astype(np.uint32), tbns=np.stack([T, B, N], 1).astype(F),
+                               texCoords=uv.astype(F), material=p.get("material", -1)))
+
+    def tex_image(info):
+        src = doc["textures"][info["index"]].get("source")
+        if src is None:
+            raise GltfError("texture without source")
+        return src
+
+    def material(mi):
+        m = dict(materialIdx=3, textureID=-1, normalMapID=-1, bumpMapID=-1, albedo=(1.0, 1.0, 1.0),
+                 emission=(0.0, 0.0, 0.0), roughness=0.0, ior=1.5, interpNormals=True, absorption=0.0,
+                 cullBackface=False, anisotropic=0.0, subsurface=0.0, clearcoatGloss=0.0, sheenTint=(1.0, 1.0, 1.0),
+                 specularTint=(1.0, 1.0, 1.0), metallic=0.0, clearcoat=0.0, specularTransmission=0.0, sheen=0.0)
+        if mi < 0:
+            return m
+        g = doc["materials"][mi]
+        pbr = g.get("pbrMetallicRoughness", {})
+        ext = g.get("extensions", {})
+        base = [F(c) for c in pbr.get("baseColorFactor", [1, 1, 1, 1])]
+        em = [F(c) for c in g.get("emissiveFactor", [0, 0, 0])]
+        strength = F(ext.get("KHR_materials_emissive_strength", {}).get("emissiveStrength", 1.0))
+        if any(c > 0 for c in em) and "emissiveTexture" not in g:
+            m["emission"] = tuple(F(c * strength) for c in em)
+        m["metallic"] = F(pbr.get("metallicFactor", 1.0))
+        m["roughness"] = F(max(min(F(pbr.get("roughnessFactor", 1.0)), F(0.7)), F(0.1)))
+        m["albedo"] = tuple(base[:3])
+        m["cullBackface"] = not g.get("doubleSided", False)
+        if "KHR_materials_transmission" in ext:
+            m["specularTransmission"] = F(ext["KHR_materials_transmission"].get("transmissionFactor", 0.0))
+            m["cullBackface"] = False
+        if "baseColorTexture" in pbr:
+            m["textureID"] = tex_image(pbr["baseColorTexture"])
+        if "normalTexture" in g:
+            m["normalMapID"] = tex_image(g["normalTexture"])
+        return m
+
+    instances = []
+    for ni, mat in visits:
+        node = doc["nodes"][ni]
+        if "mesh" not in node:
+            continue
+        for model in mesh_models[node["mesh"]]:
+            instances.append((model, mat.astype(F), material(models[model]["material"])))
+    return dict(models=models, images=images, instances=instances)

@@ -63,6 +63,8 @@ def lib():
         "reina_rt_model_load_obj_memory": [ctypes.c_char_p, u64, ctypes.POINTER(ctypes.POINTER(abi.ReinaModelData))],
         "reina_rt_image_load": [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(abi.ReinaImage))],
         "reina_rt_image_load_memory": [ctypes.c_char_p, u64, ctypes.POINTER(ctypes.POINTER(abi.ReinaImage))],
+        "reina_rt_gltf_load": [ctypes.c_char_p, u32, ctypes.POINTER(ctypes.POINTER(abi.ReinaGltfScene))],
+        "reina_rt_gltf_load_memory": [ctypes.c_char_p, u64, u32, ctypes.POINTER(ctypes.POINTER(abi.ReinaGltfScene))],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -72,6 +74,8 @@ def lib():
     L.reina_rt_model_free.restype = None
     L.reina_rt_image_free.argtypes = [ctypes.POINTER(abi.ReinaImage)]
     L.reina_rt_image_free.restype = None
+    L.reina_rt_gltf_free.argtypes = [ctypes.POINTER(abi.ReinaGltfScene)]
+    L.reina_rt_gltf_free.restype = None
     L.reina_rt_last_error.argtypes = []
     L.reina_rt_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -92,7 +96,8 @@ def exported_symbols():
             "reina_rt_bvh_quantize_host", "reina_rt_server_create", "reina_rt_server_destroy", "reina_rt_submit",
             "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats", "reina_rt_scene_export_emissive",
             "reina_rt_model_load_obj", "reina_rt_model_load_obj_memory", "reina_rt_model_free",
-            "reina_rt_image_load", "reina_rt_image_load_memory", "reina_rt_image_free"]
+            "reina_rt_image_load", "reina_rt_image_load_memory", "reina_rt_image_free",
+            "reina_rt_gltf_load", "reina_rt_gltf_load_memory", "reina_rt_gltf_free"]
 
 
 class DeviceScene:

@@ -75,6 +75,33 @@ class ReinaImage(ctypes.Structure):
     _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba8", ctypes.POINTER(ctypes.c_uint8))]
 
 
+class ReinaMaterial(ctypes.Structure):
+    _fields_ = [
+        ("materialIdx", ctypes.c_uint32), ("textureID", ctypes.c_int32), ("normalMapID", ctypes.c_int32),
+        ("bumpMapID", ctypes.c_int32), ("albedo", ctypes.c_float * 3), ("emission", ctypes.c_float * 3),
+        ("roughness", ctypes.c_float), ("ior", ctypes.c_float), ("interpNormals", ctypes.c_uint32),
+        ("absorption", ctypes.c_float), ("cullBackface", ctypes.c_uint32), ("anisotropic", ctypes.c_float),
+        ("subsurface", ctypes.c_float), ("clearcoatGloss", ctypes.c_float), ("sheenTint", ctypes.c_float * 3),
+        ("specularTint", ctypes.c_float * 3), ("metallic", ctypes.c_float), ("clearcoat", ctypes.c_float),
+        ("specularTransmission", ctypes.c_float), ("sheen", ctypes.c_float),
+    ]
+
+
+class ReinaGltfInstance(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_uint32), ("transform", ctypes.c_float * 16), ("material", ReinaMaterial)]
+
+
+class ReinaGltfScene(ctypes.Structure):
+    _fields_ = [
+        ("models", ctypes.POINTER(ReinaModelData)), ("num_models", ctypes.c_uint32),
+        ("images", ctypes.POINTER(ReinaImage)), ("num_images", ctypes.c_uint32),
+        ("instances", ctypes.POINTER(ReinaGltfInstance)), ("num_instances", ctypes.c_uint32),
+    ]
+
+
+GLTF_SKIP_UNDECODABLE_IMAGES = 0x1
+
+
 class ReinaSceneDesc(ctypes.Structure):
     _fields_ = [
         ("struct_size", ctypes.c_uint32), ("flags", ctypes.c_uint32),
