@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libreina_rt.so")
 
 # host-only translation units (no device code): g++ with the same flags
-HOST_SOURCES = ["ingest_obj.cpp", "ingest_png.cpp", "ingest_gltf.cpp"]
+HOST_SOURCES = ["ingest_obj.cpp", "ingest_png.cpp", "ingest_jpeg.cpp", "ingest_gltf.cpp"]
 SOURCES = ["api.cpp", "bvh_host.cpp", "bvh_device.hip", "integrator.hip", "trace_f32.hip", "trace_q8.hip", "trace_multi_f32.hip", "trace_multi_q8.hip", "server.hip", "server.cpp", "tail.hip", "post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

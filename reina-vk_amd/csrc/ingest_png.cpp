@@ -1,4 +1,5 @@
-// ingest_png.cpp — PNG → RGBA8, host only.
+// ingest_png.cpp — PNG → RGBA8 (and the entry points that also route JPEG
+// data to ingest_jpeg.cpp), host only.
 //
 // Replaces the stb_image calls of src/graphics/Image.cpp:10-41 for the
 // textures Reina loads (stbi_load / stbi_load_from_memory with 4 channels
@@ -29,6 +30,11 @@
 
 namespace rt {
 int fail(int code, const std::string& msg);
+struct JpegError {
+    int code;
+    std::string msg;
+};
+std::vector<uint8_t> decode_jpeg(const uint8_t* p, uint64_t n, bool flip, uint32_t& w, uint32_t& h);
 }
 using rt::fail;
 
@@ -143,9 +149,27 @@ unsigned sample(const uint8_t* row, uint64_t k, unsigned depth) {
 
 ReinaImage* decode(const uint8_t* p, uint64_t n, bool flip) {
     static const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
-    if (n >= 3 && p[0] == 0xFF && p[1] == 0xD8 && p[2] == 0xFF)
-        throw Error{REINA_RT_ERR_UNSUPPORTED, "JPEG images are not supported (PNG only)"};
-    if (n < 8 || std::memcmp(p, sig, 8) != 0) throw Error{REINA_RT_ERR_UNSUPPORTED, "not a PNG image (PNG only)"};
+    if (n >= 3 && p[0] == 0xFF && p[1] == 0xD8 && p[2] == 0xFF) {
+        uint32_t w = 0, h = 0;
+        std::vector<uint8_t> rgba;
+        try {
+            rgba = rt::decode_jpeg(p, n, flip, w, h);
+        } catch (const rt::JpegError& e) {
+            throw Error{e.code, e.msg};
+        }
+        ReinaImage* img = static_cast<ReinaImage*>(std::calloc(1, sizeof(ReinaImage)));
+        if (!img) throw std::bad_alloc();
+        img->rgba8 = static_cast<uint8_t*>(std::malloc(rgba.size()));
+        if (!img->rgba8) {
+            std::free(img);
+            throw std::bad_alloc();
+        }
+        std::memcpy(img->rgba8, rgba.data(), rgba.size());
+        img->width = w;
+        img->height = h;
+        return img;
+    }
+    if (n < 8 || std::memcmp(p, sig, 8) != 0) throw Error{REINA_RT_ERR_UNSUPPORTED, "not a PNG or JPEG image"};
     Header hd;
     uint8_t pal[256 * 4];
     unsigned pal_len = 0, trns_len = 0;

@@ -337,7 +337,10 @@ def test_gltf_errors(tmp_path):
     b.doc["accessors"][0]["count"] = 1000                          # past its bufferView
     assert code(b.glb()) == -1
     b = build_asset()
-    b.doc["images"][1] = {"uri": "data:image/jpeg;base64," + base64.b64encode(b"\xff\xd8\xff\xe0" + b"\0" * 20).decode()}
+    from PIL import Image
+    jb = io.BytesIO()
+    Image.fromarray(np.zeros((8, 8, 3), np.uint8)).save(jb, format="JPEG", progressive=True)
+    b.doc["images"][1] = {"uri": "data:image/jpeg;base64," + base64.b64encode(jb.getvalue()).decode()}
     assert code(b.glb()) == -6
     models, images, inst = gltf.load(b.glb(), True)                # skipped: texture ID -1
     assert len(images) == 1 and inst[0][2].normalMapID == -1

@@ -434,7 +434,7 @@ def test_reference_textures_match_pillow():
 
 def test_png_errors():
     with pytest.raises(reina_rt.ReinaError) as e:
-        load_image(b"\xff\xd8\xff\xe0" + b"\0" * 16)
+        load_image(b"GIF89a" + b"\0" * 16)
     assert e.value.code == -6
     good = encode_png(np.zeros((2, 2, 3), np.int64), 2, 8)
     for bad in [good[:40], good.replace(b"IHDR", b"IHDX", 1), b"not a png at all"]:
