@@ -1,7 +1,7 @@
 // reina_headless.cpp — a headless Reina host driving libreina_rt through its
 // C ABI: the render loop of src/Reina.cpp:296-392 without a window.
 //
-//   reina_headless <config.toml> <scene.rscn> <output-prefix> [push_consts.bin]
+//   reina_headless <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-prefix> [push_consts.bin]
 //
 // config.toml uses the reference's keys (config/config.toml): [camera.dof]
 // focus_dist, defocus_multiplier; [sampling] samples_per_pixel, max_bounces,
@@ -14,9 +14,12 @@
 // the accumulated sample count reaches an entry of save_on_samples the image
 // is post-processed (bloom + ACES tone map, reina_rt_postprocess) and written
 // as <prefix>_<samples>.png, as Reina's save() writes its RGBA8 swapchain
-// image (src/Reina.cpp:19-47). The scene file is the packed ReinaSceneDesc
-// arrays (reina_rt.scene.save_scene); push_consts.bin, if given, replaces the
-// camera and sampling push constants by a host's own 160-byte RtPushConsts.
+// image (src/Reina.cpp:19-47). The scene is a glTF asset (loaded with
+// reina_rt_gltf_load and packed as Models::addModel / Scene::addInstance do,
+// the reference's gltf::loadScene path of Reina.cpp:91) or the packed
+// ReinaSceneDesc arrays (reina_rt.scene.save_scene); push_consts.bin, if given,
+// replaces the camera and sampling push constants by a host's own 160-byte
+// RtPushConsts.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -144,6 +147,87 @@ SceneFile load_scene(const std::string& path) {
     return s;
 }
 
+// ---- a glTF scene packed as Models::addModel / Scene::addInstance do -------
+// (src/scene/Models.cpp:24-92: per-model offsets added to the index streams;
+// src/scene/Scene.cpp:27-54: one InstanceProperties per instance)
+SceneFile scene_from_gltf(const std::string& path) {
+    ReinaGltfScene* g = nullptr;
+    if (reina_rt_gltf_load(path.c_str(), 0, &g) != REINA_RT_OK)
+        throw std::runtime_error(std::string("reina_rt_gltf_load: ") + reina_rt_last_error());
+    SceneFile s;
+    std::vector<ReinaModelRange> ranges;
+    for (uint32_t m = 0; m < g->num_models; ++m) {
+        const ReinaModelData& md = g->models[m];
+        ReinaModelRange r{};
+        const uint32_t vo = (uint32_t)(s.vertices.size() / 4), to = (uint32_t)(s.tbns.size() / 9),
+                       xo = (uint32_t)(s.tex_coords.size() / 2);
+        r.firstVertex = vo;
+        r.firstNormal = to;
+        r.indexOffset = (uint32_t)s.indices.size();
+        r.tbnsIndexOffset = (uint32_t)s.tbn_indices.size();
+        r.texIndexOffset = md.num_tex_coords ? (uint32_t)s.tex_indices.size() : 0xFFFFFFFFu;
+        r.indexCount = r.tbnsIndexCount = (uint32_t)(md.num_indices / 3);
+        r.texIndexCount = (uint32_t)(md.num_tex_indices / 3);
+        s.vertices.insert(s.vertices.end(), md.vertices, md.vertices + 4 * md.num_vertices);
+        s.tbns.insert(s.tbns.end(), md.tbns, md.tbns + 9 * md.num_tbns);
+        s.tex_coords.insert(s.tex_coords.end(), md.tex_coords, md.tex_coords + 2 * md.num_tex_coords);
+        for (uint64_t i = 0; i < md.num_indices; ++i) s.indices.push_back(md.indices[i] + vo);
+        for (uint64_t i = 0; i < md.num_tbn_indices; ++i) s.tbn_indices.push_back(md.tbn_indices[i] + to);
+        for (uint64_t i = 0; i < md.num_tex_indices; ++i)
+            s.tex_indices.push_back(md.tex_indices[i] == 0xFFFFFFFFu ? 0xFFFFFFFFu : md.tex_indices[i] + xo);
+        ranges.push_back(r);
+    }
+    for (uint32_t t = 0; t < g->num_images; ++t) {
+        const ReinaImage& im = g->images[t];
+        s.texels.emplace_back(im.rgba8, im.rgba8 + (size_t)im.width * im.height * 4);
+    }
+    for (uint32_t t = 0; t < g->num_images; ++t)
+        s.textures.push_back(ReinaTextureDesc{g->images[t].width, g->images[t].height, s.texels[t].data()});
+    for (uint32_t i = 0; i < g->num_instances; ++i) {
+        const ReinaGltfInstance& in = g->instances[i];
+        const ReinaMaterial& m = in.material;
+        const ReinaModelRange& r = ranges[in.model];
+        ReinaInstanceProperties p{};
+        p.indicesOffset = r.indexOffset;
+        std::memcpy(p.albedo, m.albedo, sizeof p.albedo);
+        std::memcpy(p.emission, m.emission, sizeof p.emission);
+        p.tbnsIndicesOffset = r.tbnsIndexOffset;
+        p.texIndicesOffset = r.texIndexOffset;
+        p.roughness = m.roughness;
+        p.ior = m.ior;
+        p.interpNormals = m.interpNormals ? 1 : 0;
+        p.absorption = m.absorption;
+        p.textureID = m.textureID;
+        p.normalMapTexID = m.normalMapID;
+        p.bumpMapTexID = m.bumpMapID;
+        p.cullBackface = m.cullBackface ? 1u : 0u;
+        p.anisotropic = m.anisotropic;
+        p.subsurface = m.subsurface;
+        p.clearcoatGloss = m.clearcoatGloss;
+        std::memcpy(p.sheenTint, m.sheenTint, sizeof p.sheenTint);
+        std::memcpy(p.specularTint, m.specularTint, sizeof p.specularTint);
+        p.metallic = m.metallic;
+        p.clearcoat = m.clearcoat;
+        p.specularTransmission = m.specularTransmission;
+        p.sheen = m.sheen;
+        s.props.push_back(p);
+        ReinaInstanceDesc d{};
+        std::memcpy(d.transform, in.transform, sizeof d.transform);
+        d.instancePropertiesID = (uint32_t)(s.props.size() - 1);
+        d.materialIdx = m.materialIdx;
+        d.modelID = in.model;
+        s.instances.push_back(d);
+    }
+    s.models = ranges;
+    reina_rt_gltf_free(g);
+    return s;
+}
+
+bool ends_with(const std::string& s, const char* suffix) {
+    const size_t n = std::strlen(suffix);
+    return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
+}
+
 // ---- camera: glm::perspective / lookAt (src/graphics/Camera.cpp:10-11), inverted
 void mat_inverse(const double* m, double* inv) {   // 4x4 column-major, Gauss-Jordan
     double a[4][8];
@@ -266,11 +350,14 @@ void hip_check(hipError_t e, const char* what) {
 
 int run(int argc, char** argv) {
     if (argc != 4 && argc != 5) {
-        std::fprintf(stderr, "usage: %s <config.toml> <scene.rscn> <output-prefix> [push_consts.bin]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-prefix> [push_consts.bin]\n",
+                     argv[0]);
         return 2;
     }
     const Config cfg = parse_toml(argv[1]);
-    SceneFile sf = load_scene(argv[2]);
+    const std::string scene_path = argv[2];
+    SceneFile sf = (ends_with(scene_path, ".glb") || ends_with(scene_path, ".gltf")) ? scene_from_gltf(scene_path)
+                                                                                   : load_scene(scene_path);
     const std::string prefix = argv[3];
     const uint32_t W = (uint32_t)cfg.num("headless.width", 256), H = (uint32_t)cfg.num("headless.height", 256);
     const std::vector<double> eye = cfg.list("headless.eye", {0.0, 1.0, 3.4});

@@ -51,25 +51,35 @@ total_samples = 12
 ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
 
 
-@pytest.mark.parametrize("which", ["cornell", "obj_textured"])
+@pytest.mark.parametrize("which", ["cornell", "obj_textured", "gltf"])
 def test_headless_host_pngs_match_binding(cuda, tmp_path, which):
     """The scene file carries the packed arrays and (version 2) the textures:
     the OBJ scene with its PNG wall texture renders through the host as
-    through the binding."""
+    through the binding; a .glb is loaded and packed by the host itself and
+    renders as reina_rt.gltf.load_scene's arrays do."""
     from PIL import Image
     torch = cuda
+    scene_path = str(tmp_path / "cornell.rscn")
     if which == "cornell":
-        sc, cam = scenes.cornell()
+        sc, _ = scenes.cornell()
+    elif which == "obj_textured":
+        sc, _ = scenes.obj_cornell(ASSETS, subject="uv_sphere.obj")
     else:
-        sc, cam = scenes.obj_cornell(ASSETS, subject="uv_sphere.obj")
+        import test_gltf
+        from reina_rt import gltf
+        glb = test_gltf.build_asset().glb()
+        scene_path = str(tmp_path / "scene.glb")
+        (tmp_path / "scene.glb").write_bytes(glb)
+        sc = gltf.load_scene(glb)
     arrays = sc.build_arrays()
-    save_scene(arrays, str(tmp_path / "cornell.rscn"))
+    if which != "gltf":
+        save_scene(arrays, scene_path)
     (tmp_path / "config.toml").write_text(CONFIG)
     W, H = 48, 40
     pc = scenes.cornell_push(W, H, spp=4, max_bounces=6)
     pc["defocusMultiplier"] = 0.0
     (tmp_path / "pc.bin").write_bytes(np.asarray(pc).tobytes())
-    r = subprocess.run([BIN, str(tmp_path / "config.toml"), str(tmp_path / "cornell.rscn"), str(tmp_path / "out"),
+    r = subprocess.run([BIN, str(tmp_path / "config.toml"), scene_path, str(tmp_path / "out"),
                         str(tmp_path / "pc.bin")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     dev = reina_rt.DeviceScene(arrays)
