@@ -1,27 +1,33 @@
 // reina_headless.cpp — a headless Reina host driving libreina_rt through its
 // C ABI: the render loop of src/Reina.cpp:296-392 without a window.
 //
-//   reina_headless <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-prefix> [push_consts.bin]
+//   reina_headless <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-dir> [push_consts.bin]
 //
 // config.toml uses the reference's keys (config/config.toml): [camera.dof]
 // focus_dist, defocus_multiplier; [sampling] samples_per_pixel, max_bounces,
-// direct_clamp, indirect_clamp; [saving] save_on_samples;
+// direct_clamp, indirect_clamp; [saving] save_on_samples, save_on_times;
 // [postprocessing.bloom] radius, threshold, intensity;
 // [postprocessing.tonemap] exposure; plus a [headless] section for what the
 // reference takes from its window and camera: width, height, eye, center,
-// fov, total_samples, nee. Each frame is one reina_rt_trace (samples_per_pixel
-// samples, sampleBatch = frame index, running mean as Reina::traceRays); when
-// the accumulated sample count reaches an entry of save_on_samples the image
-// is post-processed (bloom + ACES tone map, reina_rt_postprocess) and written
-// as <prefix>_<samples>.png, as Reina's save() writes its RGBA8 swapchain
-// image (src/Reina.cpp:19-47). The scene is a glTF asset (loaded with
-// reina_rt_gltf_load and packed as Models::addModel / Scene::addInstance do,
-// the reference's gltf::loadScene path of Reina.cpp:91) or the packed
-// ReinaSceneDesc arrays (reina_rt.scene.save_scene); push_consts.bin, if given,
-// replaces the camera and sampling push constants by a host's own 160-byte
-// RtPushConsts.
+// fov, total_samples (the run's end), nee, print_summary (default 1). Each
+// frame is one reina_rt_trace (samples_per_pixel samples, sampleBatch = frame
+// index, running mean as Reina::traceRays). Saving follows SaveManager and
+// Clock (see SaveManager below): the image is post-processed (bloom + ACES
+// tone map, reina_rt_postprocess) and written as <output-dir>/output_<N>spp.png
+// or output_<T>sec.png, as Reina's save() writes its RGBA8 swapchain image
+// (src/Reina.cpp:19-47); after the first frame every frame prints the clock
+// summary (src/tools/Clock.cpp summary(); categories "Ray Tracing" and "Save":
+// there is no display, so post-processing runs only for a save). The scene is
+// a glTF asset (loaded with reina_rt_gltf_load and packed as Models::addModel /
+// Scene::addInstance do, the reference's gltf::loadScene path of
+// Reina.cpp:91) or the packed ReinaSceneDesc arrays
+// (reina_rt.scene.save_scene); push_consts.bin, if given, replaces the camera
+// and sampling push constants by a host's own 160-byte RtPushConsts.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -340,6 +346,83 @@ void write_png(const std::string& path, const uint8_t* rgba, uint32_t W, uint32_
     if (!f) throw std::runtime_error("could not write " + path);
 }
 
+// ---- SaveManager (src/tools/SaveManager.cpp) and Clock (src/tools/Clock.cpp) --
+// One save per frame at most: the smallest sample threshold below the clock's
+// sample count (output_<N>spp.png), else the smallest time threshold below its
+// age (output_<T>sec.png). The clock counts a frame's samples from the second
+// frame on (its first markFrame only starts the frame timer), so a threshold
+// N fires on the first frame k with N < spp * (k - 1), and that frame's image
+// holds spp * (k + 1) samples -- as in the reference.
+struct SaveManager {
+    std::vector<double> samples, times;   // ascending: front is next
+    SaveManager(std::vector<double> s, std::vector<double> t) : samples(std::move(s)), times(std::move(t)) {
+        std::sort(samples.begin(), samples.end());
+        std::sort(times.begin(), times.end());
+    }
+    bool should_save(uint64_t n, double age, std::string& name) {
+        if (!samples.empty() && (int64_t)samples.front() < (int64_t)n) {
+            name = "output_" + std::to_string((int64_t)samples.front()) + "spp.png";
+            samples.erase(samples.begin());
+            return true;
+        }
+        if (!times.empty() && times.front() < age) {
+            name = "output_" + std::to_string((int)times.front()) + "sec.png";
+            times.erase(times.begin());
+            return true;
+        }
+        return false;
+    }
+};
+
+struct Clock {
+    using tp = std::chrono::steady_clock::time_point;
+    tp created = std::chrono::steady_clock::now(), last_frame{}, last_category{};
+    bool frame_started = false, category_started = false;
+    double frame_avg = 0.0;
+    uint64_t frames = 0, samples = 0;
+    std::string category;
+    std::map<std::string, std::pair<double, uint64_t>> categories;   // average, recordings
+    static double since(tp a, tp b) { return std::chrono::duration<double>(b - a).count(); }
+    static void add(double& avg, uint64_t& n, double x) {
+        avg = n == 0 ? x : (avg * (double)n + x) / (double)(n + 1);
+        ++n;
+    }
+    double age() const { return since(created, std::chrono::steady_clock::now()); }
+    uint64_t frame_count() const { return frames; }
+    uint64_t sample_count() const { return samples; }
+    void mark_frame(uint32_t spp) {
+        const tp now = std::chrono::steady_clock::now();
+        if (!frame_started) {
+            frame_started = true;
+            last_frame = now;
+            return;
+        }
+        add(frame_avg, frames, since(last_frame, now));
+        last_frame = now;
+        samples += spp;
+    }
+    void mark_category(const std::string& c) {
+        const tp now = std::chrono::steady_clock::now();
+        categories.emplace(c, std::make_pair(0.0, (uint64_t)0));
+        if (category_started) {
+            auto& e = categories.at(category);
+            add(e.first, e.second, since(last_category, now));
+        }
+        category_started = true;
+        category = c;
+        last_category = now;
+    }
+    std::string summary() const {
+        std::ostringstream o;
+        o << "Timer age: " << age() << "s\n";
+        o << "Samples: " << samples << "\n";
+        o << "Average frame time: " << frame_avg * 1000 << "ms\n";
+        o << "Average time per spp: " << frame_avg * (double)frames * 1000 / (double)samples << "ms\n";
+        for (const auto& e : categories) o << "Average category time | " << e.first << ": " << e.second.first * 1000 << "ms\n";
+        return o.str();
+    }
+};
+
 void check(int rc, const char* what) {
     if (rc != REINA_RT_OK) throw std::runtime_error(std::string(what) + ": " + reina_rt_last_error());
 }
@@ -350,7 +433,7 @@ void hip_check(hipError_t e, const char* what) {
 
 int run(int argc, char** argv) {
     if (argc != 4 && argc != 5) {
-        std::fprintf(stderr, "usage: %s <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-prefix> [push_consts.bin]\n",
+        std::fprintf(stderr, "usage: %s <config.toml> <scene.rscn|scene.glb|scene.gltf> <output-dir> [push_consts.bin]\n",
                      argv[0]);
         return 2;
     }
@@ -358,7 +441,8 @@ int run(int argc, char** argv) {
     const std::string scene_path = argv[2];
     SceneFile sf = (ends_with(scene_path, ".glb") || ends_with(scene_path, ".gltf")) ? scene_from_gltf(scene_path)
                                                                                    : load_scene(scene_path);
-    const std::string prefix = argv[3];
+    const std::string out_dir = argv[3];
+    mkdir(out_dir.c_str(), 0755);   // an existing directory is fine
     const uint32_t W = (uint32_t)cfg.num("headless.width", 256), H = (uint32_t)cfg.num("headless.height", 256);
     const std::vector<double> eye = cfg.list("headless.eye", {0.0, 1.0, 3.4});
     const std::vector<double> center = cfg.list("headless.center", {0.0, 1.0, 0.0});
@@ -366,7 +450,8 @@ int run(int argc, char** argv) {
     const double fov = cfg.num("headless.fov", 40.0);
     const uint32_t spp = (uint32_t)cfg.num("sampling.samples_per_pixel", 8);
     const uint64_t total = (uint64_t)cfg.num("headless.total_samples", 64);
-    std::vector<double> saves = cfg.list("saving.save_on_samples", {(double)total});
+    const std::vector<double> saves = cfg.list("saving.save_on_samples", {});
+    const std::vector<double> save_times = cfg.list("saving.save_on_times", {});
 
     ReinaSceneDesc d{};
     d.struct_size = sizeof d;
@@ -419,27 +504,31 @@ int run(int argc, char** argv) {
     hip_check(hipMalloc(&scratch, (size_t)W * H * 32), "hipMalloc");
     hip_check(hipMalloc(&ldr, (size_t)W * H * 4), "hipMalloc");
     std::vector<uint8_t> host((size_t)W * H * 4);
-    uint64_t samples = 0;
-    size_t next_save = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    while (samples < total) {
+    SaveManager saver(saves, save_times);
+    Clock clock;
+    const bool print_summary = cfg.num("headless.print_summary", 1.0) != 0.0;
+    const uint64_t frames = (total + spp - 1) / spp;
+    for (uint64_t frame = 0; frame < frames; ++frame) {
+        if (print_summary && clock.frame_count() != 0) std::printf("%s\n", clock.summary().c_str());
+        clock.mark_category("Ray Tracing");
         check(reina_rt_trace(scene, &pc, image, W, H, st), "reina_rt_trace");
         pc.sampleBatch++;   // after recording, as Reina::traceRays (src/Reina.cpp:431-432)
-        samples += spp;
-        while (next_save < saves.size() && samples >= (uint64_t)saves[next_save]) {
+        hip_check(hipStreamSynchronize(st), "sync");
+        clock.mark_category("Save");
+        std::string name;
+        if (saver.should_save(clock.sample_count(), clock.age(), name)) {
             check(reina_rt_postprocess(image, scratch, ldr, W, H, &bloom, &tone, st), "reina_rt_postprocess");
             hip_check(hipMemcpyAsync(host.data(), ldr, host.size(), hipMemcpyDeviceToHost, st), "copy");
             hip_check(hipStreamSynchronize(st), "sync");
-            const std::string name = prefix + "_" + std::to_string((uint64_t)saves[next_save]) + ".png";
-            write_png(name, host.data(), W, H);
-            std::printf("saved %s after %llu samples per pixel\n", name.c_str(), (unsigned long long)samples);
-            next_save++;
+            write_png(out_dir + "/" + name, host.data(), W, H);
+            std::printf("saved %s/%s (%llu samples per pixel in the image)\n", out_dir.c_str(), name.c_str(),
+                        (unsigned long long)(frame + 1) * spp);
         }
+        clock.mark_frame(spp);
     }
     hip_check(hipStreamSynchronize(st), "sync");
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::printf("rendered %llu samples per pixel at %ux%u in %.3f s (%llu frames)\n", (unsigned long long)samples, W,
-                H, secs, (unsigned long long)pc.sampleBatch);
+    std::printf("rendered %llu samples per pixel at %ux%u in %.3f s (%llu frames)\n",
+                (unsigned long long)frames * spp, W, H, clock.age(), (unsigned long long)pc.sampleBatch);
     (void)hipFree(image);
     (void)hipFree(scratch);
     (void)hipFree(ldr);

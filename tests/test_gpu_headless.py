@@ -30,7 +30,8 @@ direct_clamp = 100
 indirect_clamp = 10
 
 [saving]
-save_on_samples = [4, 12]
+save_on_samples = [12, 4]
+save_on_times = [0]
 
 [postprocessing]
 [postprocessing.bloom]
@@ -44,8 +45,16 @@ exposure = 1
 [headless]
 width = 48
 height = 40
-total_samples = 12
+total_samples = 24
 """
+
+# SaveManager / Clock (src/tools/SaveManager.cpp, Clock.cpp) with 4 spp per
+# frame: the clock's sample count lags (it counts frames from the second on),
+# so frame k sees 4 (k - 1) samples; one save per frame, sample thresholds
+# first: frame 0 saves output_0sec.png (age > 0), frame 3 (count 8 > 4)
+# output_4spp.png, frame 5 (count 16 > 12) output_12spp.png. The image of
+# frame k holds batches 0..k.
+SAVES = {0: "output_0sec.png", 3: "output_4spp.png", 5: "output_12spp.png"}
 
 
 ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
@@ -82,19 +91,22 @@ def test_headless_host_pngs_match_binding(cuda, tmp_path, which):
     r = subprocess.run([BIN, str(tmp_path / "config.toml"), scene_path, str(tmp_path / "out"),
                         str(tmp_path / "pc.bin")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    assert sorted(os.listdir(tmp_path / "out")) == sorted(SAVES.values())
+    # the summary prints once the clock has recorded a frame time (from frame 2 on)
+    assert r.stdout.count("Average category time | Ray Tracing") == 4
     dev = reina_rt.DeviceScene(arrays)
     img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     scratch = torch.zeros((2, H, W, 4), dtype=torch.float32, device="cuda")
     ldr = torch.zeros((H, W, 4), dtype=torch.uint8, device="cuda")
     sp = torch.cuda.current_stream().cuda_stream
-    for batch in range(3):
+    for batch in range(6):
         p = np.asarray(pc).copy()
         p["sampleBatch"] = batch
         dev.trace(p, img.data_ptr(), W, H, stream=sp)
-        if batch in (0, 2):
+        if batch in SAVES:
             reina_rt.postprocess(img.data_ptr(), scratch.data_ptr(), ldr.data_ptr(), W, H, stream=sp)
             torch.cuda.synchronize()
-            got = np.asarray(Image.open(tmp_path / ("out_%d.png" % (4 * (batch + 1)))).convert("RGBA"))
+            got = np.asarray(Image.open(tmp_path / "out" / SAVES[batch]).convert("RGBA"))
             assert got.shape == (H, W, 4)
             assert np.array_equal(got, ldr.cpu().numpy()), batch
     dev.close()
