@@ -409,6 +409,34 @@ def test_running_mean_bitexact(cuda, cornell_arrays):
     dev.close()
 
 
+def test_seed_wraps_mod_2_32(cuda, cornell_arrays):
+    """Q10: the pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
+    (raytrace.rgen.glsl:259), so at 64 x 64 (W * H = 2^12) batch 2^20 reuses
+    batch 0's streams. GPU = oracle bit for bit around the wrap (a 64-bit seed
+    would differ), and batch 2^20 blended onto batch 0's image gives back that
+    image (the same samples, to the blend's rounding)."""
+    torch = cuda
+    arrays, _ = cornell_arrays
+    W = H = 64
+    wrap = (1 << 32) // (W * H)
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    for b in (wrap - 1, wrap, wrap + 1):
+        pc = scenes.cornell_push(W, H, spp=2, max_bounces=4, sample_batch=b)
+        g = gpu_render(torch, dev, pc, W, H)
+        o, _ = orc.render(pc, W, H, use_bvh=False)
+        assert np.array_equal(_bits(g), _bits(o)), "batch %d" % b
+    first = gpu_render(torch, dev, scenes.cornell_push(W, H, spp=2, max_bounces=4, sample_batch=0), W, H)
+    again = gpu_render(torch, dev, scenes.cornell_push(W, H, spp=2, max_bounces=4, sample_batch=wrap), W, H,
+                       init=first)
+    ok = np.isfinite(first)
+    assert np.allclose(again[ok], first[ok], rtol=4e-7, atol=0)
+    other = gpu_render(torch, dev, scenes.cornell_push(W, H, spp=2, max_bounces=4, sample_batch=wrap + 1), W, H,
+                       init=first)
+    assert not np.allclose(other[ok], first[ok], rtol=4e-7, atol=0)
+    dev.close()
+
+
 @pytest.mark.parametrize("spp,bounces", [(0, 8), (3, 0), (1, 1)])
 def test_render_edge_cases(cuda, cornell_arrays, spp, bounces):
     torch = cuda
