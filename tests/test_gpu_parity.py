@@ -409,6 +409,28 @@ def test_running_mean_bitexact(cuda, cornell_arrays):
     dev.close()
 
 
+@pytest.mark.parametrize("tail", [0, 1 << 30], ids=["wavefront", "tail_kernel"])
+@pytest.mark.parametrize("defocus", [0.015, 0.2], ids=["config_default", "strong"])
+def test_render_defocus_bitexact(cuda, cornell_arrays, defocus, tail):
+    """Depth of field: the hexagon lens sample of raytrace.rgen.glsl:186-200,
+    drawn from a copy of the pixel's RNG state (Q3: the lens sample does not
+    advance the stream). config/config.toml's defocus_multiplier = 1.5 is
+    0.015 after Reina.cpp's division by 100."""
+    torch = cuda
+    arrays, cam = cornell_arrays
+    W, H = 48, 40
+    pc = reina_rt.push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp=4, max_bounces=8, defocus=defocus,
+                              focus_dist=2.2)
+    dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
+    got = gpu_render(torch, dev, pc, W, H)
+    ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+    assert np.array_equal(_bits(got), _bits(ref)), "pixels differing: %d" % (_bits(got) != _bits(ref)).any(-1).sum()
+    pin = reina_rt.push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp=4, max_bounces=8)
+    assert not np.array_equal(_bits(gpu_render(torch, dev, pin, W, H)), _bits(got))   # the lens moved the rays
+    dev.close()
+
+
 def test_seed_wraps_mod_2_32(cuda, cornell_arrays):
     """Q10: the pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
     (raytrace.rgen.glsl:259), so at 64 x 64 (W * H = 2^12) batch 2^20 reuses
