@@ -72,6 +72,8 @@ def lib():
     L.or_tex_sample_at.restype = None
     L.or_bump_at.argtypes = [c_p, i32, ctypes.c_float, ctypes.c_float, c_p, c_p, c_p]
     L.or_bump_at.restype = None
+    L.or_disney_probe.argtypes = [i32, u64, c_p, c_p, c_p, ctypes.POINTER(u32), c_p]
+    L.or_disney_probe.restype = None
     _lib = L
     return L
 
@@ -217,6 +219,25 @@ def camera_ray(pc, px, py, W, H, rng):
     d = np.zeros(3, np.float32)
     lib().or_camera_ray(pcb.ctypes.data, px, py, W, H, ctypes.byref(st), _p(o), _p(d))
     return o, d, st.value
+
+
+DISNEY_PROBES = {"ggx_d": (0, 2), "smith_g1": (1, 2), "vndf": (2, 3), "pdf_refl": (3, 1), "gtr1": (4, 3),
+                 "dc": (5, 1), "diffuse": (6, 3), "sheen": (7, 3), "pdf_clearcoat": (8, 1), "clearcoat": (9, 3)}
+
+
+def disney_probe(name, n, a=None, b=None, prm=(), seed=1):
+    """Batch evaluation of one piece of the oracle's Disney BSDF in the
+    tangent frame (normal +z); see or_disney_probe. Returns (n, width) float32
+    and the final RNG state of a sampling probe."""
+    fn, width = DISNEY_PROBES[name]
+    a = None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n, 3)))
+    b = None if b is None else np.ascontiguousarray(np.broadcast_to(np.asarray(b, np.float32), (n, 3)))
+    p = np.zeros(8, np.float32)
+    p[:len(prm)] = prm
+    st = ctypes.c_uint32(seed)
+    out = np.zeros((n, width), np.float32)
+    lib().or_disney_probe(fn, n, _p(a), _p(b), _p(p), ctypes.byref(st), _p(out))
+    return out, st.value
 
 
 def diffuse_dir(n, rng):

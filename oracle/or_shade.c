@@ -12,7 +12,14 @@
  * on the whole sample (Q6), accumulatedDistance not reset per path (Q8;
  * initial value 0), random() = float(w) * 2^-32 (Q9), NaN samples dropped and
  * NaN when every sample is dropped (Q15), the TBN read even without
- * interpNormals (Q14; a model without TBN data reads zeros).
+ * interpNormals (Q14; a model without TBN data reads zeros), metal sampled by
+ * the unbounded VNDF but priced by the bounded-VNDF pdf (Q17). or_scene_set_nee
+ * switches on this build's correct NEE + MIS instead.
+ *
+ * Parity unpinned: the lobe weights with which sampleDisney / evalDisney mix
+ * the lobes (Q5: inconsistent in the reference, so no identity pins them
+ * beyond this line-by-line restatement). Each lobe beneath them is pinned
+ * against its mathematics through or_disney_probe.
  */
 #include "oracle.h"
 
@@ -527,6 +534,49 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
     r = vadd(r, vscale(fc, cW));
     r = vadd(r, vscale(fg, gW / ws));
     return r;
+}
+
+/* Batch probes of the Disney pieces above for the tests, in the tangent frame
+ * (tbn = identity, n = +z). prm holds the piece's scalar parameters; rng is one
+ * stream carried across the n samples of a sampling probe.
+ *   0: D_GGX_Aniso(a), evalDm(a)            -> out[2k], out[2k+1]  (prm ax, ay)
+ *   1: smithG(a), SmithGAniso(|a.z|, a.x, a.y) -> out[2k], out[2k+1]  (ax, ay)
+ *   2: sampleGGXVNDF(a)                     -> out[3k..]            (ax, ay)
+ *   3: pdfGGXReflection(a, b)               -> out[k]               (ax, ay)
+ *   4: sampleGTR1_attempt2                  -> out[3k..]            (alpha)
+ *   5: evalDc(alpha, a)                     -> out[k]               (alpha)
+ *   6: evalDiffuse(wi a, wo b, h = |a + b|) -> out[3k..]  (roughness, subsurface, bc[3])
+ *   7: evalSheen(wo b, h = |a + b|)         -> out[3k..]  (-, -, bc[3], sheenTint[3])
+ *   8: pdfClearcoat(wi a, wo b, h = |a + b|) -> out[k]              (clearcoatGloss)
+ *   9: sampleClearcoat(wi a)                -> out[3k..]            (clearcoatGloss) */
+void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const float* prm, uint32_t* rng,
+                     float* out) {
+    const m3 I = {V3(1.0f, 0.0f, 0.0f), V3(0.0f, 1.0f, 0.0f), V3(0.0f, 0.0f, 1.0f)};
+    const v3 nz = V3(0.0f, 0.0f, 1.0f);
+    for (uint64_t k = 0; k < n; ++k) {
+        v3 x = a ? V3(a[3 * k], a[3 * k + 1], a[3 * k + 2]) : nz;
+        v3 y = b ? V3(b[3 * k], b[3 * k + 1], b[3 * k + 2]) : nz;
+        v3 r = V3(0.0f, 0.0f, 0.0f);
+        switch (fn) {
+            case 0: out[2 * k] = D_GGX_Aniso(x, prm[0], prm[1]); out[2 * k + 1] = evalDm(x, prm[0], prm[1]); continue;
+            case 1:
+                out[2 * k] = smithG(x, prm[0], prm[1]);
+                out[2 * k + 1] = SmithGAniso(fabsf(x.z), x.x, x.y, prm[0], prm[1]);
+                continue;
+            case 2: r = sampleGGXVNDF(x, prm[0], prm[1], rng); break;
+            case 3: out[k] = pdfGGXReflection(x, y, prm[0], prm[1]); continue;
+            case 4: r = sampleGTR1_attempt2(prm[0], rng); break;
+            case 5: out[k] = evalDc(prm[0], x); continue;
+            case 6: r = evalDiffuse(prm[0], prm[1], V3(prm[2], prm[3], prm[4]), nz, x, y, vnormalize(vadd(x, y))); break;
+            case 7:
+                r = evalSheen(V3(prm[2], prm[3], prm[4]), y, vnormalize(vadd(x, y)), nz, V3(prm[5], prm[6], prm[7]));
+                break;
+            case 8: out[k] = pdfClearcoat(I, x, y, vnormalize(vadd(x, y)), prm[0]); continue;
+            case 9: r = sampleClearcoat(I, prm[0], x, rng); break;
+            default: return;
+        }
+        out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
+    }
 }
 
 /* ============================ textures ==================================== */

@@ -85,8 +85,8 @@ GREEN = (0.14, 0.45, 0.091)
 GREY = (0.73, 0.73, 0.73)
 
 
-def cornell():
-    """Returns (Scene, camera dict). 40 triangles."""
+def cornell(boxes=True):
+    """Returns (Scene, camera dict). 40 triangles (16 without the boxes)."""
     s = Scene()
     wall = lambda albedo: Material(materialIdx=0, albedo=albedo, cullBackface=True)
     I = np.eye(4)
@@ -100,12 +100,14 @@ def cornell():
     light = _model_from_quads([_quad((-0.005, 1.989, -0.03), (0.235, 0, 0), (0, 0, 0.19))])
     s.add_object(light, I, Material(materialIdx=0, albedo=(0.9, 0.9, 0.9), emission=(15.0, 15.0, 15.0),
                                     cullBackface=True))
+    cam = dict(eye=(0.0, 1.0, 3.4), center=(0.0, 1.0, 0.0), fov=40.0)
+    if not boxes:
+        return s, cam
     cube = s.define_object(_unit_cube())
     tall = _translate((-0.35, 0.6, -0.35)) @ _scale((0.6, 1.2, 0.6))
     short = _translate((0.35, 0.3, 0.3)) @ _rot_y(15.0) @ _scale((0.6, 0.6, 0.6))
     s.add_instance(cube, tall, Material(materialIdx=0, albedo=GREY))
     s.add_instance(cube, short, Material(materialIdx=0, albedo=GREY))
-    cam = dict(eye=(0.0, 1.0, 3.4), center=(0.0, 1.0, 0.0), fov=40.0)
     return s, cam
 
 
@@ -415,6 +417,72 @@ def instanced(copies=4096, seed=11, box=100.0):
     h = 0.5 * box
     cam = dict(eye=(h, h, -0.4 * box), center=(h, h, h), fov=60.0)
     return s, cam
+
+
+def disney_lobes():
+    """The Cornell box with its cubes replaced by eight spheres (radius 0.18,
+    288 triangles each) in two rows, each Disney material (materialIdx 3)
+    weighted toward one lobe of brdfDisney.h.glsl: clearcoat, sheen,
+    subsurface, anisotropic metal, rough glass (specularTransmission 1),
+    a glass / clearcoat mix, tinted specular, and every lobe at once.
+    Returns (Scene, camera dict)."""
+    s, cam = cornell(boxes=False)
+    sphere = s.define_object(_sphere_model(12, 12))
+    D = dict(materialIdx=3, ior=1.5, interpNormals=True)
+    mats = [Material(albedo=(0.8, 0.3, 0.2), roughness=0.5, clearcoat=1.0, clearcoatGloss=0.9, **D),
+            Material(albedo=(0.2, 0.4, 0.8), roughness=0.8, sheen=1.0, sheenTint=(1.0, 0.5, 0.5), **D),
+            Material(albedo=(0.9, 0.8, 0.6), roughness=0.4, subsurface=1.0, **D),
+            Material(albedo=(0.9, 0.7, 0.3), roughness=0.3, metallic=1.0, anisotropic=0.8, **D),
+            Material(albedo=(0.95, 0.95, 0.95), roughness=0.1, specularTransmission=1.0, **D),
+            Material(albedo=(0.7, 0.9, 0.7), roughness=0.2, specularTransmission=0.5, metallic=0.2,
+                     clearcoat=0.5, clearcoatGloss=0.3, **D),
+            Material(albedo=(0.6, 0.6, 0.6), roughness=0.6, metallic=0.5, specularTint=(0.3, 0.6, 1.0), **D),
+            Material(albedo=(0.5, 0.7, 0.9), roughness=0.35, metallic=0.3, clearcoat=0.3, clearcoatGloss=0.6,
+                     sheen=0.5, sheenTint=(0.5, 1.0, 0.5), subsurface=0.5, anisotropic=0.5,
+                     specularTransmission=0.3, **D)]
+    for k, m in enumerate(mats):
+        s.add_instance(sphere, _translate((-0.6 + 0.4 * (k % 4), 0.45 + 0.8 * (k // 4), -0.1 * (k // 4)))
+                       @ _scale((0.18, 0.18, 0.18)), m)
+    return s, cam
+
+
+def _slab(y_top, y_bottom, half=50.0):
+    """Two horizontal quads: the top one facing +y, the bottom one -y."""
+    return _model_from_quads([_quad((0, y_top, 0), (0, 0, half), (half, 0, 0)),
+                              _quad((0, y_bottom, 0), (half, 0, 0), (0, 0, half))])
+
+
+def _down_camera(fov=0.5):
+    return dict(eye=(0.0, 1.0, 0.0), center=(0.0, -10.0, 0.0), fov=fov, up=(0.0, 0.0, 1.0))
+
+
+def dielectric_stack():
+    """Horizontal dielectric slabs (IOR 1, roughness 0: rays at normal
+    incidence refract straight through) below a camera looking down -y:
+    B [0, -0.3] with absorption 1, C [-0.5, -0.6] emitting 2, D [-1, -3]
+    holding E [-1.4, -1.6]. With 6 bounces a path ends just after entering E
+    from inside D, 0.4 of distance accumulated and never applied."""
+    s = Scene()
+    I = np.eye(4)
+    glass = dict(materialIdx=2, albedo=(1.0, 1.0, 1.0), ior=1.0, roughness=0.0)
+    s.add_object(_slab(0.0, -0.3), I, Material(absorption=1.0, **glass))
+    s.add_object(_slab(-0.5, -0.6), I, Material(emission=(2.0, 2.0, 2.0), **glass))
+    s.add_object(_slab(-1.0, -3.0), I, Material(**glass))
+    s.add_object(_slab(-1.4, -1.6), I, Material(**glass))
+    return s, _down_camera()
+
+
+def nan_floor_slab():
+    """A clear glass slab [0, -0.5] (IOR 2.618: normal-incidence reflectance
+    0.2) above a Lambertian floor at y = -1 that emits NaN, under a camera
+    looking down -y: every sample either leaves upward (the zenith sky) or
+    reaches the floor (NaN)."""
+    s = Scene()
+    I = np.eye(4)
+    s.add_object(_slab(0.0, -0.5), I, Material(materialIdx=2, albedo=(1.0, 1.0, 1.0), ior=2.618, roughness=0.0))
+    s.add_object(_model_from_quads([_quad((0, -1.0, 0), (0, 0, 50.0), (50.0, 0, 0))]), I,
+                 Material(materialIdx=0, albedo=(0.5, 0.5, 0.5), emission=(float("nan"),) * 3))
+    return s, _down_camera()
 
 
 # ---- scene pools for batching and serving ------------------------------------------

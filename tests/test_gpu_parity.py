@@ -431,6 +431,39 @@ def test_render_defocus_bitexact(cuda, cornell_arrays, defocus, tail):
     dev.close()
 
 
+def _quirk_cases():
+    stack, scam = scenes.dielectric_stack()
+    nan, ncam = scenes.nan_floor_slab()
+    box, bcam = scenes.cornell()
+    return [("dielectric_leak_Q8", stack.build_arrays(), scam, 24, 20, 4, 6, {}),
+            ("nan_drop_Q15", nan.build_arrays(), ncam, 24, 20, 16, 12, {}),
+            ("direct_clamp_Q6", box.build_arrays(), bcam, 64, 64, 4, 6,
+             dict(direct_clamp=0.5, indirect_clamp=1e-3))]
+
+
+@pytest.mark.parametrize("tail", [0, 1 << 30], ids=["wavefront", "tail_kernel"])
+def test_render_raygen_quirks_bitexact(cuda, tail):
+    """The distance carried from one sample's glass into the next (Q8), NaN
+    samples dropped from the average (Q15) and directClamp on the whole
+    sample with indirectClamp / totalEmissiveWeight unused (Q6): the scenes
+    whose closed forms pin the oracle render bit for bit the same on the GPU."""
+    torch = cuda
+    for name, arrays, cam, W, H, spp, bounces, kw in _quirk_cases():
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, bounces,
+                         up=cam.get("up", (0.0, 1.0, 0.0)), **kw)
+        pc["totalEmissiveWeight"] = 7.0
+        dev = reina_rt.DeviceScene(arrays)
+        dev.set_tuning(tail_paths=tail)
+        got = gpu_render(torch, dev, pc, W, H)
+        ref, nrays = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)
+        same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+        assert same.all(), "%s: pixels differing: %d" % (name, (~same).any(-1).sum())
+        assert dev.stats().extension_rays == nrays, name
+        if name == "nan_drop_Q15":
+            assert (~np.isnan(got[..., 0])).mean() > 0.85
+        dev.close()
+
+
 def test_seed_wraps_mod_2_32(cuda, cornell_arrays):
     """Q10: the pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
     (raytrace.rgen.glsl:259), so at 64 x 64 (W * H = 2^12) batch 2^20 reuses
@@ -538,6 +571,31 @@ def test_render_nee_bitexact(cuda, flags, tail):
         plain = gpu_render(torch, dev, pc, W, H)
         assert not np.array_equal(_bits(plain), _bits(got))
         dev.close()
+
+
+@pytest.mark.parametrize("nee", [False, True], ids=["plain", "nee"])
+@pytest.mark.parametrize("flags,tail", [(0, 0), (0, 1 << 30), (abi.NODES_QUANTIZED, 500)])
+def test_render_disney_lobes_bitexact(cuda, nee, flags, tail):
+    """Every lobe of the Disney BSDF (clearcoat, sheen, subsurface,
+    anisotropic metal, glass with insideDielectric = choseGlass (Q4), and
+    mixes) renders bit for bit as the oracle's restatement of
+    brdfDisney.h.glsl, with and without NEE + MIS."""
+    torch = cuda
+    s, cam = scenes.disney_lobes()
+    arrays = s.build_arrays()
+    W, H = 48, 40
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp=4, max_bounces=8)
+    dev = reina_rt.DeviceScene(arrays, flags=flags)
+    dev.set_tuning(tail_paths=tail)
+    dev.set_trace_options(abi.TRACE_NEE if nee else 0)
+    got = gpu_render(torch, dev, pc, W, H)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_nee(nee)
+    ref, nrays = orc.render(pc, W, H, use_bvh=False)
+    same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), "pixels differing: %d" % (~same).any(-1).sum()
+    assert dev.stats().extension_rays == nrays
+    dev.close()
 
 
 def test_nee_needs_an_emitter(cuda):
