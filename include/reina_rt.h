@@ -287,6 +287,30 @@ typedef struct ReinaTraceStats {
  * default renders match the reference's active path. Needs an emissive
  * instance (REINA_RT_ERR_NO_EMISSIVE otherwise). */
 #define REINA_RT_TRACE_NEE         0x4u
+/* Corrections of reference behaviours that the default renders reproduce
+ * (no counterpart in the reference: each flag replaces one behaviour of its
+ * shaders by the textbook one, so frames then differ from the reference's).
+ *   FIX_LENS_RNG      the thin-lens sample advances the pixel's RNG stream;
+ *                     randomInUnitHexagon takes the state by value
+ *                     (shaders/raytrace/raytrace.rgen.glsl:194, 234), so the
+ *                     reference reuses the lens draws for the first bounce.
+ *   FIX_MEDIUM_RESET  accumulatedDistance is cleared when a sample starts;
+ *                     the reference clears only insideDielectric (:98), so
+ *                     distance travelled in glass by a path that ends there
+ *                     is charged to the next sample's first exit.
+ *   FIX_NORMALS       world normals by the inverse-transpose of the
+ *                     object-to-world matrix; the reference uses mat3(O2W)
+ *                     (closestHitCommon.h.glsl:113-114), which tilts normals
+ *                     under non-uniform scale.
+ *   FIX_METAL_PDF     the GGX reflection pdf is the density of the sampler in
+ *                     use (Heitz's VNDF, brdfDisney.h.glsl:9-40); the
+ *                     reference prices it with the bounded-VNDF pdf of Eto and
+ *                     Tokuyoshi (:55-73), larger by (i.z + t) / (k i.z + t). */
+#define REINA_RT_TRACE_FIX_LENS_RNG     0x8u
+#define REINA_RT_TRACE_FIX_MEDIUM_RESET 0x10u
+#define REINA_RT_TRACE_FIX_NORMALS      0x20u
+#define REINA_RT_TRACE_FIX_METAL_PDF    0x40u
+#define REINA_RT_TRACE_FIXES            0x78u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 

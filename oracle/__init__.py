@@ -38,6 +38,8 @@ def lib():
     L.or_export_emissive.restype = u64
     L.or_scene_set_nee.argtypes = [c_p, i32]
     L.or_scene_set_nee.restype = None
+    L.or_scene_set_fixes.argtypes = [c_p, u32]
+    L.or_scene_set_fixes.restype = None
     L.or_occluded.argtypes = [c_p, c_p]
     L.or_occluded.restype = i32
     L.or_scene_set_bvh.argtypes = [c_p, c_p, u64, c_p, u64, u32]
@@ -56,7 +58,7 @@ def lib():
     L.or_offset_position.restype = None
     L.or_sky.argtypes = [c_p, c_p]
     L.or_sky.restype = None
-    L.or_camera_ray.argtypes = [c_p, u32, u32, u32, u32, ctypes.POINTER(u32), c_p, c_p]
+    L.or_camera_ray.argtypes = [c_p, u32, u32, u32, u32, ctypes.POINTER(u32), c_p, c_p, i32]
     L.or_camera_ray.restype = None
     L.or_diffuse_dir.argtypes = [c_p, ctypes.POINTER(u32), c_p]
     L.or_diffuse_dir.restype = None
@@ -103,6 +105,11 @@ class OracleScene:
         """Next-event estimation with MIS in render / render_sum64 (the
         library's REINA_RT_TRACE_NEE)."""
         lib().or_scene_set_nee(self.h, int(bool(on)))
+
+    def set_fixes(self, fixes):
+        """The library's REINA_RT_TRACE_FIX_* corrections of reference
+        behaviours (a bit mask; 0 renders as the reference)."""
+        lib().or_scene_set_fixes(self.h, int(fixes))
 
     def export_emissive(self):
         """Light tables in the byte layout of reina_rt_scene_export_emissive."""
@@ -212,12 +219,12 @@ def sky(d):
     return out
 
 
-def camera_ray(pc, px, py, W, H, rng):
+def camera_ray(pc, px, py, W, H, rng, lens_advances=False):
     pcb = np.ascontiguousarray(np.asarray(pc).reshape(()))
     st = ctypes.c_uint32(rng)
     o = np.zeros(3, np.float32)
     d = np.zeros(3, np.float32)
-    lib().or_camera_ray(pcb.ctypes.data, px, py, W, H, ctypes.byref(st), _p(o), _p(d))
+    lib().or_camera_ray(pcb.ctypes.data, px, py, W, H, ctypes.byref(st), _p(o), _p(d), int(lens_advances))
     return o, d, st.value
 
 

@@ -122,6 +122,7 @@ uint64_t or_export_emissive(const OrScene* s, void* buf) {
 }
 
 void or_scene_set_nee(OrScene* s, int on) { s->nee = on; }
+void or_scene_set_fixes(OrScene* s, uint32_t fixes) { s->fixes = fixes; }
 
 OrScene* or_scene_create(const ReinaSceneDesc* d) {
     OrScene* s = (OrScene*)calloc(1, sizeof(OrScene));

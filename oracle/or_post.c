@@ -7,7 +7,8 @@
  *   tonemap(ping -> RGBA8)              shaders/postprocessing/tonemap/tonemapping.comp.glsl:41-84
  * Quirks kept: the Gaussian's sigma is the config's percent value while the
  * half-width is 3 * W * radius / 100 pixels, and skipped taps still add to the
- * weight sum (Q11); no sRGB encode (Q12). UNORM8 conversion: floor(c*255+0.5).
+ * weight sum (blurCommon.h.glsl:23-47); no sRGB encode of the UNORM output
+ * (tonemapping.comp.glsl:79-83). UNORM8 conversion: floor(c*255+0.5).
  */
 #include "oracle.h"
 

@@ -6,18 +6,22 @@
  * NEE is disabled exactly as in the reference (raytrace.rgen.glsl:146,
  * skipNEE = true), so weightBRDF = 1 and the direct term is 0.
  *
- * Quirks reproduced (each tested): per-pixel RNG shared across samples (Q2),
- * the hexagon lens sample drawn on a copy of the state (Q3), Disney sets
- * insideDielectric = choseGlass (Q4), the evalDisney weights (Q5), directClamp
- * on the whole sample (Q6), accumulatedDistance not reset per path (Q8;
- * initial value 0), random() = float(w) * 2^-32 (Q9), NaN samples dropped and
- * NaN when every sample is dropped (Q15), the TBN read even without
- * interpNormals (Q14; a model without TBN data reads zeros), metal sampled by
- * the unbounded VNDF but priced by the bounded-VNDF pdf (Q17). or_scene_set_nee
- * switches on this build's correct NEE + MIS instead.
+ * Reference behaviours reproduced (each tested): one RNG state per pixel
+ * shared by all samples and bounces (raytrace.rgen.glsl:259-264), the hexagon
+ * lens sample drawn on a copy of the state (:186-204), Disney sets
+ * insideDielectric = choseGlass (disney.rchit.glsl:191), the evalDisney lobe
+ * weights (brdfDisney.h.glsl:656-711), directClamp on the whole sample
+ * (raytrace.rgen.glsl:266), accumulatedDistance not reset per path (:98;
+ * initial value 0), random() = float(w) * 2^-32 (shaderCommon.h.glsl:39-45),
+ * NaN samples dropped and NaN when every sample is dropped
+ * (raytrace.rgen.glsl:268-277), the world normal by mat3(O2W) and the TBN read
+ * even without interpNormals (closestHitCommon.h.glsl:113-136; a model without
+ * TBN data reads zeros), metal sampled by the unbounded VNDF but priced by the
+ * bounded-VNDF pdf (brdfDisney.h.glsl:9-73). or_scene_set_nee switches on this
+ * build's correct NEE + MIS instead.
  *
  * Parity unpinned: the lobe weights with which sampleDisney / evalDisney mix
- * the lobes (Q5: inconsistent in the reference, so no identity pins them
+ * the lobes (inconsistent in the reference, so no identity pins them
  * beyond this line-by-line restatement). Each lobe beneath them is pinned
  * against its mathematics through or_disney_probe.
  */
@@ -132,13 +136,18 @@ static HitInfo getObjectHitInfo(const OrScene* s, const HitCtx* c) {
         r.uvy = (w[0] * bx + w[1] * by) + w[2] * bz;
     }
     m3 M = {in->c0, in->c1, in->c2};
-    r.worldNormal = vnormalize(m3mul(M, on));
-    r.worldNormalGeometry = vnormalize(m3mul(M, ong));
+    /* mat3(O2W) as the reference; its inverse-transpose with REINA_RT_TRACE_FIX_NORMALS */
+    float ndet = vdot(M.c0, vcross(M.c1, M.c2));
+    float ninv = 1.0f / ndet;
+    m3 NI = {vscale(vcross(M.c1, M.c2), ninv), vscale(vcross(M.c2, M.c0), ninv), vscale(vcross(M.c0, M.c1), ninv)};
+    m3 NM = (s->fixes & REINA_RT_TRACE_FIX_NORMALS) ? NI : M;
+    r.worldNormal = vnormalize(m3mul(NM, on));
+    r.worldNormalGeometry = vnormalize(m3mul(NM, ong));
     v3 D = c->worldRayDirection;
     r.frontFace = vdot(D, r.worldNormalGeometry) < 0.0f;
     r.worldNormal = vfaceforward(r.worldNormal, D, r.worldNormalGeometry);
     r.worldNormalGeometry = vfaceforward(r.worldNormalGeometry, D, r.worldNormalGeometry);
-    /* TBN (read unconditionally, Q14) */
+    /* TBN (read even without interpNormals) */
     v3 t1 = tbn_col(s, 3u * prim + to + 0, 0), b1 = tbn_col(s, 3u * prim + to + 0, 1), n1 = tbn_col(s, 3u * prim + to + 0, 2);
     v3 t2 = tbn_col(s, 3u * prim + to + 1, 0), b2 = tbn_col(s, 3u * prim + to + 1, 1), n2 = tbn_col(s, 3u * prim + to + 1, 2);
     v3 t3 = tbn_col(s, 3u * prim + to + 2, 0), b3 = tbn_col(s, 3u * prim + to + 2, 1), n3 = tbn_col(s, 3u * prim + to + 2, 2);
@@ -248,7 +257,8 @@ static float D_GGX_Aniso(v3 m, float ax, float ay) {                          /*
     return (inv * inv) / ((K_PI * ax) * ay);
 }
 
-static float pdfGGXReflection(v3 i, v3 o, float ax, float ay) {              /* :55-73 */
+/* bounded = 0: k = 1, the density of sampleGGXVNDF (REINA_RT_TRACE_FIX_METAL_PDF) */
+static float pdfGGXReflection(v3 i, v3 o, float ax, float ay, int bounded) {  /* :55-73 */
     v3 m = vnormalize(vadd(i, o));
     float ndf = D_GGX_Aniso(m, ax, ay);
     float aix = ax * i.x, aiy = ay * i.y;
@@ -258,7 +268,7 @@ static float pdfGGXReflection(v3 i, v3 o, float ax, float ay) {              /* 
         float a = or_clampf(or_minf(ax, ay), 0.0f, 1.0f);
         float s = 1.0f + sqrtf(i.x * i.x + i.y * i.y);
         float a2 = a * a, s2 = s * s;
-        float k = ((1.0f - a2) * s2) / (s2 + (a2 * i.z) * i.z);
+        float k = bounded ? ((1.0f - a2) * s2) / (s2 + (a2 * i.z) * i.z) : 1.0f;
         return ndf / (2.0f * (k * i.z + t));
     }
     return (ndf * (t - i.z)) / (2.0f * len2);
@@ -369,10 +379,10 @@ static v3 sampleMetal(m3 tbn, float anisotropic, float roughness, v3 wi, uint32_
     return vreflect(vneg(wi), h);
 }
 
-static float pdfMetal(m3 tbn, v3 wi, v3 wo, float anisotropic, float roughness) {           /* :287-299 */
+static float pdfMetal(m3 tbn, v3 wi, v3 wo, float anisotropic, float roughness, int bounded) { /* :287-299 */
     float ax, ay;
     alphas(anisotropic, roughness, &ax, &ay);
-    return pdfGGXReflection(m3tmul(tbn, wi), m3tmul(tbn, wo), ax, ay);
+    return pdfGGXReflection(m3tmul(tbn, wi), m3tmul(tbn, wo), ax, ay, bounded);
 }
 
 static float G1c(v3 w, float a) {                                             /* :313-319 */
@@ -470,7 +480,7 @@ static v3 evalMicrofacetRefraction(v3 bc, float anisotropic, float roughness, fl
 }
 
 static v3 glassEval(m3 tbn, v3 bc, float anisotropic, float roughness, float eta, v3 n, v3 wi, v3 wo,
-                    int didRefract, float* pdf) {                             /* :539-581 */
+                    int didRefract, int bounded, float* pdf) {                             /* :539-581 */
     v3 h = didRefract ? vnormalize(vadd(wo, vscale(wi, eta))) : vnormalize(vadd(wo, wi));
     if (vdot(h, n) < 0.0f) h = vneg(h);
     if (didRefract) {
@@ -480,7 +490,7 @@ static v3 glassEval(m3 tbn, v3 bc, float anisotropic, float roughness, float eta
         *pdf = tp;
         return f;
     }
-    *pdf = pdfMetal(tbn, wi, wo, anisotropic, roughness);
+    *pdf = pdfMetal(tbn, wi, wo, anisotropic, roughness, bounded);
     return evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, 0.0f, V3(1.0f, 1.0f, 1.0f), 1.0f, eta);
 }
 
@@ -513,7 +523,7 @@ static v3 sampleDisney(m3 tbn, v3 bc, float anisotropic, float roughness, float 
 
 static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotropic, float roughness,
                      float subsurface, float cg, float eta, float metallic, float clearcoat, float st,
-                     float sheen, int didRefract, v3 n, v3 wi, v3 wo, v3 h, float* pdf) {   /* :656-711 */
+                     float sheen, int didRefract, int bounded, v3 n, v3 wi, v3 wo, v3 h, float* pdf) {   /* :656-711 */
     float dW = (1.0f - st) * (1.0f - metallic);
     float mW = metallic;
     float cW = 0.25f * clearcoat;
@@ -523,11 +533,11 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
     float dP = pdfDiffuse(wo, n);
     v3 fs = evalSheen(bc, wo, h, n, sheenTint);
     v3 fm = evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, st, specularTint, metallic, eta);
-    float mP = pdfMetal(tbn, wi, wo, anisotropic, roughness);
+    float mP = pdfMetal(tbn, wi, wo, anisotropic, roughness, bounded);
     v3 fc = evalClearcoat(tbn, wi, wo, cg, h);
     float cP = pdfClearcoat(tbn, wi, wo, h, cg);
     float gP;
-    v3 fg = glassEval(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, &gP);
+    v3 fg = glassEval(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, &gP);
     *pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cW) + (gP * gW) / ws;
     v3 r = vscale(vadd(fd, vscale(fs, sheen)), dW / ws);
     r = vadd(r, vscale(fm, mW / ws));
@@ -542,7 +552,7 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
  *   0: D_GGX_Aniso(a), evalDm(a)            -> out[2k], out[2k+1]  (prm ax, ay)
  *   1: smithG(a), SmithGAniso(|a.z|, a.x, a.y) -> out[2k], out[2k+1]  (ax, ay)
  *   2: sampleGGXVNDF(a)                     -> out[3k..]            (ax, ay)
- *   3: pdfGGXReflection(a, b)               -> out[k]               (ax, ay)
+ *   3: pdfGGXReflection(a, b)               -> out[k]   (ax, ay, 1 for k = 1)
  *   4: sampleGTR1_attempt2                  -> out[3k..]            (alpha)
  *   5: evalDc(alpha, a)                     -> out[k]               (alpha)
  *   6: evalDiffuse(wi a, wo b, h = |a + b|) -> out[3k..]  (roughness, subsurface, bc[3])
@@ -564,7 +574,7 @@ void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const f
                 out[2 * k + 1] = SmithGAniso(fabsf(x.z), x.x, x.y, prm[0], prm[1]);
                 continue;
             case 2: r = sampleGGXVNDF(x, prm[0], prm[1], rng); break;
-            case 3: out[k] = pdfGGXReflection(x, y, prm[0], prm[1]); continue;
+            case 3: out[k] = pdfGGXReflection(x, y, prm[0], prm[1], prm[2] == 0.0f); continue;
             case 4: r = sampleGTR1_attempt2(prm[0], rng); break;
             case 5: out[k] = evalDc(prm[0], x); continue;
             case 6: r = evalDiffuse(prm[0], prm[1], V3(prm[2], prm[3], prm[4]), nz, x, y, vnormalize(vadd(x, y))); break;
@@ -778,7 +788,8 @@ static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c) {
     float pdf;
     v3 f = evalDisney(hi.tbn, albedo, pv3(p->specularTint), pv3(p->sheenTint), p->anisotropic,
                       p->roughness, p->subsurface, p->clearcoatGloss, eta, p->metallic, p->clearcoat,
-                      p->specularTransmission, p->sheen, didRefract, wn, wi, rayDir, h, &pdf);
+                      p->specularTransmission, p->sheen, didRefract, (s->fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0,
+                      wn, wi, rayDir, h, &pdf);
     float cosThetaI = or_maxf(vdot(wn, rayDir), 0.0f);
     v3 fc = vscale(f, cosThetaI);
     pld->color = V3(fc.x / pdf, fc.y / pdf, fc.z / pdf);
@@ -969,13 +980,14 @@ static void randomGaussian(uint32_t* rng, float* gx, float* gy) {
     *gy = r * or_sinf(theta);
 }
 
-/* raytrace.rgen.glsl:194-204 — state passed by value (Q3) */
-static void randomInUnitHexagon(uint32_t rng, float* px, float* py) {
+/* raytrace.rgen.glsl:194-204; advances *rng (the caller passes a copy, as the
+ * reference passes the state by value) */
+static void randomInUnitHexagon(uint32_t* rng, float* px, float* py) {
     const float sqrt3 = 1.73205080757f;
     float x, y;
     do {
-        x = 2.0f * or_random(&rng) - 1.0f;
-        y = (or_random(&rng) - 0.5f) * sqrt3;
+        x = 2.0f * or_random(rng) - 1.0f;
+        y = (or_random(rng) - 0.5f) * sqrt3;
     } while (fabsf(y) > sqrt3 * 0.5f || (sqrt3 * fabsf(x) + fabsf(y)) > sqrt3);
     *px = x;
     *py = y;
@@ -988,8 +1000,9 @@ static void m4mul(const float* m, const float* v, float* out) {
 }
 
 /* raytrace.rgen.glsl:206-247 */
+/* lens_advances: the lens draws advance *rng (REINA_RT_TRACE_FIX_LENS_RNG) */
 static void getStartingRay(const RtPushConsts* pc, float px, float py, float W, float H, uint32_t* rng,
-                           v3* o, v3* d) {
+                           v3* o, v3* d, int lens_advances) {
     float gx, gy;
     randomGaussian(rng, &gx, &gy);
     float cx = (px + 0.5f) + 0.375f * gx;
@@ -1010,7 +1023,9 @@ static void getStartingRay(const RtPushConsts* pc, float px, float py, float W, 
     v3 origin = V3(iv[12], iv[13], iv[14]);
     v3 focal = vadd(origin, vscale(rayDirection, pc->focusDist));
     float lx, ly;
-    randomInUnitHexagon(*rng, &lx, &ly);
+    uint32_t hr = *rng;   /* by value in the reference: the stream does not advance */
+    randomInUnitHexagon(&hr, &lx, &ly);
+    if (lens_advances) *rng = hr;
     lx = lx * pc->defocusMultiplier;
     ly = ly * pc->defocusMultiplier;
     v3 right = vnormalize(V3(iv[0], iv[1], iv[2]));
@@ -1022,16 +1037,16 @@ static void getStartingRay(const RtPushConsts* pc, float px, float py, float W, 
 }
 
 void or_camera_ray(const RtPushConsts* pc, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                   uint32_t* rng, float* origin, float* dir) {
+                   uint32_t* rng, float* origin, float* dir, int lens_advances) {
     v3 o, d;
-    getStartingRay(pc, (float)px, (float)py, (float)W, (float)H, rng, &o, &d);
+    getStartingRay(pc, (float)px, (float)py, (float)W, (float)H, rng, &o, &d, lens_advances);
     origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
 }
 
 /* raytrace.rgen.glsl:259-276 — the spp loop of one dispatch for one pixel,
  * seeded with `batch`; returns the number of non-NaN samples and their mean
- * (sum / float(actual): NaN when every sample was dropped, Q15). */
+ * (sum / float(actual): NaN when every sample was dropped). */
 static uint32_t pixel_mean(const OrScene* s, const RtPushConsts* pc, uint32_t batch, uint32_t x, uint32_t y,
                            uint32_t W, uint32_t H, int use_bvh, v3* mean, uint64_t* rays) {
     Payload pld;
@@ -1041,7 +1056,9 @@ static uint32_t pixel_mean(const OrScene* s, const RtPushConsts* pc, uint32_t ba
     v3 sum = V3(0.0f, 0.0f, 0.0f);
     for (uint32_t si = 0; si < pc->samplesPerPixel; ++si) {
         v3 o, d, L;
-        getStartingRay(pc, (float)x, (float)y, (float)W, (float)H, &pld.rngState, &o, &d);
+        getStartingRay(pc, (float)x, (float)y, (float)W, (float)H, &pld.rngState, &o, &d,
+                       (s->fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0);
+        if (s->fixes & REINA_RT_TRACE_FIX_MEDIUM_RESET) pld.accumulatedDistance = 0.0f;
         *rays += trace_segments(s, &pld, o, d, pc->maxBounces, use_bvh, &L);
         v3 c = V3(or_clampf(L.x, 0.0f, pc->directClamp), or_clampf(L.y, 0.0f, pc->directClamp),
                   or_clampf(L.z, 0.0f, pc->directClamp));

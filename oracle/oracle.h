@@ -56,6 +56,7 @@ typedef struct OrScene {
     int32_t* inst_emi;
     uint8_t* tri_flags;
     int nee;                 /* next-event estimation on in or_render        */
+    uint32_t fixes;          /* REINA_RT_TRACE_FIX_* corrections (0: as the reference) */
     /* optional BVH8 (exported from the library) */
     const uint8_t* nodes; uint64_t nnodes;
     uint32_t node_bytes;     /* 256: FP32 node, 80: 8-bit quantized node        */
@@ -93,7 +94,8 @@ void or_pcg_stream(uint32_t seed, uint32_t n, uint32_t* states, float* values);
 void or_offset_position(const float* p, const float* n, float* out, int dielectric, const float* dir);
 void or_sky(const float* dir, float* out);
 void or_camera_ray(const RtPushConsts* pc, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                   uint32_t* rng, float* origin, float* dir);
+                   uint32_t* rng, float* origin, float* dir, int lens_advances);
+void or_scene_set_fixes(OrScene* s, uint32_t fixes);
 void or_diffuse_dir(const float* n, uint32_t* rng, float* out);
 
 /* Full raygen (raytrace.rgen.glsl:250-285) for the pixel rectangle

@@ -506,6 +506,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     if (nee && s->emi.empty())
         return fail(REINA_RT_ERR_NO_EMISSIVE, "next-event estimation needs at least one emissive instance");
     P.nee = nee ? 1u : 0u;
+    P.fixes = s->options & REINA_RT_TRACE_FIXES;
     const bool timex = (s->options & REINA_RT_TRACE_TIME_EXTEND) != 0;
     const bool count = (s->options & REINA_RT_TRACE_COUNT) != 0;
     const uint32_t npx = W * H;
@@ -638,7 +639,7 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
 
 int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
     if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
-    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE))
+    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE | REINA_RT_TRACE_FIXES))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown trace option bits");
     s->options = options;
     return REINA_RT_OK;

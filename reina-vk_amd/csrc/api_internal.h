@@ -44,6 +44,7 @@ inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_
     P.sampleBatch = pc->sampleBatch;
     P.directClamp = pc->directClamp;
     P.nee = 0;
+    P.fixes = 0;
     return P;
 }
 

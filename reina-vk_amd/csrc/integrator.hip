@@ -57,9 +57,9 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         return;
     }
     V3 o, d;
-    starting_ray(P.cam, (float)x, (float)y, rng, o, d);
+    starting_ray(P.cam, (float)x, (float)y, rng, o, d, (P.fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0u);
     S.st0[i] = make_uint4(rng, 0u, 0u, 0u);
-    S.st1[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);   // accumulatedDistance starts at 0 (Q8)
+    S.st1[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);   // accumulatedDistance starts at 0; not reset per path
     S.st2[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     S.st3[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     if (P.nee) S.st4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

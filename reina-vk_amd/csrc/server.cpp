@@ -403,6 +403,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     r.scene = scene;
     r.P = make_trace_params(pc, W, H);
     r.P.nee = nee ? 1u : 0u;
+    r.P.fixes = scene->options & REINA_RT_TRACE_FIXES;
     r.img = (float4*)image;
     r.npx = npx;
     r.ntiles = ntiles;

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j
             S.st0[slot] = make_uint4(rng, P.spp, 0u, 0u);
             S.st3[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(P.spp));
         } else {
-            starting_ray(P.cam, (float)x, (float)y, rng, o, d);
+            starting_ray(P.cam, (float)x, (float)y, rng, o, d, (P.fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0u);
             S.st0[slot] = make_uint4(rng, 0u, 0u, 0u);
             S.st1[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
             S.st2[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

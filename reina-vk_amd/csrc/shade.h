@@ -71,8 +71,9 @@ RT_D void shade_record(const SceneView& sv, uint32_t g, float4* r) {
 }
 
 // closestHitCommon.h.glsl:52-148, from the triangle's shading record
+// fix_normals: world normals by the inverse-transpose (REINA_RT_TRACE_FIX_NORMALS)
 RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaInstanceProperties& p,
-                      const float4* rec, uint32_t prim, float bu, float bv, V3 D) {
+                      const float4* rec, uint32_t prim, float bu, float bv, V3 D, bool fix_normals) {
     HitInfo r;
     float4 q[kShadeRecF4];
 #pragma unroll
@@ -113,8 +114,15 @@ RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaIns
     }
     M3 M;
     M.c0 = c0; M.c1 = c1; M.c2 = c2;
-    r.worldNormal = normalize(mmul(M, on));
-    r.worldNormalGeometry = normalize(mmul(M, ong));
+    float det = dot(M.c0, cross(M.c1, M.c2));
+    float invdet = 1.0f / det;
+    M3 Ni;
+    Ni.c0 = scale(cross(M.c1, M.c2), invdet);
+    Ni.c1 = scale(cross(M.c2, M.c0), invdet);
+    Ni.c2 = scale(cross(M.c0, M.c1), invdet);
+    const M3& NM = fix_normals ? Ni : M;
+    r.worldNormal = normalize(mmul(NM, on));
+    r.worldNormalGeometry = normalize(mmul(NM, ong));
     r.frontFace = dot(D, r.worldNormalGeometry) < 0.0f;
     r.worldNormal = faceforward(r.worldNormal, D, r.worldNormalGeometry);
     r.worldNormalGeometry = faceforward(r.worldNormalGeometry, D, r.worldNormalGeometry);
@@ -130,12 +138,6 @@ RT_D HitInfo hit_info(const SceneView& sv, const InstanceRec& in, const ReinaIns
         else { T = add(T, scale(tk, w3[k])); B = add(B, scale(bk, w3[k])); N = add(N, scale(nk, w3[k])); }
     }
     V3 tangent = normalize(T), bitangent = normalize(B), normal = normalize(N);
-    float det = dot(M.c0, cross(M.c1, M.c2));
-    float invdet = 1.0f / det;
-    M3 Ni;
-    Ni.c0 = scale(cross(M.c1, M.c2), invdet);
-    Ni.c1 = scale(cross(M.c2, M.c0), invdet);
-    Ni.c2 = scale(cross(M.c0, M.c1), invdet);
     V3 wT = normalize(mmul(M, tangent));
     V3 wB = normalize(mmul(M, bitangent));
     V3 wN = normalize(mmul(Ni, normal));
@@ -217,7 +219,9 @@ RT_D float d_ggx_aniso(V3 m, float ax, float ay) {                          // :
     return (inv * inv) / ((kPi * ax) * ay);
 }
 
-RT_D float pdf_ggx_reflection(V3 i, V3 o, float ax, float ay) {            // :55-73
+// bounded: the bounded-VNDF pdf as the reference; else k = 1, the density of
+// sample_ggx_vndf (REINA_RT_TRACE_FIX_METAL_PDF)
+RT_D float pdf_ggx_reflection(V3 i, V3 o, float ax, float ay, bool bounded) {   // :55-73
     V3 m = normalize(add(i, o));
     float ndf = d_ggx_aniso(m, ax, ay);
     float aix = ax * i.x, aiy = ay * i.y;
@@ -227,7 +231,7 @@ RT_D float pdf_ggx_reflection(V3 i, V3 o, float ax, float ay) {            // :5
         float a = clampf_(minf_(ax, ay), 0.0f, 1.0f);
         float s = 1.0f + sqrtf_(i.x * i.x + i.y * i.y);
         float a2 = a * a, s2 = s * s;
-        float k = ((1.0f - a2) * s2) / (s2 + (a2 * i.z) * i.z);
+        float k = bounded ? ((1.0f - a2) * s2) / (s2 + (a2 * i.z) * i.z) : 1.0f;
         return ndf / (2.0f * (k * i.z + t));
     }
     return (ndf * (t - i.z)) / (2.0f * len2);
@@ -330,10 +334,10 @@ RT_D V3 sample_metal(const M3& tbn, float anisotropic, float roughness, V3 wi, u
     return reflect(neg(wi), h);
 }
 
-RT_D float pdf_metal(const M3& tbn, V3 wi, V3 wo, float anisotropic, float roughness) {           // :287-299
+RT_D float pdf_metal(const M3& tbn, V3 wi, V3 wo, float anisotropic, float roughness, bool bounded) {   // :287-299
     float ax, ay;
     alphas(anisotropic, roughness, ax, ay);
-    return pdf_ggx_reflection(tmul(tbn, wi), tmul(tbn, wo), ax, ay);
+    return pdf_ggx_reflection(tmul(tbn, wi), tmul(tbn, wo), ax, ay, bounded);
 }
 
 RT_D float g1c(V3 w, float a) {                                             // :313-319
@@ -428,7 +432,7 @@ RT_D V3 eval_microfacet_refraction(V3 bc, float anisotropic, float roughness, fl
 }
 
 RT_D V3 eval_glass(const M3& tbn, V3 bc, float anisotropic, float roughness, float eta, V3 n, V3 wi, V3 wo,
-                   bool didRefract, float& pdf) {                           // :539-581
+                   bool didRefract, bool bounded, float& pdf) {             // :539-581
     V3 h = didRefract ? normalize(add(wo, scale(wi, eta))) : normalize(add(wo, wi));
     if (dot(h, n) < 0.0f) h = neg(h);
     if (didRefract) {
@@ -438,7 +442,7 @@ RT_D V3 eval_glass(const M3& tbn, V3 bc, float anisotropic, float roughness, flo
         pdf = tp;
         return f;
     }
-    pdf = pdf_metal(tbn, wi, wo, anisotropic, roughness);
+    pdf = pdf_metal(tbn, wi, wo, anisotropic, roughness, bounded);
     return eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, 0.0f, v3(1.0f, 1.0f, 1.0f), 1.0f, eta);
 }
 
@@ -470,7 +474,7 @@ RT_D V3 sample_disney(const M3& tbn, float anisotropic, float roughness, float c
 
 RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float anisotropic, float roughness,
                     float subsurface, float cg, float eta, float metallic, float clearcoat, float st, float sheen,
-                    bool didRefract, V3 n, V3 wi, V3 wo, V3 h, float& pdf) { // :656-711
+                    bool didRefract, bool bounded, V3 n, V3 wi, V3 wo, V3 h, float& pdf) { // :656-711
     float dW = (1.0f - st) * (1.0f - metallic);
     float mW = metallic;
     float cW = 0.25f * clearcoat;
@@ -480,11 +484,11 @@ RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float a
     float dP = pdf_diffuse(wo, n);
     V3 fs = eval_sheen(bc, wo, h, n, sheenTint);
     V3 fm = eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, st, specularTint, metallic, eta);
-    float mP = pdf_metal(tbn, wi, wo, anisotropic, roughness);
+    float mP = pdf_metal(tbn, wi, wo, anisotropic, roughness, bounded);
     V3 fc = eval_clearcoat(tbn, wi, wo, cg, h);
     float cP = pdf_clearcoat(tbn, wi, wo, h, cg);
     float gP;
-    V3 fg = eval_glass(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, gP);
+    V3 fg = eval_glass(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, gP);
     pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cW) + (gP * gW) / ws;
     V3 r = scale(add(fd, scale(fs, sheen)), dW / ws);
     r = add(r, scale(fm, mW / ws));
@@ -653,8 +657,9 @@ RT_D void chs_dielectric(Payload& pld, const SceneView& sv, const HitInfo& hi, c
 }
 
 // disney.rchit.glsl:36-198
+// bounded_pdf: false with REINA_RT_TRACE_FIX_METAL_PDF
 RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O,
-                     V3 D) {
+                     V3 D, bool bounded_pdf) {
     if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
     float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
     if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
@@ -676,7 +681,7 @@ RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const
     float pdf;
     V3 f = eval_disney(hi.tbn, albedo, ldv3(p.specularTint), ldv3(p.sheenTint), p.anisotropic, p.roughness,
                        p.subsurface, p.clearcoatGloss, eta, p.metallic, p.clearcoat, p.specularTransmission, p.sheen,
-                       didRefract, wn, wi, rayDir, h, pdf);
+                       didRefract, bounded_pdf, wn, wi, rayDir, h, pdf);
     float cosThetaI = maxf_(dot(wn, rayDir), 0.0f);
     pld.color = divs(scale(f, cosThetaI), pdf);
     pld.albedo = albedo;
@@ -706,7 +711,8 @@ RT_D void m4mul(const float* m, float x, float y, float z, float w, float* o) {
     for (int r = 0; r < 4; ++r) o[r] = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * w;
 }
 
-RT_D void starting_ray(const Camera& cam, float px, float py, uint32_t& rng, V3& o, V3& d) {
+// lens_advances: the lens draws advance rng (REINA_RT_TRACE_FIX_LENS_RNG)
+RT_D void starting_ray(const Camera& cam, float px, float py, uint32_t& rng, V3& o, V3& d, bool lens_advances) {
     float u1 = maxf_(1e-5f, rnd(rng));
     float u2 = rnd(rng);
     float r = sqrtf_(-2.0f * logf_(u1));
@@ -727,7 +733,7 @@ RT_D void starting_ray(const Camera& cam, float px, float py, uint32_t& rng, V3&
     V3 rayDirection = normalize(v3(wd[0], wd[1], wd[2]));
     V3 origin = v3(cam.invView[12], cam.invView[13], cam.invView[14]);
     V3 focal = add(origin, scale(rayDirection, cam.focusDist));
-    // randomInUnitHexagon takes the state by value (Q3)
+    // randomInUnitHexagon takes the state by value: the stream does not advance
     uint32_t hr = rng;
     const float sqrt3 = 1.73205080757f;
     float hx, hy;
@@ -735,6 +741,7 @@ RT_D void starting_ray(const Camera& cam, float px, float py, uint32_t& rng, V3&
         hx = 2.0f * rnd(hr) - 1.0f;
         hy = (rnd(hr) - 0.5f) * sqrt3;
     } while (fabsf(hy) > sqrt3 * 0.5f || (sqrt3 * fabsf(hx) + fabsf(hy)) > sqrt3);
+    if (lens_advances) rng = hr;
     float lx = hx * cam.defocus, ly = hy * cam.defocus;
     V3 right = normalize(v3(cam.invView[0], cam.invView[1], cam.invView[2]));
     V3 up = normalize(v3(cam.invView[4], cam.invView[5], cam.invView[6]));

@@ -118,12 +118,12 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         const uint32_t ii = __float_as_uint(rec[0].w), pm = __float_as_uint(rec[1].w);
         const InstanceRec in = sv.inst[ii];
         const ReinaInstanceProperties& pr = sv.props[__float_as_uint(rec[2].w)];
-        HitInfo hi = hit_info(sv, in, pr, rec, pm, hh.z, hh.w, D);
+        HitInfo hi = hit_info(sv, in, pr, rec, pm, hh.z, hh.w, D, (P.fixes & REINA_RT_TRACE_FIX_NORMALS) != 0u);
         switch (in.material) {
             case 0: chs_lambert_metal(pld, sv, hi, pr, D, false); break;
             case 1: chs_lambert_metal(pld, sv, hi, pr, D, true); break;
             case 2: chs_dielectric(pld, sv, hi, pr, O, D); break;
-            default: chs_disney(pld, sv, hi, pr, O, D); break;
+            default: chs_disney(pld, sv, hi, pr, O, D, (P.fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0u); break;
         }
         if (nee && !pld.skip) {
             lambert_hit = in.material == 0u;
@@ -207,11 +207,12 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         sample += 1u;
         if (sample < P.spp) {
             uint32_t x = px % P.W, y = px / P.W;
-            starting_ray(P.cam, (float)x, (float)y, pld.rng, r.no, r.nd);
+            starting_ray(P.cam, (float)x, (float)y, pld.rng, r.no, r.nd, (P.fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0u);
             thr = v3(1.0f, 1.0f, 1.0f);
             L = v3(0.0f, 0.0f, 0.0f);
             seg = 0;
             pld.insideDielectric = false;   // raytrace.rgen.glsl:98
+            if (P.fixes & REINA_RT_TRACE_FIX_MEDIUM_RESET) pld.accumulatedDistance = 0.0f;
             s4.w = 0.0f;
             r.push = true;
         }

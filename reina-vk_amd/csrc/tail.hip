@@ -1,7 +1,7 @@
 // tail.hip — the end of a frame in one launch.
 //
 // A frame's bounce iterations shrink as pixels finish their samples, but a
-// pixel's samples are sequential (one RNG state per pixel, Q2 of the
+// pixel's samples are sequential (one RNG state per pixel in the
 // reference: shaders/raytrace/raytrace.rgen.glsl:259-264), so the last few
 // thousand pixels with long paths need hundreds more iterations, each paying
 // a launch and the latency of its slowest traversal. Once few paths remain,

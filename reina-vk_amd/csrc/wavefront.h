@@ -16,6 +16,7 @@ struct TraceParams {
     uint32_t sampleBatch;
     float directClamp;
     uint32_t nee;   // next-event estimation on (not a reference push constant)
+    uint32_t fixes; // REINA_RT_TRACE_FIX_* corrections of reference behaviours (0: as the reference)
 };
 
 // The ray queues are split into kParts partitions, one per XCD: workgroup b

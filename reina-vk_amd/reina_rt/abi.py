@@ -171,6 +171,11 @@ REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 TRACE_TIME_EXTEND = 0x1
 TRACE_COUNT = 0x2
 TRACE_NEE = 0x4
+TRACE_FIX_LENS_RNG = 0x8
+TRACE_FIX_MEDIUM_RESET = 0x10
+TRACE_FIX_NORMALS = 0x20
+TRACE_FIX_METAL_PDF = 0x40
+TRACE_FIXES = 0x78
 
 
 # status codes

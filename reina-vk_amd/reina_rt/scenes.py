@@ -69,6 +69,13 @@ def _rot_y(deg):
     return m
 
 
+def _rot_z(deg):
+    a = np.radians(deg)
+    m = np.eye(4)
+    m[0, 0], m[0, 1], m[1, 0], m[1, 1] = np.cos(a), -np.sin(a), np.sin(a), np.cos(a)
+    return m
+
+
 def _unit_cube():
     h = 0.5
     faces = [
@@ -483,6 +490,24 @@ def nan_floor_slab():
     s.add_object(_model_from_quads([_quad((0, -1.0, 0), (0, 0, 50.0), (50.0, 0, 0))]), I,
                  Material(materialIdx=0, albedo=(0.5, 0.5, 0.5), emission=(float("nan"),) * 3))
     return s, _down_camera()
+
+
+TILTED_MIRROR_XFORM = np.diag([1.5, 1.0, 1.0, 1.0]) @ _rot_z(20.0)
+
+
+def tilted_mirror():
+    """A mirror plane (metal, roughness 0, albedo (0.9, 0.8, 0.7)) through
+    the origin, object normal +y, placed by the non-uniform transform
+    TILTED_MIRROR_XFORM = diag(1.5, 1, 1) * rot_z(20 deg): its true normal
+    normalize(S^-1 R y) is tilted 13.6 deg from +y, the mat3(O2W) one
+    normalize(S R y) 28.6 deg. A camera looks down at it from (0, 3, 0.3);
+    both reflections leave the plane."""
+    s = Scene()
+    # small enough that the FP32 intersection error (ulp of the vertex
+    # distances) stays well below the 256-ulp origin offset: no self-hits
+    plane = _model_from_quads([_quad((0, 0, 0), (0, 0, 2.0), (2.0, 0, 0))])
+    s.add_object(plane, TILTED_MIRROR_XFORM, Material(materialIdx=1, albedo=(0.9, 0.8, 0.7), roughness=0.0))
+    return s, dict(eye=(0.0, 3.0, 0.3), center=(0.0, 0.0, 0.0), fov=10.0)
 
 
 # ---- scene pools for batching and serving ------------------------------------------

@@ -413,7 +413,7 @@ def test_running_mean_bitexact(cuda, cornell_arrays):
 @pytest.mark.parametrize("defocus", [0.015, 0.2], ids=["config_default", "strong"])
 def test_render_defocus_bitexact(cuda, cornell_arrays, defocus, tail):
     """Depth of field: the hexagon lens sample of raytrace.rgen.glsl:186-200,
-    drawn from a copy of the pixel's RNG state (Q3: the lens sample does not
+    drawn from a copy of the pixel's RNG state (the lens sample does not
     advance the stream). config/config.toml's defocus_multiplier = 1.5 is
     0.015 after Reina.cpp's division by 100."""
     torch = cuda
@@ -435,17 +435,17 @@ def _quirk_cases():
     stack, scam = scenes.dielectric_stack()
     nan, ncam = scenes.nan_floor_slab()
     box, bcam = scenes.cornell()
-    return [("dielectric_leak_Q8", stack.build_arrays(), scam, 24, 20, 4, 6, {}),
-            ("nan_drop_Q15", nan.build_arrays(), ncam, 24, 20, 16, 12, {}),
-            ("direct_clamp_Q6", box.build_arrays(), bcam, 64, 64, 4, 6,
+    return [("dielectric_distance_leak", stack.build_arrays(), scam, 24, 20, 4, 6, {}),
+            ("nan_samples_dropped", nan.build_arrays(), ncam, 24, 20, 16, 12, {}),
+            ("direct_clamp", box.build_arrays(), bcam, 64, 64, 4, 6,
              dict(direct_clamp=0.5, indirect_clamp=1e-3))]
 
 
 @pytest.mark.parametrize("tail", [0, 1 << 30], ids=["wavefront", "tail_kernel"])
 def test_render_raygen_quirks_bitexact(cuda, tail):
-    """The distance carried from one sample's glass into the next (Q8), NaN
-    samples dropped from the average (Q15) and directClamp on the whole
-    sample with indirectClamp / totalEmissiveWeight unused (Q6): the scenes
+    """The distance carried from one sample's glass into the next, NaN
+    samples dropped from the average and directClamp on the whole sample
+    with indirectClamp / totalEmissiveWeight unused: the scenes
     whose closed forms pin the oracle render bit for bit the same on the GPU."""
     torch = cuda
     for name, arrays, cam, W, H, spp, bounces, kw in _quirk_cases():
@@ -459,13 +459,52 @@ def test_render_raygen_quirks_bitexact(cuda, tail):
         same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
         assert same.all(), "%s: pixels differing: %d" % (name, (~same).any(-1).sum())
         assert dev.stats().extension_rays == nrays, name
-        if name == "nan_drop_Q15":
+        if name == "nan_samples_dropped":
             assert (~np.isnan(got[..., 0])).mean() > 0.85
         dev.close()
 
 
+def _fix_scenes():
+    box, bcam = scenes.cornell()
+    stack, scam = scenes.dielectric_stack()
+    mirror, mcam = scenes.tilted_mirror()
+    lobes, lcam = scenes.disney_lobes()
+    return [("cornell_defocus", box.build_arrays(), bcam, dict(defocus=0.1), abi.TRACE_FIX_LENS_RNG),
+            ("dielectric_stack", stack.build_arrays(), scam, {}, abi.TRACE_FIX_MEDIUM_RESET),
+            ("tilted_mirror", mirror.build_arrays(), mcam, {}, abi.TRACE_FIX_NORMALS),
+            ("disney_lobes", lobes.build_arrays(), lcam, {}, abi.TRACE_FIX_METAL_PDF)]
+
+
+@pytest.mark.parametrize("fixes", [abi.TRACE_FIX_LENS_RNG, abi.TRACE_FIX_MEDIUM_RESET, abi.TRACE_FIX_NORMALS,
+                                   abi.TRACE_FIX_METAL_PDF, abi.TRACE_FIXES],
+                         ids=["lens_rng", "medium_reset", "normals", "metal_pdf", "all"])
+def test_render_fixes_bitexact(cuda, fixes):
+    """Each REINA_RT_TRACE_FIX_* correction (and all together) renders bit
+    for bit as the oracle with the same correction, through the wavefront
+    and the tail kernel, and changes the frame of the scene it is about."""
+    torch = cuda
+    for name, arrays, cam, kw, target in _fix_scenes():
+        W, H = 32, 24
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 4, 6, up=cam.get("up", (0.0, 1.0, 0.0)), **kw)
+        orc = oracle.OracleScene(arrays, make_desc)
+        orc.set_fixes(fixes)
+        ref, nrays = orc.render(pc, W, H, use_bvh=False)
+        dev = reina_rt.DeviceScene(arrays)
+        dev.set_trace_options(fixes)
+        for tail in (0, 1 << 30):
+            dev.set_tuning(tail_paths=tail)
+            got = gpu_render(torch, dev, pc, W, H)
+            same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+            assert same.all(), "%s tail=%d: pixels differing: %d" % (name, tail, (~same).any(-1).sum())
+            assert dev.stats().extension_rays == nrays
+        if fixes & target:
+            dev.set_trace_options(0)
+            assert not np.array_equal(_bits(gpu_render(torch, dev, pc, W, H)), _bits(got)), name
+        dev.close()
+
+
 def test_seed_wraps_mod_2_32(cuda, cornell_arrays):
-    """Q10: the pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
+    """The pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
     (raytrace.rgen.glsl:259), so at 64 x 64 (W * H = 2^12) batch 2^20 reuses
     batch 0's streams. GPU = oracle bit for bit around the wrap (a 64-bit seed
     would differ), and batch 2^20 blended onto batch 0's image gives back that
@@ -577,7 +616,7 @@ def test_render_nee_bitexact(cuda, flags, tail):
 @pytest.mark.parametrize("flags,tail", [(0, 0), (0, 1 << 30), (abi.NODES_QUANTIZED, 500)])
 def test_render_disney_lobes_bitexact(cuda, nee, flags, tail):
     """Every lobe of the Disney BSDF (clearcoat, sheen, subsurface,
-    anisotropic metal, glass with insideDielectric = choseGlass (Q4), and
+    anisotropic metal, glass with insideDielectric = choseGlass, and
     mixes) renders bit for bit as the oracle's restatement of
     brdfDisney.h.glsl, with and without NEE + MIS."""
     torch = cuda

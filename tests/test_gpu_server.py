@@ -221,6 +221,33 @@ def test_server_nee_mix(cuda, pool, tail):
     i_nee.close()
 
 
+@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF], ids=["tail_default", "tail_off"])
+def test_server_fixes_mix(cuda, pool, tail):
+    """Requests on scenes whose trace options carry REINA_RT_TRACE_FIX_*
+    corrections, mixed with requests on the same scenes without them: every
+    image equals reina_rt_trace's on the same scene and options."""
+    torch = cuda
+    c_arrays, c_cam, c_dev = pool["cornell"]
+    i_arrays, i_cam, i_dev = pool["ico"]
+    c_fix = reina_rt.DeviceScene(c_arrays)
+    c_fix.set_trace_options(abi.TRACE_FIXES)
+    i_fix = reina_rt.DeviceScene(i_arrays)
+    i_fix.set_trace_options(abi.TRACE_FIX_METAL_PDF | abi.TRACE_FIX_LENS_RNG)
+    jobs = []
+    for dev, W, H, spp, cam in ((c_fix, 48, 40, 4, c_cam), (c_dev, 48, 40, 4, c_cam), (i_fix, 64, 36, 2, i_cam),
+                                (i_dev, 64, 36, 2, i_cam), (c_fix, 21, 13, 3, c_cam)):
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, 8, defocus=0.05)
+        jobs.append((dev, pc, W, H, None))
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, tail_paths=tail)
+    mixed = served(torch, srv, jobs)
+    for (dev, pc, W, H, init), got in zip(jobs, mixed):
+        assert _same(got, plain(torch, dev, pc, W, H, init))
+    assert not _same(mixed[0], mixed[1]) and not _same(mixed[2], mixed[3])
+    srv.close()
+    c_fix.close()
+    i_fix.close()
+
+
 def test_server_nee_refusals(cuda, pool):
     """A NEE request on a server without REINA_RT_SERVER_NEE is refused
     (REINA_RT_ERR_UNSUPPORTED), and one on a scene without emissive

@@ -1,6 +1,6 @@
 """Pins the oracle's camera rays (raytrace.rgen.glsl:194-247): the Gaussian
 pixel jitter takes exactly two PCG draws, the hexagon lens sample reads a copy
-of the state left after them and does not advance it (Q3), every defocused ray
+of the state left after them and does not advance it, every defocused ray
 passes through the pinhole ray's focal point at focusDist, and lens offsets
 are uniform over the regular hexagon with vertices (+-1, 0) scaled by
 defocusMultiplier (area 3*sqrt(3)/2; the band |x| <= 1/2 holds 2/3 of it)."""

@@ -14,8 +14,7 @@ mathematics they come from, each through the oracle's own float32 functions
   (SIGGRAPH Asia 2023, doi 10.1145/3610543.3626163): over the sphere of
   reflected directions it integrates to (i.z + t) / (k i.z + t), t =
   |(ax ix, ay iy, iz)|, k from the paper's Eq. 5-6; the renderer samples the
-  unbounded VNDF, whose density is this pdf times (k i.z + t) / (i.z + t)
-  (Q17).
+  unbounded VNDF, whose density is this pdf times (k i.z + t) / (i.z + t).
 - GTR1 (evalDc :328-333) is normalised and sampleGTR1_attempt2 (:96-111)
   draws cos(theta_h) with the CDF ln(1 - (1 - a^2) mu^2) / ln(a^2) obtained by
   integrating it; pdfClearcoat (:368-383) has as much mass over the upper

@@ -1,6 +1,6 @@
 """Pins random() (shaderCommon.h.glsl:39-45): an LCG step s' = 747796405 s + 1
 (mod 2^32) followed by the RXS-M-XS output permutation, converted as
-float(w) / 4294967295.0f, where the constant rounds to 2^32 (Q9)."""
+float(w) / 4294967295.0f, where the constant rounds to 2^32."""
 import json
 import os
 from fractions import Fraction
@@ -44,7 +44,7 @@ def _output(s):
 
 
 def _inv_output(w):
-    """Inverse of the RXS-M-XS permutation (for the Q9 known answer)."""
+    """Inverse of the RXS-M-XS permutation (for the value-one known answer)."""
     # undo w ^= w >> 22
     x = w ^ (w >> 22)
     # undo multiply
@@ -59,7 +59,7 @@ def _inv_output(w):
 
 
 def test_float_conversion_and_value_one():
-    """float(w) * 2^-32 in RNE; w >= 2^32 - 128 returns exactly 1.0 (Q9)."""
+    """float(w) * 2^-32 in RNE; w >= 2^32 - 128 returns exactly 1.0."""
     s_star = _inv_output(M - 1)
     assert _output(s_star) == M - 1
     prev = ((s_star - C) * pow(A, -1, M)) % M

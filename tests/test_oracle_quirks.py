@@ -1,22 +1,27 @@
 """Pins three behaviours of the reference's raygen loop in the oracle with
 closed forms on constructed scenes:
 
-- Q8: accumulatedDistance is reset only by the Lambertian / metal / Disney
+- accumulatedDistance is reset only by the Lambertian / metal / Disney
   hit shaders and by a dielectric exit, never at the start of a path
   (raytrace.rgen.glsl:98 resets only insideDielectric), so distance travelled
   inside glass by a path that ends there is charged to the next sample's
   first exit.
-- Q15: a NaN sample is dropped from the pixel's average
+- a NaN sample is dropped from the pixel's average
   (raytrace.rgen.glsl:268-277), not counted as zero.
-- Q6: directClamp clamps the whole path sample (raytrace.rgen.glsl:266);
+- directClamp clamps the whole path sample (raytrace.rgen.glsl:266);
   indirectClamp and totalEmissiveWeight do nothing on this path.
+- the world normal is mat3(O2W) applied to the object normal
+  (closestHitCommon.h.glsl:113-114), by a mirror under non-uniform scale.
+
+And the library's REINA_RT_TRACE_FIX_* corrections of these behaviours, each
+against its own closed form.
 """
 import numpy as np
 import pytest
 
 import oracle
-from reina_rt import scenes
-from reina_rt.scene import make_desc, push_consts
+from reina_rt import abi, scenes
+from reina_rt.scene import Material, Scene, make_desc, push_consts
 
 
 def _render(scene, cam, W, H, spp, bounces, **kw):
@@ -72,3 +77,102 @@ def test_direct_clamp_whole_sample_and_unused_constants():
     pc["totalEmissiveWeight"] = 123.0
     other, _ = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False, nthreads=4)
     assert np.array_equal(other.view(np.uint32), bits)
+
+
+# ---- the REINA_RT_TRACE_FIX_* corrections ---------------------------------------------
+def _render_fixed(scene, cam, W, H, spp, bounces, fixes, **kw):
+    arrays = scene.build_arrays()
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, bounces, up=cam.get("up", (0.0, 1.0, 0.0)),
+                     **kw)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_fixes(fixes)
+    img, _ = orc.render(pc, W, H, use_bvh=False, nthreads=4)
+    return img, pc
+
+
+def _sky64(d):
+    d = np.asarray(d, np.float64)
+    t = 0.5 * (d[..., 1] / np.linalg.norm(d, axis=-1) + 1.0)
+    lo, hi = np.array([0.1, 0.1, 0.1]), np.array([0.4, 1.7, 2.0])
+    return (lo * (1 - t)[..., None] + hi * t[..., None]) * 0.07
+
+
+@pytest.mark.parametrize("spp", [1, 4])
+def test_fix_medium_reset_clears_the_leak(spp):
+    s, cam = scenes.dielectric_stack()
+    img, _ = _render_fixed(s, cam, 6, 5, spp, 6, abi.TRACE_FIX_MEDIUM_RESET)
+    assert np.allclose(img[..., :3], 2.0 * np.exp(-0.3), rtol=1e-3)
+
+
+def test_fix_lens_rng_advances_past_the_hexagon_draws():
+    _, pc = _render_fixed(scenes.cornell()[0], scenes.cornell()[1], 4, 4, 1, 1, 0, defocus=0.3)
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        s = int(rng.integers(0, 2 ** 32))
+        o0, d0, st0 = oracle.camera_ray(pc, 2, 1, 4, 4, s)
+        o1, d1, st1 = oracle.camera_ray(pc, 2, 1, 4, 4, s, lens_advances=True)
+        assert np.array_equal(o0, o1) and np.array_equal(d0, d1)      # the same ray
+        states, vals = oracle.pcg_stream(st0, 64)                       # draws after the jitter
+        vals = vals.astype(np.float64)
+        for k in range(0, 64, 2):
+            hx, hy = 2.0 * vals[k] - 1.0, (vals[k + 1] - 0.5) * np.sqrt(3.0)
+            if abs(hy) <= np.sqrt(3.0) / 2 and np.sqrt(3.0) * abs(hx) + abs(hy) <= np.sqrt(3.0):
+                break
+        assert st1 == int(states[k + 1])                                # just past the accepted pair
+
+
+@pytest.mark.parametrize("fix", [False, True], ids=["as_reference", "inverse_transpose"])
+def test_tilted_mirror_normal_closed_form(fix):
+    """Reflection off a mirror placed by diag(1.5, 1, 1) * rot_z(20 deg): the
+    reference's mat3(O2W) normal normalize(S R y), or with FIX_NORMALS the
+    true normal normalize(S^-1 R y)."""
+    s, cam = scenes.tilted_mirror()
+    W, H = 6, 5
+    img, pc = _render_fixed(s, cam, W, H, 1, 2, abi.TRACE_FIX_NORMALS if fix else 0)
+    M = scenes.TILTED_MIRROR_XFORM[:3, :3]
+    true_n = np.linalg.inv(M).T @ np.array([0.0, 1.0, 0.0])
+    true_n /= np.linalg.norm(true_n)
+    n = true_n if fix else M @ np.array([0.0, 1.0, 0.0])
+    n /= np.linalg.norm(n)
+    albedo = np.array([0.9, 0.8, 0.7])
+    leaves = 0
+    for y in range(H):
+        for x in range(W):
+            _, d, _ = oracle.camera_ray(pc, x, y, W, H, y * W + x)
+            d = d.astype(np.float64) / np.linalg.norm(d)
+            r = d - 2.0 * (d @ n) * n
+            sky = albedo * _sky64(r)
+            if r @ true_n > 0:
+                leaves += 1
+                assert np.allclose(img[y, x, :3], sky, rtol=2e-4), (x, y)
+            else:
+                # into the mirror's half-space from an origin offset by ~1e-6:
+                # either it hits the plane again (two segments: dark) or the
+                # FP32 test misses that hit at t ~ 1e-6 and it reaches the sky
+                assert (np.allclose(img[y, x, :3], sky, rtol=2e-4) or (img[y, x, :3] == 0).all()), (x, y)
+    assert leaves == W * H
+
+
+def test_fix_metal_pdf():
+    """With k = 1 the GGX reflection pdf integrates to 1 over reflected
+    directions (it is the VNDF sampler's density); a fully metallic Disney
+    surface then weights every sample by at least as much as with the
+    bounded pdf (same directions, smaller pdf)."""
+    from test_oracle_disney import _dirs, _slope_grid
+    for ax, ay in [(0.5, 0.5), (0.3, 0.1)]:
+        for th in (0, 40, 70):
+            i = _dirs(th, 60)
+            m, _, _, w = _slope_grid(ax, ay, n_psi=2000, n_phi=128)
+            im = m @ i
+            o = 2.0 * im[:, None] * m - i
+            pdf, _ = oracle.disney_probe("pdf_refl", len(m), a=i, b=o, prm=(ax, ay, 1.0))
+            integral = np.sum(pdf[:, 0].astype(np.float64) * 4.0 * np.maximum(im, 0.0) * m[:, 2] ** 3 * w)
+            assert abs(integral - 1.0) < 5e-4, (ax, ay, th, integral)
+    s = Scene()
+    s.add_object(scenes._sphere_model(16, 16), np.eye(4),
+                 Material(materialIdx=3, albedo=(1.0, 1.0, 1.0), roughness=0.6, metallic=1.0, ior=1.5))
+    cam = dict(eye=(0.0, 0.0, 3.0), center=(0.0, 0.0, 0.0), fov=40.0)
+    ref, _ = _render_fixed(s, cam, 16, 12, 2, 2, 0)
+    fixed, _ = _render_fixed(s, cam, 16, 12, 2, 2, abi.TRACE_FIX_METAL_PDF)
+    assert (fixed[..., :3] >= ref[..., :3]).all()
+    assert (fixed[..., :3] > ref[..., :3] * 1.01).any()
