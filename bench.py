@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="apply this tuning instead of autotuning: stack_depth,refill_min,grid_blocks,shade_blocks,"
+                         "waves_per_simd,tail_paths (the config.tuning of a default run)")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
     ap.add_argument("--no-device-build", action="store_true", help="skip the device-built BVH comparison")
     ap.add_argument("--instanced", action="store_true",
@@ -119,7 +122,13 @@ def main():
     if not (a.no_secondary and a.no_autotune and a.no_quantized):
         sec, sh = build_secondary(a.secondary_reps, arrays, dev, cam, W, H, torch)
     autotune = None
-    if not a.no_autotune:
+    if a.tuning:
+        keys = ("stack_depth", "refill_min", "grid_blocks", "shade_blocks", "waves_per_simd", "tail_paths")
+        vals = [int(v) for v in a.tuning.split(",")]
+        if len(vals) != len(keys):
+            sys.exit("--tuning takes %d comma-separated integers: %s" % (len(keys), ",".join(keys)))
+        dev.set_tuning(**dict(zip(keys, vals)))
+    elif not a.no_autotune:
         autotune = run_autotune(dev, sec, sh, torch, frame=lambda: frame(0))
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
         with open(os.path.join(ROOT, "gpurun_out", "autotune_table.json"), "w") as f:
@@ -156,11 +165,14 @@ def main():
 
     if autotune is not None:
         out["config"]["autotune"] = autotune
+    if autotune is not None or a.tuning:
         out["config"]["tuning"] = dev.get_tuning()
     if not a.no_roofline:
         out["roofline"] = roofline(dev, frame, NODE_BYTES, torch)
         if (W, H, a.spp, a.depth) == (1920, 1080, 64, 8):
-            out["roofline"]["traffic"] = profiled_traffic("k_trace_icosphere_1080p_64spp.md")
+            tr, tt = profiled_traffic("k_trace_icosphere_1080p_64spp.md", dev.get_tuning())
+            out["roofline"]["traffic"] = tr
+            out["roofline"]["traffic_tuning"] = tt
 
     if not a.no_secondary:
         out["secondary_rays"] = secondary_rays(a, dev, sec, sh, torch, NODE_BYTES)
@@ -204,16 +216,25 @@ def main():
 
 
 
-def profiled_traffic(name):
-    """traffic_bytes_per_launch from a committed rocprofv3 summary, or None."""
-    p = os.path.join(ROOT, "profiles", name)
+def profiled_traffic(name, tuning):
+    """(traffic_bytes_per_launch, tuning it was measured at) from a committed
+    rocprofv3 summary: the line for this tuning (stack_depth,refill_min,
+    grid_blocks,shade_blocks,waves_per_simd,tail_paths) if the summary has
+    one, else its first; (None, None) without a summary."""
+    key = ",".join(str(tuning[k]) for k in ("stack_depth", "refill_min", "grid_blocks", "shade_blocks",
+                                             "waves_per_simd", "tail_paths"))
+    found = []
     try:
-        for line in open(p):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch "):
+                t, v = line[len("traffic_bytes_per_launch "):].split(":")
+                found.append((t.strip(), int(v)))
     except OSError:
         pass
-    return None
+    for t, v in found:
+        if t == key:
+            return v, t
+    return (found[0][1], found[0][0]) if found else (None, None)
 
 def run_autotune(dev, sec, sh, torch, frame=None):
     """Stage 1 on the secondary batch; stage 2 (when `frame` is given) on
