@@ -305,12 +305,21 @@ typedef struct ReinaTraceStats {
  *   FIX_METAL_PDF     the GGX reflection pdf is the density of the sampler in
  *                     use (Heitz's VNDF, brdfDisney.h.glsl:9-40); the
  *                     reference prices it with the bounded-VNDF pdf of Eto and
- *                     Tokuyoshi (:55-73), larger by (i.z + t) / (k i.z + t). */
+ *                     Tokuyoshi (:55-73), larger by (i.z + t) / (k i.z + t).
+ *   FIX_DISNEY_MIX    evalDisney's mixture as sampleDisney draws it: the
+ *                     clearcoat weight inside the normalising sum (the
+ *                     reference leaves it out, :680-709), the metal lobe's
+ *                     dielectric F0 from the IOR alone (the reference passes
+ *                     specularTransmission as `specular`, :692, so an opaque
+ *                     dielectric has F0 = 0), and the glass pdf times the
+ *                     probability of the branch taken (F or 1 - F, :571-580).
+ *                     With FIX_METAL_PDF the pdf is then the sampler's density. */
 #define REINA_RT_TRACE_FIX_LENS_RNG     0x8u
 #define REINA_RT_TRACE_FIX_MEDIUM_RESET 0x10u
 #define REINA_RT_TRACE_FIX_NORMALS      0x20u
 #define REINA_RT_TRACE_FIX_METAL_PDF    0x40u
-#define REINA_RT_TRACE_FIXES            0x78u
+#define REINA_RT_TRACE_FIX_DISNEY_MIX   0x80u
+#define REINA_RT_TRACE_FIXES            0xF8u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 

@@ -229,7 +229,8 @@ def camera_ray(pc, px, py, W, H, rng, lens_advances=False):
 
 
 DISNEY_PROBES = {"ggx_d": (0, 2), "smith_g1": (1, 2), "vndf": (2, 3), "pdf_refl": (3, 1), "gtr1": (4, 3),
-                 "dc": (5, 1), "diffuse": (6, 3), "sheen": (7, 3), "pdf_clearcoat": (8, 1), "clearcoat": (9, 3)}
+                 "dc": (5, 1), "diffuse": (6, 3), "sheen": (7, 3), "pdf_clearcoat": (8, 1), "clearcoat": (9, 3),
+                 "disney_sample": (10, 4), "disney_pdf": (11, 1)}
 
 
 def disney_probe(name, n, a=None, b=None, prm=(), seed=1):
@@ -239,7 +240,7 @@ def disney_probe(name, n, a=None, b=None, prm=(), seed=1):
     fn, width = DISNEY_PROBES[name]
     a = None if a is None else np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n, 3)))
     b = None if b is None else np.ascontiguousarray(np.broadcast_to(np.asarray(b, np.float32), (n, 3)))
-    p = np.zeros(8, np.float32)
+    p = np.zeros(32, np.float32)
     p[:len(prm)] = prm
     st = ctypes.c_uint32(seed)
     out = np.zeros((n, width), np.float32)

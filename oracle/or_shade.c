@@ -20,10 +20,12 @@
  * bounded-VNDF pdf (brdfDisney.h.glsl:9-73). or_scene_set_nee switches on this
  * build's correct NEE + MIS instead.
  *
- * Parity unpinned: the lobe weights with which sampleDisney / evalDisney mix
- * the lobes (inconsistent in the reference, so no identity pins them
- * beyond this line-by-line restatement). Each lobe beneath them is pinned
- * against its mathematics through or_disney_probe.
+ * Parity unpinned: three terms of evalDisney's mixture that the reference
+ * gets wrong (clearcoat outside the normalising sum, specularTransmission
+ * passed as `specular`, no glass branch probability), restated line for
+ * line. With REINA_RT_TRACE_FIX_DISNEY_MIX they are replaced, and the
+ * mixture pdf is then pinned as the density of sampleDisney. Each lobe
+ * beneath is pinned against its mathematics through or_disney_probe.
  */
 #include "oracle.h"
 
@@ -479,18 +481,26 @@ static v3 evalMicrofacetRefraction(v3 bc, float anisotropic, float roughness, fl
     return V3(r.x / den2, r.y / den2, r.z / den2);
 }
 
+/* branch_pdf: times the probability of sampleGlass's branch, F for reflection
+ * (1 under total internal reflection), 1 - F for refraction
+ * (REINA_RT_TRACE_FIX_DISNEY_MIX); the reference leaves it out */
 static v3 glassEval(m3 tbn, v3 bc, float anisotropic, float roughness, float eta, v3 n, v3 wi, v3 wo,
-                    int didRefract, int bounded, float* pdf) {                             /* :539-581 */
+                    int didRefract, int bounded, int branch_pdf, float* pdf) {              /* :539-581 */
     v3 h = didRefract ? vnormalize(vadd(wo, vscale(wi, eta))) : vnormalize(vadd(wo, wi));
     if (vdot(h, n) < 0.0f) h = vneg(h);
+    v3 wiT = m3tmul(tbn, wi), hT = m3tmul(tbn, h);
     if (didRefract) {
         float tp;
-        v3 f = evalMicrofacetRefraction(bc, anisotropic, roughness, eta, m3tmul(tbn, wi), m3tmul(tbn, wo),
-                                        m3tmul(tbn, h), &tp);
-        *pdf = tp;
+        v3 f = evalMicrofacetRefraction(bc, anisotropic, roughness, eta, wiT, m3tmul(tbn, wo), hT, &tp);
+        *pdf = branch_pdf ? tp * (1.0f - reflectance(vdot(wiT, hT), eta)) : tp;
         return f;
     }
     *pdf = pdfMetal(tbn, wi, wo, anisotropic, roughness, bounded);
+    if (branch_pdf) {
+        float c = vdot(wiT, hT);
+        float sn = sqrtf(1.0f - c * c);
+        *pdf = *pdf * ((eta * sn > 1.0f) ? 1.0f : reflectance(c, eta));
+    }
     return evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, 0.0f, V3(1.0f, 1.0f, 1.0f), 1.0f, eta);
 }
 
@@ -523,25 +533,33 @@ static v3 sampleDisney(m3 tbn, v3 bc, float anisotropic, float roughness, float 
 
 static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotropic, float roughness,
                      float subsurface, float cg, float eta, float metallic, float clearcoat, float st,
-                     float sheen, int didRefract, int bounded, v3 n, v3 wi, v3 wo, v3 h, float* pdf) {   /* :656-711 */
+                     float sheen, int didRefract, uint32_t fixes, v3 n, v3 wi, v3 wo, v3 h,
+                     float* pdf) {   /* :656-711 */
+    /* REINA_RT_TRACE_FIX_METAL_PDF: the sampler's density for GGX reflection;
+     * REINA_RT_TRACE_FIX_DISNEY_MIX: clearcoat inside the normalised mixture,
+     * specular = 1 for the metal lobe's dielectric F0, glass branch pdf */
+    int bounded = (fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0;
+    int mix = (fixes & REINA_RT_TRACE_FIX_DISNEY_MIX) != 0;
     float dW = (1.0f - st) * (1.0f - metallic);
     float mW = metallic;
     float cW = 0.25f * clearcoat;
     float gW = (1.0f - metallic) * st;
     float ws = (dW + mW) + gW;
+    if (mix) ws = ws + cW;
+    float cN = mix ? cW / ws : cW;
     v3 fd = evalDiffuse(roughness, subsurface, bc, n, wi, wo, h);
     float dP = pdfDiffuse(wo, n);
     v3 fs = evalSheen(bc, wo, h, n, sheenTint);
-    v3 fm = evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, st, specularTint, metallic, eta);
+    v3 fm = evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, mix ? 1.0f : st, specularTint, metallic, eta);
     float mP = pdfMetal(tbn, wi, wo, anisotropic, roughness, bounded);
     v3 fc = evalClearcoat(tbn, wi, wo, cg, h);
     float cP = pdfClearcoat(tbn, wi, wo, h, cg);
     float gP;
-    v3 fg = glassEval(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, &gP);
-    *pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cW) + (gP * gW) / ws;
+    v3 fg = glassEval(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, mix, &gP);
+    *pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cN) + (gP * gW) / ws;
     v3 r = vscale(vadd(fd, vscale(fs, sheen)), dW / ws);
     r = vadd(r, vscale(fm, mW / ws));
-    r = vadd(r, vscale(fc, cW));
+    r = vadd(r, vscale(fc, cN));
     r = vadd(r, vscale(fg, gW / ws));
     return r;
 }
@@ -558,7 +576,13 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
  *   6: evalDiffuse(wi a, wo b, h = |a + b|) -> out[3k..]  (roughness, subsurface, bc[3])
  *   7: evalSheen(wo b, h = |a + b|)         -> out[3k..]  (-, -, bc[3], sheenTint[3])
  *   8: pdfClearcoat(wi a, wo b, h = |a + b|) -> out[k]              (clearcoatGloss)
- *   9: sampleClearcoat(wi a)                -> out[3k..]            (clearcoatGloss) */
+ *   9: sampleClearcoat(wi a)                -> out[3k..]            (clearcoatGloss)
+ *  10: sampleDisney(wi a)                   -> out[4k..] (wo, didRefract)
+ *  11: evalDisney pdf(wi a, wo b, h = |a + b|, didRefract = b.z < 0) -> out[k]
+ *      10 and 11 read the material from prm: bc[3] 0-2, specularTint[3] 3-5,
+ *      sheenTint[3] 6-8, anisotropic 9, roughness 10, subsurface 11,
+ *      clearcoatGloss 12, eta 13, metallic 14, clearcoat 15,
+ *      specularTransmission 16, sheen 17, fixes 18 (as a float) */
 void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const float* prm, uint32_t* rng,
                      float* out) {
     const m3 I = {V3(1.0f, 0.0f, 0.0f), V3(0.0f, 1.0f, 0.0f), V3(0.0f, 0.0f, 1.0f)};
@@ -583,6 +607,21 @@ void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const f
                 break;
             case 8: out[k] = pdfClearcoat(I, x, y, vnormalize(vadd(x, y)), prm[0]); continue;
             case 9: r = sampleClearcoat(I, prm[0], x, rng); break;
+            case 10: {
+                int dr, cg;
+                r = sampleDisney(I, V3(prm[0], prm[1], prm[2]), prm[9], prm[10], prm[12], prm[13], prm[14], prm[15],
+                                 prm[16], nz, x, &dr, &cg, rng);
+                out[4 * k] = r.x; out[4 * k + 1] = r.y; out[4 * k + 2] = r.z; out[4 * k + 3] = (float)dr;
+                continue;
+            }
+            case 11: {
+                float pdf;
+                (void)evalDisney(I, V3(prm[0], prm[1], prm[2]), V3(prm[3], prm[4], prm[5]), V3(prm[6], prm[7], prm[8]),
+                                 prm[9], prm[10], prm[11], prm[12], prm[13], prm[14], prm[15], prm[16], prm[17],
+                                 y.z < 0.0f, (uint32_t)prm[18], nz, x, y, vnormalize(vadd(x, y)), &pdf);
+                out[k] = pdf;
+                continue;
+            }
             default: return;
         }
         out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
@@ -788,8 +827,7 @@ static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c) {
     float pdf;
     v3 f = evalDisney(hi.tbn, albedo, pv3(p->specularTint), pv3(p->sheenTint), p->anisotropic,
                       p->roughness, p->subsurface, p->clearcoatGloss, eta, p->metallic, p->clearcoat,
-                      p->specularTransmission, p->sheen, didRefract, (s->fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0,
-                      wn, wi, rayDir, h, &pdf);
+                      p->specularTransmission, p->sheen, didRefract, s->fixes, wn, wi, rayDir, h, &pdf);
     float cosThetaI = or_maxf(vdot(wn, rayDir), 0.0f);
     v3 fc = vscale(f, cosThetaI);
     pld->color = V3(fc.x / pdf, fc.y / pdf, fc.z / pdf);

@@ -431,18 +431,26 @@ RT_D V3 eval_microfacet_refraction(V3 bc, float anisotropic, float roughness, fl
     return divs(r, fabsf(L.z * V.z));
 }
 
+// branch_pdf: the pdf includes the probability of sample_glass's branch, F for
+// reflection (1 under total internal reflection) and 1 - F for refraction
+// (REINA_RT_TRACE_FIX_DISNEY_MIX); the reference leaves it out.
 RT_D V3 eval_glass(const M3& tbn, V3 bc, float anisotropic, float roughness, float eta, V3 n, V3 wi, V3 wo,
-                   bool didRefract, bool bounded, float& pdf) {             // :539-581
+                   bool didRefract, bool bounded, bool branch_pdf, float& pdf) {   // :539-581
     V3 h = didRefract ? normalize(add(wo, scale(wi, eta))) : normalize(add(wo, wi));
     if (dot(h, n) < 0.0f) h = neg(h);
+    const V3 wiT = tmul(tbn, wi), hT = tmul(tbn, h);
     if (didRefract) {
         float tp;
-        V3 f = eval_microfacet_refraction(bc, anisotropic, roughness, eta, tmul(tbn, wi), tmul(tbn, wo),
-                                          tmul(tbn, h), tp);
-        pdf = tp;
+        V3 f = eval_microfacet_refraction(bc, anisotropic, roughness, eta, wiT, tmul(tbn, wo), hT, tp);
+        pdf = branch_pdf ? tp * (1.0f - reflectance(dot(wiT, hT), eta)) : tp;
         return f;
     }
     pdf = pdf_metal(tbn, wi, wo, anisotropic, roughness, bounded);
+    if (branch_pdf) {
+        const float c = dot(wiT, hT);
+        const float s = sqrtf_(1.0f - c * c);
+        pdf = pdf * ((eta * s > 1.0f) ? 1.0f : reflectance(c, eta));
+    }
     return eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, 0.0f, v3(1.0f, 1.0f, 1.0f), 1.0f, eta);
 }
 
@@ -474,25 +482,32 @@ RT_D V3 sample_disney(const M3& tbn, float anisotropic, float roughness, float c
 
 RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float anisotropic, float roughness,
                     float subsurface, float cg, float eta, float metallic, float clearcoat, float st, float sheen,
-                    bool didRefract, bool bounded, V3 n, V3 wi, V3 wo, V3 h, float& pdf) { // :656-711
+                    bool didRefract, uint32_t fixes, V3 n, V3 wi, V3 wo, V3 h, float& pdf) { // :656-711
+    // fixes: REINA_RT_TRACE_FIX_METAL_PDF (sampler density for GGX reflection),
+    // REINA_RT_TRACE_FIX_DISNEY_MIX (clearcoat inside the normalised mixture,
+    // specular = 1 for the metal lobe's dielectric F0, glass branch pdf)
+    const bool bounded = (fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0u;
+    const bool mix = (fixes & REINA_RT_TRACE_FIX_DISNEY_MIX) != 0u;
     float dW = (1.0f - st) * (1.0f - metallic);
     float mW = metallic;
     float cW = 0.25f * clearcoat;
     float gW = (1.0f - metallic) * st;
     float ws = (dW + mW) + gW;
+    if (mix) ws = ws + cW;
+    const float cN = mix ? cW / ws : cW;
     V3 fd = eval_diffuse(roughness, subsurface, bc, n, wi, wo, h);
     float dP = pdf_diffuse(wo, n);
     V3 fs = eval_sheen(bc, wo, h, n, sheenTint);
-    V3 fm = eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, st, specularTint, metallic, eta);
+    V3 fm = eval_metal(tbn, bc, anisotropic, roughness, n, wi, wo, h, mix ? 1.0f : st, specularTint, metallic, eta);
     float mP = pdf_metal(tbn, wi, wo, anisotropic, roughness, bounded);
     V3 fc = eval_clearcoat(tbn, wi, wo, cg, h);
     float cP = pdf_clearcoat(tbn, wi, wo, h, cg);
     float gP;
-    V3 fg = eval_glass(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, gP);
-    pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cW) + (gP * gW) / ws;
+    V3 fg = eval_glass(tbn, bc, anisotropic, roughness, eta, n, wi, wo, didRefract, bounded, mix, gP);
+    pdf = (((dP * dW) / ws + (mP * mW) / ws) + cP * cN) + (gP * gW) / ws;
     V3 r = scale(add(fd, scale(fs, sheen)), dW / ws);
     r = add(r, scale(fm, mW / ws));
-    r = add(r, scale(fc, cW));
+    r = add(r, scale(fc, cN));
     r = add(r, scale(fg, gW / ws));
     return r;
 }
@@ -657,9 +672,9 @@ RT_D void chs_dielectric(Payload& pld, const SceneView& sv, const HitInfo& hi, c
 }
 
 // disney.rchit.glsl:36-198
-// bounded_pdf: false with REINA_RT_TRACE_FIX_METAL_PDF
+// fixes: the REINA_RT_TRACE_FIX_* bits of the trace (eval_disney reads two)
 RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O,
-                     V3 D, bool bounded_pdf) {
+                     V3 D, uint32_t fixes) {
     if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
     float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
     if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
@@ -681,7 +696,7 @@ RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const
     float pdf;
     V3 f = eval_disney(hi.tbn, albedo, ldv3(p.specularTint), ldv3(p.sheenTint), p.anisotropic, p.roughness,
                        p.subsurface, p.clearcoatGloss, eta, p.metallic, p.clearcoat, p.specularTransmission, p.sheen,
-                       didRefract, bounded_pdf, wn, wi, rayDir, h, pdf);
+                       didRefract, fixes, wn, wi, rayDir, h, pdf);
     float cosThetaI = maxf_(dot(wn, rayDir), 0.0f);
     pld.color = divs(scale(f, cosThetaI), pdf);
     pld.albedo = albedo;

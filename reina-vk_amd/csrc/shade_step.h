@@ -123,7 +123,7 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
             case 0: chs_lambert_metal(pld, sv, hi, pr, D, false); break;
             case 1: chs_lambert_metal(pld, sv, hi, pr, D, true); break;
             case 2: chs_dielectric(pld, sv, hi, pr, O, D); break;
-            default: chs_disney(pld, sv, hi, pr, O, D, (P.fixes & REINA_RT_TRACE_FIX_METAL_PDF) == 0u); break;
+            default: chs_disney(pld, sv, hi, pr, O, D, P.fixes); break;
         }
         if (nee && !pld.skip) {
             lambert_hit = in.material == 0u;

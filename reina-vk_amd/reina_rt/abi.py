@@ -175,7 +175,8 @@ TRACE_FIX_LENS_RNG = 0x8
 TRACE_FIX_MEDIUM_RESET = 0x10
 TRACE_FIX_NORMALS = 0x20
 TRACE_FIX_METAL_PDF = 0x40
-TRACE_FIXES = 0x78
+TRACE_FIX_DISNEY_MIX = 0x80
+TRACE_FIXES = 0xF8
 
 
 # status codes

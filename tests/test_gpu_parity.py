@@ -472,12 +472,12 @@ def _fix_scenes():
     return [("cornell_defocus", box.build_arrays(), bcam, dict(defocus=0.1), abi.TRACE_FIX_LENS_RNG),
             ("dielectric_stack", stack.build_arrays(), scam, {}, abi.TRACE_FIX_MEDIUM_RESET),
             ("tilted_mirror", mirror.build_arrays(), mcam, {}, abi.TRACE_FIX_NORMALS),
-            ("disney_lobes", lobes.build_arrays(), lcam, {}, abi.TRACE_FIX_METAL_PDF)]
+            ("disney_lobes", lobes.build_arrays(), lcam, {}, abi.TRACE_FIX_METAL_PDF | abi.TRACE_FIX_DISNEY_MIX)]
 
 
 @pytest.mark.parametrize("fixes", [abi.TRACE_FIX_LENS_RNG, abi.TRACE_FIX_MEDIUM_RESET, abi.TRACE_FIX_NORMALS,
-                                   abi.TRACE_FIX_METAL_PDF, abi.TRACE_FIXES],
-                         ids=["lens_rng", "medium_reset", "normals", "metal_pdf", "all"])
+                                   abi.TRACE_FIX_METAL_PDF, abi.TRACE_FIX_DISNEY_MIX, abi.TRACE_FIXES],
+                         ids=["lens_rng", "medium_reset", "normals", "metal_pdf", "disney_mix", "all"])
 def test_render_fixes_bitexact(cuda, fixes):
     """Each REINA_RT_TRACE_FIX_* correction (and all together) renders bit
     for bit as the oracle with the same correction, through the wavefront

@@ -220,3 +220,62 @@ def test_sheen_limits():
     c = np.cos(np.radians(70))
     assert np.allclose(grey[0], (1 - c) ** 5 * c, rtol=1e-5)
     assert np.allclose(tint[0], np.array(bc) / lum * (1 - c) ** 5 * c, rtol=1e-5)
+
+
+# ---- the whole mixture: sampleDisney against evalDisney's pdf ----------------------
+MATERIALS = {   # bc, specularTint, sheenTint, anisotropic, roughness, subsurface, cg, metallic, clearcoat, st, sheen
+    "clearcoat_plastic": ((0.8, 0.3, 0.2), (1, 1, 1), (0, 0, 0), 0.0, 0.5, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0),
+    "anisotropic_metal": ((0.9, 0.7, 0.3), (1, 1, 1), (0, 0, 0), 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0),
+    "four_lobes": ((0.5, 0.7, 0.9), (1, 1, 1), (0.5, 1, 0.5), 0.3, 0.5, 0.5, 0.0, 0.3, 0.5, 0.4, 0.5),
+    "glass": ((0.95, 0.95, 0.95), (1, 1, 1), (0, 0, 0), 0.0, 0.4, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0),
+}
+
+
+def _material_prm(name, fixes, eta=1.0 / 1.5):
+    bc, stint, shtint, an, rough, ss, cg, metal, cc, st, sheen = MATERIALS[name]
+    return tuple(bc) + tuple(stint) + tuple(shtint) + (an, rough, ss, cg, eta, metal, cc, st, sheen, float(fixes))
+
+
+def _mixture_chi2(name, fixes, th, n=300000, K=12, J=12, S=24):
+    wi = _dirs(th, 25)
+    prm = _material_prm(name, fixes)
+    smp, _ = oracle.disney_probe("disney_sample", n, a=wi, prm=prm, seed=5 + th)
+    wo = smp[:, :3].astype(np.float64)
+    up = wo[:, 2] > 0
+    mu = np.clip(wo[up, 2] / np.linalg.norm(wo[up], axis=1), 0, 1 - 1e-12)
+    phi = np.mod(np.arctan2(wo[up, 1], wo[up, 0]), 2 * np.pi)
+    counts = np.bincount(np.minimum((mu * K).astype(int), K - 1) * J + np.minimum((phi / (2 * np.pi) * J).astype(int), J - 1),
+                         minlength=K * J)
+    # expected: the pdf over each equal-area bin (d omega = d mu d phi)
+    mg = (np.arange(K * S) + 0.5) / (K * S)
+    pg = (np.arange(J * S) + 0.5) / (J * S) * 2 * np.pi
+    M, P = np.meshgrid(mg, pg, indexing="ij")
+    sn = np.sqrt(1 - M ** 2)
+    dirs = np.stack([sn * np.cos(P), sn * np.sin(P), M], -1).reshape(-1, 3)
+    pdf, _ = oracle.disney_probe("disney_pdf", len(dirs), a=wi, b=dirs, prm=prm)
+    cell = (1.0 / (K * S)) * (2 * np.pi / (J * S))
+    p = (pdf[:, 0].astype(np.float64) * cell).reshape(K, S, J, S).sum(axis=(1, 3)).reshape(-1)
+    e = n * p
+    ok = e >= 5
+    chi2 = np.sum((counts[ok] - e[ok]) ** 2 / e[ok])
+    return chi2, ok.sum() - 1, up.mean(), p.sum()
+
+
+@pytest.mark.parametrize("name", list(MATERIALS))
+@pytest.mark.parametrize("th", [20, 60])
+def test_fixed_mixture_pdf_is_the_sampler_density(name, th):
+    """With REINA_RT_TRACE_FIX_METAL_PDF | FIX_DISNEY_MIX, evalDisney's pdf
+    is the density of sampleDisney's directions over the upper hemisphere
+    (where the estimator's cosine is nonzero): binned chi-square, and the
+    sampled upper-hemisphere fraction equals the pdf's mass there."""
+    import reina_rt.abi as abi
+    chi2, dof, frac_up, mass_up = _mixture_chi2(name, abi.TRACE_FIX_METAL_PDF | abi.TRACE_FIX_DISNEY_MIX, th)
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+    assert abs(frac_up - mass_up) < 0.01, (frac_up, mass_up)
+
+
+def test_reference_mixture_is_not_the_sampler_density():
+    """The reference's weights (clearcoat outside the normalised sum) do not
+    give the sampler's density: the same test fails by far."""
+    chi2, dof, frac_up, mass_up = _mixture_chi2("clearcoat_plastic", 0x40, 20)
+    assert chi2 > 10 * dof, (chi2, dof)
