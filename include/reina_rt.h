@@ -320,6 +320,17 @@ typedef struct ReinaTraceStats {
 #define REINA_RT_TRACE_FIX_METAL_PDF    0x40u
 #define REINA_RT_TRACE_FIX_DISNEY_MIX   0x80u
 #define REINA_RT_TRACE_FIXES            0xF8u
+/* Independent sample streams (no counterpart in the reference, where one RNG
+ * stream per pixel runs all of a pixel's samples in sequence,
+ * shaders/raytrace/raytrace.rgen.glsl:259-277). Bits 8-11 hold log2(G),
+ * G in 1..16: a pixel's samples split into G groups of spp / G (spp must be
+ * a multiple of G) that are traced in parallel, group g seeded as the
+ * reference seeds sample batch sampleBatch * G + g:
+ * ((sampleBatch * G + g) * H + y) * W + x. The pixel's mean is the sum of
+ * the groups' sample sums (added in group order) over the sum of their
+ * counts. G = 1 is the reference. reina_rt_submit refuses G > 1. */
+#define REINA_RT_TRACE_SAMPLE_GROUPS(log2g) ((uint32_t)(log2g) << 8)
+#define REINA_RT_TRACE_SAMPLE_GROUPS_MASK   0xF00u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 

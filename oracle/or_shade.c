@@ -1085,24 +1085,35 @@ void or_camera_ray(const RtPushConsts* pc, uint32_t px, uint32_t py, uint32_t W,
 /* raytrace.rgen.glsl:259-276 — the spp loop of one dispatch for one pixel,
  * seeded with `batch`; returns the number of non-NaN samples and their mean
  * (sum / float(actual): NaN when every sample was dropped). */
+/* With REINA_RT_TRACE_SAMPLE_GROUPS(n) in the scene's options the samples split
+ * into G = 2^n groups of spp / G, group g on its own stream seeded for batch
+ * batch * G + g; the group sums are added in group order. G = 1: the
+ * reference's single stream per pixel. */
 static uint32_t pixel_mean(const OrScene* s, const RtPushConsts* pc, uint32_t batch, uint32_t x, uint32_t y,
                            uint32_t W, uint32_t H, int use_bvh, v3* mean, uint64_t* rays) {
-    Payload pld;
-    memset(&pld, 0, sizeof pld);
-    pld.rngState = (batch * H + y) * W + x;
+    const uint32_t G = 1u << ((s->fixes & REINA_RT_TRACE_SAMPLE_GROUPS_MASK) >> 8);
     uint32_t actual = 0;
     v3 sum = V3(0.0f, 0.0f, 0.0f);
-    for (uint32_t si = 0; si < pc->samplesPerPixel; ++si) {
-        v3 o, d, L;
-        getStartingRay(pc, (float)x, (float)y, (float)W, (float)H, &pld.rngState, &o, &d,
-                       (s->fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0);
-        if (s->fixes & REINA_RT_TRACE_FIX_MEDIUM_RESET) pld.accumulatedDistance = 0.0f;
-        *rays += trace_segments(s, &pld, o, d, pc->maxBounces, use_bvh, &L);
-        v3 c = V3(or_clampf(L.x, 0.0f, pc->directClamp), or_clampf(L.y, 0.0f, pc->directClamp),
-                  or_clampf(L.z, 0.0f, pc->directClamp));
-        if (visnan(c)) continue;
-        actual++;
-        sum = vadd(sum, c);
+    for (uint32_t g = 0; g < G; ++g) {
+        Payload pld;
+        memset(&pld, 0, sizeof pld);
+        pld.rngState = ((batch * G + g) * H + y) * W + x;
+        uint32_t gact = 0;
+        v3 gsum = V3(0.0f, 0.0f, 0.0f);
+        for (uint32_t si = 0; si < pc->samplesPerPixel / G; ++si) {
+            v3 o, d, L;
+            getStartingRay(pc, (float)x, (float)y, (float)W, (float)H, &pld.rngState, &o, &d,
+                           (s->fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0);
+            if (s->fixes & REINA_RT_TRACE_FIX_MEDIUM_RESET) pld.accumulatedDistance = 0.0f;
+            *rays += trace_segments(s, &pld, o, d, pc->maxBounces, use_bvh, &L);
+            v3 c = V3(or_clampf(L.x, 0.0f, pc->directClamp), or_clampf(L.y, 0.0f, pc->directClamp),
+                      or_clampf(L.z, 0.0f, pc->directClamp));
+            if (visnan(c)) continue;
+            gact++;
+            gsum = vadd(gsum, c);
+        }
+        sum = g == 0 ? gsum : vadd(sum, gsum);
+        actual += gact;
     }
     float fa = (float)actual;
     *mean = V3(sum.x / fa, sum.y / fa, sum.z / fa);

@@ -498,10 +498,18 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     if (W == 0 || H == 0) return fail(REINA_RT_ERR_INVALID_ARG, "zero image size");
     if ((uint64_t)W * H >= (1ull << 31)) return fail(REINA_RT_ERR_INVALID_ARG, "image too large");
     hipStream_t st = (hipStream_t)stream;
+    const uint32_t groups = 1u << ((s->options & REINA_RT_TRACE_SAMPLE_GROUPS_MASK) >> 8);
+    if (pc->samplesPerPixel % groups != 0u)
+        return fail(REINA_RT_ERR_INVALID_ARG, "samplesPerPixel " + std::to_string(pc->samplesPerPixel) +
+                                                  " is not a multiple of the " + std::to_string(groups) +
+                                                  " sample groups");
+    if ((uint64_t)W * H * groups >= (1ull << 31)) return fail(REINA_RT_ERR_INVALID_ARG, "image x sample groups too large");
     HIP_TRY(hipSetDevice(s->device));
-    int rc = ensure_trace_buffers(s, (uint64_t)W * H);
+    int rc = ensure_trace_buffers(s, (uint64_t)W * H * groups);
     if (rc) return rc;
     TraceParams P = make_trace_params(pc, W, H);
+    P.groups = groups;
+    P.spp = pc->samplesPerPixel / groups;
     const bool nee = (s->options & REINA_RT_TRACE_NEE) != 0;
     if (nee && s->emi.empty())
         return fail(REINA_RT_ERR_NO_EMISSIVE, "next-event estimation needs at least one emissive instance");
@@ -509,7 +517,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
     P.fixes = s->options & REINA_RT_TRACE_FIXES;
     const bool timex = (s->options & REINA_RT_TRACE_TIME_EXTEND) != 0;
     const bool count = (s->options & REINA_RT_TRACE_COUNT) != 0;
-    const uint32_t npx = W * H;
+    const uint32_t npx = W * H * groups;   // path slots: one per pixel and sample group
     s->Q.cap = part_capacity(npx);
     HIP_TRY(hipEventRecord(s->ev0, st));
     HIP_TRY(launch_init(P, s->S, s->Q, s->ctrl, s->stats, 0, npx, st));
@@ -639,7 +647,8 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
 
 int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
     if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
-    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE | REINA_RT_TRACE_FIXES))
+    if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE | REINA_RT_TRACE_FIXES |
+                    REINA_RT_TRACE_SAMPLE_GROUPS_MASK))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown trace option bits");
     s->options = options;
     return REINA_RT_OK;

@@ -45,6 +45,7 @@ inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_
     P.directClamp = pc->directClamp;
     P.nee = 0;
     P.fixes = 0;
+    P.groups = 1;
     return P;
 }
 

@@ -47,9 +47,11 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         if (threadIdx.x == 0 && base == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; stats[4] = 0; }
     }
     if (l >= npx) return;
-    const uint32_t i = base + l;
-    uint32_t x = i % P.W, y = i / P.W;
-    uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
+    const uint32_t i = base + l;   // slot: group g = i / (W * H) of pixel i % (W * H)
+    const uint32_t npix = P.W * P.H, g = i / npix, pix = i - g * npix;
+    uint32_t x = pix % P.W, y = pix / P.W;
+    // raytrace.rgen.glsl:259; group g of G takes the seed of batch sampleBatch * G + g
+    uint32_t rng = ((P.sampleBatch * P.groups + g) * P.H + y) * P.W + x;
     if (!anyrays) {
         // no traced segment: every sample contributes clamp(0) = 0
         S.st0[i] = make_uint4(rng, P.spp, 0u, 0u);
@@ -143,7 +145,13 @@ __global__ __launch_bounds__(256) void k_finish(TraceParams P, PathState S, floa
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npx) return;
     float4 s3 = S.st3[i];
-    float fa = (float)__float_as_uint(s3.w);
+    uint32_t cnt = __float_as_uint(s3.w);
+    for (uint32_t g = 1; g < P.groups; ++g) {   // sample groups: sums in group order
+        const float4 t = S.st3[i + g * npx];
+        s3 = make_float4(s3.x + t.x, s3.y + t.y, s3.z + t.z, 0.0f);
+        cnt += __float_as_uint(t.w);
+    }
+    float fa = (float)cnt;
     V3 fin = v3(s3.x / fa, s3.y / fa, s3.z / fa);
     if (P.sampleBatch > 0u) {                         // raytrace.rgen.glsl:277-282
         float4 prev = img[i];

@@ -373,6 +373,8 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     if (scene->device != s->device) return fail(REINA_RT_ERR_INVALID_ARG, "scene lives on another device");
     if (scene->quantized != s->quantized)
         return fail(REINA_RT_ERR_UNSUPPORTED, "scene node format differs from the server's");
+    if (scene->options & REINA_RT_TRACE_SAMPLE_GROUPS_MASK)
+        return fail(REINA_RT_ERR_UNSUPPORTED, "sample groups are not served; trace the scene with reina_rt_trace");
     const bool nee = (scene->options & REINA_RT_TRACE_NEE) != 0;
     if (nee && !s->nee)
         return fail(REINA_RT_ERR_UNSUPPORTED, "the scene asks for next-event estimation; the server was created "

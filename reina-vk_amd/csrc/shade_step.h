@@ -206,7 +206,7 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         if (!anynan(c)) { actual += 1u; sum = add(sum, c); }
         sample += 1u;
         if (sample < P.spp) {
-            uint32_t x = px % P.W, y = px / P.W;
+            uint32_t x = px % P.W, y = (px / P.W) % P.H;   // px: slot of a sample group
             starting_ray(P.cam, (float)x, (float)y, pld.rng, r.no, r.nd, (P.fixes & REINA_RT_TRACE_FIX_LENS_RNG) != 0u);
             thr = v3(1.0f, 1.0f, 1.0f);
             L = v3(0.0f, 0.0f, 0.0f);

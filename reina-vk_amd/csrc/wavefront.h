@@ -17,6 +17,7 @@ struct TraceParams {
     float directClamp;
     uint32_t nee;   // next-event estimation on (not a reference push constant)
     uint32_t fixes; // REINA_RT_TRACE_FIX_* corrections of reference behaviours (0: as the reference)
+    uint32_t groups;  // sample groups per pixel (1: as the reference); spp above is per group
 };
 
 // The ray queues are split into kParts partitions, one per XCD: workgroup b

@@ -177,6 +177,12 @@ TRACE_FIX_NORMALS = 0x20
 TRACE_FIX_METAL_PDF = 0x40
 TRACE_FIX_DISNEY_MIX = 0x80
 TRACE_FIXES = 0xF8
+TRACE_SAMPLE_GROUPS_MASK = 0xF00
+
+
+def TRACE_SAMPLE_GROUPS(log2g):
+    """Trace-option bits for 2**log2g independent sample streams per pixel."""
+    return (int(log2g) & 0xF) << 8
 
 
 # status codes

@@ -503,6 +503,51 @@ def test_render_fixes_bitexact(cuda, fixes):
         dev.close()
 
 
+@pytest.mark.parametrize("log2g", [1, 3])
+@pytest.mark.parametrize("nee,tail", [(False, 0), (False, 1 << 30), (True, 0)], ids=["plain", "tail_kernel", "nee"])
+def test_render_sample_groups_bitexact(cuda, cornell_arrays, log2g, nee, tail):
+    """REINA_RT_TRACE_SAMPLE_GROUPS: a pixel's samples in 2^n parallel
+    streams, progressive (sampleBatch 1 over a given image); the frame equals
+    the oracle's grouped restatement bit for bit."""
+    torch = cuda
+    arrays, cam = cornell_arrays
+    W, H = 40, 30
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 16, 8, sample_batch=1)
+    init = np.random.default_rng(3).uniform(0, 1, (H, W, 4)).astype(np.float32)
+    opts = abi.TRACE_SAMPLE_GROUPS(log2g) | (abi.TRACE_NEE if nee else 0)
+    dev = reina_rt.DeviceScene(arrays)
+    dev.set_tuning(tail_paths=tail)
+    dev.set_trace_options(opts)
+    got = gpu_render(torch, dev, pc, W, H, init)
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_fixes(opts)
+    orc.set_nee(nee)
+    ref, nrays = orc.render(pc, W, H, img=init.copy(), use_bvh=False)
+    same = (_bits(got) == _bits(ref)) | (np.isnan(got) & np.isnan(ref))
+    assert same.all(), "pixels differing: %d" % (~same).any(-1).sum()
+    assert dev.stats().extension_rays == nrays
+    dev.close()
+
+
+def test_sample_groups_errors(cuda, cornell_arrays):
+    torch = cuda
+    arrays, cam = cornell_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    dev.set_trace_options(abi.TRACE_SAMPLE_GROUPS(2))
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], 8, 8, 6, 4)        # 6 spp, 4 groups
+    with pytest.raises(reina_rt.ReinaError) as e:
+        gpu_render(torch, dev, pc, 8, 8)
+    assert e.value.code == abi.ERR_INVALID_ARG
+    srv = reina_rt.Server(path_budget_bytes=64 << 20)
+    img = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
+    pc8 = push_consts(cam["eye"], cam["center"], cam["fov"], 8, 8, 8, 4)
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srv.submit(dev, pc8, img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_UNSUPPORTED
+    srv.close()
+    dev.close()
+
+
 def test_seed_wraps_mod_2_32(cuda, cornell_arrays):
     """The pixel seed (sampleBatch * H + y) * W + x is uint32 arithmetic
     (raytrace.rgen.glsl:259), so at 64 x 64 (W * H = 2^12) batch 2^20 reuses

@@ -176,3 +176,40 @@ def test_fix_metal_pdf():
     fixed, _ = _render_fixed(s, cam, 16, 12, 2, 2, abi.TRACE_FIX_METAL_PDF)
     assert (fixed[..., :3] >= ref[..., :3]).all()
     assert (fixed[..., :3] > ref[..., :3] * 1.01).any()
+
+
+# ---- independent sample streams ----------------------------------------------------
+@pytest.mark.parametrize("log2g,batch", [(0, 0), (2, 0), (3, 5)])
+def test_sample_groups_seed_each_group_as_a_batch(log2g, batch):
+    """With G = 2^n sample groups and spp = G (one sample per group), a pixel
+    of a scene every ray misses is the mean of the sky over the G camera rays
+    drawn from the reference seeds of batches batch * G + g."""
+    G = 1 << log2g
+    s = Scene()
+    s.add_object(scenes._model_from_quads([scenes._quad((0, 0, 50.0), (0.1, 0, 0), (0, 0.1, 0))]), np.eye(4),
+                 Material(materialIdx=0))                       # behind the camera: never hit
+    W, H = 5, 4
+    pc = push_consts((0.0, 0.0, 0.0), (0.0, 0.3, -1.0), 50.0, W, H, G, 3, sample_batch=0)
+    orc = oracle.OracleScene(s.build_arrays(), make_desc)
+    orc.set_fixes(abi.TRACE_SAMPLE_GROUPS(log2g))
+    pc["sampleBatch"] = batch
+    img, _ = orc.render(pc, W, H, use_bvh=False)                 # blends into a zero image when batch > 0
+    for y in range(H):
+        for x in range(W):
+            sky = np.mean([_sky64(oracle.camera_ray(pc, x, y, W, H, ((batch * G + g) * H + y) * W + x)[1])
+                           for g in range(G)], axis=0)
+            expect = sky / (batch + 1) if batch else sky          # running mean over a zero image
+            assert np.allclose(img[y, x, :3], expect, rtol=2e-6), (x, y)
+
+
+def test_sample_groups_one_group_is_the_reference(cornell_arrays):
+    arrays, cam = cornell_arrays
+    W, H = 12, 9
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 8, 6)
+    a = oracle.OracleScene(arrays, make_desc).render(pc, W, H, use_bvh=False)[0]
+    o = oracle.OracleScene(arrays, make_desc)
+    o.set_fixes(abi.TRACE_SAMPLE_GROUPS(0))
+    assert np.array_equal(o.render(pc, W, H, use_bvh=False)[0].view(np.uint32), a.view(np.uint32))
+    o.set_fixes(abi.TRACE_SAMPLE_GROUPS(3))
+    b, nrays = o.render(pc, W, H, use_bvh=False)
+    assert not np.array_equal(b.view(np.uint32), a.view(np.uint32)) and np.isfinite(b).all()
