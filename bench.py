@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
     ap.add_argument("--no-sample-groups", action="store_true",
-                    help="skip the frame with 8 independent sample streams per pixel")
+                    help="skip the frame with 16 independent sample streams per pixel")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the path tracer has no multi-GPU path here); "
@@ -184,20 +184,20 @@ def main():
         log("quantized nodes")
         out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
 
-    if not a.no_sample_groups and a.spp % 8 == 0:
-        # the same frame with a pixel's samples in 8 independent streams: the
-        # per-pixel sample chains (the frame's critical path) are 8x shorter
+    if not a.no_sample_groups and a.spp % 16 == 0:
+        # the same frame with a pixel's samples in 16 independent streams: the
+        # per-pixel sample chains (the frame's critical path) are 16x shorter
         log("sample groups")
-        dev.set_trace_options(abi.TRACE_SAMPLE_GROUPS(3))
+        dev.set_trace_options(abi.TRACE_SAMPLE_GROUPS(4))
         r, el, gms = time_frames(frame, 1, a.steps, torch)
         st = dev.stats()
         dev.set_trace_options(0)
         out["sample_groups"] = {
-            "groups": 8, "spp_per_group": a.spp // 8,
+            "groups": 16, "spp_per_group": a.spp // 16,
             "mrays_per_s": round(r / el / 1e6, 2), "ms_per_step": round(el / max(a.steps, 1) * 1e3, 3),
             "iterations": int(st.iterations), "tail_rays": int(st.tail_rays),
-            "note": "REINA_RT_TRACE_SAMPLE_GROUPS(3): 8 streams per pixel seeded as sample batches "
-                    "sampleBatch*8+g; not the reference's sampling (one stream per pixel), so not the headline",
+            "note": "REINA_RT_TRACE_SAMPLE_GROUPS(4): 16 streams per pixel seeded as sample batches "
+                    "sampleBatch*16+g; not the reference's sampling (one stream per pixel), so not the headline",
         }
 
     if not a.no_device_build:
