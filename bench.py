@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
+    ap.add_argument("--no-image-quality", action="store_true",
+                    help="skip the RMSE of the Cornell frame against its 65,536-spp converged render")
     ap.add_argument("--no-sample-groups", action="store_true",
                     help="skip the frame with 16 independent sample streams per pixel")
     a = ap.parse_args()
@@ -183,6 +185,10 @@ def main():
     if not a.no_quantized:
         log("quantized nodes")
         out["quantized_nodes"] = quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch)
+
+    if not a.no_image_quality:
+        log("image quality")
+        out["image_quality"] = image_quality(torch)
 
     if not a.no_sample_groups and a.spp % 16 == 0:
         # the same frame with a pixel's samples in 16 independent streams: the
@@ -287,6 +293,39 @@ def run_autotune(dev, sec, sh, torch, frame=None):
         out["frame_worst_ms"] = max(r["ms"] for r in ftable)
         out["frame_table"] = ftable
     return out
+
+
+def image_quality(torch):
+    """RMSE of the 24x24 Cornell frame (depth 8) against its 65,536-spp FP64
+    converged render (tests/golden/cornell_24x24_converged.npz), at 64 spp (one
+    frame) and 1024 spp (16 progressive frames), for the reference's sampling,
+    next-event estimation, and 16 independent sample streams per pixel.
+    `predicted_rmse_64spp` is sqrt(mean(per-pixel variance of a 64-spp mean)):
+    an unbiased 64-spp estimator with the reference's variance lands on it."""
+    import numpy as np
+    import reina_rt
+    from reina_rt import abi, scenes
+    g = np.load(os.path.join(ROOT, "tests", "golden", "cornell_24x24_converged.npz"))
+    ref, var = g["mean"], g["var_batch64"]
+    sc, _ = scenes.cornell()
+    dev = reina_rt.DeviceScene(sc.build_arrays())
+    W = H = 24
+    img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    sp = torch.cuda.current_stream().cuda_stream
+    res = {"scene": "Cornell box 24x24, depth 8, directClamp 100", "converged_spp": int(g["spp"]),
+           "predicted_rmse_64spp": round(float(np.sqrt(var.mean())), 5)}
+    for name, opts in (("reference_sampling", 0), ("nee", abi.TRACE_NEE), ("sample_groups_16", abi.TRACE_SAMPLE_GROUPS(4))):
+        dev.set_trace_options(opts)
+        row = {}
+        for b in range(16):
+            dev.trace(scenes.cornell_push(W, H, spp=64, max_bounces=8, sample_batch=b), img.data_ptr(), W, H, stream=sp)
+            if b in (0, 15):
+                torch.cuda.synchronize()
+                got = img.cpu().numpy()[..., :3].astype(np.float64)
+                row["rmse_%dspp" % (64 * (b + 1))] = round(float(np.sqrt(np.mean((got - ref) ** 2))), 5)
+        res[name] = row
+    dev.close()
+    return res
 
 
 def time_frames(frame, warmup, steps, torch):
