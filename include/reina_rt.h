@@ -328,7 +328,7 @@ typedef struct ReinaTraceStats {
  * reference seeds sample batch sampleBatch * G + g:
  * ((sampleBatch * G + g) * H + y) * W + x. The pixel's mean is the sum of
  * the groups' sample sums (added in group order) over the sum of their
- * counts. G = 1 is the reference. reina_rt_submit refuses G > 1. */
+ * counts. G = 1 is the reference. Server requests take G from their scene. */
 #define REINA_RT_TRACE_SAMPLE_GROUPS(log2g) ((uint32_t)(log2g) << 8)
 #define REINA_RT_TRACE_SAMPLE_GROUPS_MASK   0xF00u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);

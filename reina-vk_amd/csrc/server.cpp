@@ -373,8 +373,12 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     if (scene->device != s->device) return fail(REINA_RT_ERR_INVALID_ARG, "scene lives on another device");
     if (scene->quantized != s->quantized)
         return fail(REINA_RT_ERR_UNSUPPORTED, "scene node format differs from the server's");
-    if (scene->options & REINA_RT_TRACE_SAMPLE_GROUPS_MASK)
-        return fail(REINA_RT_ERR_UNSUPPORTED, "sample groups are not served; trace the scene with reina_rt_trace");
+    const uint32_t groups = 1u << ((scene->options & REINA_RT_TRACE_SAMPLE_GROUPS_MASK) >> 8);
+    if (pc->samplesPerPixel % groups != 0u)
+        return fail(REINA_RT_ERR_INVALID_ARG, "samplesPerPixel " + std::to_string(pc->samplesPerPixel) +
+                                                  " is not a multiple of the " + std::to_string(groups) +
+                                                  " sample groups");
+    if ((uint64_t)W * H * groups >= (1ull << 31)) return fail(REINA_RT_ERR_INVALID_ARG, "image x sample groups too large");
     const bool nee = (scene->options & REINA_RT_TRACE_NEE) != 0;
     if (nee && !s->nee)
         return fail(REINA_RT_ERR_UNSUPPORTED, "the scene asks for next-event estimation; the server was created "
@@ -385,7 +389,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
         return fail(REINA_RT_ERR_UNSUPPORTED, "scene BVH depth " + std::to_string(scene->info.max_depth) +
                                                   " exceeds the server's stack depth " +
                                                   std::to_string(s->tune.stack_depth));
-    const uint32_t npx = W * H;
+    const uint32_t npx = W * H * groups;   // path slots: one per pixel and sample group
     const uint32_t ntiles = (npx + 1023u) >> 10;
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->failed) return fail(REINA_RT_ERR_DEVICE, "server stopped after a device error: " + s->error);
@@ -406,6 +410,8 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     r.P = make_trace_params(pc, W, H);
     r.P.nee = nee ? 1u : 0u;
     r.P.fixes = scene->options & REINA_RT_TRACE_FIXES;
+    r.P.groups = groups;
+    r.P.spp = pc->samplesPerPixel / groups;
     r.img = (float4*)image;
     r.npx = npx;
     r.ntiles = ntiles;
@@ -419,7 +425,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
 
 static void fill_status(const Request& r, ReinaRequestStatus* o) {
     o->state = r.state;
-    o->npx = r.npx;
+    o->npx = r.P.W * r.P.H;
     o->submit_s = r.submit_s;
     o->admit_s = r.admit_s;
     o->done_s = r.done_s;

@@ -37,8 +37,11 @@ __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j
     V3 o = v3(0.0f, 0.0f, 0.0f), d = o;
     if (valid) {
         if (P.nee) S.st4[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t x = k % P.W, y = k / P.W;
-        uint32_t rng = (P.sampleBatch * P.H + y) * P.W + x;   // raytrace.rgen.glsl:259
+        // slot k: sample group g = k / (W * H) of pixel k % (W * H)
+        const uint32_t npix = P.W * P.H, g = k / npix, pix = k - g * npix;
+        uint32_t x = pix % P.W, y = pix / P.W;
+        // raytrace.rgen.glsl:259; group g of G takes the seed of batch sampleBatch * G + g
+        uint32_t rng = ((P.sampleBatch * P.groups + g) * P.H + y) * P.W + x;
         if (!anyrays) {
             S.st0[slot] = make_uint4(rng, P.spp, 0u, 0u);
             S.st3[slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(P.spp));
@@ -161,9 +164,16 @@ __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathSta
 __global__ __launch_bounds__(256) void k_finish_job(const JobRec* jobs, uint32_t j, PathState S) {
     const JobRec& J = jobs[j];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= J.npx) return;
-    const float4 s3 = S.st3[J.px_base + k];
-    const float fa = (float)__float_as_uint(s3.w);
+    const uint32_t npix = J.P.W * J.P.H;   // J.npx = npix * groups slots
+    if (k >= npix) return;
+    float4 s3 = S.st3[J.px_base + k];
+    uint32_t cnt = __float_as_uint(s3.w);
+    for (uint32_t g = 1; g < J.P.groups; ++g) {   // sample groups: sums in group order
+        const float4 t = S.st3[J.px_base + k + g * npix];
+        s3 = make_float4(s3.x + t.x, s3.y + t.y, s3.z + t.z, 0.0f);
+        cnt += __float_as_uint(t.w);
+    }
+    const float fa = (float)cnt;
     V3 fin = v3(s3.x / fa, s3.y / fa, s3.z / fa);
     if (J.P.sampleBatch > 0u) {                        // raytrace.rgen.glsl:277-282
         const float4 prev = J.img[k];

@@ -540,10 +540,9 @@ def test_sample_groups_errors(cuda, cornell_arrays):
     assert e.value.code == abi.ERR_INVALID_ARG
     srv = reina_rt.Server(path_budget_bytes=64 << 20)
     img = torch.zeros((8, 8, 4), dtype=torch.float32, device="cuda")
-    pc8 = push_consts(cam["eye"], cam["center"], cam["fov"], 8, 8, 8, 4)
     with pytest.raises(reina_rt.ReinaError) as e:
-        srv.submit(dev, pc8, img.data_ptr(), 8, 8)
-    assert e.value.code == abi.ERR_UNSUPPORTED
+        srv.submit(dev, pc, img.data_ptr(), 8, 8)
+    assert e.value.code == abi.ERR_INVALID_ARG
     srv.close()
     dev.close()
 

@@ -248,6 +248,38 @@ def test_server_fixes_mix(cuda, pool, tail):
     i_fix.close()
 
 
+@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF], ids=["tail_default", "tail_off"])
+def test_server_sample_groups_mix(cuda, pool, tail):
+    """Requests on scenes traced with REINA_RT_TRACE_SAMPLE_GROUPS (4 and 16
+    streams per pixel), mixed with the same scenes without: every image
+    equals reina_rt_trace's on the same scene and options."""
+    torch = cuda
+    rng = np.random.default_rng(21)
+    c_arrays, c_cam, c_dev = pool["cornell"]
+    i_arrays, i_cam, i_dev = pool["ico"]
+    c_grp = reina_rt.DeviceScene(c_arrays)
+    c_grp.set_trace_options(abi.TRACE_SAMPLE_GROUPS(2))
+    i_grp = reina_rt.DeviceScene(i_arrays)
+    i_grp.set_trace_options(abi.TRACE_SAMPLE_GROUPS(4))
+    jobs = []
+    for dev, W, H, spp, batch, cam in ((c_grp, 48, 40, 8, 0, c_cam), (c_dev, 48, 40, 8, 0, c_cam),
+                                       (i_grp, 64, 36, 16, 1, i_cam), (i_dev, 64, 36, 16, 1, i_cam),
+                                       (c_grp, 21, 13, 4, 2, c_cam)):
+        pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, spp, 8, sample_batch=batch)
+        init = rng.uniform(0, 1, (H, W, 4)).astype(np.float32) if batch else None
+        jobs.append((dev, pc, W, H, init))
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, tail_paths=tail)
+    mixed = served(torch, srv, jobs)
+    for (dev, pc, W, H, init), got in zip(jobs, mixed):
+        assert _same(got, plain(torch, dev, pc, W, H, init))
+    assert not _same(mixed[0], mixed[1]) and not _same(mixed[2], mixed[3])
+    st = srv.stats()
+    assert st.completed == len(jobs) and st.slot_tiles_used == 0
+    srv.close()
+    c_grp.close()
+    i_grp.close()
+
+
 def test_server_nee_refusals(cuda, pool):
     """A NEE request on a server without REINA_RT_SERVER_NEE is refused
     (REINA_RT_ERR_UNSUPPORTED), and one on a scene without emissive
