@@ -73,6 +73,8 @@ def main():
                          "waves_per_simd,tail_paths (the config.tuning of a default run)")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
     ap.add_argument("--no-device-build", action="store_true", help="skip the device-built BVH comparison")
+    ap.add_argument("--instanced-variants", type=str, default="fp32,quantized,fp32_device_build",
+                    help="node formats / builds of the --instanced block, comma-separated")
     ap.add_argument("--instanced", action="store_true",
                     help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
     ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
@@ -529,7 +531,10 @@ def instanced_block(a, torch):
     sp = torch.cuda.current_stream().cuda_stream
     res = {"workload": "4096 instances x 2048-tri sphere = 8,388,608 tris, %dx%d, %d spp, depth %d" % (W, H, spp, depth)}
     devs = {}
+    wanted = set(a.instanced_variants.split(",")) | {"fp32"}   # the ray set and FP64 check use fp32
     for name, flags in (("fp32", 0), ("quantized", abi.NODES_QUANTIZED), ("fp32_device_build", abi.BUILD_DEVICE)):
+        if name not in wanted:
+            continue
         t0 = time.perf_counter()
         d = reina_rt.DeviceScene(arrays, flags=flags)
         inf = d.info()
@@ -597,8 +602,9 @@ def instanced_block(a, torch):
         res[name]["fp64_check"] = chk
     del wt
     res["fp64_oracle_s"] = round(oracle_s, 1)
-    res["hits_bit_identical_fp32_vs_quantized"] = bool(
-        np.array_equal(hits["fp32"].view(np.uint32), hits["quantized"].view(np.uint32)))
+    if "fp32" in hits and "quantized" in hits:
+        res["hits_bit_identical_fp32_vs_quantized"] = bool(
+            np.array_equal(hits["fp32"].view(np.uint32), hits["quantized"].view(np.uint32)))
     # frames
     for name, d in devs.items():
         if not a.no_autotune:
@@ -615,10 +621,15 @@ def instanced_block(a, torch):
         res[name]["rays_per_frame"] = int(rays_f // 2)
         if not a.no_roofline:
             res[name]["roofline"] = roofline(d, frame, 80 if name == "quantized" else 208, torch)
-    frame(0, devs["fp32"])
-    ref = img.clone()
-    frame(0, devs["quantized"])
-    res["frame_bit_identical_fp32_vs_quantized"] = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
+            if name == "fp32":
+                tr, tt = profiled_traffic("k_trace_instanced_8m_1080p_32spp.md", d.get_tuning())
+                res[name]["roofline"]["traffic"] = tr
+                res[name]["roofline"]["traffic_tuning"] = tt
+    if "fp32" in devs and "quantized" in devs:
+        frame(0, devs["fp32"])
+        ref = img.clone()
+        frame(0, devs["quantized"])
+        res["frame_bit_identical_fp32_vs_quantized"] = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
     for d in devs.values():
         d.close()
     return res
