@@ -79,6 +79,8 @@ def main():
                     help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
     ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched block")
+    ap.add_argument("--batched-groups", type=int, default=2,
+                    help="log2 of the sample streams per pixel of the batched block's second pass (0: none)")
     ap.add_argument("--no-served", action="store_true", help="skip the served block")
     ap.add_argument("--batched-full", action="store_true",
                     help="batched block at full size: 64 scenes of 1e3-2e6 triangles, 64 requests")
@@ -475,6 +477,18 @@ def batched_block(a, torch):
                 "triangles_total": int(sum(p["tris"] for p in pool)), "scene_setup_s": round(setup, 1),
                 "note": "plain = reina_rt_trace per request, one after another; batched = all requests "
                         "submitted to one server (shared queues, graph-replayed bounce iterations)"})
+    if a.batched_groups > 0:
+        # the same requests with 2^n independent sample streams per pixel, plain and batched alike
+        from reina_rt import abi
+        g = 1 << a.batched_groups
+        log("batched: %d sample streams per pixel" % g)
+        for p in pool:
+            p["dev"].set_trace_options(abi.TRACE_SAMPLE_GROUPS(a.batched_groups))
+        rg = loadgen.batched(torch, pool, reqs, dict(path_budget_bytes=SERVER_BUDGET))
+        res["sample_groups_%d" % g] = {k: rg[k] for k in ("plain", "batched", "speedup_images_per_s",
+                                                          "images_bit_identical") if k in rg}
+        for p in pool:
+            p["dev"].set_trace_options(0)
     for p in pool:
         p["dev"].close()
     return res
