@@ -19,7 +19,10 @@
 //     bits, >> 20 and a clamp; components named 'R', 'G', 'B' (or an Adobe
 //     marker with transform 0 in a file without JFIF marker) are taken as RGB;
 //   * grey as (y, y, y, 255).
-// Progressive, lossless, arithmetic-coded and 12-bit JPEGs, and 4-component
+//   * progressive files (ITU-T T.81 Annex G): every scan's coefficients
+//     accumulate per block, then the blocks are dequantised and transformed
+//     as above.
+// Lossless, arithmetic-coded, differential and 12-bit JPEGs, and 4-component
 // (CMYK) files, return REINA_RT_ERR_UNSUPPORTED.
 #include <algorithm>
 #include <cstdint>
@@ -68,6 +71,7 @@ struct Component {
     int w2 = 0, h2 = 0;           // padded to whole MCUs
     int dc_pred = 0;
     std::vector<uint8_t> data;    // w2 * h2 samples
+    std::vector<short> coef;      // progressive: (w2 / 8) * (h2 / 8) blocks of 64, natural order
 };
 
 struct Decoder {
@@ -82,6 +86,8 @@ struct Decoder {
     int adobe_transform = -1;
     bool jfif = false;
     bool frame = false;
+    bool progressive = false;
+    int eobrun = 0;   // progressive AC scans: blocks left in the current end-of-band run
     // entropy-coded bit reader
     uint32_t bits = 0;
     int nbits = 0;
@@ -134,8 +140,9 @@ struct Decoder {
 
     void start_of_frame(int marker, uint64_t end) {
         if (frame) bad("more than one frame");
-        if (marker == 0xC2 || marker == 0xC6 || marker == 0xCA || marker == 0xCE) unsupported("progressive JPEG");
-        if (marker != 0xC0 && marker != 0xC1) unsupported("lossless or arithmetic-coded JPEG");
+        if (marker == 0xC6 || marker == 0xCA || marker == 0xCE) unsupported("differential JPEG");
+        if (marker != 0xC0 && marker != 0xC1 && marker != 0xC2) unsupported("lossless or arithmetic-coded JPEG");
+        progressive = marker == 0xC2;
         if (u8() != 8) unsupported("only 8-bit samples");
         height = u16();
         width = u16();
@@ -169,6 +176,7 @@ struct Decoder {
             c.w2 = mcux * c.h * 8;
             c.h2 = mcuy * c.v * 8;
             c.data.assign((size_t)c.w2 * c.h2, 0);
+            if (progressive) c.coef.assign((size_t)c.w2 * c.h2, 0);   // 64 per 8x8 block
         }
         frame = true;
     }
@@ -256,6 +264,80 @@ struct Decoder {
         }
     }
 
+    // ---- progressive scans (ITU-T T.81 Annex G.1.2): coefficients accumulate in
+    // c.coef over the scans; dequantisation and the IDCT run after the last ----
+    void prog_dc(Component& c, short* coef, int ah, int al) {
+        if (ah == 0) {   // first scan: the DC difference, scaled by 2^Al
+            const int t = decode(hdc[c.td]);
+            if (t > 15) bad("bad DC magnitude");
+            c.dc_pred += t ? extend(receive(t), t) : 0;
+            coef[0] = (short)(c.dc_pred * (1 << al));
+        } else if (bit()) {   // refinement: one bit
+            coef[0] = (short)(coef[0] | (1 << al));
+        }
+    }
+    void prog_ac_first(Component& c, short* coef, int ss, int se, int al) {
+        if (eobrun > 0) { --eobrun; return; }
+        for (int k = ss; k <= se;) {
+            const int rs = decode(hac[c.ta]);
+            const int s = rs & 15, r = rs >> 4;
+            if (s == 0) {
+                if (r < 15) {   // end of band: this block and 2^r - 1 + (r bits) more
+                    eobrun = (1 << r) - 1 + (r ? receive(r) : 0);
+                    return;
+                }
+                k += 16;
+            } else {
+                k += r;
+                if (k > se) bad("AC run past the band");
+                coef[kDezigzag[k++]] = (short)(extend(receive(s), s) * (1 << al));
+            }
+        }
+    }
+    void refine(short& v, int p1) {   // correction bit of an already nonzero coefficient
+        if (bit() && (v & p1) == 0) v = (short)(v >= 0 ? v + p1 : v - p1);
+    }
+    void prog_ac_refine(Component& c, short* coef, int ss, int se, int al) {
+        const int p1 = 1 << al;
+        int k = ss;
+        if (eobrun == 0) {
+            for (; k <= se; ++k) {
+                const int rs = decode(hac[c.ta]);
+                int s = rs & 15, r = rs >> 4;
+                int value = 0;
+                if (s != 0) {
+                    if (s != 1) bad("bad refinement magnitude");
+                    value = bit() ? p1 : -p1;
+                } else if (r != 15) {   // end of band
+                    eobrun = (1 << r) + (r ? receive(r) : 0);
+                    break;
+                }
+                // pass over r zero coefficients (refining the nonzero ones on the way),
+                // then place the new one on the next zero coefficient
+                for (; k <= se; ++k) {
+                    short& v = coef[kDezigzag[k]];
+                    if (v != 0) {
+                        refine(v, p1);
+                    } else {
+                        if (r == 0) break;
+                        --r;
+                    }
+                }
+                if (value != 0) {
+                    if (k > se) bad("AC refinement past the band");
+                    coef[kDezigzag[k]] = (short)value;
+                }
+            }
+        }
+        if (eobrun > 0) {   // inside an end-of-band run: refine the nonzero coefficients left
+            for (; k <= se; ++k) {
+                short& v = coef[kDezigzag[k]];
+                if (v != 0) refine(v, p1);
+            }
+            --eobrun;
+        }
+    }
+
     void scan(uint64_t end) {
         if (!frame) bad("scan before the frame header");
         const int ns = u8();
@@ -273,15 +355,28 @@ struct Decoder {
             sc.push_back(found);
         }
         const int ss = u8(), se = u8(), a = u8();
-        if (ss != 0 || se != 63 || a != 0) unsupported("spectral selection / successive approximation");
+        const int ah = a >> 4, al = a & 15;
+        if (!progressive && (ss != 0 || se != 63 || a != 0)) bad("baseline scan with spectral selection");
+        if (progressive) {
+            if (ss == 0 ? se != 0 : (se < ss || se > 63 || ns != 1)) bad("bad progressive scan bands");
+            if (al > 13 || ah > 13) bad("bad successive approximation");
+        }
         if (pos != end) bad("bad scan header length");
         for (Component* c : sc) {
-            if (!dq_defined[c->tq]) bad("undefined quantization table");
+            if (!progressive && !dq_defined[c->tq]) bad("undefined quantization table");
             c->dc_pred = 0;
         }
+        eobrun = 0;
         reset_bits();
         short blk[64];
         const auto put = [&](Component& c, int bx, int by) {
+            if (progressive) {
+                short* coef = c.coef.data() + ((size_t)by * (c.w2 / 8) + bx) * 64;
+                if (ss == 0) prog_dc(c, coef, ah, al);
+                else if (ah == 0) prog_ac_first(c, coef, ss, se, al);
+                else prog_ac_refine(c, coef, ss, se, al);
+                return;
+            }
             decode_block(c, blk);
             idct(c.data.data() + (size_t)by * 8 * c.w2 + (size_t)bx * 8, c.w2, blk);
         };
@@ -294,6 +389,7 @@ struct Decoder {
             if (pos + 1 >= n) bad("missing restart marker");
             pos += 2;
             for (Component* c : sc) c->dc_pred = 0;
+            eobrun = 0;
             todo = restart;
         };
         if (ns == 1) {   // non-interleaved: the component's own block grid
@@ -317,6 +413,22 @@ struct Decoder {
         while (pos + 1 < n && !(p[pos] == 0xFF && p[pos + 1] != 0x00 && !(p[pos + 1] >= 0xD0 && p[pos + 1] <= 0xD7)))
             ++pos;
         reset_bits();
+    }
+
+    // dequantise and transform every block once all scans are in
+    void finish_progressive() {
+        short blk[64];
+        for (Component& c : comp) {
+            if (!dq_defined[c.tq]) bad("undefined quantization table");
+            const uint16_t* q = dq[c.tq];
+            const int bw = c.w2 / 8, bh = c.h2 / 8;
+            for (int by = 0; by < bh; ++by)
+                for (int bx = 0; bx < bw; ++bx) {
+                    const short* coef = c.coef.data() + ((size_t)by * bw + bx) * 64;
+                    for (int i = 0; i < 64; ++i) blk[i] = (short)(coef[i] * q[i]);
+                    idct(c.data.data() + (size_t)by * 8 * c.w2 + (size_t)bx * 8, c.w2, blk);
+                }
+        }
     }
 
     // ---- stb's integer IDCT ----
@@ -500,6 +612,7 @@ std::vector<uint8_t> decode_jpeg(const uint8_t* p, uint64_t n, bool flip, uint32
         if (m == 0xDA && d.pos >= n) break;   // data ran to the end without EOI
     }
     if (!d.frame) bad("no frame");
+    if (d.progressive) d.finish_progressive();
 
     const int W = d.width, H = d.height;
     std::vector<uint8_t> out((size_t)W * H * 4);
