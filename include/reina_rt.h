@@ -490,10 +490,11 @@ void reina_rt_model_free(ReinaModelData* model);
  * (stbi_set_flip_vertically_on_load(true), Image.cpp:14: row 0 = the file's
  * last row); from memory (glTF-embedded images, Image.cpp:24-41) they are
  * not. PNG (all bit depths, colour types, tRNS and Adam7 interlacing) and
- * baseline and progressive JPEG (grey or YCbCr, any subsampling, restart
- * intervals, stb's integer IDCT, upsampling and colour conversion); CMYK,
- * 12-bit, lossless and arithmetic-coded JPEGs and other formats return
- * REINA_RT_ERR_UNSUPPORTED. Released with reina_rt_image_free. */
+ * baseline and progressive JPEG (grey, YCbCr, RGB, CMYK or YCCK, any
+ * subsampling, restart intervals, stb's integer IDCT, upsampling and colour
+ * conversion); 12-bit, lossless and arithmetic-coded JPEGs (which stb refuses
+ * too) and other formats return REINA_RT_ERR_UNSUPPORTED. Released with
+ * reina_rt_image_free. */
 typedef struct ReinaImage {
     uint32_t width, height;
     uint8_t* rgba8;              /* width * height * 4 bytes, row-major */
@@ -531,7 +532,7 @@ typedef struct ReinaGltfScene {
     ReinaGltfInstance* instances; uint32_t num_instances;   /* one per node and primitive */
 } ReinaGltfScene;
 
-/* Images the decoders cannot read (CMYK or 12-bit JPEG, other formats) become texture ID -1 instead of
+/* Images the decoders cannot read (12-bit JPEG, other formats) become texture ID -1 instead of
  * failing the load (the reference's fallback for an unmapped texture). */
 #define REINA_RT_GLTF_SKIP_UNDECODABLE_IMAGES 0x1u
 

@@ -22,8 +22,12 @@
 //   * progressive files (ITU-T T.81 Annex G): every scan's coefficients
 //     accumulate per block, then the blocks are dequantised and transformed
 //     as above.
-// Lossless, arithmetic-coded, differential and 12-bit JPEGs, and 4-component
-// (CMYK) files, return REINA_RT_ERR_UNSUPPORTED.
+//   * 4-component files as stb: Adobe transform 0 (CMYK) gives each colour
+//     channel times K (x * y / 255 rounded as stb's blinn_8x8), transform 2
+//     (YCCK) converts YCbCr first and multiplies (255 - channel) by K, any
+//     other file ignores the fourth channel.
+// Lossless, arithmetic-coded, differential and 12-bit JPEGs return
+// REINA_RT_ERR_UNSUPPORTED (stb refuses them too).
 #include <algorithm>
 #include <cstdint>
 #include <cstdlib>
@@ -150,7 +154,7 @@ struct Decoder {
         if (width == 0) bad("zero width");
         if ((uint64_t)width * height > (1ull << 28)) bad("image too large");
         const int nc = u8();
-        if (nc != 1 && nc != 3) unsupported(std::to_string(nc) + "-component JPEG");
+        if (nc != 1 && nc != 3 && nc != 4) unsupported(std::to_string(nc) + "-component JPEG");
         comp.resize((size_t)nc);
         for (Component& c : comp) {
             c.id = u8();
@@ -643,7 +647,27 @@ std::vector<uint8_t> decode_jpeg(const uint8_t* p, uint64_t n, bool flip, uint32
     const bool is_rgb = nc == 3 && ((d.comp[0].id == 'R' && d.comp[1].id == 'G' && d.comp[2].id == 'B') ||
                                     (d.adobe_transform == 0 && !d.jfif));
     const int cr_r = fix(1.40200f), cr_g = -fix(0.71414f), cb_g = -fix(0.34414f), cb_b = fix(1.77200f);
-    const uint8_t* row[3] = {nullptr, nullptr, nullptr};
+    const uint8_t* row[4] = {nullptr, nullptr, nullptr, nullptr};
+    // stb's fixed-point YCbCr -> RGB of sample i into o[0..2], alpha 255
+    const auto ycc = [&](int i, uint8_t* o) {
+        const int yf = (row[0][i] << 20) + (1 << 19);
+        const int cr = row[2][i] - 128, cb = row[1][i] - 128;
+        int rr = yf + cr * cr_r;
+        int gg = yf + (cr * cr_g) + (int)((unsigned)(cb * cb_g) & 0xffff0000u);
+        int bb = yf + cb * cb_b;
+        rr >>= 20;
+        gg >>= 20;
+        bb >>= 20;
+        o[0] = (uint8_t)((unsigned)rr > 255 ? (rr < 0 ? 0 : 255) : rr);
+        o[1] = (uint8_t)((unsigned)gg > 255 ? (gg < 0 ? 0 : 255) : gg);
+        o[2] = (uint8_t)((unsigned)bb > 255 ? (bb < 0 ? 0 : 255) : bb);
+        o[3] = 255;
+    };
+    // x * y / 255 rounded as stb's blinn_8x8: t = x * y + 128, (t + (t >> 8)) >> 8
+    const auto blinn = [](int x, int y) {
+        const unsigned t = (unsigned)(x * y + 128);
+        return (uint8_t)((t + (t >> 8)) >> 8);
+    };
     for (int j = 0; j < H; ++j) {
         for (int k = 0; k < nc; ++k) {
             Resample& r = rs[(size_t)k];
@@ -661,6 +685,23 @@ std::vector<uint8_t> decode_jpeg(const uint8_t* p, uint64_t n, bool flip, uint32
                 o[0] = o[1] = o[2] = row[0][i];
                 o[3] = 255;
             }
+        } else if (nc == 4) {
+            // Adobe transform 0: CMYK as stored (inverted by convention) times K;
+            // 2: YCCK, YCbCr -> RGB, then (255 - rgb) times K; otherwise the
+            // fourth channel is ignored (stb's rules)
+            for (int i = 0; i < W; ++i, o += 4) {
+                const int m = row[3][i];
+                if (d.adobe_transform == 0) {
+                    o[0] = blinn(row[0][i], m);
+                    o[1] = blinn(row[1][i], m);
+                    o[2] = blinn(row[2][i], m);
+                    o[3] = 255;
+                } else {
+                    ycc(i, o);
+                    if (d.adobe_transform == 2)
+                        for (int c = 0; c < 3; ++c) o[c] = blinn(255 - o[c], m);
+                }
+            }
         } else if (is_rgb) {
             for (int i = 0; i < W; ++i, o += 4) {
                 o[0] = row[0][i];
@@ -669,20 +710,7 @@ std::vector<uint8_t> decode_jpeg(const uint8_t* p, uint64_t n, bool flip, uint32
                 o[3] = 255;
             }
         } else {
-            for (int i = 0; i < W; ++i, o += 4) {
-                const int yf = (row[0][i] << 20) + (1 << 19);
-                const int cr = row[2][i] - 128, cb = row[1][i] - 128;
-                int rr = yf + cr * cr_r;
-                int gg = yf + (cr * cr_g) + (int)((unsigned)(cb * cb_g) & 0xffff0000u);
-                int bb = yf + cb * cb_b;
-                rr >>= 20;
-                gg >>= 20;
-                bb >>= 20;
-                o[0] = (uint8_t)((unsigned)rr > 255 ? (rr < 0 ? 0 : 255) : rr);
-                o[1] = (uint8_t)((unsigned)gg > 255 ? (gg < 0 ? 0 : 255) : gg);
-                o[2] = (uint8_t)((unsigned)bb > 255 ? (bb < 0 ? 0 : 255) : bb);
-                o[3] = 255;
-            }
+            for (int i = 0; i < W; ++i, o += 4) ycc(i, o);
         }
     }
     w_out = (uint32_t)W;

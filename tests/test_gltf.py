@@ -339,7 +339,7 @@ def test_gltf_errors(tmp_path):
     b = build_asset()
     from PIL import Image
     jb = io.BytesIO()
-    Image.fromarray(np.zeros((8, 8, 3), np.uint8)).convert("CMYK").save(jb, format="JPEG")   # CMYK: unsupported
+    Image.fromarray(np.zeros((8, 8, 3), np.uint8)).save(jb, format="GIF")   # not PNG or JPEG: unsupported
     b.doc["images"][1] = {"uri": "data:image/jpeg;base64," + base64.b64encode(jb.getvalue()).decode()}
     assert code(b.glb()) == -6
     models, images, inst = gltf.load(b.glb(), True)                # skipped: texture ID -1
