@@ -3,6 +3,7 @@ ray-tracing hot path (see oracle.h). Test infrastructure only: used by
 tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
 product path.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -28,7 +29,34 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    L = ctypes.CDLL(LIB_PATH)
+    _lib = _bind(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+_planted = {}
+
+
+@contextlib.contextmanager
+def planted(k):
+    """Inside the block every function of this module runs a copy of the
+    oracle built with deliberate mistake k (OR_PLANT in or_shade.c /
+    or_post.c), so a test can show that its check fails on that mistake."""
+    global _lib
+    if k not in _planted:
+        name = "_plant/liboracle_plant%d.so" % k
+        r = subprocess.run(["make", "-C", HERE, "-s", name], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+        _planted[k] = _bind(ctypes.CDLL(os.path.join(HERE, name)))
+    saved = lib()
+    _lib = _planted[k]
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _bind(L):
     c_p, u32, u64, i32, f64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_double
     L.or_scene_create.argtypes = [c_p]
     L.or_scene_create.restype = c_p
@@ -76,7 +104,6 @@ def lib():
     L.or_bump_at.restype = None
     L.or_disney_probe.argtypes = [i32, u64, c_p, c_p, c_p, ctypes.POINTER(u32), c_p]
     L.or_disney_probe.restype = None
-    _lib = L
     return L
 
 
@@ -230,7 +257,7 @@ def camera_ray(pc, px, py, W, H, rng, lens_advances=False):
 
 DISNEY_PROBES = {"ggx_d": (0, 2), "smith_g1": (1, 2), "vndf": (2, 3), "pdf_refl": (3, 1), "gtr1": (4, 3),
                  "dc": (5, 1), "diffuse": (6, 3), "sheen": (7, 3), "pdf_clearcoat": (8, 1), "clearcoat": (9, 3),
-                 "disney_sample": (10, 4), "disney_pdf": (11, 1)}
+                 "disney_sample": (10, 4), "disney_pdf": (11, 1), "disney_eval": (12, 4)}
 
 
 def disney_probe(name, n, a=None, b=None, prm=(), seed=1):
@@ -264,7 +291,8 @@ def math_eval(fn, x, y=None):
     return out
 
 
-def postprocess(img, radius=5.0, threshold=1.0, intensity=0.05, exposure=1.0):
+def postprocess(img, radius=5.0, threshold=1.0, intensity=0.05, exposure=1.0, return_bloom=False):
+    """bloom + tonemap: (RGBA8 output, combined image[, blurred bloom image])."""
     H, W = img.shape[:2]
     img = np.ascontiguousarray(img, np.float32)
     ping = np.zeros_like(img)
@@ -273,4 +301,4 @@ def postprocess(img, radius=5.0, threshold=1.0, intensity=0.05, exposure=1.0):
     b = np.array([radius, threshold, intensity], np.float32)
     t = np.array([exposure], np.float32)
     lib().or_postprocess(_p(img), W, H, _p(b), _p(t), _p(ping), _p(pong), _p(out))
-    return out, ping
+    return (out, ping, pong) if return_bloom else (out, ping)

@@ -20,14 +20,29 @@
  * bounded-VNDF pdf (brdfDisney.h.glsl:9-73). or_scene_set_nee switches on this
  * build's correct NEE + MIS instead.
  *
- * Parity unpinned: three terms of evalDisney's mixture that the reference
- * gets wrong (clearcoat outside the normalising sum, specularTransmission
- * passed as `specular`, no glass branch probability), restated line for
- * line. With REINA_RT_TRACE_FIX_DISNEY_MIX they are replaced, and the
- * mixture pdf is then pinned as the density of sampleDisney. Each lobe
- * beneath is pinned against its mathematics through or_disney_probe.
+ * evalDisney's mixture keeps the reference's three odd terms (clearcoat
+ * outside the normalising sum, specularTransmission passed as `specular`, no
+ * glass branch probability). Each is pinned by a closed form built from
+ * separately pinned lobes (tests/test_oracle_disney_mixture.py): at normal
+ * incidence f and pdf reduce to D = 1/(pi ax ay), the bounded-VNDF constant
+ * and GTR1; a pure-glass pdf integrates to the bounded-VNDF reflection mass
+ * plus eta^2 times the refracted VNDF mass; with metallic = 0 and no
+ * transmission the pdf is (1 + clearcoat/4) times sampleDisney's density.
+ * With REINA_RT_TRACE_FIX_DISNEY_MIX the terms are replaced, and the mixture
+ * pdf is then the density of sampleDisney. Each lobe beneath is pinned
+ * against its mathematics through or_disney_probe.
+ *
+ * OR_PLANT (a compile-time switch, 0 by default) builds a copy of the oracle
+ * with one deliberate mistake, so the tests can show that their checks catch
+ * it: 1 clearcoat inside the sum, 2 specular = 1 in the reference mixture,
+ * 3 glass pdf times its branch probability in the reference mixture, 4 the
+ * refraction Jacobian without eta^2 (or_post.c holds 5-9).
  */
 #include "oracle.h"
+
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
 
 #include <pthread.h>
 #include <stdlib.h>
@@ -469,6 +484,9 @@ static v3 evalMicrofacetRefraction(v3 bc, float anisotropic, float roughness, fl
     denom = denom * denom;
     float eta2 = eta * eta;
     float jac = (fabsf(LdH) * eta2) / denom;
+#if OR_PLANT == 4
+    jac = fabsf(LdH) / denom;
+#endif
     *pdf = (((G1 * or_maxf(0.0f, VdH)) * D) * jac) / V.z;
     float F = reflectance(vdot(V, H), eta);
     v3 sq = V3(or_powf(bc.x, 0.5f), or_powf(bc.y, 0.5f), or_powf(bc.z, 0.5f));
@@ -486,6 +504,9 @@ static v3 evalMicrofacetRefraction(v3 bc, float anisotropic, float roughness, fl
  * (REINA_RT_TRACE_FIX_DISNEY_MIX); the reference leaves it out */
 static v3 glassEval(m3 tbn, v3 bc, float anisotropic, float roughness, float eta, v3 n, v3 wi, v3 wo,
                     int didRefract, int bounded, int branch_pdf, float* pdf) {              /* :539-581 */
+#if OR_PLANT == 3
+    branch_pdf = 1;
+#endif
     v3 h = didRefract ? vnormalize(vadd(wo, vscale(wi, eta))) : vnormalize(vadd(wo, wi));
     if (vdot(h, n) < 0.0f) h = vneg(h);
     v3 wiT = m3tmul(tbn, wi), hT = m3tmul(tbn, h);
@@ -545,12 +566,13 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
     float cW = 0.25f * clearcoat;
     float gW = (1.0f - metallic) * st;
     float ws = (dW + mW) + gW;
-    if (mix) ws = ws + cW;
-    float cN = mix ? cW / ws : cW;
+    if (mix || OR_PLANT == 1) ws = ws + cW;
+    float cN = (mix || OR_PLANT == 1) ? cW / ws : cW;
     v3 fd = evalDiffuse(roughness, subsurface, bc, n, wi, wo, h);
     float dP = pdfDiffuse(wo, n);
     v3 fs = evalSheen(bc, wo, h, n, sheenTint);
-    v3 fm = evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, mix ? 1.0f : st, specularTint, metallic, eta);
+    v3 fm = evalMetal(tbn, bc, anisotropic, roughness, n, wi, wo, h, (mix || OR_PLANT == 2) ? 1.0f : st, specularTint,
+                      metallic, eta);
     float mP = pdfMetal(tbn, wi, wo, anisotropic, roughness, bounded);
     v3 fc = evalClearcoat(tbn, wi, wo, cg, h);
     float cP = pdfClearcoat(tbn, wi, wo, h, cg);
@@ -579,7 +601,8 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
  *   9: sampleClearcoat(wi a)                -> out[3k..]            (clearcoatGloss)
  *  10: sampleDisney(wi a)                   -> out[4k..] (wo, didRefract)
  *  11: evalDisney pdf(wi a, wo b, h = |a + b|, didRefract = b.z < 0) -> out[k]
- *      10 and 11 read the material from prm: bc[3] 0-2, specularTint[3] 3-5,
+ *  12: evalDisney f and pdf (as 11)         -> out[4k..] (f.xyz, pdf)
+ *      10, 11 and 12 read the material from prm: bc[3] 0-2, specularTint[3] 3-5,
  *      sheenTint[3] 6-8, anisotropic 9, roughness 10, subsurface 11,
  *      clearcoatGloss 12, eta 13, metallic 14, clearcoat 15,
  *      specularTransmission 16, sheen 17, fixes 18 (as a float) */
@@ -620,6 +643,14 @@ void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const f
                                  prm[9], prm[10], prm[11], prm[12], prm[13], prm[14], prm[15], prm[16], prm[17],
                                  y.z < 0.0f, (uint32_t)prm[18], nz, x, y, vnormalize(vadd(x, y)), &pdf);
                 out[k] = pdf;
+                continue;
+            }
+            case 12: {
+                float pdf;
+                v3 f = evalDisney(I, V3(prm[0], prm[1], prm[2]), V3(prm[3], prm[4], prm[5]), V3(prm[6], prm[7], prm[8]),
+                                  prm[9], prm[10], prm[11], prm[12], prm[13], prm[14], prm[15], prm[16], prm[17],
+                                  y.z < 0.0f, (uint32_t)prm[18], nz, x, y, vnormalize(vadd(x, y)), &pdf);
+                out[4 * k] = f.x; out[4 * k + 1] = f.y; out[4 * k + 2] = f.z; out[4 * k + 3] = pdf;
                 continue;
             }
             default: return;
