@@ -6,10 +6,19 @@
 // thousand pixels with long paths need hundreds more iterations, each paying
 // a launch and the latency of its slowest traversal. Once few paths remain,
 // k_tail takes them over: each wave claims queue entries (one pixel each) and
-// runs extend → shade (→ shadow) steps for its 64 lanes in lockstep, a lane
-// taking the next entry when its pixel has finished all samples. Per pixel the
-// arithmetic is exactly the wavefront kernels' (trav_step, shade_ray), so the
-// image is bit-identical.
+// runs extend → shade steps for its 64 lanes, a lane taking the next entry
+// when its pixel has finished all samples. Per pixel the arithmetic is
+// exactly the wavefront kernels' (trav_step, shade_ray), so the image is
+// bit-identical whatever order the lanes' steps interleave in.
+//
+// Without next-event estimation the wave does not wait for its slowest
+// traversal: it keeps visiting nodes until `shade_min` lanes (or half the
+// lanes holding a pixel, if fewer) have finished their traversal, shades
+// those lanes only, and resumes the others where they stopped. A suspended
+// lane keeps its ray (ra, rb), closest hit and stack position; its slab
+// constants are recomputed from the ray when traversal resumes, so they are
+// not live across the shading code. With next-event estimation every step
+// runs in lockstep: extend, shade, then the shadow ray.
 //
 // The server's version (MULTI) does the same over the shared queues of many
 // requests: each entry carries its request index, the lane reads that
@@ -30,7 +39,7 @@ namespace rt {
 template <int SD, bool Q, bool NEE, bool MULTI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3))) void k_tail(SceneView sv, TraceParams P, PathState S, Queues Q_, uint32_t in_q,
                                               Ctrl* ctrl, uint32_t slot, unsigned long long* stats,
-                                              const JobRec* __restrict__ jobs, uint32_t* done) {
+                                              const JobRec* __restrict__ jobs, uint32_t* done, uint32_t shade_min) {
     __shared__ uint2 stack[SD * 256];
     uint2* stk = stack + threadIdx.x;
     const float4* rin = Q_.ray[in_q];
@@ -44,7 +53,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
     uint64_t num_nodes = sv.num_nodes;
     float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
     uint32_t nrays = 0, nshadow = 0, dummy_v = 0, dummy_t = 0;
-    for (;;) {
+    if constexpr (!NEE) {
+        // Per lane: `have` a pixel; `fresh`: its next ray (ra, rb) is not yet
+        // started; `tracing`: traversal in progress (h, ts, LDS stack);
+        // `ready`: traversal done, h holds the closest hit.
+        bool fresh = false, tracing = false, ready = false;
+        HitRec h;
+        TravState ts;
+        h.t = 0.0f; h.prim = 0u; h.u = 0.0f; h.v = 0.0f;
+        ts.cur = 0; ts.sp = 0;
+        for (;;) {
+            if (!have && !drained) {
+                for (uint32_t k = 0; k < kParts && !have; ++k) {
+                    const uint32_t p = (part + k) % kParts;
+                    const uint32_t n = ctrl[slot].count[p][0];
+                    uint32_t* f = &ctrl[slot].fetch[p][0];
+                    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
+                    const uint32_t j = atomicAdd(f, 1u);
+                    if (j >= n) continue;
+                    const uint32_t i = p * Q_.cap + j;
+                    pw = pin[i];
+                    ra = rin[2 * i];
+                    rb = rin[2 * i + 1];
+                    have = true;
+                    fresh = true;
+                    part = p;
+                }
+                if (!have) {
+                    drained = true;
+                } else if (MULTI) {
+                    J = jobs + (pw >> kSlotBits);
+                    pp = pw & ((1u << kSlotBits) - 1u);
+                    px = pp - J->px_base;
+                    nodes = Q ? static_cast<const void*>(J->sv.qnodes) : static_cast<const void*>(J->sv.nodes);
+                    tris = J->sv.tris;
+                    num_nodes = J->sv.num_nodes;
+                } else {
+                    pp = px = pw;
+                }
+            }
+            const uint64_t holding = __ballot(have);
+            if (holding == 0ull) break;
+            // extend: fresh rays start, suspended ones resume, until enough lanes are ready
+            {
+                TRay r;
+                if (fresh || tracing) tray_init(r, ra, rb);
+                if (fresh) {
+                    h = miss_record(r);
+                    ts.cur = 0;
+                    ts.sp = 0;
+                    tracing = r.valid && num_nodes != 0;
+                    ready = !tracing;
+                    fresh = false;
+                    nrays++;
+                }
+                const uint32_t half = ((uint32_t)__popcll(holding) + 1u) >> 1;
+                uint32_t need = shade_min < half ? shade_min : half;
+                if (need == 0u) need = 1u;   // every round shades at least one lane
+                for (;;) {
+                    if (__ballot(tracing) == 0ull) break;
+                    if ((uint32_t)__popcll(__ballot(ready)) >= need) break;
+                    if (tracing) {
+                        tracing = trav_step<false, Q>(nodes, tris, r, h, ts, stk, 256, dummy_v, dummy_t);
+                        ready = !tracing;
+                    }
+                }
+            }
+            // shade the lanes whose traversal finished
+            ShadeResult res;
+            res.push = false;
+            res.finished = false;
+            const bool shading = ready;
+            if (shading) {
+                const float4 hh = make_float4(h.t, __uint_as_float(h.prim), h.u, h.v);
+                res = MULTI ? shade_ray<false>(J->sv, J->P, S, pp, px, ra, rb, hh)
+                            : shade_ray<false>(sv, P, S, pp, px, ra, rb, hh);
+                ready = false;
+            }
+            if (MULTI) {
+                const bool fin = shading && res.finished;
+                uint64_t fm = __ballot(fin ? 1 : 0);
+                while (fm) {
+                    const uint32_t lead = (uint32_t)__builtin_ctzll(fm);
+                    const uint32_t jl = __shfl(pw >> kSlotBits, (int)lead);
+                    const uint64_t same = __ballot((fin && (pw >> kSlotBits) == jl) ? 1 : 0);
+                    if (__lane_id() == lead) atomicAdd(done + jl, (uint32_t)__popcll(same));
+                    fm &= ~same;
+                }
+            }
+            if (shading) {
+                if (res.push) {
+                    ra = make_float4(res.no.x, res.no.y, res.no.z, 0.0f);
+                    rb = make_float4(res.nd.x, res.nd.y, res.nd.z, 10000.0f);
+                    fresh = true;
+                } else {
+                    have = false;
+                }
+            }
+        }
+    } else for (;;) {
         // lanes without a pixel take the next queue entry (own XCD's partition first)
         if (!have && !drained) {
             for (uint32_t k = 0; k < kParts && !have; ++k) {
@@ -159,10 +266,10 @@ static void launch_tail_qn(const SceneView& sv, const TraceParams& P, const Path
                            uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const JobRec* jobs,
                            uint32_t* done, const Tuning& tu, uint32_t blocks, hipStream_t st) {
     switch (tu.stack_depth) {
-        case 8: hipLaunchKernelGGL((k_tail<8, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
-        case 16: hipLaunchKernelGGL((k_tail<16, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
-        case 24: hipLaunchKernelGGL((k_tail<24, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
-        default: hipLaunchKernelGGL((k_tail<32, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done); break;
+        case 8: hipLaunchKernelGGL((k_tail<8, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done, tu.refill_min); break;
+        case 16: hipLaunchKernelGGL((k_tail<16, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done, tu.refill_min); break;
+        case 24: hipLaunchKernelGGL((k_tail<24, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done, tu.refill_min); break;
+        default: hipLaunchKernelGGL((k_tail<32, Q, NEE, MULTI>), dim3(blocks), dim3(256), 0, st, sv, P, S, Q_, in_q, ctrl, slot, stats, jobs, done, tu.refill_min); break;
     }
 }
 
