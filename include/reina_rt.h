@@ -280,11 +280,14 @@ typedef struct ReinaTraceStats {
  * and/or next-event estimation. */
 #define REINA_RT_TRACE_TIME_EXTEND 0x1u
 #define REINA_RT_TRACE_COUNT       0x2u
-/* Next-event estimation with multiple importance sampling at Lambertian
- * hits (one light sample and shadow ray per bounce, power heuristic). The
- * reference compiles this but never runs it (skipNEE = true,
- * shaders/raytrace/raytrace.rgen.glsl:146), so it is off by default; the
- * default renders match the reference's active path. Needs an emissive
+/* Next-event estimation with multiple importance sampling at Lambertian and
+ * Disney hits (one light sample and shadow ray per bounce, power heuristic),
+ * as the reference's directLight intends for materialIDs 0 and 3
+ * (shaders/raytrace/raytrace.rgen.glsl:43-95). The reference compiles this
+ * but never runs it (skipNEE = true, :146), so it is off by default; the
+ * default renders match the reference's active path. A Disney light sample
+ * carries the expected weight of the hit shader's own continuation, so NEE
+ * changes the variance and not the expected image. Needs an emissive
  * instance (REINA_RT_ERR_NO_EMISSIVE otherwise). */
 #define REINA_RT_TRACE_NEE         0x4u
 /* Corrections of reference behaviours that the default renders reproduce

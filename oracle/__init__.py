@@ -257,7 +257,8 @@ def camera_ray(pc, px, py, W, H, rng, lens_advances=False):
 
 DISNEY_PROBES = {"ggx_d": (0, 2), "smith_g1": (1, 2), "vndf": (2, 3), "pdf_refl": (3, 1), "gtr1": (4, 3),
                  "dc": (5, 1), "diffuse": (6, 3), "sheen": (7, 3), "pdf_clearcoat": (8, 1), "clearcoat": (9, 3),
-                 "disney_sample": (10, 4), "disney_pdf": (11, 1), "disney_eval": (12, 4)}
+                 "disney_sample": (10, 4), "disney_pdf": (11, 1), "disney_eval": (12, 4),
+                 "disney_continue": (13, 7), "disney_nee": (14, 4)}
 
 
 def disney_probe(name, n, a=None, b=None, prm=(), seed=1):

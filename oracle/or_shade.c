@@ -36,7 +36,8 @@
  * with one deliberate mistake, so the tests can show that their checks catch
  * it: 1 clearcoat inside the sum, 2 specular = 1 in the reference mixture,
  * 3 glass pdf times its branch probability in the reference mixture, 4 the
- * refraction Jacobian without eta^2 (or_post.c holds 5-9).
+ * refraction Jacobian without eta^2 (or_post.c holds 5-9), 10 a Disney light
+ * sample weighted by f cos alone, 11 without its glass term.
  */
 #include "oracle.h"
 
@@ -586,6 +587,97 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
     return r;
 }
 
+/* ---- next-event estimation at Disney hits (the library's, shade.h) --------
+ * The reference's directLight evaluates evalDisney for materialID 3
+ * (raytrace.rgen.glsl:57-83) but never runs. A light sample at a Disney hit
+ * carries g(w) = q_ng f cos / pdf + q_g: the expected weight of the hit
+ * shader's own continuation (f cos / pdf from a diffuse, metal or clearcoat
+ * lobe, 1 from the glass lobe, Q4), with q_ng, q_g the densities of
+ * sampleDisney's non-glass and glass lobes at w; MIS uses q_ng + q_g. */
+
+/* sampleDisney's density at wo (brdfDisney.h.glsl:609-654), from the
+ * samplers themselves: lobe probabilities times the cosine density (diffuse),
+ * Heitz's VNDF reflected, G1(i) D(m) / (4 i.z) with m.z > 0 (metal), GTR1
+ * reflected, Dc(h) h.z / (4 |o.h|) with h oriented up (clearcoat), and for
+ * glass the VNDF reflected times the reflection probability plus the VNDF
+ * refracted times 1 - F with Walter et al.'s half-vector Jacobian
+ * |o.h| / (eta i.h + o.h)^2. png: the non-glass lobes, pg: glass. */
+static float vndf_reflection_density(v3 wiT, v3 woT, float ax, float ay) {
+    if (!(wiT.z > 0.0f)) return 0.0f;
+    const v3 m = vnormalize(vadd(wiT, woT));
+    if (!(m.z > 0.0f)) return 0.0f;
+    return (evalDm(m, ax, ay) * smithG(wiT, ax, ay)) / (4.0f * wiT.z);
+}
+
+static void disney_density(m3 tbn, float anisotropic, float roughness, float cg, float eta, float metallic,
+                           float clearcoat, float st, v3 n, v3 wi, v3 wo, float* png, float* pg) {
+    const float dW = (1.0f - st) * (1.0f - metallic);
+    const float mW = metallic;
+    const float cW = 0.25f * clearcoat;
+    const float gW = (1.0f - metallic) * st;
+    const float ws = ((dW + mW) + gW) + cW;
+    float ax, ay;
+    alphas(anisotropic, roughness, &ax, &ay);
+    const v3 wiT = m3tmul(tbn, wi), woT = m3tmul(tbn, wo);
+    const float dP = pdfDiffuse(wo, n);
+    const float mr = vndf_reflection_density(wiT, woT, ax, ay);
+    const float ag = (1.0f - cg) * 0.1f + cg * 0.001f;
+    v3 hc = vnormalize(vadd(wiT, woT));
+    if (hc.z < 0.0f) hc = vneg(hc);
+    const float ohc = fabsf(vdot(woT, hc));
+    const float cP = ohc > 0.0f ? (evalDc(ag, hc) * hc.z) / (4.0f * ohc) : 0.0f;
+    float gr = 0.0f, gt = 0.0f;
+    if (mr > 0.0f) {
+        const v3 m = vnormalize(vadd(wiT, woT));
+        const float c = vdot(wiT, m);
+        const float sn = sqrtf(1.0f - c * c);
+        gr = mr * ((eta * sn > 1.0f) ? 1.0f : reflectance(c, eta));
+    }
+    if (wiT.z > 0.0f) {   /* refraction can also leave above the horizon through a tilted microfacet */
+        v3 h = vnormalize(vadd(woT, vscale(wiT, eta)));
+        if (h.z < 0.0f) h = vneg(h);
+        const float ih = vdot(wiT, h), oh = vdot(woT, h);
+        const float sn = sqrtf(1.0f - ih * ih);
+        if (h.z > 0.0f && ih > 0.0f && oh < 0.0f && !(eta * sn > 1.0f)) {
+            const float den = eta * ih + oh;
+            const float dv = ((smithG(wiT, ax, ay) * ih) * evalDm(h, ax, ay)) / wiT.z;
+            gt = ((dv * fabsf(oh)) / (den * den)) * (1.0f - reflectance(ih, eta));
+        }
+    }
+    *png = ((dP * dW) / ws + (mr * mW) / ws) + (cP * cW) / ws;
+    *pg = ((gr + gt) * gW) / ws;
+}
+
+typedef struct {
+    m3 tbn;
+    v3 albedo, wn, wi, pos, ng;
+    float eta;
+    float p_bsdf;   /* density of the direction the hit shader sampled */
+} OrDisneyNee;
+
+static v3 disney_nee_weight(const OrDisneyNee* c, const ReinaInstanceProperties* p, uint32_t fixes, v3 w,
+                            float* ptot) {
+    const v3 h = vnormalize(vadd(w, c->wi));
+    float pdf_ev;
+    const v3 f = evalDisney(c->tbn, c->albedo, pv3(p->specularTint), pv3(p->sheenTint), p->anisotropic,
+                            p->roughness, p->subsurface, p->clearcoatGloss, c->eta, p->metallic, p->clearcoat,
+                            p->specularTransmission, p->sheen, 0, fixes, c->wn, c->wi, w, h, &pdf_ev);
+    float png, pg;
+    disney_density(c->tbn, p->anisotropic, p->roughness, p->clearcoatGloss, c->eta, p->metallic, p->clearcoat,
+                   p->specularTransmission, c->wn, c->wi, w, &png, &pg);
+    *ptot = png + pg;
+    const float cx = or_maxf(vdot(c->wn, w), 0.0f);
+#if OR_PLANT == 10
+    return vscale(f, cx);   /* the textbook f cos, ignoring how the hit shader weights its continuations */
+#endif
+#if OR_PLANT == 11
+    pg = 0.0f;              /* the glass lobe's weight-1 continuations left out */
+#endif
+    v3 g = V3(pg, pg, pg);
+    if (pdf_ev > 0.0f && cx > 0.0f) g = vadd(vscale(f, (png / pdf_ev) * cx), g);
+    return g;
+}
+
 /* Batch probes of the Disney pieces above for the tests, in the tangent frame
  * (tbn = identity, n = +z). prm holds the piece's scalar parameters; rng is one
  * stream carried across the n samples of a sampling probe.
@@ -602,7 +694,12 @@ static v3 evalDisney(m3 tbn, v3 bc, v3 specularTint, v3 sheenTint, float anisotr
  *  10: sampleDisney(wi a)                   -> out[4k..] (wo, didRefract)
  *  11: evalDisney pdf(wi a, wo b, h = |a + b|, didRefract = b.z < 0) -> out[k]
  *  12: evalDisney f and pdf (as 11)         -> out[4k..] (f.xyz, pdf)
- *      10, 11 and 12 read the material from prm: bc[3] 0-2, specularTint[3] 3-5,
+ *  13: the Disney hit shader's continuation from wi a: sampleDisney, then
+ *      its weight f cos / pdf, or 1 when the glass lobe was chosen (the
+ *      raygen skips that multiply, Q4)     -> out[7k..] (wo, weight, choseGlass)
+ *  14: next-event estimation at wi a toward wo b: disney_nee_weight's g and
+ *      the sampler's density there          -> out[4k..] (g.xyz, density)
+ *      10-14 read the material from prm: bc[3] 0-2, specularTint[3] 3-5,
  *      sheenTint[3] 6-8, anisotropic 9, roughness 10, subsurface 11,
  *      clearcoatGloss 12, eta 13, metallic 14, clearcoat 15,
  *      specularTransmission 16, sheen 17, fixes 18 (as a float) */
@@ -643,6 +740,34 @@ void or_disney_probe(int fn, uint64_t n, const float* a, const float* b, const f
                                  prm[9], prm[10], prm[11], prm[12], prm[13], prm[14], prm[15], prm[16], prm[17],
                                  y.z < 0.0f, (uint32_t)prm[18], nz, x, y, vnormalize(vadd(x, y)), &pdf);
                 out[k] = pdf;
+                continue;
+            }
+            case 13: {
+                int dr, cg;
+                v3 wo = sampleDisney(I, V3(prm[0], prm[1], prm[2]), prm[9], prm[10], prm[12], prm[13], prm[14],
+                                     prm[15], prm[16], nz, x, &dr, &cg, rng);
+                float pdf;
+                v3 f = evalDisney(I, V3(prm[0], prm[1], prm[2]), V3(prm[3], prm[4], prm[5]), V3(prm[6], prm[7], prm[8]),
+                                  prm[9], prm[10], prm[11], prm[12], prm[13], prm[14], prm[15], prm[16], prm[17],
+                                  dr, (uint32_t)prm[18], nz, x, wo, vnormalize(vadd(wo, x)), &pdf);
+                v3 wgt = vscale(f, or_maxf(vdot(nz, wo), 0.0f));
+                wgt = cg ? V3(1.0f, 1.0f, 1.0f) : V3(wgt.x / pdf, wgt.y / pdf, wgt.z / pdf);
+                out[7 * k] = wo.x; out[7 * k + 1] = wo.y; out[7 * k + 2] = wo.z;
+                out[7 * k + 3] = wgt.x; out[7 * k + 4] = wgt.y; out[7 * k + 5] = wgt.z; out[7 * k + 6] = (float)cg;
+                continue;
+            }
+            case 14: {
+                ReinaInstanceProperties pp;
+                memset(&pp, 0, sizeof pp);
+                for (int q = 0; q < 3; ++q) { pp.specularTint[q] = prm[3 + q]; pp.sheenTint[q] = prm[6 + q]; }
+                pp.anisotropic = prm[9]; pp.roughness = prm[10]; pp.subsurface = prm[11]; pp.clearcoatGloss = prm[12];
+                pp.metallic = prm[14]; pp.clearcoat = prm[15]; pp.specularTransmission = prm[16]; pp.sheen = prm[17];
+                OrDisneyNee dn;
+                dn.tbn = I; dn.albedo = V3(prm[0], prm[1], prm[2]); dn.wn = nz; dn.wi = x;
+                dn.pos = V3(0.0f, 0.0f, 0.0f); dn.ng = nz; dn.eta = prm[13]; dn.p_bsdf = 0.0f;
+                float pt;
+                v3 g = disney_nee_weight(&dn, &pp, (uint32_t)prm[18], y, &pt);
+                out[4 * k] = g.x; out[4 * k + 1] = g.y; out[4 * k + 2] = g.z; out[4 * k + 3] = pt;
                 continue;
             }
             case 12: {
@@ -832,8 +957,8 @@ static void chs_dielectric(const OrScene* s, Payload* pld, const HitCtx* c) {
     pld->eta = 0.0f;
 }
 
-/* disney.rchit.glsl:36-198 */
-static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c) {
+/* disney.rchit.glsl:36-198; dn (optional): what next-event estimation needs */
+static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c, OrDisneyNee* dn) {
     HitInfo hi = getObjectHitInfo(s, c);
     const ReinaInstanceProperties* p = c->props;
     if (p->cullBackface != 0u && !hi.frontFace) { skip_hit(pld, &hi, c); return; }
@@ -879,6 +1004,19 @@ static void chs_disney(const OrScene* s, Payload* pld, const HitCtx* c) {
         pld->accumulatedDistance = pld->accumulatedDistance + vlength(vsub(hi.worldPosition, c->worldRayOrigin));
     else
         pld->accumulatedDistance = 0.0f;
+    if (dn) {
+        dn->tbn = hi.tbn;
+        dn->albedo = albedo;
+        dn->wn = wn;
+        dn->wi = wi;
+        dn->pos = hi.worldPosition;
+        dn->ng = hi.worldNormalGeometry;
+        dn->eta = eta;
+        float png, pg;
+        disney_density(hi.tbn, p->anisotropic, p->roughness, p->clearcoatGloss, eta, p->metallic, p->clearcoat,
+                       p->specularTransmission, wn, wi, rayDir, &png, &pg);
+        dn->p_bsdf = png + pg;
+    }
 }
 
 /* raytrace.rmiss.glsl:10-19 */
@@ -895,9 +1033,11 @@ void or_sky(const float* dir, float* out) {
 /* ================== next-event estimation (the library's) ================ */
 /* The reference never runs its NEE (skipNEE = true, raytrace.rgen.glsl:146).
  * This restates the library's shade_ray<NEE> (shade_step.h): a light sample
- * at Lambertian hits whose continuation can still be traced, power-heuristic
- * MIS against the cosine pdf of the same direction, and the matching weight
- * on emitters reached by the continuation. */
+ * at Lambertian and Disney hits whose continuation can still be traced,
+ * power-heuristic MIS against the density with which the hit's own sampler
+ * draws the same direction (cosine; disney_density), and the matching weight
+ * on emitters reached by the continuation. A Disney hit samples a light also
+ * when its continuation took the glass lobe. */
 static float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
 
 static float light_pdf(float pdf_area, float d, v3 ng, v3 w, int cull) {
@@ -936,7 +1076,9 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
     v3 mis_x = V3(0.0f, 0.0f, 0.0f);   /* origin of the last Lambertian continuation */
     float mis_pb = 0.0f;               /* its cosine pdf, 0: no MIS                   */
     for (uint32_t seg = 0; seg < maxBounces; ++seg) {
-        int lambert_hit = 0, emissive_mis = 0;
+        int lambert_hit = 0, disney_hit = 0, emissive_mis = 0;
+        OrDisneyNee dn;
+        const ReinaInstanceProperties* hp = NULL;
         v3 nee_n = V3(0.0f, 0.0f, 0.0f), nee_albedo = nee_n;
         float emis_w = 1.0f;
         ReinaRay ray = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, 10000.0f};
@@ -961,11 +1103,13 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
                 case 0: chs_lambert_metal(s, pld, &c, 0); break;
                 case 1: chs_lambert_metal(s, pld, &c, 1); break;
                 case 2: chs_dielectric(s, pld, &c); break;
-                default: chs_disney(s, pld, &c); break;
+                default: chs_disney(s, pld, &c, nee ? &dn : NULL); break;
             }
             if (nee && !pld->skip) {
                 HitInfo hi = getObjectHitInfo(s, &c);
                 lambert_hit = c.inst->material == 0u;
+                disney_hit = c.inst->material == 3u;
+                hp = c.props;
                 nee_n = pld->surfaceNormal;   /* normal-mapped shading normal */
                 nee_albedo = pld->albedo;     /* textured colour */
                 const int32_t ek = s->inst_emi[s->tri_inst[hit.prim]];
@@ -988,14 +1132,17 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
             L = vadd(L, vmul(pld->color, acc));
             break;
         }
-        if (!pld->insideDielectric) {
-            v3 contrib = vadd(V3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? vscale(pld->emission, emis_w)
-                                                                        : pld->emission);
-            L = vadd(L, vmul(contrib, acc));
+        {
+            const int inside = pld->insideDielectric;
+            if (!inside) {
+                v3 contrib = vadd(V3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? vscale(pld->emission, emis_w)
+                                                                            : pld->emission);
+                L = vadd(L, vmul(contrib, acc));
+            }
             int shadow = 0;
             v3 scontrib = V3(0.0f, 0.0f, 0.0f);
             ReinaRay sr;
-            if (nee && lambert_hit && seg + 1u < maxBounces && s->nemi > 0u) {
+            if (nee && (lambert_hit || disney_hit) && seg + 1u < maxBounces && s->nemi > 0u) {
                 const float u1 = or_random(&pld->rngState), u2 = or_random(&pld->rngState);
                 const float u3 = or_random(&pld->rngState), u4 = or_random(&pld->rngState);
                 const struct OrEmissive* ei = &s->emi[cdf_pick_emi(s, u1)];
@@ -1009,30 +1156,50 @@ static uint64_t trace_segments(const OrScene* s, Payload* pld, v3 o, v3 d, uint3
                 const float alpha = (1.0f - beta) - gamma;
                 v3 y = vadd(vadd(vscale(w[0], alpha), vscale(w[1], beta)), vscale(w[2], gamma));
                 v3 ng = vnormalize(vcross(vsub(w[1], w[0]), vsub(w[2], w[0])));
-                v3 dv = vsub(y, pld->rayOrigin);
-                float dist = vlength(dv);
-                v3 wdir = vnormalize(dv);
-                float pl = light_pdf(ei->pdf_area, dist, ng, wdir, lp->cullBackface != 0u);
-                float cx = or_maxf(vdot(nee_n, wdir), 0.0f);
-                if (pl > 0.0f && cx > 0.0f && dist > 0.001f) {
-                    float pb = cx / K_PI;
-                    float wn = power_heuristic(pl, pb);
-                    v3 f = V3(nee_albedo.x / K_PI, nee_albedo.y / K_PI, nee_albedo.z / K_PI);
-                    scontrib = vmul(vscale(vmul(pv3(lp->emission), f), (cx / pl) * wn), acc);
-                    shadow = 1;
-                    ReinaRay r0 = {pld->rayOrigin.x, pld->rayOrigin.y, pld->rayOrigin.z, 0.0f,
-                                   wdir.x, wdir.y, wdir.z, dist - 0.001f};
-                    sr = r0;
+                if (lambert_hit) {
+                    v3 dv = vsub(y, pld->rayOrigin);
+                    float dist = vlength(dv);
+                    v3 wdir = vnormalize(dv);
+                    float pl = light_pdf(ei->pdf_area, dist, ng, wdir, lp->cullBackface != 0u);
+                    float cx = or_maxf(vdot(nee_n, wdir), 0.0f);
+                    if (pl > 0.0f && cx > 0.0f && dist > 0.001f) {
+                        float pb = cx / K_PI;
+                        float wn = power_heuristic(pl, pb);
+                        v3 f = V3(nee_albedo.x / K_PI, nee_albedo.y / K_PI, nee_albedo.z / K_PI);
+                        scontrib = vmul(vscale(vmul(pv3(lp->emission), f), (cx / pl) * wn), acc);
+                        shadow = 1;
+                        ReinaRay r0 = {pld->rayOrigin.x, pld->rayOrigin.y, pld->rayOrigin.z, 0.0f,
+                                       wdir.x, wdir.y, wdir.z, dist - 0.001f};
+                        sr = r0;
+                    }
+                    mis_x = pld->rayOrigin;
+                    mis_pb = or_maxf(vdot(nee_n, pld->rayDirection), 0.0f) / K_PI;
+                } else {
+                    /* the shadow ray leaves on the light's side of the surface */
+                    v3 so = offsetPositionForDielectric(dn.pos, dn.ng, vnormalize(vsub(y, dn.pos)));
+                    v3 dv = vsub(y, so);
+                    float dist = vlength(dv);
+                    v3 wdir = vnormalize(dv);
+                    float pl = light_pdf(ei->pdf_area, dist, ng, wdir, lp->cullBackface != 0u);
+                    if (pl > 0.0f && dist > 0.001f) {
+                        float pt;
+                        v3 g = disney_nee_weight(&dn, hp, s->fixes, wdir, &pt);
+                        if (g.x > 0.0f || g.y > 0.0f || g.z > 0.0f) {
+                            float wn = power_heuristic(pl, pt);
+                            scontrib = vmul(vscale(vmul(pv3(lp->emission), g), wn / pl), acc);
+                            shadow = 1;
+                            ReinaRay r0 = {so.x, so.y, so.z, 0.0f, wdir.x, wdir.y, wdir.z, dist - 0.001f};
+                            sr = r0;
+                        }
+                    }
+                    mis_x = pld->rayOrigin;
+                    mis_pb = dn.p_bsdf;
                 }
-                mis_x = pld->rayOrigin;
-                mis_pb = or_maxf(vdot(nee_n, pld->rayDirection), 0.0f) / K_PI;
             } else if (nee) {
                 mis_pb = 0.0f;
             }
-            acc = vmul(acc, pld->color);
+            if (!inside) acc = vmul(acc, pld->color);
             if (shadow && !or_occluded(s, &sr)) L = vadd(L, scontrib);
-        } else if (nee) {
-            mis_pb = 0.0f;
         }
     }
     *Lout = L;

@@ -512,6 +512,103 @@ RT_D V3 eval_disney(const M3& tbn, V3 bc, V3 specularTint, V3 sheenTint, float a
     return r;
 }
 
+// ---- next-event estimation at Disney hits ----------------------------------------
+// The reference's directLight evaluates evalDisney for materialID 3
+// (raytrace.rgen.glsl:57-83) but never runs (skipNEE = true, :146). Here a
+// light sample at a Disney hit is weighted so that NEE + MIS estimates the
+// same integral as the hit shader's own BSDF sampling: a continuation from a
+// diffuse, metal or clearcoat lobe carries f cos / pdf (evalDisney with
+// didRefract = false), one from the glass lobe carries weight 1 (the raygen
+// skips that bounce's multiply, Q4). The light sample at w therefore carries
+// g(w) = q_ng(w) f(w) cos / pdf(w) + q_g(w), with q_ng and q_g the densities
+// of sampleDisney's non-glass and glass lobes at w, and the power heuristic
+// against q_ng + q_g, the density the continuation was drawn from.
+
+// Density of sampleDisney's directions at wo (brdfDisney.h.glsl:609-654),
+// from the samplers themselves rather than the reference's pdf functions
+// (which give D > 0 for back-facing microfacets, cut clearcoat at the
+// horizon and carry eta^2 in the refraction Jacobian): lobe probabilities
+// times
+//   diffuse    cos / pi about the shading normal (sampleDiffuse);
+//   metal      Heitz's VNDF reflected: G1(i) D(m) / (4 i.z), m = |i + o|,
+//              m.z > 0;
+//   clearcoat  GTR1 reflected: Dc(h) h.z / (4 |o.h|), h = +-|i + o| with h.z > 0;
+//   glass      VNDF reflected times the reflection probability (1 under
+//              total internal reflection, else Schlick's F), plus VNDF
+//              refracted times 1 - F with Walter et al.'s half-vector
+//              Jacobian |o.h| / (eta i.h + o.h)^2, h = +-|o + eta i|.
+// png: diffuse, metal and clearcoat; pg: glass. Tangent frame tbn, i.z > 0.
+RT_D float vndf_reflection_density(V3 wiT, V3 woT, float ax, float ay) {
+    if (!(wiT.z > 0.0f)) return 0.0f;
+    const V3 m = normalize(add(wiT, woT));
+    if (!(m.z > 0.0f)) return 0.0f;
+    return (eval_dm(m, ax, ay) * smith_g(wiT, ax, ay)) / (4.0f * wiT.z);
+}
+
+RT_D void disney_density(const M3& tbn, float anisotropic, float roughness, float cg, float eta, float metallic,
+                         float clearcoat, float st, V3 n, V3 wi, V3 wo, float& png, float& pg) {
+    const float dW = (1.0f - st) * (1.0f - metallic);
+    const float mW = metallic;
+    const float cW = 0.25f * clearcoat;
+    const float gW = (1.0f - metallic) * st;
+    const float ws = ((dW + mW) + gW) + cW;
+    float ax, ay;
+    alphas(anisotropic, roughness, ax, ay);
+    const V3 wiT = tmul(tbn, wi), woT = tmul(tbn, wo);
+    const float dP = pdf_diffuse(wo, n);
+    const float mr = vndf_reflection_density(wiT, woT, ax, ay);
+    const float ag = (1.0f - cg) * 0.1f + cg * 0.001f;
+    V3 hc = normalize(add(wiT, woT));
+    if (hc.z < 0.0f) hc = neg(hc);
+    const float ohc = fabsf(dot(woT, hc));
+    const float cP = ohc > 0.0f ? (eval_dc(ag, hc) * hc.z) / (4.0f * ohc) : 0.0f;
+    float gr = 0.0f, gt = 0.0f;
+    if (mr > 0.0f) {
+        const V3 m = normalize(add(wiT, woT));
+        const float c = dot(wiT, m);
+        const float sn = sqrtf_(1.0f - c * c);
+        gr = mr * ((eta * sn > 1.0f) ? 1.0f : reflectance(c, eta));
+    }
+    if (wiT.z > 0.0f) {   // refraction can also leave above the horizon through a tilted microfacet
+        V3 h = normalize(add(woT, scale(wiT, eta)));
+        if (h.z < 0.0f) h = neg(h);
+        const float ih = dot(wiT, h), oh = dot(woT, h);
+        const float sn = sqrtf_(1.0f - ih * ih);
+        if (h.z > 0.0f && ih > 0.0f && oh < 0.0f && !(eta * sn > 1.0f)) {
+            const float den = eta * ih + oh;
+            const float dv = ((smith_g(wiT, ax, ay) * ih) * eval_dm(h, ax, ay)) / wiT.z;
+            gt = ((dv * fabsf(oh)) / (den * den)) * (1.0f - reflectance(ih, eta));
+        }
+    }
+    png = ((dP * dW) / ws + (mr * mW) / ws) + (cP * cW) / ws;
+    pg = ((gr + gt) * gW) / ws;
+}
+
+// What a Disney hit leaves for next-event estimation (chs_disney fills it).
+struct DisneyNee {
+    M3 tbn;
+    V3 albedo, wn, wi, pos, ng;
+    float eta;
+    float p_bsdf;   // q_ng + q_g of the direction the hit shader sampled
+};
+
+// g(w) of a light sample along w, and the sampler's density there (ptot).
+RT_D V3 disney_nee_weight(const DisneyNee& c, const ReinaInstanceProperties& p, uint32_t fixes, V3 w, float& ptot) {
+    const V3 h = normalize(add(w, c.wi));
+    float pdf_ev;
+    const V3 f = eval_disney(c.tbn, c.albedo, ldv3(p.specularTint), ldv3(p.sheenTint), p.anisotropic, p.roughness,
+                             p.subsurface, p.clearcoatGloss, c.eta, p.metallic, p.clearcoat, p.specularTransmission,
+                             p.sheen, false, fixes, c.wn, c.wi, w, h, pdf_ev);
+    float png, pg;
+    disney_density(c.tbn, p.anisotropic, p.roughness, p.clearcoatGloss, c.eta, p.metallic, p.clearcoat,
+                   p.specularTransmission, c.wn, c.wi, w, png, pg);
+    ptot = png + pg;
+    const float cx = maxf_(dot(c.wn, w), 0.0f);
+    V3 g = v3(pg, pg, pg);
+    if (pdf_ev > 0.0f && cx > 0.0f) g = add(scale(f, (png / pdf_ev) * cx), g);
+    return g;
+}
+
 // ---- hit shaders ---------------------------------------------------------------
 // ---- textures -------------------------------------------------------------
 // The hit shaders' texture(textures[id], uv): RGBA8 UNORM, linear filter,
@@ -674,7 +771,7 @@ RT_D void chs_dielectric(Payload& pld, const SceneView& sv, const HitInfo& hi, c
 // disney.rchit.glsl:36-198
 // fixes: the REINA_RT_TRACE_FIX_* bits of the trace (eval_disney reads two)
 RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const ReinaInstanceProperties& p, V3 O,
-                     V3 D, uint32_t fixes) {
+                     V3 D, uint32_t fixes, DisneyNee* dn = nullptr) {
     if (p.cullBackface != 0u && !hi.frontFace) { skip_hit(pld, hi, D); return; }
     float ux = hi.uvx - floorf(hi.uvx), uy = hi.uvy - floorf(hi.uvy);   // mod(uv, 1.0)
     if (p.bumpMapTexID >= 0) bump_mapping(sv, p.bumpMapTexID, ux, uy, normalize(D), hi.tbn);
@@ -711,6 +808,19 @@ RT_D void chs_disney(Payload& pld, const SceneView& sv, const HitInfo& hi, const
         pld.accumulatedDistance = pld.accumulatedDistance + length(sub(hi.worldPosition, O));
     else
         pld.accumulatedDistance = 0.0f;
+    if (dn) {
+        dn->tbn = hi.tbn;
+        dn->albedo = albedo;
+        dn->wn = wn;
+        dn->wi = wi;
+        dn->pos = hi.worldPosition;
+        dn->ng = hi.worldNormalGeometry;
+        dn->eta = eta;
+        float png, pg;
+        disney_density(hi.tbn, p.anisotropic, p.roughness, p.clearcoatGloss, eta, p.metallic, p.clearcoat,
+                       p.specularTransmission, wn, wi, rayDir, png, pg);
+        dn->p_bsdf = png + pg;
+    }
 }
 
 // ---- raygen camera (raytrace.rgen.glsl:34-41, 194-247) ------------------------

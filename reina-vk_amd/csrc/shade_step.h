@@ -25,14 +25,19 @@ struct ShadeResult {
 // The reference compiles NEE but never runs it (skipNEE = true,
 // shaders/raytrace/raytrace.rgen.glsl:146); when enabled it pairs the light
 // pdf of one direction with the BSDF pdf of another (:153-170). This is the
-// standard pairing: at a Lambertian hit x whose continuation can still be
-// traced (segment + 1 < maxBounces) one point y is sampled on the emitters
-// (nee.h.glsl:52-119: instance by weight, triangle by area, uniform point),
-// connected by a shadow ray and weighted by the power heuristic of its
-// solid-angle light pdf p_l against the cosine pdf p_b of that same
-// direction; an emitter that the continuation from x reaches is weighted by
-// p_b^2 / (p_b^2 + p_l^2) with p_l computed for that hit. Emission reached
-// from the camera or from a metal, dielectric or Disney bounce keeps weight 1.
+// standard pairing: at a Lambertian or Disney hit x whose continuation can
+// still be traced (segment + 1 < maxBounces) one point y is sampled on the
+// emitters (nee.h.glsl:52-119: instance by weight, triangle by area, uniform
+// point), connected by a shadow ray and weighted by the power heuristic of
+// its solid-angle light pdf p_l against the density p_b with which x's own
+// sampler draws that same direction (cosine for Lambertian; disney_density
+// for Disney, whose light samples carry disney_nee_weight); an emitter that
+// the continuation from x reaches is weighted by p_b^2 / (p_b^2 + p_l^2) with
+// p_l computed for that hit. Emission reached from the camera or from a
+// metal or dielectric bounce keeps weight 1. A Disney hit samples a light
+// also when its continuation took the glass lobe (the raygen then skips the
+// bounce's emission and multiply, Q4): the light sample is independent of
+// the lobe the continuation drew.
 
 // Power heuristic, beta = 2 (shaders/raytrace/pdf.h.glsl:4-7).
 RT_D float power_heuristic(float a, float b) { return (a * a) / (a * a + b * b); }
@@ -92,7 +97,9 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
     const bool nee = NEE && P.nee != 0u;
     float4 s4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (nee) s4 = S.st4[pp];
-    bool lambert_hit = false, emissive_mis = false;
+    bool lambert_hit = false, disney_hit = false, emissive_mis = false;
+    DisneyNee dn;
+    const ReinaInstanceProperties* hp = nullptr;
     V3 nee_n = r.no, nee_albedo = r.no;
     float emis_w = 1.0f;
     uint4 s0 = S.st0[pp];
@@ -123,10 +130,12 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
             case 0: chs_lambert_metal(pld, sv, hi, pr, D, false); break;
             case 1: chs_lambert_metal(pld, sv, hi, pr, D, true); break;
             case 2: chs_dielectric(pld, sv, hi, pr, O, D); break;
-            default: chs_disney(pld, sv, hi, pr, O, D, P.fixes); break;
+            default: chs_disney(pld, sv, hi, pr, O, D, P.fixes, nee ? &dn : nullptr); break;
         }
         if (nee && !pld.skip) {
             lambert_hit = in.material == 0u;
+            disney_hit = in.material == 3u;
+            hp = &pr;
             nee_n = pld.surfaceNormal;     // the shading normal (normal-mapped) and
             nee_albedo = pld.albedo;       // textured colour of the Lambertian BSDF
             // an emitter reached by the continuation of a Lambertian bounce: MIS weight
@@ -150,10 +159,13 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
         if (pld.rayHitSky) {
             L = add(L, mul(pld.color, thr));
             end = true;
-        } else if (!pld.insideDielectric) {
-            V3 contrib = add(v3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? scale(pld.emission, emis_w) : pld.emission);
-            L = add(L, mul(contrib, thr));
-            if (nee && lambert_hit && seg + 1u < P.maxBounces && sv.num_emi > 0u) {
+        } else {
+            const bool inside = pld.insideDielectric;
+            if (!inside) {
+                V3 contrib = add(v3(0.0f, 0.0f, 0.0f), (nee && emissive_mis) ? scale(pld.emission, emis_w) : pld.emission);
+                L = add(L, mul(contrib, thr));
+            }
+            if (nee && (lambert_hit || disney_hit) && seg + 1u < P.maxBounces && sv.num_emi > 0u) {
                 // light sample (4 draws after the BSDF's), connected by the shadow kernel
                 const float u1 = rnd(pld.rng), u2 = rnd(pld.rng), u3 = rnd(pld.rng), u4 = rnd(pld.rng);
                 const uint32_t k = cdf_pick(0u, sv.num_emi - 1u, u1, [&](uint32_t i) { return sv.emi[i].cdf; });
@@ -169,29 +181,49 @@ RT_D ShadeResult shade_ray(const SceneView& sv, const TraceParams& P, const Path
                 const float alpha = (1.0f - beta) - gamma;
                 const V3 y = add(add(scale(w0, alpha), scale(w1, beta)), scale(w2, gamma));
                 const V3 ng = normalize(cross(sub(w1, w0), sub(w2, w0)));
-                const V3 dv = sub(y, pld.rayOrigin);
-                const float dist = length(dv);
-                const V3 w = normalize(dv);
-                const float pl = light_pdf(ei.pdf_area, dist, ng, w, lp.cullBackface != 0u);
-                const float cx = maxf_(dot(nee_n, w), 0.0f);
-                if (pl > 0.0f && cx > 0.0f && dist > 0.001f) {
-                    const float pb = cx / kPi;
-                    const float wn = power_heuristic(pl, pb);
-                    const V3 f = divs(nee_albedo, kPi);
-                    r.scontrib = mul(scale(mul(ldv3(lp.emission), f), (cx / pl) * wn), thr);
-                    r.shadow = true;
-                    r.so = pld.rayOrigin;
-                    r.sd = w;
-                    r.stmax = dist - 0.001f;
+                if (lambert_hit) {
+                    const V3 dv = sub(y, pld.rayOrigin);
+                    const float dist = length(dv);
+                    const V3 w = normalize(dv);
+                    const float pl = light_pdf(ei.pdf_area, dist, ng, w, lp.cullBackface != 0u);
+                    const float cx = maxf_(dot(nee_n, w), 0.0f);
+                    if (pl > 0.0f && cx > 0.0f && dist > 0.001f) {
+                        const float pb = cx / kPi;
+                        const float wn = power_heuristic(pl, pb);
+                        const V3 f = divs(nee_albedo, kPi);
+                        r.scontrib = mul(scale(mul(ldv3(lp.emission), f), (cx / pl) * wn), thr);
+                        r.shadow = true;
+                        r.so = pld.rayOrigin;
+                        r.sd = w;
+                        r.stmax = dist - 0.001f;
+                    }
+                    s4 = make_float4(pld.rayOrigin.x, pld.rayOrigin.y, pld.rayOrigin.z,
+                                     maxf_(dot(nee_n, pld.rayDirection), 0.0f) / kPi);
+                } else {
+                    // the shadow ray leaves on the light's side of the surface
+                    const V3 so = offset_dielectric(dn.pos, dn.ng, normalize(sub(y, dn.pos)));
+                    const V3 dv = sub(y, so);
+                    const float dist = length(dv);
+                    const V3 w = normalize(dv);
+                    const float pl = light_pdf(ei.pdf_area, dist, ng, w, lp.cullBackface != 0u);
+                    if (pl > 0.0f && dist > 0.001f) {
+                        float pt;
+                        const V3 g = disney_nee_weight(dn, *hp, P.fixes, w, pt);
+                        if (g.x > 0.0f || g.y > 0.0f || g.z > 0.0f) {
+                            const float wn = power_heuristic(pl, pt);
+                            r.scontrib = mul(scale(mul(ldv3(lp.emission), g), wn / pl), thr);
+                            r.shadow = true;
+                            r.so = so;
+                            r.sd = w;
+                            r.stmax = dist - 0.001f;
+                        }
+                    }
+                    s4 = make_float4(pld.rayOrigin.x, pld.rayOrigin.y, pld.rayOrigin.z, dn.p_bsdf);
                 }
-                s4 = make_float4(pld.rayOrigin.x, pld.rayOrigin.y, pld.rayOrigin.z,
-                                 maxf_(dot(nee_n, pld.rayDirection), 0.0f) / kPi);
             } else if (nee) {
                 s4.w = 0.0f;
             }
-            thr = mul(thr, pld.color);
-        } else if (nee) {
-            s4.w = 0.0f;
+            if (!inside) thr = mul(thr, pld.color);
         }
     }
     seg += 1u;

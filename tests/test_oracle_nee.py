@@ -190,3 +190,38 @@ def test_direct_light_closed_form(nee):
     assert abs(got.sum() - exp.sum()) < 4.0 * err_tot + 1e-6 * exp.sum(), (got.sum(), exp.sum(), err_tot)
     z = (got - exp) / np.sqrt(np.maximum(var, 1e-12))
     assert (z * z).mean() < 1.8
+
+
+def disney_nee_zscores(fixes, nb_batches=192):
+    sc, cam = scenes.disney_lobes()
+    arrays = sc.build_arrays()
+    W = H = 10
+    orc = oracle.OracleScene(arrays, make_desc)
+    orc.set_fixes(fixes)
+    pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 16, 4, direct_clamp=1e9)
+    orc.set_nee(False)
+    sa, qa, ca = orc.render_sum64(pc, W, H, 1, 256)
+    orc.set_nee(True)
+    sb, qb, cb = orc.render_sum64(pc, W, H, 5000, nb_batches)
+    na, nb = 256.0, float(nb_batches)
+    ma, mb = sa / na, sb / nb
+    va = np.maximum(qa / na - ma * ma, 1e-12) / na
+    vb = np.maximum(qb / nb - mb * mb, 1e-12) / nb
+    return (mb - ma) / np.sqrt(va + vb), ma, mb, va, vb, na, nb
+
+
+@pytest.mark.parametrize("fixes", [0, abi.TRACE_FIX_DISNEY_MIX | abi.TRACE_FIX_METAL_PDF], ids=["reference", "fixed"])
+def test_nee_disney_equals_bsdf_only_in_expectation(fixes):
+    """Disney hits (eight spheres, one per lobe and mixes, in the Cornell box
+    under its ceiling light): NEE + MIS and BSDF-only sampling estimate the
+    same value per pixel, with the reference's evalDisney mixture and with
+    the fixed one (z-scores over batch means; depth 4, no clamping). The
+    light sample's weight is the hit shader's own expected continuation
+    weight, so NEE changes only the variance, including on glass, whose
+    continuation the raygen carries with weight 1 (Q4)."""
+    z, ma, mb, va, vb, na, nb = disney_nee_zscores(fixes)
+    assert np.abs(z).max() < 5.0, np.abs(z).max()
+    assert (z * z).mean() < 1.6, (z * z).mean()
+    assert np.median(vb * nb) < 0.8 * np.median(va * na)
+    assert abs(mb.mean() - ma.mean()) < 3.0 * np.sqrt(va.sum() + vb.sum()) / ma.size
+
