@@ -20,13 +20,16 @@ from . import DeviceScene, ReinaError, Server, abi, scenes
 from .scene import push_consts
 
 
-def make_pool(sizes, seed=0, quantized=False, threads=8):
+def make_pool(sizes, seed=0, quantized=False, threads=8, device_build=False):
     """Resident scenes of the given triangle counts (pool_scene), created
-    concurrently. Returns a list of dicts {dev, cam, tris}."""
+    concurrently; device_build: BVH8 built on the GPU (REINA_RT_BUILD_DEVICE).
+    Returns a list of dicts {dev, cam, tris}."""
+    flags = (abi.NODES_QUANTIZED if quantized else 0) | (abi.BUILD_DEVICE if device_build else 0)
+
     def one(k):
         sc, cam = scenes.pool_scene(int(sizes[k]), seed * 1000 + k)
         arrays = sc.build_arrays()
-        dev = DeviceScene(arrays, flags=abi.NODES_QUANTIZED if quantized else 0)
+        dev = DeviceScene(arrays, flags=flags)
         return {"dev": dev, "cam": cam, "tris": int(dev.info().num_triangles), "depth": int(dev.info().max_depth)}
     with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
         return list(ex.map(one, range(len(sizes))))
@@ -148,13 +151,15 @@ def served_plain(torch, pool, reqs, times, max_pending):
             "mrays_per_s": round(rays / span / 1e6, 1) if span else 0.0, "latency": _pcts(lat)}
 
 
-def served_server(torch, pool, reqs, times, max_pending, server_kw):
+def served_server(torch, pool, reqs, times, max_pending, server_kw, keep_images=False):
     """Arrivals submitted at their times to a server with a bounded FIFO of
-    max_pending (BUSY = rejected); latency = completion - arrival."""
+    max_pending (BUSY = rejected); latency = completion - arrival.
+    keep_images: the result also holds {request index: image} of every
+    completed request under "images"."""
     srv = Server(max_pending=max_pending, **server_kw)
     st0 = srv.stats()
     t0 = time.perf_counter()
-    live, lat, done_t, rejected = {}, [], [], 0
+    live, lat, done_t, rejected, images = {}, [], [], 0, {}
 
     def reap(block):
         for tk in list(live):
@@ -165,6 +170,8 @@ def served_server(torch, pool, reqs, times, max_pending, server_kw):
                 if st.state != abi.REQ_DONE:
                     continue
             k, _img = live.pop(tk)
+            if keep_images:
+                images[k] = _img
             # server clock -> bench clock: both measure from their own start; use the request's
             # time in the server (done - submit) plus the submit delay after its arrival
             lat.append(st.done_s - st.submit_s + sub_delay[k])
@@ -191,9 +198,12 @@ def served_server(torch, pool, reqs, times, max_pending, server_kw):
     srv.close()
     span = (max(done_t) - times[0]) if done_t else 0.0
     rays = st1.extension_rays - st0.extension_rays
-    return {"completed": len(lat), "rejected": rejected,
-            "images_per_s": round(len(lat) / span, 3) if span else 0.0,
-            "mrays_per_s": round(rays / span / 1e6, 1) if span else 0.0, "latency": _pcts(lat)}
+    out = {"completed": len(lat), "rejected": rejected,
+           "images_per_s": round(len(lat) / span, 3) if span else 0.0,
+           "mrays_per_s": round(rays / span / 1e6, 1) if span else 0.0, "latency": _pcts(lat)}
+    if keep_images:
+        out["images"] = images
+    return out
 
 
 def served(torch, pool, reqs, rate, duration, max_pending, server_kw, seed=0):
