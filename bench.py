@@ -27,6 +27,20 @@ BVH8 nodes (80 bytes per node, all useful), with a check that the frame is
 bit-identical to the FP32-node frame (the closest hit does not depend on the
 node format).
 
+instanced_8m: 4096 instances of a 2048-triangle sphere flattened to
+8,388,608 triangles, 1920x1080, 32 spp, depth 12: frames over FP32 and 8-bit
+nodes (both BVHs built on the device, bit-identical frames), a host-built
+FP32 tree beside them, and the closest hits of a sampled ray set against the
+FP64 oracle.
+
+batched: 64 scenes of 1e3-2e6 triangles (log-uniform), 64 requests of
+uneven resolution and spp, all submitted to one server, against the same
+requests through reina_rt_trace one after another (images bit-identical).
+
+served: Poisson arrivals at 50-400 requests/s (and 4/s, below capacity) over
+32 resident scenes of 1e4-4e6 triangles, bounded FIFO: p50 / p99 latency and
+images/s of the server against plain per-request launches.
+
 cpu_baseline: the oracle's C restatement of the reference shaders (BVH
 traversal over the same BVH8), multithreaded over pixels, on a bounded
 sample of the same frame (whole 1-spp passes for about --cpu-seconds).
@@ -73,19 +87,20 @@ def main():
                          "waves_per_simd,tail_paths (the config.tuning of a default run)")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
     ap.add_argument("--no-device-build", action="store_true", help="skip the device-built BVH comparison")
-    ap.add_argument("--instanced-variants", type=str, default="fp32,quantized,fp32_device_build",
-                    help="node formats / builds of the --instanced block, comma-separated")
-    ap.add_argument("--instanced", action="store_true",
-                    help="add the 8.4M-triangle instanced scene: FP32 vs quantized nodes, error vs the FP64 oracle")
-    ap.add_argument("--fp64-rays", type=int, default=1 << 20, help="rays checked against the FP64 oracle")
+    ap.add_argument("--instanced-variants", type=str, default="fp32,quantized,fp32_host_build",
+                    help="node formats / builds of the instanced block, comma-separated (fp32 and quantized "
+                         "are built on the device, fp32_host_build on the host)")
+    ap.add_argument("--no-instanced", action="store_true",
+                    help="skip the 8.4M-triangle instanced scene (FP32 vs quantized nodes, error vs the FP64 oracle)")
+    ap.add_argument("--fp64-rays", type=int, default=1 << 18, help="rays checked against the FP64 oracle")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched block")
     ap.add_argument("--batched-groups", type=int, default=2,
                     help="log2 of the sample streams per pixel of the batched block's second pass (0: none)")
     ap.add_argument("--no-served", action="store_true", help="skip the served block")
-    ap.add_argument("--batched-full", action="store_true",
-                    help="batched block at full size: 64 scenes of 1e3-2e6 triangles, 64 requests")
-    ap.add_argument("--served-full", action="store_true",
-                    help="served block at full size: 32 scenes of 1e4-4e6 triangles, 50-400 requests/s")
+    ap.add_argument("--small-pools", action="store_true",
+                    help="batched / served blocks on small pools (16 scenes of 1e3-2e5 triangles, 32 requests; "
+                         "8 scenes of 1e4-4e5 triangles) instead of the full configurations (64 scenes of 1e3-2e6 "
+                         "triangles, 64 requests; 32 scenes of 1e4-4e6 triangles at 50-400 requests/s)")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
@@ -221,7 +236,7 @@ def main():
         log("served requests")
         out["served"] = served_block(a, torch)
 
-    if a.instanced:
+    if not a.no_instanced:
         log("instanced 8.4M-triangle scene")
         out["instanced_8m"] = instanced_block(a, torch)
 
@@ -247,8 +262,8 @@ def main():
 def profiled_traffic(name, tuning):
     """(traffic_bytes_per_launch, tuning it was measured at) from a committed
     rocprofv3 summary: the line for this tuning (stack_depth,refill_min,
-    grid_blocks,shade_blocks,waves_per_simd,tail_paths) if the summary has
-    one, else its first; (None, None) without a summary."""
+    grid_blocks,shade_blocks,waves_per_simd,tail_paths); (None, None) when
+    the summary has no line measured at this tuning."""
     key = ",".join(str(tuning[k]) for k in ("stack_depth", "refill_min", "grid_blocks", "shade_blocks",
                                              "waves_per_simd", "tail_paths"))
     found = []
@@ -262,7 +277,7 @@ def profiled_traffic(name, tuning):
     for t, v in found:
         if t == key:
             return v, t
-    return (found[0][1], found[0][0]) if found else (None, None)
+    return None, None
 
 def run_autotune(dev, sec, sh, torch, frame=None):
     """Stage 1 on the secondary batch; stage 2 (when `frame` is given) on
@@ -464,9 +479,9 @@ def batched_block(a, torch):
 
     from reina_rt import loadgen
     rng = np.random.default_rng(64)
-    n_sc, lo, hi, n_req = (64, 1e3, 2e6, 64) if a.batched_full else (16, 1e3, 2e5, 32)
+    n_sc, lo, hi, n_req = (16, 1e3, 2e5, 32) if a.small_pools else (64, 1e3, 2e6, 64)
     t0 = time.perf_counter()
-    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=4)
+    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=4, device_build=True)
     setup = time.perf_counter() - t0
     log("batched: %d scenes built in %.1f s" % (n_sc, setup))
     reqs = loadgen.make_requests(pool, n_req, rng, [(128, 128), (256, 256), (512, 512), (1024, 1024)],
@@ -503,12 +518,12 @@ def served_block(a, torch):
 
     from reina_rt import loadgen
     rng = np.random.default_rng(5)
-    if a.served_full:
-        n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 10.0), (100.0, 10.0), (200.0, 10.0), (400.0, 10.0), (4.0, 15.0)]
-    else:
+    if a.small_pools:
         n_sc, lo, hi, runs = 8, 1e4, 4e5, [(50.0, 4.0), (4.0, 6.0)]
+    else:
+        n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 5.0), (100.0, 5.0), (200.0, 5.0), (400.0, 5.0), (4.0, 10.0)]
     t0 = time.perf_counter()
-    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=5)
+    pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=5, device_build=True)
     setup = time.perf_counter() - t0
     log("served: %d scenes built in %.1f s" % (n_sc, setup))
     out = {"scenes": n_sc, "triangles_total": int(sum(p["tris"] for p in pool)),
@@ -546,7 +561,8 @@ def instanced_block(a, torch):
     res = {"workload": "4096 instances x 2048-tri sphere = 8,388,608 tris, %dx%d, %d spp, depth %d" % (W, H, spp, depth)}
     devs = {}
     wanted = set(a.instanced_variants.split(",")) | {"fp32"}   # the ray set and FP64 check use fp32
-    for name, flags in (("fp32", 0), ("quantized", abi.NODES_QUANTIZED), ("fp32_device_build", abi.BUILD_DEVICE)):
+    for name, flags in (("fp32", abi.BUILD_DEVICE), ("quantized", abi.BUILD_DEVICE | abi.NODES_QUANTIZED),
+                        ("fp32_host_build", 0)):
         if name not in wanted:
             continue
         t0 = time.perf_counter()
