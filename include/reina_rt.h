@@ -359,6 +359,16 @@ int reina_rt_set_tuning(ReinaScene* scene, const ReinaTuning* tuning);
 int reina_rt_intersect(ReinaScene* scene, const ReinaRay* rays, ReinaHit* hits,
                        uint32_t n, uint32_t* counters, void* stream);
 
+/* reina_rt_intersect without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture and tight serving loops. The traversal kernel's
+ * claim counters (1 KiB) live in a static device array of the library, one
+ * per GPU, held until the process ends, and are zeroed by a fill kernel on
+ * `stream` at every call. At most 2^30 rays per call (REINA_RT_ERR_INVALID_ARG
+ * beyond). Calls that overlap on one GPU (different streams or threads) race
+ * on the shared counters; serialise them. Hits equal reina_rt_intersect's. */
+int reina_rt_intersect_noalloc(ReinaScene* scene, const ReinaRay* rays, ReinaHit* hits, uint32_t n, void* stream);
+
 /* Bloom (blurX with threshold, blurY, combine) then ACES-fitted tone map to
  * RGBA8 UNORM: shaders/postprocessing/bloom/{blurX,blurY,combine}.comp.glsl, tonemap/tonemapping.comp.glsl,
  * dispatch order src/Reina.cpp:472-577. in_rgba32f, scratch (2 * W*H float4)

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -666,6 +667,71 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
     TraceInput tin{nullptr, 0, n, s->fetch};
     HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters, counters != nullptr,
                          s->tune, st));
+    return REINA_RT_OK;
+}
+
+}  // extern "C"
+
+// ---- reina_rt_intersect_noalloc -----------------------------------------
+// The claim counters of the persistent traversal kernel (one 128-byte line
+// per XCD partition) live in this file-scope device array instead of the
+// scene's library-allocated control block. Each device holds its own copy;
+// its address on a device comes from hipGetSymbolAddress, taken once per
+// device with that device current. Host code never passes `scratch` itself
+// to a kernel: in host code the name is a stand-in, not a device address.
+namespace {
+constexpr uint32_t kNoAllocScratchWords = kParts * kLine;   // 1 KiB
+constexpr uint32_t kNoAllocMaxRays = 1u << 30;   // keeps every claim counter far below 2^32
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
+// The device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, uint32_t** out) {
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal " + std::to_string(device) + " out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(REINA_RT_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = (uint32_t*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return REINA_RT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int reina_rt_intersect_noalloc(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint32_t n, void* stream) {
+    if (!s || (n && (!rays || !hits))) return fail(REINA_RT_ERR_INVALID_ARG, "null argument");
+    if (n > kNoAllocMaxRays)
+        return fail(REINA_RT_ERR_INVALID_ARG, "reina_rt_intersect_noalloc takes at most " +
+                                                  std::to_string(kNoAllocMaxRays) + " rays per call, got " +
+                                                  std::to_string(n));
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(s->device));
+    uint32_t* fetch = nullptr;
+    int rc = scratch_address(s->device, &fetch);
+    if (rc) return rc;
+    if (n == 0) return REINA_RT_OK;
+    // the claim counters start at zero: a fill kernel, not hipMemsetAsync, so
+    // the call can be captured in a HIP graph
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, fetch, kNoAllocScratchWords);
+    HIP_TRY(hipGetLastError());
+    TraceInput tin{nullptr, 0, n, fetch};
+    HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, nullptr, false, s->tune, st));
     return REINA_RT_OK;
 }
 

@@ -47,6 +47,7 @@ def lib():
         "reina_rt_get_tuning": [c_p, ctypes.POINTER(abi.ReinaTuning)],
         "reina_rt_set_tuning": [c_p, ctypes.POINTER(abi.ReinaTuning)],
         "reina_rt_intersect": [c_p, c_p, c_p, u32, c_p, c_p],
+        "reina_rt_intersect_noalloc": [c_p, c_p, c_p, u32, c_p],
         "reina_rt_postprocess": [c_p, c_p, c_p, u32, u32, c_p, c_p, c_p],
         "reina_rt_debug_math": [i32, c_p, c_p, c_p, u32, c_p],
         "reina_rt_bvh_build_host": [c_p, u64, c_p, ctypes.POINTER(u64), c_p, ctypes.POINTER(u64),
@@ -97,7 +98,7 @@ def exported_symbols():
             "reina_rt_poll", "reina_rt_wait", "reina_rt_server_stats", "reina_rt_scene_export_emissive",
             "reina_rt_model_load_obj", "reina_rt_model_load_obj_memory", "reina_rt_model_free",
             "reina_rt_image_load", "reina_rt_image_load_memory", "reina_rt_image_free",
-            "reina_rt_gltf_load", "reina_rt_gltf_load_memory", "reina_rt_gltf_free"]
+            "reina_rt_gltf_load", "reina_rt_gltf_load_memory", "reina_rt_gltf_free", "reina_rt_intersect_noalloc"]
 
 
 class DeviceScene:
