@@ -380,7 +380,12 @@ def roofline(dev, frame, node_bytes, torch):
     assert stc.extension_rays == stt.extension_rays and stc.tail_rays == stt.tail_rays
     nrays = stt.extension_rays - stt.tail_rays   # rays traced by k_trace launches (the tail kernel is not counted)
     alg_bytes = nrays * RAY_BYTES + stc.node_visits * node_bytes + stc.tri_tests * TRI_BYTES
-    launches = max(stt.extend_launches, 1)
+    if stt.extend_launches == 0 or stt.extend_ms <= 0.0:
+        return {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": "k_trace (BVH8 traversal, extend)", "launches": 0,
+                "tail_rays": int(stt.tail_rays), "tail_ms": round(float(stt.tail_ms), 3),
+                "note": "every ray of the frame was traced by the tail kernel"}
+    launches = stt.extend_launches
     avg_launch_ms = stt.extend_ms / launches
     achieved = (alg_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
     return {
