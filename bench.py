@@ -86,7 +86,8 @@ def main():
                     help="apply this tuning instead of autotuning: stack_depth,refill_min,grid_blocks,shade_blocks,"
                          "waves_per_simd,tail_paths (the config.tuning of a default run)")
     ap.add_argument("--no-quantized", action="store_true", help="skip the 8-bit quantized-node comparison")
-    ap.add_argument("--no-device-build", action="store_true", help="skip the device-built BVH comparison")
+    ap.add_argument("--no-device-build", action="store_true",
+                    help="skip the host-built BVH beside the device-built one (build time, SAH, frame)")
     ap.add_argument("--instanced-variants", type=str, default="fp32,quantized,fp32_host_build",
                     help="node formats / builds of the instanced block, comma-separated (fp32 and quantized "
                          "are built on the device, fp32_host_build on the host)")
@@ -101,6 +102,8 @@ def main():
                     help="batched / served blocks on small pools (16 scenes of 1e3-2e5 triangles, 32 requests; "
                          "8 scenes of 1e4-4e5 triangles) instead of the full configurations (64 scenes of 1e3-2e6 "
                          "triangles, 64 requests; 32 scenes of 1e4-4e6 triangles at 50-400 requests/s)")
+    ap.add_argument("--admit-paths", type=int, default=1 << 21,
+                    help="served block: the server admits while fewer paths than this are in flight")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
@@ -226,8 +229,8 @@ def main():
         }
 
     if not a.no_device_build:
-        log("device-built BVH")
-        out["device_build"] = device_build_block(a, arrays, dev, frame, img, torch)
+        log("host-built BVH beside the device-built one")
+        out["bvh_build"] = bvh_build_block(a, arrays, dev, frame, img, torch)
 
     if not a.no_batched:
         log("batched requests")
@@ -411,30 +414,30 @@ def roofline(dev, frame, node_bytes, torch):
 
 
 
-def device_build_block(a, arrays, dev, frame, img, torch):
-    """The same frame over a BVH8 built on the GPU (REINA_RT_BUILD_DEVICE):
-    build time, SAH cost and depth against the host build, frame rate, and a
-    check that the frame is bit-identical (the closest hit does not depend on
-    the tree)."""
+def bvh_build_block(a, arrays, dev, frame, img, torch):
+    """The headline scene's BVH8 is built on the GPU (the library's default);
+    this builds it again on the host (REINA_RT_BUILD_HOST): build times, SAH
+    cost and depth of both, the frame rate over the host-built tree (its own
+    default tuning), and a check that the frame is bit-identical (the closest
+    hit does not depend on the tree)."""
     import reina_rt
     from reina_rt import abi
     t0 = time.perf_counter()
-    dd = reina_rt.DeviceScene(arrays, flags=abi.BUILD_DEVICE)
+    dh = reina_rt.DeviceScene(arrays, flags=abi.BUILD_HOST)
     setup = time.perf_counter() - t0
-    hi, di = dev.info(), dd.info()
-    # the device-built tree keeps its own default tuning (its depth, and so
-    # its stack depth, can differ from the host-built tree's)
+    hi, di = dh.info(), dev.info()
     frame(0, dev)
     ref = img.clone()
-    frame(0, dd)
+    frame(0, dh)
     same = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
-    rays, elapsed, _ = time_frames(lambda b, d=dd: frame(b, d), a.warmup, a.steps, torch)
-    res = {"scene_setup_s": round(setup, 3), "build_ms": round(float(di.build_ms), 1),
-           "host_build_ms": round(float(hi.build_ms), 1), "sah_cost": round(float(di.sah_cost), 3),
-           "host_sah_cost": round(float(hi.sah_cost), 3), "bvh_depth": int(di.max_depth),
-           "host_bvh_depth": int(hi.max_depth), "bvh_nodes": int(di.num_nodes),
-           "mrays_per_s": round(rays / elapsed / 1e6, 2), "frame_bit_identical_to_host_build": same}
-    dd.close()
+    rays, elapsed, _ = time_frames(lambda b, d=dh: frame(b, d), a.warmup, a.steps, torch)
+    res = {"device_build_ms": round(float(di.build_ms), 1), "host_build_ms": round(float(hi.build_ms), 1),
+           "host_scene_setup_s": round(setup, 3), "device_sah_cost": round(float(di.sah_cost), 3),
+           "host_sah_cost": round(float(hi.sah_cost), 3), "device_bvh_depth": int(di.max_depth),
+           "host_bvh_depth": int(hi.max_depth), "device_bvh_nodes": int(di.num_nodes),
+           "host_bvh_nodes": int(hi.num_nodes), "host_tree_mrays_per_s": round(rays / elapsed / 1e6, 2),
+           "frame_bit_identical_to_host_build": same}
+    dh.close()
     return res
 
 def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
@@ -534,13 +537,20 @@ def served_block(a, torch):
     out = {"scenes": n_sc, "triangles_total": int(sum(p["tris"] for p in pool)),
            "triangles_min": min(p["tris"] for p in pool), "triangles_max": max(p["tris"] for p in pool),
            "scene_setup_s": round(setup, 1), "runs": []}
+    out["admit_paths"] = a.admit_paths
     for rate, dur in runs:
         reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng,
                                      [(256, 256), (512, 512), (1280, 720), (1920, 1080)], spp_range=(16, 128))
-        r = loadgen.served(torch, pool, reqs, rate, dur, 16, dict(path_budget_bytes=SERVER_BUDGET),
-                           seed=int(rate))
-        log("served: rate %.0f/s plain %.2f img/s served %.2f img/s" % (rate, r["plain"]["images_per_s"],
-                                                                      r["served"]["images_per_s"]))
+        r = loadgen.served(torch, pool, reqs, rate, dur, 16,
+                           dict(path_budget_bytes=SERVER_BUDGET, admit_paths=a.admit_paths), seed=int(rate))
+        log("served: rate %.0f/s plain %.2f img/s served %.2f img/s, p99 %.0f / %.0f ms" % (
+            rate, r["plain"]["images_per_s"], r["served"]["images_per_s"],
+            r["plain"]["latency"].get("p99_ms", 0), r["served"]["latency"].get("p99_ms", 0)))
+        if rate == 100.0:
+            # the same arrivals with no admission target: every request that fits the path pool runs at once
+            times = loadgen.arrivals(rate, dur, np.random.default_rng(int(rate)))
+            r["served_admit_all"] = loadgen.served_server(torch, pool, reqs[:len(times)], times, 16,
+                                                          dict(path_budget_bytes=SERVER_BUDGET))
         out["runs"].append(r)
     for p in pool:
         p["dev"].close()
@@ -567,7 +577,7 @@ def instanced_block(a, torch):
     devs = {}
     wanted = set(a.instanced_variants.split(",")) | {"fp32"}   # the ray set and FP64 check use fp32
     for name, flags in (("fp32", abi.BUILD_DEVICE), ("quantized", abi.BUILD_DEVICE | abi.NODES_QUANTIZED),
-                        ("fp32_host_build", 0)):
+                        ("fp32_host_build", abi.BUILD_HOST)):
         if name not in wanted:
             continue
         t0 = time.perf_counter()
@@ -824,6 +834,41 @@ def secondary_rays(a, dev, sec, sh, torch, node_bytes):
     alg = n2 * RAY_BYTES + visits * node_bytes + tests * TRI_BYTES
     gbs = alg / (ms * 1e-3) / 1e9
     hit_frac = float((sh[:, 1].contiguous().view(torch.int32) != -1).float().mean().item())
+    f16 = None
+    if node_bytes == NODE_BYTES:
+        # the same batch with FP16 inverse directions (REINA_RT_TRACE_F16_INV): conservative
+        # slabs, so the hits must not change; only node visits and time can
+        from reina_rt import abi
+        tun = dev.get_tuning()
+        dev.set_tuning(**dict(tun, waves_per_simd=0))   # the FP16 variant is built with the compiler's budget
+
+        def timed(reps):
+            out_ms = []
+            for _ in range(reps):
+                e0.record(stream)
+                dev.intersect(sec.data_ptr(), sh16.data_ptr(), n2, stream=sp)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                out_ms.append(e0.elapsed_time(e1))
+            return float(np.median(out_ms))
+        sh16 = torch.zeros_like(sh)
+        ms32 = timed(reps_t)
+        dev.set_trace_options(abi.TRACE_F16_INV)
+        cnt16 = torch.zeros_like(cnt)
+        dev.intersect(sec.data_ptr(), sh16.data_ptr(), n2, counters_ptr=cnt16.data_ptr(), stream=sp)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(sh16.view(torch.int32), sh.view(torch.int32)))
+        v16 = int(cnt16[:, 0].double().sum().item())
+        t16 = int(cnt16[:, 1].double().sum().item())
+        ms16 = timed(reps_t)
+        dev.set_trace_options(0)
+        dev.set_tuning(**tun)
+        f16 = {"kernel_ms": round(ms16, 3), "grays_per_s": round(n2 / (ms16 * 1e-3) / 1e9, 3),
+               "fp32_inverses_same_budget_ms": round(ms32, 3),
+               "nodes_per_ray": round(v16 / n2, 3), "tris_per_ray": round(t16 / n2, 3),
+               "hits_bit_identical_to_fp32_inverses": same,
+               "note": "ray-direction inverses rounded to FP16, slab intervals widened by 2^-10 |t| "
+                       "(compiler-chosen register budget)"}
     return {
         "rays": int(n2),
         "grays_per_s": round(n2 / (ms * 1e-3) / 1e9, 3),
@@ -836,6 +881,7 @@ def secondary_rays(a, dev, sec, sh, torch, node_bytes):
         "hit_fraction": round(hit_frac, 3),
         "note": "cosine-distributed bounce rays from the primary hits of the same 1080p frame, "
                 "traced by reina_rt_intersect (same traversal kernel as the frame's extend)",
+        "f16_inverses": f16,
     }
 
 

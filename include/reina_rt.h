@@ -138,8 +138,8 @@ typedef struct ReinaInstanceDesc {
 } ReinaInstanceDesc;
 
 /* BVH / build options */
-#define REINA_RT_BUILD_HOST        0x0u   /* binned-SAH build on the host CPU      */
-#define REINA_RT_BUILD_DEVICE      0x1u   /* binned-SAH build on the GPU (Morton order) */
+#define REINA_RT_BUILD_DEVICE      0x1u   /* binned-SAH build on the GPU: the default (flags 0 too) */
+#define REINA_RT_BUILD_HOST        0x4u   /* binned-SAH build on the host CPU (threads) */
 #define REINA_RT_NODES_QUANTIZED   0x2u   /* 8-bit child boxes (80-byte nodes)      */
 
 /* The packed buffers exactly as Models::addModel produces them
@@ -334,6 +334,12 @@ typedef struct ReinaTraceStats {
  * counts. G = 1 is the reference. Server requests take G from their scene. */
 #define REINA_RT_TRACE_SAMPLE_GROUPS(log2g) ((uint32_t)(log2g) << 8)
 #define REINA_RT_TRACE_SAMPLE_GROUPS_MASK   0xF00u
+/* reina_rt_intersect on FP32 nodes only (REINA_RT_ERR_UNSUPPORTED on 8-bit
+ * nodes; reina_rt_trace ignores it): ray-direction inverses rounded to FP16,
+ * slab intervals widened by 2^-10 of |t| so the box test stays conservative.
+ * Hits are identical to the FP32 inverses'; only node visits differ. A
+ * precision experiment (no counterpart in the reference). */
+#define REINA_RT_TRACE_F16_INV              0x1000u
 int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 
@@ -389,7 +395,9 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  * sized by path_budget_bytes (REINA_RT_SERVER_BYTES_PER_PIXEL per pixel in
  * flight). Submitted requests wait in a bounded FIFO; the worker admits them
  * (in order) as soon as enough contiguous slots and a request index are
- * free, generates their camera rays into the shared queues, and runs bounce
+ * free (and, with admit_paths, while fewer paths than that are in flight:
+ * under overload this keeps fewer requests sharing the GPU, so each
+ * finishes sooner), generates their camera rays into the shared queues, and runs bounce
  * iterations over every admitted request at once (captured in a HIP graph of
  * 6 iterations). When at most tuning.tail_paths paths are live after a graph,
  * one tail launch (per-wave extend/shade steps, as reina_rt_trace's) finishes
@@ -419,8 +427,14 @@ typedef struct ReinaServerDesc {
                                     (stack_depth bounds the BVH depth of served scenes;
                                     tail_paths 0 = 2^19, REINA_RT_TAIL_OFF = never)      */
     uint32_t flags;              /* REINA_RT_SERVER_NEE or 0                             */
-    uint32_t _pad0;
+    uint32_t admit_paths;        /* admission stops while this many paths are in flight
+                                    (live paths of the last graph + paths admitted since);
+                                    a request is always admitted when nothing runs.
+                                    0 or REINA_RT_ADMIT_ALL: admit whenever slots are free */
 } ReinaServerDesc;
+
+/* ReinaServerDesc.admit_paths: no limit but the path pool. */
+#define REINA_RT_ADMIT_ALL 0xFFFFFFFFu
 
 /* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */
 #define REINA_RT_TAIL_OFF 0xFFFFFFFFu

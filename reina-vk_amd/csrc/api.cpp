@@ -51,8 +51,10 @@ static int validate(const ReinaSceneDesc* d) {
     if (d->struct_size != sizeof(ReinaSceneDesc))
         return fail(REINA_RT_ERR_ABI, "ReinaSceneDesc.struct_size mismatch (expected " +
                                           std::to_string(sizeof(ReinaSceneDesc)) + ")");
-    if (d->flags & ~(REINA_RT_BUILD_DEVICE | REINA_RT_NODES_QUANTIZED))
+    if (d->flags & ~(REINA_RT_BUILD_DEVICE | REINA_RT_BUILD_HOST | REINA_RT_NODES_QUANTIZED))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown ReinaSceneDesc.flags bits");
+    if ((d->flags & REINA_RT_BUILD_DEVICE) && (d->flags & REINA_RT_BUILD_HOST))
+        return fail(REINA_RT_ERR_INVALID_ARG, "REINA_RT_BUILD_DEVICE and REINA_RT_BUILD_HOST are exclusive");
     if (d->num_instances && (!d->instances || !d->models || !d->props))
         return fail(REINA_RT_ERR_INVALID_ARG, "instances given without models/props");
     for (uint32_t m = 0; m < d->num_models; ++m) {
@@ -289,7 +291,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     s->device = (int)d->device;
     Bvh8Result bvh;
     std::string err;
-    if (d->flags & REINA_RT_BUILD_DEVICE) {
+    if (!(d->flags & REINA_RT_BUILD_HOST)) {   // the device build is the default
         if (hipSetDevice(s->device) != hipSuccess) {
             delete s;
             return fail(REINA_RT_ERR_DEVICE, "hipSetDevice failed");
@@ -649,7 +651,7 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
 int reina_rt_set_trace_options(ReinaScene* s, uint32_t options) {
     if (!s) return fail(REINA_RT_ERR_INVALID_ARG, "scene is null");
     if (options & ~(REINA_RT_TRACE_TIME_EXTEND | REINA_RT_TRACE_COUNT | REINA_RT_TRACE_NEE | REINA_RT_TRACE_FIXES |
-                    REINA_RT_TRACE_SAMPLE_GROUPS_MASK))
+                    REINA_RT_TRACE_SAMPLE_GROUPS_MASK | REINA_RT_TRACE_F16_INV))
         return fail(REINA_RT_ERR_INVALID_ARG, "unknown trace option bits");
     s->options = options;
     return REINA_RT_OK;
@@ -663,10 +665,16 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
     int rc = ensure_trace_buffers(s, 0);
     if (rc) return rc;
     if (n == 0) return REINA_RT_OK;
+    const bool h16 = (s->options & REINA_RT_TRACE_F16_INV) != 0;
+    if (h16 && s->sv.qnodes) return fail(REINA_RT_ERR_UNSUPPORTED, "FP16 inverse directions need FP32 nodes");
     HIP_TRY(hipMemsetAsync(s->fetch, 0, kParts * kLine * 4, st));
     TraceInput tin{nullptr, 0, n, s->fetch};
-    HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters, counters != nullptr,
-                         s->tune, st));
+    if (h16)
+        HIP_TRY(launch_trace_f32_h16(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters,
+                                     counters != nullptr, s->tune, st));
+    else
+        HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters, counters != nullptr,
+                             s->tune, st));
     return REINA_RT_OK;
 }
 

@@ -50,6 +50,8 @@ struct ReinaServer {
     Tuning tune{};
     uint32_t tiles = 0;          // 1024-pixel slot tiles (multiple of kParts)
     uint32_t max_pending = 0;
+    uint64_t admit_paths = 0;    // admission target (UINT64_MAX: none)
+    uint64_t live_paths = 0;     // live paths after the last graph (worker thread only)
     // device memory
     void* dbuf = nullptr;
     PathState S{};
@@ -153,9 +155,11 @@ static void worker_main(ReinaServer* s) {
                 continue;
             }
             if (running.empty()) busy_from = std::chrono::steady_clock::now();
+            uint64_t inflight = running.empty() ? 0 : s->live_paths;
             while (!s->pending.empty()) {
                 const uint64_t tk = s->pending.front();
                 Request& r = s->reqs[tk];
+                if (!running.empty() && inflight + r.npx > s->admit_paths) break;
                 int job = -1;
                 for (uint32_t j = 0; j < kMaxJobs; ++j)
                     if (s->job_ticket[j] == 0) { job = (int)j; break; }
@@ -171,6 +175,7 @@ static void worker_main(ReinaServer* s) {
                 s->job_ticket[job] = tk;
                 running.push_back(tk);
                 admitted.push_back(r);
+                inflight += r.npx;
                 s->pending.pop_front();
             }
         }
@@ -195,6 +200,7 @@ static void worker_main(ReinaServer* s) {
         // graph leaves the next iteration's rays in queue 0, control slot 0).
         uint64_t live = 0;
         for (uint32_t p = 0; p < kParts; ++p) live += s->count_h[p * kLine];
+        s->live_paths = live;
         if (s->tune.tail_paths != 0 && live != 0 && live <= s->tune.tail_paths) {
             const unsigned long long before = s->stats_h[0];
             W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats, s->done_d,
@@ -203,6 +209,7 @@ static void worker_main(ReinaServer* s) {
             W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
             W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
             W_TRY(hipStreamSynchronize(s->st));
+            s->live_paths = 0;
             std::lock_guard<std::mutex> lk(s->mu);
             s->counts.tail_launches += 1;
             s->counts.tail_rays += s->stats_h[0] - before;
@@ -297,6 +304,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     s->nee = nee;
     s->tiles = (uint32_t)tiles;
     s->max_pending = d->max_pending ? d->max_pending : 1024;
+    s->admit_paths = (d->admit_paths == 0 || d->admit_paths == REINA_RT_ADMIT_ALL) ? UINT64_MAX : d->admit_paths;
     s->t0 = std::chrono::steady_clock::now();
     auto bail = [&](int code, const std::string& msg) {
         free_server(s);

@@ -39,7 +39,7 @@ struct AnyHitOut {
     const float4* scon;
 };
 
-template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI, bool ANY = false>
+template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI, bool ANY = false, bool H16 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(
     SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits, TraceInput in,
     unsigned long long* stats, uint32_t* counters, uint32_t refill_min, const uint32_t* __restrict__ pix,
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                             num_nodes = J.sv.num_nodes;
                             tflags = J.sv.tri_flags;
                         }
-                        tray_init(r, rays[2 * idx], rays[2 * idx + 1]);
+                        tray_init<H16>(r, rays[2 * idx], rays[2 * idx + 1]);
                         h = miss_record(r);
                         ts.cur = 0;
                         ts.sp = 0;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             continue;
         }
         if (active) {
-            if (!trav_step<COUNT, Q, ANY>(nodes, tris, r, h, ts, stk, 256, nv, nt, tflags)) {
+            if (!trav_step<COUNT, Q, ANY, H16>(nodes, tris, r, h, ts, stk, 256, nv, nt, tflags)) {
                 if constexpr (ANY) {
                     if (h.prim == 0xFFFFFFFFu) {   // unoccluded
                         const float4 c = ao.scon[idx];

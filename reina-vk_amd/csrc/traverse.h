@@ -23,6 +23,7 @@
 namespace rt {
 
 constexpr float kBoxEps = 1.52587890625e-05f;  // 2^-16
+constexpr float kBoxEpsH16 = 9.765625e-04f;    // 2^-10: slab widening with FP16 inverse directions
 constexpr float kFmaInvMax = 1099511627776.0f;  // 2^40
 constexpr float kSlabBig = 3.0e38f;
 
@@ -43,6 +44,11 @@ RT_D float comp3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1
 
 RT_D bool finite_(float x) { return (x == x) && (x - x == 0.0f); }
 
+// H16: the inverse direction is rounded to FP16 (relative error <= 2^-11,
+// so every slab distance is off by at most 2^-11 |t| plus FP32 roundings,
+// which the 2^-10 widening of slab_hit<true> covers); a ray whose inverse
+// components are not normal FP16 numbers keeps FP32 inverses.
+template <bool H16 = false>
 RT_D void tray_init(TRay& r, float4 a, float4 b) {
     r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
     r.dx = b.x; r.dy = b.y; r.dz = b.z; r.tmax = b.w;
@@ -63,6 +69,14 @@ RT_D void tray_init(TRay& r, float4 a, float4 b) {
     r.Sz = 1.0f / dkz;
     // A zero component gives an infinite inverse (see node_hits).
     r.ix = 1.0f / r.dx; r.iy = 1.0f / r.dy; r.iz = 1.0f / r.dz;
+    if constexpr (H16) {
+        auto normal16 = [](float v) { const float m = fabsf(v); return m >= 6.103515625e-05f && m <= 65504.0f; };
+        if (normal16(r.ix) && normal16(r.iy) && normal16(r.iz)) {
+            r.ix = (float)(_Float16)r.ix;
+            r.iy = (float)(_Float16)r.iy;
+            r.iz = (float)(_Float16)r.iz;
+        }
+    }
     const float oix = -(r.ox * r.ix), oiy = -(r.oy * r.iy), oiz = -(r.oz * r.iz);
     const float ex = fabsf(oix) * kBoxEps, ey = fabsf(oiy) * kBoxEps, ez = fabsf(oiz) * kBoxEps;
     r.cnx = oix - ex; r.cny = oiy - ey; r.cnz = oiz - ez;
@@ -130,13 +144,16 @@ struct HitRec {
 // for a ray inside a plane, which fmaxf/fminf ignore (closed slab), and an
 // interval end at +-inf is clamped to a finite value so the widening keeps it
 // a miss.
+template <bool H16 = false>
 RT_D bool slab_hit(float tn, float tf, float tmin, float tb) {
-    tn = fmaxf(__builtin_fmaf(-fabsf(tn), kBoxEps, tn), tmin);
-    tf = fminf(__builtin_fmaf(fabsf(tf), kBoxEps, tf), tb);
+    constexpr float eps = H16 ? kBoxEpsH16 : kBoxEps;
+    tn = fmaxf(__builtin_fmaf(-fabsf(tn), eps, tn), tmin);
+    tf = fminf(__builtin_fmaf(fabsf(tf), eps, tf), tb);
     return tn <= tf;
 }
 
 // The 8 children's intervals from their near / far planes, both forms.
+template <bool H16 = false>
 RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float* NY, const float* FY,
                     const float* NZ, const float* FZ, float tb) {
     uint32_t hitm = 0;
@@ -147,7 +164,7 @@ RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float
                                    __builtin_fmaf(NZ[k], r.iz, r.cnz));
             const float tf = fminf(fminf(__builtin_fmaf(FX[k], r.ix, r.cfx), __builtin_fmaf(FY[k], r.iy, r.cfy)),
                                    __builtin_fmaf(FZ[k], r.iz, r.cfz));
-            hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+            hitm |= (slab_hit<H16>(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
         }
     } else {
 #pragma unroll
@@ -156,7 +173,7 @@ RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float
             float tf = fminf(fminf((FX[k] - r.ox) * r.ix, (FY[k] - r.oy) * r.iy), (FZ[k] - r.oz) * r.iz);
             tn = fminf(tn, kSlabBig);
             tf = fmaxf(tf, -kSlabBig);
-            hitm |= (slab_hit(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
+            hitm |= (slab_hit<H16>(tn, tf, r.tmin, tb) ? 1u : 0u) << k;
         }
     }
     return hitm;
@@ -164,6 +181,7 @@ RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float
 
 // Slab test of the 8 children; returns the hit mask in slot order (before
 // masking out empty slots); see slab_hit for the two forms.
+template <bool H16 = false>
 RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
     const float* base = reinterpret_cast<const float*>(node);
     // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
@@ -184,7 +202,7 @@ RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
         NZ[4 * h + 0] = e.x; NZ[4 * h + 1] = e.y; NZ[4 * h + 2] = e.z; NZ[4 * h + 3] = e.w;
         FZ[4 * h + 0] = f.x; FZ[4 * h + 1] = f.y; FZ[4 * h + 2] = f.z; FZ[4 * h + 3] = f.w;
     }
-    return slab8(r, NX, FX, NY, FY, NZ, FZ, tb);
+    return slab8<H16>(r, NX, FX, NY, FY, NZ, FZ, tb);
 }
 
 // Byte j of w as float times s plus p, one rounding (q * s is exact).
@@ -240,7 +258,7 @@ struct TravState {
 // ANY: shadow-ray mode: the first triangle hit with t in [tmin, tmax] ends
 // the traversal, except back faces of triangles whose flags (kTriCull,
 // kTriMirror) say their instance culls them.
-template <bool COUNT, bool Q, bool ANY = false>
+template <bool COUNT, bool Q, bool ANY = false, bool H16 = false>
 RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec& h, TravState& ts, uint2* stk,
                     uint32_t stride, uint32_t& nvis, uint32_t& ntst, const uint8_t* tflags = nullptr) {
     uint4 hdr;
@@ -253,7 +271,7 @@ RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec
     } else {
         const Bvh8Node* node = static_cast<const Bvh8Node*>(nodes) + ts.cur;
         hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
-        hitm = node_hits(r, node, h.t);
+        hitm = node_hits<H16>(r, node, h.t);
     }
     const uint32_t child_base = hdr.x & 0xFFFFFFu, tri_base = hdr.y & 0xFFFFFFu;
     const uint32_t imask = hdr.x >> 24, leafmask = hdr.y >> 24;

@@ -137,6 +137,9 @@ hipError_t launch_connect_multi_f32(const JobRec* jobs, const Queues& Q, const P
 hipError_t launch_connect_multi_q8(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
                                    const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st);
 hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, hipStream_t st);
+hipError_t launch_trace_f32_h16(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
+                                unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
+                                hipStream_t st);
 hipError_t launch_trace_f32(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                             hipStream_t st);

@@ -182,7 +182,8 @@ class Server:
     on resident DeviceScenes share one wavefront; submit() returns a ticket
     at once, wait()/poll() report completion with timestamps."""
 
-    def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, nee=False, **tuning):
+    def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, nee=False,
+                 admit_paths=0, **tuning):
         d = abi.ReinaServerDesc()
         d.struct_size = ctypes.sizeof(abi.ReinaServerDesc)
         d.device = device
@@ -190,6 +191,7 @@ class Server:
         d.max_pending = max_pending
         d.quantized = int(bool(quantized))
         d.flags = abi.SERVER_NEE if nee else 0
+        d.admit_paths = int(admit_paths)
         for k, v in tuning.items():
             setattr(d.tuning, k, v)
         h = ctypes.c_void_p()

@@ -145,7 +145,7 @@ class ReinaServerDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("path_budget_bytes", ctypes.c_uint64), ("max_pending", ctypes.c_uint32),
                 ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning), ("flags", ctypes.c_uint32),
-                ("_pad0", ctypes.c_uint32)]
+                ("admit_paths", ctypes.c_uint32)]
 
 
 class ReinaRequestStatus(ctypes.Structure):
@@ -178,6 +178,7 @@ TRACE_FIX_METAL_PDF = 0x40
 TRACE_FIX_DISNEY_MIX = 0x80
 TRACE_FIXES = 0xF8
 TRACE_SAMPLE_GROUPS_MASK = 0xF00
+TRACE_F16_INV = 0x1000
 
 
 def TRACE_SAMPLE_GROUPS(log2g):
@@ -196,6 +197,7 @@ ERR_UNSUPPORTED = -6
 ERR_NO_EMISSIVE = -7
 ERR_BUSY = -8
 
-BUILD_HOST = 0x0
-BUILD_DEVICE = 0x1
+BUILD_DEVICE = 0x1   # the default (flags 0 builds on the device too)
+BUILD_HOST = 0x4
 NODES_QUANTIZED = 0x2
+ADMIT_ALL = 0xFFFFFFFF

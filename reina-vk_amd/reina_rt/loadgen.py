@@ -20,11 +20,11 @@ from . import DeviceScene, ReinaError, Server, abi, scenes
 from .scene import push_consts
 
 
-def make_pool(sizes, seed=0, quantized=False, threads=8, device_build=False):
+def make_pool(sizes, seed=0, quantized=False, threads=8, device_build=True):
     """Resident scenes of the given triangle counts (pool_scene), created
-    concurrently; device_build: BVH8 built on the GPU (REINA_RT_BUILD_DEVICE).
+    concurrently; device_build False: BVH8 built on the host (REINA_RT_BUILD_HOST).
     Returns a list of dicts {dev, cam, tris}."""
-    flags = (abi.NODES_QUANTIZED if quantized else 0) | (abi.BUILD_DEVICE if device_build else 0)
+    flags = (abi.NODES_QUANTIZED if quantized else 0) | (abi.BUILD_DEVICE if device_build else abi.BUILD_HOST)
 
     def one(k):
         sc, cam = scenes.pool_scene(int(sizes[k]), seed * 1000 + k)

@@ -789,7 +789,7 @@ def test_device_bvh_render_equals_host(cuda, flags, icosphere_arrays):
     arrays, cam = icosphere_arrays
     W, H = 96, 54
     pc = push_consts(cam["eye"], cam["center"], cam["fov"], W, H, 2, 8)
-    dh = reina_rt.DeviceScene(arrays)
+    dh = reina_rt.DeviceScene(arrays, flags=abi.BUILD_HOST)
     dd = reina_rt.DeviceScene(arrays, flags=flags)
     a = gpu_render(torch, dh, pc, W, H)
     b = gpu_render(torch, dd, pc, W, H)
@@ -800,3 +800,38 @@ def test_device_bvh_render_equals_host(cuda, flags, icosphere_arrays):
     assert dd.info().sah_cost < 1.05 * dh.info().sah_cost
     dh.close()
     dd.close()
+
+
+def test_intersect_f16_inverses_bitexact(cuda, icosphere_arrays):
+    """REINA_RT_TRACE_F16_INV (ray-direction inverses rounded to FP16, slab
+    intervals widened by 2^-10 |t|): the box test stays conservative, so the
+    closest hit of every ray equals the FP32-inverse query and brute force,
+    including rays along and near the axes (inverses beyond FP16's normal
+    range keep FP32); the wider slabs cost extra node visits only."""
+    torch = cuda
+    arrays, _ = icosphere_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    rays = random_rays(60000, -1.6, 1.6, 91)
+    near = random_rays(6000, -1.6, 1.6, 92)
+    for k, ax in enumerate(("dx", "dy", "dz")):          # axis-parallel and within 1e-6 of an axis
+        sl = slice(2000 * k, 2000 * (k + 1))
+        for other in ("dx", "dy", "dz"):
+            if other != ax:
+                near[other][sl] = np.where(np.arange(2000) % 2 == 0, 0.0, 1e-6).astype(np.float32)
+    rays = np.concatenate([rays, near])
+    h32, c32 = gpu_intersect(torch, dev, rays, counters=True)
+    dev.set_trace_options(abi.TRACE_F16_INV)
+    h16, c16 = gpu_intersect(torch, dev, rays, counters=True)
+    dev.set_trace_options(0)
+    assert np.array_equal(_bits(h16), _bits(h32))
+    orc = oracle.OracleScene(arrays, make_desc)
+    ref = orc.brute32(rays)
+    assert np.array_equal(h16["prim"], ref["prim"]) and np.array_equal(_bits(h16["t"]), _bits(ref["t"]))
+    assert c16[:, 0].sum() >= c32[:, 0].sum()
+    print("node visits per ray: FP32 inverses %.3f, FP16 inverses %.3f" % (c32[:, 0].mean(), c16[:, 0].mean()))
+    q = reina_rt.DeviceScene(arrays, flags=abi.NODES_QUANTIZED)
+    q.set_trace_options(abi.TRACE_F16_INV)
+    with pytest.raises(reina_rt.ReinaError):
+        gpu_intersect(torch, q, rays[:10])
+    q.close()
+    dev.close()

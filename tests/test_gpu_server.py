@@ -117,6 +117,31 @@ def test_server_mix_equals_alone(cuda, pool, tail):
     srv.close()
 
 
+def test_server_admit_paths(cuda, pool):
+    """With admit_paths = 1 a request is admitted only when nothing runs:
+    requests submitted together run one after another (each admitted after
+    the previous one completed), and every image is still its
+    reina_rt_trace image."""
+    torch = cuda
+    jobs = []
+    for name, W, H, spp in (("ico", 64, 48, 2), ("cornell", 40, 40, 3), ("inst", 48, 27, 2), ("ico", 32, 32, 1)):
+        arrays, cam, dev = pool[name]
+        pc = scenes.cornell_push(W, H, spp=spp, max_bounces=8) if name == "cornell" else _pc(cam, W, H, spp, 8)
+        jobs.append((dev, pc, W, H, None))
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1)
+    imgs, tickets = [], []
+    for dev, pc, W, H, _ in jobs:
+        img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+        imgs.append(img)
+        tickets.append(srv.submit(dev, pc, img.data_ptr(), W, H))
+    sts = [srv.wait(t) for t in tickets]
+    srv.close()
+    for a, b in zip(sts, sts[1:]):
+        assert b.admit_s >= a.done_s, (a.done_s, b.admit_s)
+    for (dev, pc, W, H, _), img in zip(jobs, imgs):
+        assert _same(img.cpu().numpy(), plain(torch, dev, pc, W, H))
+
+
 def test_server_admission_by_budget(cuda, pool):
     """A path budget of 8 tiles (8192 pixels): a request needing more is
     rejected at submit; 6 requests of 4 tiles run at most two at a time
