@@ -104,6 +104,8 @@ def main():
                          "triangles, 64 requests; 32 scenes of 1e4-4e6 triangles at 50-400 requests/s)")
     ap.add_argument("--admit-paths", type=int, default=1 << 21,
                     help="served block: the server admits while fewer paths than this are in flight")
+    ap.add_argument("--max-wait-ms", type=int, default=500,
+                    help="served block, shortest-first variant: age at which a request goes first")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
@@ -478,6 +480,19 @@ def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
 SERVER_BUDGET = 152 * (1 << 24)   # the largest path pool: 2^24 pixels in flight (2.55 GB)
 
 
+def server_tuning(torch, pool, reqs, skw, what):
+    """Server launch parameters for this pool, by reina_rt.autotune.tune_server
+    on a batch of the block's own requests."""
+    from reina_rt import autotune
+    t0 = time.perf_counter()
+    best, default, table = autotune.tune_server(
+        torch, [(pool[r["scene"]]["dev"], r["pc"], r["W"], r["H"]) for r in reqs], skw)
+    log("%s: server autotune %.1f s, best %.1f ms vs default %.1f ms" % (
+        what, time.perf_counter() - t0, best["ms"], default["ms"]))
+    return {"best": best, "default": default, "candidates": len(table), "batch_requests": len(reqs),
+            "seconds": round(time.perf_counter() - t0, 1)}
+
+
 def batched_block(a, torch):
     """Many requests of uneven size at once: scenes of log-uniform triangle
     count, resolutions from {128^2, 256^2, 512^2, 1024^2}, spp from
@@ -485,7 +500,7 @@ def batched_block(a, torch):
     vs the same requests through reina_rt_trace one after another."""
     import numpy as np
 
-    from reina_rt import loadgen
+    from reina_rt import abi, loadgen
     rng = np.random.default_rng(64)
     n_sc, lo, hi, n_req = (16, 1e3, 2e5, 32) if a.small_pools else (64, 1e3, 2e6, 64)
     t0 = time.perf_counter()
@@ -494,7 +509,17 @@ def batched_block(a, torch):
     log("batched: %d scenes built in %.1f s" % (n_sc, setup))
     reqs = loadgen.make_requests(pool, n_req, rng, [(128, 128), (256, 256), (512, 512), (1024, 1024)],
                                  spp_choices=(4, 16, 64, 256), depth=8, orbit=False)
-    res = loadgen.batched(torch, pool, reqs, dict(path_budget_bytes=SERVER_BUDGET))
+    skw = dict(path_budget_bytes=SERVER_BUDGET)
+    tuned = None
+    if not a.no_autotune:
+        tuned = server_tuning(torch, pool, reqs, skw, "batched")
+        skw = dict(skw, **{k: v for k, v in tuned["best"].items() if k != "ms"})
+    res = loadgen.batched(torch, pool, reqs, skw)
+    if tuned is not None:
+        res["server_autotune"] = tuned
+        # the same batch through a server at the default launch parameters
+        res["batched_default_tuning"] = loadgen.batched(torch, pool, reqs, dict(path_budget_bytes=SERVER_BUDGET),
+                                                        plain=False)["batched"]
     res.update({"scenes": n_sc, "triangles_min": min(p["tris"] for p in pool),
                 "triangles_max": max(p["tris"] for p in pool),
                 "triangles_total": int(sum(p["tris"] for p in pool)), "scene_setup_s": round(setup, 1),
@@ -507,7 +532,7 @@ def batched_block(a, torch):
         log("batched: %d sample streams per pixel" % g)
         for p in pool:
             p["dev"].set_trace_options(abi.TRACE_SAMPLE_GROUPS(a.batched_groups))
-        rg = loadgen.batched(torch, pool, reqs, dict(path_budget_bytes=SERVER_BUDGET))
+        rg = loadgen.batched(torch, pool, reqs, skw)
         res["sample_groups_%d" % g] = {k: rg[k] for k in ("plain", "batched", "speedup_images_per_s",
                                                           "images_bit_identical") if k in rg}
         for p in pool:
@@ -524,7 +549,7 @@ def served_block(a, torch):
     server, both with the same bounded FIFO."""
     import numpy as np
 
-    from reina_rt import loadgen
+    from reina_rt import abi, loadgen
     rng = np.random.default_rng(5)
     if a.small_pools:
         n_sc, lo, hi, runs = 8, 1e4, 4e5, [(50.0, 4.0), (4.0, 6.0)]
@@ -538,19 +563,37 @@ def served_block(a, torch):
            "triangles_min": min(p["tris"] for p in pool), "triangles_max": max(p["tris"] for p in pool),
            "scene_setup_s": round(setup, 1), "runs": []}
     out["admit_paths"] = a.admit_paths
+    res_mix = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    base = dict(path_budget_bytes=SERVER_BUDGET)
+    if not a.no_autotune:
+        treqs = loadgen.make_requests(pool, 12, np.random.default_rng(55), res_mix, spp_range=(16, 128))
+        out["server_autotune"] = server_tuning(torch, pool, treqs, base, "served")
+        base = dict(base, **{k: v for k, v in out["server_autotune"]["best"].items() if k != "ms"})
+    # the server holds at most 17 requests, queued or running, as plain does (1 rendering + 16 queued)
+    fifo = dict(base, admit_paths=a.admit_paths, max_requests=17)
+    shortest = dict(fifo, admit_order=abi.ADMIT_SHORTEST, max_wait_ms=a.max_wait_ms)
+    out["served_policy"] = ("at most 17 requests queued or running (plain: 1 rendering + 16 queued); admission in "
+                            "submission order while fewer than admit_paths paths are in flight (a request is "
+                            "always admitted when nothing runs)")
+    out["served_shortest_policy"] = ("least estimated work first (pixels x spp x depth) with the same admit_paths; "
+                                     "a request waiting %d ms or longer goes first, oldest first" % a.max_wait_ms)
     for rate, dur in runs:
-        reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng,
-                                     [(256, 256), (512, 512), (1280, 720), (1920, 1080)], spp_range=(16, 128))
-        r = loadgen.served(torch, pool, reqs, rate, dur, 16,
-                           dict(path_budget_bytes=SERVER_BUDGET, admit_paths=a.admit_paths), seed=int(rate))
-        log("served: rate %.0f/s plain %.2f img/s served %.2f img/s, p99 %.0f / %.0f ms" % (
-            rate, r["plain"]["images_per_s"], r["served"]["images_per_s"],
-            r["plain"]["latency"].get("p99_ms", 0), r["served"]["latency"].get("p99_ms", 0)))
+        reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng, res_mix, spp_range=(16, 128))
+        variants = {}
+        if rate >= 50.0:
+            variants["served_shortest"] = shortest
         if rate == 100.0:
-            # the same arrivals with no admission target: every request that fits the path pool runs at once
-            times = loadgen.arrivals(rate, dur, np.random.default_rng(int(rate)))
-            r["served_admit_all"] = loadgen.served_server(torch, pool, reqs[:len(times)], times, 16,
-                                                          dict(path_budget_bytes=SERVER_BUDGET))
+            # no admission target and no bound beyond the 16 queued: every request that fits the path pool runs
+            variants["served_admit_all"] = base
+        r = loadgen.served(torch, pool, reqs, rate, dur, 16, fifo, seed=int(rate), variants=variants)
+        log("served: rate %.0f/s plain %.2f img/s served %.2f img/s, p50 %.0f / %.0f ms, p99 %.0f / %.0f ms" % (
+            rate, r["plain"]["images_per_s"], r["served"]["images_per_s"],
+            r["plain"]["latency"].get("p50_ms", 0), r["served"]["latency"].get("p50_ms", 0),
+            r["plain"]["latency"].get("p99_ms", 0), r["served"]["latency"].get("p99_ms", 0)))
+        if "served_shortest" in r:
+            v = r["served_shortest"]
+            log("served: rate %.0f/s shortest-first %.2f img/s, p50 %.0f ms, p99 %.0f ms" % (
+                rate, v["images_per_s"], v["latency"].get("p50_ms", 0), v["latency"].get("p99_ms", 0)))
         out["runs"].append(r)
     for p in pool:
         p["dev"].close()

@@ -354,6 +354,8 @@ typedef struct ReinaTuning {
     uint32_t waves_per_simd;/* traversal register budget: 0 (compiler's choice), 5, 6 or 8      */
     uint32_t tail_paths;    /* when at most this many paths are live, the frame ends in one
                                tail launch (per-wave extend/shade steps); 0 = never            */
+    uint32_t claim_rays;    /* rays a traversal wave claims from its queue partition with one
+                               atomic: 64, 128, 256 or 512 (0 = 128)                           */
 } ReinaTuning;
 int reina_rt_get_tuning(const ReinaScene* scene, ReinaTuning* out);
 int reina_rt_set_tuning(ReinaScene* scene, const ReinaTuning* tuning);
@@ -393,13 +395,13 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  *
  * A server owns a worker thread, a stream, and a pool of path-state slots
  * sized by path_budget_bytes (REINA_RT_SERVER_BYTES_PER_PIXEL per pixel in
- * flight). Submitted requests wait in a bounded FIFO; the worker admits them
- * (in order) as soon as enough contiguous slots and a request index are
+ * flight). Submitted requests wait in a bounded queue; the worker admits them
+ * (in admit_order) as soon as enough contiguous slots and a request index are
  * free (and, with admit_paths, while fewer paths than that are in flight:
  * under overload this keeps fewer requests sharing the GPU, so each
  * finishes sooner), generates their camera rays into the shared queues, and runs bounce
  * iterations over every admitted request at once (captured in a HIP graph of
- * 6 iterations). When at most tuning.tail_paths paths are live after a graph,
+ * graph_iters iterations). When at most tuning.tail_paths paths are live after a graph,
  * one tail launch (per-wave extend/shade steps, as reina_rt_trace's) finishes
  * every running request; requests arriving meanwhile wait for it. A request
  * completes when all its pixels have finished all
@@ -420,7 +422,7 @@ typedef struct ReinaServer ReinaServer;
 typedef struct ReinaServerDesc {
     uint32_t struct_size;        /* sizeof(ReinaServerDesc)                              */
     uint32_t device;             /* HIP device ordinal (scenes must live on it)          */
-    uint64_t path_budget_bytes;  /* HBM for in-flight pixels; at least 8 * 1024 pixels   */
+    uint64_t path_budget_bytes;  /* HBM for in-flight pixels; at least 8 slot tiles      */
     uint32_t max_pending;        /* bounded FIFO; submit beyond it -> REINA_RT_ERR_BUSY  */
     uint32_t quantized;          /* node format of every scene served: 0 FP32, 1 8-bit   */
     ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
@@ -431,10 +433,30 @@ typedef struct ReinaServerDesc {
                                     (live paths of the last graph + paths admitted since);
                                     a request is always admitted when nothing runs.
                                     0 or REINA_RT_ADMIT_ALL: admit whenever slots are free */
+    uint32_t slot_tile;          /* pixels per slot tile (the unit of path-slot allocation
+                                    and of the queue-partition interleave): 256, 512, 1024,
+                                    2048 or 4096; 0 = 1024                                */
+    uint32_t graph_iters;        /* bounce iterations per graph replay, between two
+                                    admission / completion points: 6, 12, 18 or 24; 0 = 6 */
+    uint32_t admit_order;        /* REINA_RT_ADMIT_FIFO or REINA_RT_ADMIT_SHORTEST         */
+    uint32_t max_wait_ms;        /* REINA_RT_ADMIT_SHORTEST: a request waiting this long is
+                                    admitted before any shorter one (oldest first); 0 = 500 */
+    uint32_t max_requests;       /* submit -> REINA_RT_ERR_BUSY while this many requests are
+                                    queued or running (a capacity bound like a plain
+                                    renderer's 1 running + max_pending queued); 0 = none  */
 } ReinaServerDesc;
 
 /* ReinaServerDesc.admit_paths: no limit but the path pool. */
 #define REINA_RT_ADMIT_ALL 0xFFFFFFFFu
+
+/* ReinaServerDesc.admit_order. FIFO: submission order. SHORTEST: least
+ * estimated work first (path slots x samples per pixel x max bounces), except
+ * that requests waiting max_wait_ms or longer go first, oldest first, and
+ * admission stops at the first of those that does not fit: no request waits
+ * more than max_wait_ms plus the time for the requests ahead of it to free
+ * their slots. */
+#define REINA_RT_ADMIT_FIFO     0u
+#define REINA_RT_ADMIT_SHORTEST 1u
 
 /* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */
 #define REINA_RT_TAIL_OFF 0xFFFFFFFFu
@@ -456,9 +478,9 @@ typedef struct ReinaServerStats {
     uint64_t submitted, completed, rejected_busy, rejected_budget;
     uint64_t extension_rays;     /* traced by the shared kernels                   */
     uint64_t iterations;         /* bounce iterations run                          */
-    uint64_t graph_replays;      /* captured 6-iteration graphs launched           */
+    uint64_t graph_replays;      /* captured graphs (graph_iters iterations) launched */
     uint32_t pending, running;   /* now                                            */
-    uint32_t slot_tiles_total, slot_tiles_used;   /* 1024-pixel tiles              */
+    uint32_t slot_tiles_total, slot_tiles_used;   /* tiles of slot_tile pixels     */
     double   busy_s;             /* worker time with admitted requests             */
     uint64_t tail_launches;      /* tail launches that finished every running request */
     uint64_t tail_rays;          /* extension rays traced by those launches        */

@@ -623,6 +623,7 @@ int reina_rt_get_tuning(const ReinaScene* s, ReinaTuning* o) {
     o->shade_blocks = s->tune.shade_grid;
     o->waves_per_simd = s->tune.waves_per_simd;
     o->tail_paths = s->tune.tail_paths;
+    o->claim_rays = s->tune.claim;
     return REINA_RT_OK;
 }
 
@@ -639,12 +640,16 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
         return fail(REINA_RT_ERR_INVALID_ARG, "grid_blocks must be in 1..65536 and shade_blocks a multiple of 8");
     if (t->waves_per_simd != 0 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
         return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    const uint32_t claim = t->claim_rays ? t->claim_rays : 128u;
+    if (claim != 64 && claim != 128 && claim != 256 && claim != 512)
+        return fail(REINA_RT_ERR_INVALID_ARG, "claim_rays must be 0, 64, 128, 256 or 512");
     s->tune.stack_depth = sd;
     s->tune.refill_min = t->refill_min;
     s->tune.grid = t->grid_blocks;
     s->tune.shade_grid = t->shade_blocks;
     s->tune.waves_per_simd = t->waves_per_simd;
     s->tune.tail_paths = t->tail_paths;
+    s->tune.claim = claim;
     return REINA_RT_OK;
 }
 

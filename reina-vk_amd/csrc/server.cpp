@@ -1,12 +1,15 @@
 // server.cpp — request loop of the batching / serving mode (reina_rt_server_*,
 // reina_rt_submit / poll / wait).
 //
-// The worker thread alternates, at 6-iteration boundaries:
-//   1. admission: FIFO requests take a request index (queue words carry it in
-//      their top 8 bits) and a contiguous run of 1024-pixel slot tiles
-//      (first fit); their JobRec is copied to the device table and
-//      k_init_job appends their camera rays to the current queue;
-//   2. one replay of the captured graph: 6 x (k_trace MULTI, k_shade_multi);
+// The worker thread alternates, at graph boundaries (graph_iters bounce
+// iterations, a multiple of 6):
+//   1. admission: waiting requests (in submission order, or least estimated
+//      work first with an age bound) take a request index (queue words carry
+//      it in their top 8 bits) and a contiguous run of slot tiles (first
+//      fit); their JobRec is copied to the device table and k_init_job
+//      appends their camera rays to the current queue;
+//   2. one replay of the captured graph: graph_iters x (k_trace MULTI,
+//      k_shade_multi);
 //   3. a read-back of the per-request finished-pixel counters; complete
 //      requests get k_finish_job (image write) and release their slots.
 // Iteration i uses control slot i % 3 and queue i & 1; every boundary is a
@@ -29,7 +32,6 @@ using namespace rt;
 
 namespace {
 
-constexpr uint32_t kGraphIters = 6;
 
 struct Request {
     ReinaScene* scene = nullptr;
@@ -38,6 +40,7 @@ struct Request {
     uint32_t npx = 0;
     uint32_t state = REINA_RT_REQ_QUEUED;
     double submit_s = 0, admit_s = 0, done_s = 0;
+    double work = 0;   // estimated cost: path slots x samples x max bounces
     uint32_t job = 0, tile0 = 0, ntiles = 0;
 };
 
@@ -48,8 +51,14 @@ struct ReinaServer {
     bool quantized = false;
     bool nee = false;            // holds NEE state and shadow queues (REINA_RT_SERVER_NEE)
     Tuning tune{};
-    uint32_t tiles = 0;          // 1024-pixel slot tiles (multiple of kParts)
+    uint32_t tile_log2 = 10;     // slot tile: 1 << tile_log2 pixels
+    uint32_t tiles = 0;          // slot tiles (multiple of kParts)
+    uint32_t graph_iters = 6;    // bounce iterations per graph replay
+    uint32_t admit_order = REINA_RT_ADMIT_FIFO;
+    double max_wait_s = 0.5;     // REINA_RT_ADMIT_SHORTEST: age that takes precedence
     uint32_t max_pending = 0;
+    uint32_t max_requests = 0;   // queued + running bound (0: none)
+    uint32_t running = 0;        // admitted, not yet complete (mu)
     uint64_t admit_paths = 0;    // admission target (UINT64_MAX: none)
     uint64_t live_paths = 0;     // live paths after the last graph (worker thread only)
     // device memory
@@ -101,7 +110,7 @@ static hipError_t capture_graph(ReinaServer* s) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    for (uint32_t i = 0; i < kGraphIters; ++i) {
+    for (uint32_t i = 0; i < s->graph_iters; ++i) {
         const uint32_t slot = i % 3, in_q = i & 1u;
         TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
         launch_trace_multi(s->quantized, s->Q.ray[in_q], s->Q.hit, tin, s->stats, s->tune, s->Q.pix[in_q],
@@ -138,6 +147,21 @@ static void fail_worker(ReinaServer* s, const std::string& msg) {
     s->cv_done.notify_all();
 }
 
+// Position in s->pending (submission order) of the next request to admit.
+// FIFO: the oldest. SHORTEST: the oldest if it has waited max_wait_s or
+// longer, else the least estimated work (earliest on ties).
+static size_t next_pending(ReinaServer* s, double now) {
+    if (s->admit_order == REINA_RT_ADMIT_FIFO) return 0;
+    if (now - s->reqs[s->pending.front()].submit_s >= s->max_wait_s) return 0;
+    size_t best = 0;
+    double wbest = s->reqs[s->pending[0]].work;
+    for (size_t i = 1; i < s->pending.size(); ++i) {
+        const double w = s->reqs[s->pending[i]].work;
+        if (w < wbest) { best = i; wbest = w; }
+    }
+    return best;
+}
+
 static void worker_main(ReinaServer* s) {
     if (hipSetDevice(s->device) != hipSuccess) {
         fail_worker(s, "hipSetDevice failed in the server thread");
@@ -156,8 +180,10 @@ static void worker_main(ReinaServer* s) {
             }
             if (running.empty()) busy_from = std::chrono::steady_clock::now();
             uint64_t inflight = running.empty() ? 0 : s->live_paths;
+            const double tnow = s->now();
             while (!s->pending.empty()) {
-                const uint64_t tk = s->pending.front();
+                const size_t pi = next_pending(s, tnow);
+                const uint64_t tk = s->pending[pi];
                 Request& r = s->reqs[tk];
                 if (!running.empty() && inflight + r.npx > s->admit_paths) break;
                 int job = -1;
@@ -173,10 +199,11 @@ static void worker_main(ReinaServer* s) {
                 r.state = REINA_RT_REQ_RUNNING;
                 r.admit_s = s->now();
                 s->job_ticket[job] = tk;
+                s->running += 1;
                 running.push_back(tk);
                 admitted.push_back(r);
                 inflight += r.npx;
-                s->pending.pop_front();
+                s->pending.erase(s->pending.begin() + (std::ptrdiff_t)pi);
             }
         }
         for (const Request& r : admitted) {
@@ -184,11 +211,12 @@ static void worker_main(ReinaServer* s) {
             jr.sv = r.scene->sv;
             jr.P = r.P;
             jr.img = r.img;
-            jr.px_base = r.tile0 << 10;
+            jr.px_base = r.tile0 << s->tile_log2;
             jr.npx = r.npx;
             W_TRY(hipMemcpyAsync(s->jobs_d + r.job, &jr, sizeof(JobRec), hipMemcpyHostToDevice, s->st));
             W_TRY(hipMemsetAsync(s->done_d + r.job, 0, sizeof(uint32_t), s->st));
-            W_TRY(launch_init_job(s->jobs_d, r.job, r.npx, s->S, s->Q, s->ctrl, 0, 0, s->done_d, s->st));
+            W_TRY(launch_init_job(s->jobs_d, r.job, r.npx, s->S, s->Q, s->ctrl, 0, 0, s->done_d, s->tile_log2,
+                                  s->st));
         }
         if (running.empty()) continue;
         W_TRY(hipGraphLaunch(s->graph, s->st));
@@ -217,7 +245,7 @@ static void worker_main(ReinaServer* s) {
         std::vector<uint64_t> finished;
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->counts.iterations += kGraphIters;
+            s->counts.iterations += s->graph_iters;
             s->counts.graph_replays += 1;
             s->counts.extension_rays = s->stats_h[0];
             for (uint64_t tk : running) {
@@ -245,6 +273,7 @@ static void worker_main(ReinaServer* s) {
                 for (uint32_t k = 0; k < r.ntiles; ++k) s->tile_used[r.tile0 + k] = 0;
                 s->counts.slot_tiles_used -= r.ntiles;
                 s->job_ticket[r.job] = 0;
+                s->running -= 1;
                 s->counts.completed += 1;
                 for (size_t k = 0; k < running.size(); ++k)
                     if (running[k] == tk) { running.erase(running.begin() + k); break; }
@@ -279,11 +308,20 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (d->flags & ~REINA_RT_SERVER_NEE) return fail(REINA_RT_ERR_INVALID_ARG, "unknown server flag bits");
     const bool nee = (d->flags & REINA_RT_SERVER_NEE) != 0;
     const uint64_t bpp = nee ? REINA_RT_SERVER_BYTES_PER_PIXEL_NEE : REINA_RT_SERVER_BYTES_PER_PIXEL;
-    const uint64_t per_tile = bpp * 1024;
+    const uint32_t tile = d->slot_tile ? d->slot_tile : 1024u;
+    if (tile != 256 && tile != 512 && tile != 1024 && tile != 2048 && tile != 4096)
+        return fail(REINA_RT_ERR_INVALID_ARG, "slot_tile must be 0, 256, 512, 1024, 2048 or 4096");
+    const uint32_t tile_log2 = (uint32_t)__builtin_ctz(tile);
+    const uint32_t graph_iters = d->graph_iters ? d->graph_iters : 6u;
+    if (graph_iters % 6 != 0 || graph_iters > 24)
+        return fail(REINA_RT_ERR_INVALID_ARG, "graph_iters must be 0, 6, 12, 18 or 24");
+    if (d->admit_order != REINA_RT_ADMIT_FIFO && d->admit_order != REINA_RT_ADMIT_SHORTEST)
+        return fail(REINA_RT_ERR_INVALID_ARG, "admit_order must be REINA_RT_ADMIT_FIFO or REINA_RT_ADMIT_SHORTEST");
+    const uint64_t per_tile = bpp * tile;
     uint64_t tiles = d->path_budget_bytes / per_tile;
     tiles -= tiles % kParts;
-    if (tiles < kParts) return fail(REINA_RT_ERR_INVALID_ARG, "path_budget_bytes below 8 x 1024 pixels");
-    if (tiles * 1024 > (1ull << kSlotBits)) tiles = (1ull << kSlotBits) / 1024;
+    if (tiles < kParts) return fail(REINA_RT_ERR_INVALID_ARG, "path_budget_bytes below 8 slot tiles");
+    if (tiles * tile > (1ull << kSlotBits)) tiles = (1ull << kSlotBits) / tile;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(REINA_RT_ERR_DEVICE, "no HIP device available");
     if ((int)d->device >= ndev) return fail(REINA_RT_ERR_INVALID_ARG, "device ordinal out of range");
@@ -298,12 +336,20 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (tu.refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be 1..64");
     if (tu.waves_per_simd != 0 && tu.waves_per_simd != 5 && tu.waves_per_simd != 6 && tu.waves_per_simd != 8)
         return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    tu.claim = d->tuning.claim_rays ? d->tuning.claim_rays : 128u;
+    if (tu.claim != 64 && tu.claim != 128 && tu.claim != 256 && tu.claim != 512)
+        return fail(REINA_RT_ERR_INVALID_ARG, "claim_rays must be 0, 64, 128, 256 or 512");
     ReinaServer* s = new ReinaServer();
     s->device = (int)d->device;
     s->quantized = d->quantized != 0;
     s->nee = nee;
     s->tiles = (uint32_t)tiles;
+    s->tile_log2 = tile_log2;
+    s->graph_iters = graph_iters;
+    s->admit_order = d->admit_order;
+    s->max_wait_s = (d->max_wait_ms ? d->max_wait_ms : 500u) * 1e-3;
     s->max_pending = d->max_pending ? d->max_pending : 1024;
+    s->max_requests = d->max_requests;
     s->admit_paths = (d->admit_paths == 0 || d->admit_paths == REINA_RT_ADMIT_ALL) ? UINT64_MAX : d->admit_paths;
     s->t0 = std::chrono::steady_clock::now();
     auto bail = [&](int code, const std::string& msg) {
@@ -317,7 +363,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (tu.grid == 0) tu.grid = cus * (tu.stack_depth <= 8 ? 8 : 4);
     if (tu.shade_grid == 0) tu.shade_grid = cus * 16;
     s->tune = tu;
-    const uint64_t slots = tiles * 1024;
+    const uint64_t slots = tiles * tile;
     const uint64_t bytes = slots * bpp;
     if (hipMalloc(&s->dbuf, bytes) != hipSuccess)
         return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
@@ -398,7 +444,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
                                                   " exceeds the server's stack depth " +
                                                   std::to_string(s->tune.stack_depth));
     const uint32_t npx = W * H * groups;   // path slots: one per pixel and sample group
-    const uint32_t ntiles = (npx + 1023u) >> 10;
+    const uint32_t ntiles = (uint32_t)(((uint64_t)npx + (1u << s->tile_log2) - 1u) >> s->tile_log2);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->failed) return fail(REINA_RT_ERR_DEVICE, "server stopped after a device error: " + s->error);
     if (s->stop) return fail(REINA_RT_ERR_INVALID_ARG, "server is shutting down");
@@ -412,6 +458,10 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
         s->counts.rejected_busy += 1;
         return fail(REINA_RT_ERR_BUSY, "request queue full");
     }
+    if (s->max_requests && s->pending.size() + s->running >= s->max_requests) {
+        s->counts.rejected_busy += 1;
+        return fail(REINA_RT_ERR_BUSY, "server holds max_requests requests");
+    }
     const uint64_t tk = s->next_ticket++;
     Request r;
     r.scene = scene;
@@ -423,6 +473,7 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     r.img = (float4*)image;
     r.npx = npx;
     r.ntiles = ntiles;
+    r.work = (double)npx * r.P.spp * (r.P.maxBounces ? r.P.maxBounces : 1u);
     r.submit_s = s->now();
     s->reqs[tk] = r;
     s->pending.push_back(tk);

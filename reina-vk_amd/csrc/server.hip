@@ -22,10 +22,12 @@
 namespace rt {
 
 // Queue position base of partition p: p * cap. Request j's slots
-// [px_base, px_base + npx) are tile aligned, so the 256 pixels of a block
-// share one tile and one partition: one atomic per block.
+// [px_base, px_base + npx) are tile aligned and a tile holds at least 256
+// slots, so the 256 pixels of a block share one tile and one partition
+// ((slot >> tile_log2) & 7): one atomic per block.
 __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j, PathState S, Queues Q,
-                                                  Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done) {
+                                                  Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done,
+                                                  uint32_t tile_log2) {
     __shared__ uint32_t s_wave[4];
     __shared__ uint32_t s_base;
     const JobRec& J = jobs[j];
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(256) void k_init_job(const JobRec* jobs, uint32_t j
     const uint32_t before = (uint32_t)__popcll(mask & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
     if (lane == 0) s_wave[wave] = (uint32_t)__popcll(mask);
     __syncthreads();
-    const uint32_t p = ((J.px_base + blockIdx.x * blockDim.x) >> 10) & (kParts - 1u);
+    const uint32_t p = ((J.px_base + blockIdx.x * blockDim.x) >> tile_log2) & (kParts - 1u);
     if (threadIdx.x == 0) {
         uint32_t tot = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
         s_base = tot ? atomicAdd(&ctrl[cslot].count[p][0], tot) : 0u;
@@ -185,9 +187,12 @@ __global__ __launch_bounds__(256) void k_finish_job(const JobRec* jobs, uint32_t
 }
 
 hipError_t launch_init_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, const Queues& Q,
-                           Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, hipStream_t st) {
+                           Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, uint32_t tile_log2,
+                           hipStream_t st) {
     if (npx == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init_job, dim3((npx + 255) / 256), dim3(256), 0, st, jobs, j, S, Q, ctrl, cslot, in_q, done);
+    if (tile_log2 < 8 || tile_log2 > 12) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_init_job, dim3((npx + 255) / 256), dim3(256), 0, st, jobs, j, S, Q, ctrl, cslot, in_q, done,
+                       tile_log2);
     return hipGetLastError();
 }
 

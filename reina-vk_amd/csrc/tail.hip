@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
         HitRec h;
         TravState ts;
         h.t = 0.0f; h.prim = 0u; h.u = 0.0f; h.v = 0.0f;
-        ts.cur = 0; ts.sp = 0;
+        trav_begin(ts);
         for (;;) {
             if (!have && !drained) {
                 for (uint32_t k = 0; k < kParts && !have; ++k) {
@@ -100,8 +100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
                 if (fresh || tracing) tray_init(r, ra, rb);
                 if (fresh) {
                     h = miss_record(r);
-                    ts.cur = 0;
-                    ts.sp = 0;
+                    trav_begin(ts);
                     tracing = r.valid && num_nodes != 0;
                     ready = !tracing;
                     fresh = false;
@@ -190,8 +189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
         if (have) {
             tray_init(r, ra, rb);
             h = miss_record(r);
-            ts.cur = 0;
-            ts.sp = 0;
+            trav_begin(ts);
             tracing = r.valid && num_nodes != 0;
             nrays++;
         }
@@ -216,8 +214,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
                 tray_init(r, make_float4(res.so.x, res.so.y, res.so.z, 0.0f),
                           make_float4(res.sd.x, res.sd.y, res.sd.z, res.stmax));
                 hs = miss_record(r);
-                ts.cur = 0;
-                ts.sp = 0;
+                trav_begin(ts);
                 nshadow++;
                 if (!r.valid || num_nodes == 0) sh = false;
             }

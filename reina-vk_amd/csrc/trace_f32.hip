@@ -16,10 +16,10 @@ static void launch_h16_sd(const SceneView& sv, const float4* rays, float4* hits,
                           hipStream_t st) {
     if (count)
         hipLaunchKernelGGL((k_trace<SD, 1, true, false, false, false, true>), dim3(tu.grid), dim3(256), 0, st, sv, rays,
-                           hits, in, stats, counters, tu.refill_min, nullptr, nullptr, AnyHitOut{});
+                           hits, in, stats, counters, tu.refill_min, tu.claim, nullptr, nullptr, AnyHitOut{});
     else
         hipLaunchKernelGGL((k_trace<SD, 1, false, false, false, false, true>), dim3(tu.grid), dim3(256), 0, st, sv,
-                           rays, hits, in, stats, counters, tu.refill_min, nullptr, nullptr, AnyHitOut{});
+                           rays, hits, in, stats, counters, tu.refill_min, tu.claim, nullptr, nullptr, AnyHitOut{});
 }
 
 hipError_t launch_trace_f32_h16(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,

@@ -24,7 +24,7 @@ __device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uin
 
 // Closest-hit kernel, persistent with per-lane refill: a lane whose ray is
 // finished goes idle; once at least `refill_min` lanes of the wave are idle
-// they take the next rays from the wave's current chunk (kChunk rays claimed
+// they take the next rays from the wave's current chunk (`chunk` rays claimed
 // with one atomic on its partition's counter, own XCD's partition first).
 // Each loop iteration advances every active lane by one node visit.
 // MULTI: the rays belong to several requests (possibly different scenes):
@@ -42,10 +42,9 @@ struct AnyHitOut {
 template <int SD, int WAVES, bool COUNT, bool Q, bool MULTI, bool ANY = false, bool H16 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_trace(
     SceneView sv, const float4* __restrict__ rays, float4* __restrict__ hits, TraceInput in,
-    unsigned long long* stats, uint32_t* counters, uint32_t refill_min, const uint32_t* __restrict__ pix,
-    const JobRec* __restrict__ jobs, AnyHitOut ao) {
+    unsigned long long* stats, uint32_t* counters, uint32_t refill_min, uint32_t chunk,
+    const uint32_t* __restrict__ pix, const JobRec* __restrict__ jobs, AnyHitOut ao) {
     __shared__ uint2 stack[SD * 256];
-    constexpr uint32_t kChunk = 128;
     uint32_t nshadow = 0;
     if (!ANY && stats && blockIdx.x == 0 && threadIdx.x == 0) {
         uint32_t tot = 0;
@@ -87,10 +86,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                         part_range(in, p, a, sz);
                         uint32_t* f = in.fetch + p * kLine;
                         if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sz) continue;
-                        uint32_t b = atomicAdd(f, kChunk);
+                        uint32_t b = atomicAdd(f, chunk);
                         if (b >= sz) continue;
                         got_start = a + b;
-                        got_end = a + min(b + kChunk, sz);
+                        got_end = a + min(b + chunk, sz);
                         got_part = p;
                         break;
                     }
@@ -120,8 +119,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
                         }
                         tray_init<H16>(r, rays[2 * idx], rays[2 * idx + 1]);
                         h = miss_record(r);
-                        ts.cur = 0;
-                        ts.sp = 0;
+                        trav_begin(ts);
                         nv = 0;
                         nt = 0;
                         active = true;
@@ -195,14 +193,14 @@ static void launch_trace_sdw(const SceneView& sv, const float4* rays, float4* hi
     dim3 g(tu.grid), b(256);
     if constexpr (MULTI) {
         hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, true>), g, b, 0, st, sv, rays, hits, in, stats, counters,
-                           tu.refill_min, pix, jobs, AnyHitOut{});
+                           tu.refill_min, tu.claim, pix, jobs, AnyHitOut{});
     } else {
         if (count)
             hipLaunchKernelGGL((k_trace<SD, WAVES, true, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
-                               counters, tu.refill_min, pix, jobs, AnyHitOut{});
+                               counters, tu.refill_min, tu.claim, pix, jobs, AnyHitOut{});
         else
             hipLaunchKernelGGL((k_trace<SD, WAVES, false, Q, false>), g, b, 0, st, sv, rays, hits, in, stats,
-                               counters, tu.refill_min, pix, jobs, AnyHitOut{});
+                               counters, tu.refill_min, tu.claim, pix, jobs, AnyHitOut{});
     }
 }
 
@@ -236,7 +234,7 @@ static void launch_connect_sd(const SceneView& sv, const Queues& Q_, const PathS
                               unsigned long long* stats, const Tuning& tu, hipStream_t st) {
     TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
     hipLaunchKernelGGL((k_trace<SD, 1, false, Q, false, true>), dim3(tu.grid), dim3(256), 0, st, sv, Q_.sray,
-                       nullptr, tin, stats, nullptr, tu.refill_min, nullptr, nullptr, AnyHitOut{S.st2, Q_.scon});
+                       nullptr, tin, stats, nullptr, tu.refill_min, tu.claim, nullptr, nullptr, AnyHitOut{S.st2, Q_.scon});
 }
 
 template <bool Q>
@@ -259,7 +257,7 @@ static void launch_connect_multi_sd(const JobRec* jobs, const Queues& Q_, const 
                                     hipStream_t st) {
     TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
     hipLaunchKernelGGL((k_trace<SD, 1, false, Q, true, true>), dim3(tu.grid), dim3(256), 0, st, SceneView{}, Q_.sray,
-                       nullptr, tin, stats, nullptr, tu.refill_min, pix, jobs, AnyHitOut{S.st2, Q_.scon});
+                       nullptr, tin, stats, nullptr, tu.refill_min, tu.claim, pix, jobs, AnyHitOut{S.st2, Q_.scon});
 }
 
 template <bool Q>

@@ -4,8 +4,9 @@
 // One ray per lane. Per visited node the lane tests the 8 child boxes (slabs,
 // near/far plane chosen by the direction sign, interval widened by 2^-16 of
 // |t| and of |o/d| so no box holding a hit triangle is rejected), tests the
-// triangles of every hit leaf immediately (watertight test of Woop, Benthin
-// and Wald, JCGT 2013), and turns the hit internal children into one "group"
+// triangles of every hit leaf before the next node's boxes, one leaf per step
+// with the next node's fetch overlapping the last leaf's (watertight test of
+// Woop, Benthin and Wald, JCGT 2013), and turns the hit internal children into one "group"
 // {child_base, internal-slot mask, ordered hit mask}. Children are visited in
 // slot order i ^ oct (oct = ray direction sign bits; the builder places each
 // child in the slot of its octant), so a group's lowest ordered bit is the
@@ -179,32 +180,6 @@ RT_D uint32_t slab8(const TRay& r, const float* NX, const float* FX, const float
     return hitm;
 }
 
-// Slab test of the 8 children; returns the hit mask in slot order (before
-// masking out empty slots); see slab_hit for the two forms.
-template <bool H16 = false>
-RT_D uint32_t node_hits(const TRay& r, const Bvh8Node* node, float tb) {
-    const float* base = reinterpret_cast<const float*>(node);
-    // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
-    const float4* nx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 0 : 8));
-    const float4* fx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 8 : 0));
-    const float4* ny = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 16 : 24));
-    const float4* fy = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 24 : 16));
-    const float4* nz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 32 : 40));
-    const float4* fz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 40 : 32));
-    float NX[8], FX[8], NY[8], FY[8], NZ[8], FZ[8];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        float4 a = nx[h], b = fx[h], c = ny[h], d = fy[h], e = nz[h], f = fz[h];
-        NX[4 * h + 0] = a.x; NX[4 * h + 1] = a.y; NX[4 * h + 2] = a.z; NX[4 * h + 3] = a.w;
-        FX[4 * h + 0] = b.x; FX[4 * h + 1] = b.y; FX[4 * h + 2] = b.z; FX[4 * h + 3] = b.w;
-        NY[4 * h + 0] = c.x; NY[4 * h + 1] = c.y; NY[4 * h + 2] = c.z; NY[4 * h + 3] = c.w;
-        FY[4 * h + 0] = d.x; FY[4 * h + 1] = d.y; FY[4 * h + 2] = d.z; FY[4 * h + 3] = d.w;
-        NZ[4 * h + 0] = e.x; NZ[4 * h + 1] = e.y; NZ[4 * h + 2] = e.z; NZ[4 * h + 3] = e.w;
-        FZ[4 * h + 0] = f.x; FZ[4 * h + 1] = f.y; FZ[4 * h + 2] = f.z; FZ[4 * h + 3] = f.w;
-    }
-    return slab8<H16>(r, NX, FX, NY, FY, NZ, FZ, tb);
-}
-
 // Byte j of w as float times s plus p, one rounding (q * s is exact).
 RT_D float dq8(uint32_t w, int j, float s, float p) {
     return __builtin_fmaf((float)((w >> (8 * j)) & 0xFFu), s, p);
@@ -241,15 +216,66 @@ RT_D uint32_t permute8(uint32_t m, uint32_t oct) {
     return m;
 }
 
-// Traversal state of one lane between node visits.
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+
+// Traversal state of one lane between steps.
 struct TravState {
-    uint32_t cur;   // next node to visit
-    int sp;         // LDS stack depth
+    uint32_t cur;    // next node to visit (kNoNode: none left)
+    int sp;          // LDS stack depth
+    uint32_t pm;     // hit leaves of the last visited node not yet tested (ordered slot mask)
+    uint32_t pb;     // that node's first triangle
+    uint32_t pml, pmh;   // its leaf records, 8 bits per slot: count << 5 | offset
 };
 
-// One node visit: box tests of the 8 children, the triangles of every hit
-// leaf, then the next node (first child of this node's group, else the top
-// group of the stack). Returns false when the traversal is finished.
+RT_D void trav_begin(TravState& ts) {
+    ts.cur = 0;
+    ts.sp = 0;
+    ts.pm = 0;
+    ts.pb = 0;
+    ts.pml = 0;
+    ts.pmh = 0;
+}
+
+// Triangle test of one leaf triangle against the closest hit so far;
+// ANY: returns true when the hit ends a shadow ray (see trav_step).
+template <bool ANY>
+RT_D bool leaf_tri(const TRay& r, float4 p0, float4 p1, float4 p2, HitRec& h, const uint8_t* tflags) {
+    const uint32_t id = __float_as_uint(p0.w);
+    float t, u, v;
+    if constexpr (ANY) {
+        if (wtest(r, p0, p1, p2, h.t, t, u, v)) {
+            const uint8_t fl = tflags[id];
+            bool pass = false;
+            if (fl & kTriCull) {
+                const float ex = p1.x - p0.x, ey = p1.y - p0.y, ez = p1.z - p0.z;
+                const float fx = p2.x - p0.x, fy = p2.y - p0.y, fz = p2.z - p0.z;
+                float g = (r.dx * (ey * fz - ez * fy) + r.dy * (ez * fx - ex * fz)) + r.dz * (ex * fy - ey * fx);
+                if (fl & kTriMirror) g = -g;
+                pass = !(g < 0.0f);   // back face of a culling instance
+            }
+            if (!pass) {
+                h.t = t; h.prim = id; h.u = u; h.v = v;
+                return true;
+            }
+        }
+        return false;
+    } else {
+        if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
+            h.t = t; h.prim = id; h.u = u; h.v = v;
+        }
+        return false;
+    }
+}
+
+// One traversal step of a lane: the triangles of one hit leaf of the last
+// visited node (at most 2, fetched together), and, once no other leaf is
+// pending, the next node (box tests of its 8 children). Both are
+// fetched before either is tested, so the triangle and node fetches overlap;
+// the leaf is tested first, so its hit shortens the node's box tests exactly
+// as when a node's leaves are tested before the next node is visited (the
+// node visits are the same). The next node is the first child of the node's
+// group, else the top group of the stack. Returns false when the traversal
+// is finished (no node and no leaf left, or an ANY hit).
 // `stk` points at this lane's first LDS stack entry; entries are `stride`
 // uint2 apart.
 // Q selects the node format: false = FP32 Bvh8Node, true = QBvh8Node.
@@ -261,76 +287,112 @@ struct TravState {
 template <bool COUNT, bool Q, bool ANY = false, bool H16 = false>
 RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec& h, TravState& ts, uint2* stk,
                     uint32_t stride, uint32_t& nvis, uint32_t& ntst, const uint8_t* tflags = nullptr) {
-    uint4 hdr;
-    uint32_t hitm;
-    if constexpr (Q) {
-        const uint4* qn = reinterpret_cast<const uint4*>(static_cast<const QBvh8Node*>(nodes) + ts.cur);
-        const uint4 h0 = qn[0];
-        hdr = qn[1];
-        hitm = node_hits_q8(r, h0, qn[2], qn[3], qn[4], h.t);
-    } else {
-        const Bvh8Node* node = static_cast<const Bvh8Node*>(nodes) + ts.cur;
-        hdr = *reinterpret_cast<const uint4*>(reinterpret_cast<const float*>(node) + 48);
-        hitm = node_hits<H16>(r, node, h.t);
+    // fetch the first two triangles of the lowest pending leaf; a third one
+    // stays pending as a one-triangle leaf (its record rewritten) for the next step
+    const bool do_leaf = ts.pm != 0u;
+    uint32_t cnt = 0;
+    float4 a0, a1, a2, b0, b1, b2;
+    if (do_leaf) {
+        const uint32_t k = (uint32_t)__builtin_ctz(ts.pm) ^ r.oct;
+        const uint32_t sh = 8u * (k & 3u);
+        const uint32_t m = ((k < 4 ? ts.pml : ts.pmh) >> sh) & 0xFFu;
+        const uint32_t off = m & 31u;
+        cnt = (m >> 5) & 3u;
+        if (cnt > 2u) {
+            const uint32_t rest = (1u << 5) | (off + 2u);   // offsets stay below 24 (8 leaves x 3)
+            if (k < 4) ts.pml = (ts.pml & ~(0xFFu << sh)) | (rest << sh);
+            else ts.pmh = (ts.pmh & ~(0xFFu << sh)) | (rest << sh);
+            cnt = 2u;
+        } else {
+            ts.pm &= ts.pm - 1u;
+        }
+        const float4* tv = tris + 3ull * (ts.pb + off);
+        if (cnt > 0u) { a0 = tv[0]; a1 = tv[1]; a2 = tv[2]; }
+        if (cnt > 1u) { b0 = tv[3]; b1 = tv[4]; b2 = tv[5]; }
     }
-    const uint32_t child_base = hdr.x & 0xFFFFFFu, tri_base = hdr.y & 0xFFFFFFu;
-    const uint32_t imask = hdr.x >> 24, leafmask = hdr.y >> 24;
-    const uint32_t meta_lo = hdr.z, meta_hi = hdr.w;
-    if (COUNT) nvis++;
-    uint32_t om = permute8(hitm & imask, r.oct);
-    uint32_t lm = permute8(hitm & leafmask, r.oct);
-    // leaves of this node, in slot order i ^ oct
-    while (lm) {
-        uint32_t i = __builtin_ctz(lm);
-        lm &= lm - 1u;
-        uint32_t k = i ^ r.oct;
-        uint32_t m = ((k < 4 ? meta_lo : meta_hi) >> (8 * (k & 3))) & 0xFFu;
-        uint32_t cnt = (m >> 5) & 3u, off = m & 31u;
-        for (uint32_t j = 0; j < cnt; ++j) {
-            const float4* tv = tris + 3ull * (tri_base + off + j);
-            float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
-            uint32_t id = __float_as_uint(p0.w);
-            float t, u, v;
-            if (COUNT) ntst++;
-            if constexpr (ANY) {
-                if (wtest(r, p0, p1, p2, h.t, t, u, v)) {
-                    const uint8_t fl = tflags[id];
-                    bool pass = false;
-                    if (fl & kTriCull) {
-                        const float ex = p1.x - p0.x, ey = p1.y - p0.y, ez = p1.z - p0.z;
-                        const float fx = p2.x - p0.x, fy = p2.y - p0.y, fz = p2.z - p0.z;
-                        float g = (r.dx * (ey * fz - ez * fy) + r.dy * (ez * fx - ex * fz)) + r.dz * (ex * fy - ey * fx);
-                        if (fl & kTriMirror) g = -g;
-                        pass = !(g < 0.0f);   // back face of a culling instance
-                    }
-                    if (!pass) {
-                        h.t = t; h.prim = id; h.u = u; h.v = v;
-                        return false;
-                    }
-                }
-            } else if (wtest(r, p0, p1, p2, h.t, t, u, v) && ((t < h.t) || (t == h.t && id < h.prim))) {
-                h.t = t; h.prim = id; h.u = u; h.v = v;
-            }
+    // the next node, once no leaf is left pending
+    const bool do_node = ts.cur != kNoNode && ts.pm == 0u;
+    // fetch the node
+    uint4 hdr = make_uint4(0u, 0u, 0u, 0u);
+    uint4 q0 = hdr, qx = hdr, qy = hdr, qz = hdr;
+    float4 pl[12];
+    if (do_node) {
+        if constexpr (Q) {
+            const uint4* qn = reinterpret_cast<const uint4*>(static_cast<const QBvh8Node*>(nodes) + ts.cur);
+            q0 = qn[0];
+            hdr = qn[1];
+            qx = qn[2];
+            qy = qn[3];
+            qz = qn[4];
+        } else {
+            const float* base = reinterpret_cast<const float*>(static_cast<const Bvh8Node*>(nodes) + ts.cur);
+            hdr = *reinterpret_cast<const uint4*>(base + 48);
+            // near/far planes per axis by direction sign (lo <= hi, so this equals min/max)
+            const float4* nx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 0 : 8));
+            const float4* fx = reinterpret_cast<const float4*>(base + (r.ix >= 0.0f ? 8 : 0));
+            const float4* ny = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 16 : 24));
+            const float4* fy = reinterpret_cast<const float4*>(base + (r.iy >= 0.0f ? 24 : 16));
+            const float4* nz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 32 : 40));
+            const float4* fz = reinterpret_cast<const float4*>(base + (r.iz >= 0.0f ? 40 : 32));
+            pl[0] = nx[0]; pl[1] = nx[1]; pl[2] = fx[0]; pl[3] = fx[1];
+            pl[4] = ny[0]; pl[5] = ny[1]; pl[6] = fy[0]; pl[7] = fy[1];
+            pl[8] = nz[0]; pl[9] = nz[1]; pl[10] = fz[0]; pl[11] = fz[1];
         }
     }
-    if (om) {
-        uint32_t i = __builtin_ctz(om);
-        om &= om - 1u;
-        uint32_t k = i ^ r.oct;
-        if (om) { stk[ts.sp * stride] = make_uint2(child_base, imask | (om << 8)); ts.sp++; }
-        ts.cur = child_base + __builtin_popcount(imask & ((1u << k) - 1u));
-        return true;
+    // test the leaf, in triangle order
+    if (do_leaf) {
+        if (COUNT) ntst += cnt;
+        if (cnt > 0u && leaf_tri<ANY>(r, a0, a1, a2, h, tflags)) return false;
+        if (cnt > 1u && leaf_tri<ANY>(r, b0, b1, b2, h, tflags)) return false;
     }
-    if (ts.sp == 0) return false;
-    uint2 g = stk[(ts.sp - 1) * stride];
-    uint32_t gm = g.y >> 8, gi = g.y & 0xFFu;
-    uint32_t i = __builtin_ctz(gm);
-    gm &= gm - 1u;
-    uint32_t k = i ^ r.oct;
-    ts.cur = g.x + __builtin_popcount(gi & ((1u << k) - 1u));
-    if (gm) stk[(ts.sp - 1) * stride] = make_uint2(g.x, gi | (gm << 8));
-    else ts.sp--;
-    return true;
+    // test the node; its hit leaves become pending
+    if (do_node) {
+        if (COUNT) nvis++;
+        uint32_t hitm;
+        if constexpr (Q) {
+            hitm = node_hits_q8(r, q0, qx, qy, qz, h.t);
+        } else {
+            float NX[8], FX[8], NY[8], FY[8], NZ[8], FZ[8];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const float4 a = pl[j], b = pl[2 + j], c = pl[4 + j], d = pl[6 + j], e = pl[8 + j], f = pl[10 + j];
+                NX[4 * j + 0] = a.x; NX[4 * j + 1] = a.y; NX[4 * j + 2] = a.z; NX[4 * j + 3] = a.w;
+                FX[4 * j + 0] = b.x; FX[4 * j + 1] = b.y; FX[4 * j + 2] = b.z; FX[4 * j + 3] = b.w;
+                NY[4 * j + 0] = c.x; NY[4 * j + 1] = c.y; NY[4 * j + 2] = c.z; NY[4 * j + 3] = c.w;
+                FY[4 * j + 0] = d.x; FY[4 * j + 1] = d.y; FY[4 * j + 2] = d.z; FY[4 * j + 3] = d.w;
+                NZ[4 * j + 0] = e.x; NZ[4 * j + 1] = e.y; NZ[4 * j + 2] = e.z; NZ[4 * j + 3] = e.w;
+                FZ[4 * j + 0] = f.x; FZ[4 * j + 1] = f.y; FZ[4 * j + 2] = f.z; FZ[4 * j + 3] = f.w;
+            }
+            hitm = slab8<H16>(r, NX, FX, NY, FY, NZ, FZ, h.t);
+        }
+        const uint32_t child_base = hdr.x & 0xFFFFFFu;
+        const uint32_t imask = hdr.x >> 24, leafmask = hdr.y >> 24;
+        ts.pm = permute8(hitm & leafmask, r.oct);
+        ts.pb = hdr.y & 0xFFFFFFu;
+        ts.pml = hdr.z;
+        ts.pmh = hdr.w;
+        uint32_t om = permute8(hitm & imask, r.oct);
+        if (om) {
+            const uint32_t i = __builtin_ctz(om);
+            om &= om - 1u;
+            const uint32_t k = i ^ r.oct;
+            if (om) { stk[ts.sp * stride] = make_uint2(child_base, imask | (om << 8)); ts.sp++; }
+            ts.cur = child_base + __builtin_popcount(imask & ((1u << k) - 1u));
+        } else if (ts.sp == 0) {
+            ts.cur = kNoNode;
+        } else {
+            const uint2 g = stk[(ts.sp - 1) * stride];
+            uint32_t gm = g.y >> 8;
+            const uint32_t gi = g.y & 0xFFu;
+            const uint32_t i = __builtin_ctz(gm);
+            gm &= gm - 1u;
+            const uint32_t k = i ^ r.oct;
+            ts.cur = g.x + __builtin_popcount(gi & ((1u << k) - 1u));
+            if (gm) stk[(ts.sp - 1) * stride] = make_uint2(g.x, gi | (gm << 8));
+            else ts.sp--;
+        }
+    }
+    return ts.cur != kNoNode || ts.pm != 0u;
 }
 
 RT_D HitRec miss_record(const TRay& r) {

@@ -94,6 +94,7 @@ struct Tuning {
     uint32_t shade_grid;    // blocks of the shade kernel
     uint32_t waves_per_simd;  // register budget of the traversal kernel: 0 (compiler's choice), 5, 6 or 8
     uint32_t tail_paths;      // live paths at which the frame ends in one tail launch (0: never)
+    uint32_t claim = 128;     // rays a traversal wave claims with one atomic: 64, 128, 256 or 512
 };
 
 // Rays to trace: kParts partitions. Partitioned mode (counts != null):
@@ -121,8 +122,11 @@ hipError_t launch_trace_multi_f32(const float4* rays, float4* hits, const TraceI
                                   const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st);
 hipError_t launch_trace_multi_q8(const float4* rays, float4* hits, const TraceInput& in, unsigned long long* stats,
                                  const Tuning& tu, const uint32_t* pix, const JobRec* jobs, hipStream_t st);
+// tile_log2: log2 of the server's slot tile (8..12); slot l starts in
+// partition (l >> tile_log2) & 7.
 hipError_t launch_init_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, const Queues& Q,
-                           Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, hipStream_t st);
+                           Ctrl* ctrl, uint32_t cslot, uint32_t in_q, uint32_t* done, uint32_t tile_log2,
+                           hipStream_t st);
 // nee: the server holds next-event-estimation state (S.st4, Q.sray, Q.scon);
 // each request's TraceParams.nee then selects NEE for its rays, and every
 // queue entry gets a shadow ray or an empty one.

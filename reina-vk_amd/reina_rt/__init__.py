@@ -143,13 +143,14 @@ class DeviceScene:
         t = abi.ReinaTuning()
         check(lib().reina_rt_get_tuning(self.handle, ctypes.byref(t)))
         return dict(stack_depth=t.stack_depth, refill_min=t.refill_min, grid_blocks=t.grid_blocks,
-                    shade_blocks=t.shade_blocks, waves_per_simd=t.waves_per_simd, tail_paths=t.tail_paths)
+                    shade_blocks=t.shade_blocks, waves_per_simd=t.waves_per_simd, tail_paths=t.tail_paths,
+                    claim_rays=t.claim_rays)
 
     def set_tuning(self, **kw):
         cur = self.get_tuning()
         cur.update(kw)
         t = abi.ReinaTuning(cur["stack_depth"], cur["refill_min"], cur["grid_blocks"], cur["shade_blocks"],
-                            cur["waves_per_simd"], cur["tail_paths"])
+                            cur["waves_per_simd"], cur["tail_paths"], cur["claim_rays"])
         check(lib().reina_rt_set_tuning(self.handle, ctypes.byref(t)))
 
     def set_trace_options(self, options):
@@ -180,10 +181,15 @@ class DeviceScene:
 class Server:
     """Batching / serving request loop (reina_rt_server_*): render requests
     on resident DeviceScenes share one wavefront; submit() returns a ticket
-    at once, wait()/poll() report completion with timestamps."""
+    at once, wait()/poll() report completion with timestamps.
+
+    slot_tile, graph_iters, admit_order (abi.ADMIT_FIFO / ADMIT_SHORTEST),
+    max_wait_ms and max_requests are ReinaServerDesc's fields (0 = default);
+    the remaining keyword arguments are ReinaTuning fields."""
 
     def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, nee=False,
-                 admit_paths=0, **tuning):
+                 admit_paths=0, slot_tile=0, graph_iters=0, admit_order=0, max_wait_ms=0, max_requests=0,
+                 **tuning):
         d = abi.ReinaServerDesc()
         d.struct_size = ctypes.sizeof(abi.ReinaServerDesc)
         d.device = device
@@ -192,6 +198,11 @@ class Server:
         d.quantized = int(bool(quantized))
         d.flags = abi.SERVER_NEE if nee else 0
         d.admit_paths = int(admit_paths)
+        d.slot_tile = int(slot_tile)
+        d.graph_iters = int(graph_iters)
+        d.admit_order = int(admit_order)
+        d.max_wait_ms = int(max_wait_ms)
+        d.max_requests = int(max_requests)
         for k, v in tuning.items():
             setattr(d.tuning, k, v)
         h = ctypes.c_void_p()

@@ -138,14 +138,17 @@ class ReinaTraceStats(ctypes.Structure):
 class ReinaTuning(ctypes.Structure):
     _fields_ = [("stack_depth", ctypes.c_uint32), ("refill_min", ctypes.c_uint32),
                 ("grid_blocks", ctypes.c_uint32), ("shade_blocks", ctypes.c_uint32),
-                ("waves_per_simd", ctypes.c_uint32), ("tail_paths", ctypes.c_uint32)]
+                ("waves_per_simd", ctypes.c_uint32), ("tail_paths", ctypes.c_uint32),
+                ("claim_rays", ctypes.c_uint32)]
 
 
 class ReinaServerDesc(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("path_budget_bytes", ctypes.c_uint64), ("max_pending", ctypes.c_uint32),
                 ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning), ("flags", ctypes.c_uint32),
-                ("admit_paths", ctypes.c_uint32)]
+                ("admit_paths", ctypes.c_uint32), ("slot_tile", ctypes.c_uint32), ("graph_iters", ctypes.c_uint32),
+                ("admit_order", ctypes.c_uint32), ("max_wait_ms", ctypes.c_uint32),
+                ("max_requests", ctypes.c_uint32)]
 
 
 class ReinaRequestStatus(ctypes.Structure):
@@ -201,3 +204,5 @@ BUILD_DEVICE = 0x1   # the default (flags 0 builds on the device too)
 BUILD_HOST = 0x4
 NODES_QUANTIZED = 0x2
 ADMIT_ALL = 0xFFFFFFFF
+ADMIT_FIFO = 0
+ADMIT_SHORTEST = 1

@@ -12,6 +12,11 @@ candidate settings (the best of stage 1, the defaults, each with several
 tail-launch thresholds) and installs the fastest: a frame's bounce launches
 shrink as pixels finish, so the best setting for one large batch is not
 always the best for a frame.
+
+tune_server searches a server's launch parameters (the ReinaTuning fields,
+the slot tile and the graph length) for a batch of requests over a scene
+pool, one parameter at a time from the defaults, by the wall time of the
+whole batch through one server.
 """
 import time
 import itertools
@@ -68,3 +73,65 @@ def tune_frame(dev, torch, frame, candidates, reps=2):
     win = min(table, key=lambda r: r["ms"])
     dev.set_tuning(**{k: v for k, v in win.items() if k != "ms"})
     return win, table
+
+
+SERVER_AXES = (("refill_min", (16, 32)), ("grid_blocks", ("4cu", "8cu")), ("waves_per_simd", (0, 8)),
+               ("tail_paths", (1 << 19, 1 << 20, 1 << 21)), ("claim_rays", (128, 64, 256)),
+               ("slot_tile", (1024, 256, 4096)), ("graph_iters", (6, 12, 24)))
+
+
+def server_stack_depth(depths):
+    """Smallest server stack depth (8, 16, 24, 32) that serves scenes of
+    these BVH depths (a scene needs depth - 1 entries)."""
+    need = max(max(depths) - 1, 1)
+    return next(d for d in (8, 16, 24, 32) if d >= need)
+
+
+def tune_server(torch, scenes_reqs, server_kw, cus=None, reps=1, axes=SERVER_AXES):
+    """scenes_reqs: list of (DeviceScene, push constants, W, H), rendered as
+    one batch (all submitted at once) per candidate. server_kw: the fixed
+    Server arguments (path_budget_bytes, ...). Coordinate search: start from
+    the defaults, sweep one axis at a time holding the others at the best so
+    far, keep a value only if the batch is faster. Returns (best, default,
+    table); best and default are Server keyword dicts plus "ms"."""
+    from . import Server
+    if cus is None:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+    sd = server_stack_depth([int(d.info().max_depth) for d, _, _, _ in scenes_reqs])
+    imgs = [torch.empty((H, W, 4), dtype=torch.float32, device="cuda") for _, _, W, H in scenes_reqs]
+    torch.cuda.synchronize()
+
+    def run(cand):
+        kw = dict(server_kw, **cand)
+        best = None
+        srv = Server(**kw)
+        try:
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                tks = [srv.submit(d, pc, img.data_ptr(), W, H) for (d, pc, W, H), img in zip(scenes_reqs, imgs)]
+                for t in tks:
+                    srv.wait(t)
+                ms = (time.perf_counter() - t0) * 1e3
+                best = ms if best is None else min(best, ms)
+        finally:
+            srv.close()
+        return round(best, 2)
+
+    def val(k, v):
+        return cus * int(v[0]) if k == "grid_blocks" else v
+
+    cur = {"stack_depth": sd, "shade_blocks": cus * 16}
+    for k, vals in axes:
+        cur[k] = val(k, vals[0])
+    default = dict(cur, ms=run(cur))
+    table = [default]
+    best = default
+    for k, vals in axes:
+        for v in vals[1:]:
+            c = {kk: vv for kk, vv in best.items() if kk != "ms"}
+            c[k] = val(k, v)
+            r = dict(c, ms=run(c))
+            table.append(r)
+            if r["ms"] < best["ms"]:
+                best = r
+    return best, default, table

@@ -67,16 +67,18 @@ def _pcts(lat):
             "mean_ms": round(float(a.mean()), 2), "max_ms": round(float(a.max()), 2)}
 
 
-def batched(torch, pool, reqs, server_kw, check=True):
+def batched(torch, pool, reqs, server_kw, check=True, plain=True):
     """Every request submitted at once. Plain: reina_rt_trace one after
-    another. Batched: one server run. Same images required."""
-    imgs_p = [torch.zeros((r["H"], r["W"], 4), dtype=torch.float32, device="cuda") for r in reqs]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rays_p = 0
-    for r, img in zip(reqs, imgs_p):
-        rays_p += _plain_one(torch, pool, r, img)
-    t_plain = time.perf_counter() - t0
+    another. Batched: one server run. Same images required. plain=False:
+    the server run only (no "plain" entry, no image comparison)."""
+    rays_p, t_plain, imgs_p = 0, None, None
+    if plain:
+        imgs_p = [torch.zeros((r["H"], r["W"], 4), dtype=torch.float32, device="cuda") for r in reqs]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r, img in zip(reqs, imgs_p):
+            rays_p += _plain_one(torch, pool, r, img)
+        t_plain = time.perf_counter() - t0
     imgs_b = [torch.zeros((r["H"], r["W"], 4), dtype=torch.float32, device="cuda") for r in reqs]
     torch.cuda.synchronize()
     srv = Server(**server_kw)
@@ -89,24 +91,26 @@ def batched(torch, pool, reqs, server_kw, check=True):
     st1 = srv.stats()
     srv.close()
     same = None
-    if check:
+    if check and plain:
         same = all(bool(torch.equal(a.view(torch.int32), b.view(torch.int32))) for a, b in zip(imgs_p, imgs_b))
     rays_b = st1.extension_rays - st0.extension_rays
     lat = [s.done_s - s.submit_s for s in sts]
-    return {
+    out = {
         "requests": len(reqs),
         "pixels": int(sum(r["W"] * r["H"] for r in reqs)),
         "paths": int(sum(r["W"] * r["H"] * r["spp"] for r in reqs)),
-        "plain": {"seconds": round(t_plain, 3), "images_per_s": round(len(reqs) / t_plain, 3),
-                  "mrays_per_s": round(rays_p / t_plain / 1e6, 1), "extension_rays": int(rays_p)},
         "batched": {"seconds": round(t_batch, 3), "images_per_s": round(len(reqs) / t_batch, 3),
                     "mrays_per_s": round(rays_b / t_batch / 1e6, 1), "extension_rays": int(rays_b),
                     "iterations": int(st1.iterations - st0.iterations),
                     "tail_launches": int(st1.tail_launches - st0.tail_launches),
                     "tail_rays": int(st1.tail_rays - st0.tail_rays), "latency": _pcts(lat)},
-        "speedup_images_per_s": round(t_plain / t_batch, 3),
-        "images_bit_identical": same,
     }
+    if plain:
+        out["plain"] = {"seconds": round(t_plain, 3), "images_per_s": round(len(reqs) / t_plain, 3),
+                        "mrays_per_s": round(rays_p / t_plain / 1e6, 1), "extension_rays": int(rays_p)}
+        out["speedup_images_per_s"] = round(t_plain / t_batch, 3)
+        out["images_bit_identical"] = same
+    return out
 
 
 def arrivals(rate, duration, rng):
@@ -206,18 +210,25 @@ def served_server(torch, pool, reqs, times, max_pending, server_kw, keep_images=
     return out
 
 
-def served(torch, pool, reqs, rate, duration, max_pending, server_kw, seed=0):
+def served(torch, pool, reqs, rate, duration, max_pending, server_kw, seed=0, variants=None):
     """Open-loop Poisson arrivals at `rate` per second for `duration`
     seconds (requests reqs[0], reqs[1], ... in arrival order), plain and
-    served; offered vs achieved rates and latency percentiles."""
+    served; offered vs achieved rates and latency percentiles. variants:
+    {name: server keyword dict}, each served the same arrivals too."""
     rng = np.random.default_rng(seed)
     times = arrivals(rate, duration, rng)
     reqs = reqs[:len(times)]
     plain = served_plain(torch, pool, reqs, times, max_pending)
     srv = served_server(torch, pool, reqs, times, max_pending, server_kw)
-    return {"rate_per_s": rate, "duration_s": duration, "offered": len(times), "max_pending": max_pending,
-            "overload_policy": "bounded FIFO of max_pending waiting requests; arrivals that find it full "
-                               "are rejected (reina_rt_submit returns REINA_RT_ERR_BUSY)",
-            "plain": plain, "served": srv,
-            "speedup_images_per_s": round(srv["images_per_s"] / plain["images_per_s"], 3)
-            if plain["images_per_s"] else None}
+
+    def ratio(x):
+        return round(x["images_per_s"] / plain["images_per_s"], 3) if plain["images_per_s"] else None
+    out = {"rate_per_s": rate, "duration_s": duration, "offered": len(times), "max_pending": max_pending,
+           "overload_policy": "bounded queue of max_pending waiting requests; arrivals that find it full "
+                              "are rejected (reina_rt_submit returns REINA_RT_ERR_BUSY)",
+           "plain": plain, "served": srv, "speedup_images_per_s": ratio(srv)}
+    for name, kw in (variants or {}).items():
+        v = served_server(torch, pool, reqs, times, max_pending, kw)
+        v["speedup_images_per_s"] = ratio(v)
+        out[name] = v
+    return out

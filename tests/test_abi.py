@@ -189,3 +189,21 @@ def test_bvh_build_host_rejects_null():
     L = reina_rt.lib()
     nb = ctypes.c_uint64(0)
     assert L.reina_rt_bvh_build_host(None, 5, None, ctypes.byref(nb), None, ctypes.byref(nb), None) == abi.ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("field,value,word", [("slot_tile", 300, "slot_tile"), ("slot_tile", 8192, "slot_tile"),
+                                              ("graph_iters", 7, "graph_iters"), ("graph_iters", 30, "graph_iters"),
+                                              ("admit_order", 2, "admit_order")])
+def test_server_create_rejects_bad_fields(field, value, word):
+    """The server's slot tile, graph length and admission order are checked
+    before any device work: INVALID_ARG naming the field, GPU or not."""
+    with pytest.raises(reina_rt.ReinaError) as e:
+        reina_rt.Server(path_budget_bytes=1 << 26, **{field: value})
+    assert e.value.code == abi.ERR_INVALID_ARG and word in str(e.value)
+
+
+def test_server_budget_counts_slot_tiles():
+    """path_budget_bytes must hold 8 slot tiles of the requested size."""
+    with pytest.raises(reina_rt.ReinaError) as e:
+        reina_rt.Server(path_budget_bytes=8 * 4096 * abi.SERVER_BYTES_PER_PIXEL - 1, slot_tile=4096)
+    assert e.value.code == abi.ERR_INVALID_ARG and "8 slot tiles" in str(e.value)

@@ -702,10 +702,17 @@ def test_rmse_vs_converged(cuda, nee):
     (sampleBatch 200000+) against the 65,536-spp FP64 converged render in
     tests/golden (made by tests/golden/make_converged.py). With no bias the
     mean squared error of a frame equals the per-pixel variance of a frame
-    (plus the converged render's own, 1/1024 of a frame's): the measured MSE
-    matches the one predicted from the frames' spread within 2 sigma of its
-    standard error; the relative RMSE is printed. NEE frames are compared
-    with the same BSDF-only converged render (equal expectation)."""
+    (plus the converged render's own, 1/1024 of a frame's).
+
+    Standard-error model: with F frames x_f, their mean xb and unbiased
+    variance v (per pixel and channel), mean_f (x_f - ref)^2 = (F-1)/F v +
+    (xb - ref)^2, so measured minus predicted MSE is the pixel mean of
+    D = (xb - ref)^2 - v / F - var_ref, whose expectation is 0 without bias.
+    Pixels draw independent sample streams, so D's standard error is the
+    spread of its per-pixel values (channels summed) over sqrt(pixels): the
+    difference must lie within 2 of those standard errors. The relative RMSE
+    is printed. NEE frames are compared with the same BSDF-only converged
+    render (equal expectation)."""
     import os
     torch = cuda
     g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell_24x24_converged.npz"))
@@ -727,13 +734,17 @@ def test_rmse_vs_converged(cuda, nee):
     dev.close()
     # sampleBatch > 0 blends with the (zero) previous image: undo the running mean
     fr = np.array([x * (200001 + k) for k, x in enumerate(frames)])
+    F = len(fr)
     v = fr.var(0, ddof=1)
     mse = ((fr - ref) ** 2).mean(axis=(1, 2, 3))
     pred = (v + var_ref).mean()
-    se = mse.std(ddof=1) / np.sqrt(len(mse))
+    d = (((fr.mean(0) - ref) ** 2) - v / F - var_ref).sum(-1)   # per pixel, channels summed
+    assert np.isclose(d.mean() / 3.0, mse.mean() - pred, rtol=1e-6, atol=1e-12)
+    se = d.std(ddof=1) / np.sqrt(d.size) / 3.0
     rrmse = np.sqrt(((fr - ref) ** 2 / (ref ** 2 + 1e-3)).mean())
-    print("nee=%s mse=%.4g pred=%.4g se=%.3g rel_rmse=%.4f" % (nee, mse.mean(), pred, se, rrmse))
-    assert abs(mse.mean() - pred) < 2.0 * se + 0.1 * pred
+    print("nee=%s mse=%.4g pred=%.4g se=%.3g z=%.2f rel_rmse=%.4f" % (
+        nee, mse.mean(), pred, se, (mse.mean() - pred) / se, rrmse))
+    assert abs(mse.mean() - pred) < 2.0 * se
     if not nee:
         # the frames' variance agrees with the converged render's own batch variance
         assert 0.7 < v.mean() / g["var_batch64"].mean() < 1.4

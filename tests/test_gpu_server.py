@@ -80,15 +80,7 @@ def test_server_request_equals_trace_and_oracle(cuda, pool):
     srv.close()
 
 
-@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF, 3000], ids=["tail_default", "tail_off", "tail_3000"])
-def test_server_mix_equals_alone(cuda, pool, tail):
-    """Eight requests on three scenes (different resolutions including a
-    partial last tile, spp, depths, accumulation with sampleBatch > 0)
-    submitted together: each image equals its reina_rt_trace image and its
-    image when served alone, whether the shared queues end with the tail
-    launch at once (default threshold 2^19 paths), never, or after some
-    graph iterations (3000 paths)."""
-    torch = cuda
+def _mix_jobs(pool):
     rng = np.random.default_rng(3)
     jobs = []
     for k, (name, W, H, spp, depth, batch) in enumerate([
@@ -100,6 +92,19 @@ def test_server_mix_equals_alone(cuda, pool, tail):
             else _pc(cam, W, H, spp, depth, batch)
         init = rng.uniform(0, 1, (H, W, 4)).astype(np.float32) if batch else None
         jobs.append((dev, pc, W, H, init))
+    return jobs
+
+
+@pytest.mark.parametrize("tail", [0, abi.TAIL_OFF, 3000], ids=["tail_default", "tail_off", "tail_3000"])
+def test_server_mix_equals_alone(cuda, pool, tail):
+    """Eight requests on three scenes (different resolutions including a
+    partial last tile, spp, depths, accumulation with sampleBatch > 0)
+    submitted together: each image equals its reina_rt_trace image and its
+    image when served alone, whether the shared queues end with the tail
+    launch at once (default threshold 2^19 paths), never, or after some
+    graph iterations (3000 paths)."""
+    torch = cuda
+    jobs = _mix_jobs(pool)
     srv = reina_rt.Server(path_budget_bytes=256 << 20, tail_paths=tail)
     mixed = served(torch, srv, jobs)
     st_mix = srv.stats()
@@ -115,6 +120,65 @@ def test_server_mix_equals_alone(cuda, pool, tail):
     else:
         assert st_mix.tail_launches >= 1 and 0 < st_mix.tail_rays < st_mix.extension_rays
     srv.close()
+
+
+@pytest.mark.parametrize("tile,iters,claim,tail", [(256, 12, 64, 3000), (4096, 24, 512, 0), (512, 18, 256, abi.TAIL_OFF)],
+                         ids=["tile256_g12_c64", "tile4096_g24_c512", "tile512_g18_c256"])
+def test_server_tunables_equal_trace(cuda, pool, tile, iters, claim, tail):
+    """The server's slot tile, graph length and ray-claim chunk change only
+    the schedule: the eight-request mix still gives every request its
+    reina_rt_trace image, the path pool is counted in tiles of that size,
+    and iterations advance a graph at a time."""
+    torch = cuda
+    jobs = _mix_jobs(pool)
+    budget = 256 << 20
+    srv = reina_rt.Server(path_budget_bytes=budget, slot_tile=tile, graph_iters=iters, claim_rays=claim,
+                          tail_paths=tail)
+    got = served(torch, srv, jobs)
+    st = srv.stats()
+    srv.close()
+    for (dev, pc, W, H, init), g in zip(jobs, got):
+        assert _same(g, plain(torch, dev, pc, W, H, init))
+    tiles = budget // (abi.SERVER_BYTES_PER_PIXEL * tile)
+    assert st.slot_tiles_total == tiles - tiles % 8 and st.slot_tiles_used == 0
+    assert st.iterations == iters * st.graph_replays and st.completed == len(jobs)
+
+
+@pytest.mark.parametrize("max_wait_ms", [60000, 1], ids=["shortest", "aged_fifo"])
+def test_server_admit_shortest(cuda, pool, max_wait_ms):
+    """REINA_RT_ADMIT_SHORTEST with admit_paths = 1 (one request at a time):
+    three requests submitted while a long one runs are admitted in order of
+    estimated work (pixels x spp x depth) when none has waited max_wait_ms;
+    once all have waited that long, in submission order. Images equal
+    reina_rt_trace's either way."""
+    import time
+    torch = cuda
+    _, icam, idev = pool["ico"]
+    jobs = []   # estimated work: 96*96*4*8 = 294912, 32*32*2*8 = 16384, 64*48*3*12 = 110592
+    for name, W, H, spp, depth in (("ico", 96, 96, 4, 8), ("cornell", 32, 32, 2, 8), ("inst", 64, 48, 3, 12)):
+        _, cam, dev = pool[name]
+        pc = scenes.cornell_push(W, H, spp=spp, max_bounces=depth) if name == "cornell" else _pc(cam, W, H, spp, depth)
+        jobs.append((dev, pc, W, H))
+    imgs = [torch.zeros((H, W, 4), dtype=torch.float32, device="cuda") for _, _, W, H in jobs]
+    bimg = torch.zeros((1024, 1024, 4), dtype=torch.float32, device="cuda")
+    bpc = _pc(icam, 1024, 1024, 64, 8)
+    torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's stream)
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, admit_order=abi.ADMIT_SHORTEST,
+                          max_wait_ms=max_wait_ms)
+    btk = srv.submit(idev, bpc, bimg.data_ptr(), 1024, 1024)
+    for _ in range(5000):
+        if srv.poll(btk).state == abi.REQ_RUNNING:
+            break
+        time.sleep(0.001)
+    tks = [srv.submit(dev, pc, img.data_ptr(), W, H) for (dev, pc, W, H), img in zip(jobs, imgs)]
+    bst = srv.wait(btk)
+    sts = [srv.wait(t) for t in tks]
+    srv.close()
+    assert all(s.submit_s < bst.done_s for s in sts)   # all were waiting while the long request ran
+    order = sorted(range(3), key=lambda k: sts[k].admit_s)
+    assert order == ([1, 2, 0] if max_wait_ms > 1000 else [0, 1, 2]), [s.admit_s for s in sts]
+    for (dev, pc, W, H), img in zip(jobs, imgs):
+        assert _same(img.cpu().numpy(), plain(torch, dev, pc, W, H))
 
 
 def test_server_admit_paths(cuda, pool):
@@ -180,6 +244,38 @@ def test_server_bounded_queue(cuda, pool):
     for t in tickets:
         assert srv.wait(t).state == abi.REQ_DONE
     assert srv.stats().rejected_busy == busy
+    srv.close()
+
+
+def test_server_max_requests(cuda, pool):
+    """max_requests = 3 bounds queued + running requests: with one long
+    request running (admit_paths = 1 keeps the others queued), two more are
+    accepted and the fourth submit returns BUSY, though max_pending is 16;
+    once all complete the server accepts again."""
+    import time
+    torch = cuda
+    arrays, cam, dev = pool["ico"]
+    imgs = [torch.zeros((512, 512, 4) if k == 0 else (256, 256, 4), dtype=torch.float32, device="cuda")
+            for k in range(5)]
+    pcs = [_pc(cam, 512, 512, 256, 8) if k == 0 else _pc(cam, 256, 256, 1, 8) for k in range(5)]
+    torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's stream)
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, max_pending=16, max_requests=3, admit_paths=1)
+    t0 = srv.submit(dev, pcs[0], imgs[0].data_ptr(), 512, 512)
+    for _ in range(5000):
+        if srv.poll(t0).state == abi.REQ_RUNNING:
+            break
+        time.sleep(0.001)
+    tickets = [t0, srv.submit(dev, pcs[1], imgs[1].data_ptr(), 256, 256),
+               srv.submit(dev, pcs[2], imgs[2].data_ptr(), 256, 256)]
+    with pytest.raises(reina_rt.ReinaError) as e:
+        srv.submit(dev, pcs[3], imgs[3].data_ptr(), 256, 256)
+    assert e.value.code == abi.ERR_BUSY and "max_requests" in str(e.value)
+    assert srv.poll(t0).state == abi.REQ_RUNNING   # the bound was hit while the long request ran
+    for t in tickets:
+        assert srv.wait(t).state == abi.REQ_DONE
+    t4 = srv.submit(dev, pcs[4], imgs[4].data_ptr(), 256, 256)
+    assert srv.wait(t4).state == abi.REQ_DONE
+    assert srv.stats().rejected_busy == 1
     srv.close()
 
 
@@ -330,3 +426,23 @@ def test_server_nee_refusals(cuda, pool):
     srvn.close()
     dark.close()
     c_nee.close()
+
+
+def test_tune_server_returns_server_arguments(cuda, pool):
+    """reina_rt.autotune.tune_server: a coordinate search over the given axes
+    from their first values; the winner is a valid set of Server arguments
+    whose batch renders the reina_rt_trace images."""
+    from reina_rt import autotune
+    torch = cuda
+    jobs = _mix_jobs(pool)[:4]
+    batch = [(dev, pc, W, H) for dev, pc, W, H, init in jobs if init is None]
+    axes = (("claim_rays", (128, 64)), ("slot_tile", (1024, 256)), ("graph_iters", (6, 12)))
+    best, default, table = autotune.tune_server(torch, batch, dict(path_budget_bytes=64 << 20), axes=axes)
+    assert len(table) == 4 and default["claim_rays"] == 128 and default["slot_tile"] == 1024
+    assert best["ms"] == min(r["ms"] for r in table)
+    assert best["stack_depth"] == autotune.server_stack_depth([int(d.info().max_depth) for d, _, _, _ in batch])
+    srv = reina_rt.Server(path_budget_bytes=64 << 20, **{k: v for k, v in best.items() if k != "ms"})
+    got = served(torch, srv, [(d, pc, W, H, None) for d, pc, W, H in batch])
+    srv.close()
+    for (d, pc, W, H), g in zip(batch, got):
+        assert _same(g, plain(torch, d, pc, W, H))
