@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--no-instanced", action="store_true",
                     help="skip the 8.4M-triangle instanced scene (FP32 vs quantized nodes, error vs the FP64 oracle)")
     ap.add_argument("--fp64-rays", type=int, default=1 << 18, help="rays checked against the FP64 oracle")
+    ap.add_argument("--instanced-tuning", type=str, default=None,
+                    help="instanced block: apply this tuning (as --tuning) to every variant instead of autotuning")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched block")
     ap.add_argument("--batched-groups", type=int, default=2,
                     help="log2 of the sample streams per pixel of the batched block's second pass (0: none)")
@@ -106,6 +108,10 @@ def main():
                     help="served block: the server admits while fewer paths than this are in flight")
     ap.add_argument("--max-wait-ms", type=int, default=500,
                     help="served block, shortest-first variant: age at which a request goes first")
+    ap.add_argument("--served-rates", type=str, default=None,
+                    help="served block: only these arrival rates (requests/s, comma-separated), 5 s each")
+    ap.add_argument("--served-variants", action="store_true",
+                    help="with --served-rates: also run the shortest-first / admit-all / inline-tail variants")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
     ap.add_argument("--calib-copy", action="store_true", help="also run a 256 MiB device copy (PMC calibration)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the OBJ / PNG loader timing")
@@ -151,11 +157,7 @@ def main():
         sec, sh = build_secondary(a.secondary_reps, arrays, dev, cam, W, H, torch)
     autotune = None
     if a.tuning:
-        keys = ("stack_depth", "refill_min", "grid_blocks", "shade_blocks", "waves_per_simd", "tail_paths")
-        vals = [int(v) for v in a.tuning.split(",")]
-        if len(vals) != len(keys):
-            sys.exit("--tuning takes %d comma-separated integers: %s" % (len(keys), ",".join(keys)))
-        dev.set_tuning(**dict(zip(keys, vals)))
+        dev.set_tuning(**parse_tuning(a.tuning, "--tuning"))
     elif not a.no_autotune:
         autotune = run_autotune(dev, sec, sh, torch, frame=lambda: frame(0))
         os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
@@ -264,6 +266,16 @@ def main():
 
 
 
+TUNING_KEYS = ("stack_depth", "refill_min", "grid_blocks", "shade_blocks", "waves_per_simd", "tail_paths")
+
+
+def parse_tuning(text, flag):
+    vals = [int(v) for v in text.split(",")]
+    if len(vals) != len(TUNING_KEYS):
+        sys.exit("%s takes %d comma-separated integers: %s" % (flag, len(TUNING_KEYS), ",".join(TUNING_KEYS)))
+    return dict(zip(TUNING_KEYS, vals))
+
+
 def profiled_traffic(name, tuning):
     """(traffic_bytes_per_launch, tuning it was measured at) from a committed
     rocprofv3 summary: the line for this tuning (stack_depth,refill_min,
@@ -279,6 +291,8 @@ def profiled_traffic(name, tuning):
                 found.append((t.strip(), int(v)))
     except OSError:
         pass
+    if tuning.get("claim_rays", 128) != 128:   # profiles are taken at the default claim chunk
+        return None, None
     for t, v in found:
         if t == key:
             return v, t
@@ -480,13 +494,15 @@ def quantized_nodes(a, arrays, dev, frame, img, sec, sh, torch):
 SERVER_BUDGET = 152 * (1 << 24)   # the largest path pool: 2^24 pixels in flight (2.55 GB)
 
 
-def server_tuning(torch, pool, reqs, skw, what):
+def server_tuning(torch, pool, reqs, skw, what, axes=None):
     """Server launch parameters for this pool, by reina_rt.autotune.tune_server
-    on a batch of the block's own requests."""
+    on a batch of the block's own requests (axes: the searched parameters,
+    default all; the others stay as skw sets them)."""
     from reina_rt import autotune
     t0 = time.perf_counter()
     best, default, table = autotune.tune_server(
-        torch, [(pool[r["scene"]]["dev"], r["pc"], r["W"], r["H"]) for r in reqs], skw)
+        torch, [(pool[r["scene"]]["dev"], r["pc"], r["W"], r["H"]) for r in reqs], skw,
+        axes=axes or autotune.SERVER_AXES)
     log("%s: server autotune %.1f s, best %.1f ms vs default %.1f ms" % (
         what, time.perf_counter() - t0, best["ms"], default["ms"]))
     return {"best": best, "default": default, "candidates": len(table), "batch_requests": len(reqs),
@@ -555,6 +571,8 @@ def served_block(a, torch):
         n_sc, lo, hi, runs = 8, 1e4, 4e5, [(50.0, 4.0), (4.0, 6.0)]
     else:
         n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 5.0), (100.0, 5.0), (200.0, 5.0), (400.0, 5.0), (4.0, 10.0)]
+    if a.served_rates:
+        runs = [(float(r), 5.0) for r in a.served_rates.split(",")]
     t0 = time.perf_counter()
     pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=5, device_build=True)
     setup = time.perf_counter() - t0
@@ -564,10 +582,14 @@ def served_block(a, torch):
            "scene_setup_s": round(setup, 1), "runs": []}
     out["admit_paths"] = a.admit_paths
     res_mix = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
-    base = dict(path_budget_bytes=SERVER_BUDGET)
+    # latency choices, fixed: tail launch at 2^20 live paths beside the loop, admission points every 6
+    # iterations (a sweep at 50-400 requests/s gave p50 and p99 below plain's at every rate with these)
+    base = dict(path_budget_bytes=SERVER_BUDGET, tail_paths=1 << 20, graph_iters=6)
     if not a.no_autotune:
+        from reina_rt import autotune
         treqs = loadgen.make_requests(pool, 12, np.random.default_rng(55), res_mix, spp_range=(16, 128))
-        out["server_autotune"] = server_tuning(torch, pool, treqs, base, "served")
+        axes = tuple(ax for ax in autotune.SERVER_AXES if ax[0] not in ("tail_paths", "graph_iters"))
+        out["server_autotune"] = server_tuning(torch, pool, treqs, base, "served", axes=axes)
         base = dict(base, **{k: v for k, v in out["server_autotune"]["best"].items() if k != "ms"})
     # the server holds at most 17 requests, queued or running, as plain does (1 rendering + 16 queued)
     fifo = dict(base, admit_paths=a.admit_paths, max_requests=17)
@@ -580,11 +602,14 @@ def served_block(a, torch):
     for rate, dur in runs:
         reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng, res_mix, spp_range=(16, 128))
         variants = {}
-        if rate >= 50.0:
-            variants["served_shortest"] = shortest
-        if rate == 100.0:
-            # no admission target and no bound beyond the 16 queued: every request that fits the path pool runs
-            variants["served_admit_all"] = base
+        if not a.served_rates or a.served_variants:
+            if rate >= 50.0:
+                variants["served_shortest"] = shortest
+            if rate == 100.0:
+                # no admission target and no bound beyond the 16 queued: every request that fits the pool runs
+                variants["served_admit_all"] = base
+                # the tail launch as one step of the loop (arrivals wait for it) instead of beside it
+                variants["served_tail_inline"] = dict(fifo, tail_inline=True)
         r = loadgen.served(torch, pool, reqs, rate, dur, 16, fifo, seed=int(rate), variants=variants)
         log("served: rate %.0f/s plain %.2f img/s served %.2f img/s, p50 %.0f / %.0f ms, p99 %.0f / %.0f ms" % (
             rate, r["plain"]["images_per_s"], r["served"]["images_per_s"],
@@ -695,7 +720,10 @@ def instanced_block(a, torch):
             np.array_equal(hits["fp32"].view(np.uint32), hits["quantized"].view(np.uint32)))
     # frames
     for name, d in devs.items():
-        if not a.no_autotune:
+        if a.instanced_tuning:
+            d.set_tuning(**parse_tuning(a.instanced_tuning, "--instanced-tuning"))
+            res[name]["tuning"] = d.get_tuning()
+        elif not a.no_autotune:
             at = run_autotune(d, sec, torch.zeros((sec.shape[0], 4), device="cuda:0"), torch,
                               frame=lambda d=d: frame(0, d))
             at.pop("table")

@@ -351,7 +351,7 @@ typedef struct ReinaTuning {
     uint32_t refill_min;    /* idle lanes per wave that trigger a refill, 1..64                */
     uint32_t grid_blocks;   /* persistent traversal blocks (256 threads)                       */
     uint32_t shade_blocks;  /* shade blocks (256 threads), a multiple of 8                      */
-    uint32_t waves_per_simd;/* traversal register budget: 0 (compiler's choice), 5, 6 or 8      */
+    uint32_t waves_per_simd;/* traversal register budget: 0 (compiler's choice), 4, 5, 6 or 8      */
     uint32_t tail_paths;    /* when at most this many paths are live, the frame ends in one
                                tail launch (per-wave extend/shade steps); 0 = never            */
     uint32_t claim_rays;    /* rays a traversal wave claims from its queue partition with one
@@ -403,7 +403,9 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  * iterations over every admitted request at once (captured in a HIP graph of
  * graph_iters iterations). When at most tuning.tail_paths paths are live after a graph,
  * one tail launch (per-wave extend/shade steps, as reina_rt_trace's) finishes
- * every running request; requests arriving meanwhile wait for it. A request
+ * every running request; it runs beside the loop, which keeps admitting and
+ * running new requests (REINA_RT_SERVER_TAIL_INLINE: requests arriving
+ * meanwhile wait for it). A request
  * completes when all its pixels have finished all
  * samples; its image is then written. Scenes must stay alive until their
  * requests complete. submit / poll / wait may be called from any thread. */
@@ -416,6 +418,13 @@ int reina_rt_postprocess(const float* in_rgba32f, float* scratch_rgba32f, uint8_
  * (as reina_rt_trace does on that scene); without it such requests are
  * refused with REINA_RT_ERR_UNSUPPORTED. */
 #define REINA_RT_SERVER_NEE 0x1u
+/* ReinaServerDesc.flags: run the tail launch on the worker's stream as one
+ * step of the request loop, so requests arriving meanwhile wait for it.
+ * Without it the tail launch runs on a second stream with a grid of
+ * tail_cu_percent of the GPU's resident blocks, while the loop keeps admitting requests and
+ * running graphs for them on the shared queues; the server then holds a tail
+ * queue of 288 * tuning.tail_paths bytes. */
+#define REINA_RT_SERVER_TAIL_INLINE 0x2u
 
 typedef struct ReinaServer ReinaServer;
 
@@ -430,8 +439,9 @@ typedef struct ReinaServerDesc {
                                     tail_paths 0 = 2^19, REINA_RT_TAIL_OFF = never)      */
     uint32_t flags;              /* REINA_RT_SERVER_NEE or 0                             */
     uint32_t admit_paths;        /* admission stops while this many paths are in flight
-                                    (live paths of the last graph + paths admitted since);
-                                    a request is always admitted when nothing runs.
+                                    in the shared queues (live paths of the last graph +
+                                    paths admitted since; a tail launch's are not counted);
+                                    a request is always admitted when nothing runs there.
                                     0 or REINA_RT_ADMIT_ALL: admit whenever slots are free */
     uint32_t slot_tile;          /* pixels per slot tile (the unit of path-slot allocation
                                     and of the queue-partition interleave): 256, 512, 1024,
@@ -444,6 +454,8 @@ typedef struct ReinaServerDesc {
     uint32_t max_requests;       /* submit -> REINA_RT_ERR_BUSY while this many requests are
                                     queued or running (a capacity bound like a plain
                                     renderer's 1 running + max_pending queued); 0 = none  */
+    uint32_t tail_cu_percent;    /* share of the GPU the concurrent tail launch may take, in
+                                    percent of the tail kernel's resident blocks; 0 = 50  */
 } ReinaServerDesc;
 
 /* ReinaServerDesc.admit_paths: no limit but the path pool. */

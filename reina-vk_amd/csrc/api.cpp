@@ -638,8 +638,8 @@ int reina_rt_set_tuning(ReinaScene* s, const ReinaTuning* t) {
     if (t->grid_blocks < 1 || t->grid_blocks > 65536 || t->shade_blocks < kParts || t->shade_blocks > 65536 ||
         t->shade_blocks % kParts)
         return fail(REINA_RT_ERR_INVALID_ARG, "grid_blocks must be in 1..65536 and shade_blocks a multiple of 8");
-    if (t->waves_per_simd != 0 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
-        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    if (t->waves_per_simd != 0 && t->waves_per_simd != 4 && t->waves_per_simd != 5 && t->waves_per_simd != 6 && t->waves_per_simd != 8)
+        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 4, 5, 6 or 8");
     const uint32_t claim = t->claim_rays ? t->claim_rays : 128u;
     if (claim != 64 && claim != 128 && claim != 256 && claim != 512)
         return fail(REINA_RT_ERR_INVALID_ARG, "claim_rays must be 0, 64, 128, 256 or 512");

@@ -12,6 +12,13 @@
 //      k_shade_multi);
 //   3. a read-back of the per-request finished-pixel counters; complete
 //      requests get k_finish_job (image write) and release their slots.
+//   4. when at most tail_paths paths are live, the tail launch takes them
+//      over: k_tail_gather moves them to the tail queue (the shared queues
+//      are then empty) and k_tail MULTI runs on the tail stream with a grid
+//      of a share of the GPU's resident blocks, while the loop admits new
+//      requests and runs their graphs beside it; the tail's requests complete once its event
+//      has fired. One tail launch is in flight at a time; meanwhile the
+//      shared queues' paths keep running in graphs whatever their number.
 // Iteration i uses control slot i % 3 and queue i & 1; every boundary is a
 // multiple of 6, so admission always appends to control slot 0 / queue 0.
 #include <hip/hip_runtime.h>
@@ -76,6 +83,14 @@ struct ReinaServer {
     uint32_t* count_h = nullptr;         // ctrl[0].count after a graph: the live paths per partition
     hipStream_t st = nullptr;
     hipGraphExec_t graph = nullptr;
+    // tail launch beside the loop (not REINA_RT_SERVER_TAIL_INLINE)
+    bool tail_async = false;
+    hipStream_t st_tail = nullptr;
+    uint32_t tail_max_blocks = 0;             // grid cap of the concurrent tail launch
+    hipEvent_t ev_gathered = nullptr, ev_tail = nullptr;
+    Queues TQ{};                              // tail queue: ray[0], pix[0], cap entries per partition
+    Ctrl* tctrl = nullptr;
+    void* tbuf = nullptr;
     // bookkeeping (mu)
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -162,23 +177,117 @@ static size_t next_pending(ReinaServer* s, double now) {
     return best;
 }
 
+// Writes the images of finished requests (k_finish_job on the worker's
+// stream), then releases their slots and request indices and wakes waiters;
+// they are removed from `list`.
+static bool finish_requests(ReinaServer* s, const std::vector<uint64_t>& fin, std::vector<uint64_t>& list) {
+    if (fin.empty()) return true;
+    for (uint64_t tk : fin) {
+        Request r;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            r = s->reqs[tk];
+        }
+        const hipError_t e = launch_finish_job(s->jobs_d, r.job, r.npx, s->S, s->st);
+        if (e != hipSuccess) {
+            fail_worker(s, std::string("launch_finish_job: ") + hipGetErrorString(e));
+            return false;
+        }
+    }
+    const hipError_t e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) {
+        fail_worker(s, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double t = s->now();
+    for (uint64_t tk : fin) {
+        Request& r = s->reqs[tk];
+        r.state = REINA_RT_REQ_DONE;
+        r.done_s = t;
+        for (uint32_t k = 0; k < r.ntiles; ++k) s->tile_used[r.tile0 + k] = 0;
+        s->counts.slot_tiles_used -= r.ntiles;
+        s->job_ticket[r.job] = 0;
+        s->running -= 1;
+        s->counts.completed += 1;
+        for (size_t k = 0; k < list.size(); ++k)
+            if (list[k] == tk) { list.erase(list.begin() + (std::ptrdiff_t)k); break; }
+    }
+    s->cv_done.notify_all();
+    return true;
+}
+
+// Reads back the finished-pixel counters and ray counts (worker's stream).
+static bool read_counters(ReinaServer* s) {
+    hipError_t e = hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->st);
+    if (e != hipSuccess) {
+        fail_worker(s, std::string("counter read-back: ") + hipGetErrorString(e));
+        return false;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->counts.extension_rays = s->stats_h[0];
+    s->counts.tail_rays = s->stats_h[4];
+    return true;
+}
+
+// The tail launch in flight has ended: every request it held is complete.
+static bool end_tail(ReinaServer* s, std::vector<uint64_t>& tailing) {
+    if (!read_counters(s)) return false;
+    bool all = true;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (uint64_t tk : tailing) {
+            const Request& r = s->reqs[tk];
+            all = all && s->done_h[r.job] >= r.npx;
+        }
+    }
+    if (!all) {   // the tail launch runs every path it holds to the end
+        fail_worker(s, "a request left the tail launch unfinished");
+        return false;
+    }
+    const std::vector<uint64_t> fin = tailing;
+    return finish_requests(s, fin, tailing);
+}
+
 static void worker_main(ReinaServer* s) {
     if (hipSetDevice(s->device) != hipSuccess) {
         fail_worker(s, "hipSetDevice failed in the server thread");
         return;
     }
-    std::vector<uint64_t> running;   // tickets of admitted requests
+    std::vector<uint64_t> running;   // admitted requests whose paths are in the shared queues
+    std::vector<uint64_t> tailing;   // requests the tail launch in flight is finishing
+    bool tail_busy = false;
     auto busy_from = std::chrono::steady_clock::now();
+    auto add_busy = [&]() {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->counts.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy_from).count();
+    };
     for (;;) {
+        // a tail launch that has ended: its requests complete
+        if (tail_busy) {
+            const hipError_t q = hipEventQuery(s->ev_tail);
+            if (q == hipSuccess) {
+                if (!end_tail(s, tailing)) return;
+                tail_busy = false;
+                if (running.empty()) add_busy();
+            } else if (q != hipErrorNotReady) {
+                fail_worker(s, std::string("tail launch: ") + hipGetErrorString(q));
+                return;
+            }
+        }
         std::vector<Request> admitted;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             if (running.empty() && s->pending.empty()) {
-                if (s->stop) break;
-                s->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+                if (s->stop && !tail_busy) break;
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(tail_busy ? 1 : 20));
                 continue;
             }
-            if (running.empty()) busy_from = std::chrono::steady_clock::now();
+            if (running.empty() && !tail_busy) busy_from = std::chrono::steady_clock::now();
+            // paths in flight: the shared queues' (a tail launch holds its own)
             uint64_t inflight = running.empty() ? 0 : s->live_paths;
             const double tnow = s->now();
             while (!s->pending.empty()) {
@@ -220,74 +329,76 @@ static void worker_main(ReinaServer* s) {
         }
         if (running.empty()) continue;
         W_TRY(hipGraphLaunch(s->graph, s->st));
-        W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
-        W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
         W_TRY(hipMemcpyAsync(s->count_h, s->ctrl[0].count, sizeof(Ctrl::count), hipMemcpyDeviceToHost, s->st));
-        W_TRY(hipStreamSynchronize(s->st));
-        // Few paths left: one tail launch finishes every running request (the
-        // graph leaves the next iteration's rays in queue 0, control slot 0).
-        uint64_t live = 0;
-        for (uint32_t p = 0; p < kParts; ++p) live += s->count_h[p * kLine];
-        s->live_paths = live;
-        if (s->tune.tail_paths != 0 && live != 0 && live <= s->tune.tail_paths) {
-            const unsigned long long before = s->stats_h[0];
-            W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats, s->done_d,
-                                    s->tune, (uint32_t)live, s->st));
-            W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
-            W_TRY(hipMemcpyAsync(s->done_h, s->done_d, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
-            W_TRY(hipMemcpyAsync(s->stats_h, s->stats, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
-            W_TRY(hipStreamSynchronize(s->st));
-            s->live_paths = 0;
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->counts.tail_launches += 1;
-            s->counts.tail_rays += s->stats_h[0] - before;
-        }
-        std::vector<uint64_t> finished;
+        if (!read_counters(s)) return;
         {
             std::lock_guard<std::mutex> lk(s->mu);
             s->counts.iterations += s->graph_iters;
             s->counts.graph_replays += 1;
-            s->counts.extension_rays = s->stats_h[0];
+        }
+        uint64_t live = 0;
+        uint32_t maxc = 0;
+        for (uint32_t p = 0; p < kParts; ++p) {
+            const uint32_t c = s->count_h[p * kLine];
+            live += c;
+            maxc = c > maxc ? c : maxc;
+        }
+        s->live_paths = live;
+        bool to_tail = false;
+        if (s->tune.tail_paths != 0 && live != 0 && live <= s->tune.tail_paths) {
+            if (!s->tail_async) {
+                // Few paths left: one tail launch finishes every running request (the
+                // graph leaves the next iteration's rays in queue 0, control slot 0).
+                W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->Q, 0, s->ctrl, 0, s->stats,
+                                        s->done_d, s->tune, (uint32_t)live, s->st));
+                W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
+                if (!read_counters(s)) return;
+                s->live_paths = 0;
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->counts.tail_launches += 1;
+            } else if (!tail_busy) {
+                // The live paths move to the tail queue and finish on the tail stream; the
+                // shared queues are empty again and take new requests at once. (While a
+                // tail launch is in flight, the few live paths keep running in graphs.)
+                W_TRY(launch_tail_gather(s->Q, s->ctrl, s->TQ, s->tctrl, maxc, s->st));
+                W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
+                W_TRY(hipEventRecord(s->ev_gathered, s->st));
+                W_TRY(hipStreamWaitEvent(s->st_tail, s->ev_gathered, 0));
+                W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->TQ, 0, s->tctrl, 0, s->stats,
+                                        s->done_d, s->tune, (uint32_t)live, s->st_tail, s->tail_max_blocks));
+                W_TRY(hipEventRecord(s->ev_tail, s->st_tail));
+                tail_busy = true;
+                to_tail = true;
+                s->live_paths = 0;
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->counts.tail_launches += 1;
+            }
+        }
+        std::vector<uint64_t> finished;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
             for (uint64_t tk : running) {
                 const Request& r = s->reqs[tk];
                 if (s->done_h[r.job] >= r.npx) finished.push_back(tk);
             }
         }
-        if (finished.empty()) continue;
-        for (uint64_t tk : finished) {
-            Request r;
-            {
-                std::lock_guard<std::mutex> lk(s->mu);
-                r = s->reqs[tk];
-            }
-            W_TRY(launch_finish_job(s->jobs_d, r.job, r.npx, s->S, s->st));
+        if (!finish_requests(s, finished, running)) return;
+        if (to_tail) {
+            tailing.insert(tailing.end(), running.begin(), running.end());
+            running.clear();
         }
-        W_TRY(hipStreamSynchronize(s->st));
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            const double t = s->now();
-            for (uint64_t tk : finished) {
-                Request& r = s->reqs[tk];
-                r.state = REINA_RT_REQ_DONE;
-                r.done_s = t;
-                for (uint32_t k = 0; k < r.ntiles; ++k) s->tile_used[r.tile0 + k] = 0;
-                s->counts.slot_tiles_used -= r.ntiles;
-                s->job_ticket[r.job] = 0;
-                s->running -= 1;
-                s->counts.completed += 1;
-                for (size_t k = 0; k < running.size(); ++k)
-                    if (running[k] == tk) { running.erase(running.begin() + k); break; }
-            }
-            if (running.empty())
-                s->counts.busy_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - busy_from).count();
-            s->cv_done.notify_all();
-        }
+        if (running.empty() && !tail_busy) add_busy();
     }
 }
 
 static void free_server(ReinaServer* s) {
     if (s->graph) hipGraphExecDestroy(s->graph);
     if (s->st) hipStreamDestroy(s->st);
+    if (s->st_tail) hipStreamDestroy(s->st_tail);
+    if (s->ev_gathered) hipEventDestroy(s->ev_gathered);
+    if (s->ev_tail) hipEventDestroy(s->ev_tail);
+    if (s->tbuf) hipFree(s->tbuf);
+    if (s->tctrl) hipFree(s->tctrl);
     if (s->dbuf) hipFree(s->dbuf);
     if (s->ctrl) hipFree(s->ctrl);
     if (s->jobs_h) hipHostFree(s->jobs_h);
@@ -305,7 +416,9 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (d->struct_size != sizeof(ReinaServerDesc))
         return fail(REINA_RT_ERR_ABI, "ReinaServerDesc.struct_size mismatch (expected " +
                                           std::to_string(sizeof(ReinaServerDesc)) + ")");
-    if (d->flags & ~REINA_RT_SERVER_NEE) return fail(REINA_RT_ERR_INVALID_ARG, "unknown server flag bits");
+    if (d->flags & ~(REINA_RT_SERVER_NEE | REINA_RT_SERVER_TAIL_INLINE))
+        return fail(REINA_RT_ERR_INVALID_ARG, "unknown server flag bits");
+    if (d->tail_cu_percent > 100) return fail(REINA_RT_ERR_INVALID_ARG, "tail_cu_percent must be 0..100");
     const bool nee = (d->flags & REINA_RT_SERVER_NEE) != 0;
     const uint64_t bpp = nee ? REINA_RT_SERVER_BYTES_PER_PIXEL_NEE : REINA_RT_SERVER_BYTES_PER_PIXEL;
     const uint32_t tile = d->slot_tile ? d->slot_tile : 1024u;
@@ -334,8 +447,8 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
         return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");
     if (tu.refill_min == 0) tu.refill_min = 16;
     if (tu.refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be 1..64");
-    if (tu.waves_per_simd != 0 && tu.waves_per_simd != 5 && tu.waves_per_simd != 6 && tu.waves_per_simd != 8)
-        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 5, 6 or 8");
+    if (tu.waves_per_simd != 0 && tu.waves_per_simd != 4 && tu.waves_per_simd != 5 && tu.waves_per_simd != 6 && tu.waves_per_simd != 8)
+        return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 4, 5, 6 or 8");
     tu.claim = d->tuning.claim_rays ? d->tuning.claim_rays : 128u;
     if (tu.claim != 64 && tu.claim != 128 && tu.claim != 256 && tu.claim != 512)
         return fail(REINA_RT_ERR_INVALID_ARG, "claim_rays must be 0, 64, 128, 256 or 512");
@@ -395,6 +508,31 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
         hipHostMalloc((void**)&s->count_h, sizeof(Ctrl::count), 0) != hipSuccess)
         return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipHostMalloc failed");
     if (hipStreamCreate(&s->st) != hipSuccess) return bail(REINA_RT_ERR_DEVICE, "hipStreamCreate failed");
+    s->tail_async = !(d->flags & REINA_RT_SERVER_TAIL_INLINE) && tu.tail_paths != 0;
+    if (s->tail_async) {
+        // tail queue: every live path of a tail launch keeps its partition and position
+        const uint64_t tcap = tu.tail_paths < slots / kParts ? tu.tail_paths : slots / kParts;
+        s->TQ.cap = (uint32_t)tcap;
+        const uint64_t tbytes = kParts * tcap * (32 + 4);
+        if (hipMalloc(&s->tbuf, tbytes) != hipSuccess)
+            return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc of the " + std::to_string(tbytes) + "-byte tail queue failed");
+        s->TQ.ray[0] = (float4*)s->tbuf;
+        s->TQ.pix[0] = (uint32_t*)((char*)s->tbuf + kParts * tcap * 32);
+        if (hipMalloc(&s->tctrl, sizeof(Ctrl)) != hipSuccess || hipMemset(s->tctrl, 0, sizeof(Ctrl)) != hipSuccess)
+            return bail(REINA_RT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+        // The tail launch takes tail_cu_percent of the blocks the tail kernel can keep
+        // resident (3 per CU at its 3-wave register budget, 2 with NEE); the graphs'
+        // kernels fill the rest of the GPU.
+        const uint32_t pct = d->tail_cu_percent ? d->tail_cu_percent : 50u;
+        const uint64_t full = (uint64_t)cus * (nee ? 2u : 3u);
+        s->tail_max_blocks = (uint32_t)(full * pct / 100u);
+        if (s->tail_max_blocks < kParts) s->tail_max_blocks = kParts;
+        const hipError_t e = hipStreamCreate(&s->st_tail);
+        if (e != hipSuccess) return bail(REINA_RT_ERR_DEVICE, std::string("tail stream: ") + hipGetErrorString(e));
+        if (hipEventCreateWithFlags(&s->ev_gathered, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_tail, hipEventDisableTiming) != hipSuccess)
+            return bail(REINA_RT_ERR_DEVICE, "hipEventCreate failed");
+    }
     hipError_t e = capture_graph(s);
     if (e != hipSuccess) return bail(REINA_RT_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
     s->tile_used.assign(s->tiles, 0);
@@ -415,6 +553,7 @@ int reina_rt_server_destroy(ReinaServer* s) {
     if (s->worker.joinable()) s->worker.join();
     hipSetDevice(s->device);
     hipStreamSynchronize(s->st);
+    if (s->st_tail) hipStreamSynchronize(s->st_tail);
     free_server(s);
     return REINA_RT_OK;
 }

@@ -217,6 +217,34 @@ hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const
     return hipGetLastError();
 }
 
+// Moves the live rays of queue 0 (counts in ctrl[0]) into the tail queue TQ
+// (same partition, same position), sets tctrl's counts and clears its claim
+// counters. Partition p = blockIdx.y.
+__global__ __launch_bounds__(256) void k_tail_gather(Queues Q, const Ctrl* ctrl, Queues TQ, Ctrl* tctrl) {
+    const uint32_t p = blockIdx.y;
+    const uint32_t n = ctrl[0].count[p][0];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t i = p * Q.cap + j, o = p * TQ.cap + j;
+        TQ.ray[0][2 * o] = Q.ray[0][2 * i];
+        TQ.ray[0][2 * o + 1] = Q.ray[0][2 * i + 1];
+        TQ.pix[0][o] = Q.pix[0][i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tctrl->count[p][0] = n;
+        tctrl->fetch[p][0] = 0;
+        tctrl->sfetch[p][0] = 0;
+    }
+}
+
+hipError_t launch_tail_gather(const Queues& Q, const Ctrl* ctrl, const Queues& TQ, Ctrl* tctrl, uint32_t max_count,
+                              hipStream_t st) {
+    if (max_count > TQ.cap) return hipErrorInvalidValue;
+    uint32_t gx = (max_count + 255u) / 256u;
+    gx = gx < 1u ? 1u : (gx > 512u ? 512u : gx);
+    hipLaunchKernelGGL(k_tail_gather, dim3(gx, kParts), dim3(256), 0, st, Q, ctrl, TQ, tctrl);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_multi(bool quantized, const float4* rays, float4* hits, const TraceInput& in,
                               unsigned long long* stats, const Tuning& tu, const uint32_t* pix, const JobRec* jobs,
                               hipStream_t st) {

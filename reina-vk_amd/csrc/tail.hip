@@ -292,8 +292,9 @@ hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathStat
 
 hipError_t launch_tail_multi(bool quantized, bool nee, const JobRec* jobs, const PathState& S, const Queues& Q,
                              uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
-                             const Tuning& tu, uint32_t live, hipStream_t st) {
-    const uint32_t blocks = tail_blocks(live);
+                             const Tuning& tu, uint32_t live, hipStream_t st, uint32_t max_blocks) {
+    uint32_t blocks = tail_blocks(live);
+    if (max_blocks != 0 && blocks > max_blocks) blocks = max_blocks < kParts ? kParts : max_blocks;
     const SceneView sv{};
     const TraceParams P{};
     if (quantized) {

@@ -209,6 +209,7 @@ static void launch_trace_sd(const SceneView& sv, const float4* rays, float4* hit
                             unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                             const uint32_t* pix, const JobRec* jobs, hipStream_t st) {
     switch (tu.waves_per_simd) {
+        case 4: launch_trace_sdw<Q, MULTI, SD, 4>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
         case 5: launch_trace_sdw<Q, MULTI, SD, 5>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
         case 6: launch_trace_sdw<Q, MULTI, SD, 6>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;
         case 8: launch_trace_sdw<Q, MULTI, SD, 8>(sv, rays, hits, in, stats, counters, count, tu, pix, jobs, st); break;

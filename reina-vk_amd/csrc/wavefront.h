@@ -92,7 +92,7 @@ struct Tuning {
     uint32_t refill_min;    // idle lanes of a wave that trigger a refill (1..64)
     uint32_t grid;          // persistent blocks of 256 threads
     uint32_t shade_grid;    // blocks of the shade kernel
-    uint32_t waves_per_simd;  // register budget of the traversal kernel: 0 (compiler's choice), 5, 6 or 8
+    uint32_t waves_per_simd;  // register budget of the traversal kernel: 0 (compiler's choice), 4, 5, 6 or 8
     uint32_t tail_paths;      // live paths at which the frame ends in one tail launch (0: never)
     uint32_t claim = 128;     // rays a traversal wave claims with one atomic: 64, 128, 256 or 512
 };
@@ -141,6 +141,11 @@ hipError_t launch_connect_multi_f32(const JobRec* jobs, const Queues& Q, const P
 hipError_t launch_connect_multi_q8(const JobRec* jobs, const Queues& Q, const PathState& S, Ctrl* ctrl, uint32_t slot,
                                    const uint32_t* pix, unsigned long long* stats, const Tuning& tu, hipStream_t st);
 hipError_t launch_finish_job(const JobRec* jobs, uint32_t j, uint32_t npx, const PathState& S, hipStream_t st);
+// Server tail launch on its own stream: the live rays of queue 0 move to the
+// tail queue TQ (cap entries per partition; max_count = the largest partition
+// count, at most TQ.cap) with their counts in tctrl.
+hipError_t launch_tail_gather(const Queues& Q, const Ctrl* ctrl, const Queues& TQ, Ctrl* tctrl, uint32_t max_count,
+                              hipStream_t st);
 hipError_t launch_trace_f32_h16(const SceneView& sv, const float4* rays, float4* hits, const TraceInput& in,
                                 unsigned long long* stats, uint32_t* counters, bool count, const Tuning& tu,
                                 hipStream_t st);
@@ -168,10 +173,12 @@ hipError_t launch_tail(const SceneView& sv, const TraceParams& P, const PathStat
                        uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, const Tuning& tu,
                        uint32_t live, hipStream_t st);
 // Server version: finishes every request in the shared queues (pixels that
-// finish their last sample are counted in done[request]).
+// finish their last sample are counted in done[request]). max_blocks caps
+// the grid (0: one block per 256 live paths), so a tail launch can leave
+// room on the GPU for kernels of another stream.
 hipError_t launch_tail_multi(bool quantized, bool nee, const JobRec* jobs, const PathState& S, const Queues& Q,
                              uint32_t in_q, Ctrl* ctrl, uint32_t slot, unsigned long long* stats, uint32_t* done,
-                             const Tuning& tu, uint32_t live, hipStream_t st);
+                             const Tuning& tu, uint32_t live, hipStream_t st, uint32_t max_blocks = 0);
 hipError_t launch_debug_math(int fn, const float* in, const float* in2, float* out, uint32_t n, hipStream_t st);
 hipError_t launch_postprocess(const float4* in, float4* ping, float4* pong, uchar4* out, uint32_t W, uint32_t H,
                               const BloomPushConsts& b, const TonemappingPushConsts& t, hipStream_t st);

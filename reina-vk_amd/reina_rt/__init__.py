@@ -184,25 +184,27 @@ class Server:
     at once, wait()/poll() report completion with timestamps.
 
     slot_tile, graph_iters, admit_order (abi.ADMIT_FIFO / ADMIT_SHORTEST),
-    max_wait_ms and max_requests are ReinaServerDesc's fields (0 = default);
-    the remaining keyword arguments are ReinaTuning fields."""
+    max_wait_ms, max_requests and tail_cu_percent are ReinaServerDesc's
+    fields (0 = default); tail_inline sets REINA_RT_SERVER_TAIL_INLINE; the
+    remaining keyword arguments are ReinaTuning fields."""
 
     def __init__(self, path_budget_bytes=1 << 30, max_pending=1024, quantized=False, device=0, nee=False,
                  admit_paths=0, slot_tile=0, graph_iters=0, admit_order=0, max_wait_ms=0, max_requests=0,
-                 **tuning):
+                 tail_inline=False, tail_cu_percent=0, **tuning):
         d = abi.ReinaServerDesc()
         d.struct_size = ctypes.sizeof(abi.ReinaServerDesc)
         d.device = device
         d.path_budget_bytes = int(path_budget_bytes)
         d.max_pending = max_pending
         d.quantized = int(bool(quantized))
-        d.flags = abi.SERVER_NEE if nee else 0
+        d.flags = (abi.SERVER_NEE if nee else 0) | (abi.SERVER_TAIL_INLINE if tail_inline else 0)
         d.admit_paths = int(admit_paths)
         d.slot_tile = int(slot_tile)
         d.graph_iters = int(graph_iters)
         d.admit_order = int(admit_order)
         d.max_wait_ms = int(max_wait_ms)
         d.max_requests = int(max_requests)
+        d.tail_cu_percent = int(tail_cu_percent)
         for k, v in tuning.items():
             setattr(d.tuning, k, v)
         h = ctypes.c_void_p()

@@ -148,7 +148,7 @@ class ReinaServerDesc(ctypes.Structure):
                 ("quantized", ctypes.c_uint32), ("tuning", ReinaTuning), ("flags", ctypes.c_uint32),
                 ("admit_paths", ctypes.c_uint32), ("slot_tile", ctypes.c_uint32), ("graph_iters", ctypes.c_uint32),
                 ("admit_order", ctypes.c_uint32), ("max_wait_ms", ctypes.c_uint32),
-                ("max_requests", ctypes.c_uint32)]
+                ("max_requests", ctypes.c_uint32), ("tail_cu_percent", ctypes.c_uint32)]
 
 
 class ReinaRequestStatus(ctypes.Structure):
@@ -204,5 +204,6 @@ BUILD_DEVICE = 0x1   # the default (flags 0 builds on the device too)
 BUILD_HOST = 0x4
 NODES_QUANTIZED = 0x2
 ADMIT_ALL = 0xFFFFFFFF
+SERVER_TAIL_INLINE = 0x2
 ADMIT_FIFO = 0
 ADMIT_SHORTEST = 1

@@ -23,7 +23,7 @@ import itertools
 
 
 def tune_traversal(dev, torch, rays_ptr, hits_ptr, n, stream=None, cus=None, reps=3,
-                   refills=(8, 16, 32, 64), blocks_per_cu=(4, 8), depths=None, waves=(0, 5, 6, 8)):
+                   refills=(8, 16, 32, 64), blocks_per_cu=(4, 8), depths=None, waves=(0, 4, 5, 6, 8)):
     stream = stream or torch.cuda.current_stream()
     sp = stream.cuda_stream
     if cus is None:

@@ -182,17 +182,19 @@ def test_server_admit_shortest(cuda, pool, max_wait_ms):
 
 
 def test_server_admit_paths(cuda, pool):
-    """With admit_paths = 1 a request is admitted only when nothing runs:
-    requests submitted together run one after another (each admitted after
-    the previous one completed), and every image is still its
-    reina_rt_trace image."""
+    """With admit_paths = 1 a request is admitted only when nothing runs in
+    the shared queues (tail launches off here: a tail launch's paths are not
+    counted, so with tails the next request starts beside it): requests
+    submitted together run one after another (each admitted after the
+    previous one completed), and every image is still its reina_rt_trace
+    image."""
     torch = cuda
     jobs = []
     for name, W, H, spp in (("ico", 64, 48, 2), ("cornell", 40, 40, 3), ("inst", 48, 27, 2), ("ico", 32, 32, 1)):
         arrays, cam, dev = pool[name]
         pc = scenes.cornell_push(W, H, spp=spp, max_bounces=8) if name == "cornell" else _pc(cam, W, H, spp, 8)
         jobs.append((dev, pc, W, H, None))
-    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1)
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, tail_paths=abi.TAIL_OFF)
     imgs, tickets = [], []
     for dev, pc, W, H, _ in jobs:
         img = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
@@ -446,3 +448,42 @@ def test_tune_server_returns_server_arguments(cuda, pool):
     srv.close()
     for (d, pc, W, H), g in zip(batch, got):
         assert _same(g, plain(torch, d, pc, W, H))
+
+
+@pytest.mark.parametrize("inline", [False, True], ids=["tail_beside", "tail_inline"])
+def test_server_admits_while_tail_runs(cuda, pool, inline):
+    """A 1024x1024, 256-spp request goes to the tail launch after its first
+    graph (its 2^20 pixels are at most tail_paths). A small request submitted
+    while that tail runs is admitted and completes before the long one when the
+    tail runs beside the loop (the default); with REINA_RT_SERVER_TAIL_INLINE it
+    waits for the tail. Both images equal reina_rt_trace's either way."""
+    import time
+    torch = cuda
+    _, icam, idev = pool["ico"]
+    _, ccam, cdev = pool["cornell"]
+    big = torch.zeros((1024, 1024, 4), dtype=torch.float32, device="cuda")
+    small = torch.zeros((64, 64, 4), dtype=torch.float32, device="cuda")
+    bpc = _pc(icam, 1024, 1024, 256, 8)
+    spc = scenes.cornell_push(64, 64, spp=4, max_bounces=8)
+    torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's streams)
+    srv = reina_rt.Server(path_budget_bytes=512 << 20, tail_paths=1 << 20, tail_inline=inline)
+    tb = srv.submit(idev, bpc, big.data_ptr(), 1024, 1024)
+    for _ in range(5000):
+        if srv.poll(tb).state == abi.REQ_RUNNING:
+            break
+        time.sleep(0.001)
+    time.sleep(0.1)   # its first graph takes a few ms, its tail launch about half a second or more
+    if not inline:
+        assert srv.stats().tail_launches == 1
+    ts = srv.submit(cdev, spc, small.data_ptr(), 64, 64)
+    sst = srv.wait(ts)
+    bst = srv.wait(tb)
+    st = srv.stats()
+    srv.close()
+    if inline:
+        assert sst.admit_s >= bst.done_s
+    else:
+        assert sst.admit_s < bst.done_s and sst.done_s < bst.done_s, (sst.admit_s, sst.done_s, bst.done_s)
+    assert st.completed == 2 and st.tail_rays > 0
+    assert _same(small.cpu().numpy(), plain(torch, cdev, spc, 64, 64))
+    assert _same(big.cpu().numpy(), plain(torch, idev, bpc, 1024, 1024))
