@@ -34,12 +34,16 @@ FP32 tree beside them, and the closest hits of a sampled ray set against the
 FP64 oracle.
 
 batched: 64 scenes of 1e3-2e6 triangles (log-uniform), 64 requests of
-uneven resolution and spp, all submitted to one server, against the same
+uneven resolution and spp, all submitted to one server whose launch
+parameters the server autotuner chose on the same batch, against the same
 requests through reina_rt_trace one after another (images bit-identical).
 
-served: Poisson arrivals at 50-400 requests/s (and 4/s, below capacity) over
-32 resident scenes of 1e4-4e6 triangles, bounded FIFO: p50 / p99 latency and
-images/s of the server against plain per-request launches.
+served: Poisson arrivals at 50-400 requests/s for 10 s each (and 4/s, below
+capacity) over 32 resident scenes of 1e4-4e6 triangles. Plain renders one
+request at a time with 16 queued; the server holds at most 17 requests,
+queued or running, with the tail launch beside its loop: p50 / p99 latency
+and images/s of both, plus shortest-first admission, no admission bound and
+the inline tail launch as variants.
 
 cpu_baseline: the oracle's C restatement of the reference shaders (BVH
 traversal over the same BVH8), multithreaded over pixels, on a bounded
@@ -109,7 +113,7 @@ def main():
     ap.add_argument("--max-wait-ms", type=int, default=500,
                     help="served block, shortest-first variant: age at which a request goes first")
     ap.add_argument("--served-rates", type=str, default=None,
-                    help="served block: only these arrival rates (requests/s, comma-separated), 5 s each")
+                    help="served block: only these arrival rates (requests/s, comma-separated), 10 s each")
     ap.add_argument("--served-variants", action="store_true",
                     help="with --served-rates: also run the shortest-first / admit-all / inline-tail variants")
     ap.add_argument("--secondary-reps", type=int, default=4, help="secondary rays per primary hit")
@@ -570,9 +574,9 @@ def served_block(a, torch):
     if a.small_pools:
         n_sc, lo, hi, runs = 8, 1e4, 4e5, [(50.0, 4.0), (4.0, 6.0)]
     else:
-        n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 5.0), (100.0, 5.0), (200.0, 5.0), (400.0, 5.0), (4.0, 10.0)]
+        n_sc, lo, hi, runs = 32, 1e4, 4e6, [(50.0, 10.0), (100.0, 10.0), (200.0, 10.0), (400.0, 10.0), (4.0, 10.0)]
     if a.served_rates:
-        runs = [(float(r), 5.0) for r in a.served_rates.split(",")]
+        runs = [(float(r), 10.0) for r in a.served_rates.split(",")]
     t0 = time.perf_counter()
     pool = loadgen.make_pool(loadgen.log_uniform(rng, lo, hi, n_sc), seed=5, device_build=True)
     setup = time.perf_counter() - t0
@@ -603,7 +607,7 @@ def served_block(a, torch):
         reqs = loadgen.make_requests(pool, int(rate * dur * 1.5) + 16, rng, res_mix, spp_range=(16, 128))
         variants = {}
         if not a.served_rates or a.served_variants:
-            if rate >= 50.0:
+            if rate in (100.0, 200.0):
                 variants["served_shortest"] = shortest
             if rate == 100.0:
                 # no admission target and no bound beyond the 16 queued: every request that fits the pool runs
