@@ -592,7 +592,8 @@ def served_block(a, torch):
     if not a.no_autotune:
         from reina_rt import autotune
         treqs = loadgen.make_requests(pool, 12, np.random.default_rng(55), res_mix, spp_range=(16, 128))
-        axes = tuple(ax for ax in autotune.SERVER_AXES if ax[0] not in ("tail_paths", "graph_iters"))
+        axes = tuple(ax for ax in autotune.SERVER_AXES
+                     if ax[0] not in ("tail_paths", "graph_iters", "admit_order", "tail_cu_percent"))
         out["server_autotune"] = server_tuning(torch, pool, treqs, base, "served", axes=axes)
         base = dict(base, **{k: v for k, v in out["server_autotune"]["best"].items() if k != "ms"})
     # the server holds at most 17 requests, queued or running, as plain does (1 rendering + 16 queued)

@@ -448,27 +448,33 @@ typedef struct ReinaServerDesc {
                                     2048 or 4096; 0 = 1024                                */
     uint32_t graph_iters;        /* bounce iterations per graph replay, between two
                                     admission / completion points: 6, 12, 18 or 24; 0 = 6 */
-    uint32_t admit_order;        /* REINA_RT_ADMIT_FIFO or REINA_RT_ADMIT_SHORTEST         */
-    uint32_t max_wait_ms;        /* REINA_RT_ADMIT_SHORTEST: a request waiting this long is
-                                    admitted before any shorter one (oldest first); 0 = 500 */
+    uint32_t admit_order;        /* REINA_RT_ADMIT_FIFO, _SHORTEST or _LONGEST              */
+    uint32_t max_wait_ms;        /* SHORTEST / LONGEST: a request waiting this long goes
+                                    before the others (oldest first); 0 = 500             */
     uint32_t max_requests;       /* submit -> REINA_RT_ERR_BUSY while this many requests are
                                     queued or running (a capacity bound like a plain
                                     renderer's 1 running + max_pending queued); 0 = none  */
-    uint32_t tail_cu_percent;    /* share of the GPU the concurrent tail launch may take, in
-                                    percent of the tail kernel's resident blocks; 0 = 50  */
+    uint32_t tail_cu_percent;    /* share of the GPU the concurrent tail launch takes, in
+                                    percent of the tail kernel's resident blocks, 1..100;
+                                    0 = 50 when requests are waiting at its launch, else
+                                    the whole GPU                                          */
 } ReinaServerDesc;
 
 /* ReinaServerDesc.admit_paths: no limit but the path pool. */
 #define REINA_RT_ADMIT_ALL 0xFFFFFFFFu
 
 /* ReinaServerDesc.admit_order. FIFO: submission order. SHORTEST: least
- * estimated work first (path slots x samples per pixel x max bounces), except
- * that requests waiting max_wait_ms or longer go first, oldest first, and
+ * estimated work first (path slots x samples per pixel x max bounces).
+ * LONGEST: longest per-pixel sample chain first (samples per pixel x max
+ * bounces, then more path slots), which shortens the makespan of a batch,
+ * since a pixel's samples run in sequence. With SHORTEST and LONGEST,
+ * requests waiting max_wait_ms or longer go first, oldest first, and
  * admission stops at the first of those that does not fit: no request waits
  * more than max_wait_ms plus the time for the requests ahead of it to free
  * their slots. */
 #define REINA_RT_ADMIT_FIFO     0u
 #define REINA_RT_ADMIT_SHORTEST 1u
+#define REINA_RT_ADMIT_LONGEST  2u
 
 /* ReinaServerDesc.tuning.tail_paths: never end with a tail launch. */
 #define REINA_RT_TAIL_OFF 0xFFFFFFFFu

@@ -14,9 +14,10 @@
 //      requests get k_finish_job (image write) and release their slots.
 //   4. when at most tail_paths paths are live, the tail launch takes them
 //      over: k_tail_gather moves them to the tail queue (the shared queues
-//      are then empty) and k_tail MULTI runs on the tail stream with a grid
-//      of a share of the GPU's resident blocks, while the loop admits new
-//      requests and runs their graphs beside it; the tail's requests complete once its event
+//      are then empty) and k_tail MULTI runs on the tail stream, while the
+//      loop admits new requests and runs their graphs beside it (its grid is
+//      a share of the GPU's resident blocks when requests are waiting at its
+//      launch, else the whole GPU); the tail's requests complete once its event
 //      has fired. One tail launch is in flight at a time; meanwhile the
 //      shared queues' paths keep running in graphs whatever their number.
 // Iteration i uses control slot i % 3 and queue i & 1; every boundary is a
@@ -48,6 +49,7 @@ struct Request {
     uint32_t state = REINA_RT_REQ_QUEUED;
     double submit_s = 0, admit_s = 0, done_s = 0;
     double work = 0;   // estimated cost: path slots x samples x max bounces
+    double chain = 0;  // per-pixel sample chain: samples x max bounces
     uint32_t job = 0, tile0 = 0, ntiles = 0;
 };
 
@@ -87,6 +89,7 @@ struct ReinaServer {
     bool tail_async = false;
     hipStream_t st_tail = nullptr;
     uint32_t tail_max_blocks = 0;             // grid cap of the concurrent tail launch
+    bool tail_share_fixed = false;            // tail_cu_percent given: the cap holds even with nothing waiting
     hipEvent_t ev_gathered = nullptr, ev_tail = nullptr;
     Queues TQ{};                              // tail queue: ray[0], pix[0], cap entries per partition
     Ctrl* tctrl = nullptr;
@@ -163,16 +166,20 @@ static void fail_worker(ReinaServer* s, const std::string& msg) {
 }
 
 // Position in s->pending (submission order) of the next request to admit.
-// FIFO: the oldest. SHORTEST: the oldest if it has waited max_wait_s or
-// longer, else the least estimated work (earliest on ties).
+// FIFO: the oldest. SHORTEST / LONGEST: the oldest if it has waited
+// max_wait_s or longer, else the least estimated work / the longest sample
+// chain, then the most work (earliest on ties).
 static size_t next_pending(ReinaServer* s, double now) {
     if (s->admit_order == REINA_RT_ADMIT_FIFO) return 0;
     if (now - s->reqs[s->pending.front()].submit_s >= s->max_wait_s) return 0;
     size_t best = 0;
-    double wbest = s->reqs[s->pending[0]].work;
+    const Request* b = &s->reqs[s->pending[0]];
     for (size_t i = 1; i < s->pending.size(); ++i) {
-        const double w = s->reqs[s->pending[i]].work;
-        if (w < wbest) { best = i; wbest = w; }
+        const Request* r = &s->reqs[s->pending[i]];
+        const bool better = s->admit_order == REINA_RT_ADMIT_SHORTEST
+                                ? r->work < b->work
+                                : (r->chain > b->chain || (r->chain == b->chain && r->work > b->work));
+        if (better) { best = i; b = r; }
     }
     return best;
 }
@@ -360,12 +367,20 @@ static void worker_main(ReinaServer* s) {
                 // The live paths move to the tail queue and finish on the tail stream; the
                 // shared queues are empty again and take new requests at once. (While a
                 // tail launch is in flight, the few live paths keep running in graphs.)
+                // with nothing waiting to run beside it, the tail launch takes the whole GPU
+                // (unless tail_cu_percent fixes its share)
+                bool waiting = s->tail_share_fixed;
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    waiting = waiting || !s->pending.empty();
+                }
                 W_TRY(launch_tail_gather(s->Q, s->ctrl, s->TQ, s->tctrl, maxc, s->st));
                 W_TRY(hipMemsetAsync(&s->ctrl[0], 0, sizeof(Ctrl), s->st));
                 W_TRY(hipEventRecord(s->ev_gathered, s->st));
                 W_TRY(hipStreamWaitEvent(s->st_tail, s->ev_gathered, 0));
                 W_TRY(launch_tail_multi(s->quantized, s->nee, s->jobs_d, s->S, s->TQ, 0, s->tctrl, 0, s->stats,
-                                        s->done_d, s->tune, (uint32_t)live, s->st_tail, s->tail_max_blocks));
+                                        s->done_d, s->tune, (uint32_t)live, s->st_tail,
+                                        waiting ? s->tail_max_blocks : 0u));
                 W_TRY(hipEventRecord(s->ev_tail, s->st_tail));
                 tail_busy = true;
                 to_tail = true;
@@ -428,8 +443,9 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     const uint32_t graph_iters = d->graph_iters ? d->graph_iters : 6u;
     if (graph_iters % 6 != 0 || graph_iters > 24)
         return fail(REINA_RT_ERR_INVALID_ARG, "graph_iters must be 0, 6, 12, 18 or 24");
-    if (d->admit_order != REINA_RT_ADMIT_FIFO && d->admit_order != REINA_RT_ADMIT_SHORTEST)
-        return fail(REINA_RT_ERR_INVALID_ARG, "admit_order must be REINA_RT_ADMIT_FIFO or REINA_RT_ADMIT_SHORTEST");
+    if (d->admit_order != REINA_RT_ADMIT_FIFO && d->admit_order != REINA_RT_ADMIT_SHORTEST &&
+        d->admit_order != REINA_RT_ADMIT_LONGEST)
+        return fail(REINA_RT_ERR_INVALID_ARG, "admit_order must be REINA_RT_ADMIT_FIFO, _SHORTEST or _LONGEST");
     const uint64_t per_tile = bpp * tile;
     uint64_t tiles = d->path_budget_bytes / per_tile;
     tiles -= tiles % kParts;
@@ -527,6 +543,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
         const uint64_t full = (uint64_t)cus * (nee ? 2u : 3u);
         s->tail_max_blocks = (uint32_t)(full * pct / 100u);
         if (s->tail_max_blocks < kParts) s->tail_max_blocks = kParts;
+        s->tail_share_fixed = d->tail_cu_percent != 0;
         const hipError_t e = hipStreamCreate(&s->st_tail);
         if (e != hipSuccess) return bail(REINA_RT_ERR_DEVICE, std::string("tail stream: ") + hipGetErrorString(e));
         if (hipEventCreateWithFlags(&s->ev_gathered, hipEventDisableTiming) != hipSuccess ||
@@ -612,7 +629,8 @@ int reina_rt_submit(ReinaServer* s, ReinaScene* scene, const RtPushConsts* pc, f
     r.img = (float4*)image;
     r.npx = npx;
     r.ntiles = ntiles;
-    r.work = (double)npx * r.P.spp * (r.P.maxBounces ? r.P.maxBounces : 1u);
+    r.chain = (double)r.P.spp * (r.P.maxBounces ? r.P.maxBounces : 1u);
+    r.work = (double)npx * r.chain;
     r.submit_s = s->now();
     s->reqs[tk] = r;
     s->pending.push_back(tk);

@@ -207,3 +207,4 @@ ADMIT_ALL = 0xFFFFFFFF
 SERVER_TAIL_INLINE = 0x2
 ADMIT_FIFO = 0
 ADMIT_SHORTEST = 1
+ADMIT_LONGEST = 2

@@ -14,9 +14,9 @@ shrink as pixels finish, so the best setting for one large batch is not
 always the best for a frame.
 
 tune_server searches a server's launch parameters (the ReinaTuning fields,
-the slot tile and the graph length) for a batch of requests over a scene
-pool, one parameter at a time from the defaults, by the wall time of the
-whole batch through one server.
+the slot tile, the graph length and the admission order) for a batch of
+requests over a scene pool, one parameter at a time from the defaults, by
+the wall time of the whole batch through one server.
 """
 import time
 import itertools
@@ -75,9 +75,10 @@ def tune_frame(dev, torch, frame, candidates, reps=2):
     return win, table
 
 
-SERVER_AXES = (("refill_min", (16, 32)), ("grid_blocks", ("4cu", "8cu")), ("waves_per_simd", (0, 8)),
+SERVER_AXES = (("refill_min", (16, 32)), ("grid_blocks", ("4cu", "8cu")), ("waves_per_simd", (0, 4, 8)),
                ("tail_paths", (1 << 19, 1 << 20, 1 << 21)), ("claim_rays", (128, 64, 256)),
-               ("slot_tile", (1024, 256, 4096)), ("graph_iters", (6, 12, 24)))
+               ("slot_tile", (1024, 256, 4096)), ("graph_iters", (6, 12, 24)), ("admit_order", (0, 2)),
+               ("tail_cu_percent", (0, 50, 100)))
 
 
 def server_stack_depth(depths):

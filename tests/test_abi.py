@@ -193,7 +193,7 @@ def test_bvh_build_host_rejects_null():
 
 @pytest.mark.parametrize("field,value,word", [("slot_tile", 300, "slot_tile"), ("slot_tile", 8192, "slot_tile"),
                                               ("graph_iters", 7, "graph_iters"), ("graph_iters", 30, "graph_iters"),
-                                              ("admit_order", 2, "admit_order")])
+                                              ("admit_order", 3, "admit_order")])
 def test_server_create_rejects_bad_fields(field, value, word):
     """The server's slot tile, graph length and admission order are checked
     before any device work: INVALID_ARG naming the field, GPU or not."""

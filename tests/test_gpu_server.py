@@ -144,17 +144,20 @@ def test_server_tunables_equal_trace(cuda, pool, tile, iters, claim, tail):
     assert st.iterations == iters * st.graph_replays and st.completed == len(jobs)
 
 
-@pytest.mark.parametrize("max_wait_ms", [60000, 1], ids=["shortest", "aged_fifo"])
-def test_server_admit_shortest(cuda, pool, max_wait_ms):
-    """REINA_RT_ADMIT_SHORTEST with admit_paths = 1 (one request at a time):
-    three requests submitted while a long one runs are admitted in order of
-    estimated work (pixels x spp x depth) when none has waited max_wait_ms;
-    once all have waited that long, in submission order. Images equal
-    reina_rt_trace's either way."""
+@pytest.mark.parametrize("order,max_wait_ms,expect", [(abi.ADMIT_SHORTEST, 60000, [1, 2, 0]),
+                                                      (abi.ADMIT_LONGEST, 60000, [2, 0, 1]),
+                                                      (abi.ADMIT_SHORTEST, 1, [0, 1, 2])],
+                         ids=["shortest", "longest", "aged_fifo"])
+def test_server_admit_order(cuda, pool, order, max_wait_ms, expect):
+    """REINA_RT_ADMIT_SHORTEST / _LONGEST with admit_paths = 1 (one request
+    at a time): three requests submitted while a long one runs are admitted
+    in order of estimated work (pixels x spp x depth), or of sample chain
+    (spp x depth), when none has waited max_wait_ms; once all have waited that
+    long, in submission order. Images equal reina_rt_trace's either way."""
     import time
     torch = cuda
     _, icam, idev = pool["ico"]
-    jobs = []   # estimated work: 96*96*4*8 = 294912, 32*32*2*8 = 16384, 64*48*3*12 = 110592
+    jobs = []   # work: 96*96*4*8 = 294912, 32*32*2*8 = 16384, 64*48*3*12 = 110592; chain: 32, 16, 36
     for name, W, H, spp, depth in (("ico", 96, 96, 4, 8), ("cornell", 32, 32, 2, 8), ("inst", 64, 48, 3, 12)):
         _, cam, dev = pool[name]
         pc = scenes.cornell_push(W, H, spp=spp, max_bounces=depth) if name == "cornell" else _pc(cam, W, H, spp, depth)
@@ -163,8 +166,7 @@ def test_server_admit_shortest(cuda, pool, max_wait_ms):
     bimg = torch.zeros((1024, 1024, 4), dtype=torch.float32, device="cuda")
     bpc = _pc(icam, 1024, 1024, 64, 8)
     torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's stream)
-    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, admit_order=abi.ADMIT_SHORTEST,
-                          max_wait_ms=max_wait_ms)
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, admit_order=order, max_wait_ms=max_wait_ms)
     btk = srv.submit(idev, bpc, bimg.data_ptr(), 1024, 1024)
     for _ in range(5000):
         if srv.poll(btk).state == abi.REQ_RUNNING:
@@ -175,8 +177,8 @@ def test_server_admit_shortest(cuda, pool, max_wait_ms):
     sts = [srv.wait(t) for t in tks]
     srv.close()
     assert all(s.submit_s < bst.done_s for s in sts)   # all were waiting while the long request ran
-    order = sorted(range(3), key=lambda k: sts[k].admit_s)
-    assert order == ([1, 2, 0] if max_wait_ms > 1000 else [0, 1, 2]), [s.admit_s for s in sts]
+    got = sorted(range(3), key=lambda k: sts[k].admit_s)
+    assert got == expect, [s.admit_s for s in sts]
     for (dev, pc, W, H), img in zip(jobs, imgs):
         assert _same(img.cpu().numpy(), plain(torch, dev, pc, W, H))
 
@@ -455,8 +457,9 @@ def test_server_admits_while_tail_runs(cuda, pool, inline):
     """A 1024x1024, 256-spp request goes to the tail launch after its first
     graph (its 2^20 pixels are at most tail_paths). A small request submitted
     while that tail runs is admitted and completes before the long one when the
-    tail runs beside the loop (the default); with REINA_RT_SERVER_TAIL_INLINE it
-    waits for the tail. Both images equal reina_rt_trace's either way."""
+    tail runs beside the loop on half of the GPU (tail_cu_percent = 50); with
+    REINA_RT_SERVER_TAIL_INLINE it waits for the tail. Both images equal
+    reina_rt_trace's either way."""
     import time
     torch = cuda
     _, icam, idev = pool["ico"]
@@ -466,7 +469,7 @@ def test_server_admits_while_tail_runs(cuda, pool, inline):
     bpc = _pc(icam, 1024, 1024, 256, 8)
     spc = scenes.cornell_push(64, 64, spp=4, max_bounces=8)
     torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's streams)
-    srv = reina_rt.Server(path_budget_bytes=512 << 20, tail_paths=1 << 20, tail_inline=inline)
+    srv = reina_rt.Server(path_budget_bytes=512 << 20, tail_paths=1 << 20, tail_inline=inline, tail_cu_percent=50)
     tb = srv.submit(idev, bpc, big.data_ptr(), 1024, 1024)
     for _ in range(5000):
         if srv.poll(tb).state == abi.REQ_RUNNING:
