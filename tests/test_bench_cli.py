@@ -37,6 +37,9 @@ def test_traffic_only_from_a_profile_at_the_same_tuning(tmp_path, monkeypatch):
     assert b.profiled_traffic("k.md", dict(tun, grid_blocks=1024, waves_per_simd=0)) == (None, None)   # renamed
     assert b.profiled_traffic("k.md", dict(tun, claim_rays=256)) == (None, None)          # profiles use 128
     assert b.profiled_traffic("missing.md", tun) == (None, None)
+    assert b.profiled_tunings("k.md") == [dict(stack_depth=16, refill_min=32, grid_blocks=2048, shade_blocks=4096,
+                                               waves_per_simd=8, tail_paths=1048576)]
+    assert b.profiled_tunings("missing.md") == []
 
 
 def test_parse_tuning():
