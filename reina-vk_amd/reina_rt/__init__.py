@@ -13,7 +13,8 @@ from . import abi
 from .scene import Material, ModelData, Scene, load_image, load_obj, make_desc, push_consts
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libreina_rt.so")
+# REINA_RT_LIBRARY selects another build of the library (default: the in-tree one)
+LIB_PATH = os.environ.get("REINA_RT_LIBRARY") or os.path.join(os.path.dirname(_HERE), "libreina_rt.so")
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "reina_rt.h")
 
 _lib = None
