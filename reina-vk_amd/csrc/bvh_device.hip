@@ -525,41 +525,120 @@ __device__ __forceinline__ float area_f(const BNode& b) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-// A subtree of at most kLeafMax triangles becomes one leaf slot when that is
-// no dearer than splitting it by the surface area heuristic (triangle test
-// and node visit cost 1 each), as in the host builder.
-__device__ __forceinline__ bool inner(const BNode* nodes, const BNode& x) {
-    if (x.left == kNone) return false;
-    if (x.count > kLeafMax) return true;
-    const float pa = area_f(x);
-    if (!(pa > 0.0f)) return false;
-    const BNode& l = nodes[x.left];
-    const BNode& r = nodes[x.right];
-    const float split = 1.0f + (area_f(l) * (float)l.count + area_f(r) * (float)r.count) / pa;
-    return (float)x.count > split;
+// SAH-optimal collapse (Ylitie, Karras and Laine, "Efficient incoherent ray
+// traversal on GPUs through compressed wide BVHs", HPG 2017): for every
+// binary node, C[i-1] is the least surface-area cost of its subtree spread
+// over i child slots of a BVH8 node (i = 1..8), computed bottom-up:
+//   C(x, 1) = min(leaf: A x kCollapseTriCost x count (count <= kLeafMax),
+//                 node: A x kCollapseNodeCost + D(x, 8))
+//   C(x, i) = min(C(x, i - 1), D(x, i)),  D(x, i) = min_k C(l, k) + C(r, i - k).
+// k[i-1] keeps the left child's slots of D(x, i) when that is the minimum
+// (0: C(x, i) = C(x, i - 1)); split8 is the argmin of D(x, 8), the 8 slots of
+// x's own BVH8 node. A triangle (id < n) is a leaf of cost A x kCollapseTriCost.
+struct DpRec {
+    float c[8];
+    uint8_t k[8];
+    uint8_t leaf, split8, pad0, pad1;
+};
+
+__device__ __forceinline__ float dp_cost(const BNode* nodes, const DpRec* dp, uint32_t n, uint32_t id, int slots) {
+    return id < n ? area_f(nodes[id]) * kCollapseTriCost : dp[id - n].c[slots - 1];
 }
 
-// Children of one BVH8 node in slot order (kNone = empty slot).
-__device__ void collapse_children(const BNode* nodes, uint32_t bn, uint32_t* slot_child) {
+// Parent of every binary node (the root's stays kNone) and cleared arrival counters.
+__global__ void k_parents(const BNode* __restrict__ nodes, uint32_t n, uint32_t* __restrict__ parent,
+                          uint32_t* __restrict__ arrive) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1u) return;
+    const uint32_t x = n + i;
+    parent[nodes[x].left] = x;
+    parent[nodes[x].right] = x;
+    arrive[i] = 0u;
+}
+
+// One thread per triangle walks up; the second thread to reach a node (both
+// children done) computes its record and goes on, the first stops. The
+// fences publish a record before its arrival is counted and order the
+// sibling's record before it is read (device scope: the XCDs' L2s).
+__global__ void k_collapse_dp(const BNode* __restrict__ nodes, uint32_t n, const uint32_t* __restrict__ parent,
+                              uint32_t* __restrict__ arrive, DpRec* dp) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t x = parent[t];
+    while (x != kNone) {
+        __threadfence();
+        if (atomicAdd(&arrive[x - n], 1u) == 0u) return;
+        __threadfence();
+        const BNode& b = nodes[x];
+        float cl[8], cr[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            cl[j] = dp_cost(nodes, dp, n, b.left, j + 1);
+            cr[j] = dp_cost(nodes, dp, n, b.right, j + 1);
+        }
+        DpRec r;
+        float dist[8];
+        uint8_t dk[8];
+        dist[0] = INFINITY;
+        dk[0] = 0;
+#pragma unroll
+        for (int j = 2; j <= 8; ++j) {
+            float best = INFINITY;
+            int bk = 1;
+            for (int k = 1; k < j; ++k) {
+                const float v = cl[k - 1] + cr[j - k - 1];
+                if (v < best) { best = v; bk = k; }
+            }
+            dist[j - 1] = best;
+            dk[j - 1] = (uint8_t)bk;
+        }
+        const float A = area_f(b);
+        const float cint = A * kCollapseNodeCost + dist[7];
+        const float cleaf = b.count <= kLeafMax ? A * kCollapseTriCost * (float)b.count : INFINITY;
+        r.leaf = cleaf <= cint ? 1 : 0;
+        r.split8 = dk[7];
+        r.pad0 = r.pad1 = 0;
+        r.c[0] = fminf(cleaf, cint);
+        r.k[0] = 0;
+#pragma unroll
+        for (int j = 2; j <= 8; ++j) {
+            if (dist[j - 1] < r.c[j - 2]) { r.c[j - 1] = dist[j - 1]; r.k[j - 1] = dk[j - 1]; }
+            else { r.c[j - 1] = r.c[j - 2]; r.k[j - 1] = 0; }
+        }
+        dp[x - n] = r;
+        x = parent[x];
+    }
+}
+
+// Whether binary node x becomes a BVH8 node (else a leaf slot).
+__device__ __forceinline__ bool inner(const DpRec* dp, uint32_t n, uint32_t x) {
+    return x >= n && dp[x - n].leaf == 0;
+}
+
+// Children of one BVH8 node in slot order (kNone = empty slot): the 8-slot
+// distribution of binary node bn, expanded.
+__device__ void collapse_children(const BNode* nodes, const DpRec* dp, uint32_t n, uint32_t bn, uint32_t* slot_child) {
     uint32_t ch[8];
     int nc = 0;
     const BNode& b = nodes[bn];
-    if (!inner(nodes, b)) {
+    if (!inner(dp, n, bn)) {
         ch[nc++] = bn;   // a leaf root
     } else {
-        ch[nc++] = b.left;
-        ch[nc++] = b.right;
-        while (nc < 8) {
-            int bi = -1;
-            float ba = -1.0f;
-            for (int c = 0; c < nc; ++c) {
-                const BNode& x = nodes[ch[c]];
-                if (inner(nodes, x) && area_f(x) > ba) { ba = area_f(x); bi = c; }
-            }
-            if (bi < 0) break;
-            const BNode& o = nodes[ch[bi]];
-            ch[bi] = o.left;
-            ch[nc++] = o.right;
+        uint32_t sx[8];
+        int sj[8], sp = 0;
+        const int k8 = dp[bn - n].split8;
+        sx[sp] = b.right; sj[sp++] = 8 - k8;
+        sx[sp] = b.left; sj[sp++] = k8;
+        while (sp > 0) {
+            const uint32_t m = sx[--sp];
+            int j = sj[sp];
+            if (m < n) j = 1;
+            while (j > 1 && dp[m - n].k[j - 1] == 0) j--;
+            if (j == 1) { ch[nc++] = m; continue; }
+            const int k = dp[m - n].k[j - 1];
+            const BNode& o = nodes[m];
+            sx[sp] = o.right; sj[sp++] = j - k;
+            sx[sp] = o.left; sj[sp++] = k;
         }
     }
     // octant slots: greedy on the signed centroid offsets from the node centre
@@ -593,25 +672,26 @@ __device__ void collapse_children(const BNode* nodes, uint32_t bn, uint32_t* slo
     }
 }
 
-__global__ void k_collapse_count(const BNode* __restrict__ nodes, const Item* __restrict__ items, uint32_t nitems,
+__global__ void k_collapse_count(const BNode* __restrict__ nodes, const DpRec* __restrict__ dp, uint32_t n,
+                                 const Item* __restrict__ items, uint32_t nitems,
                                  uint32_t* __restrict__ kids, uint32_t* __restrict__ n_int, uint32_t* __restrict__ n_tri) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nitems) return;
     uint32_t sc[8];
-    collapse_children(nodes, items[i].bnode, sc);
+    collapse_children(nodes, dp, n, items[i].bnode, sc);
     uint32_t ni = 0, nt = 0;
     for (int s = 0; s < 8; ++s) {
         kids[8 * i + s] = sc[s];
         if (sc[s] == kNone) continue;
-        const BNode& x = nodes[sc[s]];
-        if (inner(nodes, x)) ni++;
-        else nt += x.count;
+        if (inner(dp, n, sc[s])) ni++;
+        else nt += nodes[sc[s]].count;
     }
     n_int[i] = ni;
     n_tri[i] = nt;
 }
 
-__global__ void k_collapse_write(const BNode* __restrict__ nodes, const Item* __restrict__ items, uint32_t nitems,
+__global__ void k_collapse_write(const BNode* __restrict__ nodes, const DpRec* __restrict__ dp, uint32_t n,
+                                 const Item* __restrict__ items, uint32_t nitems,
                                  const uint32_t* __restrict__ kids, const uint32_t* __restrict__ scan_int,
                                  const uint32_t* __restrict__ scan_tri, uint32_t node_base, uint32_t tri_base0,
                                  const float* __restrict__ wtri,
@@ -638,7 +718,7 @@ __global__ void k_collapse_write(const BNode* __restrict__ nodes, const Item* __
         nd.hi_x[s] = nextafterf(x.hi[0], INFINITY);
         nd.hi_y[s] = nextafterf(x.hi[1], INFINITY);
         nd.hi_z[s] = nextafterf(x.hi[2], INFINITY);
-        if (inner(nodes, x)) {
+        if (inner(dp, n, c)) {
             meta[s] = (uint8_t)(0x80u | rank);
             next[scan_int[i] + rank] = Item{c, child_base + rank};
             imask |= 1u << s;
@@ -902,7 +982,18 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
             return false;
         }
     }
-    // collapse
+    // collapse: the SAH-optimal slot distribution of every binary node, bottom-up
+    DBuf<uint32_t> d_parent, d_arrive;
+    DBuf<DpRec> d_dp;
+    D_TRY(d_parent.alloc(2ull * n)); D_TRY(d_arrive.alloc(n)); D_TRY(d_dp.alloc(n));
+    D_TRY(hipMemsetAsync(d_parent.p, 0xFF, sizeof(uint32_t) * (2ull * n), st));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_parents, dim3(blocks(n - 1)), dim3(256), 0, st, d_nodes.p, n, d_parent.p, d_arrive.p);
+        D_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_collapse_dp, dim3(blocks(n)), dim3(256), 0, st, d_nodes.p, n, d_parent.p, d_arrive.p,
+                           d_dp.p);
+        D_TRY(hipGetLastError());
+    }
     const uint32_t n_internal = n - 1;
     DBuf<Bvh8Node> d_out;
     DBuf<float4> d_otri;
@@ -924,8 +1015,8 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
     Item* nitems_buf = d_it1.p;
     while (nitems > 0) {
         depth++;
-        hipLaunchKernelGGL(k_collapse_count, dim3(blocks(nitems, 64)), dim3(64), 0, st, d_nodes.p, items, nitems,
-                           d_ck.p, d_ni.p, d_nt.p);
+        hipLaunchKernelGGL(k_collapse_count, dim3(blocks(nitems, 64)), dim3(64), 0, st, d_nodes.p, d_dp.p, n, items,
+                           nitems, d_ck.p, d_ni.p, d_nt.p);
         D_TRY(hipGetLastError());
         D_TRY(hipMemsetAsync(d_ni.p + nitems, 0, 4, st));
         D_TRY(hipMemsetAsync(d_nt.p + nitems, 0, 4, st));
@@ -939,8 +1030,8 @@ bool build_bvh8_device(const float* tri, uint64_t ntri, Bvh8Result& out, std::st
             err = "device BVH collapse overflow";
             return false;
         }
-        hipLaunchKernelGGL(k_collapse_write, dim3(blocks(nitems, 64)), dim3(64), 0, st, d_nodes.p, items, nitems,
-                           d_ck.p, d_si.p, d_stri.p, out_nodes, out_tris, d_w.p, d_out.p, d_otri.p, nitems_buf);
+        hipLaunchKernelGGL(k_collapse_write, dim3(blocks(nitems, 64)), dim3(64), 0, st, d_nodes.p, d_dp.p, n, items,
+                           nitems, d_ck.p, d_si.p, d_stri.p, out_nodes, out_tris, d_w.p, d_out.p, d_otri.p, nitems_buf);
         D_TRY(hipGetLastError());
         out_nodes += tot[0];
         out_tris += tot[1];

@@ -5,10 +5,14 @@
 //   1. Binned SAH BVH2 (32 bins per axis, all three axes), leaves of at most
 //      3 triangles; nodes of one subtree are built in parallel on std::threads
 //      below the top levels.
-//   2. Collapse to BVH8: a node's children are grown from its two BVH2 children
-//      by repeatedly opening the internal child of largest surface area until
-//      there are 8 (Ylitie, Karras, Laine: "Efficient Incoherent Ray Traversal
-//      on GPUs Through Compressed Wide BVHs", HPG 2017).
+//   2. Collapse to BVH8 by the surface-area-optimal dynamic program of
+//      Ylitie, Karras and Laine ("Efficient Incoherent Ray Traversal on GPUs
+//      Through Compressed Wide BVHs", HPG 2017): bottom-up, the least cost of
+//      every BVH2 subtree in 1..8 slots of a BVH8 node (a leaf of at most 3
+//      triangles, a BVH8 node, or a split of the slots between its children;
+//      costs kCollapseNodeCost per node and kCollapseTriCost per triangle,
+//      times surface area), then each BVH8 node's children are its 8-slot
+//      distribution, expanded.
 //   3. Children are placed in slots by octant: slot s stands for the direction
 //      (+-1,+-1,+-1) of its bits, and a greedy assignment gives each slot the
 //      child whose centroid lies furthest in that direction from the parent's
@@ -247,6 +251,100 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
         th.join();
     }
 
+    // ---- SAH-optimal collapse (dynamic program over the binary tree) ----------
+    // Ylitie, Karras and Laine, HPG 2017, as in bvh_device.hip: C(n, i) is the
+    // least surface-area cost of subtree n spread over i slots of a BVH8 node;
+    // dk keeps the left child's slots of the best split (0: as with i - 1 slots).
+    const size_t nb = b.nodes.size();
+    std::vector<uint32_t> sub_first(nb), sub_cnt(nb);
+    std::vector<float> C(nb * 8);      // C[n * 8 + i - 1]: cost of subtree n in i slots
+    std::vector<int8_t> dk(nb * 8);    // i >= 2: the best split's left slots, or 0 = as with i - 1
+    std::vector<uint8_t> as_leaf(nb);  // the choice of C(n, 1): one leaf, else one BVH8 node
+    const float cN = kCollapseNodeCost, cP = kCollapseTriCost;
+    {
+        // nodes in pre-order; visited in reverse, children come before parents
+        std::vector<int32_t> order;
+        order.reserve(nb);
+        std::vector<int32_t> st{0};
+        while (!st.empty()) {
+            const int32_t n = st.back();
+            st.pop_back();
+            order.push_back(n);
+            if (b.nodes[n].left >= 0) {
+                st.push_back(b.nodes[n].left);
+                st.push_back(b.nodes[n].right);
+            }
+        }
+        for (size_t q = order.size(); q-- > 0;) {
+            const int32_t n = order[q];
+            const B2Node& x = b.nodes[n];
+            const float A = x.box.area();
+            float* c = &C[(size_t)n * 8];
+            if (x.left < 0) {
+                sub_first[n] = x.first;
+                sub_cnt[n] = x.count;
+                as_leaf[n] = 1;
+                for (int i = 0; i < 8; ++i) {
+                    c[i] = A * cP * (float)x.count;
+                    dk[(size_t)n * 8 + i] = 0;
+                }
+                continue;
+            }
+            sub_first[n] = std::min(sub_first[x.left], sub_first[x.right]);
+            sub_cnt[n] = sub_cnt[x.left] + sub_cnt[x.right];
+            const float* cl = &C[(size_t)x.left * 8];
+            const float* cr = &C[(size_t)x.right * 8];
+            float dist[8];
+            int8_t dsplit[8];
+            for (int j = 2; j <= 8; ++j) {
+                float best = INFINITY;
+                int bk = 1;
+                for (int k = 1; k < j; ++k) {
+                    const float v = cl[k - 1] + cr[j - k - 1];
+                    if (v < best) {
+                        best = v;
+                        bk = k;
+                    }
+                }
+                dist[j - 1] = best;
+                dsplit[j - 1] = (int8_t)bk;
+            }
+            const float cint = A * cN + dist[7];
+            const float cleaf = sub_cnt[n] <= kMaxLeafTris ? A * cP * (float)sub_cnt[n] : INFINITY;
+            as_leaf[n] = cleaf <= cint ? 1 : 0;
+            c[0] = std::min(cleaf, cint);
+            dk[(size_t)n * 8] = 0;
+            for (int j = 2; j <= 8; ++j) {
+                const bool split = dist[j - 1] < c[j - 2];
+                c[j - 1] = split ? dist[j - 1] : c[j - 2];
+                dk[(size_t)n * 8 + j - 1] = split ? dsplit[j - 1] : 0;
+            }
+        }
+    }
+    // children of a BVH8 node at binary node n: expand its 8-slot distribution
+    auto expand = [&](int32_t n, std::vector<Child>& ch) {
+        // the node's own 8 slots: k to the left child, 8 - k to the right
+        const int32_t l = b.nodes[n].left, r = b.nodes[n].right;
+        int k = 1;
+        float best = INFINITY;
+        for (int kk = 1; kk < 8; ++kk) {
+            const float v = C[(size_t)l * 8 + kk - 1] + C[(size_t)r * 8 + 8 - kk - 1];
+            if (v < best) {
+                best = v;
+                k = kk;
+            }
+        }
+        std::vector<std::pair<int32_t, int>> st{{r, 8 - k}, {l, k}};
+        while (!st.empty()) {
+            auto [m, j] = st.back(); st.pop_back();
+            while (j > 1 && dk[(size_t)m * 8 + j - 1] == 0) j--;
+            if (j == 1) { ch.push_back({m, as_leaf[m] != 0}); continue; }
+            const int kk = dk[(size_t)m * 8 + j - 1];
+            st.push_back({b.nodes[m].right, j - kk});
+            st.push_back({b.nodes[m].left, kk});
+        }
+    };
+
     // ---- collapse to BVH8, breadth-first ---------------------------------------
     struct Pending { int32_t b2; uint32_t node; uint32_t depth; };
     std::vector<Pending> queue;
@@ -261,22 +359,8 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
         out.max_depth = std::max(out.max_depth, pd.depth);
         const B2Node& src = b.nodes[pd.b2];
         std::vector<Child> ch;
-        if (src.left < 0) ch.push_back({pd.b2, true});
-        else { ch.push_back({src.left, b.nodes[src.left].left < 0}); ch.push_back({src.right, b.nodes[src.right].left < 0}); }
-        while (ch.size() < 8) {
-            int best = -1;
-            float ba = -1.0f;
-            for (size_t k = 0; k < ch.size(); ++k) {
-                if (ch[k].leaf) continue;
-                float a = b.nodes[ch[k].b2].box.area();
-                if (a > ba) { ba = a; best = (int)k; }
-            }
-            if (best < 0) break;
-            const B2Node& o = b.nodes[ch[best].b2];
-            Child l{o.left, b.nodes[o.left].left < 0}, r{o.right, b.nodes[o.right].left < 0};
-            ch[best] = l;
-            ch.push_back(r);
-        }
+        if (src.left < 0 || (pd.b2 == 0 && as_leaf[0])) ch.push_back({pd.b2, true});
+        else expand(pd.b2, ch);
         // octant slot assignment
         const Aabb& pb = src.box;
         float pc[3] = {0.5f * (pb.lo[0] + pb.hi[0]), 0.5f * (pb.lo[1] + pb.hi[1]), 0.5f * (pb.lo[2] + pb.hi[2])};
@@ -331,17 +415,18 @@ bool build_bvh8_host(const float* tri, uint64_t ntri, Bvh8Result& out, std::stri
             nd.hi_z[s] = std::nextafter(cn.box.hi[2], INFINITY);
             float ca = cn.box.area();
             if (ch[c].leaf) {
-                nd.meta[s] = leaf_meta(cn.count, toff);
-                for (uint32_t j = 0; j < cn.count; ++j) {
-                    uint32_t t = b.ref[cn.first + j];
+                const uint32_t lcount = sub_cnt[ch[c].b2], lfirst = sub_first[ch[c].b2];
+                nd.meta[s] = leaf_meta(lcount, toff);
+                for (uint32_t j = 0; j < lcount; ++j) {
+                    uint32_t t = b.ref[lfirst + j];
                     const float* v = &tri[9ull * t];
                     float id;
                     std::memcpy(&id, &t, 4);
                     float rec[12] = {v[0], v[1], v[2], id, v[3], v[4], v[5], 0.0f, v[6], v[7], v[8], 0.0f};
                     out.tris.insert(out.tris.end(), rec, rec + 12);
                 }
-                toff += cn.count;
-                if (root_area > 0.0f) sah += (double)ca / root_area * cn.count;
+                toff += lcount;
+                if (root_area > 0.0f) sah += (double)ca / root_area * lcount;
             } else {
                 nd.meta[s] = (uint8_t)(0x80u | rank);
                 out.nodes.emplace_back();

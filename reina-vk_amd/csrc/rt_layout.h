@@ -58,6 +58,11 @@ constexpr uint32_t kQNodeUsefulBytes = 80;
 constexpr uint32_t kTriUsefulBytes = 36;   // 3 x float3 per tested triangle
 constexpr uint32_t kMaxLeafTris = 3;
 constexpr uint32_t kMaxStackDepth = 32;    // largest LDS group stack per lane (entries)
+// Surface-area costs of the BVH8 collapse (both builders): a node visit, and a
+// triangle test relative to it (the traversal tests a leaf's triangles while
+// the next node is fetched, so a triangle costs about half a node).
+constexpr float kCollapseNodeCost = 1.0f;
+constexpr float kCollapseTriCost = 0.5f;
 
 // Triangles in leaf order: 3 x float4 = (v0.xyz, id bits), (v1.xyz, 0), (v2.xyz, 0).
 
