@@ -474,9 +474,14 @@ def bvh_build_block(a, arrays, dev, frame, img, torch):
     this builds it again on the host (REINA_RT_BUILD_HOST): build times, SAH
     cost and depth of both, the frame rate over the host-built tree (its own
     default tuning), and a check that the frame is bit-identical (the closest
-    hit does not depend on the tree)."""
+    hit does not depend on the tree). The headline scene was the process's
+    first, so its build time includes the runtime's first-use costs; the
+    device build is timed again here on a second scene."""
     import reina_rt
     from reina_rt import abi
+    dd = reina_rt.DeviceScene(arrays, flags=abi.BUILD_DEVICE)
+    warm_ms = float(dd.info().build_ms)
+    dd.close()
     t0 = time.perf_counter()
     dh = reina_rt.DeviceScene(arrays, flags=abi.BUILD_HOST)
     setup = time.perf_counter() - t0
@@ -486,7 +491,8 @@ def bvh_build_block(a, arrays, dev, frame, img, torch):
     frame(0, dh)
     same = bool(torch.equal(ref.view(torch.int32), img.view(torch.int32)))
     rays, elapsed, _ = time_frames(lambda b, d=dh: frame(b, d), a.warmup, a.steps, torch)
-    res = {"device_build_ms": round(float(di.build_ms), 1), "host_build_ms": round(float(hi.build_ms), 1),
+    res = {"device_build_ms": round(warm_ms, 1), "device_first_build_ms": round(float(di.build_ms), 1),
+           "host_build_ms": round(float(hi.build_ms), 1),
            "host_scene_setup_s": round(setup, 3), "device_sah_cost": round(float(di.sah_cost), 3),
            "host_sah_cost": round(float(hi.sah_cost), 3), "device_bvh_depth": int(di.max_depth),
            "host_bvh_depth": int(hi.max_depth), "device_bvh_nodes": int(di.num_nodes),
