@@ -370,8 +370,8 @@ int reina_rt_intersect(ReinaScene* scene, const ReinaRay* rays, ReinaHit* hits,
 /* reina_rt_intersect without any allocation (no counterpart in the
  * reference): for callers that cannot allocate device memory, such as code
  * under HIP graph capture and tight serving loops. The traversal kernel's
- * claim counters (1 KiB) live in a static device array of the library, one
- * per GPU, held until the process ends, and are zeroed by a fill kernel on
+ * claim counters and used-up flag (1152 bytes) live in a static device
+ * array of the library, one per GPU, held until the process ends, and are zeroed by a fill kernel on
  * `stream` at every call. At most 2^30 rays per call (REINA_RT_ERR_INVALID_ARG
  * beyond). Calls that overlap on one GPU (different streams or threads) race
  * on the shared counters; serialise them. Hits equal reina_rt_intersect's. */

@@ -458,8 +458,9 @@ int reina_rt_bvh_quantize_host(const void* nodes, uint64_t nodes_bytes, void* qn
 
 static int ensure_trace_buffers(ReinaScene* s, uint64_t npx) {
     if (!s->trace_ready) {
-        // 3 iteration slots, the stats words, and the claim counters of reina_rt_intersect
-        if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 256 + kParts * kLine * 4));
+        // 3 iteration slots, the stats words, and the claim counters and used-up
+        // flag of reina_rt_intersect
+        if (!s->ctrl) HIP_TRY(hipMalloc(&s->ctrl, 3 * sizeof(Ctrl) + 256 + (kParts + 1) * kLine * 4));
         s->stats = (unsigned long long*)((char*)s->ctrl + 3 * sizeof(Ctrl));
         s->fetch = (uint32_t*)((char*)s->ctrl + 3 * sizeof(Ctrl) + 256);
         if (!s->h_count) HIP_TRY(hipHostMalloc((void**)&s->h_count, 2 * sizeof(Ctrl::count), hipHostMallocDefault));
@@ -551,7 +552,7 @@ int reina_rt_trace(ReinaScene* s, const RtPushConsts* pc, float* image, uint32_t
             }
             HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used], st));
         }
-        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
+        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0], &s->ctrl[slot].done[0]};
         HIP_TRY(launch_trace(s->sv, s->Q.ray[in_q], s->Q.hit, tin, s->stats, nullptr, count, s->tune, st));
         if (timex) HIP_TRY(hipEventRecord(s->xev[2 * s->xev_used++ + 1], st));
         HIP_TRY(launch_shade(s->sv, P, s->S, s->Q, in_q, s->ctrl, slot, s->tune.shade_grid, nee, st));
@@ -672,8 +673,8 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
     if (n == 0) return REINA_RT_OK;
     const bool h16 = (s->options & REINA_RT_TRACE_F16_INV) != 0;
     if (h16 && s->sv.qnodes) return fail(REINA_RT_ERR_UNSUPPORTED, "FP16 inverse directions need FP32 nodes");
-    HIP_TRY(hipMemsetAsync(s->fetch, 0, kParts * kLine * 4, st));
-    TraceInput tin{nullptr, 0, n, s->fetch};
+    HIP_TRY(hipMemsetAsync(s->fetch, 0, (kParts + 1) * kLine * 4, st));
+    TraceInput tin{nullptr, 0, n, s->fetch, s->fetch + kParts * kLine};
     if (h16)
         HIP_TRY(launch_trace_f32_h16(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, counters,
                                      counters != nullptr, s->tune, st));
@@ -693,7 +694,7 @@ int reina_rt_intersect(ReinaScene* s, const ReinaRay* rays, ReinaHit* hits, uint
 // device with that device current. Host code never passes `scratch` itself
 // to a kernel: in host code the name is a stand-in, not a device address.
 namespace {
-constexpr uint32_t kNoAllocScratchWords = kParts * kLine;   // 1 KiB
+constexpr uint32_t kNoAllocScratchWords = (kParts + 1) * kLine;   // claim counters and used-up flag, 1152 bytes
 constexpr uint32_t kNoAllocMaxRays = 1u << 30;   // keeps every claim counter far below 2^32
 constexpr int kNoAllocMaxDevices = 64;
 }  // namespace
@@ -743,7 +744,7 @@ int reina_rt_intersect_noalloc(ReinaScene* s, const ReinaRay* rays, ReinaHit* hi
     // the call can be captured in a HIP graph
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(256), 0, st, fetch, kNoAllocScratchWords);
     HIP_TRY(hipGetLastError());
-    TraceInput tin{nullptr, 0, n, fetch};
+    TraceInput tin{nullptr, 0, n, fetch, fetch + kParts * kLine};
     HIP_TRY(launch_trace(s->sv, (const float4*)rays, (float4*)hits, tin, nullptr, nullptr, false, s->tune, st));
     return REINA_RT_OK;
 }

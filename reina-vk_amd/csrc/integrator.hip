@@ -44,6 +44,10 @@ __global__ __launch_bounds__(256) void k_init(TraceParams P, PathState S, Queues
         ctrl[slot].count[p][0] = c;
         ctrl[slot].fetch[p][0] = 0;
         ctrl[slot].sfetch[p][0] = 0;
+        if (p == 0) {
+            ctrl[slot].done[0] = 0;
+            ctrl[slot].sdone[0] = 0;
+        }
         if (threadIdx.x == 0 && base == 0) { stats[0] = 0; stats[1] = 0; stats[2] = 0; stats[3] = 0; stats[4] = 0; }
     }
     if (l >= npx) return;
@@ -85,6 +89,10 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sv, TraceParams P, Path
         ctrl[zslot].count[threadIdx.x][0] = 0;
         ctrl[zslot].fetch[threadIdx.x][0] = 0;
         ctrl[zslot].sfetch[threadIdx.x][0] = 0;
+        if (threadIdx.x == 0) {
+            ctrl[zslot].done[0] = 0;
+            ctrl[zslot].sdone[0] = 0;
+        }
     }
     const uint32_t p = blockIdx.x % kParts;
     const uint32_t nblk = gridDim.x / kParts;       // blocks serving this partition

@@ -130,7 +130,7 @@ static hipError_t capture_graph(ReinaServer* s) {
     if (e != hipSuccess) return e;
     for (uint32_t i = 0; i < s->graph_iters; ++i) {
         const uint32_t slot = i % 3, in_q = i & 1u;
-        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0]};
+        TraceInput tin{&s->ctrl[slot].count[0][0], s->Q.cap, 0, &s->ctrl[slot].fetch[0][0], &s->ctrl[slot].done[0]};
         launch_trace_multi(s->quantized, s->Q.ray[in_q], s->Q.hit, tin, s->stats, s->tune, s->Q.pix[in_q],
                            s->jobs_d, s->st);
         launch_shade_multi(s->jobs_d, s->S, s->Q, in_q, s->ctrl, slot, s->done_d, s->tune.shade_grid, s->nee, s->st);

@@ -95,6 +95,10 @@ __global__ __launch_bounds__(256) void k_shade_multi(const JobRec* jobs, PathSta
         ctrl[zslot].count[threadIdx.x][0] = 0;
         ctrl[zslot].fetch[threadIdx.x][0] = 0;
         ctrl[zslot].sfetch[threadIdx.x][0] = 0;
+        if (threadIdx.x == 0) {
+            ctrl[zslot].done[0] = 0;
+            ctrl[zslot].sdone[0] = 0;
+        }
     }
     const uint32_t p = blockIdx.x % kParts;
     const uint32_t nblk = gridDim.x / kParts;

@@ -25,7 +25,14 @@ __device__ __forceinline__ void part_range(const TraceInput& in, uint32_t p, uin
 // Closest-hit kernel, persistent with per-lane refill: a lane whose ray is
 // finished goes idle; once at least `refill_min` lanes of the wave are idle
 // they take the next rays from the wave's current chunk (`chunk` rays claimed
-// with one atomic on its partition's counter, own XCD's partition first).
+// with one atomic add on its partition's counter, own XCD's partition first).
+// A claim reads in.done, a mask of the used-up partitions on a line of its
+// own, and skips them. On the partition it claimed from last it adds without
+// reading the counter first: a read of a line that other waves' atomics hit
+// costs several times the add. On the others it reads the counter first, which
+// keeps failed adds (as costly) rare when rays run out. The claim that takes a
+// partition's last rays sets its bit, so waves that start after every
+// partition is used up stop after one read.
 // Each loop iteration advances every active lane by one node visit.
 // MULTI: the rays belong to several requests (possibly different scenes):
 // pix[idx] >> 24 names the request whose scene the ray is traced in; sv is
@@ -61,6 +68,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     uint32_t part = blockIdx.x % kParts;    // wave-uniform: partition to claim from next
     bool active = false;
     bool exhausted = false;                 // wave-uniform
+    uint32_t used_up = 0;                   // wave-uniform: partitions found used up
     uint32_t pool_next = 0, pool_end = 0;   // wave-uniform
     uint32_t idx = 0;
     TRay r;
@@ -78,25 +86,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
             if (pool_next >= pool_end) {
                 // claim a chunk: own partition first, then the others; a plain
                 // read skips partitions that are already used up
-                uint32_t got_start = 0xFFFFFFFFu, got_end = 0, got_part = part;
+                uint32_t got_start = 0xFFFFFFFFu, got_end = 0, got_part = part, got_used = used_up;
                 if (lane == 0) {
+                    if (in.done) got_used |= __hip_atomic_load(in.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     for (uint32_t k = 0; k < kParts; ++k) {
-                        uint32_t p = (part + k) % kParts;
+                        const uint32_t p = (part + k) % kParts;
+                        if (got_used & (1u << p)) continue;
                         uint32_t a, sz;
                         part_range(in, p, a, sz);
                         uint32_t* f = in.fetch + p * kLine;
-                        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sz) continue;
-                        uint32_t b = atomicAdd(f, chunk);
-                        if (b >= sz) continue;
-                        got_start = a + b;
-                        got_end = a + min(b + chunk, sz);
-                        got_part = p;
-                        break;
+                        if (sz == 0u ||
+                            (k > 0u && __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sz)) {
+                            got_used |= 1u << p;
+                            continue;
+                        }
+                        const uint32_t b = atomicAdd(f, chunk);
+                        if (b < sz) {
+                            got_start = a + b;
+                            got_end = a + min(b + chunk, sz);
+                            got_part = p;
+                            // the claim that takes the partition's last rays marks it used up
+                            if (in.done && sz - b <= chunk) atomicOr(in.done, 1u << p);
+                            break;
+                        }
+                        got_used |= 1u << p;
                     }
                 }
-                got_start = __shfl(got_start, 0);
-                got_end = __shfl(got_end, 0);
-                part = __shfl(got_part, 0);
+                // every lane is active here, so the first active lane is lane 0
+                // and its values come back in scalar registers
+                got_start = __builtin_amdgcn_readfirstlane(got_start);
+                got_end = __builtin_amdgcn_readfirstlane(got_end);
+                part = __builtin_amdgcn_readfirstlane(got_part);
+                used_up = __builtin_amdgcn_readfirstlane(got_used);
                 if (got_start == 0xFFFFFFFFu) {
                     exhausted = true;
                 } else {
@@ -233,7 +254,7 @@ static hipError_t launch_trace_fmt(const SceneView& sv, const float4* rays, floa
 template <bool Q, int SD>
 static void launch_connect_sd(const SceneView& sv, const Queues& Q_, const PathState& S, Ctrl* ctrl, uint32_t slot,
                               unsigned long long* stats, const Tuning& tu, hipStream_t st) {
-    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
+    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0], &ctrl[slot].sdone[0]};
     hipLaunchKernelGGL((k_trace<SD, 1, false, Q, false, true>), dim3(tu.grid), dim3(256), 0, st, sv, Q_.sray,
                        nullptr, tin, stats, nullptr, tu.refill_min, tu.claim, nullptr, nullptr, AnyHitOut{S.st2, Q_.scon});
 }
@@ -256,7 +277,7 @@ template <bool Q, int SD>
 static void launch_connect_multi_sd(const JobRec* jobs, const Queues& Q_, const PathState& S, Ctrl* ctrl,
                                     uint32_t slot, const uint32_t* pix, unsigned long long* stats, const Tuning& tu,
                                     hipStream_t st) {
-    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0]};
+    TraceInput tin{&ctrl[slot].count[0][0], Q_.cap, 0, &ctrl[slot].sfetch[0][0], &ctrl[slot].sdone[0]};
     hipLaunchKernelGGL((k_trace<SD, 1, false, Q, true, true>), dim3(tu.grid), dim3(256), 0, st, SceneView{}, Q_.sray,
                        nullptr, tin, stats, nullptr, tu.refill_min, tu.claim, pix, jobs, AnyHitOut{S.st2, Q_.scon});
 }

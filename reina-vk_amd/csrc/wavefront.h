@@ -30,12 +30,17 @@ constexpr uint32_t kParts = 8;
 constexpr uint32_t kLine = 32;   // uint32 words per 128-byte line
 
 // Control words of one bounce iteration: per partition the queue length and
-// the extend kernel's claim counter, each on its own 128-byte line. Three
-// slots rotate so that iteration i's shade zeroes the slot of iteration i+2.
+// the extend kernel's claim counter, each on its own 128-byte line, and the
+// mask of the partitions whose rays are all claimed. Three slots rotate so
+// that iteration i's shade zeroes the slot of iteration i+2.
 struct Ctrl {
     uint32_t count[kParts][kLine];
     uint32_t fetch[kParts][kLine];
     uint32_t sfetch[kParts][kLine];   // claim counters of the shadow-ray kernel
+    // word 0: bit p is set by the claim that takes the last rays of the
+    // extend kernel's partition p (sdone: of the shadow-ray kernel's)
+    uint32_t done[kLine];
+    uint32_t sdone[kLine];
 };
 
 // Pixel i starts in partition ((i >> 10) & 7) at position
@@ -100,12 +105,15 @@ struct Tuning {
 // Rays to trace: kParts partitions. Partitioned mode (counts != null):
 // partition p holds counts[p * kLine] rays starting at p * cap. Flat mode:
 // n rays split into kParts contiguous ranges. fetch: kParts claim counters
-// (stride kLine), zero before the launch.
+// (stride kLine), zero before the launch; done (optional): a word on a line of
+// its own, zero before the launch, whose bit p the claim of partition p's last
+// rays sets.
 struct TraceInput {
     const uint32_t* counts;
     uint32_t cap;
     uint32_t n;
     uint32_t* fetch;
+    uint32_t* done;
 };
 
 // Closest hit for every ray of `in`. stats (optional): stats[0] += rays;
