@@ -32,6 +32,52 @@
 
 namespace rt {
 
+// The queue entries for the lanes of a wave that want one (`want`): one
+// atomic add for all of them on the partition the wave claimed from last (the
+// counter is read first only when stealing from another partition: a read of
+// a line that other waves' atomics hit costs several times the add). Every
+// lane of the wave must be active. `part` and `used` (a mask of partitions
+// found used up) are wave-uniform; `drained` is set once every partition is
+// used up. Returns this lane's entry index, or 0xFFFFFFFF for none.
+__device__ __forceinline__ uint32_t tail_claim(Ctrl* ctrl, uint32_t slot, uint32_t cap, bool want, uint32_t& part,
+                                               uint32_t& used, bool& drained) {
+    const uint64_t wm = __ballot(want);
+    if (wm == 0ull || drained) return 0xFFFFFFFFu;
+    const uint32_t nw = (uint32_t)__popcll(wm);
+    uint32_t gs = 0xFFFFFFFFu, gn = 0, gp = part, gu = used;
+    if (__lane_id() == 0) {
+        for (uint32_t k = 0; k < kParts; ++k) {
+            const uint32_t p = (part + k) % kParts;
+            if (gu & (1u << p)) continue;
+            const uint32_t n = ctrl[slot].count[p][0];
+            uint32_t* f = &ctrl[slot].fetch[p][0];
+            if (n == 0u || (k > 0u && __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n)) {
+                gu |= 1u << p;
+                continue;
+            }
+            const uint32_t j = atomicAdd(f, nw);
+            if (j < n) {
+                gs = p * cap + j;
+                gn = min(nw, n - j);
+                gp = p;
+                break;
+            }
+            gu |= 1u << p;
+        }
+    }
+    gs = __builtin_amdgcn_readfirstlane(gs);
+    gn = __builtin_amdgcn_readfirstlane(gn);
+    part = __builtin_amdgcn_readfirstlane(gp);
+    used = __builtin_amdgcn_readfirstlane(gu);
+    if (gs == 0xFFFFFFFFu) {
+        drained = true;
+        return 0xFFFFFFFFu;
+    }
+    const uint64_t lt = (__lane_id() == 0) ? 0ull : (~0ull >> (64 - __lane_id()));
+    const uint32_t rank = (uint32_t)__popcll(wm & lt);
+    return (want && rank < gn) ? gs + rank : 0xFFFFFFFFu;
+}
+
 // Register budget: 3 waves per SIMD without next-event estimation (168
 // VGPRs, a few bytes more scratch than the compiler's own 188 VGPRs at 2
 // waves; the 1080p frame measured 2.6% faster); the NEE build keeps the
@@ -44,8 +90,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
     uint2* stk = stack + threadIdx.x;
     const float4* rin = Q_.ray[in_q];
     const uint32_t* pin = Q_.pix[in_q];
-    uint32_t part = blockIdx.x % kParts;
-    bool have = false, drained = false;
+    uint32_t part = blockIdx.x % kParts;   // wave-uniform
+    uint32_t used = 0;                      // wave-uniform: partitions found used up
+    bool have = false, drained = false;     // drained: wave-uniform
     uint32_t pw = 0, pp = 0, px = 0;
     const JobRec* J = nullptr;
     const void* nodes = Q ? static_cast<const void*>(sv.qnodes) : static_cast<const void*>(sv.nodes);
@@ -63,25 +110,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
         h.t = 0.0f; h.prim = 0u; h.u = 0.0f; h.v = 0.0f;
         trav_begin(ts);
         for (;;) {
-            if (!have && !drained) {
-                for (uint32_t k = 0; k < kParts && !have; ++k) {
-                    const uint32_t p = (part + k) % kParts;
-                    const uint32_t n = ctrl[slot].count[p][0];
-                    uint32_t* f = &ctrl[slot].fetch[p][0];
-                    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
-                    const uint32_t j = atomicAdd(f, 1u);
-                    if (j >= n) continue;
-                    const uint32_t i = p * Q_.cap + j;
-                    pw = pin[i];
-                    ra = rin[2 * i];
-                    rb = rin[2 * i + 1];
-                    have = true;
-                    fresh = true;
-                    part = p;
-                }
-                if (!have) {
-                    drained = true;
-                } else if (MULTI) {
+            const uint32_t i = tail_claim(ctrl, slot, Q_.cap, !have, part, used, drained);
+            if (i != 0xFFFFFFFFu) {
+                pw = pin[i];
+                ra = rin[2 * i];
+                rb = rin[2 * i + 1];
+                have = true;
+                fresh = true;
+                if (MULTI) {
                     J = jobs + (pw >> kSlotBits);
                     pp = pw & ((1u << kSlotBits) - 1u);
                     px = pp - J->px_base;
@@ -151,25 +187,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NEE ? 0 : 3
             }
         }
     } else for (;;) {
-        // lanes without a pixel take the next queue entry (own XCD's partition first)
-        if (!have && !drained) {
-            for (uint32_t k = 0; k < kParts && !have; ++k) {
-                const uint32_t p = (part + k) % kParts;
-                const uint32_t n = ctrl[slot].count[p][0];
-                uint32_t* f = &ctrl[slot].fetch[p][0];
-                if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
-                const uint32_t j = atomicAdd(f, 1u);
-                if (j >= n) continue;
-                const uint32_t i = p * Q_.cap + j;
-                pw = pin[i];
-                ra = rin[2 * i];
-                rb = rin[2 * i + 1];
-                have = true;
-                part = p;
-            }
-            if (!have) {
-                drained = true;
-            } else if (MULTI) {
+        // lanes without a pixel take the next queue entries (own XCD's partition first)
+        const uint32_t i = tail_claim(ctrl, slot, Q_.cap, !have, part, used, drained);
+        if (i != 0xFFFFFFFFu) {
+            pw = pin[i];
+            ra = rin[2 * i];
+            rb = rin[2 * i + 1];
+            have = true;
+            if (MULTI) {
                 J = jobs + (pw >> kSlotBits);
                 pp = pw & ((1u << kSlotBits) - 1u);
                 px = pp - J->px_base;
