@@ -153,6 +153,34 @@ def test_intersect_bitexact_icosphere(cuda, icosphere_arrays):
     dev.close()
 
 
+def test_intersect_claims_any_batch_size(cuda, icosphere_arrays):
+    """The traversal kernel's claims give every ray exactly once, whatever the
+    batch size and launch shape: partitions with no rays (fewer rays than
+    partitions), claims that take a partition's last rays exactly or run past
+    its end, and grids far larger than the batch, whose late waves find every
+    partition used up. Hits equal the full batch's (pinned to the oracle by
+    test_intersect_bitexact_icosphere) and, for the small batches, the
+    oracle's brute-force scan."""
+    torch = cuda
+    arrays, cam = icosphere_arrays
+    dev = reina_rt.DeviceScene(arrays)
+    orc = oracle.OracleScene(arrays, make_desc)
+    rays = random_rays(70001, -1.5, 1.5, 21)
+    full = gpu_intersect(torch, dev, rays)
+    small = rays[:1025]
+    ref = orc.brute32(small)
+    assert np.array_equal(full["prim"][:1025], ref["prim"])
+    assert np.array_equal(_bits(full["t"][:1025]), _bits(ref["t"]))
+    assert (full["prim"] != 0xFFFFFFFF).sum() > 20000   # the batch hits the sphere
+    for grid, claim in ((1, 64), (8, 512), (256, 128), (2048, 64), (4096, 512)):
+        dev.set_tuning(grid_blocks=grid, claim_rays=claim)
+        for m in (1, 5, 8, 9, 63, 64, 65, 1023, 1025, 8 * 512 + 3, 70001):
+            h = gpu_intersect(torch, dev, rays[:m])
+            assert np.array_equal(_bits(h.view(np.float32).reshape(-1, 4)),
+                                  _bits(full[:m].view(np.float32).reshape(-1, 4))), (grid, claim, m)
+    dev.close()
+
+
 def test_intersect_quantized_bitexact(cuda, icosphere_arrays):
     """8-bit quantized nodes: the GPU hit equals brute force bit for bit, and
     the per-ray node visits / triangle tests equal the oracle's traversal of
