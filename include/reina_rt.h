@@ -344,8 +344,10 @@ int reina_rt_set_trace_options(ReinaScene* scene, uint32_t options);
 int reina_rt_trace_stats(ReinaScene* scene, ReinaTraceStats* out);
 
 /* Launch parameters of the traversal / shading kernels. Defaults are set at
- * scene creation (stack depth from the BVH depth); the autotuner in the
- * Python package searches them per scene. */
+ * scene creation (stack depth from the BVH depth, refill 32, 8 traversal
+ * blocks per CU at stack depth 8 and 4 above, 16 shade blocks per CU, the
+ * compiler's register budget, tail launch at 2^20 live paths, claims of 128
+ * rays); the autotuner in the Python package searches them per scene. */
 typedef struct ReinaTuning {
     uint32_t stack_depth;   /* LDS stack entries per lane: 8, 16, 24 or 32 (>= BVH depth - 1) */
     uint32_t refill_min;    /* idle lanes per wave that trigger a refill, 1..64                */
@@ -436,7 +438,8 @@ typedef struct ReinaServerDesc {
     uint32_t quantized;          /* node format of every scene served: 0 FP32, 1 8-bit   */
     ReinaTuning tuning;          /* shared-kernel launch parameters; zeros = defaults
                                     (stack_depth bounds the BVH depth of served scenes;
-                                    tail_paths 0 = 2^19, REINA_RT_TAIL_OFF = never)      */
+                                    tail_paths 0 = 2^20, refill_min 0 = 32,
+                                    REINA_RT_TAIL_OFF = never)                           */
     uint32_t flags;              /* REINA_RT_SERVER_NEE or 0                             */
     uint32_t admit_paths;        /* admission stops while this many paths are in flight
                                     in the shared queues (live paths of the last graph +

@@ -340,7 +340,7 @@ int reina_rt_scene_create(const ReinaSceneDesc* d, ReinaScene** out) {
     uint32_t cus = (uint32_t)prop.multiProcessorCount;
     uint32_t need = bvh.max_depth > 1 ? bvh.max_depth - 1 : 1;
     s->tune.stack_depth = need <= 8 ? 8 : (need <= 16 ? 16 : (need <= 24 ? 24 : 32));
-    s->tune.refill_min = 16;
+    s->tune.refill_min = 32;
     s->tune.grid = cus * (s->tune.stack_depth <= 8 ? 8 : 4);
     s->tune.shade_grid = cus * 16;   // fills every CU to its register-limited occupancy (latency hiding)
     s->tune.tail_paths = kDefaultTailPaths;

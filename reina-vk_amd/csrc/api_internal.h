@@ -26,7 +26,7 @@ int fail(int code, const std::string& msg);
 
 // Live paths at or below which reina_rt_trace ends the frame in one tail
 // launch (ReinaTuning.tail_paths default).
-constexpr uint32_t kDefaultTailPaths = 1u << 19;
+constexpr uint32_t kDefaultTailPaths = 1u << 20;
 
 // Kernel parameters of one vkCmdTraceRaysKHR-equivalent render.
 inline TraceParams make_trace_params(const RtPushConsts* pc, uint32_t W, uint32_t H) {

@@ -461,7 +461,7 @@ int reina_rt_server_create(const ReinaServerDesc* d, ReinaServer** out) {
     if (tu.stack_depth == 0) tu.stack_depth = 16;
     if (tu.stack_depth != 8 && tu.stack_depth != 16 && tu.stack_depth != 24 && tu.stack_depth != 32)
         return fail(REINA_RT_ERR_INVALID_ARG, "stack_depth must be 8, 16, 24 or 32");
-    if (tu.refill_min == 0) tu.refill_min = 16;
+    if (tu.refill_min == 0) tu.refill_min = 32;
     if (tu.refill_min > 64) return fail(REINA_RT_ERR_INVALID_ARG, "refill_min must be 1..64");
     if (tu.waves_per_simd != 0 && tu.waves_per_simd != 4 && tu.waves_per_simd != 5 && tu.waves_per_simd != 6 && tu.waves_per_simd != 8)
         return fail(REINA_RT_ERR_INVALID_ARG, "waves_per_simd must be 0, 4, 5, 6 or 8");

@@ -101,7 +101,7 @@ def test_server_mix_equals_alone(cuda, pool, tail):
     partial last tile, spp, depths, accumulation with sampleBatch > 0)
     submitted together: each image equals its reina_rt_trace image and its
     image when served alone, whether the shared queues end with the tail
-    launch at once (default threshold 2^19 paths), never, or after some
+    launch at once (default threshold 2^20 paths), never, or after some
     graph iterations (3000 paths)."""
     torch = cuda
     jobs = _mix_jobs(pool)
@@ -166,7 +166,10 @@ def test_server_admit_order(cuda, pool, order, max_wait_ms, expect):
     bimg = torch.zeros((1024, 1024, 4), dtype=torch.float32, device="cuda")
     bpc = _pc(icam, 1024, 1024, 64, 8)
     torch.cuda.synchronize()   # (a device-wide sync after this point would also wait for the server's stream)
-    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, admit_order=order, max_wait_ms=max_wait_ms)
+    # tail at 2^19 paths: the long request (2^20 pixels) stays in the shared
+    # queues until half its pixels are done, so the three wait behind it
+    srv = reina_rt.Server(path_budget_bytes=256 << 20, admit_paths=1, admit_order=order, max_wait_ms=max_wait_ms,
+                          tail_paths=1 << 19)
     btk = srv.submit(idev, bpc, bimg.data_ptr(), 1024, 1024)
     for _ in range(5000):
         if srv.poll(btk).state == abi.REQ_RUNNING:
