@@ -287,6 +287,12 @@ RT_D bool leaf_tri(const TRay& r, float4 p0, float4 p1, float4 p2, HitRec& h, co
 template <bool COUNT, bool Q, bool ANY = false, bool H16 = false>
 RT_D bool trav_step(const void* nodes, const float4* tris, const TRay& r, HitRec& h, TravState& ts, uint2* stk,
                     uint32_t stride, uint32_t& nvis, uint32_t& ntst, const uint8_t* tflags = nullptr) {
+    // Every load of the previous step has been consumed by its test, so this
+    // wait costs nothing; it tells the compiler so. Without it, the compiler
+    // sees a path on which a node fetch is never tested (the test is under its
+    // own do_node branch) and waits for all loads before this step's node
+    // fetch, which then no longer overlaps the leaf fetch issued before it.
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt and lgkmcnt unconstrained
     // fetch the first two triangles of the lowest pending leaf; a third one
     // stays pending as a one-triangle leaf (its record rewritten) for the next step
     const bool do_leaf = ts.pm != 0u;
