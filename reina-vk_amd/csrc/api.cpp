@@ -211,37 +211,37 @@ static int scene_upload(ReinaScene* s, const ReinaSceneDesc* d, const std::vecto
     for (uint32_t t = 0; t < d->num_textures; ++t)   // byte order r, g, b, a = the packed word's low to high byte
         HIP_TRY(hipMemcpy(a + parts[p_texel].off + 4 * tinfo[t].base, d->textures[t].rgba8,
                           4ull * tinfo[t].w * tinfo[t].h, hipMemcpyHostToDevice));
-    s->sv.nodes = s->quantized ? nullptr : (const Bvh8Node*)(a + parts[p_nodes].off);
-    s->sv.qnodes = s->quantized ? (const QBvh8Node*)(a + parts[p_nodes].off) : nullptr;
-    s->sv.tris = (const float4*)(a + parts[p_tris].off);
-    s->sv.tri_inst = (const uint32_t*)(a + parts[p_ti].off);
-    s->sv.tri_prim = (const uint32_t*)(a + parts[p_tp].off);
-    s->sv.inst = (const InstanceRec*)(a + parts[p_in].off);
-    s->sv.props = (const ReinaInstanceProperties*)(a + parts[p_pr].off);
-    s->sv.vertices = (const float4*)(a + parts[p_v].off);
-    s->sv.indices = (const uint32_t*)(a + parts[p_i].off);
-    s->sv.tbns = (const float*)(a + parts[p_t].off);
-    s->sv.tbn_indices = (const uint32_t*)(a + parts[p_tix].off);
-    s->sv.tex_coords = (const float*)(a + parts[p_tc].off);
-    s->sv.tex_indices = (const uint32_t*)(a + parts[p_tci].off);
+    s->sv.nodes = s->quantized ? nullptr : (const Bvh8Node RT_GLOBAL*)(a + parts[p_nodes].off);
+    s->sv.qnodes = s->quantized ? (const QBvh8Node RT_GLOBAL*)(a + parts[p_nodes].off) : nullptr;
+    s->sv.tris = (const float4 RT_GLOBAL*)(a + parts[p_tris].off);
+    s->sv.tri_inst = (const uint32_t RT_GLOBAL*)(a + parts[p_ti].off);
+    s->sv.tri_prim = (const uint32_t RT_GLOBAL*)(a + parts[p_tp].off);
+    s->sv.inst = (const InstanceRec RT_GLOBAL*)(a + parts[p_in].off);
+    s->sv.props = (const ReinaInstanceProperties RT_GLOBAL*)(a + parts[p_pr].off);
+    s->sv.vertices = (const float4 RT_GLOBAL*)(a + parts[p_v].off);
+    s->sv.indices = (const uint32_t RT_GLOBAL*)(a + parts[p_i].off);
+    s->sv.tbns = (const float RT_GLOBAL*)(a + parts[p_t].off);
+    s->sv.tbn_indices = (const uint32_t RT_GLOBAL*)(a + parts[p_tix].off);
+    s->sv.tex_coords = (const float RT_GLOBAL*)(a + parts[p_tc].off);
+    s->sv.tex_indices = (const uint32_t RT_GLOBAL*)(a + parts[p_tci].off);
     s->sv.num_tbns = d->num_tbns;
     s->sv.num_tbn_indices = d->num_tbn_indices;
     s->sv.num_tex_coords = d->num_tex_coords;
     s->sv.num_nodes = s->host_nodes.size();
     s->sv.num_tris = tri_inst.size();
-    s->sv.emi = (const EmissiveInst*)(a + parts[p_emi].off);
-    s->sv.tri_cdf = (const float*)(a + parts[p_cdf].off);
-    s->sv.inst_emi = (const int32_t*)(a + parts[p_ie].off);
-    s->sv.tri_flags = (const uint8_t*)(a + parts[p_tf].off);
+    s->sv.emi = (const EmissiveInst RT_GLOBAL*)(a + parts[p_emi].off);
+    s->sv.tri_cdf = (const float RT_GLOBAL*)(a + parts[p_cdf].off);
+    s->sv.inst_emi = (const int32_t RT_GLOBAL*)(a + parts[p_ie].off);
+    s->sv.tri_flags = (const uint8_t RT_GLOBAL*)(a + parts[p_tf].off);
     s->sv.num_emi = (uint32_t)s->emi.size();
     s->sv.num_inst = (uint32_t)inst.size();
-    s->sv.tex = (const TexInfo*)(a + parts[p_tinfo].off);
-    s->sv.texels = (const uint32_t*)(a + parts[p_texel].off);
+    s->sv.tex = (const TexInfo RT_GLOBAL*)(a + parts[p_tinfo].off);
+    s->sv.texels = (const uint32_t RT_GLOBAL*)(a + parts[p_texel].off);
     s->sv.num_tex = d->num_textures;
     float4* rec = (float4*)(a + parts[p_rec].off);
     HIP_TRY(launch_shade_records(s->sv, rec, (uint32_t)tri_inst.size(), nullptr));
     HIP_TRY(hipStreamSynchronize(nullptr));
-    s->sv.srec = rec;
+    s->sv.srec = (const float4 RT_GLOBAL*)rec;
     return REINA_RT_OK;
 }
 

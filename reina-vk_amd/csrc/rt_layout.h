@@ -8,6 +8,18 @@
 
 namespace rt {
 
+// Pointers that kernels read from memory rather than from their arguments
+// (the scene views of a server's job table) are typed as global-memory
+// pointers in device code. Otherwise the compiler cannot tell global from LDS
+// or private memory and emits flat loads, which also count against the LDS
+// counter, so the traversal's LDS stack operations would wait for them. The
+// layout is the same for the host compiler, which sees plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RT_GLOBAL __attribute__((address_space(1)))
+#else
+#define RT_GLOBAL
+#endif
+
 // One BVH8 node, 256 bytes (two 128-byte cache lines). Child boxes are SoA
 // so each coordinate of all 8 children is one 32-byte run:
 //   lo_x[8] hi_x[8] lo_y[8] hi_y[8] lo_z[8] hi_z[8]  (192 B)
@@ -107,39 +119,39 @@ struct TexInfo {
 };
 
 struct SceneView {
-    const Bvh8Node* nodes;       // FP32 nodes (null when quantized)
-    const QBvh8Node* qnodes;     // quantized nodes (null when FP32)
-    const float4* tris;          // 3 per triangle
-    const uint32_t* tri_inst;    // flattened id -> instance
-    const uint32_t* tri_prim;    // flattened id -> gl_PrimitiveID
-    const InstanceRec* inst;
-    const ReinaInstanceProperties* props;
-    const float4* vertices;
-    const uint32_t* indices;
-    const float* tbns;
-    const uint32_t* tbn_indices;
-    const float* tex_coords;
-    const uint32_t* tex_indices;
+    const Bvh8Node RT_GLOBAL* nodes;       // FP32 nodes (null when quantized)
+    const QBvh8Node RT_GLOBAL* qnodes;     // quantized nodes (null when FP32)
+    const float4 RT_GLOBAL* tris;          // 3 per triangle
+    const uint32_t RT_GLOBAL* tri_inst;    // flattened id -> instance
+    const uint32_t RT_GLOBAL* tri_prim;    // flattened id -> gl_PrimitiveID
+    const InstanceRec RT_GLOBAL* inst;
+    const ReinaInstanceProperties RT_GLOBAL* props;
+    const float4 RT_GLOBAL* vertices;
+    const uint32_t RT_GLOBAL* indices;
+    const float RT_GLOBAL* tbns;
+    const uint32_t RT_GLOBAL* tbn_indices;
+    const float RT_GLOBAL* tex_coords;
+    const uint32_t RT_GLOBAL* tex_indices;
     uint64_t num_tbns;
     uint64_t num_tbn_indices;
     uint64_t num_tex_coords;
     uint64_t num_nodes;
     uint64_t num_tris;
     // next-event estimation
-    const EmissiveInst* emi;
-    const float* tri_cdf;
-    const int32_t* inst_emi;     // instance -> emissive record, -1 if not emissive
-    const uint8_t* tri_flags;
+    const EmissiveInst RT_GLOBAL* emi;
+    const float RT_GLOBAL* tri_cdf;
+    const int32_t RT_GLOBAL* inst_emi;     // instance -> emissive record, -1 if not emissive
+    const uint8_t RT_GLOBAL* tri_flags;
     uint32_t num_emi;
     uint32_t num_inst;
     // shading record of flattened triangle g, kShadeRecF4 float4 at srec + kShadeRecF4 * g:
     // (v0.xyz, instance), (v1.xyz, primitive), (v2.xyz, properties index), then T, B, N of
     // vertex 0, 1, 2 (27 floats, 1 pad): the object-space values hit_info reads, gathered
     // once per scene
-    const float4* srec;
+    const float4 RT_GLOBAL* srec;
     // textures of InstanceProperties.textureID / normalMapTexID / bumpMapTexID
-    const TexInfo* tex;
-    const uint32_t* texels;
+    const TexInfo RT_GLOBAL* tex;
+    const uint32_t RT_GLOBAL* texels;
     uint32_t num_tex;
 };
 constexpr uint32_t kShadeRecF4 = 10;   // 160 bytes

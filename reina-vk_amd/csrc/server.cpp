@@ -326,7 +326,7 @@ static void worker_main(ReinaServer* s) {
             JobRec& jr = s->jobs_h[r.job];
             jr.sv = r.scene->sv;
             jr.P = r.P;
-            jr.img = r.img;
+            jr.img = (float4 RT_GLOBAL*)r.img;
             jr.px_base = r.tile0 << s->tile_log2;
             jr.npx = r.npx;
             W_TRY(hipMemcpyAsync(s->jobs_d + r.job, &jr, sizeof(JobRec), hipMemcpyHostToDevice, s->st));

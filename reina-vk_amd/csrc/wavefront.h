@@ -80,7 +80,7 @@ constexpr uint32_t kMaxJobs = 256;
 struct JobRec {
     SceneView sv;
     TraceParams P;
-    float4* img;
+    float4 RT_GLOBAL* img;
     uint32_t px_base;   // first path-state slot
     uint32_t npx;       // W * H
 };
