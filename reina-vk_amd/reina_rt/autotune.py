@@ -75,8 +75,9 @@ def tune_frame(dev, torch, frame, candidates, reps=2):
     return win, table
 
 
-SERVER_AXES = (("refill_min", (16, 32)), ("grid_blocks", ("4cu", "8cu")), ("waves_per_simd", (0, 4, 8)),
-               ("tail_paths", (1 << 19, 1 << 20, 1 << 21)), ("claim_rays", (128, 64, 256)),
+# The first value of each axis is the library's default, where the search starts.
+SERVER_AXES = (("refill_min", (32, 16)), ("grid_blocks", ("4cu", "8cu")), ("waves_per_simd", (0, 4, 8)),
+               ("tail_paths", (1 << 20, 1 << 19, 1 << 21)), ("claim_rays", (128, 64, 256)),
                ("slot_tile", (1024, 256, 4096)), ("graph_iters", (6, 12, 24)), ("admit_order", (0, 2)),
                ("tail_cu_percent", (0, 50, 100)))
 
